@@ -1,0 +1,180 @@
+/*
+ * yarnball_hip.h — C ABI of the MI355X (gfx950) YarnBall stepper.
+ *
+ * Every entry point is plain C: POD structs, raw pointers and sizes, int status
+ * codes.  No C++ types or exceptions cross this boundary.  The C++ drop-in facade
+ * (include/YarnBall/YarnBall.h, class YarnBall::Sim) and the Python mirror
+ * (yarnball_amd/sim.py) are both thin layers over these functions.
+ *
+ * Reference interface being replaced: class YarnBall::Sim
+ * (KittenEngine/YarnBall/YarnBall.h:116-206) and its implementation in
+ * KittenEngine/YarnBall/YarnBall.cu and the files under KittenEngine/YarnBall/sim.
+ *
+ * Status codes: 0 = ok; 1 = YB_ERROR_MAX_COLLISIONS_PER_SEGMENT_EXCEEDED
+ * (YarnBall.h:120); 2 = YB_WARNING_SEGMENT_INTERPENETRATION (YarnBall.h:121);
+ * negative values are API / HIP errors, described by yb_last_message().
+ */
+#ifndef YARNBALL_HIP_H
+#define YARNBALL_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define YB_MAX_COLLISIONS_PER_SEGMENT 128 /* YarnBall.h:20 */
+
+enum {
+    YB_OK = 0,
+    YB_ERROR_MAX_COLLISIONS_PER_SEGMENT_EXCEEDED = 1, /* YarnBall.h:120 */
+    YB_WARNING_SEGMENT_INTERPENETRATION = 2,          /* YarnBall.h:121 */
+    YB_ERR_INVALID_ARGUMENT = -1,
+    YB_ERR_NOT_CONFIGURED = -2,
+    YB_ERR_HIP = -3,
+    YB_ERR_VALIDATION = -4,
+    YB_ERR_NO_DEVICE = -5,
+};
+
+/* Vertex flag bits, YarnBall.h:34-40. */
+enum {
+    YB_FLAG_HAS_PREV = 1,
+    YB_FLAG_HAS_NEXT = 2,
+    YB_FLAG_HAS_NEXT_ORIENTATION = 4,
+    YB_FLAG_FIX_ORIENTATION = 8,
+    YB_FLAG_COLLIDING = 16,
+};
+
+/* 32-byte AoS vertex record, byte-identical to YarnBall::Vertex (YarnBall.h:52-61). */
+typedef struct yb_vertex {
+    float pos[3];
+    float invMass;
+    float lRest;
+    float kStretch;
+    int32_t connectionIndex;
+    uint32_t flags;
+} yb_vertex;
+
+/* Simulation parameters: the scalar part of YarnBall::MetaData (YarnBall.h:63-114)
+ * without its device pointers.  h, lastH, time, detectionRadius and
+ * scaledDetectionRadius are written by yb_advance(), as Sim::advance and
+ * Sim::uploadMeta do (step.cpp:101-118, YarnBall.cu:226-228). */
+typedef struct yb_params {
+    float gravity[3];
+    int32_t numItr;
+    float worldFloorNormal[3];
+    float worldFloorPos;
+    float h;
+    float lastH;
+    float time;
+    int32_t numVerts;
+    float damping;
+    float drag;
+    float frictionCoeff;
+    float kCollision;
+    float kFriction;
+    float detectionRadius;
+    float scaledDetectionRadius;
+    float radius;
+    float accelerationRatio;
+    float barrierThickness;
+    float detectionScaler;
+    float bvhRebuildPeriod;
+    int32_t detectionPeriod;
+    float maxSegLen;
+    float minSegLen;
+    int32_t useStepSizeLimit;
+    int32_t useVelocityRadius;
+} yb_params;
+
+typedef struct yb_sim yb_sim; /* opaque device-side simulation handle */
+
+/* ---- host-side set-up math (no device needed) ---------------------------- */
+
+/* Constructor defaults of Sim::Sim (YarnBall.cu:11-51): fills *p and, when the
+ * arrays are non-null, the per-vertex defaults.  vels: 3n floats, qRests: 4n. */
+int yb_default_params(yb_params* p, int numVerts, float maxH);
+int yb_default_vertices(int n, yb_vertex* verts, float* vels, float* qs, float* qRests);
+
+/* Sim::configure host part (YarnBall.cu:91-157): completes flags, rest lengths,
+ * minimal-twist frames qs (4n), masses, rest Darboux vectors qRests (4n),
+ * initialInvMasses (n, may be null), and min/maxSegLen in *p.  Returns
+ * YB_ERR_VALIDATION with a message for "Dangling segment" / "0 length segment". */
+int yb_configure_host(int n, yb_vertex* verts, float* qs, float* qRests, float density,
+                      float* initialInvMasses, yb_params* p);
+/* Sim::setKStretch (YarnBall.cu:202-210). */
+int yb_set_kstretch_host(int n, yb_vertex* verts, float kStretch);
+/* Sim::setKBend (YarnBall.cu:212-224). */
+int yb_set_kbend_host(int n, const yb_vertex* verts, float* qRests, float kBend);
+/* Sim::uploadMeta validation and derived radii (YarnBall.cu:226-235). */
+int yb_validate_params(yb_params* p);
+
+/* ---- device lifecycle ---------------------------------------------------- */
+
+/* Device allocation part of Sim::configure (YarnBall.cu:166-198): SoA buffers,
+ * one HIP stream, contact lists, BVH storage.  n >= 3 (YarnBall.cu:12). */
+int yb_create(int n, yb_sim** out);
+/* Sim::~Sim (YarnBall.cu:53-85). */
+void yb_destroy(yb_sim* s);
+
+/* Sim::upload (YarnBall.cu:247-254): AoS host arrays -> device SoA.
+ * verts: n records; vels: 3n; qs: 4n (x,y,z,w); qRests: 4n.  Synchronous. */
+int yb_upload(yb_sim* s, const yb_vertex* verts, const float* vels, const float* qs,
+              const float* qRests);
+/* Sim::download (YarnBall.cu:256-261): device SoA -> AoS host.  Any pointer may
+ * be null to skip that array.  Synchronous. */
+int yb_download(yb_sim* s, yb_vertex* verts, float* vels, float* qs);
+/* Sim::zeroVelocities (YarnBall.cu:263-274). */
+int yb_zero_velocities(yb_sim* s);
+
+/* Sim::advance (step.cpp:101-120) + Sim::checkErrors (YarnBall.cu:276-304).
+ * Validates *p like uploadMeta, splits h into ceil(h/maxH) substeps, runs them on
+ * the handle's stream and blocks until done.  Updates p->h, p->lastH, p->time and
+ * the derived radii.  Returns 0, 1 (contact-list overflow; state may be corrupt),
+ * or a negative error.  *warning receives 2 when a penetration was detected. */
+int yb_advance(yb_sim* s, yb_params* p, float h, float maxH, int* warning);
+/* Number of substeps run so far (Sim::stepCount, YarnBall.h:163). */
+int64_t yb_step_count(yb_sim* s);
+
+/* Root AABB of the broadphase tree, as Sim::bounds / LBVH::bounds
+ * (YarnBall.cu:87-89; collision.cu:109): out[0..2] min, out[3..5] max. */
+int yb_bounds(yb_sim* s, float out[6]);
+/* Per-segment contact counts (the data printCollisionStats reads,
+ * statistics.cpp:5-24).  out: n ints. */
+int yb_collision_counts(yb_sim* s, int32_t* out);
+
+/* Last error message (thread-local), for negative status codes. */
+const char* yb_last_message(void);
+
+/* ---- stage-level access used by the parity tests -------------------------
+ * These expose intermediate device state so that each stage can be compared
+ * with the CPU oracle in isolation.  They have no counterpart in the reference
+ * (whose state is private), and are not needed by production callers. */
+
+/* Run only the substep prologue: initItr (iteration.cu:7-41) and, when detect!=0,
+ * the broadphase + narrowphase of detectCollisions (collision.cu:94-119) and
+ * the step limit (collision.cu:121-172).  Uses p and h as yb_advance would. */
+int yb_debug_prologue(yb_sim* s, yb_params* p, float h, int detect, int rebuild);
+/* Copy out intermediate arrays (any pointer may be null):
+ * dx: 4n floats (xyz + step limit), aabb: 8n floats (lo xyz _, hi xyz _),
+ * morton: n u32 indexed by segment, order: n i32 (sorted leaf -> segment),
+ * numCols: n, cols: 128n slot-major (cols[seg + n*slot]). */
+int yb_debug_read(yb_sim* s, float* dx, float* aabb, uint32_t* morton, int32_t* order,
+                  int32_t* numCols, int32_t* cols);
+/* Karras tree arrays after the last rebuild: child: 2(n-1) ints, (left, right)
+ * per internal node, an internal child as its index and a leaf as ~segment;
+ * parent: (2n-1) ints, (parent << 1 | side) for internal nodes [0, n-1) and then
+ * for leaves in sorted order, -1 for the root; box: 16 floats per internal node
+ * (left lo, left hi, right lo, right hi as xyzw; w of the lo entries holds the
+ * child code bits) after the last refit.
+ * Returns YB_ERR_INVALID_ARGUMENT if no tree was built. */
+int yb_debug_tree(yb_sim* s, int32_t* child, int32_t* parent, float* box);
+/* Overwrite the device warm-start/iterate dx (4n floats) and lastVels (3n). */
+int yb_debug_write_dx(yb_sim* s, const float* dx);
+int yb_debug_read_last_vels(yb_sim* s, float* lastVels);
+int yb_debug_write_last_vels(yb_sim* s, const float* lastVels);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* YARNBALL_HIP_H */

@@ -1,0 +1,124 @@
+"""Discrete stages on the device vs the oracle's FP32 twin: warm start, padded
+boxes, Morton codes, sorted order and contact sets must match bit for bit; the
+Karras tree must satisfy its structural invariants."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from oracle import orc32
+from yarnball_amd import scenes
+from yarnball_amd._lib import check, ptr
+
+from helpers import contact_sets, gpu_sim
+
+
+def prologue(sc, seed=0, vel_scale=0.02):
+    rng = np.random.default_rng(seed)
+    sc = sc.copy()
+    free = sc.verts["invMass"] != 0
+    sc.vels[free] = (rng.normal(0, vel_scale, (free.sum(), 3))).astype(np.float32)
+    sim = gpu_sim(sc)
+    n = sc.n
+    lv = (rng.normal(0, vel_scale, (n, 3))).astype(np.float32)
+    check(sim._lib.yb_debug_write_last_vels(sim.handle, ptr(lv)))
+    p = sim.meta.copy()
+    p.lastH = np.float32(7e-4)
+    check(sim._lib.yb_debug_prologue(sim.handle, ctypes.byref(p), np.float32(sc.maxH), 1, 1))
+    dx = np.zeros((n, 4), np.float32)
+    aabb = np.zeros((n, 8), np.float32)
+    morton = np.zeros(n, np.uint32)
+    order = np.zeros(n, np.int32)
+    nc = np.zeros(n, np.int32)
+    cols = np.zeros(n * 128, np.int32)
+    check(sim._lib.yb_debug_read(sim.handle, ptr(dx), ptr(aabb), ptr(morton), ptr(order), ptr(nc),
+                                 ptr(cols)))
+    return sim, sc, p, lv, dict(dx=dx, aabb=aabb, morton=morton, order=order, nc=nc,
+                                cols=cols.reshape(128, n))
+
+
+def twin(sc, p, lv):
+    dxw, _ = orc32.warm_start(p, sc.verts["invMass"], sc.vels, lv)
+    ab = orc32.aabb(p, sc.verts["pos"], sc.verts["flags"], dxw)
+    box = orc32.scene_box(ab)
+    codes = orc32.morton(sc.verts["flags"], ab, box)
+    order = orc32.sort(codes)
+    nc, cols, err = orc32.contacts(p, sc.verts["pos"], sc.verts["flags"],
+                                   sc.verts["connectionIndex"], dxw, ab)
+    return dict(dx=dxw, aabb=ab, box=box, morton=codes, order=order, nc=nc, cols=cols, err=err)
+
+
+SCENES = [
+    ("knit_16x512", lambda: scenes.knit_patch(rows=16, per=512)),
+    ("cable_19x512", lambda: scenes.twisted_cable(strands=19, per=512)),
+    ("readme", scenes.readme_scene),
+]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,make", SCENES, ids=[s[0] for s in SCENES])
+def test_discrete_stages_bit_exact(name, make):
+    sim, sc, p, lv, g = prologue(make())
+    t = twin(sc, p, lv)
+    assert np.array_equal(g["dx"][:, :3].view(np.uint32), t["dx"].view(np.uint32)), "warm start"
+    gab = np.concatenate([g["aabb"][:, 0:3], g["aabb"][:, 4:7]], axis=1)
+    assert np.array_equal(gab.view(np.uint32), t["aabb"].view(np.uint32)), "boxes"
+    assert np.array_equal(g["morton"], t["morton"]), "morton codes"
+    assert np.array_equal(g["order"], t["order"]), "sorted order"
+    assert contact_sets(g["nc"], g["cols"]) == contact_sets(t["nc"], t["cols"]), "contact sets"
+    sim_box = np.concatenate(sim.bounds())
+    assert np.array_equal(sim_box.view(np.uint32), t["box"].view(np.uint32)), "root box"
+    print(f"{name}: contacts {int(t['nc'].sum())}")
+    sim.close()
+
+
+@pytest.mark.gpu
+def test_karras_tree_invariants():
+    sim, sc, p, lv, g = prologue(scenes.knit_patch(rows=16, per=512))
+    n = sc.n
+    child = np.zeros(2 * (n - 1), np.int32)
+    parent = np.zeros(2 * n - 1, np.int32)
+    box = np.zeros(16 * (n - 1), np.float32)
+    check(sim._lib.yb_debug_tree(sim.handle, ptr(child), ptr(parent), ptr(box)))
+    child = child.reshape(n - 1, 2)
+    box = box.reshape(n - 1, 4, 4)
+    order = g["order"]
+    rank = np.empty(n, np.int64)
+    rank[order] = np.arange(n)
+    assert parent[0] == -1
+    # Each internal node covers a contiguous range of sorted leaves; boxes are
+    # exact unions; parent pointers agree with child pointers.
+    aabb = g["aabb"]
+    seen = np.zeros(n, np.int32)
+
+    def walk(node):
+        lo_r, hi_r = n, -1
+        for side in range(2):
+            c = int(child[node, side])
+            if c < 0:
+                seg = ~c
+                k = rank[seg]
+                assert parent[n - 1 + k] == (node << 1 | side)
+                seen[seg] += 1
+                lo, hi = aabb[seg, 0:3], aabb[seg, 4:7]
+                rlo, rhi = k, k
+            else:
+                assert parent[c] == (node << 1 | side)
+                rlo, rhi, lo, hi = walk(c)
+            assert np.array_equal(box[node, 2 * side, :3], lo)
+            assert np.array_equal(box[node, 2 * side + 1, :3], hi)
+            lo_r, hi_r = min(lo_r, rlo), max(hi_r, rhi)
+            if side == 0:
+                left_hi = rhi
+            else:
+                assert rlo == left_hi + 1, "children must be adjacent ranges"
+        lo = np.minimum(box[node, 0, :3], box[node, 2, :3])
+        hi = np.maximum(box[node, 1, :3], box[node, 3, :3])
+        return lo_r, hi_r, lo, hi
+
+    import sys
+    sys.setrecursionlimit(10000)
+    lo_r, hi_r, _, _ = walk(0)
+    assert (lo_r, hi_r) == (0, n - 1)
+    assert np.all(seen == 1), "every leaf exactly once"
+    sim.close()

@@ -1,0 +1,152 @@
+"""ctypes binding of libyarnball_hip.so (the C ABI declared in include/yarnball_hip.h).
+
+The library is built in-tree by ``python -m yarnball_amd.build`` (or
+``__graft_entry__.build()``).  Loading fails loudly when it is missing: there is
+no CPU fallback for the stepper.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libyarnball_hip.so")
+
+MAX_COLLISIONS_PER_SEGMENT = 128
+
+YB_OK = 0
+YB_ERROR_MAX_COLLISIONS_PER_SEGMENT_EXCEEDED = 1
+YB_WARNING_SEGMENT_INTERPENETRATION = 2
+
+FLAG_HAS_PREV = 1
+FLAG_HAS_NEXT = 2
+FLAG_HAS_NEXT_ORIENTATION = 4
+FLAG_FIX_ORIENTATION = 8
+FLAG_COLLIDING = 16
+
+# 32-byte vertex record, YarnBall.h:52-61.
+VERTEX_DTYPE = np.dtype(
+    [
+        ("pos", "<f4", (3,)),
+        ("invMass", "<f4"),
+        ("lRest", "<f4"),
+        ("kStretch", "<f4"),
+        ("connectionIndex", "<i4"),
+        ("flags", "<u4"),
+    ]
+)
+assert VERTEX_DTYPE.itemsize == 32
+
+
+class Params(ctypes.Structure):
+    """yb_params: the scalar fields of MetaData (YarnBall.h:63-114)."""
+
+    _fields_ = [
+        ("gravity", ctypes.c_float * 3),
+        ("numItr", ctypes.c_int32),
+        ("worldFloorNormal", ctypes.c_float * 3),
+        ("worldFloorPos", ctypes.c_float),
+        ("h", ctypes.c_float),
+        ("lastH", ctypes.c_float),
+        ("time", ctypes.c_float),
+        ("numVerts", ctypes.c_int32),
+        ("damping", ctypes.c_float),
+        ("drag", ctypes.c_float),
+        ("frictionCoeff", ctypes.c_float),
+        ("kCollision", ctypes.c_float),
+        ("kFriction", ctypes.c_float),
+        ("detectionRadius", ctypes.c_float),
+        ("scaledDetectionRadius", ctypes.c_float),
+        ("radius", ctypes.c_float),
+        ("accelerationRatio", ctypes.c_float),
+        ("barrierThickness", ctypes.c_float),
+        ("detectionScaler", ctypes.c_float),
+        ("bvhRebuildPeriod", ctypes.c_float),
+        ("detectionPeriod", ctypes.c_int32),
+        ("maxSegLen", ctypes.c_float),
+        ("minSegLen", ctypes.c_float),
+        ("useStepSizeLimit", ctypes.c_int32),
+        ("useVelocityRadius", ctypes.c_int32),
+    ]
+
+    def copy(self):
+        c = Params()
+        ctypes.memmove(ctypes.byref(c), ctypes.byref(self), ctypes.sizeof(Params))
+        return c
+
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_F = ctypes.c_float
+
+# name -> (restype, argtypes); every symbol the C header declares.
+SIGNATURES = {
+    "yb_default_params": (_I, [ctypes.POINTER(Params), _I, _F]),
+    "yb_default_vertices": (_I, [_I, _P, _P, _P, _P]),
+    "yb_configure_host": (_I, [_I, _P, _P, _P, _F, _P, ctypes.POINTER(Params)]),
+    "yb_set_kstretch_host": (_I, [_I, _P, _F]),
+    "yb_set_kbend_host": (_I, [_I, _P, _P, _F]),
+    "yb_validate_params": (_I, [ctypes.POINTER(Params)]),
+    "yb_create": (_I, [_I, ctypes.POINTER(_P)]),
+    "yb_destroy": (None, [_P]),
+    "yb_upload": (_I, [_P, _P, _P, _P, _P]),
+    "yb_download": (_I, [_P, _P, _P, _P]),
+    "yb_zero_velocities": (_I, [_P]),
+    "yb_advance": (_I, [_P, ctypes.POINTER(Params), _F, _F, ctypes.POINTER(_I)]),
+    "yb_step_count": (ctypes.c_int64, [_P]),
+    "yb_bounds": (_I, [_P, _P]),
+    "yb_collision_counts": (_I, [_P, _P]),
+    "yb_last_message": (ctypes.c_char_p, []),
+    "yb_debug_prologue": (_I, [_P, ctypes.POINTER(Params), _F, _I, _I]),
+    "yb_debug_read": (_I, [_P, _P, _P, _P, _P, _P, _P]),
+    "yb_debug_tree": (_I, [_P, _P, _P, _P]),
+    "yb_debug_write_dx": (_I, [_P, _P]),
+    "yb_debug_read_last_vels": (_I, [_P, _P]),
+    "yb_debug_write_last_vels": (_I, [_P, _P]),
+}
+
+_lib = None
+
+
+def load():
+    """Load libyarnball_hip.so; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -m yarnball_amd.build` "
+            "(the stepper has no CPU fallback)"
+        )
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def ptr(a):
+    """Raw data pointer of a C-contiguous numpy array (or None)."""
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"], "arrays passed to the C ABI must be contiguous"
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def last_message():
+    return load().yb_last_message().decode(errors="replace")
+
+
+class YarnBallError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+def check(code, where=""):
+    if code != YB_OK:
+        msg = last_message() or where
+        raise YarnBallError(code, msg)
+    return code

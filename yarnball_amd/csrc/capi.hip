@@ -1,0 +1,479 @@
+// capi.hip — extern "C" entry points of libyarnball_hip.so: device lifecycle,
+// substep scheduling with captured HIP graphs, error reporting.
+//
+// Scheduling mirrors Sim::advance (step.cpp:101-120): a frame of length h is
+// split into ceil(h / maxH) substeps; each substep is initItr, optional contact
+// detection every detectionPeriod substeps (full tree rebuild when
+// bvhRebuildPeriod of simulated time has passed, refit otherwise,
+// collision.cu:101-108), the step limit, numItr sweeps and endItr.  Unlike the
+// reference, detection needs no host round trip (the reference reads the
+// root box and the pair count back at collision.cu:109,111), so every substep
+// -- detection included -- is a single graph launch; the only host sync per
+// advance is the error-flag read of checkErrors (YarnBall.cu:276-304).
+#include <math.h>
+#include <string.h>
+
+#include <string>
+
+#include "yb_internal.h"
+
+namespace yb {
+
+static thread_local std::string g_msg;
+void set_message(const std::string& m) { g_msg = m; }
+
+}  // namespace yb
+
+using namespace yb;
+
+#define HIPCHECK(expr)                                                              \
+    do {                                                                            \
+        hipError_t e_ = (expr);                                                     \
+        if (e_ != hipSuccess) {                                                     \
+            set_message(std::string(#expr) + ": " + hipGetErrorString(e_));         \
+            return YB_ERR_HIP;                                                      \
+        }                                                                           \
+    } while (0)
+
+extern "C" const char* yb_last_message(void) { return g_msg.c_str(); }
+
+static void free_all(yb_sim* s) {
+    Buffers& b = s->b;
+    void* ptrs[] = {b.pos,     b.prm,     b.vel,      b.lastVel, b.yv,      b.dx[0],
+                    b.dx[1],   b.q[0],    b.q[1],     b.qRest,   b.numCols, b.cols,
+                    b.aabb,    b.sceneBox, b.blockBox, b.morton, b.keys[0], b.keys[1],
+                    b.vals[0], b.vals[1], b.hist,     b.nodes,   b.parent,  b.visit,
+                    b.err,     b.params,  b.stage};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    b = Buffers();
+    for (int i = 0; i < 3; i++)
+        if (s->graphs[i]) {
+            (void)hipGraphExecDestroy(s->graphs[i]);
+            s->graphs[i] = nullptr;
+        }
+    if (s->stream) {
+        (void)hipStreamDestroy(s->stream);
+        s->stream = nullptr;
+    }
+}
+
+template <typename T>
+static hipError_t dalloc(T** p, size_t count) {
+    return hipMalloc((void**)p, sizeof(T) * (count ? count : 1));
+}
+
+extern "C" int yb_create(int n, yb_sim** out) {
+    if (!out) {
+        set_message("yb_create: null output pointer");
+        return YB_ERR_INVALID_ARGUMENT;
+    }
+    *out = nullptr;
+    if (n < 3) {
+        set_message("Too little vertices");  // YarnBall.cu:12
+        return YB_ERR_INVALID_ARGUMENT;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        set_message("yb_create: no HIP device visible");
+        return YB_ERR_NO_DEVICE;
+    }
+    yb_sim* s = new yb_sim();
+    s->n = n;
+    s->sortTiles = (n + kSortTile - 1) / kSortTile;
+    s->boxBlocks = (n + 255) / 256;
+    if (s->boxBlocks > 1024) s->boxBlocks = 1024;
+    Buffers& b = s->b;
+    const size_t N = (size_t)n;
+    hipError_t e = hipSuccess;
+#define ALLOC(p, c) \
+    if (e == hipSuccess) e = dalloc(&(p), (c))
+    ALLOC(b.pos, N);
+    ALLOC(b.prm, N);
+    ALLOC(b.vel, N);
+    ALLOC(b.lastVel, N);
+    ALLOC(b.yv, N);
+    ALLOC(b.dx[0], N);
+    ALLOC(b.dx[1], N);
+    ALLOC(b.q[0], N);
+    ALLOC(b.q[1], N);
+    ALLOC(b.qRest, N);
+    ALLOC(b.numCols, N);
+    ALLOC(b.cols, N * kMaxCols);
+    ALLOC(b.aabb, 2 * N);
+    ALLOC(b.sceneBox, 2);
+    ALLOC(b.blockBox, 2 * (size_t)s->boxBlocks);
+    ALLOC(b.morton, N);
+    ALLOC(b.keys[0], N);
+    ALLOC(b.keys[1], N);
+    ALLOC(b.vals[0], N);
+    ALLOC(b.vals[1], N);
+    ALLOC(b.hist, (size_t)kSortBins * s->sortTiles);
+    ALLOC(b.nodes, 4 * (N - 1));
+    ALLOC(b.parent, 2 * N - 1);
+    ALLOC(b.visit, N - 1);
+    ALLOC(b.err, 2);
+    ALLOC(b.params, 1);
+#undef ALLOC
+    b.order = b.vals[0];
+    b.stageBytes = N * (sizeof(yb_vertex) + 11 * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&b.stage, b.stageBytes);
+    if (e == hipSuccess) e = hipStreamCreate(&s->stream);
+    if (e == hipSuccess) e = hipMemset(b.lastVel, 0, sizeof(float4) * N);
+    if (e == hipSuccess) e = hipMemset(b.vel, 0, sizeof(float4) * N);
+    if (e == hipSuccess) e = hipMemset(b.dx[0], 0, sizeof(float4) * N);
+    if (e == hipSuccess) e = hipMemset(b.dx[1], 0, sizeof(float4) * N);
+    if (e == hipSuccess) e = hipMemset(b.yv, 0, sizeof(float4) * N);
+    if (e == hipSuccess) e = hipMemset(b.numCols, 0, sizeof(int32_t) * N);
+    if (e == hipSuccess) e = hipMemset(b.err, 0, sizeof(int32_t) * 2);
+    if (e == hipSuccess) e = hipMemset(b.nodes, 0, sizeof(float4) * 4 * (N - 1));
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        set_message(std::string("yb_create: ") + hipGetErrorString(e));
+        free_all(s);
+        delete s;
+        return YB_ERR_HIP;
+    }
+    *out = s;
+    return YB_OK;
+}
+
+extern "C" void yb_destroy(yb_sim* s) {
+    if (!s) return;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    free_all(s);
+    delete s;
+}
+
+extern "C" int yb_upload(yb_sim* s, const yb_vertex* verts, const float* vels, const float* qs,
+                         const float* qRests) {
+    if (!s || !verts || !vels || !qs || !qRests) {
+        set_message("yb_upload: null argument");
+        return YB_ERR_INVALID_ARGUMENT;
+    }
+    const size_t N = (size_t)s->n;
+    // Every index the kernels follow must stay inside the vertex array.
+    if (verts[N - 1].flags & YB_FLAG_HAS_NEXT) {
+        set_message("yb_upload: the last vertex cannot have a next vertex");
+        return YB_ERR_INVALID_ARGUMENT;
+    }
+    for (size_t i = 0; i < N; i++) {
+        const int c = verts[i].connectionIndex;
+        if (c < -1 || c >= (int)N) {
+            set_message("yb_upload: connectionIndex out of range at vertex " + std::to_string(i));
+            return YB_ERR_INVALID_ARGUMENT;
+        }
+    }
+    char* st = (char*)s->b.stage;
+    HIPCHECK(hipMemcpyAsync(st, verts, sizeof(yb_vertex) * N, hipMemcpyHostToDevice, s->stream));
+    st += sizeof(yb_vertex) * N;
+    HIPCHECK(hipMemcpyAsync(st, vels, sizeof(float) * 3 * N, hipMemcpyHostToDevice, s->stream));
+    st += sizeof(float) * 3 * N;
+    HIPCHECK(hipMemcpyAsync(st, qs, sizeof(float) * 4 * N, hipMemcpyHostToDevice, s->stream));
+    st += sizeof(float) * 4 * N;
+    HIPCHECK(hipMemcpyAsync(st, qRests, sizeof(float) * 4 * N, hipMemcpyHostToDevice, s->stream));
+    launch_upload(s);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    return YB_OK;
+}
+
+extern "C" int yb_download(yb_sim* s, yb_vertex* verts, float* vels, float* qs) {
+    if (!s) {
+        set_message("yb_download: null handle");
+        return YB_ERR_INVALID_ARGUMENT;
+    }
+    const size_t N = (size_t)s->n;
+    launch_download(s);
+    HIPCHECK(hipGetLastError());
+    const char* st = (const char*)s->b.stage;
+    if (verts)
+        HIPCHECK(hipMemcpyAsync(verts, st, sizeof(yb_vertex) * N, hipMemcpyDeviceToHost, s->stream));
+    st += sizeof(yb_vertex) * N;
+    if (vels)
+        HIPCHECK(hipMemcpyAsync(vels, st, sizeof(float) * 3 * N, hipMemcpyDeviceToHost, s->stream));
+    st += sizeof(float) * 3 * N;
+    if (qs) HIPCHECK(hipMemcpyAsync(qs, st, sizeof(float) * 4 * N, hipMemcpyDeviceToHost, s->stream));
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    return YB_OK;
+}
+
+extern "C" int yb_zero_velocities(yb_sim* s) {
+    if (!s) return YB_ERR_INVALID_ARGUMENT;
+    launch_zero_vel(s);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    return YB_OK;
+}
+
+static DevParams make_dev_params(const yb_params* p, int n) {
+    DevParams d;
+    memset(&d, 0, sizeof(d));
+    d.gx = p->gravity[0];
+    d.gy = p->gravity[1];
+    d.gz = p->gravity[2];
+    d.h = p->h;
+    d.lastH = p->lastH;
+    d.dampingH = p->damping / p->h;
+    d.drag = p->drag;
+    d.frictionCoeff = p->frictionCoeff;
+    d.kCollision = p->kCollision;
+    d.kFriction = p->kFriction;
+    d.detectionRadius = p->detectionRadius;
+    d.scaledDetectionRadius = p->scaledDetectionRadius;
+    d.radius = p->radius;
+    d.accelerationRatio = p->accelerationRatio;
+    d.barrierThickness = p->barrierThickness;
+    d.detectionScaler = p->detectionScaler;
+    d.useStepSizeLimit = p->useStepSizeLimit;
+    d.useVelocityRadius = p->useVelocityRadius;
+    d.n = n;
+    return d;
+}
+
+static int upload_params(yb_sim* s, const yb_params* p) {
+    DevParams d = make_dev_params(p, s->n);
+    HIPCHECK(hipMemcpyAsync(s->b.params, &d, sizeof(DevParams), hipMemcpyHostToDevice, s->stream));
+    // The source is a stack object: wait for the (tiny) copy before returning.
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    return YB_OK;
+}
+
+// Enqueue one substep in detection mode m (0 none, 1 refit, 2 rebuild).
+static void enqueue_substep(yb_sim* s, int mode, int numItr) {
+    launch_init(s);
+    if (mode >= 1) {
+        launch_aabb(s);
+        if (mode == 2) launch_rebuild(s);
+        launch_refit(s);
+        launch_query(s);
+    }
+    launch_step_limit(s);
+    for (int k = 0; k < numItr; k++) launch_sweep(s, k & 1);
+    launch_end(s, numItr & 1);
+}
+
+static int rebuild_graphs(yb_sim* s, int numItr) {
+    if (s->graphItr == numItr && s->graphs[0]) return YB_OK;
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    for (int m = 0; m < 3; m++) {
+        if (s->graphs[m]) {
+            HIPCHECK(hipGraphExecDestroy(s->graphs[m]));
+            s->graphs[m] = nullptr;
+        }
+        hipGraph_t g;
+        HIPCHECK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+        enqueue_substep(s, m, numItr);
+        HIPCHECK(hipStreamEndCapture(s->stream, &g));
+        hipError_t e = hipGraphInstantiate(&s->graphs[m], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIPCHECK(e);
+    }
+    s->graphItr = numItr;
+    return YB_OK;
+}
+
+static int check_errors(yb_sim* s, int* warning) {
+    int32_t err[2] = {0, 0};
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(err, s->b.err, sizeof(err), hipMemcpyDeviceToHost, s->stream));
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    if (err[0] != 0 || err[1] != 0) {
+        launch_fill_i32(s, s->b.err, 0, 2);
+        HIPCHECK(hipStreamSynchronize(s->stream));
+    }
+    if (warning) *warning = err[1];
+    if (err[0] == YB_ERROR_MAX_COLLISIONS_PER_SEGMENT_EXCEEDED) {
+        set_message("MAX_COLLISIONS_PER_SEGMENT exceeded");
+        return YB_ERROR_MAX_COLLISIONS_PER_SEGMENT_EXCEEDED;
+    }
+    if (err[0] != 0) {
+        set_message("Indescript error");
+        return YB_ERR_HIP;
+    }
+    return YB_OK;
+}
+
+// Host-side decision of the detection mode for the current substep
+// (step.cpp:113-114, collision.cu:101-108).
+static int next_mode(yb_sim* s, const yb_params* p) {
+    if (!(p->detectionPeriod > 0 && s->stepCounter % p->detectionPeriod == 0)) return 0;
+    if (!s->treeBuilt || s->lastBVHRebuild >= p->bvhRebuildPeriod) {
+        s->lastBVHRebuild = 0.f;
+        s->treeBuilt = true;
+        return 2;
+    }
+    s->lastBVHRebuild += p->h * p->detectionPeriod;
+    return 1;
+}
+
+extern "C" int yb_advance(yb_sim* s, yb_params* p, float h, float maxH, int* warning) {
+    if (warning) *warning = 0;
+    if (!s || !p) {
+        set_message("yb_advance: null argument");
+        return YB_ERR_INVALID_ARGUMENT;
+    }
+    if (!(h > 0)) return YB_OK;  // step.cpp:102
+    if (!(maxH > 0)) {
+        set_message("yb_advance: maxH must be positive");
+        return YB_ERR_INVALID_ARGUMENT;
+    }
+    if (p->numItr < 0) {
+        set_message("yb_advance: numItr must be >= 0");
+        return YB_ERR_INVALID_ARGUMENT;
+    }
+    int steps = (int)ceilf(h / maxH);
+    if (steps < 1) steps = 1;
+    p->lastH = p->h;
+    p->h = h / (float)steps;
+    int rc = yb_validate_params(p);
+    if (rc != YB_OK) return rc;
+    rc = rebuild_graphs(s, p->numItr);
+    if (rc != YB_OK) return rc;
+    rc = upload_params(s, p);
+    if (rc != YB_OK) return rc;
+    for (int st = 0; st < steps; st++, s->stepCounter++) {
+        const int mode = next_mode(s, p);
+        HIPCHECK(hipGraphLaunch(s->graphs[mode], s->stream));
+    }
+    p->time += h;
+    return check_errors(s, warning);
+}
+
+extern "C" int64_t yb_step_count(yb_sim* s) { return s ? s->stepCounter : 0; }
+
+extern "C" int yb_bounds(yb_sim* s, float out[6]) {
+    if (!s || !out) return YB_ERR_INVALID_ARGUMENT;
+    float4 box[2];
+    HIPCHECK(hipMemcpyAsync(box, s->b.sceneBox, sizeof(box), hipMemcpyDeviceToHost, s->stream));
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    if (!s->treeBuilt) {
+        box[0] = make_float4(INFINITY, INFINITY, INFINITY, 0.f);
+        box[1] = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.f);
+    }
+    out[0] = box[0].x;
+    out[1] = box[0].y;
+    out[2] = box[0].z;
+    out[3] = box[1].x;
+    out[4] = box[1].y;
+    out[5] = box[1].z;
+    return YB_OK;
+}
+
+extern "C" int yb_collision_counts(yb_sim* s, int32_t* out) {
+    if (!s || !out) return YB_ERR_INVALID_ARGUMENT;
+    HIPCHECK(hipMemcpyAsync(out, s->b.numCols, sizeof(int32_t) * s->n, hipMemcpyDeviceToHost,
+                            s->stream));
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    return YB_OK;
+}
+
+// ---- stage-level access ------------------------------------------------------
+
+extern "C" int yb_debug_prologue(yb_sim* s, yb_params* p, float h, int detect, int rebuild) {
+    if (!s || !p) return YB_ERR_INVALID_ARGUMENT;
+    p->h = h;
+    int rc = yb_validate_params(p);
+    if (rc != YB_OK) return rc;
+    rc = upload_params(s, p);
+    if (rc != YB_OK) return rc;
+    launch_init(s);
+    if (detect) {
+        launch_aabb(s);
+        if (rebuild || !s->treeBuilt) {
+            launch_rebuild(s);
+            s->treeBuilt = true;
+        }
+        launch_refit(s);
+        launch_query(s);
+    }
+    launch_step_limit(s);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    return check_errors(s, nullptr);
+}
+
+extern "C" int yb_debug_read(yb_sim* s, float* dx, float* aabb, uint32_t* morton, int32_t* order,
+                             int32_t* numCols, int32_t* cols) {
+    if (!s) return YB_ERR_INVALID_ARGUMENT;
+    const size_t N = (size_t)s->n;
+    const hipMemcpyKind k = hipMemcpyDeviceToHost;
+    if (dx) HIPCHECK(hipMemcpyAsync(dx, s->b.dx[0], sizeof(float4) * N, k, s->stream));
+    if (aabb) HIPCHECK(hipMemcpyAsync(aabb, s->b.aabb, sizeof(float4) * 2 * N, k, s->stream));
+    if (morton) HIPCHECK(hipMemcpyAsync(morton, s->b.morton, sizeof(uint32_t) * N, k, s->stream));
+    if (order) HIPCHECK(hipMemcpyAsync(order, s->b.order, sizeof(int32_t) * N, k, s->stream));
+    if (numCols) HIPCHECK(hipMemcpyAsync(numCols, s->b.numCols, sizeof(int32_t) * N, k, s->stream));
+    if (cols)
+        HIPCHECK(hipMemcpyAsync(cols, s->b.cols, sizeof(int32_t) * N * kMaxCols, k, s->stream));
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    return YB_OK;
+}
+
+extern "C" int yb_debug_tree(yb_sim* s, int32_t* child, int32_t* parent, float* box) {
+    if (!s) return YB_ERR_INVALID_ARGUMENT;
+    if (!s->treeBuilt) {
+        set_message("yb_debug_tree: no tree built yet");
+        return YB_ERR_INVALID_ARGUMENT;
+    }
+    const size_t N = (size_t)s->n;
+    float4* nodes = new float4[4 * (N - 1)];
+    hipError_t e = hipMemcpyAsync(nodes, s->b.nodes, sizeof(float4) * 4 * (N - 1),
+                                  hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess && parent)
+        e = hipMemcpyAsync(parent, s->b.parent, sizeof(int32_t) * (2 * N - 1),
+                           hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) {
+        delete[] nodes;
+        set_message(std::string("yb_debug_tree: ") + hipGetErrorString(e));
+        return YB_ERR_HIP;
+    }
+    for (size_t i = 0; i + 1 < N; i++) {
+        if (child) {
+            int32_t cl, cr;
+            memcpy(&cl, &nodes[4 * i].w, 4);
+            memcpy(&cr, &nodes[4 * i + 2].w, 4);
+            child[2 * i] = cl;
+            child[2 * i + 1] = cr;
+        }
+        if (box) memcpy(&box[16 * i], &nodes[4 * i], sizeof(float4) * 4);
+    }
+    delete[] nodes;
+    return YB_OK;
+}
+
+extern "C" int yb_debug_write_dx(yb_sim* s, const float* dx) {
+    if (!s || !dx) return YB_ERR_INVALID_ARGUMENT;
+    HIPCHECK(hipMemcpyAsync(s->b.dx[0], dx, sizeof(float4) * s->n, hipMemcpyHostToDevice, s->stream));
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    return YB_OK;
+}
+
+extern "C" int yb_debug_read_last_vels(yb_sim* s, float* lastVels) {
+    if (!s || !lastVels) return YB_ERR_INVALID_ARGUMENT;
+    float4* tmp = new float4[s->n];
+    hipError_t e = hipMemcpyAsync(tmp, s->b.lastVel, sizeof(float4) * s->n, hipMemcpyDeviceToHost,
+                                  s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    for (int i = 0; i < s->n && e == hipSuccess; i++) {
+        lastVels[3 * i] = tmp[i].x;
+        lastVels[3 * i + 1] = tmp[i].y;
+        lastVels[3 * i + 2] = tmp[i].z;
+    }
+    delete[] tmp;
+    HIPCHECK(e);
+    return YB_OK;
+}
+
+extern "C" int yb_debug_write_last_vels(yb_sim* s, const float* lastVels) {
+    if (!s || !lastVels) return YB_ERR_INVALID_ARGUMENT;
+    float4* tmp = new float4[s->n];
+    for (int i = 0; i < s->n; i++)
+        tmp[i] = make_float4(lastVels[3 * i], lastVels[3 * i + 1], lastVels[3 * i + 2], 0.f);
+    hipError_t e = hipMemcpyAsync(s->b.lastVel, tmp, sizeof(float4) * s->n, hipMemcpyHostToDevice,
+                                  s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    delete[] tmp;
+    HIPCHECK(e);
+    return YB_OK;
+}

@@ -1,0 +1,551 @@
+// kernels_discrete.hip — substep prologue and the LBVH broadphase on gfx950.
+//
+// This translation unit is compiled with -ffp-contract=off: every kernel here
+// feeds a discrete decision (contact accepted or not, Morton cell, sort order),
+// and the CPU oracle's FP32 twin repeats the same IEEE operations in the same
+// order, so codes, orders and contact sets match bit for bit.
+//
+//   k_init        inertial target + gravity-extrapolated warm start  (iteration.cu:7-41)
+//   k_aabb        padded segment boxes                                 (collision.cu:10-37)
+//   k_box_*       scene box reduction (the root box LBVH::bounds returns)
+//   k_morton      30-bit Morton code of each box centre in the scene box
+//   k_sort_*      stable LSD radix sort, 4 x 8-bit passes, LDS ranking
+//   k_karras      Karras (2012) radix tree over the sorted codes
+//   k_refit       bottom-up box refit with per-node arrival counters
+//   k_query       stack traversal + narrowphase filter (collision.cu:39-92)
+//                 writing each segment's own contact list: no atomics, no
+//                 host round trip for the pair count.
+#include "yb_internal.h"
+#include "ybmath.h"
+
+namespace yb {
+
+__device__ __forceinline__ v3 xyz(float4 a) { return v3{a.x, a.y, a.z}; }
+__device__ __forceinline__ float4 f4(v3 a, float w) { return make_float4(a.x, a.y, a.z, w); }
+
+// ---------------------------------------------------------------------------
+// initItr (iteration.cu:7-41): yv = h v + h^2 g for free vertices,
+// dx0 = h v + h^2 s g with s = clamp(a.g / |g|^2, 0, 1), a = (v - v_last)/lastH.
+__global__ __launch_bounds__(256) void k_init(const DevParams* __restrict__ P,
+                                              const float4* __restrict__ pos,
+                                              const float4* __restrict__ vel,
+                                              float4* __restrict__ lastVel,
+                                              float4* __restrict__ yv,
+                                              float4* __restrict__ dx0) {
+    const int n = P->n;
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    const float h = P->h;
+    const v3 g = v3{P->gx, P->gy, P->gz};
+    const v3 u = xyz(vel[v]);
+    v3 dx = h * u;
+    const v3 lv = xyz(lastVel[v]);
+    lastVel[v] = f4(u, 0.f);
+    if (pos[v].w != 0.0f) {
+        yv[v] = f4(dx + (h * h) * g, 0.f);
+        const float g2 = len2(g);
+        if (g2 > 0.0f) {
+            const float lh = P->lastH;
+            const v3 a = v3{(u.x - lv.x) / lh, (u.y - lv.y) / lh, (u.z - lv.z) / lh};
+            const float s = clampf(dot(a, g) / g2, 0.0f, 1.0f);
+            dx += (h * h * s) * g;
+        }
+    }
+    dx0[v] = f4(dx, 0.f);
+}
+
+// ---------------------------------------------------------------------------
+// Segment boxes (collision.cu:10-37).  Vertices without a next vertex get the
+// empty box (+inf, -inf), which overlaps nothing.
+__global__ __launch_bounds__(256) void k_aabb(const DevParams* __restrict__ P,
+                                              const float4* __restrict__ pos,
+                                              const float4* __restrict__ prm,
+                                              const float4* __restrict__ dx,
+                                              float4* __restrict__ aabb) {
+    const int n = P->n;
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    const uint32_t flags = __float_as_uint(prm[v].w);
+    v3 lo = v3{INFINITY, INFINITY, INFINITY};
+    v3 hi = v3{-INFINITY, -INFINITY, -INFINITY};
+    if (flags & YB_FLAG_HAS_NEXT) {
+        const v3 p0 = xyz(pos[v]);
+        const v3 p1 = xyz(pos[v + 1]);
+        lo = vmin(p0, p1);
+        hi = vmax(p0, p1);
+        float pad = P->scaledDetectionRadius;
+        if (P->useVelocityRadius)
+            pad += sqrtf(fmaxf(len2(xyz(dx[v])), len2(xyz(dx[v + 1]))));
+        lo = lo - v3{pad, pad, pad};
+        hi = hi + v3{pad, pad, pad};
+    }
+    aabb[2 * v] = f4(lo, 0.f);
+    aabb[2 * v + 1] = f4(hi, 0.f);
+}
+
+// ---------------------------------------------------------------------------
+// Scene box: min/max are exact, so the result is independent of the order.
+constexpr int kBoxThreads = 256;
+__global__ __launch_bounds__(kBoxThreads) void k_box_partial(int n, const float4* __restrict__ aabb,
+                                                             float4* __restrict__ blockBox) {
+    __shared__ float sm[6][kBoxThreads];
+    v3 lo = v3{INFINITY, INFINITY, INFINITY};
+    v3 hi = v3{-INFINITY, -INFINITY, -INFINITY};
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+        lo = vmin(lo, xyz(aabb[2 * v]));
+        hi = vmax(hi, xyz(aabb[2 * v + 1]));
+    }
+    const int t = threadIdx.x;
+    sm[0][t] = lo.x; sm[1][t] = lo.y; sm[2][t] = lo.z;
+    sm[3][t] = hi.x; sm[4][t] = hi.y; sm[5][t] = hi.z;
+    __syncthreads();
+    for (int s = kBoxThreads / 2; s > 0; s >>= 1) {
+        if (t < s) {
+            for (int k = 0; k < 3; k++) sm[k][t] = fminf(sm[k][t], sm[k][t + s]);
+            for (int k = 3; k < 6; k++) sm[k][t] = fmaxf(sm[k][t], sm[k][t + s]);
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        blockBox[2 * blockIdx.x] = make_float4(sm[0][0], sm[1][0], sm[2][0], 0.f);
+        blockBox[2 * blockIdx.x + 1] = make_float4(sm[3][0], sm[4][0], sm[5][0], 0.f);
+    }
+}
+
+__global__ __launch_bounds__(kBoxThreads) void k_box_final(int nb, const float4* __restrict__ blockBox,
+                                                           float4* __restrict__ sceneBox) {
+    __shared__ float sm[6][kBoxThreads];
+    v3 lo = v3{INFINITY, INFINITY, INFINITY};
+    v3 hi = v3{-INFINITY, -INFINITY, -INFINITY};
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+        lo = vmin(lo, xyz(blockBox[2 * b]));
+        hi = vmax(hi, xyz(blockBox[2 * b + 1]));
+    }
+    const int t = threadIdx.x;
+    sm[0][t] = lo.x; sm[1][t] = lo.y; sm[2][t] = lo.z;
+    sm[3][t] = hi.x; sm[4][t] = hi.y; sm[5][t] = hi.z;
+    __syncthreads();
+    for (int s = kBoxThreads / 2; s > 0; s >>= 1) {
+        if (t < s) {
+            for (int k = 0; k < 3; k++) sm[k][t] = fminf(sm[k][t], sm[k][t + s]);
+            for (int k = 3; k < 6; k++) sm[k][t] = fmaxf(sm[k][t], sm[k][t + s]);
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        sceneBox[0] = make_float4(sm[0][0], sm[1][0], sm[2][0], 0.f);
+        sceneBox[1] = make_float4(sm[3][0], sm[4][0], sm[5][0], 0.f);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Morton codes: box centre quantised to 1024 cells per axis of the scene box,
+// bits interleaved x-major.  Non-segments get 0xFFFFFFFF and sort last.
+__global__ __launch_bounds__(256) void k_morton(int n, const float4* __restrict__ prm,
+                                                const float4* __restrict__ aabb,
+                                                const float4* __restrict__ sceneBox,
+                                                uint32_t* __restrict__ morton,
+                                                uint32_t* __restrict__ keys,
+                                                int32_t* __restrict__ vals) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    uint32_t code = 0xFFFFFFFFu;
+    if (__float_as_uint(prm[v].w) & YB_FLAG_HAS_NEXT) {
+        const v3 slo = xyz(sceneBox[0]);
+        const v3 shi = xyz(sceneBox[1]);
+        const v3 ext = shi - slo;
+        const v3 lo = xyz(aabb[2 * v]);
+        const v3 hi = xyz(aabb[2 * v + 1]);
+        const v3 c = (lo + hi) * 0.5f;
+        const uint32_t qx = quant10(c.x, slo.x, ext.x);
+        const uint32_t qy = quant10(c.y, slo.y, ext.y);
+        const uint32_t qz = quant10(c.z, slo.z, ext.z);
+        code = (expand_bits10(qx) << 2) | (expand_bits10(qy) << 1) | expand_bits10(qz);
+    }
+    morton[v] = code;
+    keys[v] = code;
+    vals[v] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Stable LSD radix sort of (key, value), 8 bits per pass.
+// Pass = histogram per 2048-key tile -> one exclusive scan over the
+// digit-major [256 x tiles] table -> scatter.  The scatter ranks keys of equal
+// digit in index order: within a 64-lane wave by ballot matching of the 8
+// digit bits, across the block's 4 waves through LDS counters, across strips
+// by a running per-digit count.
+__global__ __launch_bounds__(kSortThreads) void k_sort_hist(int n, int shift, int tiles,
+                                                            const uint32_t* __restrict__ keys,
+                                                            uint32_t* __restrict__ hist) {
+    __shared__ uint32_t lh[kSortBins];
+    const int t = threadIdx.x;
+    lh[t] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * kSortTile;
+#pragma unroll
+    for (int k = 0; k < kSortItems; k++) {
+        const int idx = base + k * kSortThreads + t;
+        if (idx < n) atomicAdd(&lh[(keys[idx] >> shift) & (kSortBins - 1)], 1u);
+    }
+    __syncthreads();
+    hist[t * tiles + blockIdx.x] = lh[t];
+}
+
+constexpr int kScanThreads = 1024;
+__global__ __launch_bounds__(kScanThreads) void k_sort_scan(int total, uint32_t* __restrict__ hist) {
+    __shared__ uint32_t part[kScanThreads];
+    const int t = threadIdx.x;
+    const int chunk = (total + kScanThreads - 1) / kScanThreads;
+    const int b = t * chunk;
+    const int e = min(b + chunk, total);
+    uint32_t sum = 0;
+    for (int i = b; i < e; i++) sum += hist[i];
+    part[t] = sum;
+    __syncthreads();
+    // Hillis-Steele inclusive scan of the per-thread sums.
+    for (int off = 1; off < kScanThreads; off <<= 1) {
+        uint32_t x = (t >= off) ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += x;
+        __syncthreads();
+    }
+    uint32_t run = part[t] - sum;
+    for (int i = b; i < e; i++) {
+        const uint32_t c = hist[i];
+        hist[i] = run;
+        run += c;
+    }
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_sort_scatter(int n, int shift, int tiles,
+                                                               const uint32_t* __restrict__ kin,
+                                                               const int32_t* __restrict__ vin,
+                                                               uint32_t* __restrict__ kout,
+                                                               int32_t* __restrict__ vout,
+                                                               const uint32_t* __restrict__ hist) {
+    constexpr int kWaves = kSortThreads / 64;
+    __shared__ uint32_t run[kSortBins];
+    __shared__ uint32_t wcnt[kWaves][kSortBins];
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    const int w = t >> 6;
+    run[t] = hist[t * tiles + blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < kWaves; k++) wcnt[k][t] = 0;
+    __syncthreads();
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const int base = blockIdx.x * kSortTile;
+    for (int k = 0; k < kSortItems; k++) {
+        const int idx = base + k * kSortThreads + t;
+        const bool valid = idx < n;
+        uint32_t key = 0;
+        int32_t val = 0;
+        if (valid) {
+            key = kin[idx];
+            val = vin[idx];
+        }
+        const uint32_t d = (key >> shift) & (kSortBins - 1);
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int bit = 0; bit < kSortBits; bit++) {
+            const bool on = (d >> bit) & 1u;
+            const uint64_t m = __ballot(on);
+            peers &= on ? m : ~m;
+        }
+        const int rank = __popcll(peers & below);
+        const bool leader = valid && ((peers & below) == 0ull);
+        if (leader) wcnt[w][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t dst = run[d];
+            for (int ww = 0; ww < w; ww++) dst += wcnt[ww][d];
+            dst += (uint32_t)rank;
+            kout[dst] = key;
+            vout[dst] = val;
+        }
+        __syncthreads();
+        uint32_t tot = 0;
+#pragma unroll
+        for (int ww = 0; ww < kWaves; ww++) {
+            tot += wcnt[ww][t];
+            wcnt[ww][t] = 0;
+        }
+        run[t] += tot;
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Karras (2012) radix tree.  Keys are made unique by the sorted position:
+// delta(i,j) = clz(k_i ^ k_j), or 32 + clz(i ^ j) for equal codes.
+// Node i (internal, 0..n-2) stores its children's boxes inline:
+// nodes[4i+0] = (lo_left, child code), nodes[4i+1] = (hi_left, -),
+// nodes[4i+2] = (lo_right, child code), nodes[4i+3] = (hi_right, -),
+// child code = internal index (>= 0) or ~segment for a leaf.
+__device__ __forceinline__ int kdelta(const uint32_t* __restrict__ keys, int n, int a, int b) {
+    if (b < 0 || b > n - 1) return -1;
+    const uint32_t ka = keys[a];
+    const uint32_t kb = keys[b];
+    if (ka == kb) return 32 + __clz((uint32_t)(a ^ b));
+    return __clz(ka ^ kb);
+}
+
+__global__ __launch_bounds__(256) void k_karras(int n, const uint32_t* __restrict__ keys,
+                                                const int32_t* __restrict__ order,
+                                                float4* __restrict__ nodes,
+                                                int32_t* __restrict__ parent) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    const int d = (kdelta(keys, n, i, i + 1) - kdelta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
+    const int dmin = kdelta(keys, n, i, i - d);
+    int lmax = 2;
+    while (kdelta(keys, n, i, i + lmax * d) > dmin) lmax <<= 1;
+    int l = 0;
+    for (int t = lmax >> 1; t >= 1; t >>= 1)
+        if (kdelta(keys, n, i, i + (l + t) * d) > dmin) l += t;
+    const int j = i + l * d;
+    const int dnode = kdelta(keys, n, i, j);
+    int s = 0;
+    int t = l;
+    do {
+        t = (t + 1) >> 1;
+        if (kdelta(keys, n, i, i + (s + t) * d) > dnode) s += t;
+    } while (t > 1);
+    const int gamma = i + s * d + min(d, 0);
+    const int lo = min(i, j);
+    const int hi = max(i, j);
+    int cl, cr;
+    if (lo == gamma) {
+        cl = ~order[gamma];
+        parent[(n - 1) + gamma] = (i << 1) | 0;
+    } else {
+        cl = gamma;
+        parent[gamma] = (i << 1) | 0;
+    }
+    if (hi == gamma + 1) {
+        cr = ~order[gamma + 1];
+        parent[(n - 1) + gamma + 1] = (i << 1) | 1;
+    } else {
+        cr = gamma + 1;
+        parent[gamma + 1] = (i << 1) | 1;
+    }
+    nodes[4 * i + 0].w = __int_as_float(cl);
+    nodes[4 * i + 2].w = __int_as_float(cr);
+    if (i == 0) parent[0] = -1;
+}
+
+// ---------------------------------------------------------------------------
+// Refit: each leaf writes its box into its parent's slot, then climbs.  The
+// second of the two arrivals at a node (per-node counter) forms the union
+// and continues upward; the first stops.  Device-scope fences make the
+// sibling's slot visible across XCDs before the union is read.
+__global__ __launch_bounds__(256) void k_refit(int n, const int32_t* __restrict__ order,
+                                               const float4* __restrict__ aabb,
+                                               float4* nodes, const int32_t* __restrict__ parent,
+                                               int32_t* visit, float4* __restrict__ sceneBox) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int seg = order[k];
+    float4 lo = aabb[2 * seg];
+    float4 hi = aabb[2 * seg + 1];
+    int code = ~seg;
+    int slot = parent[(n - 1) + k];
+    while (true) {
+        const int p = slot >> 1;
+        const int side = slot & 1;
+        nodes[4 * p + 2 * side] = make_float4(lo.x, lo.y, lo.z, __int_as_float(code));
+        nodes[4 * p + 2 * side + 1] = make_float4(hi.x, hi.y, hi.z, 0.f);
+        __threadfence();
+        const int old = atomicAdd(&visit[p], 1);
+        if (old == 0) return;
+        __threadfence();
+        const float4 a0 = nodes[4 * p + 0];
+        const float4 a1 = nodes[4 * p + 1];
+        const float4 b0 = nodes[4 * p + 2];
+        const float4 b1 = nodes[4 * p + 3];
+        lo = make_float4(fminf(a0.x, b0.x), fminf(a0.y, b0.y), fminf(a0.z, b0.z), 0.f);
+        hi = make_float4(fmaxf(a1.x, b1.x), fmaxf(a1.y, b1.y), fmaxf(a1.z, b1.z), 0.f);
+        if (p == 0) {
+            sceneBox[0] = lo;
+            sceneBox[1] = hi;
+            return;
+        }
+        code = p;
+        slot = parent[p];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Query + narrowphase (collision.cu:39-92).  One thread per segment, taken in
+// Morton order so that neighbouring lanes walk similar paths.  Each overlapping
+// leaf j is filtered as the unordered pair (x, y) = (min, max): glued
+// endpoints and |x - y| <= 2 are exempt, and the pair is kept when the closest
+// distance between the segments is below 2 * scaledDetectionRadius plus both
+// segments' largest warm-start displacement.  Because the test is symmetric,
+// segment i lists j exactly when j lists i -- the set the reference builds
+// with two atomicAdds per pair.
+constexpr int kQueryThreads = 128;
+constexpr int kStack = 64;
+
+__device__ __forceinline__ bool overlap(v3 qlo, v3 qhi, float4 lo, float4 hi) {
+    return (qlo.x <= hi.x) && (lo.x <= qhi.x) && (qlo.y <= hi.y) && (lo.y <= qhi.y) &&
+           (qlo.z <= hi.z) && (lo.z <= qhi.z);
+}
+
+__device__ __forceinline__ float vel_pad(const float4* __restrict__ dx, int a) {
+    return sqrtf(fmaxf(len2(xyz(dx[a])), len2(xyz(dx[a + 1]))));
+}
+
+__device__ __forceinline__ bool pair_accepted(const DevParams* __restrict__ P,
+                                              const float4* __restrict__ pos,
+                                              const float4* __restrict__ prm,
+                                              const float4* __restrict__ dx, int i, int j) {
+    const int x = min(i, j);
+    const int y = max(i, j);
+    const int c0 = __float_as_int(prm[x].z);
+    const int c1 = __float_as_int(prm[x + 1].z);
+    if (c0 == y || c1 == y || c0 == y + 1 || c1 == y + 1) return false;
+    if (y - x <= 2) return false;
+    const v3 a0 = xyz(pos[x]);
+    const v3 a1 = xyz(pos[x + 1]) - a0;
+    const v3 b0 = xyz(pos[y]) - a0;
+    const v3 b1 = xyz(pos[y + 1]) - a0;
+    float u, w;
+    closest_uv(v3{0.f, 0.f, 0.f}, a1, b0, b1, u, w);
+    const v3 nrm = u * a1 - mixv(b0, b1, w);
+    const float d2 = len2(nrm);
+    float r = 2.0f * P->scaledDetectionRadius;
+    if (P->useVelocityRadius) {
+        r += vel_pad(dx, x);
+        r += vel_pad(dx, y);
+    }
+    return d2 < r * r;
+}
+
+__global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __restrict__ P,
+                                                         const int32_t* __restrict__ order,
+                                                         const float4* __restrict__ aabb,
+                                                         const float4* __restrict__ nodes,
+                                                         const float4* __restrict__ pos,
+                                                         const float4* __restrict__ prm,
+                                                         const float4* __restrict__ dx,
+                                                         int32_t* __restrict__ numCols,
+                                                         int32_t* __restrict__ cols,
+                                                         int32_t* __restrict__ err) {
+    __shared__ int32_t stk[kStack][kQueryThreads];
+    const int n = P->n;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int i = order[k];
+    if (!(__float_as_uint(prm[i].w) & YB_FLAG_HAS_NEXT)) {
+        numCols[i] = 0;
+        return;
+    }
+    const v3 qlo = xyz(aabb[2 * i]);
+    const v3 qhi = xyz(aabb[2 * i + 1]);
+    const int t = threadIdx.x;
+    int sp = 0;
+    int node = 0;
+    int cnt = 0;
+    bool overflowStack = false;
+    while (true) {
+        const float4 l0 = nodes[4 * node + 0];
+        const float4 l1 = nodes[4 * node + 1];
+        const float4 r0 = nodes[4 * node + 2];
+        const float4 r1 = nodes[4 * node + 3];
+        const int cl = __float_as_int(l0.w);
+        const int cr = __float_as_int(r0.w);
+        const bool hl = overlap(qlo, qhi, l0, l1);
+        const bool hr = overlap(qlo, qhi, r0, r1);
+        if (hl && cl < 0) {
+            const int j = ~cl;
+            if (j != i && pair_accepted(P, pos, prm, dx, i, j)) {
+                if (cnt < kMaxCols) cols[i + n * cnt] = j;
+                cnt++;
+            }
+        }
+        if (hr && cr < 0) {
+            const int j = ~cr;
+            if (j != i && pair_accepted(P, pos, prm, dx, i, j)) {
+                if (cnt < kMaxCols) cols[i + n * cnt] = j;
+                cnt++;
+            }
+        }
+        const bool tl = hl && cl >= 0;
+        const bool tr = hr && cr >= 0;
+        if (tl || tr) {
+            node = tl ? cl : cr;
+            if (tl && tr) {
+                if (sp < kStack) {
+                    stk[sp][t] = cr;
+                    sp++;
+                } else {
+                    overflowStack = true;
+                }
+            }
+        } else {
+            if (sp == 0) break;
+            sp--;
+            node = stk[sp][t];
+        }
+    }
+    if (cnt > kMaxCols) {
+        err[0] = YB_ERROR_MAX_COLLISIONS_PER_SEGMENT_EXCEEDED;
+        cnt = kMaxCols;
+    }
+    if (overflowStack) err[0] = YB_ERROR_MAX_COLLISIONS_PER_SEGMENT_EXCEEDED;
+    numCols[i] = cnt;
+}
+
+static inline int blocks(int n, int b) { return (n + b - 1) / b; }
+
+// ---- launchers -------------------------------------------------------------
+void launch_init(yb_sim* s) {
+    hipLaunchKernelGGL(k_init, dim3(blocks(s->n, 256)), dim3(256), 0, s->stream, s->b.params,
+                       s->b.pos, s->b.vel, s->b.lastVel, s->b.yv, s->b.dx[0]);
+}
+
+void launch_aabb(yb_sim* s) {
+    hipLaunchKernelGGL(k_aabb, dim3(blocks(s->n, 256)), dim3(256), 0, s->stream, s->b.params,
+                       s->b.pos, s->b.prm, s->b.dx[0], s->b.aabb);
+}
+
+void launch_rebuild(yb_sim* s) {
+    const int n = s->n;
+    hipLaunchKernelGGL(k_box_partial, dim3(s->boxBlocks), dim3(kBoxThreads), 0, s->stream, n,
+                       s->b.aabb, s->b.blockBox);
+    hipLaunchKernelGGL(k_box_final, dim3(1), dim3(kBoxThreads), 0, s->stream, s->boxBlocks,
+                       s->b.blockBox, s->b.sceneBox);
+    hipLaunchKernelGGL(k_morton, dim3(blocks(n, 256)), dim3(256), 0, s->stream, n, s->b.prm,
+                       s->b.aabb, s->b.sceneBox, s->b.morton, s->b.keys[0], s->b.vals[0]);
+    const int tiles = s->sortTiles;
+    for (int pass = 0; pass < 4; pass++) {
+        const int in = pass & 1;
+        const int shift = pass * kSortBits;
+        hipLaunchKernelGGL(k_sort_hist, dim3(tiles), dim3(kSortThreads), 0, s->stream, n, shift,
+                           tiles, s->b.keys[in], s->b.hist);
+        hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(kScanThreads), 0, s->stream,
+                           tiles * kSortBins, s->b.hist);
+        hipLaunchKernelGGL(k_sort_scatter, dim3(tiles), dim3(kSortThreads), 0, s->stream, n, shift,
+                           tiles, s->b.keys[in], s->b.vals[in], s->b.keys[in ^ 1],
+                           s->b.vals[in ^ 1], s->b.hist);
+    }
+    // After 4 passes the sorted pairs are back in keys[0] / vals[0] == order.
+    hipLaunchKernelGGL(k_karras, dim3(blocks(n - 1, 256)), dim3(256), 0, s->stream, n,
+                       s->b.keys[0], s->b.order, s->b.nodes, s->b.parent);
+}
+
+void launch_refit(yb_sim* s) {
+    launch_fill_i32(s, s->b.visit, 0, s->n - 1);
+    hipLaunchKernelGGL(k_refit, dim3(blocks(s->n, 256)), dim3(256), 0, s->stream, s->n,
+                       s->b.order, s->b.aabb, s->b.nodes, s->b.parent, s->b.visit,
+                       s->b.sceneBox);
+}
+
+void launch_query(yb_sim* s) {
+    hipLaunchKernelGGL(k_query, dim3(blocks(s->n, kQueryThreads)), dim3(kQueryThreads), 0,
+                       s->stream, s->b.params, s->b.order, s->b.aabb, s->b.nodes, s->b.pos,
+                       s->b.prm, s->b.dx[0], s->b.numCols, s->b.cols, s->b.err);
+}
+
+}  // namespace yb

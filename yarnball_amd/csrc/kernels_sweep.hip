@@ -1,0 +1,416 @@
+// kernels_sweep.hip — per-substep solver kernels of the stepper on gfx950.
+//
+//   k_step_limit   per-segment no-tunnelling step bound   (collision.cu:121-159)
+//   k_sweep_dx     one block-Jacobi Newton sweep on dx    (cosserat.cu:20-207)
+//   k_sweep_q      closed-form Stable-Cosserat frame update (cosserat.cu:209-253)
+//   k_end          velocity update + advection             (iteration.cu:48-61)
+//   k_upload / k_download   AoS <-> SoA transposes        (YarnBall.cu:239-261)
+//
+// Schedule: strict Jacobi.  Sweep k reads dx[k&1], q[k&1] and writes dx[~k&1];
+// the frame update then reads the new dx and the old frames and writes
+// q[~k&1].  The reference updates both arrays in place (asynchronous across
+// blocks); the double buffers make every iterate reproducible.
+//
+// Wave mapping of k_sweep_dx: each 64-lane wavefront owns 63 consecutive
+// vertices.  Lane l works on vertex base + l - 1: it evaluates its own segment
+// (i, i+1) -- stretch-shear plus all contacts of that segment -- and hands the
+// endpoint-1 force/Hessian to lane l+1 with a cross-lane shift.  Lane 0 is the
+// halo that only feeds lane 1.  This replaces the reference's 2-sector LDS
+// block reduction with register-only wavefront exchange.
+#include "yb_internal.h"
+#include "ybmath.h"
+
+namespace yb {
+
+__device__ __forceinline__ v3 xyz(float4 a) { return v3{a.x, a.y, a.z}; }
+__device__ __forceinline__ rot asrot(float4 a) { return rot{a.x, a.y, a.z, a.w}; }
+__device__ __forceinline__ float4 f4(v3 a, float w) { return make_float4(a.x, a.y, a.z, w); }
+__device__ __forceinline__ float4 f4r(rot a) { return make_float4(a.x, a.y, a.z, a.w); }
+
+// ---------------------------------------------------------------------------
+// Step limit (collision.cu:121-159): the largest move for which the padded
+// broadphase query still sees every possible contact, and at most 45% of the
+// distance to every listed contact.
+__global__ __launch_bounds__(256) void k_step_limit(const DevParams* __restrict__ P,
+                                                    const float4* __restrict__ pos,
+                                                    const float4* __restrict__ prm,
+                                                    float4* __restrict__ dx,
+                                                    const int32_t* __restrict__ numCols,
+                                                    const int32_t* __restrict__ cols) {
+    const int n = P->n;
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    float minDist = INFINITY;
+    const uint32_t flags = __float_as_uint(prm[v].w);
+    float4 d4 = dx[v];
+    if (P->useStepSizeLimit && (flags & YB_FLAG_HAS_NEXT)) {
+        v3 a0 = xyz(pos[v]);
+        v3 a1 = xyz(pos[v + 1]) - a0;
+        minDist = P->detectionRadius * (P->detectionScaler - 1.0f);
+        if (P->useVelocityRadius) minDist += len(xyz(d4));
+        const int nc = numCols[v];
+        for (int i = 0; i < nc; i++) {
+            int o = cols[v + n * i];
+            v3 b0 = xyz(pos[o]) - a0;
+            v3 b1 = xyz(pos[o + 1]) - a0;
+            float u, w;
+            closest_uv(v3{0.f, 0.f, 0.f}, a1, b0, b1, u, w);
+            v3 nrm = u * a1 - mixv(b0, b1, w);
+            minDist = fminf(minDist, 0.45f * len(nrm));
+        }
+    }
+    d4.w = minDist;
+    dx[v] = d4;
+}
+
+// ---------------------------------------------------------------------------
+// One Newton sweep (cosserat.cu:20-207).  For vertex i:
+//   H = m/h^2 I + glue + stretch(seg i-1, seg i) + barrier/friction(contacts)
+//   f = m/h^2 ((y-x) - dx) + ...
+//   dx += accelerationRatio * H^{-1} f, clamped to the step limit.
+__global__ __launch_bounds__(256) void k_sweep_dx(const DevParams* __restrict__ P,
+                                                  const float4* __restrict__ pos,
+                                                  const float4* __restrict__ prm,
+                                                  const float4* __restrict__ yv,
+                                                  const float4* __restrict__ q,
+                                                  const float4* __restrict__ dxIn,
+                                                  float4* __restrict__ dxOut,
+                                                  const int32_t* __restrict__ numCols,
+                                                  const int32_t* __restrict__ cols) {
+    const int n = P->n;
+    const int lane = threadIdx.x & 63;
+    const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int v = wave * 63 + lane - 1;
+    const bool in = (v >= 0) && (v < n);
+
+    const float h = P->h;
+    const float damping = P->dampingH;
+
+    v3 f = v3{0.f, 0.f, 0.f};
+    v3 f2 = v3{0.f, 0.f, 0.f};
+    sym3 H = sym_zero();
+    sym3 H2 = sym_zero();
+    float segLimit = INFINITY;
+    float4 p4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 d4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    uint32_t flags = 0;
+
+    if (in) {
+        p4 = pos[v];
+        const float4 r4 = prm[v];
+        d4 = dxIn[v];
+        flags = __float_as_uint(r4.w);
+        const int cid = __float_as_int(r4.z);
+        const v3 x0 = xyz(p4);
+        const v3 dx = xyz(d4);
+
+        // Inertia (cosserat.cu:40-44); yv holds y - x.
+        const float mh2 = 1.0f / (p4.w * h * h);
+        H = sym_diag(mh2);
+        f = mh2 * (xyz(yv[v]) - dx);
+
+        // Glue spring to a connected vertex (cosserat.cu:47-53).
+        if (cid >= 0) {
+            const float stiffness = 40.0f;
+            v3 pc = xyz(pos[cid]);
+            v3 dc = xyz(dxIn[cid]);
+            f -= stiffness * (((x0 - pc) + (dx - dc)) + damping * dx);
+            sym_add_diag(H, stiffness * (1.0f + damping));
+        }
+
+        if (flags & YB_FLAG_HAS_NEXT) {
+            const v3 p1 = xyz(pos[v + 1]);
+            const v3 p1dx = xyz(dxIn[v + 1]);
+            segLimit = d4.w;
+
+            // Cosserat stretch-shear (cosserat.cu:67-81).
+            const float invl = 1.0f / r4.x;
+            const v3 e = (p1 - x0) + (p1dx - dx);
+            const v3 c = e * invl - rrotate_e1(asrot(q[v]));
+            const float k = r4.y * invl;
+            float d = k * invl;
+            f += k * c - (damping * d) * dx;
+            f2 = -k * c - (damping * d) * p1dx;
+            d *= 1.0f + damping;
+            sym_add_diag(H, d);
+            sym_add_diag(H2, d);
+
+            // IPC barrier + lagged friction over this segment's contacts
+            // (cosserat.cu:83-148).
+            const float fricK = P->kFriction;
+            const float invb = 1.0f / P->barrierThickness;
+            const float radius2 = 2.0f * P->radius;
+            const float fricMu = P->frictionCoeff;
+            const float kCol = P->kCollision * invb;
+            const int nc = numCols[v];
+            for (int i = 0; i < nc; i++) {
+                const int o = cols[v + n * i];
+                const v3 b0 = xyz(pos[o]);
+                const v3 b1 = xyz(pos[o + 1]);
+                const v3 db0 = xyz(dxIn[o]);
+                const v3 db1 = xyz(dxIn[o + 1]);
+                float u, w;
+                closest_uv(v3{0.f, 0.f, 0.f}, e, (b0 - x0) + (db0 - dx), (b1 - x0) + (db1 - dx), u,
+                           w);
+                const v3 dpos = mixv(x0, p1, u) - mixv(b0, b1, w);
+                const v3 ddpos = mixv(dx, p1dx, u) - mixv(db0, db1, w);
+                v3 nrm = dpos + ddpos;
+                float dist = len(nrm);
+                nrm *= 1.0f / dist;
+                const float w1 = u;
+                const float w0 = 1.0f - u;
+
+                float dd = (dist - radius2) * invb;
+                if (dd > 1.0f) continue;  // outside the barrier
+                dd = fmaxf(dd, 1e-3f);
+
+                // b(d) = -(d-1)^2 ln d; ff = -kCol b'(d), dH = kCol/b b''(d)
+                const float invd = 1.0f / dd;
+                const float logd = logf(dd);
+                float dH = (-3.0f + (2.0f + invd) * invd - 2.0f * logd) * kCol * invb;
+                const float ff = -(1.0f - dd) * (dd - 1.0f + 2.0f * dd * logd) * invd * kCol;
+                f += (ff * w0 - damping * dH * w0 * w0 * dot(nrm, dx)) * nrm;
+                f2 += (ff * w1 - damping * dH * w1 * w1 * dot(nrm, p1dx)) * nrm;
+                dH *= 1.0f + damping;
+                const sym3 op = sym_outer(nrm);
+                sym_axpy(H, op, dH * w0 * w0);
+                sym_axpy(H2, op, dH * w1 * w1);
+
+                // Lagged friction on the tangential relative displacement.
+                const v3 uu = ddpos - dot(nrm, ddpos) * nrm;
+                const float ul = len(uu);
+                if (ul > 0.0f) {
+                    const float f1 = fminf(fricK, fricMu * ff / ul);
+                    const float s0 = w0 * w0 * f1;
+                    const float s1 = w1 * w1 * f1;
+                    f -= (f1 * w0) * uu;
+                    f2 -= (f1 * w1) * uu;
+                    // H -= (nn^T - I) s
+                    sym_axpy(H, op, -s0);
+                    sym_add_diag(H, s0);
+                    sym_axpy(H2, op, -s1);
+                    sym_add_diag(H2, s1);
+                }
+            }
+        }
+    }
+
+    // Hand segment (i-1)'s endpoint-1 terms to vertex i (cosserat.cu:175-193).
+    const float pf0 = __shfl_up(f2.x, 1, 64);
+    const float pf1 = __shfl_up(f2.y, 1, 64);
+    const float pf2 = __shfl_up(f2.z, 1, 64);
+    float ph[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) ph[k] = __shfl_up(H2.a[k], 1, 64);
+    const float pl = __shfl_up(segLimit, 1, 64);
+
+    if (!in || lane == 0) return;
+
+    float limit = segLimit;
+    if (flags & YB_FLAG_HAS_PREV) {
+        limit = fminf(limit, pl);
+        f += v3{pf0, pf1, pf2};
+#pragma unroll
+        for (int k = 0; k < 6; k++) H.a[k] += ph[k];
+    }
+
+    float4 out = d4;
+    if (p4.w != 0.0f) {
+        // Local 3x3 solve (cosserat.cu:195-205).
+        v3 dx = xyz(d4);
+        dx += P->accelerationRatio * sym_solve(H, f);
+        const float l = len(dx);
+        if (l > limit && l > 0.0f) dx *= limit / l;
+        out = f4(dx, d4.w);
+    }
+    dxOut[v] = out;
+}
+
+// ---------------------------------------------------------------------------
+// Frame update (cosserat.cu:209-253): with v = -2 kStretch e / l and the
+// neighbours' rest-rotated frames b, q = normalize((v b e1) + (|v| + |b|) b).
+__global__ __launch_bounds__(256) void k_sweep_q(const DevParams* __restrict__ P,
+                                                 const float4* __restrict__ pos,
+                                                 const float4* __restrict__ prm,
+                                                 const float4* __restrict__ dxNew,
+                                                 const float4* __restrict__ qIn,
+                                                 float4* __restrict__ qOut,
+                                                 const float4* __restrict__ qRest) {
+    const int n = P->n;
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    const float4 r4 = prm[v];
+    const uint32_t flags = __float_as_uint(r4.w);
+    rot q0 = asrot(qIn[v]);
+    if (!(flags & YB_FLAG_FIX_ORIENTATION) && (flags & YB_FLAG_HAS_NEXT)) {
+        const v3 x0 = xyz(pos[v]);
+        const v3 p1 = xyz(pos[v + 1]);
+        const v3 dx = xyz(dxNew[v]);
+        const v3 p1dx = xyz(dxNew[v + 1]);
+        v3 e = (p1 - x0) + (p1dx - dx);
+        v3 vv = v3{e.x / r4.x, e.y / r4.x, e.z / r4.x};
+        vv *= -2.0f * r4.y;
+
+        rot b = rot{0.f, 0.f, 0.f, 0.f};
+        if (flags & YB_FLAG_HAS_PREV) {
+            const rot qr = asrot(qRest[v - 1]);
+            const rot qq = asrot(qIn[v - 1]);
+            const float s = rdot(rmul(rinv(qq), q0), qr) > 0.0f ? 1.0f : -1.0f;
+            b = radd(b, rscale(rmul(qq, qr), s));
+        }
+        if (flags & YB_FLAG_HAS_NEXT_ORIENTATION) {
+            const rot qr = asrot(qRest[v]);
+            const rot qq = asrot(qIn[v + 1]);
+            const float s = rdot(rmul(rinv(q0), qq), qr) > 0.0f ? 1.0f : -1.0f;
+            b = radd(b, rscale(rmul(qq, rinv(qr)), s));
+        }
+        const float lambda = len(vv) + rlen(b);
+        const rot t = rmul(rmul(mkrot(vv, 0.0f), b), rot{1.f, 0.f, 0.f, 0.f});
+        q0 = rnormalize(radd(t, rscale(b, lambda)));
+    }
+    qOut[v] = f4r(q0);
+}
+
+// ---------------------------------------------------------------------------
+// End of substep (iteration.cu:48-61).
+// When the sweep count is odd the final frames sit in the second buffer and are
+// copied back so that every substep starts from q[0].
+__global__ __launch_bounds__(256) void k_end(const DevParams* __restrict__ P,
+                                             float4* __restrict__ pos,
+                                             float4* __restrict__ vel,
+                                             const float4* __restrict__ dxFin,
+                                             const float4* __restrict__ qFin,
+                                             float4* __restrict__ q0) {
+    const int n = P->n;
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    const float h = P->h;
+    const float invH = 1.0f / h;
+    const v3 dx = xyz(dxFin[v]);
+    float4 p4 = pos[v];
+    if (p4.w != 0.0f) {
+        const float s = 1.0f - P->drag * h;
+        vel[v] = f4((dx * invH) * s, 0.0f);
+    }
+    p4.x += dx.x;
+    p4.y += dx.y;
+    p4.z += dx.z;
+    pos[v] = p4;
+    if (qFin != q0) q0[v] = qFin[v];
+}
+
+// ---------------------------------------------------------------------------
+// Upload / download transposes.  Staging layout: n yb_vertex records, then
+// 3n floats of velocity, 4n of frames, 4n of rest frames.
+__global__ void k_upload(int n, const yb_vertex* __restrict__ sv, const float* __restrict__ svel,
+                         const float* __restrict__ sq, const float* __restrict__ sqr,
+                         float4* __restrict__ pos, float4* __restrict__ prm,
+                         float4* __restrict__ vel, float4* __restrict__ q,
+                         float4* __restrict__ qRest) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    const yb_vertex a = sv[v];
+    pos[v] = make_float4(a.pos[0], a.pos[1], a.pos[2], a.invMass);
+    prm[v] = make_float4(a.lRest, a.kStretch, __int_as_float(a.connectionIndex),
+                         __uint_as_float(a.flags));
+    vel[v] = make_float4(svel[3 * v], svel[3 * v + 1], svel[3 * v + 2], 0.f);
+    q[v] = make_float4(sq[4 * v], sq[4 * v + 1], sq[4 * v + 2], sq[4 * v + 3]);
+    qRest[v] = make_float4(sqr[4 * v], sqr[4 * v + 1], sqr[4 * v + 2], sqr[4 * v + 3]);
+}
+
+__global__ void k_download(int n, yb_vertex* __restrict__ sv, float* __restrict__ svel,
+                           float* __restrict__ sq, const float4* __restrict__ pos,
+                           const float4* __restrict__ prm, const float4* __restrict__ vel,
+                           const float4* __restrict__ q) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    const float4 p = pos[v];
+    const float4 r = prm[v];
+    yb_vertex a;
+    a.pos[0] = p.x;
+    a.pos[1] = p.y;
+    a.pos[2] = p.z;
+    a.invMass = p.w;
+    a.lRest = r.x;
+    a.kStretch = r.y;
+    a.connectionIndex = __float_as_int(r.z);
+    a.flags = __float_as_uint(r.w);
+    sv[v] = a;
+    const float4 u = vel[v];
+    svel[3 * v] = u.x;
+    svel[3 * v + 1] = u.y;
+    svel[3 * v + 2] = u.z;
+    const float4 qq = q[v];
+    sq[4 * v] = qq.x;
+    sq[4 * v + 1] = qq.y;
+    sq[4 * v + 2] = qq.z;
+    sq[4 * v + 3] = qq.w;
+}
+
+__global__ void k_zero_vel(int n, float4* __restrict__ vel, float4* __restrict__ lastVel) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    vel[v] = make_float4(0.f, 0.f, 0.f, 0.f);
+    lastVel[v] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+__global__ void k_fill_i32(int32_t* __restrict__ p, int32_t val, int count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = val;
+}
+
+static inline int blocks(int n, int b) { return (n + b - 1) / b; }
+
+// ---- launchers -------------------------------------------------------------
+void launch_upload(yb_sim* s) {
+    const int n = s->n;
+    char* st = (char*)s->b.stage;
+    const yb_vertex* sv = (const yb_vertex*)st;
+    const float* svel = (const float*)(st + sizeof(yb_vertex) * n);
+    const float* sq = svel + 3 * (size_t)n;
+    const float* sqr = sq + 4 * (size_t)n;
+    hipLaunchKernelGGL(k_upload, dim3(blocks(n, 256)), dim3(256), 0, s->stream, n, sv, svel, sq,
+                       sqr, s->b.pos, s->b.prm, s->b.vel, s->b.q[0], s->b.qRest);
+}
+
+void launch_download(yb_sim* s) {
+    const int n = s->n;
+    char* st = (char*)s->b.stage;
+    yb_vertex* sv = (yb_vertex*)st;
+    float* svel = (float*)(st + sizeof(yb_vertex) * n);
+    float* sq = svel + 3 * (size_t)n;
+    hipLaunchKernelGGL(k_download, dim3(blocks(n, 256)), dim3(256), 0, s->stream, n, sv, svel, sq,
+                       s->b.pos, s->b.prm, s->b.vel, s->b.q[0]);
+}
+
+void launch_zero_vel(yb_sim* s) {
+    hipLaunchKernelGGL(k_zero_vel, dim3(blocks(s->n, 256)), dim3(256), 0, s->stream, s->n,
+                       s->b.vel, s->b.lastVel);
+}
+
+void launch_fill_i32(yb_sim* s, int32_t* p, int32_t v, int count) {
+    if (count <= 0) return;
+    hipLaunchKernelGGL(k_fill_i32, dim3(blocks(count, 256)), dim3(256), 0, s->stream, p, v, count);
+}
+
+void launch_step_limit(yb_sim* s) {
+    hipLaunchKernelGGL(k_step_limit, dim3(blocks(s->n, 256)), dim3(256), 0, s->stream, s->b.params,
+                       s->b.pos, s->b.prm, s->b.dx[0], s->b.numCols, s->b.cols);
+}
+
+void launch_sweep(yb_sim* s, int in) {
+    const int out = in ^ 1;
+    const int waves = blocks(s->n + 1, 63);
+    hipLaunchKernelGGL(k_sweep_dx, dim3(blocks(waves * 64, 256)), dim3(256), 0, s->stream,
+                       s->b.params, s->b.pos, s->b.prm, s->b.yv, s->b.q[in], s->b.dx[in],
+                       s->b.dx[out], s->b.numCols, s->b.cols);
+    hipLaunchKernelGGL(k_sweep_q, dim3(blocks(s->n, 256)), dim3(256), 0, s->stream, s->b.params,
+                       s->b.pos, s->b.prm, s->b.dx[out], s->b.q[in], s->b.q[out], s->b.qRest);
+}
+
+void launch_end(yb_sim* s, int fin) {
+    hipLaunchKernelGGL(k_end, dim3(blocks(s->n, 256)), dim3(256), 0, s->stream, s->b.params,
+                       s->b.pos, s->b.vel, s->b.dx[fin], s->b.q[fin], s->b.q[0]);
+}
+
+}  // namespace yb

@@ -1,0 +1,120 @@
+// yb_internal.h — device-side state of one simulation handle and the kernel
+// launchers shared between the translation units of libyarnball_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/yarnball_hip.h"
+
+namespace yb {
+
+// Per-advance constants read by every kernel through one device pointer (the
+// role MetaData* plays in the reference, YarnBall.h:63-114).  Uploaded once per
+// advance outside the captured graphs, so graphs stay valid when the user edits
+// kCollision, gravity, etc. between frames.
+struct DevParams {
+    float gx, gy, gz;
+    float h;
+    float lastH;
+    float dampingH;  // meta.damping / h (cosserat.cu:29)
+    float drag;
+    float frictionCoeff;
+    float kCollision;
+    float kFriction;
+    float detectionRadius;
+    float scaledDetectionRadius;
+    float radius;
+    float accelerationRatio;
+    float barrierThickness;
+    float detectionScaler;
+    int32_t useStepSizeLimit;
+    int32_t useVelocityRadius;
+    int32_t n;
+    int32_t pad_;
+};
+
+constexpr int kMaxCols = YB_MAX_COLLISIONS_PER_SEGMENT;
+constexpr int kSortBits = 8;
+constexpr int kSortBins = 1 << kSortBits;
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 8;
+constexpr int kSortTile = kSortThreads * kSortItems;
+
+// SoA device state.  Every per-vertex array is float4 (16 B, one dwordx4 per
+// lane, fully coalesced):
+//   pos   = (x, y, z, invMass)          prm = (lRest, kStretch, cid bits, flags bits)
+//   vel   = (v, 0)   lastVel = (v_prev, 0)   yv = (y - x, 0)
+//   dx[2] = ping-pong iterate (dx, stepLimit of segment i)
+//   q[2]  = ping-pong segment frames,   qRest = rest Darboux * stiffness
+struct Buffers {
+    float4* pos = nullptr;
+    float4* prm = nullptr;
+    float4* vel = nullptr;
+    float4* lastVel = nullptr;
+    float4* yv = nullptr;
+    float4* dx[2] = {nullptr, nullptr};
+    float4* q[2] = {nullptr, nullptr};
+    float4* qRest = nullptr;
+    int32_t* numCols = nullptr;
+    int32_t* cols = nullptr;  // slot-major: cols[seg + n*slot]
+    // broadphase
+    float4* aabb = nullptr;       // 2n: lo (xyz), hi (xyz) per segment
+    float4* sceneBox = nullptr;   // 2 float4: union of all segment boxes
+    float4* blockBox = nullptr;   // 2 per reduction block
+    uint32_t* morton = nullptr;   // n, indexed by segment
+    uint32_t* keys[2] = {nullptr, nullptr};
+    int32_t* vals[2] = {nullptr, nullptr};
+    uint32_t* hist = nullptr;     // kSortBins * numTiles
+    int32_t* order = nullptr;     // sorted leaf -> segment
+    float4* nodes = nullptr;      // 4 per internal node: lo0, hi0, lo1, hi1 (child ids in .w)
+    int32_t* parent = nullptr;    // 2n-1: (parent << 1) | side
+    int32_t* visit = nullptr;     // n-1 refit counters
+    int32_t* err = nullptr;       // [0] error code, [1] warning code
+    DevParams* params = nullptr;
+    // upload / download staging
+    void* stage = nullptr;
+    size_t stageBytes = 0;
+};
+
+}  // namespace yb
+
+struct yb_sim {
+    int n = 0;
+    hipStream_t stream = nullptr;
+    yb::Buffers b;
+    int sortTiles = 0;
+    int boxBlocks = 0;
+    bool treeBuilt = false;
+    float lastBVHRebuild = INFINITY;  // sim time since the last full rebuild
+    int64_t stepCounter = 0;
+    // captured substep graphs: [detect mode 0=none,1=refit,2=rebuild]
+    hipGraphExec_t graphs[3] = {nullptr, nullptr, nullptr};
+    int graphItr = -1;
+    int graphLimit = -1;
+    int graphVel = -1;
+    float hostBox[8];
+};
+
+namespace yb {
+
+void set_message(const std::string& m);
+
+// ---- launchers (all enqueue on s->stream) ---------------------------------
+void launch_upload(yb_sim* s);     // staging AoS -> SoA
+void launch_download(yb_sim* s);   // SoA -> staging AoS
+void launch_zero_vel(yb_sim* s);
+void launch_init(yb_sim* s);                    // iteration.cu:7-41
+void launch_step_limit(yb_sim* s);              // collision.cu:121-172
+void launch_sweep(yb_sim* s, int in);           // cosserat.cu:20-253, reads dx[in], q[in]
+void launch_end(yb_sim* s, int fin);            // iteration.cu:48-61
+void launch_fill_i32(yb_sim* s, int32_t* p, int32_t v, int count);
+// broadphase (collision.cu:10-119 + the LBVH the reference links)
+void launch_aabb(yb_sim* s);
+void launch_rebuild(yb_sim* s);    // scene box, Morton, sort, Karras
+void launch_refit(yb_sim* s);
+void launch_query(yb_sim* s);      // traversal + narrowphase -> contact lists
+
+}  // namespace yb

@@ -1,0 +1,171 @@
+"""Python mirror of YarnBall::Sim (KittenEngine/YarnBall/YarnBall.h:116-206).
+
+Same names, argument meaning and error behaviour as the reference class; every
+call goes through the C ABI of libyarnball_hip.so.  Host arrays are numpy:
+
+* ``verts``   structured array of VERTEX_DTYPE (pos, invMass, lRest, kStretch,
+              connectionIndex, flags) -- the 32-byte Vertex record;
+* ``vels``    (n, 3) float32;
+* ``qs``      (n, 4) float32 rotors (x, y, z, w);
+* ``qRests``  (n, 4) float32 rest Darboux rotors scaled by the bend stiffness;
+* ``meta``    a ``Params`` ctypes struct with MetaData's field names.
+
+As in the reference, the device copy is authoritative between ``upload()`` and
+``download()`` (README.md:100-102).
+"""
+import ctypes
+import sys
+
+import numpy as np
+
+from . import _lib
+from ._lib import Params, VERTEX_DTYPE, YarnBallError, check, ptr
+
+
+class Sim:
+    ERROR_NONE = 0
+    ERROR_MAX_COLLISIONS_PER_SEGMENT_EXCEEDED = 1
+    WARNING_SEGMENT_INTERPENETRATION = 2
+
+    def __init__(self, numVerts):
+        # YarnBall.cu:11-51
+        if numVerts < 3:
+            raise RuntimeError("Too little vertices")
+        lib = _lib.load()
+        self._lib = lib
+        n = int(numVerts)
+        self.maxH = 1e-3
+        self.meta = Params()
+        lib.yb_default_params(ctypes.byref(self.meta), n, self.maxH)
+        self.verts = np.zeros(n, dtype=VERTEX_DTYPE)
+        self.vels = np.zeros((n, 3), dtype=np.float32)
+        self.qs = np.zeros((n, 4), dtype=np.float32)
+        self.qRests = np.zeros((n, 4), dtype=np.float32)
+        check(lib.yb_default_vertices(n, ptr(self.verts), ptr(self.vels), ptr(self.qs),
+                                      ptr(self.qRests)))
+        self.initialInvMasses = np.zeros(n, dtype=np.float32)
+        inf = np.float32(np.inf)
+        self.currentBounds = (np.array([inf] * 3, np.float32), np.array([-inf] * 3, np.float32))
+        self.lastErrorCode = self.ERROR_NONE
+        self.lastWarningCode = self.ERROR_NONE
+        self.printErrors = True
+        self._h = None
+
+    # -- lifecycle -------------------------------------------------------------
+    @property
+    def numVerts(self):
+        return int(self.meta.numVerts)
+
+    def configure(self, density=1e-3):
+        """Sim::configure (YarnBall.cu:91-200)."""
+        n = self.numVerts
+        lib = self._lib
+        check(lib.yb_configure_host(n, ptr(self.verts), ptr(self.qs), ptr(self.qRests),
+                                    float(density), ptr(self.initialInvMasses),
+                                    ctypes.byref(self.meta)), "configure")
+        if self._h is None:
+            h = ctypes.c_void_p()
+            check(lib.yb_create(n, ctypes.byref(h)), "configure")
+            self._h = h
+        check(lib.yb_validate_params(ctypes.byref(self.meta)), "uploadMeta")
+        self.upload()
+
+    def _need_handle(self):
+        if self._h is None:
+            raise RuntimeError("No rest length. Must call configure()")
+
+    def setKStretch(self, k=1e-2):
+        """Sim::setKStretch (YarnBall.cu:202-210)."""
+        self._need_handle()
+        check(self._lib.yb_set_kstretch_host(self.numVerts, ptr(self.verts), float(k)))
+
+    def setKBend(self, k=3e-9):
+        """Sim::setKBend (YarnBall.cu:212-224)."""
+        self._need_handle()
+        check(self._lib.yb_set_kbend_host(self.numVerts, ptr(self.verts), ptr(self.qRests),
+                                          float(k)))
+
+    def upload(self):
+        """Sim::upload (YarnBall.cu:247-254)."""
+        self._need_handle()
+        check(self._lib.yb_upload(self._h, ptr(self.verts), ptr(self.vels), ptr(self.qs),
+                                  ptr(self.qRests)), "upload")
+
+    def download(self):
+        """Sim::download (YarnBall.cu:256-261)."""
+        if self._h is None:
+            return
+        check(self._lib.yb_download(self._h, ptr(self.verts), ptr(self.vels), ptr(self.qs)),
+              "download")
+
+    def zeroVelocities(self):
+        """Sim::zeroVelocities (YarnBall.cu:271-274)."""
+        if self._h is None:
+            return
+        check(self._lib.yb_zero_velocities(self._h))
+
+    # -- stepping --------------------------------------------------------------
+    def advance(self, dt):
+        """Sim::advance (step.cpp:101-120) + checkErrors (YarnBall.cu:276-304).
+
+        Returns the simulated time advanced."""
+        if dt <= 0:
+            return 0.0
+        self._need_handle()
+        warn = ctypes.c_int(0)
+        rc = self._lib.yb_advance(self._h, ctypes.byref(self.meta), float(dt), float(self.maxH),
+                                  ctypes.byref(warn))
+        if warn.value == self.WARNING_SEGMENT_INTERPENETRATION:
+            self.lastWarningCode = warn.value
+            if self.printErrors:
+                print("WARNING: Interpenetration detection. This can be due to unstable contacts",
+                      file=sys.stderr)
+        if rc == self.ERROR_MAX_COLLISIONS_PER_SEGMENT_EXCEEDED:
+            self.lastErrorCode = rc
+            print("ERROR: MAX_COLLISIONS_PER_SEGMENT exceeded. Current simulation state may be "
+                  "corrupted!", file=sys.stderr)
+            raise YarnBallError(rc, "MAX_COLLISIONS_PER_SEGMENT exceeded")
+        check(rc, "advance")
+        return float(dt)
+
+    def step(self, dt=None):
+        """Sim::step (step.cpp:122-124): one maxH step."""
+        self.advance(self.maxH)
+
+    def stepCount(self):
+        return int(self._lib.yb_step_count(self._h)) if self._h is not None else 0
+
+    def bounds(self):
+        """Root box of the broadphase tree (Sim::bounds, YarnBall.cu:87-89)."""
+        out = np.zeros(6, np.float32)
+        if self._h is not None:
+            check(self._lib.yb_bounds(self._h, ptr(out)))
+        return out[:3].copy(), out[3:].copy()
+
+    def collisionCounts(self):
+        out = np.zeros(self.numVerts, np.int32)
+        self._need_handle()
+        check(self._lib.yb_collision_counts(self._h, ptr(out)))
+        return out
+
+    def printCollisionStats(self):
+        """Sim::printCollisionStats (statistics.cpp:5-24)."""
+        c = self.collisionCounts()
+        print(f"Total collisions: {int(c[c > 0].sum()) // 2}")
+        print(f"Segments with collisions: {int((c > 0).sum())}/{self.numVerts}")
+
+    # -- stage-level access (parity tests) --------------------------------------
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h is not None:
+            self._lib.yb_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
