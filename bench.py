@@ -1,0 +1,194 @@
+#!/usr/bin/env python3
+"""Throughput benchmark of the YarnBall stepper on one MI355X.
+
+Workload: the knit patch, 256 rows x 4096 vertices = 1,048,576 vertices
+(interlocked sinusoidal rows, dense contact from t = 0, one edge pinned, the
+opposite edge pulled at 5 cm/s, gravity -z), 1/30 s frames, maxH = 1e-3
+(34 substeps per frame), detection every substep, numItr sweeps per substep.
+
+A step is one frame: Sim.advance(1/30) over the whole patch, state resident
+in HBM.  value = vertices x substeps / second over the timed frames.
+
+Also reported:
+  roofline     the dominant kernel's algorithmic bytes per launch / its average
+               launch time, measured live with HIP events on the stepper's
+               stream (yb_profile), against MI355X's 8 TB/s HBM peak;
+  cpu_baseline the CPU oracle (FP64, OpenMP) on a bounded sample of the same
+               workload, on this host's cores.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes per vertex of each stage (compulsory FP32 SoA traffic;
+# neighbour and contact gathers excluded), c = mean contacts per segment.
+def stage_bytes(stage, c):
+    return {
+        "init": 76.0,
+        "step_limit": 36.0 + 4.0 * c,
+        "sweep_dx": 96.0 + 4.0 * c,
+        "sweep_q": 84.0,
+        "end": 52.0,
+        "aabb": 52.0,
+    }.get(stage)
+
+
+def build_workload(name, numItr):
+    from yarnball_amd import scenes
+
+    if name == "knit_patch":
+        return scenes.knit_patch(rows=256, per=4096, numItr=numItr)
+    if name == "twisted_cable":
+        return scenes.twisted_cable(strands=64, per=2048, numItr=numItr)
+    if name == "knit_patch_small":
+        return scenes.knit_patch(rows=32, per=1024, numItr=numItr)
+    raise SystemExit(f"unknown workload {name}")
+
+
+def cpu_baseline(sc, substeps, threads):
+    from oracle import Oracle, load
+
+    lib = load()
+    if threads > 0:
+        lib.orc_set_threads(threads)
+    used = lib.orc_get_threads()
+    orc = Oracle(sc)
+    p = sc.params.copy()
+    # one untimed substep (first-touch of the oracle's arrays)
+    orc.advance(p, sc.maxH, sc.maxH)
+    t0 = time.perf_counter()
+    for _ in range(substeps):
+        orc.advance(p, sc.maxH, sc.maxH)
+    dt = time.perf_counter() - t0
+    orc.close()
+    return sc.n * substeps / dt, used, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="knit_patch")
+    ap.add_argument("--numItr", type=int, default=8)
+    ap.add_argument("--cpu-substeps", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-frames", type=int, default=1)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the stepper runs on a single MI355X "
+              "(multi-GPU sharding is design-only)", file=sys.stderr)
+        sys.exit(2)
+
+    from yarnball_amd import Sim, scenes
+    from yarnball_amd._lib import STAGES, check, ptr
+
+    t_build = time.perf_counter()
+    sc = build_workload(args.workload, args.numItr)
+    print(f"[bench] workload {sc.name}: {sc.n} vertices, built in "
+          f"{time.perf_counter() - t_build:.1f}s", file=sys.stderr, flush=True)
+
+    sim = Sim(sc.n)
+    scenes.load_into(sim, sc)
+    frame = np.float32(sc.frame_dt)
+    substeps_per_frame = int(np.ceil(frame / np.float32(sc.maxH)))
+
+    for w in range(args.warmup):
+        sim.advance(frame)
+        print(f"[bench] warmup frame {w} done", file=sys.stderr, flush=True)
+
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        sim.advance(frame)  # synchronous: returns after the stream has drained
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    vsub = sc.n * substeps_per_frame * args.steps
+    value = vsub / elapsed
+    print(f"[bench] timed {args.steps} frames in {elapsed:.3f}s", file=sys.stderr, flush=True)
+
+    # Live per-stage timing on the stepper's stream.
+    ms = np.zeros(len(STAGES), np.float64)
+    nl = np.zeros(len(STAGES), np.int32)
+    p = sim.meta
+    check(sim._lib.yb_profile(sim.handle, ctypes.byref(p), np.float32(frame / substeps_per_frame),
+                              substeps_per_frame * args.profile_frames, ptr(ms), ptr(nl)))
+    counts = sim.collisionCounts()
+    segs = (sim.verts["flags"] & 2) != 0
+    c = float(counts[segs].mean()) if segs.any() else 0.0
+    per_launch = {s: (ms[i] / nl[i] if nl[i] else 0.0) for i, s in enumerate(STAGES)}
+    total = {s: ms[i] for i, s in enumerate(STAGES)}
+    dominant = max(total, key=total.get)
+    for s in STAGES:
+        i = STAGES.index(s)
+        if nl[i]:
+            b = stage_bytes(s, c)
+            gbs = (b * sc.n / (per_launch[s] * 1e-3) / 1e9) if b else float("nan")
+            print(f"[bench] stage {s:11s} launches {nl[i]:5d} avg {per_launch[s]*1e3:9.1f} us "
+                  f"total {ms[i]:8.2f} ms  alg GB/s {gbs:8.1f}", file=sys.stderr)
+    b = stage_bytes(dominant, c) or 0.0
+    alg_bytes = b * sc.n
+    achieved = alg_bytes / (per_launch[dominant] * 1e-3) / 1e9 if per_launch[dominant] else 0.0
+    roofline = {
+        "kernel": dominant,
+        "bound": "hbm",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "alg_bytes_per_launch": int(alg_bytes),
+        "avg_launch_us": round(per_launch[dominant] * 1e3, 2),
+        "mean_contacts_per_segment": round(c, 3),
+    }
+
+    cpu = None
+    if not args.no_cpu:
+        v, used, dt = cpu_baseline(sc, args.cpu_substeps, args.cpu_threads)
+        cpu = {"value": round(v, 1), "unit": "vertex-substeps/s", "cores": int(used),
+               "kind": "port",
+               "sample": f"FP64 OpenMP oracle, {args.cpu_substeps} substeps of {sc.name} "
+                         f"({dt:.1f}s)"}
+        print(f"[bench] cpu baseline {v:.3e} vertex-substeps/s on {used} threads",
+              file=sys.stderr)
+
+    out = {
+        "metric": "vertex-substeps/s",
+        "value": round(value, 1),
+        "unit": "vertex-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": sc.name,
+            "vertices": sc.n,
+            "substeps_per_step": substeps_per_frame,
+            "numItr": int(sim.meta.numItr),
+            "method": "plain",
+            "detection_period": int(sim.meta.detectionPeriod),
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    sim.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -174,6 +174,27 @@ int yb_debug_write_dx(yb_sim* s, const float* dx);
 int yb_debug_read_last_vels(yb_sim* s, float* lastVels);
 int yb_debug_write_last_vels(yb_sim* s, const float* lastVels);
 
+/* ---- live per-stage timing (measurement harness) --------------------------
+ * Runs `substeps` substeps of length h with the yb_advance schedule, enqueued
+ * without graphs and with a HIP event between consecutive kernels on the
+ * handle's stream.  ms[k] receives the summed GPU time and launches[k] the
+ * launch count of stage k (arrays of YB_NUM_STAGES).  No counterpart in the
+ * reference (which has no timing instrument beyond a wall clock,
+ * KittenEngine/main.cpp:123-149). */
+enum {
+    YB_STAGE_INIT = 0,        /* iteration.cu:7-41 */
+    YB_STAGE_AABB = 1,        /* collision.cu:10-37 */
+    YB_STAGE_REBUILD = 2,     /* LBVH compute: scene box, Morton, sort, Karras */
+    YB_STAGE_REFIT = 3,       /* LBVH refit */
+    YB_STAGE_QUERY = 4,       /* LBVH query + collision.cu:39-92 */
+    YB_STAGE_STEP_LIMIT = 5,  /* collision.cu:121-159 */
+    YB_STAGE_SWEEP_DX = 6,    /* cosserat.cu:20-207 */
+    YB_STAGE_SWEEP_Q = 7,     /* cosserat.cu:209-253 */
+    YB_STAGE_END = 8,         /* iteration.cu:48-61 */
+    YB_NUM_STAGES = 9
+};
+int yb_profile(yb_sim* s, yb_params* p, float h, int substeps, double* ms, int32_t* launches);
+
 #ifdef __cplusplus
 }
 #endif

@@ -280,15 +280,17 @@ extern "C" void orc32_morton(int n, const uint32_t* flags, const float* aabb6, c
                              uint32_t* codes) {
     const F3 slo = {box6[0], box6[1], box6[2]};
     const F3 shi = {box6[3], box6[4], box6[5]};
-    const F3 ext = sub(shi, slo);
+    const F3 e3 = sub(shi, slo);
+    // cubic cells spanning the largest extent
+    const float ext = std::fmax(std::fmax(e3.x, e3.y), e3.z);
 #pragma omp parallel for schedule(static)
     for (int v = 0; v < n; v++) {
         uint32_t code = 0xFFFFFFFFu;
         if (has(flags[v], YB_FLAG_HAS_NEXT)) {
             const float* a = aabb6 + 6 * (size_t)v;
             const F3 c = mulr(add(F3{a[0], a[1], a[2]}, F3{a[3], a[4], a[5]}), 0.5f);
-            code = (expand10(quant(c.x, slo.x, ext.x)) << 2) |
-                   (expand10(quant(c.y, slo.y, ext.y)) << 1) | expand10(quant(c.z, slo.z, ext.z));
+            code = (expand10(quant(c.x, slo.x, ext)) << 2) |
+                   (expand10(quant(c.y, slo.y, ext)) << 1) | expand10(quant(c.z, slo.z, ext));
         }
         codes[v] = code;
     }

@@ -72,27 +72,32 @@ def test_discrete_stages_bit_exact(name, make):
     sim.close()
 
 
-@pytest.mark.gpu
-def test_karras_tree_invariants():
-    sim, sc, p, lv, g = prologue(scenes.knit_patch(rows=16, per=512))
-    n = sc.n
+TREE_SCENES = [
+    ("knit_16x512", lambda: scenes.knit_patch(rows=16, per=512)),   # 8 refit blocks
+    ("cable_19x513", lambda: scenes.twisted_cable(strands=19, per=513)),  # ragged last block
+    ("readme", scenes.readme_scene),                                # single block
+]
+
+
+def check_tree(sim, n, order, aabb):
+    """Structural invariants of the Karras tree and exactness of refit boxes."""
+    import sys
+
     child = np.zeros(2 * (n - 1), np.int32)
     parent = np.zeros(2 * n - 1, np.int32)
     box = np.zeros(16 * (n - 1), np.float32)
     check(sim._lib.yb_debug_tree(sim.handle, ptr(child), ptr(parent), ptr(box)))
     child = child.reshape(n - 1, 2)
     box = box.reshape(n - 1, 4, 4)
-    order = g["order"]
     rank = np.empty(n, np.int64)
     rank[order] = np.arange(n)
     assert parent[0] == -1
-    # Each internal node covers a contiguous range of sorted leaves; boxes are
-    # exact unions; parent pointers agree with child pointers.
-    aabb = g["aabb"]
     seen = np.zeros(n, np.int32)
+    sys.setrecursionlimit(10000)
 
     def walk(node):
         lo_r, hi_r = n, -1
+        left_hi = None
         for side in range(2):
             c = int(child[node, side])
             if c < 0:
@@ -102,11 +107,13 @@ def test_karras_tree_invariants():
                 seen[seg] += 1
                 lo, hi = aabb[seg, 0:3], aabb[seg, 4:7]
                 rlo, rhi = k, k
+                # leaf slot carries the leaf's signed velocity pad
+                assert box[node, 2 * side + 1, 3] == aabb[seg, 7]
             else:
                 assert parent[c] == (node << 1 | side)
                 rlo, rhi, lo, hi = walk(c)
-            assert np.array_equal(box[node, 2 * side, :3], lo)
-            assert np.array_equal(box[node, 2 * side + 1, :3], hi)
+            assert np.array_equal(box[node, 2 * side, :3], lo), f"node {node} side {side} lo"
+            assert np.array_equal(box[node, 2 * side + 1, :3], hi), f"node {node} side {side} hi"
             lo_r, hi_r = min(lo_r, rlo), max(hi_r, rhi)
             if side == 0:
                 left_hi = rhi
@@ -116,9 +123,27 @@ def test_karras_tree_invariants():
         hi = np.maximum(box[node, 1, :3], box[node, 3, :3])
         return lo_r, hi_r, lo, hi
 
-    import sys
-    sys.setrecursionlimit(10000)
-    lo_r, hi_r, _, _ = walk(0)
+    lo_r, hi_r, rlo, rhi = walk(0)
     assert (lo_r, hi_r) == (0, n - 1)
     assert np.all(seen == 1), "every leaf exactly once"
+    root = np.concatenate(sim.bounds())
+    assert np.array_equal(root[:3], rlo) and np.array_equal(root[3:], rhi), "root box"
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,make", TREE_SCENES, ids=[s[0] for s in TREE_SCENES])
+def test_karras_tree_invariants(name, make):
+    sim, sc, p, lv, g = prologue(make())
+    check_tree(sim, sc.n, g["order"], g["aabb"])
+    # Move the scene, then refit the same topology (no rebuild) and re-check.
+    for _ in range(3):
+        sim.advance(sim.maxH)
+    p2 = sim.meta.copy()
+    check(sim._lib.yb_debug_prologue(sim.handle, ctypes.byref(p2), np.float32(sim.maxH), 1, 0))
+    n = sc.n
+    aabb = np.zeros((n, 8), np.float32)
+    order = np.zeros(n, np.int32)
+    check(sim._lib.yb_debug_read(sim.handle, None, ptr(aabb), None, ptr(order), None, None))
+    assert np.array_equal(order, g["order"]), "refit must keep the sorted order"
+    check_tree(sim, n, order, aabb)
     sim.close()

@@ -103,7 +103,10 @@ SIGNATURES = {
     "yb_debug_write_dx": (_I, [_P, _P]),
     "yb_debug_read_last_vels": (_I, [_P, _P]),
     "yb_debug_write_last_vels": (_I, [_P, _P]),
+    "yb_profile": (_I, [_P, ctypes.POINTER(Params), _F, _I, _P, _P]),
 }
+
+STAGES = ["init", "aabb", "rebuild", "refit", "query", "step_limit", "sweep_dx", "sweep_q", "end"]
 
 _lib = None
 

@@ -14,6 +14,7 @@
 #include <string.h>
 
 #include <string>
+#include <vector>
 
 #include "yb_internal.h"
 
@@ -42,7 +43,8 @@ static void free_all(yb_sim* s) {
     void* ptrs[] = {b.pos,     b.prm,     b.vel,      b.lastVel, b.yv,      b.dx[0],
                     b.dx[1],   b.q[0],    b.q[1],     b.qRest,   b.numCols, b.cols,
                     b.aabb,    b.sceneBox, b.blockBox, b.morton, b.keys[0], b.keys[1],
-                    b.vals[0], b.vals[1], b.hist,     b.nodes,   b.parent,  b.visit,
+                    b.vals[0], b.vals[1], b.hist,     b.nodes,   b.parent,  b.rng,
+                    b.kids,    b.strad,   b.stradCount, b.leafNeed, b.sufBox, b.preBox, b.blkTab,
                     b.err,     b.params,  b.stage};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -83,6 +85,11 @@ extern "C" int yb_create(int n, yb_sim** out) {
     s->sortTiles = (n + kSortTile - 1) / kSortTile;
     s->boxBlocks = (n + 255) / 256;
     if (s->boxBlocks > 1024) s->boxBlocks = 1024;
+    s->refitBlocks = (n + kRefitLeaves - 1) / kRefitLeaves;
+    s->refitLevels = 1;
+    while ((1 << s->refitLevels) <= s->refitBlocks) s->refitLevels++;
+    s->refitTopBlocks = (n / kRefitLeaves) * 64 / 256 + 1;
+    if (s->refitTopBlocks > 2048) s->refitTopBlocks = 2048;
     Buffers& b = s->b;
     const size_t N = (size_t)n;
     hipError_t e = hipSuccess;
@@ -111,7 +118,14 @@ extern "C" int yb_create(int n, yb_sim** out) {
     ALLOC(b.hist, (size_t)kSortBins * s->sortTiles);
     ALLOC(b.nodes, 4 * (N - 1));
     ALLOC(b.parent, 2 * N - 1);
-    ALLOC(b.visit, N - 1);
+    ALLOC(b.rng, N - 1);
+    ALLOC(b.kids, N - 1);
+    ALLOC(b.strad, N - 1);
+    ALLOC(b.stradCount, 1);
+    ALLOC(b.leafNeed, N);
+    ALLOC(b.sufBox, 2 * N);
+    ALLOC(b.preBox, 2 * N);
+    ALLOC(b.blkTab, 2 * (size_t)s->refitBlocks * s->refitLevels);
     ALLOC(b.err, 2);
     ALLOC(b.params, 1);
 #undef ALLOC
@@ -476,4 +490,71 @@ extern "C" int yb_debug_write_last_vels(yb_sim* s, const float* lastVels) {
     delete[] tmp;
     HIPCHECK(e);
     return YB_OK;
+}
+
+// ---- live per-stage timing ----------------------------------------------------
+// Runs `substeps` substeps exactly as yb_advance schedules them, but enqueued
+// eagerly with a HIP event between consecutive kernels on the handle's stream,
+// and accumulates each stage's GPU time.  Stage ids are YB_STAGE_*.
+extern "C" int yb_profile(yb_sim* s, yb_params* p, float h, int substeps, double* ms,
+                          int32_t* launches) {
+    if (!s || !p || !ms || !launches || substeps < 1) return YB_ERR_INVALID_ARGUMENT;
+    for (int k = 0; k < YB_NUM_STAGES; k++) {
+        ms[k] = 0.0;
+        launches[k] = 0;
+    }
+    p->lastH = p->h;
+    p->h = h;
+    int rc = yb_validate_params(p);
+    if (rc != YB_OK) return rc;
+    rc = upload_params(s, p);
+    if (rc != YB_OK) return rc;
+    const int maxEv = 64 + 2 * p->numItr;
+    std::vector<hipEvent_t> ev(maxEv);
+    for (auto& e : ev) HIPCHECK(hipEventCreate(&e));
+    std::vector<int> stage(maxEv);
+    for (int st = 0; st < substeps; st++, s->stepCounter++) {
+        const int mode = next_mode(s, p);
+        int ne = 0;
+        HIPCHECK(hipEventRecord(ev[ne++], s->stream));
+        auto mark = [&](int sid) {
+            stage[ne] = sid;
+            (void)hipEventRecord(ev[ne++], s->stream);
+        };
+        launch_init(s);
+        mark(YB_STAGE_INIT);
+        if (mode >= 1) {
+            launch_aabb(s);
+            mark(YB_STAGE_AABB);
+            if (mode == 2) {
+                launch_rebuild(s);
+                mark(YB_STAGE_REBUILD);
+            }
+            launch_refit(s);
+            mark(YB_STAGE_REFIT);
+            launch_query(s);
+            mark(YB_STAGE_QUERY);
+        }
+        launch_step_limit(s);
+        mark(YB_STAGE_STEP_LIMIT);
+        for (int k = 0; k < p->numItr; k++) {
+            launch_sweep_dx(s, k & 1);
+            mark(YB_STAGE_SWEEP_DX);
+            launch_sweep_q(s, k & 1);
+            mark(YB_STAGE_SWEEP_Q);
+        }
+        launch_end(s, p->numItr & 1);
+        mark(YB_STAGE_END);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipEventSynchronize(ev[ne - 1]));
+        for (int k = 1; k < ne; k++) {
+            float t = 0.f;
+            HIPCHECK(hipEventElapsedTime(&t, ev[k - 1], ev[k]));
+            ms[stage[k]] += t;
+            launches[stage[k]] += 1;
+        }
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    p->time += h * substeps;
+    return check_errors(s, nullptr);
 }

@@ -11,7 +11,7 @@
 //   k_morton      30-bit Morton code of each box centre in the scene box
 //   k_sort_*      stable LSD radix sort, 4 x 8-bit passes, LDS ranking
 //   k_karras      Karras (2012) radix tree over the sorted codes
-//   k_refit       bottom-up box refit with per-node arrival counters
+//   k_refit_*     fence-free refit: in-block LDS climbing + range unions
 //   k_query       stack traversal + narrowphase filter (collision.cu:39-92)
 //                 writing each segment's own contact list: no atomics, no
 //                 host round trip for the pair count.
@@ -56,7 +56,9 @@ __global__ __launch_bounds__(256) void k_init(const DevParams* __restrict__ P,
 
 // ---------------------------------------------------------------------------
 // Segment boxes (collision.cu:10-37).  Vertices without a next vertex get the
-// empty box (+inf, -inf), which overlaps nothing.
+// empty box (+inf, -inf), which overlaps nothing.  hi.w carries the segment's
+// velocity pad sqrt(max(|dx_i|^2, |dx_i+1|^2)) for the narrowphase radius, with
+// the sign bit set when either endpoint is glued (connectionIndex >= 0).
 __global__ __launch_bounds__(256) void k_aabb(const DevParams* __restrict__ P,
                                               const float4* __restrict__ pos,
                                               const float4* __restrict__ prm,
@@ -65,22 +67,26 @@ __global__ __launch_bounds__(256) void k_aabb(const DevParams* __restrict__ P,
     const int n = P->n;
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n) return;
-    const uint32_t flags = __float_as_uint(prm[v].w);
+    const float4 r0 = prm[v];
+    const uint32_t flags = __float_as_uint(r0.w);
     v3 lo = v3{INFINITY, INFINITY, INFINITY};
     v3 hi = v3{-INFINITY, -INFINITY, -INFINITY};
+    float vpad = 0.0f;
+    bool glued = false;
     if (flags & YB_FLAG_HAS_NEXT) {
         const v3 p0 = xyz(pos[v]);
         const v3 p1 = xyz(pos[v + 1]);
         lo = vmin(p0, p1);
         hi = vmax(p0, p1);
+        vpad = sqrtf(fmaxf(len2(xyz(dx[v])), len2(xyz(dx[v + 1]))));
         float pad = P->scaledDetectionRadius;
-        if (P->useVelocityRadius)
-            pad += sqrtf(fmaxf(len2(xyz(dx[v])), len2(xyz(dx[v + 1]))));
+        if (P->useVelocityRadius) pad += vpad;
         lo = lo - v3{pad, pad, pad};
         hi = hi + v3{pad, pad, pad};
+        glued = (__float_as_int(r0.z) >= 0) || (__float_as_int(prm[v + 1].z) >= 0);
     }
     aabb[2 * v] = f4(lo, 0.f);
-    aabb[2 * v + 1] = f4(hi, 0.f);
+    aabb[2 * v + 1] = f4(hi, glued ? -vpad : vpad);
 }
 
 // ---------------------------------------------------------------------------
@@ -139,8 +145,10 @@ __global__ __launch_bounds__(kBoxThreads) void k_box_final(int nb, const float4*
 }
 
 // ---------------------------------------------------------------------------
-// Morton codes: box centre quantised to 1024 cells per axis of the scene box,
-// bits interleaved x-major.  Non-segments get 0xFFFFFFFF and sort last.
+// Morton codes: box centre quantised to 1024 cubic cells spanning the largest
+// extent of the scene box (isotropic, so a long thin scene does not get
+// sliver cells), bits interleaved x-major.  Non-segments get 0xFFFFFFFF and
+// sort last.
 __global__ __launch_bounds__(256) void k_morton(int n, const float4* __restrict__ prm,
                                                 const float4* __restrict__ aabb,
                                                 const float4* __restrict__ sceneBox,
@@ -153,13 +161,14 @@ __global__ __launch_bounds__(256) void k_morton(int n, const float4* __restrict_
     if (__float_as_uint(prm[v].w) & YB_FLAG_HAS_NEXT) {
         const v3 slo = xyz(sceneBox[0]);
         const v3 shi = xyz(sceneBox[1]);
-        const v3 ext = shi - slo;
+        const v3 e3 = shi - slo;
+        const float ext = fmaxf(fmaxf(e3.x, e3.y), e3.z);
         const v3 lo = xyz(aabb[2 * v]);
         const v3 hi = xyz(aabb[2 * v + 1]);
         const v3 c = (lo + hi) * 0.5f;
-        const uint32_t qx = quant10(c.x, slo.x, ext.x);
-        const uint32_t qy = quant10(c.y, slo.y, ext.y);
-        const uint32_t qz = quant10(c.z, slo.z, ext.z);
+        const uint32_t qx = quant10(c.x, slo.x, ext);
+        const uint32_t qy = quant10(c.y, slo.y, ext);
+        const uint32_t qz = quant10(c.z, slo.z, ext);
         code = (expand_bits10(qx) << 2) | (expand_bits10(qy) << 1) | expand_bits10(qz);
     }
     morton[v] = code;
@@ -279,9 +288,11 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(int n, int shift,
 // Karras (2012) radix tree.  Keys are made unique by the sorted position:
 // delta(i,j) = clz(k_i ^ k_j), or 32 + clz(i ^ j) for equal codes.
 // Node i (internal, 0..n-2) stores its children's boxes inline:
-// nodes[4i+0] = (lo_left, child code), nodes[4i+1] = (hi_left, -),
-// nodes[4i+2] = (lo_right, child code), nodes[4i+3] = (hi_right, -),
-// child code = internal index (>= 0) or ~segment for a leaf.
+// nodes[4i+0] = (lo_left, child code), nodes[4i+1] = (hi_left, leaf pad),
+// nodes[4i+2] = (lo_right, child code), nodes[4i+3] = (hi_right, leaf pad),
+// child code = internal index (>= 0) or ~segment for a leaf; "leaf pad" is the
+// leaf's signed velocity pad (see k_aabb).  Also written per node: its sorted
+// leaf range rng[i] = (first, last) and kids[i] with leaves as ~sorted position.
 __device__ __forceinline__ int kdelta(const uint32_t* __restrict__ keys, int n, int a, int b) {
     if (b < 0 || b > n - 1) return -1;
     const uint32_t ka = keys[a];
@@ -293,7 +304,11 @@ __device__ __forceinline__ int kdelta(const uint32_t* __restrict__ keys, int n, 
 __global__ __launch_bounds__(256) void k_karras(int n, const uint32_t* __restrict__ keys,
                                                 const int32_t* __restrict__ order,
                                                 float4* __restrict__ nodes,
-                                                int32_t* __restrict__ parent) {
+                                                int32_t* __restrict__ parent,
+                                                int2* __restrict__ rng, int2* __restrict__ kids,
+                                                int32_t* __restrict__ strad,
+                                                int32_t* __restrict__ stradCount,
+                                                int32_t* __restrict__ leafNeed) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n - 1) return;
     const int d = (kdelta(keys, n, i, i + 1) - kdelta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
@@ -314,64 +329,226 @@ __global__ __launch_bounds__(256) void k_karras(int n, const uint32_t* __restric
     const int gamma = i + s * d + min(d, 0);
     const int lo = min(i, j);
     const int hi = max(i, j);
-    int cl, cr;
+    int cl, cr, kl, kr;
     if (lo == gamma) {
         cl = ~order[gamma];
+        kl = ~gamma;
         parent[(n - 1) + gamma] = (i << 1) | 0;
     } else {
-        cl = gamma;
+        cl = kl = gamma;
         parent[gamma] = (i << 1) | 0;
     }
     if (hi == gamma + 1) {
         cr = ~order[gamma + 1];
+        kr = ~(gamma + 1);
         parent[(n - 1) + gamma + 1] = (i << 1) | 1;
     } else {
-        cr = gamma + 1;
+        cr = kr = gamma + 1;
         parent[gamma + 1] = (i << 1) | 1;
     }
     nodes[4 * i + 0].w = __int_as_float(cl);
     nodes[4 * i + 2].w = __int_as_float(cr);
+    rng[i] = make_int2(lo, hi);
+    kids[i] = make_int2(kl, kr);
     if (i == 0) parent[0] = -1;
+    // Nodes whose leaf range crosses a refit-block boundary are finished by
+    // the range-union pass; their end leaves need suffix / prefix unions.
+    if (lo / kRefitLeaves != hi / kRefitLeaves) {
+        const int slot = atomicAdd(stradCount, 1);
+        strad[slot] = i;
+        atomicOr(&leafNeed[lo], 1);
+        atomicOr(&leafNeed[hi], 2);
+    }
 }
 
 // ---------------------------------------------------------------------------
-// Refit: each leaf writes its box into its parent's slot, then climbs.  The
-// second of the two arrivals at a node (per-node counter) forms the union
-// and continues upward; the first stops.  Device-scope fences make the
-// sibling's slot visible across XCDs before the union is read.
-__global__ __launch_bounds__(256) void k_refit(int n, const int32_t* __restrict__ order,
-                                               const float4* __restrict__ aabb,
-                                               float4* nodes, const int32_t* __restrict__ parent,
-                                               int32_t* visit, float4* __restrict__ sceneBox) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const int seg = order[k];
-    float4 lo = aabb[2 * seg];
-    float4 hi = aabb[2 * seg + 1];
-    int code = ~seg;
-    int slot = parent[(n - 1) + k];
-    while (true) {
-        const int p = slot >> 1;
-        const int side = slot & 1;
-        nodes[4 * p + 2 * side] = make_float4(lo.x, lo.y, lo.z, __int_as_float(code));
-        nodes[4 * p + 2 * side + 1] = make_float4(hi.x, hi.y, hi.z, 0.f);
-        __threadfence();
-        const int old = atomicAdd(&visit[p], 1);
-        if (old == 0) return;
-        __threadfence();
-        const float4 a0 = nodes[4 * p + 0];
-        const float4 a1 = nodes[4 * p + 1];
-        const float4 b0 = nodes[4 * p + 2];
-        const float4 b1 = nodes[4 * p + 3];
-        lo = make_float4(fminf(a0.x, b0.x), fminf(a0.y, b0.y), fminf(a0.z, b0.z), 0.f);
-        hi = make_float4(fmaxf(a1.x, b1.x), fmaxf(a1.y, b1.y), fmaxf(a1.z, b1.z), 0.f);
-        if (p == 0) {
+// Refit without device-scope fences.  Sorted leaves are split into blocks of
+// kRefitLeaves.  (1) k_refit_local: one workgroup per block climbs from every
+// leaf through the nodes whose range lies inside the block, with LDS arrival
+// counters and LDS boxes (the second arrival at a node forms the union); it
+// writes those nodes' child boxes, the box of every child whose parent
+// straddles a block boundary, the block's union, and suffix / prefix unions
+// at the leaves that end a straddling range.  (2) k_refit_table: a sparse
+// table of block unions.  (3) k_refit_top: each straddling node's box is the
+// union of a suffix, at most two overlapping sparse-table windows and a
+// prefix -- computed directly, so no node waits on another.  Every step is a
+// min/max, hence exact and independent of scheduling.
+constexpr int kRefitThreads = 256;
+constexpr int kRefitPer = kRefitLeaves / kRefitThreads;
+
+__device__ __forceinline__ void box_union(float4& lo, float4& hi, float4 lo2, float4 hi2) {
+    lo = make_float4(fminf(lo.x, lo2.x), fminf(lo.y, lo2.y), fminf(lo.z, lo2.z), 0.f);
+    hi = make_float4(fmaxf(hi.x, hi2.x), fmaxf(hi.y, hi2.y), fmaxf(hi.z, hi2.z), 0.f);
+}
+__device__ __forceinline__ void store_xyz(float4* p, float4 v) {
+    float* f = reinterpret_cast<float*>(p);
+    f[0] = v.x;
+    f[1] = v.y;
+    f[2] = v.z;
+}
+
+__global__ __launch_bounds__(kRefitThreads) void k_refit_local(
+    int n, const int32_t* __restrict__ order, const float4* __restrict__ aabb,
+    float4* __restrict__ nodes, const int32_t* __restrict__ parent, const int2* __restrict__ rng,
+    const int2* __restrict__ kids, const int32_t* __restrict__ leafNeed,
+    float4* __restrict__ sufBox, float4* __restrict__ preBox, float4* __restrict__ blkBox,
+    float4* __restrict__ sceneBox) {
+    __shared__ float4 lbox[kRefitLeaves][2];
+    __shared__ float4 nbox[kRefitLeaves][2];
+    __shared__ int32_t cnt[kRefitLeaves];
+    __shared__ float red[6][kRefitThreads];
+    const int t = threadIdx.x;
+    const int base = blockIdx.x * kRefitLeaves;
+    const int end = min(base + kRefitLeaves, n);
+    float4 blo = make_float4(INFINITY, INFINITY, INFINITY, 0.f);
+    float4 bhi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.f);
+    for (int r = 0; r < kRefitPer; r++) {
+        const int k = base + t + kRefitThreads * r;
+        cnt[t + kRefitThreads * r] = 0;
+        if (k < end) {
+            const int seg = order[k];
+            const float4 a = aabb[2 * seg];
+            const float4 b = aabb[2 * seg + 1];
+            lbox[k - base][0] = a;
+            lbox[k - base][1] = b;
+            box_union(blo, bhi, a, b);
+        }
+    }
+    red[0][t] = blo.x; red[1][t] = blo.y; red[2][t] = blo.z;
+    red[3][t] = bhi.x; red[4][t] = bhi.y; red[5][t] = bhi.z;
+    __syncthreads();
+    for (int st = kRefitThreads / 2; st > 0; st >>= 1) {
+        if (t < st) {
+            for (int c = 0; c < 3; c++) red[c][t] = fminf(red[c][t], red[c][t + st]);
+            for (int c = 3; c < 6; c++) red[c][t] = fmaxf(red[c][t], red[c][t + st]);
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        blkBox[2 * blockIdx.x] = make_float4(red[0][0], red[1][0], red[2][0], 0.f);
+        blkBox[2 * blockIdx.x + 1] = make_float4(red[3][0], red[4][0], red[5][0], 0.f);
+    }
+
+    // Climb.
+    for (int r = 0; r < kRefitPer; r++) {
+        const int k = base + t + kRefitThreads * r;
+        if (k >= end) continue;
+        float4 lo = lbox[k - base][0];
+        float4 hi = lbox[k - base][1];
+        int slot = parent[(n - 1) + k];
+        bool leaf = true;
+        while (slot >= 0) {
+            const int p = slot >> 1;
+            const int2 pr = rng[p];
+            if (pr.x < base || pr.y >= end) {
+                // p straddles: publish this child's box in p's slot for traversal.
+                const int side = slot & 1;
+                store_xyz(&nodes[4 * p + 2 * side], lo);
+                nodes[4 * p + 2 * side + 1] = make_float4(hi.x, hi.y, hi.z, leaf ? hi.w : 0.f);
+                break;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            const int old = atomicAdd(&cnt[p - base], 1);
+            if (old == 0) break;
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            const int2 kc = kids[p];
+            float4 lo0, hi0, lo1, hi1;
+            if (kc.x < 0) { lo0 = lbox[~kc.x - base][0]; hi0 = lbox[~kc.x - base][1]; }
+            else { lo0 = nbox[kc.x - base][0]; hi0 = nbox[kc.x - base][1]; }
+            if (kc.y < 0) { lo1 = lbox[~kc.y - base][0]; hi1 = lbox[~kc.y - base][1]; }
+            else { lo1 = nbox[kc.y - base][0]; hi1 = nbox[kc.y - base][1]; }
+            store_xyz(&nodes[4 * p + 0], lo0);
+            nodes[4 * p + 1] = make_float4(hi0.x, hi0.y, hi0.z, kc.x < 0 ? hi0.w : 0.f);
+            store_xyz(&nodes[4 * p + 2], lo1);
+            nodes[4 * p + 3] = make_float4(hi1.x, hi1.y, hi1.z, kc.y < 0 ? hi1.w : 0.f);
+            lo = lo0;
+            hi = hi0;
+            box_union(lo, hi, lo1, hi1);
+            nbox[p - base][0] = lo;
+            nbox[p - base][1] = hi;
+            if (p == 0) {
+                sceneBox[0] = lo;
+                sceneBox[1] = hi;
+            }
+            leaf = false;
+            slot = parent[p];
+        }
+    }
+
+    // Suffix / prefix unions at the ends of straddling ranges.
+    for (int r = 0; r < kRefitPer; r++) {
+        const int k = base + t + kRefitThreads * r;
+        if (k >= end) continue;
+        const int need = leafNeed[k];
+        if (need & 1) {
+            float4 lo = lbox[k - base][0], hi = lbox[k - base][1];
+            for (int m = k + 1; m < end; m++) box_union(lo, hi, lbox[m - base][0], lbox[m - base][1]);
+            sufBox[2 * k] = lo;
+            sufBox[2 * k + 1] = hi;
+        }
+        if (need & 2) {
+            float4 lo = lbox[k - base][0], hi = lbox[k - base][1];
+            for (int m = base; m < k; m++) box_union(lo, hi, lbox[m - base][0], lbox[m - base][1]);
+            preBox[2 * k] = lo;
+            preBox[2 * k + 1] = hi;
+        }
+    }
+}
+
+// Sparse table over block unions: level L, entry b = union of blocks
+// [b, b + 2^L).  One workgroup; levels separated by barriers.
+__global__ __launch_bounds__(1024) void k_refit_table(int nb, int levels,
+                                                      float4* __restrict__ tab) {
+    for (int L = 1; L < levels; L++) {
+        __syncthreads();
+        const int half = 1 << (L - 1);
+        const float4* src = tab + (size_t)2 * nb * (L - 1);
+        float4* dst = tab + (size_t)2 * nb * L;
+        for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+            float4 lo = src[2 * b], hi = src[2 * b + 1];
+            if (b + half < nb) box_union(lo, hi, src[2 * (b + half)], src[2 * (b + half) + 1]);
+            dst[2 * b] = lo;
+            dst[2 * b + 1] = hi;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_refit_top(int n, int nb,
+                                                   const int32_t* __restrict__ strad,
+                                                   const int32_t* __restrict__ stradCount,
+                                                   const int2* __restrict__ rng,
+                                                   const int32_t* __restrict__ parent,
+                                                   const float4* __restrict__ sufBox,
+                                                   const float4* __restrict__ preBox,
+                                                   const float4* __restrict__ tab,
+                                                   float4* __restrict__ nodes,
+                                                   float4* __restrict__ sceneBox) {
+    const int count = *stradCount;
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < count; s += gridDim.x * blockDim.x) {
+        const int i = strad[s];
+        const int2 r = rng[i];
+        float4 lo = sufBox[2 * r.x], hi = sufBox[2 * r.x + 1];
+        box_union(lo, hi, preBox[2 * r.y], preBox[2 * r.y + 1]);
+        const int b0 = r.x / kRefitLeaves + 1;
+        const int b1 = r.y / kRefitLeaves - 1;
+        if (b1 >= b0) {
+            const int span = b1 - b0 + 1;
+            const int L = 31 - __clz(span);
+            const float4* T = tab + (size_t)2 * nb * L;
+            box_union(lo, hi, T[2 * b0], T[2 * b0 + 1]);
+            const int b2 = b1 - (1 << L) + 1;
+            box_union(lo, hi, T[2 * b2], T[2 * b2 + 1]);
+        }
+        if (i == 0) {
             sceneBox[0] = lo;
             sceneBox[1] = hi;
-            return;
+        } else {
+            const int slot = parent[i];
+            const int p = slot >> 1;
+            const int side = slot & 1;
+            store_xyz(&nodes[4 * p + 2 * side], lo);
+            nodes[4 * p + 2 * side + 1] = make_float4(hi.x, hi.y, hi.z, 0.f);
         }
-        code = p;
-        slot = parent[p];
     }
 }
 
@@ -392,32 +569,38 @@ __device__ __forceinline__ bool overlap(v3 qlo, v3 qhi, float4 lo, float4 hi) {
            (qlo.z <= hi.z) && (lo.z <= qhi.z);
 }
 
-__device__ __forceinline__ float vel_pad(const float4* __restrict__ dx, int a) {
-    return sqrtf(fmaxf(len2(xyz(dx[a])), len2(xyz(dx[a + 1]))));
-}
+struct SegInfo {
+    v3 p0, p1;
+    float vpad;
+    bool glued;
+};
 
 __device__ __forceinline__ bool pair_accepted(const DevParams* __restrict__ P,
-                                              const float4* __restrict__ pos,
-                                              const float4* __restrict__ prm,
-                                              const float4* __restrict__ dx, int i, int j) {
-    const int x = min(i, j);
-    const int y = max(i, j);
-    const int c0 = __float_as_int(prm[x].z);
-    const int c1 = __float_as_int(prm[x + 1].z);
-    if (c0 == y || c1 == y || c0 == y + 1 || c1 == y + 1) return false;
+                                              const float4* __restrict__ prm, int i,
+                                              const SegInfo& si, int j, const SegInfo& sj) {
+    const bool iFirst = i < j;
+    const int x = iFirst ? i : j;
+    const int y = iFirst ? j : i;
     if (y - x <= 2) return false;
-    const v3 a0 = xyz(pos[x]);
-    const v3 a1 = xyz(pos[x + 1]) - a0;
-    const v3 b0 = xyz(pos[y]) - a0;
-    const v3 b1 = xyz(pos[y + 1]) - a0;
+    const SegInfo& sx = iFirst ? si : sj;
+    const SegInfo& sy = iFirst ? sj : si;
+    if (sx.glued) {
+        const int c0 = __float_as_int(prm[x].z);
+        const int c1 = __float_as_int(prm[x + 1].z);
+        if (c0 == y || c1 == y || c0 == y + 1 || c1 == y + 1) return false;
+    }
+    const v3 a0 = sx.p0;
+    const v3 a1 = sx.p1 - a0;
+    const v3 b0 = sy.p0 - a0;
+    const v3 b1 = sy.p1 - a0;
     float u, w;
     closest_uv(v3{0.f, 0.f, 0.f}, a1, b0, b1, u, w);
     const v3 nrm = u * a1 - mixv(b0, b1, w);
     const float d2 = len2(nrm);
     float r = 2.0f * P->scaledDetectionRadius;
     if (P->useVelocityRadius) {
-        r += vel_pad(dx, x);
-        r += vel_pad(dx, y);
+        r += sx.vpad;
+        r += sy.vpad;
     }
     return d2 < r * r;
 }
@@ -428,7 +611,6 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
                                                          const float4* __restrict__ nodes,
                                                          const float4* __restrict__ pos,
                                                          const float4* __restrict__ prm,
-                                                         const float4* __restrict__ dx,
                                                          int32_t* __restrict__ numCols,
                                                          int32_t* __restrict__ cols,
                                                          int32_t* __restrict__ err) {
@@ -441,8 +623,15 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
         numCols[i] = 0;
         return;
     }
-    const v3 qlo = xyz(aabb[2 * i]);
-    const v3 qhi = xyz(aabb[2 * i + 1]);
+    const float4 qlo4 = aabb[2 * i];
+    const float4 qhi4 = aabb[2 * i + 1];
+    const v3 qlo = xyz(qlo4);
+    const v3 qhi = xyz(qhi4);
+    SegInfo si;
+    si.p0 = xyz(pos[i]);
+    si.p1 = xyz(pos[i + 1]);
+    si.vpad = fabsf(qhi4.w);
+    si.glued = signbit(qhi4.w);
     const int t = threadIdx.x;
     int sp = 0;
     int node = 0;
@@ -457,18 +646,22 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
         const int cr = __float_as_int(r0.w);
         const bool hl = overlap(qlo, qhi, l0, l1);
         const bool hr = overlap(qlo, qhi, r0, r1);
-        if (hl && cl < 0) {
-            const int j = ~cl;
-            if (j != i && pair_accepted(P, pos, prm, dx, i, j)) {
-                if (cnt < kMaxCols) cols[i + n * cnt] = j;
-                cnt++;
-            }
-        }
-        if (hr && cr < 0) {
-            const int j = ~cr;
-            if (j != i && pair_accepted(P, pos, prm, dx, i, j)) {
-                if (cnt < kMaxCols) cols[i + n * cnt] = j;
-                cnt++;
+#pragma unroll
+        for (int side = 0; side < 2; side++) {
+            const bool hit = side ? hr : hl;
+            const int c = side ? cr : cl;
+            if (hit && c < 0 && ~c != i) {
+                const int j = ~c;
+                const float pw = side ? r1.w : l1.w;
+                SegInfo sj;
+                sj.p0 = xyz(pos[j]);
+                sj.p1 = xyz(pos[j + 1]);
+                sj.vpad = fabsf(pw);
+                sj.glued = signbit(pw);
+                if (pair_accepted(P, prm, i, si, j, sj)) {
+                    if (cnt < kMaxCols) cols[i + n * cnt] = j;
+                    cnt++;
+                }
             }
         }
         const bool tl = hl && cl >= 0;
@@ -531,21 +724,31 @@ void launch_rebuild(yb_sim* s) {
                            s->b.vals[in ^ 1], s->b.hist);
     }
     // After 4 passes the sorted pairs are back in keys[0] / vals[0] == order.
+    launch_fill_i32(s, s->b.stradCount, 0, 1);
+    launch_fill_i32(s, s->b.leafNeed, 0, n);
     hipLaunchKernelGGL(k_karras, dim3(blocks(n - 1, 256)), dim3(256), 0, s->stream, n,
-                       s->b.keys[0], s->b.order, s->b.nodes, s->b.parent);
+                       s->b.keys[0], s->b.order, s->b.nodes, s->b.parent, s->b.rng, s->b.kids,
+                       s->b.strad, s->b.stradCount, s->b.leafNeed);
 }
 
 void launch_refit(yb_sim* s) {
-    launch_fill_i32(s, s->b.visit, 0, s->n - 1);
-    hipLaunchKernelGGL(k_refit, dim3(blocks(s->n, 256)), dim3(256), 0, s->stream, s->n,
-                       s->b.order, s->b.aabb, s->b.nodes, s->b.parent, s->b.visit,
-                       s->b.sceneBox);
+    const int n = s->n;
+    hipLaunchKernelGGL(k_refit_local, dim3(s->refitBlocks), dim3(kRefitThreads), 0, s->stream, n,
+                       s->b.order, s->b.aabb, s->b.nodes, s->b.parent, s->b.rng, s->b.kids,
+                       s->b.leafNeed, s->b.sufBox, s->b.preBox, s->b.blkTab, s->b.sceneBox);
+    if (s->refitBlocks > 1) {
+        hipLaunchKernelGGL(k_refit_table, dim3(1), dim3(1024), 0, s->stream, s->refitBlocks,
+                           s->refitLevels, s->b.blkTab);
+        hipLaunchKernelGGL(k_refit_top, dim3(s->refitTopBlocks), dim3(256), 0, s->stream, n,
+                           s->refitBlocks, s->b.strad, s->b.stradCount, s->b.rng, s->b.parent,
+                           s->b.sufBox, s->b.preBox, s->b.blkTab, s->b.nodes, s->b.sceneBox);
+    }
 }
 
 void launch_query(yb_sim* s) {
     hipLaunchKernelGGL(k_query, dim3(blocks(s->n, kQueryThreads)), dim3(kQueryThreads), 0,
                        s->stream, s->b.params, s->b.order, s->b.aabb, s->b.nodes, s->b.pos,
-                       s->b.prm, s->b.dx[0], s->b.numCols, s->b.cols, s->b.err);
+                       s->b.prm, s->b.numCols, s->b.cols, s->b.err);
 }
 
 }  // namespace yb

@@ -398,14 +398,23 @@ void launch_step_limit(yb_sim* s) {
                        s->b.pos, s->b.prm, s->b.dx[0], s->b.numCols, s->b.cols);
 }
 
-void launch_sweep(yb_sim* s, int in) {
+void launch_sweep_dx(yb_sim* s, int in) {
     const int out = in ^ 1;
     const int waves = blocks(s->n + 1, 63);
     hipLaunchKernelGGL(k_sweep_dx, dim3(blocks(waves * 64, 256)), dim3(256), 0, s->stream,
                        s->b.params, s->b.pos, s->b.prm, s->b.yv, s->b.q[in], s->b.dx[in],
                        s->b.dx[out], s->b.numCols, s->b.cols);
+}
+
+void launch_sweep_q(yb_sim* s, int in) {
+    const int out = in ^ 1;
     hipLaunchKernelGGL(k_sweep_q, dim3(blocks(s->n, 256)), dim3(256), 0, s->stream, s->b.params,
                        s->b.pos, s->b.prm, s->b.dx[out], s->b.q[in], s->b.q[out], s->b.qRest);
+}
+
+void launch_sweep(yb_sim* s, int in) {
+    launch_sweep_dx(s, in);
+    launch_sweep_q(s, in);
 }
 
 void launch_end(yb_sim* s, int fin) {

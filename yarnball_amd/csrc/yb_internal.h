@@ -42,6 +42,7 @@ constexpr int kSortBins = 1 << kSortBits;
 constexpr int kSortThreads = 256;
 constexpr int kSortItems = 8;
 constexpr int kSortTile = kSortThreads * kSortItems;
+constexpr int kRefitLeaves = 1024;  // sorted leaves per refit workgroup
 
 // SoA device state.  Every per-vertex array is float4 (16 B, one dwordx4 per
 // lane, fully coalesced):
@@ -71,7 +72,14 @@ struct Buffers {
     int32_t* order = nullptr;     // sorted leaf -> segment
     float4* nodes = nullptr;      // 4 per internal node: lo0, hi0, lo1, hi1 (child ids in .w)
     int32_t* parent = nullptr;    // 2n-1: (parent << 1) | side
-    int32_t* visit = nullptr;     // n-1 refit counters
+    int2* rng = nullptr;          // n-1: sorted leaf range of each internal node
+    int2* kids = nullptr;         // n-1: children, leaves as ~sorted position
+    int32_t* strad = nullptr;     // internal nodes whose range crosses a refit block
+    int32_t* stradCount = nullptr;
+    int32_t* leafNeed = nullptr;  // n: bit0 suffix union needed, bit1 prefix union
+    float4* sufBox = nullptr;     // 2n
+    float4* preBox = nullptr;     // 2n
+    float4* blkTab = nullptr;     // sparse table of refit-block unions, 2*nb per level
     int32_t* err = nullptr;       // [0] error code, [1] warning code
     DevParams* params = nullptr;
     // upload / download staging
@@ -87,6 +95,9 @@ struct yb_sim {
     yb::Buffers b;
     int sortTiles = 0;
     int boxBlocks = 0;
+    int refitBlocks = 0;
+    int refitLevels = 0;
+    int refitTopBlocks = 0;
     bool treeBuilt = false;
     float lastBVHRebuild = INFINITY;  // sim time since the last full rebuild
     int64_t stepCounter = 0;
@@ -109,6 +120,8 @@ void launch_zero_vel(yb_sim* s);
 void launch_init(yb_sim* s);                    // iteration.cu:7-41
 void launch_step_limit(yb_sim* s);              // collision.cu:121-172
 void launch_sweep(yb_sim* s, int in);           // cosserat.cu:20-253, reads dx[in], q[in]
+void launch_sweep_dx(yb_sim* s, int in);        // cosserat.cu:20-207
+void launch_sweep_q(yb_sim* s, int in);         // cosserat.cu:209-253
 void launch_end(yb_sim* s, int fin);            // iteration.cu:48-61
 void launch_fill_i32(yb_sim* s, int32_t* p, int32_t v, int count);
 // broadphase (collision.cu:10-119 + the LBVH the reference links)
