@@ -143,6 +143,13 @@ int yb_bounds(yb_sim* s, float out[6]);
  * statistics.cpp:5-24).  out: n ints. */
 int yb_collision_counts(yb_sim* s, int32_t* out);
 
+/* Interpenetration monitor (WARNING_SEGMENT_INTERPENETRATION, declared at
+ * YarnBall.h:121 but never raised by the reference): the smallest distance
+ * between listed contact segments divided by 2 * radius, over every substep
+ * start and every advance end since the last reset.  Below 1 means two yarn
+ * cores overlap; advance then reports the warning.  INFINITY if no contact. */
+int yb_min_separation(yb_sim* s, float* out, int reset);
+
 /* Last error message (thread-local), for negative status codes. */
 const char* yb_last_message(void);
 
@@ -174,6 +181,26 @@ int yb_debug_write_dx(yb_sim* s, const float* dx);
 int yb_debug_read_last_vels(yb_sim* s, float* lastVels);
 int yb_debug_write_last_vels(yb_sim* s, const float* lastVels);
 
+/* ---- residual-targeted solving ---------------------------------------------
+ * No counterpart in the reference, which always runs meta.numItr sweeps
+ * (step.cpp:83-84).  With residualTol > 0 every substep runs sweeps until the
+ * relative Newton residual
+ *     r = sqrt( sum_free |dx_{k+1} - dx_k|^2 / sum_free l_i^2 )
+ * -- the RMS change of the iterate per sweep in units of the rest length, l_i
+ * the rest length of vertex i's segment (the previous one at a strand end) --
+ * is <= residualTol, or maxSweeps sweeps have run; numItr is then ignored.  Sweeps are launched in captured chunks of `chunk`.
+ * residualTol <= 0 restores the fixed numItr schedule. */
+typedef struct yb_solver_stats_t {
+    int64_t substeps;     /* substeps run in residual-targeted mode */
+    int64_t sweeps;       /* sweeps summed over those substeps */
+    int32_t maxSweepsUsed;
+    int32_t notConverged; /* substeps that stopped at maxSweeps above residualTol */
+    float lastResidual;
+    float maxResidual;    /* largest final residual of a substep */
+} yb_solver_stats_t;
+int yb_set_solver(yb_sim* s, float residualTol, int maxSweeps, int chunk);
+int yb_solver_stats(yb_sim* s, yb_solver_stats_t* out, int reset);
+
 /* ---- live per-stage timing (measurement harness) --------------------------
  * Runs `substeps` substeps of length h with the yb_advance schedule, enqueued
  * without graphs and with a HIP event between consecutive kernels on the
@@ -191,7 +218,8 @@ enum {
     YB_STAGE_SWEEP_DX = 6,    /* cosserat.cu:20-207 */
     YB_STAGE_SWEEP_Q = 7,     /* cosserat.cu:209-253 */
     YB_STAGE_END = 8,         /* iteration.cu:48-61 */
-    YB_NUM_STAGES = 9
+    YB_STAGE_RESIDUAL = 9,    /* residual reduction / solver control */
+    YB_NUM_STAGES = 10
 };
 int yb_profile(yb_sim* s, yb_params* p, float h, int substeps, double* ms, int32_t* launches);
 

@@ -30,6 +30,8 @@ SIGS = {
     "orc_contacts": (None, [_P, _P, _P]),
     "orc_last_prologue": (None, [_P, _P, _P]),
     "orc_last_residual": (ctypes.c_double, [_P]),
+    "orc_set_solver": (None, [_P, ctypes.c_double, _I]),
+    "orc_last_sweeps": (_I, [_P]),
     "orc32_warm_start": (None, [_I, _PP, _P, _P, _P, _P, _P]),
     "orc32_aabb": (None, [_I, _PP, _P, _P, _P, _P]),
     "orc32_scene_box": (None, [_I, _P, _P]),
@@ -120,6 +122,12 @@ class Oracle:
         dxw = np.zeros((self.n, 3))
         self.lib.orc_last_prologue(self.o, ptr(lim), ptr(dxw))
         return lim, dxw
+
+    def set_solver(self, tol, max_sweeps):
+        self.lib.orc_set_solver(self.o, float(tol), int(max_sweeps))
+
+    def last_sweeps(self):
+        return int(self.lib.orc_last_sweeps(self.o))
 
     def residual(self):
         return float(self.lib.orc_last_residual(self.o))

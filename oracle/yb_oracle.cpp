@@ -387,7 +387,17 @@ struct orc_sim {
     std::vector<double> y, dx, dxNext, qNext, stepLimit, dxWarm;
     int64_t stepCounter = 0;
     double lastResidual = 0.0;
+    double tol = 0.0;     // > 0: sweep until the relative residual <= tol
+    int maxSweeps = 0;
+    int lastSweeps = 0;
+    int64_t totalSweeps = 0;
 };
+
+extern "C" void orc_set_solver(orc_sim* o, double tol, int maxSweeps) {
+    o->tol = tol;
+    o->maxSweeps = maxSweeps;
+}
+extern "C" int orc_last_sweeps(const orc_sim* o) { return o->lastSweeps; }
 
 extern "C" void orc_set_threads(int n) {
     if (n > 0) omp_set_num_threads(n);
@@ -639,7 +649,11 @@ extern "C" int orc_substep(orc_sim* o, const yb_params* p, int doDetect) {
     // numItr strict-Jacobi sweeps: dx then frames (cosserat.cu:20-253).
     std::vector<SegTerms> seg(n);
     double num = 0.0, den = 0.0;
-    for (int it = 0; it < p->numItr; it++) {
+    const bool adaptive = o->tol > 0;
+    const int itMax = adaptive ? o->maxSweeps : p->numItr;
+    int used = 0;
+    for (int it = 0; it < itMax; it++) {
+        used = it + 1;
 #pragma omp parallel for schedule(static)
         for (int s = 0; s < n; s++)
             if (has(o->flags[s], YB_FLAG_HAS_NEXT)) seg[s] = seg_terms(o, p, damping, s, o->dx);
@@ -680,7 +694,10 @@ extern "C" int orc_substep(orc_sim* o, const yb_params* p, int doDetect) {
             if (l > limit && l > 0) nd = mulr(nd, limit / l);
             s3(o->dxNext, v, nd);
             num += len2(sub(nd, d));
-            den += len2(nd);
+            double li = o->lRest[v];
+            if (!has(o->flags[v], YB_FLAG_HAS_NEXT) && has(o->flags[v], YB_FLAG_HAS_PREV))
+                li = o->lRest[v - 1];
+            den += li * li;
         }
         std::swap(o->dx, o->dxNext);
 
@@ -715,10 +732,13 @@ extern "C" int orc_substep(orc_sim* o, const yb_params* p, int doDetect) {
             s4(o->qNext, v, q0);
         }
         std::swap(o->q, o->qNext);
+        // Relative Newton residual of this sweep: RMS |dx_k+1 - dx_k| over free
+        // vertices in units of the rest length.
+        o->lastResidual = den > 0 ? std::sqrt(num / den) : 0.0;
+        if (adaptive && o->lastResidual <= o->tol) break;
     }
-    // Relative Newton residual of the final sweep: |dx_K - dx_{K-1}| / |dx_K|
-    // over free vertices.
-    o->lastResidual = den > 0 ? std::sqrt(num / den) : 0.0;
+    o->lastSweeps = used;
+    o->totalSweeps += used;
 
     // endItr (iteration.cu:48-61).
     const double s = 1.0 - (double)p->drag * h;

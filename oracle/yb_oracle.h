@@ -55,6 +55,11 @@ int orc_advance(orc_sim* o, yb_params* p, float h, float maxH);
 void orc_contacts(const orc_sim* o, int32_t* numCols, int32_t* cols);
 /* Step limits (per segment) and warm-start dx (3n) of the last substep. */
 void orc_last_prologue(const orc_sim* o, double* stepLimit, double* dxWarm);
+/* Residual-targeted mode (tol > 0): sweeps until
+ * sqrt(sum_free |dx_k+1 - dx_k|^2 / sum_free l_i^2) <= tol or maxSweeps;
+ * tol <= 0 runs p->numItr sweeps. */
+void orc_set_solver(orc_sim* o, double tol, int maxSweeps);
+int orc_last_sweeps(const orc_sim* o);
 /* Relative Newton residual of the last substep's final iterate (see .cpp). */
 double orc_last_residual(const orc_sim* o);
 

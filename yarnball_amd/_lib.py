@@ -38,6 +38,17 @@ VERTEX_DTYPE = np.dtype(
 assert VERTEX_DTYPE.itemsize == 32
 
 
+class SolverStats(ctypes.Structure):
+    _fields_ = [
+        ("substeps", ctypes.c_int64),
+        ("sweeps", ctypes.c_int64),
+        ("maxSweepsUsed", ctypes.c_int32),
+        ("notConverged", ctypes.c_int32),
+        ("lastResidual", ctypes.c_float),
+        ("maxResidual", ctypes.c_float),
+    ]
+
+
 class Params(ctypes.Structure):
     """yb_params: the scalar fields of MetaData (YarnBall.h:63-114)."""
 
@@ -104,9 +115,13 @@ SIGNATURES = {
     "yb_debug_read_last_vels": (_I, [_P, _P]),
     "yb_debug_write_last_vels": (_I, [_P, _P]),
     "yb_profile": (_I, [_P, ctypes.POINTER(Params), _F, _I, _P, _P]),
+    "yb_set_solver": (_I, [_P, _F, _I, _I]),
+    "yb_min_separation": (_I, [_P, _P, _I]),
+    "yb_solver_stats": (_I, [_P, ctypes.POINTER(SolverStats), _I]),
 }
 
-STAGES = ["init", "aabb", "rebuild", "refit", "query", "step_limit", "sweep_dx", "sweep_q", "end"]
+STAGES = ["init", "aabb", "rebuild", "refit", "query", "step_limit", "sweep_dx", "sweep_q", "end",
+          "residual"]
 
 _lib = None
 

@@ -13,6 +13,8 @@
 #include <math.h>
 #include <string.h>
 
+#include <algorithm>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -45,14 +47,28 @@ static void free_all(yb_sim* s) {
                     b.aabb,    b.sceneBox, b.blockBox, b.morton, b.keys[0], b.keys[1],
                     b.vals[0], b.vals[1], b.hist,     b.nodes,   b.parent,  b.rng,
                     b.kids,    b.strad,   b.stradCount, b.leafNeed, b.sufBox, b.preBox, b.blkTab,
-                    b.err,     b.params,  b.stage};
+                    b.err,     b.params,  b.stage, b.ctl, b.partial, b.minSep};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
+    if (b.hostFlag) (void)hipHostFree(b.hostFlag);
     b = Buffers();
-    for (int i = 0; i < 3; i++)
-        if (s->graphs[i]) {
-            (void)hipGraphExecDestroy(s->graphs[i]);
-            s->graphs[i] = nullptr;
+    for (int i = 0; i < 3; i++) {
+        if (s->graphs[i]) (void)hipGraphExecDestroy(s->graphs[i]);
+        if (s->proGraphs[i]) (void)hipGraphExecDestroy(s->proGraphs[i]);
+        s->graphs[i] = s->proGraphs[i] = nullptr;
+    }
+    if (s->endGraph) (void)hipGraphExecDestroy(s->endGraph);
+    s->endGraph = nullptr;
+    if (s->chunkGraphs) {
+        for (int c = 0; c < s->numChunks; c++)
+            if (s->chunkGraphs[c]) (void)hipGraphExecDestroy(s->chunkGraphs[c]);
+        delete[] s->chunkGraphs;
+        s->chunkGraphs = nullptr;
+    }
+    for (int i = 0; i < 2; i++)
+        if (s->chunkEv[i]) {
+            (void)hipEventDestroy(s->chunkEv[i]);
+            s->chunkEv[i] = nullptr;
         }
     if (s->stream) {
         (void)hipStreamDestroy(s->stream);
@@ -128,7 +144,23 @@ extern "C" int yb_create(int n, yb_sim** out) {
     ALLOC(b.blkTab, 2 * (size_t)s->refitBlocks * s->refitLevels);
     ALLOC(b.err, 2);
     ALLOC(b.params, 1);
+    ALLOC(b.ctl, 1);
+    ALLOC(b.minSep, 1);
+    ALLOC(b.partial, (size_t)sweep_blocks(n));
 #undef ALLOC
+    if (e == hipSuccess) e = hipHostMalloc((void**)&b.hostFlag, 2 * sizeof(int32_t));
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&s->chunkEv[0], hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&s->chunkEv[1], hipEventDisableTiming);
+    if (e == hipSuccess) {
+        const float inf = INFINITY;
+        e = hipMemcpy(b.minSep, &inf, sizeof(float), hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess) {
+        SolverCtl z;
+        memset(&z, 0, sizeof(z));
+        z.doneAt = 0x7fffffff;
+        e = hipMemcpy(b.ctl, &z, sizeof(z), hipMemcpyHostToDevice);
+    }
     b.order = b.vals[0];
     b.stageBytes = N * (sizeof(yb_vertex) + 11 * sizeof(float));
     if (e == hipSuccess) e = hipMalloc(&b.stage, b.stageBytes);
@@ -188,6 +220,19 @@ extern "C" int yb_upload(yb_sim* s, const yb_vertex* verts, const float* vels, c
     HIPCHECK(hipMemcpyAsync(st, qRests, sizeof(float) * 4 * N, hipMemcpyHostToDevice, s->stream));
     launch_upload(s);
     HIPCHECK(hipGetLastError());
+    // Residual normaliser: sum over free vertices of l_i^2, l_i the rest length
+    // of the vertex's own segment (or of the previous one at a strand end).
+    double denom = 0.0;
+    for (size_t i = 0; i < N; i++) {
+        if (verts[i].invMass == 0.f) continue;
+        double l = verts[i].lRest;
+        if (!(verts[i].flags & YB_FLAG_HAS_NEXT) && (verts[i].flags & YB_FLAG_HAS_PREV) && i > 0)
+            l = verts[i - 1].lRest;
+        denom += l * l;
+    }
+    s->denomHost = denom;
+    HIPCHECK(hipMemcpyAsync(&s->b.ctl->denom, &s->denomHost, sizeof(double),
+                            hipMemcpyHostToDevice, s->stream));
     HIPCHECK(hipStreamSynchronize(s->stream));
     return YB_OK;
 }
@@ -253,37 +298,191 @@ static int upload_params(yb_sim* s, const yb_params* p) {
     return YB_OK;
 }
 
-// Enqueue one substep in detection mode m (0 none, 1 refit, 2 rebuild).
-static void enqueue_substep(yb_sim* s, int mode, int numItr) {
-    launch_init(s);
-    if (mode >= 1) {
-        launch_aabb(s);
-        if (mode == 2) launch_rebuild(s);
-        launch_refit(s);
-        launch_query(s);
+// Profiling hook: when s->prof is set (eager, uncaptured enqueue), an event
+// is recorded after every kernel with the stage it belongs to.
+struct ProfState {
+    std::vector<hipEvent_t> ev;
+    std::vector<int> stage;
+    int used = 0;
+};
+
+static void mark(yb_sim* s, int stage) {
+    ProfState* ps = (ProfState*)s->prof;
+    if (!ps) return;
+    if (ps->used == (int)ps->ev.size()) {
+        hipEvent_t e;
+        (void)hipEventCreate(&e);
+        ps->ev.push_back(e);
+        ps->stage.push_back(0);
     }
-    launch_step_limit(s);
-    for (int k = 0; k < numItr; k++) launch_sweep(s, k & 1);
-    launch_end(s, numItr & 1);
+    ps->stage[ps->used] = stage;
+    (void)hipEventRecord(ps->ev[ps->used++], s->stream);
 }
 
+// Substep prologue in detection mode m (0 none, 1 refit, 2 rebuild):
+// initItr, detectCollisions, recomputeStepLimit (step.cpp:111-116, 80-81).
+static void enqueue_prologue(yb_sim* s, int mode, bool adaptive) {
+    launch_init(s);
+    mark(s, YB_STAGE_INIT);
+    if (mode >= 1) {
+        launch_aabb(s);
+        mark(s, YB_STAGE_AABB);
+        if (mode == 2) {
+            launch_rebuild(s);
+            mark(s, YB_STAGE_REBUILD);
+        }
+        launch_refit(s);
+        mark(s, YB_STAGE_REFIT);
+        launch_query(s);
+        mark(s, YB_STAGE_QUERY);
+    }
+    launch_step_limit(s);
+    mark(s, YB_STAGE_STEP_LIMIT);
+    if (adaptive) {
+        launch_ctl_reset(s);
+        mark(s, YB_STAGE_RESIDUAL);
+    }
+}
+
+static void enqueue_sweep(yb_sim* s, int k, bool adaptive) {
+    launch_sweep_dx(s, k, adaptive);
+    mark(s, YB_STAGE_SWEEP_DX);
+    if (adaptive) {
+        launch_residual(s, k);
+        mark(s, YB_STAGE_RESIDUAL);
+    }
+    launch_sweep_q(s, k, adaptive);
+    mark(s, YB_STAGE_SWEEP_Q);
+}
+
+static void enqueue_end(yb_sim* s, int fin, bool adaptive) {
+    launch_end(s, fin, adaptive);
+    mark(s, YB_STAGE_END);
+}
+
+static int capture(yb_sim* s, hipGraphExec_t* out, const std::function<void()>& body) {
+    if (*out) {
+        HIPCHECK(hipGraphExecDestroy(*out));
+        *out = nullptr;
+    }
+    hipGraph_t g;
+    HIPCHECK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    body();
+    HIPCHECK(hipStreamEndCapture(s->stream, &g));
+    hipError_t e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIPCHECK(e);
+    return YB_OK;
+}
+
+// Fixed-sweep mode: one graph per detection mode holds the whole substep
+// (the reference's stepGraph plus initItr and detection, step.cpp:70-99).
 static int rebuild_graphs(yb_sim* s, int numItr) {
     if (s->graphItr == numItr && s->graphs[0]) return YB_OK;
     HIPCHECK(hipStreamSynchronize(s->stream));
     for (int m = 0; m < 3; m++) {
-        if (s->graphs[m]) {
-            HIPCHECK(hipGraphExecDestroy(s->graphs[m]));
-            s->graphs[m] = nullptr;
-        }
-        hipGraph_t g;
-        HIPCHECK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-        enqueue_substep(s, m, numItr);
-        HIPCHECK(hipStreamEndCapture(s->stream, &g));
-        hipError_t e = hipGraphInstantiate(&s->graphs[m], g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        HIPCHECK(e);
+        int rc = capture(s, &s->graphs[m], [&]() {
+            enqueue_prologue(s, m, false);
+            for (int k = 0; k < numItr; k++) enqueue_sweep(s, k, false);
+            enqueue_end(s, numItr & 1, false);
+        });
+        if (rc != YB_OK) return rc;
     }
     s->graphItr = numItr;
+    return YB_OK;
+}
+
+// Residual-targeted mode: prologue graphs, chunk graphs of s->chunk sweeps
+// with the residual reduction after each dx sweep, and the end graph.
+static int rebuild_adaptive_graphs(yb_sim* s) {
+    if (s->adaptiveReady) return YB_OK;
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    for (int m = 0; m < 3; m++) {
+        int rc = capture(s, &s->proGraphs[m], [&]() { enqueue_prologue(s, m, true); });
+        if (rc != YB_OK) return rc;
+    }
+    if (s->chunkGraphs) {
+        for (int c = 0; c < s->numChunks; c++)
+            if (s->chunkGraphs[c]) (void)hipGraphExecDestroy(s->chunkGraphs[c]);
+        delete[] s->chunkGraphs;
+    }
+    s->numChunks = (s->maxSweeps + s->chunk - 1) / s->chunk;
+    s->chunkGraphs = new hipGraphExec_t[s->numChunks]();
+    for (int c = 0; c < s->numChunks; c++) {
+        const int k0 = c * s->chunk;
+        const int k1 = std::min(k0 + s->chunk, s->maxSweeps);
+        int rc = capture(s, &s->chunkGraphs[c], [&]() {
+            for (int k = k0; k < k1; k++) enqueue_sweep(s, k, true);
+        });
+        if (rc != YB_OK) return rc;
+    }
+    int rc = capture(s, &s->endGraph, [&]() { enqueue_end(s, 0, true); });
+    if (rc != YB_OK) return rc;
+    s->adaptiveReady = true;
+    return YB_OK;
+}
+
+extern "C" int yb_set_solver(yb_sim* s, float residualTol, int maxSweeps, int chunk) {
+    if (!s) return YB_ERR_INVALID_ARGUMENT;
+    if (residualTol > 0 && (maxSweeps < 1 || chunk < 1)) {
+        set_message("yb_set_solver: maxSweeps and chunk must be >= 1");
+        return YB_ERR_INVALID_ARGUMENT;
+    }
+    if (residualTol != s->tol || maxSweeps != s->maxSweeps || chunk != s->chunk)
+        s->adaptiveReady = false;
+    s->tol = residualTol;
+    s->maxSweeps = maxSweeps;
+    s->chunk = chunk;
+    return YB_OK;
+}
+
+extern "C" int yb_solver_stats(yb_sim* s, yb_solver_stats_t* out, int reset) {
+    if (!s || !out) return YB_ERR_INVALID_ARGUMENT;
+    SolverCtl c;
+    HIPCHECK(hipMemcpyAsync(&c, s->b.ctl, sizeof(c), hipMemcpyDeviceToHost, s->stream));
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    out->substeps = (int64_t)c.substeps;
+    out->sweeps = (int64_t)c.sweeps;
+    out->maxSweepsUsed = c.maxUsed;
+    out->notConverged = c.notConverged;
+    out->lastResidual = c.resid;
+    out->maxResidual = c.maxResid;
+    if (reset) {
+        SolverCtl z;
+        memset(&z, 0, sizeof(z));
+        z.doneAt = 0x7fffffff;
+        z.denom = s->denomHost;
+        HIPCHECK(hipMemcpyAsync(s->b.ctl, &z, sizeof(z), hipMemcpyHostToDevice, s->stream));
+        HIPCHECK(hipStreamSynchronize(s->stream));
+    }
+    return YB_OK;
+}
+
+// One residual-targeted substep: prologue, chunks until converged (the host
+// reads the convergence flag after chunk c while chunk c+1 already runs; a
+// chunk launched after convergence exits immediately), end.
+static int adaptive_substep(yb_sim* s, int mode) {
+    HIPCHECK(hipGraphLaunch(s->proGraphs[mode], s->stream));
+    int launched = 0;
+    auto launch_chunk = [&](int c) -> int {
+        HIPCHECK(hipGraphLaunch(s->chunkGraphs[c], s->stream));
+        HIPCHECK(hipMemcpyAsync(&s->b.hostFlag[c & 1], &s->b.ctl->doneAt, sizeof(int32_t),
+                                hipMemcpyDeviceToHost, s->stream));
+        HIPCHECK(hipEventRecord(s->chunkEv[c & 1], s->stream));
+        launched = c + 1;
+        return YB_OK;
+    };
+    int rc = launch_chunk(0);
+    if (rc != YB_OK) return rc;
+    for (int c = 0; c < s->numChunks; c++) {
+        if (launched < s->numChunks && launched == c + 1) {
+            rc = launch_chunk(c + 1);
+            if (rc != YB_OK) return rc;
+        }
+        HIPCHECK(hipEventSynchronize(s->chunkEv[c & 1]));
+        if (s->b.hostFlag[c & 1] != 0x7fffffff) break;
+    }
+    HIPCHECK(hipGraphLaunch(s->endGraph, s->stream));
     return YB_OK;
 }
 
@@ -342,16 +541,46 @@ extern "C" int yb_advance(yb_sim* s, yb_params* p, float h, float maxH, int* war
     p->h = h / (float)steps;
     int rc = yb_validate_params(p);
     if (rc != YB_OK) return rc;
-    rc = rebuild_graphs(s, p->numItr);
-    if (rc != YB_OK) return rc;
+    if (s->tol <= 0) {
+        rc = rebuild_graphs(s, p->numItr);
+        if (rc != YB_OK) return rc;
+    }
     rc = upload_params(s, p);
     if (rc != YB_OK) return rc;
+    const bool adaptive = s->tol > 0;
+    if (adaptive) {
+        rc = rebuild_adaptive_graphs(s);
+        if (rc != YB_OK) return rc;
+    }
     for (int st = 0; st < steps; st++, s->stepCounter++) {
         const int mode = next_mode(s, p);
-        HIPCHECK(hipGraphLaunch(s->graphs[mode], s->stream));
+        if (adaptive) {
+            rc = adaptive_substep(s, mode);
+            if (rc != YB_OK) return rc;
+        } else {
+            HIPCHECK(hipGraphLaunch(s->graphs[mode], s->stream));
+        }
+    }
+    // Interpenetration check of the final state (the step limit checks every
+    // substep's starting state).
+    if (p->detectionPeriod > 0) {
+        launch_separation(s);
+        HIPCHECK(hipGetLastError());
     }
     p->time += h;
     return check_errors(s, warning);
+}
+
+extern "C" int yb_min_separation(yb_sim* s, float* out, int reset) {
+    if (!s || !out) return YB_ERR_INVALID_ARGUMENT;
+    HIPCHECK(hipMemcpyAsync(out, s->b.minSep, sizeof(float), hipMemcpyDeviceToHost, s->stream));
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    if (reset) {
+        const float inf = INFINITY;
+        HIPCHECK(hipMemcpyAsync(s->b.minSep, &inf, sizeof(float), hipMemcpyHostToDevice, s->stream));
+        HIPCHECK(hipStreamSynchronize(s->stream));
+    }
+    return YB_OK;
 }
 
 extern "C" int64_t yb_step_count(yb_sim* s) { return s ? s->stepCounter : 0; }
@@ -391,17 +620,13 @@ extern "C" int yb_debug_prologue(yb_sim* s, yb_params* p, float h, int detect, i
     if (rc != YB_OK) return rc;
     rc = upload_params(s, p);
     if (rc != YB_OK) return rc;
-    launch_init(s);
+    int mode = 0;
     if (detect) {
-        launch_aabb(s);
-        if (rebuild || !s->treeBuilt) {
-            launch_rebuild(s);
-            s->treeBuilt = true;
-        }
-        launch_refit(s);
-        launch_query(s);
+        mode = (rebuild || !s->treeBuilt) ? 2 : 1;
+        s->treeBuilt = true;
+        if (mode == 2) s->lastBVHRebuild = 0.f;
     }
-    launch_step_limit(s);
+    enqueue_prologue(s, mode, false);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(s->stream));
     return check_errors(s, nullptr);
@@ -509,52 +734,48 @@ extern "C" int yb_profile(yb_sim* s, yb_params* p, float h, int substeps, double
     if (rc != YB_OK) return rc;
     rc = upload_params(s, p);
     if (rc != YB_OK) return rc;
-    const int maxEv = 64 + 2 * p->numItr;
-    std::vector<hipEvent_t> ev(maxEv);
-    for (auto& e : ev) HIPCHECK(hipEventCreate(&e));
-    std::vector<int> stage(maxEv);
-    for (int st = 0; st < substeps; st++, s->stepCounter++) {
-        const int mode = next_mode(s, p);
-        int ne = 0;
-        HIPCHECK(hipEventRecord(ev[ne++], s->stream));
-        auto mark = [&](int sid) {
-            stage[ne] = sid;
-            (void)hipEventRecord(ev[ne++], s->stream);
-        };
-        launch_init(s);
-        mark(YB_STAGE_INIT);
-        if (mode >= 1) {
-            launch_aabb(s);
-            mark(YB_STAGE_AABB);
-            if (mode == 2) {
-                launch_rebuild(s);
-                mark(YB_STAGE_REBUILD);
-            }
-            launch_refit(s);
-            mark(YB_STAGE_REFIT);
-            launch_query(s);
-            mark(YB_STAGE_QUERY);
-        }
-        launch_step_limit(s);
-        mark(YB_STAGE_STEP_LIMIT);
-        for (int k = 0; k < p->numItr; k++) {
-            launch_sweep_dx(s, k & 1);
-            mark(YB_STAGE_SWEEP_DX);
-            launch_sweep_q(s, k & 1);
-            mark(YB_STAGE_SWEEP_Q);
-        }
-        launch_end(s, p->numItr & 1);
-        mark(YB_STAGE_END);
-        HIPCHECK(hipGetLastError());
-        HIPCHECK(hipEventSynchronize(ev[ne - 1]));
-        for (int k = 1; k < ne; k++) {
+    const bool adaptive = s->tol > 0;
+    ProfState ps;
+    s->prof = &ps;
+    auto harvest = [&]() -> int {
+        for (int k = 1; k < ps.used; k++) {
             float t = 0.f;
-            HIPCHECK(hipEventElapsedTime(&t, ev[k - 1], ev[k]));
-            ms[stage[k]] += t;
-            launches[stage[k]] += 1;
+            if (ps.stage[k] < 0) continue;  // gap marker: host latency, not a kernel
+            HIPCHECK(hipEventElapsedTime(&t, ps.ev[k - 1], ps.ev[k]));
+            ms[ps.stage[k]] += t;
+            launches[ps.stage[k]] += 1;
         }
+        return YB_OK;
+    };
+    for (int st = 0; st < substeps && rc == YB_OK; st++, s->stepCounter++) {
+        const int mode = next_mode(s, p);
+        ps.used = 0;
+        mark(s, -1);
+        enqueue_prologue(s, mode, adaptive);
+        int fin = p->numItr & 1;
+        if (adaptive) {
+            // one sweep at a time so that no early-exit launch is timed
+            for (int k = 0; k < s->maxSweeps; k++) {
+                mark(s, -1);
+                enqueue_sweep(s, k, true);
+                int32_t done = 0;
+                HIPCHECK(hipMemcpyAsync(&done, &s->b.ctl->doneAt, 4, hipMemcpyDeviceToHost,
+                                        s->stream));
+                HIPCHECK(hipStreamSynchronize(s->stream));
+                if (done != 0x7fffffff) break;
+            }
+        } else {
+            for (int k = 0; k < p->numItr; k++) enqueue_sweep(s, k, false);
+        }
+        mark(s, -1);
+        enqueue_end(s, fin, adaptive);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipStreamSynchronize(s->stream));
+        rc = harvest();
     }
-    for (auto& e : ev) (void)hipEventDestroy(e);
+    s->prof = nullptr;
+    for (auto& e : ps.ev) (void)hipEventDestroy(e);
+    if (rc != YB_OK) return rc;
     p->time += h * substeps;
     return check_errors(s, nullptr);
 }

@@ -30,38 +30,58 @@ __device__ __forceinline__ float4 f4r(rot a) { return make_float4(a.x, a.y, a.z,
 // ---------------------------------------------------------------------------
 // Step limit (collision.cu:121-159): the largest move for which the padded
 // broadphase query still sees every possible contact, and at most 45% of the
-// distance to every listed contact.
+// distance to every listed contact.  The same contact distances also feed the
+// interpenetration check the reference declares but never sets (YarnBall.h:121):
+// a listed pair closer than 2 * radius raises WARNING_SEGMENT_INTERPENETRATION,
+// and the smallest distance / (2 * radius) seen is kept in *minSep.
+__device__ __forceinline__ void reduce_min_sep(float sep, float* minSep) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) sep = fminf(sep, __shfl_xor(sep, off, 64));
+    // positive floats order like their bit patterns
+    if ((threadIdx.x & 63) == 0 && sep < INFINITY)
+        atomicMin(reinterpret_cast<int*>(minSep), __float_as_int(sep));
+}
+
+template <bool WRITE_LIMIT>
 __global__ __launch_bounds__(256) void k_step_limit(const DevParams* __restrict__ P,
                                                     const float4* __restrict__ pos,
                                                     const float4* __restrict__ prm,
                                                     float4* __restrict__ dx,
                                                     const int32_t* __restrict__ numCols,
-                                                    const int32_t* __restrict__ cols) {
+                                                    const int32_t* __restrict__ cols,
+                                                    int32_t* __restrict__ err,
+                                                    float* __restrict__ minSep) {
     const int n = P->n;
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= n) return;
     float minDist = INFINITY;
-    const uint32_t flags = __float_as_uint(prm[v].w);
-    float4 d4 = dx[v];
-    if (P->useStepSizeLimit && (flags & YB_FLAG_HAS_NEXT)) {
-        v3 a0 = xyz(pos[v]);
-        v3 a1 = xyz(pos[v + 1]) - a0;
-        minDist = P->detectionRadius * (P->detectionScaler - 1.0f);
-        if (P->useVelocityRadius) minDist += len(xyz(d4));
-        const int nc = numCols[v];
-        for (int i = 0; i < nc; i++) {
-            int o = cols[v + n * i];
-            v3 b0 = xyz(pos[o]) - a0;
-            v3 b1 = xyz(pos[o + 1]) - a0;
-            float u, w;
-            closest_uv(v3{0.f, 0.f, 0.f}, a1, b0, b1, u, w);
-            v3 nrm = u * a1 - mixv(b0, b1, w);
-            minDist = fminf(minDist, 0.45f * len(nrm));
+    float minContact = INFINITY;
+    if (v < n) {
+        const uint32_t flags = __float_as_uint(prm[v].w);
+        if (flags & YB_FLAG_HAS_NEXT) {
+            const v3 a0 = xyz(pos[v]);
+            const v3 a1 = xyz(pos[v + 1]) - a0;
+            const int nc = numCols[v];
+            for (int i = 0; i < nc; i++) {
+                const int o = cols[v + n * i];
+                const v3 b0 = xyz(pos[o]) - a0;
+                const v3 b1 = xyz(pos[o + 1]) - a0;
+                float u, w;
+                closest_uv(v3{0.f, 0.f, 0.f}, a1, b0, b1, u, w);
+                minContact = fminf(minContact, len(u * a1 - mixv(b0, b1, w)));
+            }
+            if (WRITE_LIMIT && P->useStepSizeLimit) {
+                minDist = P->detectionRadius * (P->detectionScaler - 1.0f);
+                if (P->useVelocityRadius) minDist += len(xyz(dx[v]));
+                minDist = fminf(minDist, 0.45f * minContact);
+            }
         }
+        if (WRITE_LIMIT) dx[v].w = minDist;
+        if (minContact < 2.0f * P->radius) err[1] = YB_WARNING_SEGMENT_INTERPENETRATION;
     }
-    d4.w = minDist;
-    dx[v] = d4;
+    reduce_min_sep(minContact / (2.0f * P->radius), minSep);
 }
+
+__device__ __forceinline__ void block_residual(float num, double* partial);
 
 // ---------------------------------------------------------------------------
 // One Newton sweep (cosserat.cu:20-207).  For vertex i:
@@ -76,7 +96,11 @@ __global__ __launch_bounds__(256) void k_sweep_dx(const DevParams* __restrict__ 
                                                   const float4* __restrict__ dxIn,
                                                   float4* __restrict__ dxOut,
                                                   const int32_t* __restrict__ numCols,
-                                                  const int32_t* __restrict__ cols) {
+                                                  const int32_t* __restrict__ cols,
+                                                  const SolverCtl* __restrict__ ctl, int sweep,
+                                                  double* __restrict__ partial) {
+    // Residual-targeted mode: sweeps after convergence do nothing.
+    if (ctl && ctl->doneAt < sweep) return;
     const int n = P->n;
     const int lane = threadIdx.x & 63;
     const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
@@ -204,26 +228,80 @@ __global__ __launch_bounds__(256) void k_sweep_dx(const DevParams* __restrict__ 
     for (int k = 0; k < 6; k++) ph[k] = __shfl_up(H2.a[k], 1, 64);
     const float pl = __shfl_up(segLimit, 1, 64);
 
-    if (!in || lane == 0) return;
-
-    float limit = segLimit;
-    if (flags & YB_FLAG_HAS_PREV) {
-        limit = fminf(limit, pl);
-        f += v3{pf0, pf1, pf2};
+    float num = 0.0f;  // |dx_new - dx_old|^2
+    if (in && lane != 0) {
+        float limit = segLimit;
+        if (flags & YB_FLAG_HAS_PREV) {
+            limit = fminf(limit, pl);
+            f += v3{pf0, pf1, pf2};
 #pragma unroll
-        for (int k = 0; k < 6; k++) H.a[k] += ph[k];
+            for (int k = 0; k < 6; k++) H.a[k] += ph[k];
+        }
+        float4 out = d4;
+        if (p4.w != 0.0f) {
+            // Local 3x3 solve (cosserat.cu:195-205).
+            v3 dx = xyz(d4);
+            dx += P->accelerationRatio * sym_solve(H, f);
+            const float l = len(dx);
+            if (l > limit && l > 0.0f) dx *= limit / l;
+            out = f4(dx, d4.w);
+            num = len2(dx - xyz(d4));
+        }
+        dxOut[v] = out;
     }
+    if (partial) block_residual(num, partial);
+}
 
-    float4 out = d4;
-    if (p4.w != 0.0f) {
-        // Local 3x3 solve (cosserat.cu:195-205).
-        v3 dx = xyz(d4);
-        dx += P->accelerationRatio * sym_solve(H, f);
-        const float l = len(dx);
-        if (l > limit && l > 0.0f) dx *= limit / l;
-        out = f4(dx, d4.w);
+// Per-sweep relative Newton residual
+//   r = sqrt( sum_free |dx_k+1 - dx_k|^2 / sum_free l_i^2 ),
+// the RMS change of the iterate in units of the rest length (l_i: the rest
+// length of vertex i's own segment, or of the previous one at a strand end).
+// Per-block partial sums here, one small reduction in k_residual.
+__device__ __forceinline__ void block_residual(float num, double* partial) {
+    __shared__ float sn[4];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) num += __shfl_xor(num, off, 64);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) sn[w] = num;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); k++) a += sn[k];
+        partial[blockIdx.x] = a;
     }
-    dxOut[v] = out;
+}
+
+__global__ __launch_bounds__(256) void k_residual(SolverCtl* __restrict__ ctl,
+                                                  const double* __restrict__ partial, int nb,
+                                                  int sweep, float tol, int maxSweeps) {
+    if (ctl->doneAt < sweep) return;
+    __shared__ double sa[256];
+    double a = 0.0;
+    for (int k = threadIdx.x; k < nb; k += blockDim.x) a += partial[k];
+    sa[threadIdx.x] = a;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) sa[threadIdx.x] += sa[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double den = ctl->denom;
+        const float r = den > 0.0 ? (float)sqrt(sa[0] / den) : 0.0f;
+        ctl->resid = r;
+        if (r <= tol || sweep == maxSweeps - 1) {
+            ctl->doneAt = sweep;
+            ctl->fin = (sweep & 1) ^ 1;
+            ctl->sweeps += (unsigned long long)(sweep + 1);
+            ctl->substeps += 1ull;
+            if (sweep + 1 > ctl->maxUsed) ctl->maxUsed = sweep + 1;
+            if (r > tol) ctl->notConverged += 1;
+            if (r > ctl->maxResid) ctl->maxResid = r;
+        }
+    }
+}
+
+__global__ void k_ctl_reset(SolverCtl* __restrict__ ctl) {
+    if (threadIdx.x == 0) ctl->doneAt = 0x7fffffff;
 }
 
 // ---------------------------------------------------------------------------
@@ -235,7 +313,9 @@ __global__ __launch_bounds__(256) void k_sweep_q(const DevParams* __restrict__ P
                                                  const float4* __restrict__ dxNew,
                                                  const float4* __restrict__ qIn,
                                                  float4* __restrict__ qOut,
-                                                 const float4* __restrict__ qRest) {
+                                                 const float4* __restrict__ qRest,
+                                                 const SolverCtl* __restrict__ ctl, int sweep) {
+    if (ctl && ctl->doneAt < sweep) return;
     const int n = P->n;
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n) return;
@@ -280,10 +360,17 @@ __global__ __launch_bounds__(256) void k_end(const DevParams* __restrict__ P,
                                              float4* __restrict__ vel,
                                              const float4* __restrict__ dxFin,
                                              const float4* __restrict__ qFin,
-                                             float4* __restrict__ q0) {
+                                             float4* __restrict__ q0,
+                                             const SolverCtl* __restrict__ ctl,
+                                             const float4* __restrict__ dxAlt,
+                                             const float4* __restrict__ qAlt) {
     const int n = P->n;
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n) return;
+    if (ctl && ctl->fin == 1) {  // residual-targeted mode: final iterate in buffer 1
+        dxFin = dxAlt;
+        qFin = qAlt;
+    }
     const float h = P->h;
     const float invH = 1.0f / h;
     const v3 dx = xyz(dxFin[v]);
@@ -394,32 +481,49 @@ void launch_fill_i32(yb_sim* s, int32_t* p, int32_t v, int count) {
 }
 
 void launch_step_limit(yb_sim* s) {
-    hipLaunchKernelGGL(k_step_limit, dim3(blocks(s->n, 256)), dim3(256), 0, s->stream, s->b.params,
-                       s->b.pos, s->b.prm, s->b.dx[0], s->b.numCols, s->b.cols);
+    hipLaunchKernelGGL(k_step_limit<true>, dim3(blocks(s->n, 256)), dim3(256), 0, s->stream,
+                       s->b.params, s->b.pos, s->b.prm, s->b.dx[0], s->b.numCols, s->b.cols,
+                       s->b.err, s->b.minSep);
 }
 
-void launch_sweep_dx(yb_sim* s, int in) {
+void launch_separation(yb_sim* s) {
+    hipLaunchKernelGGL(k_step_limit<false>, dim3(blocks(s->n, 256)), dim3(256), 0, s->stream,
+                       s->b.params, s->b.pos, s->b.prm, s->b.dx[0], s->b.numCols, s->b.cols,
+                       s->b.err, s->b.minSep);
+}
+
+int sweep_blocks(int n) { return blocks(blocks(n + 1, 63) * 64, 256); }
+
+void launch_sweep_dx(yb_sim* s, int sweep, bool adaptive) {
+    const int in = sweep & 1;
     const int out = in ^ 1;
-    const int waves = blocks(s->n + 1, 63);
-    hipLaunchKernelGGL(k_sweep_dx, dim3(blocks(waves * 64, 256)), dim3(256), 0, s->stream,
+    hipLaunchKernelGGL(k_sweep_dx, dim3(sweep_blocks(s->n)), dim3(256), 0, s->stream,
                        s->b.params, s->b.pos, s->b.prm, s->b.yv, s->b.q[in], s->b.dx[in],
-                       s->b.dx[out], s->b.numCols, s->b.cols);
+                       s->b.dx[out], s->b.numCols, s->b.cols, adaptive ? s->b.ctl : nullptr,
+                       sweep, adaptive ? s->b.partial : nullptr);
 }
 
-void launch_sweep_q(yb_sim* s, int in) {
+void launch_residual(yb_sim* s, int sweep) {
+    hipLaunchKernelGGL(k_residual, dim3(1), dim3(256), 0, s->stream, s->b.ctl, s->b.partial,
+                       sweep_blocks(s->n), sweep, s->tol, s->maxSweeps);
+}
+
+void launch_sweep_q(yb_sim* s, int sweep, bool adaptive) {
+    const int in = sweep & 1;
     const int out = in ^ 1;
     hipLaunchKernelGGL(k_sweep_q, dim3(blocks(s->n, 256)), dim3(256), 0, s->stream, s->b.params,
-                       s->b.pos, s->b.prm, s->b.dx[out], s->b.q[in], s->b.q[out], s->b.qRest);
+                       s->b.pos, s->b.prm, s->b.dx[out], s->b.q[in], s->b.q[out], s->b.qRest,
+                       adaptive ? s->b.ctl : nullptr, sweep);
 }
 
-void launch_sweep(yb_sim* s, int in) {
-    launch_sweep_dx(s, in);
-    launch_sweep_q(s, in);
+void launch_ctl_reset(yb_sim* s) {
+    hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(64), 0, s->stream, s->b.ctl);
 }
 
-void launch_end(yb_sim* s, int fin) {
+void launch_end(yb_sim* s, int fin, bool adaptive) {
     hipLaunchKernelGGL(k_end, dim3(blocks(s->n, 256)), dim3(256), 0, s->stream, s->b.params,
-                       s->b.pos, s->b.vel, s->b.dx[fin], s->b.q[fin], s->b.q[0]);
+                       s->b.pos, s->b.vel, s->b.dx[fin], s->b.q[fin], s->b.q[0],
+                       adaptive ? s->b.ctl : nullptr, s->b.dx[1], s->b.q[1]);
 }
 
 }  // namespace yb

@@ -36,6 +36,19 @@ struct DevParams {
     int32_t pad_;
 };
 
+// Residual-targeted solver state (device side, one per handle).
+struct SolverCtl {
+    int32_t doneAt;  // sweep index at which the current substep converged (INT_MAX: running)
+    int32_t fin;     // buffer (0/1) holding the final iterate
+    float resid;     // relative residual of the last evaluated sweep
+    int32_t maxUsed;
+    unsigned long long sweeps;    // sweeps run, summed over substeps
+    unsigned long long substeps;  // substeps run in this mode
+    int32_t notConverged;         // substeps that hit maxSweeps above the tolerance
+    float maxResid;               // largest final residual
+    double denom;                 // sum over free vertices of l_i^2 (set at upload)
+};
+
 constexpr int kMaxCols = YB_MAX_COLLISIONS_PER_SEGMENT;
 constexpr int kSortBits = 8;
 constexpr int kSortBins = 1 << kSortBits;
@@ -81,7 +94,11 @@ struct Buffers {
     float4* preBox = nullptr;     // 2n
     float4* blkTab = nullptr;     // sparse table of refit-block unions, 2*nb per level
     int32_t* err = nullptr;       // [0] error code, [1] warning code
+    float* minSep = nullptr;      // smallest contact distance / (2 radius) seen
     DevParams* params = nullptr;
+    SolverCtl* ctl = nullptr;
+    double* partial = nullptr;    // per-block residual partial sums
+    int32_t* hostFlag = nullptr;  // pinned: doneAt read back after each chunk
     // upload / download staging
     void* stage = nullptr;
     size_t stageBytes = 0;
@@ -104,6 +121,19 @@ struct yb_sim {
     // captured substep graphs: [detect mode 0=none,1=refit,2=rebuild]
     hipGraphExec_t graphs[3] = {nullptr, nullptr, nullptr};
     int graphItr = -1;
+    // residual-targeted mode (tol > 0): prologue graphs per detection mode,
+    // one graph per chunk of sweeps, one end graph
+    float tol = 0.f;
+    int maxSweeps = 0;
+    int chunk = 8;
+    hipGraphExec_t proGraphs[3] = {nullptr, nullptr, nullptr};
+    hipGraphExec_t endGraph = nullptr;
+    hipGraphExec_t* chunkGraphs = nullptr;
+    int numChunks = 0;
+    bool adaptiveReady = false;
+    hipEvent_t chunkEv[2] = {nullptr, nullptr};
+    void* prof = nullptr;  // eager-profiling event log (capi.hip)
+    double denomHost = 0.0;
     int graphLimit = -1;
     int graphVel = -1;
     float hostBox[8];
@@ -119,10 +149,15 @@ void launch_download(yb_sim* s);   // SoA -> staging AoS
 void launch_zero_vel(yb_sim* s);
 void launch_init(yb_sim* s);                    // iteration.cu:7-41
 void launch_step_limit(yb_sim* s);              // collision.cu:121-172
-void launch_sweep(yb_sim* s, int in);           // cosserat.cu:20-253, reads dx[in], q[in]
-void launch_sweep_dx(yb_sim* s, int in);        // cosserat.cu:20-207
-void launch_sweep_q(yb_sim* s, int in);         // cosserat.cu:209-253
-void launch_end(yb_sim* s, int fin);            // iteration.cu:48-61
+void launch_separation(yb_sim* s);              // contact distances of the current state
+// Sweep k reads dx[k&1], q[k&1] and writes the other buffer.  adaptive: the
+// kernel honours SolverCtl (early exit after convergence, residual partials).
+void launch_sweep_dx(yb_sim* s, int sweep, bool adaptive);  // cosserat.cu:20-207
+void launch_sweep_q(yb_sim* s, int sweep, bool adaptive);   // cosserat.cu:209-253
+void launch_residual(yb_sim* s, int sweep);
+void launch_ctl_reset(yb_sim* s);
+void launch_end(yb_sim* s, int fin, bool adaptive);         // iteration.cu:48-61
+int sweep_blocks(int n);
 void launch_fill_i32(yb_sim* s, int32_t* p, int32_t v, int count);
 // broadphase (collision.cu:10-119 + the LBVH the reference links)
 void launch_aabb(yb_sim* s);

@@ -95,7 +95,7 @@ def readme_scene():
     seg = 0.002
     pos = np.zeros((n, 3), np.float32)
     for i in range(32):
-        x = np.float32(0.00002) * np.float32(i * (i - 16)) + np.float32(seg)
+        x = (np.float32(0.00002) * np.float32(i)) * np.float32(i - 16) + np.float32(seg)
         pos[i] = (x, -np.float32(seg) * np.float32(i), 0.0)
         pos[i + 32] = (-x, -np.float32(seg) * np.float32(i), 0.0)
     sc = _configure("readme_64", pos, breaks_after=[31], pinned=[0, 32, 63], kCollision=1e-7,

@@ -154,6 +154,28 @@ class Sim:
         print(f"Total collisions: {int(c[c > 0].sum()) // 2}")
         print(f"Segments with collisions: {int((c > 0).sum())}/{self.numVerts}")
 
+    # -- residual-targeted solving (no counterpart in the reference) -----------
+    def setSolver(self, residualTol, maxSweeps=512, chunk=8):
+        """Sweep every substep until the relative Newton residual
+        RMS |dx_k+1 - dx_k| / rest length <= residualTol (or maxSweeps); residualTol <= 0
+        restores the fixed meta.numItr schedule."""
+        self._need_handle()
+        check(self._lib.yb_set_solver(self._h, float(residualTol), int(maxSweeps), int(chunk)))
+
+    def solverStats(self, reset=False):
+        st = _lib.SolverStats()
+        self._need_handle()
+        check(self._lib.yb_solver_stats(self._h, ctypes.byref(st), int(bool(reset))))
+        return {f: getattr(st, f) for f, _ in st._fields_}
+
+    def minSeparation(self, reset=False):
+        """Smallest contact distance / (2 radius) since the last reset (below 1:
+        interpenetration, reported as WARNING_SEGMENT_INTERPENETRATION)."""
+        out = np.zeros(1, np.float32)
+        self._need_handle()
+        check(self._lib.yb_min_separation(self._h, ptr(out), int(bool(reset))))
+        return float(out[0])
+
     # -- stage-level access (parity tests) --------------------------------------
     @property
     def handle(self):
