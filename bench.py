@@ -36,8 +36,9 @@ def stage_bytes(stage, c):
     return {
         "init": 76.0,
         "step_limit": 36.0 + 4.0 * c,
-        "sweep_dx": 96.0 + 4.0 * c,
-        "sweep_q": 84.0,
+        # fused sweep: pos+invMass 16, lRest/kStretch/cid/flags 16, y 12,
+        # dx+stepLimit 16, q 16, qRest 16, numCols 4, ids 4c; writes dx 12, q 16
+        "sweep": 124.0 + 4.0 * c,
         "end": 52.0,
         "aabb": 52.0,
     }.get(stage)
@@ -55,7 +56,7 @@ def build_workload(name, numItr):
     raise SystemExit(f"unknown workload {name}")
 
 
-def cpu_baseline(sc, substeps, threads):
+def cpu_baseline(sc, substeps, threads, residual, max_sweeps):
     from oracle import Oracle, load
 
     lib = load()
@@ -63,6 +64,8 @@ def cpu_baseline(sc, substeps, threads):
         lib.orc_set_threads(threads)
     used = lib.orc_get_threads()
     orc = Oracle(sc)
+    if residual > 0:
+        orc.set_solver(residual, max_sweeps)
     p = sc.params.copy()
     # one untimed substep (first-touch of the oracle's arrays)
     orc.advance(p, sc.maxH, sc.maxH)
@@ -77,14 +80,17 @@ def cpu_baseline(sc, substeps, threads):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="knit_patch")
     ap.add_argument("--numItr", type=int, default=8)
-    ap.add_argument("--cpu-substeps", type=int, default=16)
+    ap.add_argument("--cpu-substeps", type=int, default=1)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--profile-frames", type=int, default=1)
+    ap.add_argument("--profile-substeps", type=int, default=2)
+    ap.add_argument("--residual", type=float, default=1e-6,
+                    help="relative Newton residual target; <= 0 runs numItr sweeps")
+    ap.add_argument("--max-sweeps", type=int, default=6000)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the stepper runs on a single MI355X "
@@ -101,6 +107,8 @@ def main():
 
     sim = Sim(sc.n)
     scenes.load_into(sim, sc)
+    if args.residual > 0:
+        sim.setSolver(args.residual, maxSweeps=args.max_sweeps, chunk=16, checkEvery=4)
     frame = np.float32(sc.frame_dt)
     substeps_per_frame = int(np.ceil(frame / np.float32(sc.maxH)))
 
@@ -108,10 +116,18 @@ def main():
         sim.advance(frame)
         print(f"[bench] warmup frame {w} done", file=sys.stderr, flush=True)
 
+    if args.residual > 0:
+        sim.solverStats(reset=True)
     t0 = time.perf_counter()
     for k in range(args.steps):
         sim.advance(frame)  # synchronous: returns after the stream has drained
+        print(f"[bench] frame {k} done", file=sys.stderr, flush=True)
     t1 = time.perf_counter()
+    stats = sim.solverStats() if args.residual > 0 else None
+    if stats:
+        print(f"[bench] solver: {stats['sweeps'] / max(stats['substeps'], 1):.1f} sweeps/substep, "
+              f"max {stats['maxSweepsUsed']}, not converged {stats['notConverged']}, "
+              f"max final residual {stats['maxResidual']:.2e}", file=sys.stderr, flush=True)
     elapsed = t1 - t0
     vsub = sc.n * substeps_per_frame * args.steps
     value = vsub / elapsed
@@ -122,13 +138,17 @@ def main():
     nl = np.zeros(len(STAGES), np.int32)
     p = sim.meta
     check(sim._lib.yb_profile(sim.handle, ctypes.byref(p), np.float32(frame / substeps_per_frame),
-                              substeps_per_frame * args.profile_frames, ptr(ms), ptr(nl)))
+                              args.profile_substeps, ptr(ms), ptr(nl)))
     counts = sim.collisionCounts()
     segs = (sim.verts["flags"] & 2) != 0
     c = float(counts[segs].mean()) if segs.any() else 0.0
     per_launch = {s: (ms[i] / nl[i] if nl[i] else 0.0) for i, s in enumerate(STAGES)}
     total = {s: ms[i] for i, s in enumerate(STAGES)}
     dominant = max(total, key=total.get)
+    if stage_bytes(dominant, 0.0) is None:
+        # roofline is stated for the dominant kernel with an algorithmic byte model
+        dominant = max((s for s in STAGES if stage_bytes(s, 0.0) is not None and nl[STAGES.index(s)]),
+                       key=total.get)
     for s in STAGES:
         i = STAGES.index(s)
         if nl[i]:
@@ -154,11 +174,12 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        v, used, dt = cpu_baseline(sc, args.cpu_substeps, args.cpu_threads)
+        v, used, dt = cpu_baseline(sc, args.cpu_substeps, args.cpu_threads, args.residual,
+                                   args.max_sweeps)
         cpu = {"value": round(v, 1), "unit": "vertex-substeps/s", "cores": int(used),
                "kind": "port",
-               "sample": f"FP64 OpenMP oracle, {args.cpu_substeps} substeps of {sc.name} "
-                         f"({dt:.1f}s)"}
+               "sample": f"FP64 OpenMP oracle, {args.cpu_substeps} substeps of {sc.name}, "
+                         f"same solver schedule ({dt:.1f}s)"}
         print(f"[bench] cpu baseline {v:.3e} vertex-substeps/s on {used} threads",
               file=sys.stderr)
 
@@ -179,7 +200,11 @@ def main():
             "workload": sc.name,
             "vertices": sc.n,
             "substeps_per_step": substeps_per_frame,
-            "numItr": int(sim.meta.numItr),
+            "numItr": int(sim.meta.numItr) if args.residual <= 0 else None,
+            "residual_tol": args.residual if args.residual > 0 else None,
+            "mean_sweeps_per_substep": (round(stats["sweeps"] / max(stats["substeps"], 1), 1)
+                                        if stats else int(sim.meta.numItr)),
+            "not_converged_substeps": stats["notConverged"] if stats else None,
             "method": "plain",
             "detection_period": int(sim.meta.detectionPeriod),
         },

@@ -198,7 +198,7 @@ typedef struct yb_solver_stats_t {
     float lastResidual;
     float maxResidual;    /* largest final residual of a substep */
 } yb_solver_stats_t;
-int yb_set_solver(yb_sim* s, float residualTol, int maxSweeps, int chunk);
+int yb_set_solver(yb_sim* s, float residualTol, int maxSweeps, int chunk, int checkEvery);
 int yb_solver_stats(yb_sim* s, yb_solver_stats_t* out, int reset);
 
 /* ---- live per-stage timing (measurement harness) --------------------------
@@ -215,8 +215,8 @@ enum {
     YB_STAGE_REFIT = 3,       /* LBVH refit */
     YB_STAGE_QUERY = 4,       /* LBVH query + collision.cu:39-92 */
     YB_STAGE_STEP_LIMIT = 5,  /* collision.cu:121-159 */
-    YB_STAGE_SWEEP_DX = 6,    /* cosserat.cu:20-207 */
-    YB_STAGE_SWEEP_Q = 7,     /* cosserat.cu:209-253 */
+    YB_STAGE_SWEEP = 6,       /* cosserat.cu:20-253 (dx step + frame update, fused) */
+    YB_STAGE_UNUSED = 7,
     YB_STAGE_END = 8,         /* iteration.cu:48-61 */
     YB_STAGE_RESIDUAL = 9,    /* residual reduction / solver control */
     YB_NUM_STAGES = 10

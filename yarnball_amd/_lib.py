@@ -115,12 +115,12 @@ SIGNATURES = {
     "yb_debug_read_last_vels": (_I, [_P, _P]),
     "yb_debug_write_last_vels": (_I, [_P, _P]),
     "yb_profile": (_I, [_P, ctypes.POINTER(Params), _F, _I, _P, _P]),
-    "yb_set_solver": (_I, [_P, _F, _I, _I]),
+    "yb_set_solver": (_I, [_P, _F, _I, _I, _I]),
     "yb_min_separation": (_I, [_P, _P, _I]),
     "yb_solver_stats": (_I, [_P, ctypes.POINTER(SolverStats), _I]),
 }
 
-STAGES = ["init", "aabb", "rebuild", "refit", "query", "step_limit", "sweep_dx", "sweep_q", "end",
+STAGES = ["init", "aabb", "rebuild", "refit", "query", "step_limit", "sweep", "unused", "end",
           "residual"]
 
 _lib = None

@@ -344,15 +344,20 @@ static void enqueue_prologue(yb_sim* s, int mode, bool adaptive) {
     }
 }
 
+// In residual-targeted mode the residual is evaluated after every
+// s->checkEvery-th sweep (and after the last allowed one).
+static bool checks_residual(const yb_sim* s, int k) {
+    return (k + 1) % s->checkEvery == 0 || k == s->maxSweeps - 1;
+}
+
 static void enqueue_sweep(yb_sim* s, int k, bool adaptive) {
-    launch_sweep_dx(s, k, adaptive);
-    mark(s, YB_STAGE_SWEEP_DX);
-    if (adaptive) {
+    const bool check = adaptive && checks_residual(s, k);
+    launch_sweep(s, k, adaptive, check);
+    mark(s, YB_STAGE_SWEEP);
+    if (check) {
         launch_residual(s, k);
         mark(s, YB_STAGE_RESIDUAL);
     }
-    launch_sweep_q(s, k, adaptive);
-    mark(s, YB_STAGE_SWEEP_Q);
 }
 
 static void enqueue_end(yb_sim* s, int fin, bool adaptive) {
@@ -422,17 +427,20 @@ static int rebuild_adaptive_graphs(yb_sim* s) {
     return YB_OK;
 }
 
-extern "C" int yb_set_solver(yb_sim* s, float residualTol, int maxSweeps, int chunk) {
+extern "C" int yb_set_solver(yb_sim* s, float residualTol, int maxSweeps, int chunk,
+                             int checkEvery) {
     if (!s) return YB_ERR_INVALID_ARGUMENT;
-    if (residualTol > 0 && (maxSweeps < 1 || chunk < 1)) {
-        set_message("yb_set_solver: maxSweeps and chunk must be >= 1");
+    if (residualTol > 0 && (maxSweeps < 1 || chunk < 1 || checkEvery < 1)) {
+        set_message("yb_set_solver: maxSweeps, chunk and checkEvery must be >= 1");
         return YB_ERR_INVALID_ARGUMENT;
     }
-    if (residualTol != s->tol || maxSweeps != s->maxSweeps || chunk != s->chunk)
+    if (residualTol != s->tol || maxSweeps != s->maxSweeps || chunk != s->chunk ||
+        checkEvery != s->checkEvery)
         s->adaptiveReady = false;
     s->tol = residualTol;
     s->maxSweeps = maxSweeps;
     s->chunk = chunk;
+    s->checkEvery = checkEvery;
     return YB_OK;
 }
 
@@ -758,6 +766,7 @@ extern "C" int yb_profile(yb_sim* s, yb_params* p, float h, int substeps, double
             for (int k = 0; k < s->maxSweeps; k++) {
                 mark(s, -1);
                 enqueue_sweep(s, k, true);
+                if (!checks_residual(s, k)) continue;
                 int32_t done = 0;
                 HIPCHECK(hipMemcpyAsync(&done, &s->b.ctl->doneAt, 4, hipMemcpyDeviceToHost,
                                         s->stream));

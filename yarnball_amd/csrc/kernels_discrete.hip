@@ -396,40 +396,22 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_local(
     __shared__ float4 lbox[kRefitLeaves][2];
     __shared__ float4 nbox[kRefitLeaves][2];
     __shared__ int32_t cnt[kRefitLeaves];
-    __shared__ float red[6][kRefitThreads];
     const int t = threadIdx.x;
     const int base = blockIdx.x * kRefitLeaves;
     const int end = min(base + kRefitLeaves, n);
-    float4 blo = make_float4(INFINITY, INFINITY, INFINITY, 0.f);
-    float4 bhi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.f);
+    const int cntLeaves = end - base;
     for (int r = 0; r < kRefitPer; r++) {
         const int k = base + t + kRefitThreads * r;
         cnt[t + kRefitThreads * r] = 0;
         if (k < end) {
             const int seg = order[k];
-            const float4 a = aabb[2 * seg];
-            const float4 b = aabb[2 * seg + 1];
-            lbox[k - base][0] = a;
-            lbox[k - base][1] = b;
-            box_union(blo, bhi, a, b);
+            lbox[k - base][0] = aabb[2 * seg];
+            lbox[k - base][1] = aabb[2 * seg + 1];
         }
     }
-    red[0][t] = blo.x; red[1][t] = blo.y; red[2][t] = blo.z;
-    red[3][t] = bhi.x; red[4][t] = bhi.y; red[5][t] = bhi.z;
     __syncthreads();
-    for (int st = kRefitThreads / 2; st > 0; st >>= 1) {
-        if (t < st) {
-            for (int c = 0; c < 3; c++) red[c][t] = fminf(red[c][t], red[c][t + st]);
-            for (int c = 3; c < 6; c++) red[c][t] = fmaxf(red[c][t], red[c][t + st]);
-        }
-        __syncthreads();
-    }
-    if (t == 0) {
-        blkBox[2 * blockIdx.x] = make_float4(red[0][0], red[1][0], red[2][0], 0.f);
-        blkBox[2 * blockIdx.x + 1] = make_float4(red[3][0], red[4][0], red[5][0], 0.f);
-    }
 
-    // Climb.
+    // Climb through the nodes whose range lies in this block, in LDS only.
     for (int r = 0; r < kRefitPer; r++) {
         const int k = base + t + kRefitThreads * r;
         if (k >= end) continue;
@@ -452,45 +434,89 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_local(
             if (old == 0) break;
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
             const int2 kc = kids[p];
-            float4 lo0, hi0, lo1, hi1;
-            if (kc.x < 0) { lo0 = lbox[~kc.x - base][0]; hi0 = lbox[~kc.x - base][1]; }
-            else { lo0 = nbox[kc.x - base][0]; hi0 = nbox[kc.x - base][1]; }
+            float4 lo1, hi1;
+            if (kc.x < 0) { lo = lbox[~kc.x - base][0]; hi = lbox[~kc.x - base][1]; }
+            else { lo = nbox[kc.x - base][0]; hi = nbox[kc.x - base][1]; }
             if (kc.y < 0) { lo1 = lbox[~kc.y - base][0]; hi1 = lbox[~kc.y - base][1]; }
             else { lo1 = nbox[kc.y - base][0]; hi1 = nbox[kc.y - base][1]; }
-            store_xyz(&nodes[4 * p + 0], lo0);
-            nodes[4 * p + 1] = make_float4(hi0.x, hi0.y, hi0.z, kc.x < 0 ? hi0.w : 0.f);
-            store_xyz(&nodes[4 * p + 2], lo1);
-            nodes[4 * p + 3] = make_float4(hi1.x, hi1.y, hi1.z, kc.y < 0 ? hi1.w : 0.f);
-            lo = lo0;
-            hi = hi0;
             box_union(lo, hi, lo1, hi1);
             nbox[p - base][0] = lo;
             nbox[p - base][1] = hi;
-            if (p == 0) {
-                sceneBox[0] = lo;
-                sceneBox[1] = hi;
-            }
             leaf = false;
             slot = parent[p];
         }
     }
+    __syncthreads();
 
-    // Suffix / prefix unions at the ends of straddling ranges.
+    // Child slots of every node completed in this block (coalesced pass).
+    for (int r = 0; r < kRefitPer; r++) {
+        const int p = base + t + kRefitThreads * r;
+        if (p >= end || p >= n - 1) continue;
+        const int2 pr = rng[p];
+        if (pr.x < base || pr.y >= end) continue;
+        const int2 kc = kids[p];
+        float4 lo0, hi0, lo1, hi1;
+        if (kc.x < 0) { lo0 = lbox[~kc.x - base][0]; hi0 = lbox[~kc.x - base][1]; }
+        else { lo0 = nbox[kc.x - base][0]; hi0 = nbox[kc.x - base][1]; }
+        if (kc.y < 0) { lo1 = lbox[~kc.y - base][0]; hi1 = lbox[~kc.y - base][1]; }
+        else { lo1 = nbox[kc.y - base][0]; hi1 = nbox[kc.y - base][1]; }
+        store_xyz(&nodes[4 * p + 0], lo0);
+        nodes[4 * p + 1] = make_float4(hi0.x, hi0.y, hi0.z, kc.x < 0 ? hi0.w : 0.f);
+        store_xyz(&nodes[4 * p + 2], lo1);
+        nodes[4 * p + 3] = make_float4(hi1.x, hi1.y, hi1.z, kc.y < 0 ? hi1.w : 0.f);
+        if (p == 0) {  // the whole tree fits in this block
+            box_union(lo0, hi0, lo1, hi1);
+            sceneBox[0] = lo0;
+            sceneBox[1] = hi0;
+        }
+    }
+
+    // Inclusive prefix and suffix unions over the block's leaves (Hillis-Steele
+    // in LDS, reusing nbox for the suffix), for the ends of straddling ranges.
+    for (int off = 1; off < kRefitLeaves; off <<= 1) {
+        float4 pl[kRefitPer][2], sl[kRefitPer][2];
+        for (int r = 0; r < kRefitPer; r++) {
+            const int m = t + kRefitThreads * r;
+            pl[r][0] = lbox[m][0];
+            pl[r][1] = lbox[m][1];
+            if (off == 1) {
+                sl[r][0] = lbox[m][0];
+                sl[r][1] = lbox[m][1];
+            } else {
+                sl[r][0] = nbox[m][0];
+                sl[r][1] = nbox[m][1];
+            }
+            if (m - off >= 0 && m < cntLeaves) box_union(pl[r][0], pl[r][1], lbox[m - off][0], lbox[m - off][1]);
+            const float4* src = (off == 1) ? &lbox[0][0] : &nbox[0][0];
+            if (m + off < cntLeaves && m < cntLeaves)
+                box_union(sl[r][0], sl[r][1], src[2 * (m + off)], src[2 * (m + off) + 1]);
+        }
+        __syncthreads();
+        for (int r = 0; r < kRefitPer; r++) {
+            const int m = t + kRefitThreads * r;
+            lbox[m][0] = pl[r][0];
+            lbox[m][1] = pl[r][1];
+            nbox[m][0] = sl[r][0];
+            nbox[m][1] = sl[r][1];
+        }
+        __syncthreads();
+    }
+    // lbox now holds prefix unions, nbox suffix unions.
     for (int r = 0; r < kRefitPer; r++) {
         const int k = base + t + kRefitThreads * r;
         if (k >= end) continue;
         const int need = leafNeed[k];
         if (need & 1) {
-            float4 lo = lbox[k - base][0], hi = lbox[k - base][1];
-            for (int m = k + 1; m < end; m++) box_union(lo, hi, lbox[m - base][0], lbox[m - base][1]);
-            sufBox[2 * k] = lo;
-            sufBox[2 * k + 1] = hi;
+            sufBox[2 * k] = nbox[k - base][0];
+            sufBox[2 * k + 1] = nbox[k - base][1];
         }
         if (need & 2) {
-            float4 lo = lbox[k - base][0], hi = lbox[k - base][1];
-            for (int m = base; m < k; m++) box_union(lo, hi, lbox[m - base][0], lbox[m - base][1]);
-            preBox[2 * k] = lo;
-            preBox[2 * k + 1] = hi;
+            preBox[2 * k] = lbox[k - base][0];
+            preBox[2 * k + 1] = lbox[k - base][1];
+        }
+        if (k == base) {  // block union = prefix at the last leaf
+            blkBox[2 * blockIdx.x] = lbox[cntLeaves - 1][0];
+            blkBox[2 * blockIdx.x + 1] = lbox[cntLeaves - 1][1];
         }
     }
 }
@@ -553,8 +579,7 @@ __global__ __launch_bounds__(256) void k_refit_top(int n, int nb,
 }
 
 // ---------------------------------------------------------------------------
-// Query + narrowphase (collision.cu:39-92).  One thread per segment, taken in
-// Morton order so that neighbouring lanes walk similar paths.  Each overlapping
+// Query + narrowphase (collision.cu:39-92).  One thread per segment.  Each overlapping
 // leaf j is filtered as the unordered pair (x, y) = (min, max): glued
 // endpoints and |x - y| <= 2 are exempt, and the pair is kept when the closest
 // distance between the segments is below 2 * scaledDetectionRadius plus both
@@ -616,9 +641,11 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
                                                          int32_t* __restrict__ err) {
     __shared__ int32_t stk[kStack][kQueryThreads];
     const int n = P->n;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const int i = order[k];
+    // Segments in index order: consecutive segments of a strand are spatial
+    // neighbours (coherent traversals) and the slot-major list writes
+    // cols[i + n * slot] coalesce across the wave.
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
     if (!(__float_as_uint(prm[i].w) & YB_FLAG_HAS_NEXT)) {
         numCols[i] = 0;
         return;

@@ -1,22 +1,18 @@
 // kernels_sweep.hip — per-substep solver kernels of the stepper on gfx950.
 //
 //   k_step_limit   per-segment no-tunnelling step bound   (collision.cu:121-159)
-//   k_sweep_dx     one block-Jacobi Newton sweep on dx    (cosserat.cu:20-207)
-//   k_sweep_q      closed-form Stable-Cosserat frame update (cosserat.cu:209-253)
+//   k_sweep        one block-Jacobi sweep: Newton step on dx (cosserat.cu:20-207)
+//                  fused with the closed-form frame update (cosserat.cu:209-253)
 //   k_end          velocity update + advection             (iteration.cu:48-61)
 //   k_upload / k_download   AoS <-> SoA transposes        (YarnBall.cu:239-261)
 //
 // Schedule: strict Jacobi.  Sweep k reads dx[k&1], q[k&1] and writes dx[~k&1];
-// the frame update then reads the new dx and the old frames and writes
-// q[~k&1].  The reference updates both arrays in place (asynchronous across
-// blocks); the double buffers make every iterate reproducible.
+// the frame update uses the new dx and the old frames and writes q[~k&1].  The
+// reference updates both arrays in place (asynchronous across blocks); the
+// double buffers make every iterate reproducible.
 //
-// Wave mapping of k_sweep_dx: each 64-lane wavefront owns 63 consecutive
-// vertices.  Lane l works on vertex base + l - 1: it evaluates its own segment
-// (i, i+1) -- stretch-shear plus all contacts of that segment -- and hands the
-// endpoint-1 force/Hessian to lane l+1 with a cross-lane shift.  Lane 0 is the
-// halo that only feeds lane 1.  This replaces the reference's 2-sector LDS
-// block reduction with register-only wavefront exchange.
+// This translation unit holds only continuous (tolerance-compared) arithmetic
+// and is built with fast single-precision divide / sqrt.
 #include "yb_internal.h"
 #include "ybmath.h"
 
@@ -37,8 +33,10 @@ __device__ __forceinline__ float4 f4r(rot a) { return make_float4(a.x, a.y, a.z,
 __device__ __forceinline__ void reduce_min_sep(float sep, float* minSep) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) sep = fminf(sep, __shfl_xor(sep, off, 64));
-    // positive floats order like their bit patterns
-    if ((threadIdx.x & 63) == 0 && sep < INFINITY)
+    // Positive floats order like their bit patterns.  Only a wave that beats
+    // the running minimum issues the atomic, so after the first substep almost
+    // none do.
+    if ((threadIdx.x & 63) == 0 && sep < INFINITY && sep < *(volatile float*)minSep)
         atomicMin(reinterpret_cast<int*>(minSep), __float_as_int(sep));
 }
 
@@ -83,28 +81,60 @@ __global__ __launch_bounds__(256) void k_step_limit(const DevParams* __restrict_
 
 __device__ __forceinline__ void block_residual(float num, double* partial);
 
+// Closest points of segment e = (0 -> e) and (B0 -> B1), specialised from
+// closest_uv (Common.h:318-334) with the segment's |e|^2 and 1/|e|^2 hoisted
+// out of the contact loop.
+__device__ __forceinline__ void closest_e(v3 e, float a, float inva, v3 B0, v3 B1, float& u,
+                                         float& w) {
+    const v3 A1 = B1 - B0;
+    const float b = dot(e, A1);
+    const float c = len2(A1);
+    float vy = b * dot(e, B0) - a * dot(A1, B0);
+    vy = vy / (a * c - b * b);
+    if (!isfinite(vy)) vy = 0.5f;
+    vy = clampf(vy, 0.f, 1.f);
+    const float ux = clampf(dot(mixv(B0, B1, vy), e) * inva, 0.f, 1.f);
+    vy = clampf(dot(e * ux - B0, A1) / c, 0.f, 1.f);
+    u = ux;
+    w = vy;
+}
+
 // ---------------------------------------------------------------------------
-// One Newton sweep (cosserat.cu:20-207).  For vertex i:
-//   H = m/h^2 I + glue + stretch(seg i-1, seg i) + barrier/friction(contacts)
-//   f = m/h^2 ((y-x) - dx) + ...
-//   dx += accelerationRatio * H^{-1} f, clamped to the step limit.
-__global__ __launch_bounds__(256) void k_sweep_dx(const DevParams* __restrict__ P,
-                                                  const float4* __restrict__ pos,
-                                                  const float4* __restrict__ prm,
-                                                  const float4* __restrict__ yv,
-                                                  const float4* __restrict__ q,
-                                                  const float4* __restrict__ dxIn,
-                                                  float4* __restrict__ dxOut,
-                                                  const int32_t* __restrict__ numCols,
-                                                  const int32_t* __restrict__ cols,
-                                                  const SolverCtl* __restrict__ ctl, int sweep,
-                                                  double* __restrict__ partial) {
+// One block-Jacobi sweep: Newton step on dx (cosserat.cu:20-207) followed by
+// the closed-form frame update (cosserat.cu:209-253), fused.
+//
+// Vertex i:  H = m/h^2 I + glue + stretch(seg i-1, seg i) + barrier/friction
+//            f = m/h^2 ((y-x) - dx) + ...
+//            dx' = dx + accelerationRatio H^{-1} f, clamped to the step limit.
+// Segment i: v = -2 kStretch e'/l with e' from dx', b = neighbour frames
+//            rotated by the rest Darboux rotors (sign-corrected), then
+//            q' = normalize((v b e1) + (|v| + |b|) b).
+//
+// Wave tile: lane l works on vertex 62 w + l - 1.  Lanes 1..62 own their
+// vertex (write dx' and q').  Lane 0 is a halo that only hands its segment's
+// endpoint-1 terms and its frame to lane 1.  Lane 63 computes dx' of the next
+// tile's first vertex without writing it, so that lane 62's frame update has
+// dx'(i+1); it also supplies q(i+1).  All exchange is cross-lane shifts --
+// no LDS, no barrier (the reference uses a 2-sector LDS reduction).
+__global__ __launch_bounds__(256) void k_sweep(const DevParams* __restrict__ P,
+                                               const float4* __restrict__ pos,
+                                               const float4* __restrict__ prm,
+                                               const float4* __restrict__ yv,
+                                               const float4* __restrict__ qIn,
+                                               float4* __restrict__ qOut,
+                                               const float4* __restrict__ qRest,
+                                               const float4* __restrict__ dxIn,
+                                               float4* __restrict__ dxOut,
+                                               const int32_t* __restrict__ numCols,
+                                               const int32_t* __restrict__ cols,
+                                               const SolverCtl* __restrict__ ctl, int sweep,
+                                               double* __restrict__ partial) {
     // Residual-targeted mode: sweeps after convergence do nothing.
     if (ctl && ctl->doneAt < sweep) return;
     const int n = P->n;
     const int lane = threadIdx.x & 63;
     const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-    const int v = wave * 63 + lane - 1;
+    const int v = wave * 62 + lane - 1;
     const bool in = (v >= 0) && (v < n);
 
     const float h = P->h;
@@ -116,13 +146,19 @@ __global__ __launch_bounds__(256) void k_sweep_dx(const DevParams* __restrict__ 
     sym3 H2 = sym_zero();
     float segLimit = INFINITY;
     float4 p4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 r4 = make_float4(1.f, 0.f, 0.f, 0.f);
     float4 d4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 q4 = make_float4(0.f, 0.f, 0.f, 1.f);
+    float4 qr4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    v3 p1 = v3{0.f, 0.f, 0.f};
     uint32_t flags = 0;
 
     if (in) {
         p4 = pos[v];
-        const float4 r4 = prm[v];
+        r4 = prm[v];
         d4 = dxIn[v];
+        q4 = qIn[v];
+        qr4 = qRest[v];
         flags = __float_as_uint(r4.w);
         const int cid = __float_as_int(r4.z);
         const v3 x0 = xyz(p4);
@@ -136,21 +172,21 @@ __global__ __launch_bounds__(256) void k_sweep_dx(const DevParams* __restrict__ 
         // Glue spring to a connected vertex (cosserat.cu:47-53).
         if (cid >= 0) {
             const float stiffness = 40.0f;
-            v3 pc = xyz(pos[cid]);
-            v3 dc = xyz(dxIn[cid]);
+            const v3 pc = xyz(pos[cid]);
+            const v3 dc = xyz(dxIn[cid]);
             f -= stiffness * (((x0 - pc) + (dx - dc)) + damping * dx);
             sym_add_diag(H, stiffness * (1.0f + damping));
         }
 
         if (flags & YB_FLAG_HAS_NEXT) {
-            const v3 p1 = xyz(pos[v + 1]);
+            p1 = xyz(pos[v + 1]);
             const v3 p1dx = xyz(dxIn[v + 1]);
             segLimit = d4.w;
 
             // Cosserat stretch-shear (cosserat.cu:67-81).
             const float invl = 1.0f / r4.x;
             const v3 e = (p1 - x0) + (p1dx - dx);
-            const v3 c = e * invl - rrotate_e1(asrot(q[v]));
+            const v3 c = e * invl - rrotate_e1(asrot(q4));
             const float k = r4.y * invl;
             float d = k * invl;
             f += k * c - (damping * d) * dx;
@@ -161,65 +197,71 @@ __global__ __launch_bounds__(256) void k_sweep_dx(const DevParams* __restrict__ 
 
             // IPC barrier + lagged friction over this segment's contacts
             // (cosserat.cu:83-148).
-            const float fricK = P->kFriction;
-            const float invb = 1.0f / P->barrierThickness;
-            const float radius2 = 2.0f * P->radius;
-            const float fricMu = P->frictionCoeff;
-            const float kCol = P->kCollision * invb;
             const int nc = numCols[v];
-            for (int i = 0; i < nc; i++) {
-                const int o = cols[v + n * i];
-                const v3 b0 = xyz(pos[o]);
-                const v3 b1 = xyz(pos[o + 1]);
-                const v3 db0 = xyz(dxIn[o]);
-                const v3 db1 = xyz(dxIn[o + 1]);
-                float u, w;
-                closest_uv(v3{0.f, 0.f, 0.f}, e, (b0 - x0) + (db0 - dx), (b1 - x0) + (db1 - dx), u,
-                           w);
-                const v3 dpos = mixv(x0, p1, u) - mixv(b0, b1, w);
-                const v3 ddpos = mixv(dx, p1dx, u) - mixv(db0, db1, w);
-                v3 nrm = dpos + ddpos;
-                float dist = len(nrm);
-                nrm *= 1.0f / dist;
-                const float w1 = u;
-                const float w0 = 1.0f - u;
+            if (nc > 0) {
+                const float fricK = P->kFriction;
+                const float invb = 1.0f / P->barrierThickness;
+                const float radius2 = 2.0f * P->radius;
+                const float fricMu = P->frictionCoeff;
+                const float kCol = P->kCollision * invb;
+                const float ea = len2(e);
+                const float inva = 1.0f / ea;
+                int o = cols[v];
+                for (int i = 0; i < nc; i++) {
+                    // next partner id fetched one iteration ahead
+                    const int oNext = (i + 1 < nc) ? cols[v + n * (i + 1)] : o;
+                    const v3 b0 = xyz(pos[o]);
+                    const v3 b1 = xyz(pos[o + 1]);
+                    const v3 db0 = xyz(dxIn[o]);
+                    const v3 db1 = xyz(dxIn[o + 1]);
+                    o = oNext;
+                    float u, w;
+                    closest_e(e, ea, inva, (b0 - x0) + (db0 - dx), (b1 - x0) + (db1 - dx), u, w);
+                    const v3 dpos = mixv(x0, p1, u) - mixv(b0, b1, w);
+                    const v3 ddpos = mixv(dx, p1dx, u) - mixv(db0, db1, w);
+                    v3 nrm = dpos + ddpos;
+                    const float dist = len(nrm);
+                    nrm *= 1.0f / dist;
+                    const float w1 = u;
+                    const float w0 = 1.0f - u;
 
-                float dd = (dist - radius2) * invb;
-                if (dd > 1.0f) continue;  // outside the barrier
-                dd = fmaxf(dd, 1e-3f);
+                    float dd = (dist - radius2) * invb;
+                    if (dd > 1.0f) continue;  // outside the barrier
+                    dd = fmaxf(dd, 1e-3f);
 
-                // b(d) = -(d-1)^2 ln d; ff = -kCol b'(d), dH = kCol/b b''(d)
-                const float invd = 1.0f / dd;
-                const float logd = logf(dd);
-                float dH = (-3.0f + (2.0f + invd) * invd - 2.0f * logd) * kCol * invb;
-                const float ff = -(1.0f - dd) * (dd - 1.0f + 2.0f * dd * logd) * invd * kCol;
-                f += (ff * w0 - damping * dH * w0 * w0 * dot(nrm, dx)) * nrm;
-                f2 += (ff * w1 - damping * dH * w1 * w1 * dot(nrm, p1dx)) * nrm;
-                dH *= 1.0f + damping;
-                const sym3 op = sym_outer(nrm);
-                sym_axpy(H, op, dH * w0 * w0);
-                sym_axpy(H2, op, dH * w1 * w1);
+                    // b(d) = -(d-1)^2 ln d; ff = -kCol b'(d), dH = kCol/b b''(d)
+                    const float invd = 1.0f / dd;
+                    const float logd = __logf(dd);
+                    float dH = (-3.0f + (2.0f + invd) * invd - 2.0f * logd) * kCol * invb;
+                    const float ff = -(1.0f - dd) * (dd - 1.0f + 2.0f * dd * logd) * invd * kCol;
+                    f += (ff * w0 - damping * dH * w0 * w0 * dot(nrm, dx)) * nrm;
+                    f2 += (ff * w1 - damping * dH * w1 * w1 * dot(nrm, p1dx)) * nrm;
+                    dH *= 1.0f + damping;
+                    const sym3 op = sym_outer(nrm);
+                    sym_axpy(H, op, dH * w0 * w0);
+                    sym_axpy(H2, op, dH * w1 * w1);
 
-                // Lagged friction on the tangential relative displacement.
-                const v3 uu = ddpos - dot(nrm, ddpos) * nrm;
-                const float ul = len(uu);
-                if (ul > 0.0f) {
-                    const float f1 = fminf(fricK, fricMu * ff / ul);
-                    const float s0 = w0 * w0 * f1;
-                    const float s1 = w1 * w1 * f1;
-                    f -= (f1 * w0) * uu;
-                    f2 -= (f1 * w1) * uu;
-                    // H -= (nn^T - I) s
-                    sym_axpy(H, op, -s0);
-                    sym_add_diag(H, s0);
-                    sym_axpy(H2, op, -s1);
-                    sym_add_diag(H2, s1);
+                    // Lagged friction on the tangential relative displacement.
+                    const v3 uu = ddpos - dot(nrm, ddpos) * nrm;
+                    const float ul = len(uu);
+                    if (ul > 0.0f) {
+                        const float f1 = fminf(fricK, fricMu * ff / ul);
+                        const float s0 = w0 * w0 * f1;
+                        const float s1 = w1 * w1 * f1;
+                        f -= (f1 * w0) * uu;
+                        f2 -= (f1 * w1) * uu;
+                        // H -= (nn^T - I) s
+                        sym_axpy(H, op, -s0);
+                        sym_add_diag(H, s0);
+                        sym_axpy(H2, op, -s1);
+                        sym_add_diag(H2, s1);
+                    }
                 }
             }
         }
     }
 
-    // Hand segment (i-1)'s endpoint-1 terms to vertex i (cosserat.cu:175-193).
+    // Segment (i-1)'s endpoint-1 terms, frame and rest rotor (cosserat.cu:175-193).
     const float pf0 = __shfl_up(f2.x, 1, 64);
     const float pf1 = __shfl_up(f2.y, 1, 64);
     const float pf2 = __shfl_up(f2.z, 1, 64);
@@ -227,8 +269,14 @@ __global__ __launch_bounds__(256) void k_sweep_dx(const DevParams* __restrict__ 
 #pragma unroll
     for (int k = 0; k < 6; k++) ph[k] = __shfl_up(H2.a[k], 1, 64);
     const float pl = __shfl_up(segLimit, 1, 64);
+    const rot qPrev = rot{__shfl_up(q4.x, 1, 64), __shfl_up(q4.y, 1, 64), __shfl_up(q4.z, 1, 64),
+                          __shfl_up(q4.w, 1, 64)};
+    const rot qrPrev = rot{__shfl_up(qr4.x, 1, 64), __shfl_up(qr4.y, 1, 64),
+                           __shfl_up(qr4.z, 1, 64), __shfl_up(qr4.w, 1, 64)};
 
-    float num = 0.0f;  // |dx_new - dx_old|^2
+    // Newton step.
+    v3 dxNew = xyz(d4);
+    float num = 0.0f;  // |dx' - dx|^2 for the residual
     if (in && lane != 0) {
         float limit = segLimit;
         if (flags & YB_FLAG_HAS_PREV) {
@@ -237,19 +285,46 @@ __global__ __launch_bounds__(256) void k_sweep_dx(const DevParams* __restrict__ 
 #pragma unroll
             for (int k = 0; k < 6; k++) H.a[k] += ph[k];
         }
-        float4 out = d4;
         if (p4.w != 0.0f) {
             // Local 3x3 solve (cosserat.cu:195-205).
-            v3 dx = xyz(d4);
-            dx += P->accelerationRatio * sym_solve(H, f);
-            const float l = len(dx);
-            if (l > limit && l > 0.0f) dx *= limit / l;
-            out = f4(dx, d4.w);
-            num = len2(dx - xyz(d4));
+            dxNew += P->accelerationRatio * sym_solve(H, f);
+            const float l = len(dxNew);
+            if (l > limit && l > 0.0f) dxNew *= limit / l;
+            num = len2(dxNew - xyz(d4));
         }
-        dxOut[v] = out;
     }
-    if (partial) block_residual(num, partial);
+
+    // Next vertex's new dx and the next segment's frame.
+    const v3 dxNext = v3{__shfl_down(dxNew.x, 1, 64), __shfl_down(dxNew.y, 1, 64),
+                         __shfl_down(dxNew.z, 1, 64)};
+    const rot qNext = rot{__shfl_down(q4.x, 1, 64), __shfl_down(q4.y, 1, 64),
+                          __shfl_down(q4.z, 1, 64), __shfl_down(q4.w, 1, 64)};
+
+    const bool owner = in && lane >= 1 && lane <= 62;
+    if (owner) {
+        dxOut[v] = f4(dxNew, d4.w);
+        rot q0 = asrot(q4);
+        if (!(flags & YB_FLAG_FIX_ORIENTATION) && (flags & YB_FLAG_HAS_NEXT)) {
+            const v3 e = (p1 - xyz(p4)) + (dxNext - dxNew);
+            v3 vv = v3{e.x / r4.x, e.y / r4.x, e.z / r4.x};
+            vv *= -2.0f * r4.y;
+            rot b = rot{0.f, 0.f, 0.f, 0.f};
+            if (flags & YB_FLAG_HAS_PREV) {
+                const float sg = rdot(rmul(rinv(qPrev), q0), qrPrev) > 0.0f ? 1.0f : -1.0f;
+                b = radd(b, rscale(rmul(qPrev, qrPrev), sg));
+            }
+            if (flags & YB_FLAG_HAS_NEXT_ORIENTATION) {
+                const rot qr = asrot(qr4);
+                const float sg = rdot(rmul(rinv(q0), qNext), qr) > 0.0f ? 1.0f : -1.0f;
+                b = radd(b, rscale(rmul(qNext, rinv(qr)), sg));
+            }
+            const float lambda = len(vv) + rlen(b);
+            const rot t = rmul(rmul(mkrot(vv, 0.0f), b), rot{1.f, 0.f, 0.f, 0.f});
+            q0 = rnormalize(radd(t, rscale(b, lambda)));
+        }
+        qOut[v] = f4r(q0);
+    }
+    if (partial) block_residual(owner ? num : 0.0f, partial);
 }
 
 // Per-sweep relative Newton residual
@@ -271,16 +346,16 @@ __device__ __forceinline__ void block_residual(float num, double* partial) {
     }
 }
 
-__global__ __launch_bounds__(256) void k_residual(SolverCtl* __restrict__ ctl,
-                                                  const double* __restrict__ partial, int nb,
-                                                  int sweep, float tol, int maxSweeps) {
+__global__ __launch_bounds__(1024) void k_residual(SolverCtl* __restrict__ ctl,
+                                                   const double* __restrict__ partial, int nb,
+                                                   int sweep, float tol, int maxSweeps) {
     if (ctl->doneAt < sweep) return;
-    __shared__ double sa[256];
+    __shared__ double sa[1024];
     double a = 0.0;
     for (int k = threadIdx.x; k < nb; k += blockDim.x) a += partial[k];
     sa[threadIdx.x] = a;
     __syncthreads();
-    for (int st = 128; st > 0; st >>= 1) {
+    for (int st = 512; st > 0; st >>= 1) {
         if ((int)threadIdx.x < st) sa[threadIdx.x] += sa[threadIdx.x + st];
         __syncthreads();
     }
@@ -302,53 +377,6 @@ __global__ __launch_bounds__(256) void k_residual(SolverCtl* __restrict__ ctl,
 
 __global__ void k_ctl_reset(SolverCtl* __restrict__ ctl) {
     if (threadIdx.x == 0) ctl->doneAt = 0x7fffffff;
-}
-
-// ---------------------------------------------------------------------------
-// Frame update (cosserat.cu:209-253): with v = -2 kStretch e / l and the
-// neighbours' rest-rotated frames b, q = normalize((v b e1) + (|v| + |b|) b).
-__global__ __launch_bounds__(256) void k_sweep_q(const DevParams* __restrict__ P,
-                                                 const float4* __restrict__ pos,
-                                                 const float4* __restrict__ prm,
-                                                 const float4* __restrict__ dxNew,
-                                                 const float4* __restrict__ qIn,
-                                                 float4* __restrict__ qOut,
-                                                 const float4* __restrict__ qRest,
-                                                 const SolverCtl* __restrict__ ctl, int sweep) {
-    if (ctl && ctl->doneAt < sweep) return;
-    const int n = P->n;
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= n) return;
-    const float4 r4 = prm[v];
-    const uint32_t flags = __float_as_uint(r4.w);
-    rot q0 = asrot(qIn[v]);
-    if (!(flags & YB_FLAG_FIX_ORIENTATION) && (flags & YB_FLAG_HAS_NEXT)) {
-        const v3 x0 = xyz(pos[v]);
-        const v3 p1 = xyz(pos[v + 1]);
-        const v3 dx = xyz(dxNew[v]);
-        const v3 p1dx = xyz(dxNew[v + 1]);
-        v3 e = (p1 - x0) + (p1dx - dx);
-        v3 vv = v3{e.x / r4.x, e.y / r4.x, e.z / r4.x};
-        vv *= -2.0f * r4.y;
-
-        rot b = rot{0.f, 0.f, 0.f, 0.f};
-        if (flags & YB_FLAG_HAS_PREV) {
-            const rot qr = asrot(qRest[v - 1]);
-            const rot qq = asrot(qIn[v - 1]);
-            const float s = rdot(rmul(rinv(qq), q0), qr) > 0.0f ? 1.0f : -1.0f;
-            b = radd(b, rscale(rmul(qq, qr), s));
-        }
-        if (flags & YB_FLAG_HAS_NEXT_ORIENTATION) {
-            const rot qr = asrot(qRest[v]);
-            const rot qq = asrot(qIn[v + 1]);
-            const float s = rdot(rmul(rinv(q0), qq), qr) > 0.0f ? 1.0f : -1.0f;
-            b = radd(b, rscale(rmul(qq, rinv(qr)), s));
-        }
-        const float lambda = len(vv) + rlen(b);
-        const rot t = rmul(rmul(mkrot(vv, 0.0f), b), rot{1.f, 0.f, 0.f, 0.f});
-        q0 = rnormalize(radd(t, rscale(b, lambda)));
-    }
-    qOut[v] = f4r(q0);
 }
 
 // ---------------------------------------------------------------------------
@@ -492,28 +520,20 @@ void launch_separation(yb_sim* s) {
                        s->b.err, s->b.minSep);
 }
 
-int sweep_blocks(int n) { return blocks(blocks(n + 1, 63) * 64, 256); }
+int sweep_blocks(int n) { return blocks(blocks(n, 62) * 64, 256); }
 
-void launch_sweep_dx(yb_sim* s, int sweep, bool adaptive) {
+void launch_sweep(yb_sim* s, int sweep, bool adaptive, bool partials) {
     const int in = sweep & 1;
     const int out = in ^ 1;
-    hipLaunchKernelGGL(k_sweep_dx, dim3(sweep_blocks(s->n)), dim3(256), 0, s->stream,
-                       s->b.params, s->b.pos, s->b.prm, s->b.yv, s->b.q[in], s->b.dx[in],
-                       s->b.dx[out], s->b.numCols, s->b.cols, adaptive ? s->b.ctl : nullptr,
-                       sweep, adaptive ? s->b.partial : nullptr);
+    hipLaunchKernelGGL(k_sweep, dim3(sweep_blocks(s->n)), dim3(256), 0, s->stream, s->b.params,
+                       s->b.pos, s->b.prm, s->b.yv, s->b.q[in], s->b.q[out], s->b.qRest,
+                       s->b.dx[in], s->b.dx[out], s->b.numCols, s->b.cols,
+                       adaptive ? s->b.ctl : nullptr, sweep, partials ? s->b.partial : nullptr);
 }
 
 void launch_residual(yb_sim* s, int sweep) {
-    hipLaunchKernelGGL(k_residual, dim3(1), dim3(256), 0, s->stream, s->b.ctl, s->b.partial,
+    hipLaunchKernelGGL(k_residual, dim3(1), dim3(1024), 0, s->stream, s->b.ctl, s->b.partial,
                        sweep_blocks(s->n), sweep, s->tol, s->maxSweeps);
-}
-
-void launch_sweep_q(yb_sim* s, int sweep, bool adaptive) {
-    const int in = sweep & 1;
-    const int out = in ^ 1;
-    hipLaunchKernelGGL(k_sweep_q, dim3(blocks(s->n, 256)), dim3(256), 0, s->stream, s->b.params,
-                       s->b.pos, s->b.prm, s->b.dx[out], s->b.q[in], s->b.q[out], s->b.qRest,
-                       adaptive ? s->b.ctl : nullptr, sweep);
 }
 
 void launch_ctl_reset(yb_sim* s) {

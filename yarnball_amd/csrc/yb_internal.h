@@ -126,6 +126,7 @@ struct yb_sim {
     float tol = 0.f;
     int maxSweeps = 0;
     int chunk = 8;
+    int checkEvery = 1;
     hipGraphExec_t proGraphs[3] = {nullptr, nullptr, nullptr};
     hipGraphExec_t endGraph = nullptr;
     hipGraphExec_t* chunkGraphs = nullptr;
@@ -152,8 +153,7 @@ void launch_step_limit(yb_sim* s);              // collision.cu:121-172
 void launch_separation(yb_sim* s);              // contact distances of the current state
 // Sweep k reads dx[k&1], q[k&1] and writes the other buffer.  adaptive: the
 // kernel honours SolverCtl (early exit after convergence, residual partials).
-void launch_sweep_dx(yb_sim* s, int sweep, bool adaptive);  // cosserat.cu:20-207
-void launch_sweep_q(yb_sim* s, int sweep, bool adaptive);   // cosserat.cu:209-253
+void launch_sweep(yb_sim* s, int sweep, bool adaptive, bool partials);  // cosserat.cu:20-253
 void launch_residual(yb_sim* s, int sweep);
 void launch_ctl_reset(yb_sim* s);
 void launch_end(yb_sim* s, int fin, bool adaptive);         // iteration.cu:48-61

@@ -155,12 +155,13 @@ class Sim:
         print(f"Segments with collisions: {int((c > 0).sum())}/{self.numVerts}")
 
     # -- residual-targeted solving (no counterpart in the reference) -----------
-    def setSolver(self, residualTol, maxSweeps=512, chunk=8):
+    def setSolver(self, residualTol, maxSweeps=512, chunk=8, checkEvery=1):
         """Sweep every substep until the relative Newton residual
         RMS |dx_k+1 - dx_k| / rest length <= residualTol (or maxSweeps); residualTol <= 0
         restores the fixed meta.numItr schedule."""
         self._need_handle()
-        check(self._lib.yb_set_solver(self._h, float(residualTol), int(maxSweeps), int(chunk)))
+        check(self._lib.yb_set_solver(self._h, float(residualTol), int(maxSweeps), int(chunk),
+                                      int(checkEvery)))
 
     def solverStats(self, reset=False):
         st = _lib.SolverStats()
