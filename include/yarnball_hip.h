@@ -201,6 +201,44 @@ typedef struct yb_solver_stats_t {
 int yb_set_solver(yb_sim* s, float residualTol, int maxSweeps, int chunk, int checkEvery);
 int yb_solver_stats(yb_sim* s, yb_solver_stats_t* out, int reset);
 
+/* ---- learned warm start ------------------------------------------------------
+ * No counterpart in the reference, whose warm start is the gravity
+ * extrapolation of iteration.cu:18-34.  A 48 -> 128 -> 128 -> 3 ReLU MLP
+ * (bf16 on the matrix cores, fp32 accumulation) predicts each free vertex's
+ * correction (dx* - dx0)/L from 48 local features; the corrected dx0, clamped
+ * to the step limit, starts the sweeps.  Non-finite predictions keep dx0.
+ * Parameter vector (YB_MLP_PARAMS floats): W1[48][128], b1[128], W2[128][128],
+ * b2[128], W3[128][3], b3[3] (YB_MLP_TRAINABLE values), then the feature
+ * means[48], inverse standard deviations[48] and output scales[3]. */
+#define YB_MLP_INPUTS 48
+#define YB_MLP_HIDDEN 128
+#define YB_MLP_OUTPUTS 3
+#define YB_MLP_TRAINABLE 23171
+#define YB_MLP_PARAMS 23270
+int yb_mlp_load(yb_sim* s, const float* params, int count);
+/* mode 0: plain warm start, 1: learned (needs yb_mlp_load). */
+int yb_set_warm_start(yb_sim* s, int mode);
+/* out[0] predictions applied, out[1] fallbacks to the plain warm start. */
+int yb_mlp_counters(yb_sim* s, int64_t out[2], int reset);
+/* Training-data recording: each substep appends its n feature rows (raw,
+ * unnormalised) and, after convergence, the n target corrections, until
+ * `capacity` rows are stored.  capacity 0 stops recording. */
+int yb_record(yb_sim* s, int64_t capacity);
+int yb_record_fetch(yb_sim* s, float* X, float* T, int64_t* count);
+/* Training on the device (own forward / backward kernels, Adam):
+ * X n x 48 raw features, T n x 3 targets; params in/out (normalisation
+ * constants are read, the trainable part is updated).  lossHistory (epochs
+ * floats, may be null) receives the mean squared error per sample. */
+int yb_mlp_train(const float* X, const float* T, int64_t n, float* params, int epochs, int batch,
+                 float lr, uint32_t seed, float* lossHistory);
+/* Gradient of L = (1/n) sum_samples |y - t/scale|^2 over n samples (a multiple of
+ * 16) at `params`: grad receives YB_MLP_TRAINABLE floats, *loss the summed
+ * squared error.  Used to check the backward kernels. */
+int yb_mlp_grad(const float* X, const float* T, int64_t n, const float* params, float* grad,
+                float* loss);
+/* Forward pass: Y (n x 3) in normalised output units. */
+int yb_mlp_forward(const float* X, int64_t n, const float* params, float* Y);
+
 /* ---- live per-stage timing (measurement harness) --------------------------
  * Runs `substeps` substeps of length h with the yb_advance schedule, enqueued
  * without graphs and with a HIP event between consecutive kernels on the
@@ -216,7 +254,7 @@ enum {
     YB_STAGE_QUERY = 4,       /* LBVH query + collision.cu:39-92 */
     YB_STAGE_STEP_LIMIT = 5,  /* collision.cu:121-159 */
     YB_STAGE_SWEEP = 6,       /* cosserat.cu:20-253 (dx step + frame update, fused) */
-    YB_STAGE_UNUSED = 7,
+    YB_STAGE_MLP = 7,         /* learned warm start: features + inference */
     YB_STAGE_END = 8,         /* iteration.cu:48-61 */
     YB_STAGE_RESIDUAL = 9,    /* residual reduction / solver control */
     YB_NUM_STAGES = 10

@@ -117,11 +117,25 @@ SIGNATURES = {
     "yb_profile": (_I, [_P, ctypes.POINTER(Params), _F, _I, _P, _P]),
     "yb_set_solver": (_I, [_P, _F, _I, _I, _I]),
     "yb_min_separation": (_I, [_P, _P, _I]),
+    "yb_mlp_load": (_I, [_P, _P, _I]),
+    "yb_set_warm_start": (_I, [_P, _I]),
+    "yb_mlp_counters": (_I, [_P, _P, _I]),
+    "yb_record": (_I, [_P, ctypes.c_int64]),
+    "yb_record_fetch": (_I, [_P, _P, _P, ctypes.POINTER(ctypes.c_int64)]),
+    "yb_mlp_train": (_I, [_P, _P, ctypes.c_int64, _P, _I, _I, _F, ctypes.c_uint32, _P]),
+    "yb_mlp_grad": (_I, [_P, _P, ctypes.c_int64, _P, _P, _P]),
+    "yb_mlp_forward": (_I, [_P, ctypes.c_int64, _P, _P]),
     "yb_solver_stats": (_I, [_P, ctypes.POINTER(SolverStats), _I]),
 }
 
-STAGES = ["init", "aabb", "rebuild", "refit", "query", "step_limit", "sweep", "unused", "end",
+STAGES = ["init", "aabb", "rebuild", "refit", "query", "step_limit", "sweep", "mlp", "end",
           "residual"]
+
+MLP_INPUTS = 48
+MLP_HIDDEN = 128
+MLP_OUTPUTS = 3
+MLP_TRAINABLE = 23171
+MLP_PARAMS = 23270
 
 _lib = None
 

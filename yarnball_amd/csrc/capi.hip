@@ -47,7 +47,9 @@ static void free_all(yb_sim* s) {
                     b.aabb,    b.sceneBox, b.blockBox, b.morton, b.keys[0], b.keys[1],
                     b.vals[0], b.vals[1], b.hist,     b.nodes,   b.parent,  b.rng,
                     b.kids,    b.strad,   b.stradCount, b.leafNeed, b.sufBox, b.preBox, b.blkTab,
-                    b.err,     b.params,  b.stage, b.ctl, b.partial, b.minSep};
+                    b.err,     b.params,  b.stage, b.ctl, b.partial, b.minSep,
+                    b.mlpPrm,  b.mlpTiles, b.mlpBias, b.featB, b.corrPrev, b.dxPlain,
+                    b.mlpCount, b.recX, b.recT, b.recCount};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (b.hostFlag) (void)hipHostFree(b.hostFlag);
@@ -146,6 +148,14 @@ extern "C" int yb_create(int n, yb_sim** out) {
     ALLOC(b.params, 1);
     ALLOC(b.ctl, 1);
     ALLOC(b.minSep, 1);
+    ALLOC(b.mlpPrm, kMlpParams);
+    ALLOC(b.mlpTiles, 168 * 256);
+    ALLOC(b.mlpBias, 2 * kMlpHid + 16);
+    ALLOC(b.featB, N * kMlpIn);
+    ALLOC(b.corrPrev, N);
+    ALLOC(b.dxPlain, N);
+    ALLOC(b.mlpCount, 2);
+    ALLOC(b.recCount, 1);
     ALLOC(b.partial, (size_t)sweep_blocks(n));
 #undef ALLOC
     if (e == hipSuccess) e = hipHostMalloc((void**)&b.hostFlag, 2 * sizeof(int32_t));
@@ -173,6 +183,9 @@ extern "C" int yb_create(int n, yb_sim** out) {
     if (e == hipSuccess) e = hipMemset(b.numCols, 0, sizeof(int32_t) * N);
     if (e == hipSuccess) e = hipMemset(b.err, 0, sizeof(int32_t) * 2);
     if (e == hipSuccess) e = hipMemset(b.nodes, 0, sizeof(float4) * 4 * (N - 1));
+    if (e == hipSuccess) e = hipMemset(b.corrPrev, 0, sizeof(float4) * N);
+    if (e == hipSuccess) e = hipMemset(b.mlpCount, 0, sizeof(unsigned long long) * 2);
+    if (e == hipSuccess) e = hipMemset(b.recCount, 0, sizeof(int64_t));
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
         set_message(std::string("yb_create: ") + hipGetErrorString(e));
@@ -338,6 +351,12 @@ static void enqueue_prologue(yb_sim* s, int mode, bool adaptive) {
     }
     launch_step_limit(s);
     mark(s, YB_STAGE_STEP_LIMIT);
+    const bool learned = s->warmStart == 1, record = s->recCap > 0;
+    if (learned || record) {
+        launch_features(s, learned, record);
+        if (learned) launch_mlp_apply(s);
+        mark(s, YB_STAGE_MLP);
+    }
     if (adaptive) {
         launch_ctl_reset(s);
         mark(s, YB_STAGE_RESIDUAL);
@@ -362,6 +381,7 @@ static void enqueue_sweep(yb_sim* s, int k, bool adaptive) {
 
 static void enqueue_end(yb_sim* s, int fin, bool adaptive) {
     launch_end(s, fin, adaptive);
+    if (s->warmStart == 1 || s->recCap > 0) launch_correction(s, fin, adaptive, s->recCap > 0);
     mark(s, YB_STAGE_END);
 }
 
@@ -384,6 +404,7 @@ static int capture(yb_sim* s, hipGraphExec_t* out, const std::function<void()>& 
 // (the reference's stepGraph plus initItr and detection, step.cpp:70-99).
 static int rebuild_graphs(yb_sim* s, int numItr) {
     if (s->graphItr == numItr && s->graphs[0]) return YB_OK;
+    s->graphItr = -1;
     HIPCHECK(hipStreamSynchronize(s->stream));
     for (int m = 0; m < 3; m++) {
         int rc = capture(s, &s->graphs[m], [&]() {

@@ -1,0 +1,845 @@
+// mlp.hip — learned warm start for the Newton sweeps (no counterpart in the
+// reference, whose warm start is the gravity extrapolation of
+// iteration.cu:18-34).
+//
+// A 48 -> 128 -> 128 -> 3 MLP (ReLU) predicts, per vertex, the correction
+// c = (dx* - dx0) / L between the converged iterate dx* and the plain warm
+// start dx0, in units of the local rest length L.  The prediction replaces
+// dx0 before the first sweep; the residual-targeted sweeps then run to the
+// same tolerance as without it, so the plain solver remains the judge of the
+// final state.  A prediction that is not finite keeps dx0 (counted as a
+// fallback); every prediction is clamped to the step limit like a sweep.
+//
+// Arithmetic: bf16 operands on the matrix cores
+// (v_mfma_f32_16x16x16bf16_1k), fp32 accumulation, fp32 master weights.
+// Orientation: activations are kept transposed (neurons x samples), weights
+// are the A operand, so a layer's 16x16 output tile in the accumulator layout
+// (lane l holds rows 4*(l/16)+i of column l%16) is exactly the B operand of
+// the next layer's K-step -- no shuffles or LDS round trips between layers.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <numeric>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "yb_internal.h"
+#include "ybmath.h"
+
+namespace yb {
+
+typedef short s4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ f4_t mfma16(s4_t a, s4_t b, f4_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ v3 xyz4(float4 a) { return v3{a.x, a.y, a.z}; }
+
+// fp32 -> bf16, round to nearest even.
+__host__ __device__ __forceinline__ uint16_t to_bf16(float f) {
+    uint32_t u = __builtin_bit_cast(uint32_t, f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ s4_t pack4(f4_t v) {
+    s4_t r;
+    r.x = (short)to_bf16(v.x);
+    r.y = (short)to_bf16(v.y);
+    r.z = (short)to_bf16(v.z);
+    r.w = (short)to_bf16(v.w);
+    return r;
+}
+__device__ __forceinline__ f4_t relu4(f4_t v) {
+    return f4_t{fmaxf(v.x, 0.f), fmaxf(v.y, 0.f), fmaxf(v.z, 0.f), fmaxf(v.w, 0.f)};
+}
+__device__ __forceinline__ f4_t mask4(f4_t g, f4_t z) {
+    return f4_t{z.x > 0.f ? g.x : 0.f, z.y > 0.f ? g.y : 0.f, z.z > 0.f ? g.z : 0.f,
+                z.w > 0.f ? g.w : 0.f};
+}
+
+// ---- packed weight tiles ------------------------------------------------------
+// One tile = 64 lanes x 4 bf16.  Forward tiles hold W^T (A[m][k] = W[k][m]),
+// backward tiles hold W (A[m][k] = W[m][k]); lane l, element i covers
+// m = 16 t + l%16, k = 16 s + 4 (l/16) + i.
+constexpr int kTileElems = 256;
+constexpr int kT1 = 0;                  // W1^T: t 0..7, s 0..2
+constexpr int kT2 = kT1 + 8 * 3;        // W2^T: t 0..7, s 0..7
+constexpr int kT3 = kT2 + 8 * 8;        // W3^T: s 0..7 (t = 0, rows 0..2 real)
+constexpr int kFwdTiles = kT3 + 8;      // 96
+constexpr int kB3 = kFwdTiles;          // W3:   t 0..7 (s = 0, cols 0..2 real)
+constexpr int kB2 = kB3 + 8;            // W2:   t 0..7, s 0..7
+constexpr int kAllTiles = kB2 + 64;     // 168
+
+__device__ __forceinline__ float wget(const float* W, int rows, int cols, int r, int c) {
+    return (r < rows && c < cols) ? W[r * cols + c] : 0.f;
+}
+
+__global__ void k_mlp_pack(const float* __restrict__ prm, uint16_t* __restrict__ tiles,
+                           float* __restrict__ bias) {
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    const float* W1 = prm + kMlpW1;
+    const float* W2 = prm + kMlpW2;
+    const float* W3 = prm + kMlpW3;
+    if (id < kAllTiles * 64) {
+        const int tile = id >> 6, l = id & 63, g = l >> 4, r = l & 15;
+        float v[4];
+        for (int i = 0; i < 4; i++) {
+            const int k = 4 * g + i;
+            if (tile < kT2) {  // W1^T
+                const int t = (tile - kT1) / 3, s = (tile - kT1) % 3;
+                v[i] = wget(W1, kMlpIn, kMlpHid, 16 * s + k, 16 * t + r);
+            } else if (tile < kT3) {  // W2^T
+                const int t = (tile - kT2) / 8, s = (tile - kT2) % 8;
+                v[i] = wget(W2, kMlpHid, kMlpHid, 16 * s + k, 16 * t + r);
+            } else if (tile < kB3) {  // W3^T
+                const int s = tile - kT3;
+                v[i] = wget(W3, kMlpHid, kMlpOut, 16 * s + k, r);
+            } else if (tile < kB2) {  // W3
+                const int t = tile - kB3;
+                v[i] = wget(W3, kMlpHid, kMlpOut, 16 * t + r, k);
+            } else {  // W2
+                const int t = (tile - kB2) / 8, s = (tile - kB2) % 8;
+                v[i] = wget(W2, kMlpHid, kMlpHid, 16 * t + r, 16 * s + k);
+            }
+            tiles[tile * kTileElems + l * 4 + i] = to_bf16(v[i]);
+        }
+    }
+    // biases (fp32, outputs padded to 16) and normalisation constants
+    if (id < kMlpHid) {
+        bias[id] = prm[kMlpB1 + id];
+        bias[kMlpHid + id] = prm[kMlpB2 + id];
+    }
+    if (id < 16) bias[2 * kMlpHid + id] = id < kMlpOut ? prm[kMlpB3 + id] : 0.f;
+}
+
+__device__ __forceinline__ s4_t tile_ld(const uint16_t* tiles, int tile, int lane) {
+    return *reinterpret_cast<const s4_t*>(tiles + tile * kTileElems + lane * 4);
+}
+__device__ __forceinline__ f4_t bias_ld(const float* b, int base) {
+    return f4_t{b[base], b[base + 1], b[base + 2], b[base + 3]};
+}
+
+struct Fwd {
+    f4_t z1[8], z2[8], y;
+    s4_t h1[8], h2[8];
+};
+
+// Forward pass of 16 samples held by one wave: x[s] are the B operands of the
+// input K-steps (lane l: sample l%16, features 16 s + 4 (l/16) + i).
+__device__ __forceinline__ void mlp_forward(const uint16_t* tiles, const float* bias,
+                                            const s4_t x[3], Fwd& F) {
+    const int lane = threadIdx.x & 63;
+    const int g = lane >> 4;
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+        f4_t acc = bias_ld(bias, 16 * t + 4 * g);
+#pragma unroll
+        for (int s = 0; s < 3; s++) acc = mfma16(tile_ld(tiles, kT1 + t * 3 + s, lane), x[s], acc);
+        F.z1[t] = acc;
+        F.h1[t] = pack4(relu4(acc));
+    }
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+        f4_t acc = bias_ld(bias, kMlpHid + 16 * t + 4 * g);
+#pragma unroll
+        for (int s = 0; s < 8; s++) acc = mfma16(tile_ld(tiles, kT2 + t * 8 + s, lane), F.h1[s], acc);
+        F.z2[t] = acc;
+        F.h2[t] = pack4(relu4(acc));
+    }
+    f4_t acc = bias_ld(bias, 2 * kMlpHid + 4 * g);
+#pragma unroll
+    for (int s = 0; s < 8; s++) acc = mfma16(tile_ld(tiles, kT3 + s, lane), F.h2[s], acc);
+    F.y = acc;  // lanes 0..15: outputs 0..2 (and a zero pad) of sample l%16
+}
+
+// ---- features -----------------------------------------------------------------
+// 48 features of vertex i (L = rest length of its segment, or of the previous
+// one at a strand end; h = substep):
+//   0-11  neighbour offsets (x_j - x_i)/L for j = i-1, i+1, i-2, i+2 (0 if absent)
+//  12-14  v_i h / L          15-17  (y_i - x_i)/L       18-20  dx0_i / L
+//  21-22  contacts of segments i and i-1 (/8)
+//  23-26  frame of segment i   27-30  frame of segment i-1
+//  31-35  has i-1, has i+1, has i-2, has i+2, pinned
+//  36-44  previous substep's converged correction at i, i-1, i+1
+//  45     step limit / L (clamped to 10)   46  |g| h^2 / L
+//  47     log10(1 + k_stretch h^2 / (m L^2)) / 4
+__device__ __forceinline__ float rest_len(const float4* prm, int v, uint32_t flags) {
+    if (!(flags & YB_FLAG_HAS_NEXT) && (flags & YB_FLAG_HAS_PREV)) return prm[v - 1].x;
+    return prm[v].x;
+}
+
+__global__ __launch_bounds__(256) void k_features(
+    const DevParams* __restrict__ P, const float4* __restrict__ pos, const float4* __restrict__ prm,
+    const float4* __restrict__ vel, const float4* __restrict__ yv, const float4* __restrict__ dx0,
+    const float4* __restrict__ q, const int32_t* __restrict__ numCols,
+    const float4* __restrict__ corrPrev, const float* __restrict__ mlpPrm,
+    uint16_t* __restrict__ featB, float* __restrict__ recX, const int64_t* __restrict__ recCount,
+    int64_t recCap, float4* __restrict__ dxPlain) {
+    const int n = P->n;
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    const float4 r4 = prm[v];
+    const uint32_t fl = __float_as_uint(r4.w);
+    const bool hp = fl & YB_FLAG_HAS_PREV, hn = fl & YB_FLAG_HAS_NEXT;
+    const float L = rest_len(prm, v, fl);
+    const float iL = 1.0f / L;
+    const float h = P->h;
+    const v3 x = xyz4(pos[v]);
+    float f[kMlpIn];
+    for (int k = 0; k < kMlpIn; k++) f[k] = 0.f;
+    bool hpp = false, hnn = false;
+    if (hp) {
+        const v3 d = (xyz4(pos[v - 1]) - x) * iL;
+        f[0] = d.x; f[1] = d.y; f[2] = d.z;
+        hpp = __float_as_uint(prm[v - 1].w) & YB_FLAG_HAS_PREV;
+        if (hpp) {
+            const v3 e = (xyz4(pos[v - 2]) - x) * iL;
+            f[6] = e.x; f[7] = e.y; f[8] = e.z;
+        }
+    }
+    if (hn) {
+        const v3 d = (xyz4(pos[v + 1]) - x) * iL;
+        f[3] = d.x; f[4] = d.y; f[5] = d.z;
+        hnn = __float_as_uint(prm[v + 1].w) & YB_FLAG_HAS_NEXT;
+        if (hnn) {
+            const v3 e = (xyz4(pos[v + 2]) - x) * iL;
+            f[9] = e.x; f[10] = e.y; f[11] = e.z;
+        }
+    }
+    const v3 u = xyz4(vel[v]) * (h * iL);
+    f[12] = u.x; f[13] = u.y; f[14] = u.z;
+    const bool pinned = pos[v].w == 0.0f;
+    if (!pinned) {
+        const v3 y = xyz4(yv[v]) * iL;
+        f[15] = y.x; f[16] = y.y; f[17] = y.z;
+    }
+    const float4 d0 = dx0[v];
+    const v3 d = xyz4(d0) * iL;
+    f[18] = d.x; f[19] = d.y; f[20] = d.z;
+    f[21] = hn ? numCols[v] * 0.125f : 0.f;
+    f[22] = hp ? numCols[v - 1] * 0.125f : 0.f;
+    const float4 qi = q[hn ? v : (hp ? v - 1 : v)];
+    const float4 qp = q[hp ? v - 1 : v];
+    f[23] = qi.x; f[24] = qi.y; f[25] = qi.z; f[26] = qi.w;
+    f[27] = qp.x; f[28] = qp.y; f[29] = qp.z; f[30] = qp.w;
+    f[31] = hp; f[32] = hn; f[33] = hpp; f[34] = hnn; f[35] = pinned;
+    const float4 c0 = corrPrev[v];
+    f[36] = c0.x; f[37] = c0.y; f[38] = c0.z;
+    if (hp) {
+        const float4 c = corrPrev[v - 1];
+        f[39] = c.x; f[40] = c.y; f[41] = c.z;
+    }
+    if (hn) {
+        const float4 c = corrPrev[v + 1];
+        f[42] = c.x; f[43] = c.y; f[44] = c.z;
+    }
+    f[45] = fminf(d0.w * iL, 10.0f);
+    const float g2 = sqrtf(P->gx * P->gx + P->gy * P->gy + P->gz * P->gz);
+    f[46] = g2 * h * h * iL;
+    const float kst = r4.y * iL * iL * h * h * pos[v].w;
+    f[47] = log10f(1.0f + kst) * 0.25f;
+
+    if (recX) {
+        const int64_t slot = *recCount + v;
+        if (slot < recCap)
+            for (int k = 0; k < kMlpIn; k++) recX[slot * kMlpIn + k] = f[k];
+    }
+    if (featB) {
+        const float* mean = mlpPrm + kMlpMean;
+        const float* istd = mlpPrm + kMlpIStd;
+        for (int k = 0; k < kMlpIn; k++)
+            featB[(size_t)v * kMlpIn + k] = to_bf16((f[k] - mean[k]) * istd[k]);
+    }
+    dxPlain[v] = d0;
+}
+
+// ---- inference + warm-start application ----------------------------------------
+__global__ __launch_bounds__(256) void k_mlp_apply(const DevParams* __restrict__ P,
+                                                   const uint16_t* __restrict__ tilesG,
+                                                   const float* __restrict__ biasG,
+                                                   const float* __restrict__ mlpPrm,
+                                                   const uint16_t* __restrict__ featB,
+                                                   const float4* __restrict__ pos,
+                                                   const float4* __restrict__ prm,
+                                                   float4* __restrict__ dx,
+                                                   unsigned long long* __restrict__ counters) {
+    __shared__ uint16_t tiles[kFwdTiles * kTileElems];
+    __shared__ float bias[2 * kMlpHid + 16];
+    for (int k = threadIdx.x; k < kFwdTiles * kTileElems / 8; k += blockDim.x)
+        reinterpret_cast<uint4*>(tiles)[k] = reinterpret_cast<const uint4*>(tilesG)[k];
+    for (int k = threadIdx.x; k < 2 * kMlpHid + 16; k += blockDim.x) bias[k] = biasG[k];
+    __syncthreads();
+    const int n = P->n;
+    const int lane = threadIdx.x & 63;
+    const int g = lane >> 4, r = lane & 15;
+    const int wavesTotal = gridDim.x * (blockDim.x >> 6);
+    const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const float o0 = mlpPrm[kMlpOScale], o1 = mlpPrm[kMlpOScale + 1], o2 = mlpPrm[kMlpOScale + 2];
+    unsigned applied = 0, fallback = 0;
+    for (int base = wave * 16; base < n; base += wavesTotal * 16) {
+        const int v = base + r;
+        s4_t x[3];
+#pragma unroll
+        for (int s = 0; s < 3; s++) {
+            if (v < n)
+                x[s] = *reinterpret_cast<const s4_t*>(featB + (size_t)v * kMlpIn + 16 * s + 4 * g);
+            else
+                x[s] = s4_t{0, 0, 0, 0};
+        }
+        Fwd F;
+        mlp_forward(tiles, bias, x, F);
+        if (g == 0 && v < n) {
+            const float4 p4 = pos[v];
+            if (p4.w != 0.0f) {
+                const uint32_t fl = __float_as_uint(prm[v].w);
+                const float L = rest_len(prm, v, fl);
+                float4 d4 = dx[v];
+                const v3 c = v3{F.y.x * o0, F.y.y * o1, F.y.z * o2} * L;
+                if (isfinite(c.x) && isfinite(c.y) && isfinite(c.z)) {
+                    v3 nd = xyz4(d4) + c;
+                    const float l = len(nd);
+                    if (l > d4.w && l > 0.0f) nd *= d4.w / l;
+                    d4.x = nd.x;
+                    d4.y = nd.y;
+                    d4.z = nd.z;
+                    dx[v] = d4;
+                    applied++;
+                } else {
+                    fallback++;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        applied += __shfl_xor(applied, off, 64);
+        fallback += __shfl_xor(fallback, off, 64);
+    }
+    if (lane == 0 && (applied | fallback)) {
+        atomicAdd(&counters[0], (unsigned long long)applied);
+        atomicAdd(&counters[1], (unsigned long long)fallback);
+    }
+}
+
+// ---- end of substep: converged correction ---------------------------------------
+// corrPrev = (dx* - dx0) / L for the next substep's features; in recording mode
+// the same values are the training targets of this substep's features.
+__global__ __launch_bounds__(256) void k_correction(const DevParams* __restrict__ P,
+                                                    const float4* __restrict__ prm,
+                                                    const float4* __restrict__ dxA,
+                                                    const float4* __restrict__ dxB,
+                                                    const SolverCtl* __restrict__ ctl, int fin,
+                                                    const float4* __restrict__ dxPlain,
+                                                    float4* __restrict__ corrPrev,
+                                                    float* __restrict__ recT,
+                                                    const int64_t* __restrict__ recCount,
+                                                    int64_t recCap) {
+    const int n = P->n;
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    if (ctl) fin = ctl->fin;
+    const float4 d = (fin ? dxB : dxA)[v];
+    const float4 p = dxPlain[v];
+    const uint32_t fl = __float_as_uint(prm[v].w);
+    const float iL = 1.0f / rest_len(prm, v, fl);
+    const float4 c = make_float4((d.x - p.x) * iL, (d.y - p.y) * iL, (d.z - p.z) * iL, 0.f);
+    corrPrev[v] = c;
+    if (recT) {
+        const int64_t slot = *recCount + v;
+        if (slot < recCap) {
+            recT[slot * 3 + 0] = c.x;
+            recT[slot * 3 + 1] = c.y;
+            recT[slot * 3 + 2] = c.z;
+        }
+    }
+}
+
+__global__ void k_rec_advance(int64_t* __restrict__ recCount, int n, int64_t recCap) {
+    if (threadIdx.x == 0) {
+        const int64_t c = *recCount + n;
+        *recCount = c < recCap ? c : recCap;
+    }
+}
+
+// ---- training ---------------------------------------------------------------------
+// Normalised bf16 inputs and targets.
+__global__ void k_norm_data(const float* __restrict__ X, const float* __restrict__ T, int64_t n,
+                            const float* __restrict__ mlpPrm, uint16_t* __restrict__ Xb,
+                            float* __restrict__ Tn) {
+    const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id < n * kMlpIn) {
+        const int k = (int)(id % kMlpIn);
+        Xb[id] = to_bf16((X[id] - mlpPrm[kMlpMean + k]) * mlpPrm[kMlpIStd + k]);
+    }
+    if (T && id < n * kMlpOut) {
+        const int k = (int)(id % kMlpOut);
+        Tn[id] = T[id] / mlpPrm[kMlpOScale + k];
+    }
+}
+
+// Forward + backward propagation for one minibatch; one wave per 16 samples.
+// Writes the transposed bf16 activations / gradients (neuron x sample) that
+// the weight-gradient GEMM consumes, and per-wave squared-error sums.
+struct TrainBufs {
+    uint16_t *XT, *H1T, *H2T, *G1T, *G2T, *GYT;  // [rows][B]
+};
+
+__global__ __launch_bounds__(256) void k_train_fb(const uint16_t* __restrict__ tiles,
+                                                  const float* __restrict__ bias,
+                                                  const uint16_t* __restrict__ Xb,
+                                                  const float* __restrict__ Tn,
+                                                  const int32_t* __restrict__ perm, int batch,
+                                                  TrainBufs tb, float* __restrict__ lossPart,
+                                                  float gradScale) {
+    const int lane = threadIdx.x & 63;
+    const int g = lane >> 4, r = lane & 15;
+    const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int base = wave * 16;
+    if (base >= batch) return;
+    const int sIdx = base + r;  // sample within the batch
+    const int row = perm[sIdx];
+    s4_t x[3];
+#pragma unroll
+    for (int s = 0; s < 3; s++) {
+        x[s] = *reinterpret_cast<const s4_t*>(Xb + (size_t)row * kMlpIn + 16 * s + 4 * g);
+        for (int i = 0; i < 4; i++) tb.XT[(size_t)(16 * s + 4 * g + i) * batch + sIdx] = (uint16_t)x[s][i];
+    }
+    Fwd F;
+    mlp_forward(tiles, bias, x, F);
+    // dL/dy for L = sum (y - t)^2 * gradScale (rows 0..2 of lanes g == 0)
+    f4_t dy = f4_t{0.f, 0.f, 0.f, 0.f};
+    float se = 0.f;
+    if (g == 0) {
+        for (int i = 0; i < 3; i++) {
+            const float e = F.y[i] - Tn[(size_t)row * 3 + i];
+            se += e * e;
+            dy[i] = 2.0f * e * gradScale;
+        }
+    }
+    const s4_t dyb = pack4(dy);
+    for (int i = 0; i < 4; i++) tb.GYT[(size_t)(4 * g + i) * batch + sIdx] = (uint16_t)dyb[i];
+    s4_t g2b[8];
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+        f4_t acc = mfma16(tile_ld(tiles, kB3 + t, lane), dyb, f4_t{0.f, 0.f, 0.f, 0.f});
+        g2b[t] = pack4(mask4(acc, F.z2[t]));
+    }
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+        f4_t acc = f4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < 8; s++) acc = mfma16(tile_ld(tiles, kB2 + t * 8 + s, lane), g2b[s], acc);
+        const s4_t g1b = pack4(mask4(acc, F.z1[t]));
+        for (int i = 0; i < 4; i++) {
+            const size_t rr = (size_t)(16 * t + 4 * g + i) * batch + sIdx;
+            tb.G1T[rr] = (uint16_t)g1b[i];
+            tb.G2T[rr] = (uint16_t)g2b[t][i];
+            tb.H1T[rr] = (uint16_t)F.h1[t][i];
+            tb.H2T[rr] = (uint16_t)F.h2[t][i];
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) se += __shfl_xor(se, off, 64);
+    if (lane == 0) lossPart[wave] = se;
+}
+
+// Weight gradients as a split-K GEMM over the batch: task = (tile, split).
+// Tiles 0..23: dW1^T (h1 tile t x feature tile u), 24..87: dW2^T (h2 t x h1 u),
+// 88..95: dW3^T (output rows x h2 u).
+constexpr int kGradTiles = 96;
+__global__ __launch_bounds__(64) void k_train_wgrad(TrainBufs tb, int batch, int splits,
+                                                    float* __restrict__ part) {
+    const int task = blockIdx.x;
+    const int tile = task % kGradTiles, split = task / kGradTiles;
+    const int lane = threadIdx.x;
+    const int g = lane >> 4, r = lane & 15;
+    const uint16_t *A, *B;
+    int ta, tbb;
+    if (tile < 24) {
+        A = tb.G1T; B = tb.XT; ta = tile / 3; tbb = tile % 3;
+    } else if (tile < 88) {
+        A = tb.G2T; B = tb.H1T; ta = (tile - 24) / 8; tbb = (tile - 24) % 8;
+    } else {
+        A = tb.GYT; B = tb.H2T; ta = 0; tbb = tile - 88;
+    }
+    const int per = batch / splits;
+    const int k0 = split * per, k1 = k0 + per;
+    f4_t acc = f4_t{0.f, 0.f, 0.f, 0.f};
+    const uint16_t* arow = A + (size_t)(16 * ta + r) * batch + 4 * g;
+    const uint16_t* brow = B + (size_t)(16 * tbb + r) * batch + 4 * g;
+    for (int k = k0; k < k1; k += 16)
+        acc = mfma16(*reinterpret_cast<const s4_t*>(arow + k),
+                     *reinterpret_cast<const s4_t*>(brow + k), acc);
+    *reinterpret_cast<f4_t*>(part + ((size_t)task * 64 + lane) * 4) = acc;
+}
+
+// Sum the splits into the canonical gradient vector.
+__global__ void k_train_gsum(const float* __restrict__ part, int splits, float* __restrict__ grad) {
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;  // (tile, lane, i)
+    if (id >= kGradTiles * 256) return;
+    const int tile = id / 256, rem = id % 256, lane = rem / 4, i = rem % 4;
+    const int g = lane >> 4, r = lane & 15;
+    float s = 0.f;
+    for (int sp = 0; sp < splits; sp++) s += part[((size_t)(sp * kGradTiles + tile) * 64 + lane) * 4 + i];
+    const int m = 4 * g + i;  // accumulator row
+    if (tile < 24) {          // D[h1][feat] -> dW1[feat][h1]
+        const int h1 = 16 * (tile / 3) + m, f = 16 * (tile % 3) + r;
+        grad[kMlpW1 + f * kMlpHid + h1] = s;
+    } else if (tile < 88) {   // D[h2][h1] -> dW2[h1][h2]
+        const int h2 = 16 * ((tile - 24) / 8) + m, h1 = 16 * ((tile - 24) % 8) + r;
+        grad[kMlpW2 + h1 * kMlpHid + h2] = s;
+    } else {                  // D[out][h2] -> dW3[h2][out]
+        const int o = m, h2 = 16 * (tile - 88) + r;
+        if (o < kMlpOut) grad[kMlpW3 + h2 * kMlpOut + o] = s;
+    }
+}
+
+// Bias gradients: row sums of G1T, G2T, GYT; one block per bias.
+__global__ __launch_bounds__(256) void k_train_bgrad(TrainBufs tb, int batch,
+                                                     float* __restrict__ grad) {
+    __shared__ float red[256];
+    const int b = blockIdx.x;
+    const uint16_t* rowp;
+    int dst;
+    if (b < kMlpHid) {
+        rowp = tb.G1T + (size_t)b * batch; dst = kMlpB1 + b;
+    } else if (b < 2 * kMlpHid) {
+        rowp = tb.G2T + (size_t)(b - kMlpHid) * batch; dst = kMlpB2 + b - kMlpHid;
+    } else {
+        rowp = tb.GYT + (size_t)(b - 2 * kMlpHid) * batch; dst = kMlpB3 + b - 2 * kMlpHid;
+    }
+    float s = 0.f;
+    for (int k = threadIdx.x; k < batch; k += blockDim.x) {
+        s += __builtin_bit_cast(float, (uint32_t)rowp[k] << 16);
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) grad[dst] = red[0];
+}
+
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, int count, float lr, float b1, float b2, float eps,
+                       float c1, float c2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// Forward only, for tests: raw outputs (normalised units) of n samples.
+__global__ __launch_bounds__(256) void k_forward_only(const uint16_t* __restrict__ tiles,
+                                                      const float* __restrict__ bias,
+                                                      const uint16_t* __restrict__ Xb, int n,
+                                                      float* __restrict__ Y) {
+    const int lane = threadIdx.x & 63;
+    const int g = lane >> 4, r = lane & 15;
+    const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int base = wave * 16;
+    if (base >= n) return;
+    const int v = base + r;
+    s4_t x[3];
+    for (int s = 0; s < 3; s++)
+        x[s] = v < n ? *reinterpret_cast<const s4_t*>(Xb + (size_t)v * kMlpIn + 16 * s + 4 * g)
+                     : s4_t{0, 0, 0, 0};
+    Fwd F;
+    mlp_forward(tiles, bias, x, F);
+    if (g == 0 && v < n)
+        for (int i = 0; i < 3; i++) Y[(size_t)v * 3 + i] = F.y[i];
+}
+
+static inline int nblk(int64_t n, int b) { return (int)((n + b - 1) / b); }
+
+// ---- launchers used by the stepper ------------------------------------------------
+void launch_mlp_pack(yb_sim* s) {
+    hipLaunchKernelGGL(k_mlp_pack, dim3(nblk(kAllTiles * 64, 256)), dim3(256), 0, s->stream,
+                       s->b.mlpPrm, s->b.mlpTiles, s->b.mlpBias);
+}
+
+void launch_features(yb_sim* s, bool infer, bool record) {
+    hipLaunchKernelGGL(k_features, dim3(nblk(s->n, 256)), dim3(256), 0, s->stream, s->b.params,
+                       s->b.pos, s->b.prm, s->b.vel, s->b.yv, s->b.dx[0], s->b.q[0], s->b.numCols,
+                       s->b.corrPrev, s->b.mlpPrm, infer ? s->b.featB : nullptr,
+                       record ? s->b.recX : nullptr, s->b.recCount, s->recCap, s->b.dxPlain);
+}
+
+void launch_mlp_apply(yb_sim* s) {
+    hipLaunchKernelGGL(k_mlp_apply, dim3(std::min(nblk(s->n, 64), 2048)), dim3(256), 0, s->stream,
+                       s->b.params, s->b.mlpTiles, s->b.mlpBias, s->b.mlpPrm, s->b.featB, s->b.pos,
+                       s->b.prm, s->b.dx[0], s->b.mlpCount);
+}
+
+void launch_correction(yb_sim* s, int fin, bool adaptive, bool record) {
+    hipLaunchKernelGGL(k_correction, dim3(nblk(s->n, 256)), dim3(256), 0, s->stream, s->b.params,
+                       s->b.prm, s->b.dx[0], s->b.dx[1], adaptive ? s->b.ctl : nullptr, fin,
+                       s->b.dxPlain, s->b.corrPrev, record ? s->b.recT : nullptr, s->b.recCount,
+                       s->recCap);
+    if (record)
+        hipLaunchKernelGGL(k_rec_advance, dim3(1), dim3(64), 0, s->stream, s->b.recCount, s->n,
+                           s->recCap);
+}
+
+}  // namespace yb
+
+using namespace yb;
+
+#define HIPCHECK(expr)                                                              \
+    do {                                                                            \
+        hipError_t e_ = (expr);                                                     \
+        if (e_ != hipSuccess) {                                                     \
+            set_message(std::string(#expr) + ": " + hipGetErrorString(e_));         \
+            return YB_ERR_HIP;                                                      \
+        }                                                                           \
+    } while (0)
+
+// ---- training entry points (no simulation handle) ----------------------------------
+namespace {
+
+struct Trainer {
+    hipStream_t st = nullptr;
+    float *prm = nullptr, *grad = nullptr, *m = nullptr, *v = nullptr, *bias = nullptr;
+    float *X = nullptr, *T = nullptr, *Tn = nullptr, *part = nullptr, *loss = nullptr;
+    uint16_t *tiles = nullptr, *Xb = nullptr;
+    int32_t* perm = nullptr;
+    TrainBufs tb{};
+    int batch = 0, splits = 0;
+    ~Trainer() {
+        void* ps[] = {prm, grad, m, v, bias, X, T, Tn, part, loss, tiles, Xb, perm,
+                      tb.XT, tb.H1T, tb.H2T, tb.G1T, tb.G2T, tb.GYT};
+        for (void* p : ps)
+            if (p) (void)hipFree(p);
+        if (st) (void)hipStreamDestroy(st);
+    }
+};
+
+int trainer_init(Trainer& tr, const float* X, const float* T, int64_t n, const float* params,
+                 int batch) {
+    HIPCHECK(hipStreamCreate(&tr.st));
+    tr.batch = batch;
+    tr.splits = std::max(1, std::min(16, batch / 1024));
+    while (batch % (16 * tr.splits)) tr.splits--;
+    HIPCHECK(hipMalloc(&tr.prm, sizeof(float) * kMlpParams));
+    HIPCHECK(hipMalloc(&tr.grad, sizeof(float) * kMlpParams));
+    HIPCHECK(hipMalloc(&tr.m, sizeof(float) * kMlpTrainable));
+    HIPCHECK(hipMalloc(&tr.v, sizeof(float) * kMlpTrainable));
+    HIPCHECK(hipMalloc(&tr.bias, sizeof(float) * (2 * kMlpHid + 16)));
+    HIPCHECK(hipMalloc(&tr.tiles, sizeof(uint16_t) * kAllTiles * kTileElems));
+    HIPCHECK(hipMalloc(&tr.X, sizeof(float) * n * kMlpIn));
+    HIPCHECK(hipMalloc(&tr.Xb, sizeof(uint16_t) * n * kMlpIn));
+    if (T) {
+        HIPCHECK(hipMalloc(&tr.T, sizeof(float) * n * kMlpOut));
+        HIPCHECK(hipMalloc(&tr.Tn, sizeof(float) * n * kMlpOut));
+    }
+    HIPCHECK(hipMalloc(&tr.perm, sizeof(int32_t) * std::max<int64_t>(n, batch)));
+    HIPCHECK(hipMalloc(&tr.part, sizeof(float) * (size_t)tr.splits * kGradTiles * 256));
+    HIPCHECK(hipMalloc(&tr.loss, sizeof(float) * (batch / 16)));
+    uint16_t** rows[] = {&tr.tb.XT, &tr.tb.H1T, &tr.tb.H2T, &tr.tb.G1T, &tr.tb.G2T, &tr.tb.GYT};
+    const int nrows[] = {kMlpIn, kMlpHid, kMlpHid, kMlpHid, kMlpHid, 16};
+    for (int k = 0; k < 6; k++) HIPCHECK(hipMalloc(rows[k], sizeof(uint16_t) * nrows[k] * batch));
+    HIPCHECK(hipMemcpyAsync(tr.prm, params, sizeof(float) * kMlpParams, hipMemcpyHostToDevice, tr.st));
+    HIPCHECK(hipMemcpyAsync(tr.X, X, sizeof(float) * n * kMlpIn, hipMemcpyHostToDevice, tr.st));
+    if (T)
+        HIPCHECK(hipMemcpyAsync(tr.T, T, sizeof(float) * n * kMlpOut, hipMemcpyHostToDevice, tr.st));
+    HIPCHECK(hipMemsetAsync(tr.m, 0, sizeof(float) * kMlpTrainable, tr.st));
+    HIPCHECK(hipMemsetAsync(tr.v, 0, sizeof(float) * kMlpTrainable, tr.st));
+    hipLaunchKernelGGL(k_norm_data, dim3(nblk(n * kMlpIn, 256)), dim3(256), 0, tr.st, tr.X, tr.T,
+                       n, tr.prm, tr.Xb, tr.Tn);
+    HIPCHECK(hipGetLastError());
+    return YB_OK;
+}
+
+// One minibatch gradient at the current weights; loss sum into *lossOut.
+int trainer_grad(Trainer& tr, float* lossOut) {
+    hipLaunchKernelGGL(k_mlp_pack, dim3(nblk(kAllTiles * 64, 256)), dim3(256), 0, tr.st, tr.prm,
+                       tr.tiles, tr.bias);
+    const float scale = 1.0f / tr.batch;
+    hipLaunchKernelGGL(k_train_fb, dim3(nblk(tr.batch / 16, 4)), dim3(256), 0, tr.st, tr.tiles,
+                       tr.bias, tr.Xb, tr.Tn, tr.perm, tr.batch, tr.tb, tr.loss, scale);
+    hipLaunchKernelGGL(k_train_wgrad, dim3(kGradTiles * tr.splits), dim3(64), 0, tr.st, tr.tb,
+                       tr.batch, tr.splits, tr.part);
+    hipLaunchKernelGGL(k_train_gsum, dim3(nblk(kGradTiles * 256, 256)), dim3(256), 0, tr.st,
+                       tr.part, tr.splits, tr.grad);
+    hipLaunchKernelGGL(k_train_bgrad, dim3(2 * kMlpHid + kMlpOut), dim3(256), 0, tr.st, tr.tb,
+                       tr.batch, tr.grad);
+    HIPCHECK(hipGetLastError());
+    if (lossOut) {
+        std::vector<float> lp(tr.batch / 16);
+        HIPCHECK(hipMemcpyAsync(lp.data(), tr.loss, sizeof(float) * lp.size(),
+                                hipMemcpyDeviceToHost, tr.st));
+        HIPCHECK(hipStreamSynchronize(tr.st));
+        double s = 0;
+        for (float x : lp) s += x;
+        *lossOut = (float)s;
+    }
+    return YB_OK;
+}
+
+}  // namespace
+
+extern "C" int yb_mlp_forward(const float* X, int64_t n, const float* params, float* Y) {
+    if (!X || !params || !Y || n < 1) return YB_ERR_INVALID_ARGUMENT;
+    Trainer tr;
+    int rc = trainer_init(tr, X, nullptr, n, params, 16);
+    if (rc != YB_OK) return rc;
+    float* dY = nullptr;
+    HIPCHECK(hipMalloc(&dY, sizeof(float) * n * 3));
+    hipLaunchKernelGGL(k_mlp_pack, dim3(nblk(kAllTiles * 64, 256)), dim3(256), 0, tr.st, tr.prm,
+                       tr.tiles, tr.bias);
+    hipLaunchKernelGGL(k_forward_only, dim3(nblk((n + 15) / 16, 4)), dim3(256), 0, tr.st, tr.tiles,
+                       tr.bias, tr.Xb, (int)n, dY);
+    hipError_t e = hipMemcpyAsync(Y, dY, sizeof(float) * n * 3, hipMemcpyDeviceToHost, tr.st);
+    if (e == hipSuccess) e = hipStreamSynchronize(tr.st);
+    (void)hipFree(dY);
+    HIPCHECK(e);
+    return YB_OK;
+}
+
+extern "C" int yb_mlp_grad(const float* X, const float* T, int64_t n, const float* params,
+                           float* grad, float* loss) {
+    if (!X || !T || !params || !grad || n < 16 || n % 16) {
+        set_message("yb_mlp_grad: n must be a positive multiple of 16");
+        return YB_ERR_INVALID_ARGUMENT;
+    }
+    Trainer tr;
+    int rc = trainer_init(tr, X, T, n, params, (int)n);
+    if (rc != YB_OK) return rc;
+    std::vector<int32_t> idx(n);
+    std::iota(idx.begin(), idx.end(), 0);
+    HIPCHECK(hipMemcpyAsync(tr.perm, idx.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, tr.st));
+    rc = trainer_grad(tr, loss);
+    if (rc != YB_OK) return rc;
+    HIPCHECK(hipMemcpyAsync(grad, tr.grad, sizeof(float) * kMlpTrainable, hipMemcpyDeviceToHost,
+                            tr.st));
+    HIPCHECK(hipStreamSynchronize(tr.st));
+    return YB_OK;
+}
+
+extern "C" int yb_mlp_train(const float* X, const float* T, int64_t n, float* params, int epochs,
+                            int batch, float lr, uint32_t seed, float* lossHistory) {
+    if (!X || !T || !params || n < batch || batch < 16 || batch % 16 || epochs < 1) {
+        set_message("yb_mlp_train: need n >= batch, batch a multiple of 16, epochs >= 1");
+        return YB_ERR_INVALID_ARGUMENT;
+    }
+    Trainer tr;
+    int rc = trainer_init(tr, X, T, n, params, batch);
+    if (rc != YB_OK) return rc;
+    std::mt19937 rng(seed);
+    std::vector<int32_t> perm(n);
+    std::iota(perm.begin(), perm.end(), 0);
+    const int steps = (int)(n / batch);
+    int t = 0;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    for (int ep = 0; ep < epochs; ep++) {
+        std::shuffle(perm.begin(), perm.end(), rng);
+        HIPCHECK(hipMemcpyAsync(tr.perm, perm.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice,
+                                tr.st));
+        double epLoss = 0.0;
+        for (int k = 0; k < steps; k++) {
+            // the minibatch is the contiguous slice k of the permutation
+            int32_t* saved = tr.perm;
+            tr.perm = saved + (size_t)k * batch;
+            float l = 0.f;
+            rc = trainer_grad(tr, (k == steps - 1 || lossHistory) ? &l : nullptr);
+            tr.perm = saved;
+            if (rc != YB_OK) return rc;
+            epLoss += l;
+            t++;
+            const float c1 = 1.f - powf(b1, (float)t), c2 = 1.f - powf(b2, (float)t);
+            hipLaunchKernelGGL(k_adam, dim3(nblk(kMlpTrainable, 256)), dim3(256), 0, tr.st, tr.prm,
+                               tr.grad, tr.m, tr.v, kMlpTrainable, lr, b1, b2, eps, c1, c2);
+            HIPCHECK(hipGetLastError());
+        }
+        if (lossHistory) lossHistory[ep] = (float)(epLoss / ((double)steps * batch));
+    }
+    HIPCHECK(hipMemcpyAsync(params, tr.prm, sizeof(float) * kMlpTrainable, hipMemcpyDeviceToHost,
+                            tr.st));
+    HIPCHECK(hipStreamSynchronize(tr.st));
+    return YB_OK;
+}
+
+// ---- simulation-handle entry points -------------------------------------------------
+extern "C" int yb_mlp_load(yb_sim* s, const float* params, int count) {
+    if (!s || !params || count != kMlpParams) {
+        set_message("yb_mlp_load: expected " + std::to_string(kMlpParams) + " floats");
+        return YB_ERR_INVALID_ARGUMENT;
+    }
+    HIPCHECK(hipMemcpyAsync(s->b.mlpPrm, params, sizeof(float) * kMlpParams,
+                            hipMemcpyHostToDevice, s->stream));
+    launch_mlp_pack(s);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    s->mlpLoaded = true;
+    return YB_OK;
+}
+
+extern "C" int yb_set_warm_start(yb_sim* s, int mode) {
+    if (!s || mode < 0 || mode > 1) return YB_ERR_INVALID_ARGUMENT;
+    if (mode == 1 && !s->mlpLoaded) {
+        set_message("yb_set_warm_start: load a model with yb_mlp_load first");
+        return YB_ERR_INVALID_ARGUMENT;
+    }
+    if (mode != s->warmStart) {
+        s->warmStart = mode;
+        s->graphItr = -1;
+        s->adaptiveReady = false;
+    }
+    return YB_OK;
+}
+
+extern "C" int yb_mlp_counters(yb_sim* s, int64_t out[2], int reset) {
+    if (!s || !out) return YB_ERR_INVALID_ARGUMENT;
+    unsigned long long c[2];
+    HIPCHECK(hipMemcpyAsync(c, s->b.mlpCount, sizeof(c), hipMemcpyDeviceToHost, s->stream));
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    out[0] = (int64_t)c[0];
+    out[1] = (int64_t)c[1];
+    if (reset) {
+        HIPCHECK(hipMemsetAsync(s->b.mlpCount, 0, sizeof(c), s->stream));
+        HIPCHECK(hipStreamSynchronize(s->stream));
+    }
+    return YB_OK;
+}
+
+extern "C" int yb_record(yb_sim* s, int64_t capacity) {
+    if (!s || capacity < 0) return YB_ERR_INVALID_ARGUMENT;
+    if (s->b.recX) (void)hipFree(s->b.recX);
+    if (s->b.recT) (void)hipFree(s->b.recT);
+    s->b.recX = s->b.recT = nullptr;
+    s->recCap = capacity;
+    if (capacity > 0) {
+        HIPCHECK(hipMalloc(&s->b.recX, sizeof(float) * capacity * kMlpIn));
+        HIPCHECK(hipMalloc(&s->b.recT, sizeof(float) * capacity * kMlpOut));
+    }
+    HIPCHECK(hipMemsetAsync(s->b.recCount, 0, sizeof(int64_t), s->stream));
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    s->graphItr = -1;
+    s->adaptiveReady = false;
+    return YB_OK;
+}
+
+extern "C" int yb_record_fetch(yb_sim* s, float* X, float* T, int64_t* count) {
+    if (!s || !count) return YB_ERR_INVALID_ARGUMENT;
+    int64_t c = 0;
+    HIPCHECK(hipMemcpyAsync(&c, s->b.recCount, sizeof(c), hipMemcpyDeviceToHost, s->stream));
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    *count = c;
+    if (c > 0 && X)
+        HIPCHECK(hipMemcpyAsync(X, s->b.recX, sizeof(float) * c * kMlpIn, hipMemcpyDeviceToHost,
+                                s->stream));
+    if (c > 0 && T)
+        HIPCHECK(hipMemcpyAsync(T, s->b.recT, sizeof(float) * c * kMlpOut, hipMemcpyDeviceToHost,
+                                s->stream));
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    return YB_OK;
+}

@@ -50,6 +50,23 @@ struct SolverCtl {
 };
 
 constexpr int kMaxCols = YB_MAX_COLLISIONS_PER_SEGMENT;
+
+// Learned warm start: parameter vector layout (floats), see yarnball_hip.h.
+constexpr int kMlpIn = YB_MLP_INPUTS;
+constexpr int kMlpHid = YB_MLP_HIDDEN;
+constexpr int kMlpOut = YB_MLP_OUTPUTS;
+constexpr int kMlpW1 = 0;
+constexpr int kMlpB1 = kMlpW1 + kMlpIn * kMlpHid;
+constexpr int kMlpW2 = kMlpB1 + kMlpHid;
+constexpr int kMlpB2 = kMlpW2 + kMlpHid * kMlpHid;
+constexpr int kMlpW3 = kMlpB2 + kMlpHid;
+constexpr int kMlpB3 = kMlpW3 + kMlpHid * kMlpOut;
+constexpr int kMlpTrainable = kMlpB3 + kMlpOut;
+constexpr int kMlpMean = kMlpTrainable;
+constexpr int kMlpIStd = kMlpMean + kMlpIn;
+constexpr int kMlpOScale = kMlpIStd + kMlpIn;
+constexpr int kMlpParams = kMlpOScale + kMlpOut;
+static_assert(kMlpParams == YB_MLP_PARAMS, "parameter layout");
 constexpr int kSortBits = 8;
 constexpr int kSortBins = 1 << kSortBits;
 constexpr int kSortThreads = 256;
@@ -99,6 +116,17 @@ struct Buffers {
     SolverCtl* ctl = nullptr;
     double* partial = nullptr;    // per-block residual partial sums
     int32_t* hostFlag = nullptr;  // pinned: doneAt read back after each chunk
+    // learned warm start
+    float* mlpPrm = nullptr;          // kMlpParams floats
+    uint16_t* mlpTiles = nullptr;     // packed bf16 weight tiles
+    float* mlpBias = nullptr;         // 2*hidden + 16
+    uint16_t* featB = nullptr;        // n x 48 normalised bf16 features
+    float4* corrPrev = nullptr;       // n: last converged correction (dx* - dx0)/L
+    float4* dxPlain = nullptr;        // n: plain warm start of the current substep
+    unsigned long long* mlpCount = nullptr;  // [applied, fallback]
+    float* recX = nullptr;            // recording: cap x 48 raw features
+    float* recT = nullptr;            // recording: cap x 3 targets
+    int64_t* recCount = nullptr;
     // upload / download staging
     void* stage = nullptr;
     size_t stageBytes = 0;
@@ -135,6 +163,9 @@ struct yb_sim {
     hipEvent_t chunkEv[2] = {nullptr, nullptr};
     void* prof = nullptr;  // eager-profiling event log (capi.hip)
     double denomHost = 0.0;
+    int warmStart = 0;     // 0 plain (iteration.cu:18-34), 1 learned
+    bool mlpLoaded = false;
+    int64_t recCap = 0;
     int graphLimit = -1;
     int graphVel = -1;
     float hostBox[8];
@@ -158,6 +189,11 @@ void launch_residual(yb_sim* s, int sweep);
 void launch_ctl_reset(yb_sim* s);
 void launch_end(yb_sim* s, int fin, bool adaptive);         // iteration.cu:48-61
 int sweep_blocks(int n);
+// learned warm start (mlp.hip)
+void launch_mlp_pack(yb_sim* s);
+void launch_features(yb_sim* s, bool infer, bool record);
+void launch_mlp_apply(yb_sim* s);
+void launch_correction(yb_sim* s, int fin, bool adaptive, bool record);
 void launch_fill_i32(yb_sim* s, int32_t* p, int32_t v, int count);
 // broadphase (collision.cu:10-119 + the LBVH the reference links)
 void launch_aabb(yb_sim* s);
