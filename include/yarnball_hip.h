@@ -176,6 +176,9 @@ int yb_debug_read(yb_sim* s, float* dx, float* aabb, uint32_t* morton, int32_t* 
  * child code bits) after the last refit.
  * Returns YB_ERR_INVALID_ARGUMENT if no tree was built. */
 int yb_debug_tree(yb_sim* s, int32_t* child, int32_t* parent, float* box);
+/* Plain warm start of the last prologue (4n floats) and the first n recorded
+ * feature rows (48n floats) -- for the learned-warm-start tests. */
+int yb_debug_read_mlp(yb_sim* s, float* dxPlain, float* features);
 /* Overwrite the device warm-start/iterate dx (4n floats) and lastVels (3n). */
 int yb_debug_write_dx(yb_sim* s, const float* dx);
 int yb_debug_read_last_vels(yb_sim* s, float* lastVels);
@@ -205,8 +208,10 @@ int yb_solver_stats(yb_sim* s, yb_solver_stats_t* out, int reset);
  * No counterpart in the reference, whose warm start is the gravity
  * extrapolation of iteration.cu:18-34.  A 48 -> 128 -> 128 -> 3 ReLU MLP
  * (bf16 on the matrix cores, fp32 accumulation) predicts each free vertex's
- * correction (dx* - dx0)/L from 48 local features; the corrected dx0, clamped
- * to the step limit, starts the sweeps.  Non-finite predictions keep dx0.
+ * correction (dx* - dx0)/L from 48 local features, as a residual on top of the
+ * previous substep's converged correction (feature 36-38): the applied
+ * correction is c_prev + MLP * output scale.  The corrected dx0, clamped to
+ * the step limit, starts the sweeps.  Non-finite predictions keep dx0.
  * Parameter vector (YB_MLP_PARAMS floats): W1[48][128], b1[128], W2[128][128],
  * b2[128], W3[128][3], b3[3] (YB_MLP_TRAINABLE values), then the feature
  * means[48], inverse standard deviations[48] and output scales[3]. */
@@ -216,17 +221,39 @@ int yb_solver_stats(yb_sim* s, yb_solver_stats_t* out, int reset);
 #define YB_MLP_TRAINABLE 23171
 #define YB_MLP_PARAMS 23270
 int yb_mlp_load(yb_sim* s, const float* params, int count);
-/* mode 0: plain warm start, 1: learned (needs yb_mlp_load). */
+/* mode 0: plain warm start, 1: learned (needs yb_mlp_load), 2: temporal --
+ * dx0 plus the previous substep's converged correction, no network (the
+ * ablation baseline for the learned mode).  After a non-plain warm start the
+ * frames get one closed-form update from the new dx, so the first sweep
+ * starts from consistent positions and frames; OR the mode with
+ * YB_WARM_NO_FRAME_REFRESH to skip that (ablation). */
+#define YB_WARM_NO_FRAME_REFRESH 0x100
+/* The learned correction is low-passed along each strand with a [1 2 1]/4
+ * filter before it is applied; YB_WARM_NO_SMOOTHING skips that (ablation). */
+#define YB_WARM_NO_SMOOTHING 0x200
+/* Substep-level fallback (residual-targeted mode, learned warm start): each
+ * substep starts from the learned or the plain warm start, whichever has the
+ * lower running average of sweeps to convergence; every 8th substep runs the
+ * other one to keep its average current.  The plain solver still decides
+ * convergence either way.  YB_WARM_NO_FALLBACK always uses the learned start. */
+#define YB_WARM_NO_FALLBACK 0x400
 int yb_set_warm_start(yb_sim* s, int mode);
-/* out[0] predictions applied, out[1] fallbacks to the plain warm start. */
+/* Vertex level: out[0] predictions applied, out[1] vertices that kept the
+ * plain warm start because the prediction was not finite. */
 int yb_mlp_counters(yb_sim* s, int64_t out[2], int reset);
+/* Substep level (residual-targeted mode): out[0] substeps started from the
+ * session's warm start, out[1] substeps started from the plain fallback. */
+int yb_warm_start_substeps(yb_sim* s, int64_t out[2], int reset);
 /* Training-data recording: each substep appends its n feature rows (raw,
  * unnormalised) and, after convergence, the n target corrections, until
  * `capacity` rows are stored.  capacity 0 stops recording. */
 int yb_record(yb_sim* s, int64_t capacity);
+/* Pause (0) / resume (1) recording without rebuilding the captured graphs. */
+int yb_record_enable(yb_sim* s, int on);
 int yb_record_fetch(yb_sim* s, float* X, float* T, int64_t* count);
 /* Training on the device (own forward / backward kernels, Adam):
- * X n x 48 raw features, T n x 3 targets; params in/out (normalisation
+ * X n x 48 raw features, T n x 3 targets of the network itself (for the warm
+ * start: the converged correction minus features 36-38); params in/out (normalisation
  * constants are read, the trainable part is updated).  lossHistory (epochs
  * floats, may be null) receives the mean squared error per sample. */
 int yb_mlp_train(const float* X, const float* T, int64_t n, float* params, int epochs, int batch,

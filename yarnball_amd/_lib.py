@@ -114,13 +114,16 @@ SIGNATURES = {
     "yb_debug_write_dx": (_I, [_P, _P]),
     "yb_debug_read_last_vels": (_I, [_P, _P]),
     "yb_debug_write_last_vels": (_I, [_P, _P]),
+    "yb_debug_read_mlp": (_I, [_P, _P, _P]),
     "yb_profile": (_I, [_P, ctypes.POINTER(Params), _F, _I, _P, _P]),
     "yb_set_solver": (_I, [_P, _F, _I, _I, _I]),
     "yb_min_separation": (_I, [_P, _P, _I]),
     "yb_mlp_load": (_I, [_P, _P, _I]),
     "yb_set_warm_start": (_I, [_P, _I]),
     "yb_mlp_counters": (_I, [_P, _P, _I]),
+    "yb_warm_start_substeps": (_I, [_P, _P, _I]),
     "yb_record": (_I, [_P, ctypes.c_int64]),
+    "yb_record_enable": (_I, [_P, _I]),
     "yb_record_fetch": (_I, [_P, _P, _P, ctypes.POINTER(ctypes.c_int64)]),
     "yb_mlp_train": (_I, [_P, _P, ctypes.c_int64, _P, _I, _I, _F, ctypes.c_uint32, _P]),
     "yb_mlp_grad": (_I, [_P, _P, ctypes.c_int64, _P, _P, _P]),

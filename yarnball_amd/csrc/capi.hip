@@ -48,7 +48,7 @@ static void free_all(yb_sim* s) {
                     b.vals[0], b.vals[1], b.hist,     b.nodes,   b.parent,  b.rng,
                     b.kids,    b.strad,   b.stradCount, b.leafNeed, b.sufBox, b.preBox, b.blkTab,
                     b.err,     b.params,  b.stage, b.ctl, b.partial, b.minSep,
-                    b.mlpPrm,  b.mlpTiles, b.mlpBias, b.featB, b.corrPrev, b.dxPlain,
+                    b.mlpPrm,  b.mlpTiles, b.mlpBias, b.featB, b.corrPrev, b.dxPlain, b.predC,
                     b.mlpCount, b.recX, b.recT, b.recCount};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -56,8 +56,11 @@ static void free_all(yb_sim* s) {
     b = Buffers();
     for (int i = 0; i < 3; i++) {
         if (s->graphs[i]) (void)hipGraphExecDestroy(s->graphs[i]);
-        if (s->proGraphs[i]) (void)hipGraphExecDestroy(s->proGraphs[i]);
-        s->graphs[i] = s->proGraphs[i] = nullptr;
+        for (int w = 0; w < 2; w++) {
+            if (s->proGraphs[w][i]) (void)hipGraphExecDestroy(s->proGraphs[w][i]);
+            s->proGraphs[w][i] = nullptr;
+        }
+        s->graphs[i] = nullptr;
     }
     if (s->endGraph) (void)hipGraphExecDestroy(s->endGraph);
     s->endGraph = nullptr;
@@ -154,8 +157,9 @@ extern "C" int yb_create(int n, yb_sim** out) {
     ALLOC(b.featB, N * kMlpIn);
     ALLOC(b.corrPrev, N);
     ALLOC(b.dxPlain, N);
+    ALLOC(b.predC, N);
     ALLOC(b.mlpCount, 2);
-    ALLOC(b.recCount, 1);
+    ALLOC(b.recCount, 2);
     ALLOC(b.partial, (size_t)sweep_blocks(n));
 #undef ALLOC
     if (e == hipSuccess) e = hipHostMalloc((void**)&b.hostFlag, 2 * sizeof(int32_t));
@@ -185,7 +189,7 @@ extern "C" int yb_create(int n, yb_sim** out) {
     if (e == hipSuccess) e = hipMemset(b.nodes, 0, sizeof(float4) * 4 * (N - 1));
     if (e == hipSuccess) e = hipMemset(b.corrPrev, 0, sizeof(float4) * N);
     if (e == hipSuccess) e = hipMemset(b.mlpCount, 0, sizeof(unsigned long long) * 2);
-    if (e == hipSuccess) e = hipMemset(b.recCount, 0, sizeof(int64_t));
+    if (e == hipSuccess) e = hipMemset(b.recCount, 0, 2 * sizeof(int64_t));
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
         set_message(std::string("yb_create: ") + hipGetErrorString(e));
@@ -334,7 +338,10 @@ static void mark(yb_sim* s, int stage) {
 
 // Substep prologue in detection mode m (0 none, 1 refit, 2 rebuild):
 // initItr, detectCollisions, recomputeStepLimit (step.cpp:111-116, 80-81).
-static void enqueue_prologue(yb_sim* s, int mode, bool adaptive) {
+// warm: the warm start of this prologue (0 plain, 1 learned, 2 temporal); a
+// plain prologue of a learned session still writes the plain start for the
+// end-of-substep correction that the next learned substep reads.
+static void enqueue_prologue(yb_sim* s, int mode, bool adaptive, int warm) {
     launch_init(s);
     mark(s, YB_STAGE_INIT);
     if (mode >= 1) {
@@ -351,10 +358,17 @@ static void enqueue_prologue(yb_sim* s, int mode, bool adaptive) {
     }
     launch_step_limit(s);
     mark(s, YB_STAGE_STEP_LIMIT);
-    const bool learned = s->warmStart == 1, record = s->recCap > 0;
-    if (learned || record) {
+    const bool learned = warm == 1, record = s->recCap > 0;
+    if (learned || record || s->warmStart == 1) {
         launch_features(s, learned, record);
         if (learned) launch_mlp_apply(s);
+        mark(s, YB_STAGE_MLP);
+    } else if (warm == 2) {
+        launch_temporal(s);
+        mark(s, YB_STAGE_MLP);
+    }
+    if (warm != 0 && s->frameRefresh) {
+        launch_frame_refresh(s);
         mark(s, YB_STAGE_MLP);
     }
     if (adaptive) {
@@ -381,7 +395,7 @@ static void enqueue_sweep(yb_sim* s, int k, bool adaptive) {
 
 static void enqueue_end(yb_sim* s, int fin, bool adaptive) {
     launch_end(s, fin, adaptive);
-    if (s->warmStart == 1 || s->recCap > 0) launch_correction(s, fin, adaptive, s->recCap > 0);
+    if (s->warmStart != 0 || s->recCap > 0) launch_correction(s, fin, adaptive, s->recCap > 0);
     mark(s, YB_STAGE_END);
 }
 
@@ -408,7 +422,7 @@ static int rebuild_graphs(yb_sim* s, int numItr) {
     HIPCHECK(hipStreamSynchronize(s->stream));
     for (int m = 0; m < 3; m++) {
         int rc = capture(s, &s->graphs[m], [&]() {
-            enqueue_prologue(s, m, false);
+            enqueue_prologue(s, m, false, s->warmStart);
             for (int k = 0; k < numItr; k++) enqueue_sweep(s, k, false);
             enqueue_end(s, numItr & 1, false);
         });
@@ -420,13 +434,28 @@ static int rebuild_graphs(yb_sim* s, int numItr) {
 
 // Residual-targeted mode: prologue graphs, chunk graphs of s->chunk sweeps
 // with the residual reduction after each dx sweep, and the end graph.
+// probe gaps of the substep-level warm-start chooser (choose_start below)
+constexpr int kProbeMin = 8, kProbeMax = 64;
+
 static int rebuild_adaptive_graphs(yb_sim* s) {
     if (s->adaptiveReady) return YB_OK;
     HIPCHECK(hipStreamSynchronize(s->stream));
-    for (int m = 0; m < 3; m++) {
-        int rc = capture(s, &s->proGraphs[m], [&]() { enqueue_prologue(s, m, true); });
-        if (rc != YB_OK) return rc;
-    }
+    const bool twoStarts = s->warmStart == 1 && s->fallback;
+    for (int w = 0; w < 2; w++)
+        for (int m = 0; m < 3; m++) {
+            if (w == 1 && !twoStarts) {
+                if (s->proGraphs[w][m]) (void)hipGraphExecDestroy(s->proGraphs[w][m]);
+                s->proGraphs[w][m] = nullptr;
+                continue;
+            }
+            int rc = capture(s, &s->proGraphs[w][m],
+                             [&]() { enqueue_prologue(s, m, true, w == 0 ? s->warmStart : 0); });
+            if (rc != YB_OK) return rc;
+        }
+    s->avgSweeps[0] = s->avgSweeps[1] = 0.0;
+    s->chooserStep = 0;
+    s->probeGap = kProbeMin;
+    s->nextProbe = kProbeMin;
     if (s->chunkGraphs) {
         for (int c = 0; c < s->numChunks; c++)
             if (s->chunkGraphs[c]) (void)hipGraphExecDestroy(s->chunkGraphs[c]);
@@ -490,9 +519,41 @@ extern "C" int yb_solver_stats(yb_sim* s, yb_solver_stats_t* out, int reset) {
 // One residual-targeted substep: prologue, chunks until converged (the host
 // reads the convergence flag after chunk c while chunk c+1 already runs; a
 // chunk launched after convergence exits immediately), end.
+// Warm start of the next substep of a learned session with the fallback on:
+// 0 = learned, 1 = plain.  The start with the lower running sweep average is
+// used; the other one is probed after s->probeGap substeps, a gap that starts
+// at kProbeMin and doubles (up to kProbeMax) each time the probe loses.
+static int best_start(const yb_sim* s) { return s->avgSweeps[0] <= s->avgSweeps[1] ? 0 : 1; }
+static int choose_start(const yb_sim* s) {
+    if (!(s->warmStart == 1 && s->fallback)) return 0;
+    if (s->avgSweeps[0] == 0.0) return 0;  // first a learned sample,
+    if (s->avgSweeps[1] == 0.0) return 1;  // then a plain one
+    const int best = best_start(s);
+    return s->chooserStep >= s->nextProbe ? 1 - best : best;
+}
+
+static void update_chooser(yb_sim* s, int start, int sweeps) {
+    if (!(s->warmStart == 1 && s->fallback)) return;
+    const bool ready = s->avgSweeps[0] != 0.0 && s->avgSweeps[1] != 0.0;
+    const bool probe = ready && start != best_start(s);
+    // Sweep counts drift within a frame, so a probe's sample replaces the old
+    // value of its start; the start in use keeps a running average (weight 1/2
+    // on the newest sample) over its consecutive substeps.
+    double& avg = s->avgSweeps[start];
+    avg = (avg == 0.0 || probe) ? (double)sweeps : 0.5 * (avg + (double)sweeps);
+    s->chooserStep++;
+    if (probe || !ready) {
+        const bool won = s->avgSweeps[1] != 0.0 && best_start(s) == start;
+        s->probeGap = won ? kProbeMin : std::min(2 * s->probeGap, kProbeMax);
+        s->nextProbe = s->chooserStep + s->probeGap;
+    }
+}
+
 static int adaptive_substep(yb_sim* s, int mode) {
-    HIPCHECK(hipGraphLaunch(s->proGraphs[mode], s->stream));
+    const int start = choose_start(s);
+    HIPCHECK(hipGraphLaunch(s->proGraphs[start][mode], s->stream));
     int launched = 0;
+    int done = s->maxSweeps;  // sweeps of this substep (all of them if not converged)
     auto launch_chunk = [&](int c) -> int {
         HIPCHECK(hipGraphLaunch(s->chunkGraphs[c], s->stream));
         HIPCHECK(hipMemcpyAsync(&s->b.hostFlag[c & 1], &s->b.ctl->doneAt, sizeof(int32_t),
@@ -509,9 +570,14 @@ static int adaptive_substep(yb_sim* s, int mode) {
             if (rc != YB_OK) return rc;
         }
         HIPCHECK(hipEventSynchronize(s->chunkEv[c & 1]));
-        if (s->b.hostFlag[c & 1] != 0x7fffffff) break;
+        if (s->b.hostFlag[c & 1] != 0x7fffffff) {
+            done = s->b.hostFlag[c & 1] + 1;
+            break;
+        }
     }
     HIPCHECK(hipGraphLaunch(s->endGraph, s->stream));
+    s->warmSubsteps[start]++;
+    update_chooser(s, start, done);
     return YB_OK;
 }
 
@@ -655,7 +721,7 @@ extern "C" int yb_debug_prologue(yb_sim* s, yb_params* p, float h, int detect, i
         s->treeBuilt = true;
         if (mode == 2) s->lastBVHRebuild = 0.f;
     }
-    enqueue_prologue(s, mode, false);
+    enqueue_prologue(s, mode, false, s->warmStart);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(s->stream));
     return check_errors(s, nullptr);
@@ -780,7 +846,7 @@ extern "C" int yb_profile(yb_sim* s, yb_params* p, float h, int substeps, double
         const int mode = next_mode(s, p);
         ps.used = 0;
         mark(s, -1);
-        enqueue_prologue(s, mode, adaptive);
+        enqueue_prologue(s, mode, adaptive, s->warmStart);
         int fin = p->numItr & 1;
         if (adaptive) {
             // one sweep at a time so that no early-exit launch is timed

@@ -380,6 +380,52 @@ __global__ void k_ctl_reset(SolverCtl* __restrict__ ctl) {
 }
 
 // ---------------------------------------------------------------------------
+// Frame refresh after a non-plain warm start: one closed-form frame update
+// (cosserat.cu:209-253) from the warm-start dx, q[0] -> q[1], then copied
+// back so that the first sweep sees frames consistent with its dx.
+__global__ __launch_bounds__(256) void k_frames(const DevParams* __restrict__ P,
+                                                const float4* __restrict__ pos,
+                                                const float4* __restrict__ prm,
+                                                const float4* __restrict__ dx,
+                                                const float4* __restrict__ qIn,
+                                                float4* __restrict__ qOut,
+                                                const float4* __restrict__ qRest) {
+    const int n = P->n;
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    const float4 r4 = prm[v];
+    const uint32_t flags = __float_as_uint(r4.w);
+    rot q0 = asrot(qIn[v]);
+    if (!(flags & YB_FLAG_FIX_ORIENTATION) && (flags & YB_FLAG_HAS_NEXT)) {
+        const v3 e = (xyz(pos[v + 1]) - xyz(pos[v])) + (xyz(dx[v + 1]) - xyz(dx[v]));
+        v3 vv = v3{e.x / r4.x, e.y / r4.x, e.z / r4.x};
+        vv *= -2.0f * r4.y;
+        rot b = rot{0.f, 0.f, 0.f, 0.f};
+        if (flags & YB_FLAG_HAS_PREV) {
+            const rot qr = asrot(qRest[v - 1]);
+            const rot qq = asrot(qIn[v - 1]);
+            const float sg = rdot(rmul(rinv(qq), q0), qr) > 0.0f ? 1.0f : -1.0f;
+            b = radd(b, rscale(rmul(qq, qr), sg));
+        }
+        if (flags & YB_FLAG_HAS_NEXT_ORIENTATION) {
+            const rot qr = asrot(qRest[v]);
+            const rot qq = asrot(qIn[v + 1]);
+            const float sg = rdot(rmul(rinv(q0), qq), qr) > 0.0f ? 1.0f : -1.0f;
+            b = radd(b, rscale(rmul(qq, rinv(qr)), sg));
+        }
+        const float lambda = len(vv) + rlen(b);
+        const rot t = rmul(rmul(mkrot(vv, 0.0f), b), rot{1.f, 0.f, 0.f, 0.f});
+        q0 = rnormalize(radd(t, rscale(b, lambda)));
+    }
+    qOut[v] = f4r(q0);
+}
+
+__global__ void k_copy4(const float4* __restrict__ src, float4* __restrict__ dst, int n) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n) dst[v] = src[v];
+}
+
+// ---------------------------------------------------------------------------
 // End of substep (iteration.cu:48-61).
 // When the sweep count is odd the final frames sit in the second buffer and are
 // copied back so that every substep starts from q[0].
@@ -534,6 +580,13 @@ void launch_sweep(yb_sim* s, int sweep, bool adaptive, bool partials) {
 void launch_residual(yb_sim* s, int sweep) {
     hipLaunchKernelGGL(k_residual, dim3(1), dim3(1024), 0, s->stream, s->b.ctl, s->b.partial,
                        sweep_blocks(s->n), sweep, s->tol, s->maxSweeps);
+}
+
+void launch_frame_refresh(yb_sim* s) {
+    hipLaunchKernelGGL(k_frames, dim3(blocks(s->n, 256)), dim3(256), 0, s->stream, s->b.params,
+                       s->b.pos, s->b.prm, s->b.dx[0], s->b.q[0], s->b.q[1], s->b.qRest);
+    hipLaunchKernelGGL(k_copy4, dim3(blocks(s->n, 256)), dim3(256), 0, s->stream, s->b.q[1],
+                       s->b.q[0], s->n);
 }
 
 void launch_ctl_reset(yb_sim* s) {

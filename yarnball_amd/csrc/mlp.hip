@@ -4,8 +4,10 @@
 //
 // A 48 -> 128 -> 128 -> 3 MLP (ReLU) predicts, per vertex, the correction
 // c = (dx* - dx0) / L between the converged iterate dx* and the plain warm
-// start dx0, in units of the local rest length L.  The prediction replaces
-// dx0 before the first sweep; the residual-targeted sweeps then run to the
+// start dx0, in units of the local rest length L, as a residual on top of the
+// previous substep's converged correction c_prev (also one of its inputs):
+// c = c_prev + MLP(features).  The prediction replaces dx0 before the first
+// sweep; the residual-targeted sweeps then run to the
 // same tolerance as without it, so the plain solver remains the judge of the
 // final state.  A prediction that is not finite keeps dx0 (counted as a
 // fallback); every prediction is clamped to the step limit like a sweep.
@@ -243,7 +245,7 @@ __global__ __launch_bounds__(256) void k_features(
     const float kst = r4.y * iL * iL * h * h * pos[v].w;
     f[47] = log10f(1.0f + kst) * 0.25f;
 
-    if (recX) {
+    if (recX && recCount[1]) {
         const int64_t slot = *recCount + v;
         if (slot < recCap)
             for (int k = 0; k < kMlpIn; k++) recX[slot * kMlpIn + k] = f[k];
@@ -265,8 +267,8 @@ __global__ __launch_bounds__(256) void k_mlp_apply(const DevParams* __restrict__
                                                    const uint16_t* __restrict__ featB,
                                                    const float4* __restrict__ pos,
                                                    const float4* __restrict__ prm,
-                                                   float4* __restrict__ dx,
-                                                   unsigned long long* __restrict__ counters) {
+                                                   const float4* __restrict__ corrPrev,
+                                                   float4* __restrict__ predC) {
     __shared__ uint16_t tiles[kFwdTiles * kTileElems];
     __shared__ float bias[2 * kMlpHid + 16];
     for (int k = threadIdx.x; k < kFwdTiles * kTileElems / 8; k += blockDim.x)
@@ -279,7 +281,6 @@ __global__ __launch_bounds__(256) void k_mlp_apply(const DevParams* __restrict__
     const int wavesTotal = gridDim.x * (blockDim.x >> 6);
     const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const float o0 = mlpPrm[kMlpOScale], o1 = mlpPrm[kMlpOScale + 1], o2 = mlpPrm[kMlpOScale + 2];
-    unsigned applied = 0, fallback = 0;
     for (int base = wave * 16; base < n; base += wavesTotal * 16) {
         const int v = base + r;
         s4_t x[3];
@@ -293,27 +294,53 @@ __global__ __launch_bounds__(256) void k_mlp_apply(const DevParams* __restrict__
         Fwd F;
         mlp_forward(tiles, bias, x, F);
         if (g == 0 && v < n) {
-            const float4 p4 = pos[v];
-            if (p4.w != 0.0f) {
-                const uint32_t fl = __float_as_uint(prm[v].w);
-                const float L = rest_len(prm, v, fl);
-                float4 d4 = dx[v];
-                const v3 c = v3{F.y.x * o0, F.y.y * o1, F.y.z * o2} * L;
-                if (isfinite(c.x) && isfinite(c.y) && isfinite(c.z)) {
-                    v3 nd = xyz4(d4) + c;
-                    const float l = len(nd);
-                    if (l > d4.w && l > 0.0f) nd *= d4.w / l;
-                    d4.x = nd.x;
-                    d4.y = nd.y;
-                    d4.z = nd.z;
-                    dx[v] = d4;
-                    applied++;
-                } else {
-                    fallback++;
-                }
-            }
+            // skip connection: previous converged correction + learned residual
+            const float4 cp = corrPrev[v];
+            predC[v] = make_float4(cp.x + F.y.x * o0, cp.y + F.y.y * o1, cp.z + F.y.z * o2, 0.f);
         }
     }
+}
+
+// Low-pass the predicted correction along each strand with the binomial
+// [1 2 1]/4 filter (a missing neighbour counts as the vertex itself), then
+// apply it: dx0 + L c, clamped to the step limit.  Block-Jacobi damps the
+// alternating (odd-even) mode along a strand as slowly as the smoothest one,
+// and a per-vertex regression error is close to white noise; the filter
+// removes that mode from the prediction before the sweeps see it.
+__global__ __launch_bounds__(256) void k_mlp_smooth_apply(const DevParams* __restrict__ P,
+                                                          const float4* __restrict__ predC,
+                                                          const float4* __restrict__ pos,
+                                                          const float4* __restrict__ prm,
+                                                          float4* __restrict__ dx,
+                                                          unsigned long long* __restrict__ counters,
+                                                          int passes) {
+    const int n = P->n;
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned applied = 0, fallback = 0;
+    if (v < n && pos[v].w != 0.0f) {
+        const uint32_t fl = __float_as_uint(prm[v].w);
+        const float4 c0 = predC[v];
+        v3 c = v3{c0.x, c0.y, c0.z};
+        if (passes > 0) {
+            const float4 cm = (fl & YB_FLAG_HAS_PREV) ? predC[v - 1] : c0;
+            const float4 cq = (fl & YB_FLAG_HAS_NEXT) ? predC[v + 1] : c0;
+            c = v3{0.25f * (cm.x + 2.0f * c0.x + cq.x), 0.25f * (cm.y + 2.0f * c0.y + cq.y),
+                   0.25f * (cm.z + 2.0f * c0.z + cq.z)};
+        }
+        const float L = rest_len(prm, v, fl);
+        float4 d4 = dx[v];
+        const v3 cL = c * L;
+        if (isfinite(cL.x) && isfinite(cL.y) && isfinite(cL.z)) {
+            v3 nd = xyz4(d4) + cL;
+            const float l = len(nd);
+            if (l > d4.w && l > 0.0f) nd *= d4.w / l;
+            dx[v] = make_float4(nd.x, nd.y, nd.z, d4.w);
+            applied = 1;
+        } else {
+            fallback = 1;
+        }
+    }
+    const int lane = threadIdx.x & 63;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         applied += __shfl_xor(applied, off, 64);
@@ -323,6 +350,29 @@ __global__ __launch_bounds__(256) void k_mlp_apply(const DevParams* __restrict__
         atomicAdd(&counters[0], (unsigned long long)applied);
         atomicAdd(&counters[1], (unsigned long long)fallback);
     }
+}
+
+// ---- temporal warm start (ablation baseline, no network) --------------------------
+// dx0 + L * (previous substep's converged correction), clamped to the step limit.
+__global__ __launch_bounds__(256) void k_temporal(const DevParams* __restrict__ P,
+                                                  const float4* __restrict__ pos,
+                                                  const float4* __restrict__ prm,
+                                                  const float4* __restrict__ corrPrev,
+                                                  float4* __restrict__ dx,
+                                                  float4* __restrict__ dxPlain) {
+    const int n = P->n;
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    float4 d4 = dx[v];
+    dxPlain[v] = d4;
+    if (pos[v].w == 0.0f) return;
+    const uint32_t fl = __float_as_uint(prm[v].w);
+    const float L = rest_len(prm, v, fl);
+    const float4 c = corrPrev[v];
+    v3 nd = xyz4(d4) + v3{c.x, c.y, c.z} * L;
+    const float l = len(nd);
+    if (l > d4.w && l > 0.0f) nd *= d4.w / l;
+    dx[v] = make_float4(nd.x, nd.y, nd.z, d4.w);
 }
 
 // ---- end of substep: converged correction ---------------------------------------
@@ -348,7 +398,7 @@ __global__ __launch_bounds__(256) void k_correction(const DevParams* __restrict_
     const float iL = 1.0f / rest_len(prm, v, fl);
     const float4 c = make_float4((d.x - p.x) * iL, (d.y - p.y) * iL, (d.z - p.z) * iL, 0.f);
     corrPrev[v] = c;
-    if (recT) {
+    if (recT && recCount[1]) {
         const int64_t slot = *recCount + v;
         if (slot < recCap) {
             recT[slot * 3 + 0] = c.x;
@@ -359,7 +409,7 @@ __global__ __launch_bounds__(256) void k_correction(const DevParams* __restrict_
 }
 
 __global__ void k_rec_advance(int64_t* __restrict__ recCount, int n, int64_t recCap) {
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0 && recCount[1]) {
         const int64_t c = *recCount + n;
         *recCount = c < recCap ? c : recCap;
     }
@@ -574,10 +624,18 @@ void launch_features(yb_sim* s, bool infer, bool record) {
                        record ? s->b.recX : nullptr, s->b.recCount, s->recCap, s->b.dxPlain);
 }
 
+void launch_temporal(yb_sim* s) {
+    hipLaunchKernelGGL(k_temporal, dim3(nblk(s->n, 256)), dim3(256), 0, s->stream, s->b.params,
+                       s->b.pos, s->b.prm, s->b.corrPrev, s->b.dx[0], s->b.dxPlain);
+}
+
 void launch_mlp_apply(yb_sim* s) {
     hipLaunchKernelGGL(k_mlp_apply, dim3(std::min(nblk(s->n, 64), 2048)), dim3(256), 0, s->stream,
                        s->b.params, s->b.mlpTiles, s->b.mlpBias, s->b.mlpPrm, s->b.featB, s->b.pos,
-                       s->b.prm, s->b.dx[0], s->b.mlpCount);
+                       s->b.prm, s->b.corrPrev, s->b.predC);
+    hipLaunchKernelGGL(k_mlp_smooth_apply, dim3(nblk(s->n, 256)), dim3(256), 0, s->stream,
+                       s->b.params, s->b.predC, s->b.pos, s->b.prm, s->b.dx[0], s->b.mlpCount,
+                       s->mlpSmooth);
 }
 
 void launch_correction(yb_sim* s, int fin, bool adaptive, bool record) {
@@ -784,7 +842,18 @@ extern "C" int yb_mlp_load(yb_sim* s, const float* params, int count) {
 }
 
 extern "C" int yb_set_warm_start(yb_sim* s, int mode) {
-    if (!s || mode < 0 || mode > 1) return YB_ERR_INVALID_ARGUMENT;
+    if (!s || (mode & 0xff) < 0 || (mode & 0xff) > 2) return YB_ERR_INVALID_ARGUMENT;
+    const int refresh = (mode & YB_WARM_NO_FRAME_REFRESH) ? 0 : 1;
+    const int smooth = (mode & YB_WARM_NO_SMOOTHING) ? 0 : 1;
+    const int fallback = (mode & YB_WARM_NO_FALLBACK) ? 0 : 1;
+    mode &= 0xff;
+    if (refresh != s->frameRefresh || smooth != s->mlpSmooth || fallback != s->fallback) {
+        s->frameRefresh = refresh;
+        s->mlpSmooth = smooth;
+        s->fallback = fallback;
+        s->graphItr = -1;
+        s->adaptiveReady = false;
+    }
     if (mode == 1 && !s->mlpLoaded) {
         set_message("yb_set_warm_start: load a model with yb_mlp_load first");
         return YB_ERR_INVALID_ARGUMENT;
@@ -794,6 +863,14 @@ extern "C" int yb_set_warm_start(yb_sim* s, int mode) {
         s->graphItr = -1;
         s->adaptiveReady = false;
     }
+    return YB_OK;
+}
+
+extern "C" int yb_warm_start_substeps(yb_sim* s, int64_t out[2], int reset) {
+    if (!s || !out) return YB_ERR_INVALID_ARGUMENT;
+    out[0] = s->warmSubsteps[0];
+    out[1] = s->warmSubsteps[1];
+    if (reset) s->warmSubsteps[0] = s->warmSubsteps[1] = 0;
     return YB_OK;
 }
 
@@ -821,10 +898,20 @@ extern "C" int yb_record(yb_sim* s, int64_t capacity) {
         HIPCHECK(hipMalloc(&s->b.recX, sizeof(float) * capacity * kMlpIn));
         HIPCHECK(hipMalloc(&s->b.recT, sizeof(float) * capacity * kMlpOut));
     }
-    HIPCHECK(hipMemsetAsync(s->b.recCount, 0, sizeof(int64_t), s->stream));
+    const int64_t init[2] = {0, capacity > 0 ? 1 : 0};
+    HIPCHECK(hipMemcpyAsync(s->b.recCount, init, sizeof(init), hipMemcpyHostToDevice, s->stream));
     HIPCHECK(hipStreamSynchronize(s->stream));
     s->graphItr = -1;
     s->adaptiveReady = false;
+    return YB_OK;
+}
+
+extern "C" int yb_record_enable(yb_sim* s, int on) {
+    if (!s || s->recCap <= 0) return YB_ERR_INVALID_ARGUMENT;
+    const int64_t flag = on ? 1 : 0;
+    HIPCHECK(hipMemcpyAsync(s->b.recCount + 1, &flag, sizeof(flag), hipMemcpyHostToDevice,
+                            s->stream));
+    HIPCHECK(hipStreamSynchronize(s->stream));
     return YB_OK;
 }
 
@@ -840,6 +927,25 @@ extern "C" int yb_record_fetch(yb_sim* s, float* X, float* T, int64_t* count) {
     if (c > 0 && T)
         HIPCHECK(hipMemcpyAsync(T, s->b.recT, sizeof(float) * c * kMlpOut, hipMemcpyDeviceToHost,
                                 s->stream));
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    return YB_OK;
+}
+
+// Stage-level access for the tests: the plain warm start of the last prologue
+// (4n floats) and the first n recorded feature rows (48n floats, raw).
+extern "C" int yb_debug_read_mlp(yb_sim* s, float* dxPlain, float* features) {
+    if (!s) return YB_ERR_INVALID_ARGUMENT;
+    if (dxPlain)
+        HIPCHECK(hipMemcpyAsync(dxPlain, s->b.dxPlain, sizeof(float4) * s->n, hipMemcpyDeviceToHost,
+                                s->stream));
+    if (features) {
+        if (!s->b.recX || s->recCap < s->n) {
+            set_message("yb_debug_read_mlp: recording buffer smaller than n");
+            return YB_ERR_INVALID_ARGUMENT;
+        }
+        HIPCHECK(hipMemcpyAsync(features, s->b.recX, sizeof(float) * s->n * kMlpIn,
+                                hipMemcpyDeviceToHost, s->stream));
+    }
     HIPCHECK(hipStreamSynchronize(s->stream));
     return YB_OK;
 }

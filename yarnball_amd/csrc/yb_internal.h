@@ -123,10 +123,11 @@ struct Buffers {
     uint16_t* featB = nullptr;        // n x 48 normalised bf16 features
     float4* corrPrev = nullptr;       // n: last converged correction (dx* - dx0)/L
     float4* dxPlain = nullptr;        // n: plain warm start of the current substep
+    float4* predC = nullptr;          // n: predicted correction before smoothing
     unsigned long long* mlpCount = nullptr;  // [applied, fallback]
     float* recX = nullptr;            // recording: cap x 48 raw features
     float* recT = nullptr;            // recording: cap x 3 targets
-    int64_t* recCount = nullptr;
+    int64_t* recCount = nullptr;      // [rows stored, recording enabled]
     // upload / download staging
     void* stage = nullptr;
     size_t stageBytes = 0;
@@ -155,7 +156,9 @@ struct yb_sim {
     int maxSweeps = 0;
     int chunk = 8;
     int checkEvery = 1;
-    hipGraphExec_t proGraphs[3] = {nullptr, nullptr, nullptr};
+    // [0]: prologue with the session's warm start; [1]: its plain-start variant
+    // (learned sessions with the substep-level fallback)
+    hipGraphExec_t proGraphs[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
     hipGraphExec_t endGraph = nullptr;
     hipGraphExec_t* chunkGraphs = nullptr;
     int numChunks = 0;
@@ -163,7 +166,17 @@ struct yb_sim {
     hipEvent_t chunkEv[2] = {nullptr, nullptr};
     void* prof = nullptr;  // eager-profiling event log (capi.hip)
     double denomHost = 0.0;
-    int warmStart = 0;     // 0 plain (iteration.cu:18-34), 1 learned
+    int warmStart = 0;     // 0 plain (iteration.cu:18-34), 1 learned, 2 temporal
+    int frameRefresh = 1;  // refresh frames after a non-plain warm start
+    int mlpSmooth = 1;     // [1 2 1]/4 passes on the predicted correction (0 or 1)
+    // Substep-level fallback of a learned session in residual-targeted mode:
+    // the chooser runs the warm start whose sweep-count average is lower and
+    // probes the other one from time to time.
+    int fallback = 1;
+    double avgSweeps[2] = {0.0, 0.0};  // [0] learned, [1] plain; 0 = no sample yet
+    int64_t chooserStep = 0, nextProbe = 8;
+    int probeGap = 8;
+    int64_t warmSubsteps[2] = {0, 0};  // substeps run with [0] learned, [1] plain start
     bool mlpLoaded = false;
     int64_t recCap = 0;
     int graphLimit = -1;
@@ -187,12 +200,14 @@ void launch_separation(yb_sim* s);              // contact distances of the curr
 void launch_sweep(yb_sim* s, int sweep, bool adaptive, bool partials);  // cosserat.cu:20-253
 void launch_residual(yb_sim* s, int sweep);
 void launch_ctl_reset(yb_sim* s);
+void launch_frame_refresh(yb_sim* s);  // closed-form frame update from dx[0]
 void launch_end(yb_sim* s, int fin, bool adaptive);         // iteration.cu:48-61
 int sweep_blocks(int n);
 // learned warm start (mlp.hip)
 void launch_mlp_pack(yb_sim* s);
 void launch_features(yb_sim* s, bool infer, bool record);
 void launch_mlp_apply(yb_sim* s);
+void launch_temporal(yb_sim* s);
 void launch_correction(yb_sim* s, int fin, bool adaptive, bool record);
 void launch_fill_i32(yb_sim* s, int32_t* p, int32_t v, int count);
 // broadphase (collision.cu:10-119 + the LBVH the reference links)

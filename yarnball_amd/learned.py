@@ -34,6 +34,15 @@ def unpack(p):
             p[W3:B3].reshape(H, O), p[B3:MEAN], p[MEAN:ISTD], p[ISTD:OSCALE], p[OSCALE:])
 
 
+CPREV = slice(36, 39)  # feature columns of the previous converged correction
+
+
+def residual_targets(X, C):
+    """Network targets for recorded corrections C: the model adds its output to
+    the previous substep's correction (features 36-38)."""
+    return (np.asarray(C, np.float32) - np.asarray(X, np.float32)[:, CPREV]).astype(np.float32)
+
+
 def init_params(X, T, seed=0):
     """He-initialised weights plus normalisation constants from the data."""
     rng = np.random.default_rng(seed)
@@ -53,6 +62,10 @@ def record(sim, capacity):
     check(sim._lib.yb_record(sim.handle, int(capacity)))
 
 
+def record_enable(sim, on):
+    check(sim._lib.yb_record_enable(sim.handle, int(bool(on))))
+
+
 def fetch(sim):
     n = ctypes.c_int64(0)
     check(sim._lib.yb_record_fetch(sim.handle, None, None, ctypes.byref(n)))
@@ -63,6 +76,7 @@ def fetch(sim):
 
 
 def train(X, T, params, epochs=8, batch=16384, lr=1e-3, seed=0):
+    """Device training (Adam); returns (params, per-epoch mean loss)."""
     lib = _lib.load()
     p = np.ascontiguousarray(params, np.float32).copy()
     hist = np.zeros(epochs, np.float32)
@@ -91,15 +105,37 @@ def grad(X, T, params):
     return g, loss.value
 
 
-def use(sim, params):
-    """Load a model into a simulation and switch its warm start to learned."""
+NO_FRAME_REFRESH = 0x100
+NO_SMOOTHING = 0x200
+NO_FALLBACK = 0x400
+
+
+def use(sim, params, frame_refresh=True, smooth=True, fallback=True):
+    """Load a model into a simulation and switch its warm start to learned.
+
+    fallback: in residual-targeted mode, start each substep from whichever of
+    the learned and the plain warm start has needed fewer sweeps lately."""
     p = np.ascontiguousarray(params, np.float32)
     check(sim._lib.yb_mlp_load(sim.handle, ptr(p), MLP_PARAMS))
-    check(sim._lib.yb_set_warm_start(sim.handle, 1))
+    mode = (1 | (0 if frame_refresh else NO_FRAME_REFRESH) | (0 if smooth else NO_SMOOTHING)
+            | (0 if fallback else NO_FALLBACK))
+    check(sim._lib.yb_set_warm_start(sim.handle, mode))
+
+
+def warm_substeps(sim, reset=False):
+    """Substeps started from the session's warm start and from the plain fallback."""
+    out = np.zeros(2, np.int64)
+    check(sim._lib.yb_warm_start_substeps(sim.handle, ptr(out), int(bool(reset))))
+    return {"learned": int(out[0]), "plain": int(out[1])}
 
 
 def plain(sim):
     check(sim._lib.yb_set_warm_start(sim.handle, 0))
+
+
+def temporal(sim, frame_refresh=True):
+    """Warm start from the previous substep's converged correction (no network)."""
+    check(sim._lib.yb_set_warm_start(sim.handle, 2 | (0 if frame_refresh else NO_FRAME_REFRESH)))
 
 
 def counters(sim, reset=False):
