@@ -7,14 +7,19 @@ opposite edge pulled at 5 cm/s, gravity -z), 1/30 s frames, maxH = 1e-3
 (34 substeps per frame), detection every substep, numItr sweeps per substep.
 
 A step is one frame: Sim.advance(1/30) over the whole patch, state resident
-in HBM.  value = vertices x substeps / second over the timed frames.
+in HBM, solved every substep to the relative Newton residual 1e-6.  The plain
+warm start and the learned one (the committed model, with the substep-level
+plain fallback) run the same frames from the same initial state; value =
+vertices x substeps / second of the learned run over the timed frames, with
+the plain run's rate and the ratio beside it.
 
 Also reported:
   roofline     the dominant kernel's algorithmic bytes per launch / its average
                launch time, measured live with HIP events on the stepper's
                stream (yb_profile), against MI355X's 8 TB/s HBM peak;
   cpu_baseline the CPU oracle (FP64, OpenMP) on a bounded sample of the same
-               workload, on this host's cores.
+               workload, on this host's cores: one substep with its sweeps
+               capped at the plain GPU run's mean per substep.
 """
 import argparse
 import ctypes
@@ -67,14 +72,66 @@ def cpu_baseline(sc, substeps, threads, residual, max_sweeps):
     if residual > 0:
         orc.set_solver(residual, max_sweeps)
     p = sc.params.copy()
-    # one untimed substep (first-touch of the oracle's arrays)
+    # one untimed substep at 2 sweeps (first-touch of the oracle's arrays)
+    if residual > 0:
+        orc.set_solver(residual, 2)
     orc.advance(p, sc.maxH, sc.maxH)
+    if residual > 0:
+        orc.set_solver(residual, max_sweeps)
     t0 = time.perf_counter()
+    sweeps = 0
     for _ in range(substeps):
         orc.advance(p, sc.maxH, sc.maxH)
+        sweeps += lib.orc_last_sweeps(orc.o) if residual > 0 else int(p.numItr)
     dt = time.perf_counter() - t0
     orc.close()
-    return sc.n * substeps / dt, used, dt
+    return sc.n * substeps / dt, used, dt, sweeps
+
+
+def run_method(sc, args, method, model):
+    """Warm-up frames, then the timed frames, of one warm-start method on a
+    fresh simulation; returns the open simulation and the measurements."""
+    from yarnball_amd import Sim, learned, scenes
+
+    sim = Sim(sc.n)
+    scenes.load_into(sim, sc)
+    if args.residual > 0:
+        sim.setSolver(args.residual, maxSweeps=args.max_sweeps, chunk=16, checkEvery=4)
+    if method == "learned":
+        learned.use(sim, model)
+    frame = np.float32(sc.frame_dt)
+    substeps_per_frame = int(np.ceil(frame / np.float32(sc.maxH)))
+    for w in range(args.warmup):
+        sim.advance(frame)
+        print(f"[bench] {method}: warmup frame {w} done", file=sys.stderr, flush=True)
+    if args.residual > 0:
+        sim.solverStats(reset=True)
+    if method == "learned":
+        learned.warm_substeps(sim, reset=True)
+        learned.counters(sim, reset=True)
+    sim.minSeparation(reset=True)
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        sim.advance(frame)  # synchronous: returns after the stream has drained
+        print(f"[bench] {method}: frame {k} done", file=sys.stderr, flush=True)
+    elapsed = time.perf_counter() - t0
+    r = {"sim": sim, "elapsed": elapsed, "substeps_per_frame": substeps_per_frame,
+         "value": sc.n * substeps_per_frame * args.steps / elapsed,
+         "min_separation": sim.minSeparation()}
+    if args.residual > 0:
+        st = sim.solverStats()
+        r["stats"] = st
+        r["sweeps_per_substep"] = st["sweeps"] / max(st["substeps"], 1)
+        print(f"[bench] {method}: {r['sweeps_per_substep']:.1f} sweeps/substep, max "
+              f"{st['maxSweepsUsed']}, not converged {st['notConverged']}, max final residual "
+              f"{st['maxResidual']:.2e}", file=sys.stderr, flush=True)
+    if method == "learned":
+        r["warm_substeps"] = learned.warm_substeps(sim)
+        r["vertex_counters"] = learned.counters(sim)
+    print(f"[bench] {method}: timed {args.steps} frames in {elapsed:.3f}s "
+          f"({r['value']:.4e} vertex-substeps/s), min separation {r['min_separation']:.3f} x 2r",
+          file=sys.stderr, flush=True)
+    return r
 
 
 def main():
@@ -91,47 +148,40 @@ def main():
     ap.add_argument("--residual", type=float, default=1e-6,
                     help="relative Newton residual target; <= 0 runs numItr sweeps")
     ap.add_argument("--max-sweeps", type=int, default=6000)
+    ap.add_argument("--method", choices=["plain", "learned", "both"], default="both",
+                    help="warm start(s) to run; value is the last one's rate")
+    ap.add_argument("--model", default=os.path.join(ROOT, "yarnball_amd", "models",
+                                                    "warmstart.npy"))
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the stepper runs on a single MI355X "
               "(multi-GPU sharding is design-only)", file=sys.stderr)
         sys.exit(2)
 
-    from yarnball_amd import Sim, scenes
     from yarnball_amd._lib import STAGES, check, ptr
 
     t_build = time.perf_counter()
     sc = build_workload(args.workload, args.numItr)
     print(f"[bench] workload {sc.name}: {sc.n} vertices, built in "
           f"{time.perf_counter() - t_build:.1f}s", file=sys.stderr, flush=True)
-
-    sim = Sim(sc.n)
-    scenes.load_into(sim, sc)
-    if args.residual > 0:
-        sim.setSolver(args.residual, maxSweeps=args.max_sweeps, chunk=16, checkEvery=4)
+    methods = ["plain", "learned"] if args.method == "both" else [args.method]
+    model = None
+    if "learned" in methods:
+        model = np.load(args.model, allow_pickle=False)
+    runs = {}
+    for m in methods:
+        runs[m] = run_method(sc, args, m, model)
+    main_m = methods[-1]
+    ref = runs["plain" if "plain" in runs else main_m]
+    for m in methods:
+        if runs[m] is not ref:
+            runs[m]["sim"].close()
+    sim = ref["sim"]
     frame = np.float32(sc.frame_dt)
-    substeps_per_frame = int(np.ceil(frame / np.float32(sc.maxH)))
-
-    for w in range(args.warmup):
-        sim.advance(frame)
-        print(f"[bench] warmup frame {w} done", file=sys.stderr, flush=True)
-
-    if args.residual > 0:
-        sim.solverStats(reset=True)
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        sim.advance(frame)  # synchronous: returns after the stream has drained
-        print(f"[bench] frame {k} done", file=sys.stderr, flush=True)
-    t1 = time.perf_counter()
-    stats = sim.solverStats() if args.residual > 0 else None
-    if stats:
-        print(f"[bench] solver: {stats['sweeps'] / max(stats['substeps'], 1):.1f} sweeps/substep, "
-              f"max {stats['maxSweepsUsed']}, not converged {stats['notConverged']}, "
-              f"max final residual {stats['maxResidual']:.2e}", file=sys.stderr, flush=True)
-    elapsed = t1 - t0
-    vsub = sc.n * substeps_per_frame * args.steps
-    value = vsub / elapsed
-    print(f"[bench] timed {args.steps} frames in {elapsed:.3f}s", file=sys.stderr, flush=True)
+    substeps_per_frame = ref["substeps_per_frame"]
+    value = runs[main_m]["value"]
+    elapsed = runs[main_m]["elapsed"]
+    stats = runs[main_m].get("stats")
 
     # Live per-stage timing on the stepper's stream.
     ms = np.zeros(len(STAGES), np.float64)
@@ -174,17 +224,38 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        v, used, dt = cpu_baseline(sc, args.cpu_substeps, args.cpu_threads, args.residual,
-                                   args.max_sweeps)
+        cap = args.max_sweeps
+        if args.residual > 0:
+            cap = max(1, int(round(ref["sweeps_per_substep"])))
+        v, used, dt, sw = cpu_baseline(sc, args.cpu_substeps, args.cpu_threads, args.residual, cap)
         cpu = {"value": round(v, 1), "unit": "vertex-substeps/s", "cores": int(used),
                "kind": "port",
-               "sample": f"FP64 OpenMP oracle, {args.cpu_substeps} substeps of {sc.name}, "
-                         f"same solver schedule ({dt:.1f}s)"}
+               "sample": f"FP64 OpenMP oracle, {args.cpu_substeps} substep(s) of {sc.name} from "
+                         f"its initial state, {sw} sweeps (residual target {args.residual:g}, "
+                         f"sweeps capped at the plain GPU run's mean per substep), {dt:.1f}s"}
         print(f"[bench] cpu baseline {v:.3e} vertex-substeps/s on {used} threads",
               file=sys.stderr)
 
+    method_rows = {}
+    for m, r in runs.items():
+        row = {"value": round(r["value"], 1), "ms_per_step": round(r["elapsed"] / args.steps * 1e3, 3),
+               "min_separation_x2r": round(r["min_separation"], 3)}
+        if "stats" in r:
+            row["sweeps_per_substep"] = round(r["sweeps_per_substep"], 1)
+            row["not_converged_substeps"] = int(r["stats"]["notConverged"])
+            row["max_final_residual"] = float(r["stats"]["maxResidual"])
+        if m == "learned":
+            ws = r["warm_substeps"]
+            row["substeps_learned_start"] = ws["learned"]
+            row["substeps_plain_fallback"] = ws["plain"]
+            row["fallback_rate"] = round(ws["plain"] / max(ws["learned"] + ws["plain"], 1), 3)
+            row["vertex_fallbacks"] = r["vertex_counters"]["fallback"]
+        method_rows[m] = row
+    speedup = (round(runs["learned"]["value"] / runs["plain"]["value"], 3)
+               if "learned" in runs and "plain" in runs else None)
+
     out = {
-        "metric": "vertex-substeps/s",
+        "metric": "vertex-substeps/s (plain vs learned warm-start, equal residual) + HBM GB/s",
         "value": round(value, 1),
         "unit": "vertex-substeps/s",
         "n_gpus": 1,
@@ -205,9 +276,12 @@ def main():
             "mean_sweeps_per_substep": (round(stats["sweeps"] / max(stats["substeps"], 1), 1)
                                         if stats else int(sim.meta.numItr)),
             "not_converged_substeps": stats["notConverged"] if stats else None,
-            "method": "plain",
+            "method": ("learned warm start with substep-level plain fallback"
+                       if main_m == "learned" else "plain warm start"),
             "detection_period": int(sim.meta.detectionPeriod),
         },
+        "methods": method_rows,
+        "learned_vs_plain": speedup,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
