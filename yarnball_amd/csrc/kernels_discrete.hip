@@ -643,8 +643,10 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
     const int n = P->n;
     // Segments in index order: consecutive segments of a strand are spatial
     // neighbours (coherent traversals) and the slot-major list writes
-    // cols[i + n * slot] coalesce across the wave.
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    // cols[i + n * slot] coalesce across the wave.  Blocks walk the index
+    // range XCD by XCD (xcd_tile), so the tree leaves and partner segments
+    // that one stretch of strands visits stay in one L2.
+    const int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     if (!(__float_as_uint(prm[i].w) & YB_FLAG_HAS_NEXT)) {
         numCols[i] = 0;

@@ -50,7 +50,7 @@ __global__ __launch_bounds__(256) void k_step_limit(const DevParams* __restrict_
                                                     int32_t* __restrict__ err,
                                                     float* __restrict__ minSep) {
     const int n = P->n;
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    const int v = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     float minDist = INFINITY;
     float minContact = INFINITY;
     if (v < n) {
@@ -79,7 +79,7 @@ __global__ __launch_bounds__(256) void k_step_limit(const DevParams* __restrict_
     reduce_min_sep(minContact / (2.0f * P->radius), minSep);
 }
 
-__device__ __forceinline__ void block_residual(float num, double* partial);
+__device__ __forceinline__ void block_residual(float num, double* partial, int tile);
 
 // Closest points of segment e = (0 -> e) and (B0 -> B1), specialised from
 // closest_uv (Common.h:318-334) with the segment's |e|^2 and 1/|e|^2 hoisted
@@ -133,7 +133,8 @@ __global__ __launch_bounds__(256) void k_sweep(const DevParams* __restrict__ P,
     if (ctl && ctl->doneAt < sweep) return;
     const int n = P->n;
     const int lane = threadIdx.x & 63;
-    const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int wave = (int)((tile * blockDim.x + threadIdx.x) >> 6);
     const int v = wave * 62 + lane - 1;
     const bool in = (v >= 0) && (v < n);
 
@@ -324,7 +325,7 @@ __global__ __launch_bounds__(256) void k_sweep(const DevParams* __restrict__ P,
         }
         qOut[v] = f4r(q0);
     }
-    if (partial) block_residual(owner ? num : 0.0f, partial);
+    if (partial) block_residual(owner ? num : 0.0f, partial, tile);
 }
 
 // Per-sweep relative Newton residual
@@ -332,7 +333,7 @@ __global__ __launch_bounds__(256) void k_sweep(const DevParams* __restrict__ P,
 // the RMS change of the iterate in units of the rest length (l_i: the rest
 // length of vertex i's own segment, or of the previous one at a strand end).
 // Per-block partial sums here, one small reduction in k_residual.
-__device__ __forceinline__ void block_residual(float num, double* partial) {
+__device__ __forceinline__ void block_residual(float num, double* partial, int tile) {
     __shared__ float sn[4];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) num += __shfl_xor(num, off, 64);
@@ -342,7 +343,7 @@ __device__ __forceinline__ void block_residual(float num, double* partial) {
     if (threadIdx.x == 0) {
         double a = 0.0;
         for (int k = 0; k < (int)(blockDim.x >> 6); k++) a += sn[k];
-        partial[blockIdx.x] = a;
+        partial[tile] = a;
     }
 }
 

@@ -51,6 +51,17 @@ struct SolverCtl {
 
 constexpr int kMaxCols = YB_MAX_COLLISIONS_PER_SEGMENT;
 
+// The 8 XCDs each have their own L2, and block b of a launch runs on XCD b % 8.
+// Tile index of block b of nb such that every XCD works through one contiguous
+// range of tiles: the strands a tile gathers from (the neighbouring rows of a
+// patch lie a few thousand vertices away) are then fetched into one L2, not
+// into all eight.  A bijection onto [0, nb) for any nb.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+    const int q = nb >> 3, r = nb & 7;
+    const int x = b & 7, k = b >> 3;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
+}
+
 // Learned warm start: parameter vector layout (floats), see yarnball_hip.h.
 constexpr int kMlpIn = YB_MLP_INPUTS;
 constexpr int kMlpHid = YB_MLP_HIDDEN;
