@@ -49,6 +49,12 @@ def stage_bytes(stage, c):
     }.get(stage)
 
 
+# HBM bytes per vertex of one k_sweep launch on the knit patch (256 x 4096,
+# residual 1e-6, c = 4.42 contacts per segment), from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE scaled by a copy kernel's counts (profiles/sweep_knit1M_hbm_traffic.md).
+MEASURED_TRAFFIC = {("knit_patch", "sweep"): 160.4}
+
+
 def build_workload(name, numItr):
     from yarnball_amd import scenes
 
@@ -99,6 +105,8 @@ def run_method(sc, args, method, model):
         sim.setSolver(args.residual, maxSweeps=args.max_sweeps, chunk=16, checkEvery=4)
     if method == "learned":
         learned.use(sim, model)
+    if args.no_graphs:
+        sim.setGraphs(False)
     frame = np.float32(sc.frame_dt)
     substeps_per_frame = int(np.ceil(frame / np.float32(sc.maxH)))
     for w in range(args.warmup):
@@ -150,6 +158,8 @@ def main():
     ap.add_argument("--max-sweeps", type=int, default=6000)
     ap.add_argument("--method", choices=["plain", "learned", "both"], default="both",
                     help="warm start(s) to run; value is the last one's rate")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="enqueue kernels directly instead of HIP graphs (profiler runs)")
     ap.add_argument("--model", default=os.path.join(ROOT, "yarnball_amd", "models",
                                                     "warmstart.npy"))
     args = ap.parse_args()
@@ -216,7 +226,8 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "traffic": (round(MEASURED_TRAFFIC[(args.workload, dominant)] * sc.n)
+                    if (args.workload, dominant) in MEASURED_TRAFFIC and args.residual > 0 else None),
         "alg_bytes_per_launch": int(alg_bytes),
         "avg_launch_us": round(per_launch[dominant] * 1e3, 2),
         "mean_contacts_per_segment": round(c, 3),
@@ -279,6 +290,7 @@ def main():
             "method": ("learned warm start with substep-level plain fallback"
                        if main_m == "learned" else "plain warm start"),
             "detection_period": int(sim.meta.detectionPeriod),
+            "launch": "eager" if args.no_graphs else "hip graphs",
         },
         "methods": method_rows,
         "learned_vs_plain": speedup,

@@ -204,6 +204,12 @@ typedef struct yb_solver_stats_t {
 int yb_set_solver(yb_sim* s, float residualTol, int maxSweeps, int chunk, int checkEvery);
 int yb_solver_stats(yb_sim* s, yb_solver_stats_t* out, int reset);
 
+/* Launch mode.  on = 1 (default): each substep's kernels run as captured HIP
+ * graphs (step.cpp:36-59 likewise captures the sweep chain).  on = 0: the same
+ * kernels are enqueued one by one on the handle's stream -- same results,
+ * more host launch overhead; for tools that cannot trace graph launches. */
+int yb_set_graphs(yb_sim* s, int on);
+
 /* ---- learned warm start ------------------------------------------------------
  * No counterpart in the reference, whose warm start is the gravity
  * extrapolation of iteration.cu:18-34.  A 48 -> 128 -> 128 -> 3 ReLU MLP

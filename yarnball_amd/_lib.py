@@ -122,6 +122,7 @@ SIGNATURES = {
     "yb_set_warm_start": (_I, [_P, _I]),
     "yb_mlp_counters": (_I, [_P, _P, _I]),
     "yb_warm_start_substeps": (_I, [_P, _P, _I]),
+    "yb_set_graphs": (_I, [_P, _I]),
     "yb_record": (_I, [_P, ctypes.c_int64]),
     "yb_record_enable": (_I, [_P, _I]),
     "yb_record_fetch": (_I, [_P, _P, _P, ctypes.POINTER(ctypes.c_int64)]),

@@ -320,19 +320,47 @@ static int upload_params(yb_sim* s, const yb_params* p) {
 struct ProfState {
     std::vector<hipEvent_t> ev;
     std::vector<int> stage;
+    std::vector<int> count;  // launches between the previous event and this one
+    std::vector<int> tag;    // first sweep index of a run of sweeps / residuals, else -1
     int used = 0;
+    int curTag = -1;
+    int doneAt = 0x7fffffff;  // runs tagged after the converged sweep are not harvested
+    int held = 0, heldStage = -1;
 };
+
+// Stage timing for yb_profile: an event after each stage.  A run of
+// back-to-back launches of one stage (the sweeps between two residual checks)
+// is timed by one interval: hold() counts a launch without an event, the next
+// mark() of that stage closes the run.  No marker sits between the launches of
+// a run, since each marker's release fence would write back the L2 between
+// kernels that otherwise run back to back.  stage -1 restarts the clock after
+// host-side waits and copies.
+static void hold(yb_sim* s, int stage) {
+    ProfState* ps = (ProfState*)s->prof;
+    if (!ps) return;
+    ps->heldStage = stage;
+    ps->held++;
+}
 
 static void mark(yb_sim* s, int stage) {
     ProfState* ps = (ProfState*)s->prof;
     if (!ps) return;
+    int count = 1;
+    if (ps->held) {
+        if (ps->heldStage == stage) count += ps->held;
+        ps->held = 0;
+    }
     if (ps->used == (int)ps->ev.size()) {
         hipEvent_t e;
         (void)hipEventCreate(&e);
         ps->ev.push_back(e);
         ps->stage.push_back(0);
+        ps->count.push_back(0);
+        ps->tag.push_back(-1);
     }
     ps->stage[ps->used] = stage;
+    ps->count[ps->used] = count;
+    ps->tag[ps->used] = ps->curTag;
     (void)hipEventRecord(ps->ev[ps->used++], s->stream);
 }
 
@@ -383,10 +411,14 @@ static bool checks_residual(const yb_sim* s, int k) {
     return (k + 1) % s->checkEvery == 0 || k == s->maxSweeps - 1;
 }
 
-static void enqueue_sweep(yb_sim* s, int k, bool adaptive) {
+// last: the final sweep of a fixed schedule (ends the run of sweeps in a profile)
+static void enqueue_sweep(yb_sim* s, int k, bool adaptive, bool last = false) {
     const bool check = adaptive && checks_residual(s, k);
     launch_sweep(s, k, adaptive, check);
-    mark(s, YB_STAGE_SWEEP);
+    if (check || last)
+        mark(s, YB_STAGE_SWEEP);
+    else
+        hold(s, YB_STAGE_SWEEP);
     if (check) {
         launch_residual(s, k);
         mark(s, YB_STAGE_RESIDUAL);
@@ -432,11 +464,11 @@ static int rebuild_graphs(yb_sim* s, int numItr) {
     return YB_OK;
 }
 
-// Residual-targeted mode: prologue graphs, chunk graphs of s->chunk sweeps
-// with the residual reduction after each dx sweep, and the end graph.
 // probe gaps of the substep-level warm-start chooser (choose_start below)
 constexpr int kProbeMin = 8, kProbeMax = 64;
 
+// Residual-targeted mode: prologue graphs, chunk graphs of s->chunk sweeps
+// with the residual reduction after each dx sweep, and the end graph.
 static int rebuild_adaptive_graphs(yb_sim* s) {
     if (s->adaptiveReady) return YB_OK;
     HIPCHECK(hipStreamSynchronize(s->stream));
@@ -491,6 +523,15 @@ extern "C" int yb_set_solver(yb_sim* s, float residualTol, int maxSweeps, int ch
     s->maxSweeps = maxSweeps;
     s->chunk = chunk;
     s->checkEvery = checkEvery;
+    return YB_OK;
+}
+
+extern "C" int yb_set_graphs(yb_sim* s, int on) {
+    if (!s) return YB_ERR_INVALID_ARGUMENT;
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    s->useGraphs = on ? 1 : 0;
+    s->graphItr = -1;
+    s->adaptiveReady = false;
     return YB_OK;
 }
 
@@ -551,11 +592,20 @@ static void update_chooser(yb_sim* s, int start, int sweeps) {
 
 static int adaptive_substep(yb_sim* s, int mode) {
     const int start = choose_start(s);
-    HIPCHECK(hipGraphLaunch(s->proGraphs[start][mode], s->stream));
+    if (s->useGraphs)
+        HIPCHECK(hipGraphLaunch(s->proGraphs[start][mode], s->stream));
+    else
+        enqueue_prologue(s, mode, true, start == 0 ? s->warmStart : 0);
+    const int numChunks = (s->maxSweeps + s->chunk - 1) / s->chunk;
     int launched = 0;
     int done = s->maxSweeps;  // sweeps of this substep (all of them if not converged)
     auto launch_chunk = [&](int c) -> int {
-        HIPCHECK(hipGraphLaunch(s->chunkGraphs[c], s->stream));
+        if (s->useGraphs) {
+            HIPCHECK(hipGraphLaunch(s->chunkGraphs[c], s->stream));
+        } else {
+            const int k1 = std::min((c + 1) * s->chunk, s->maxSweeps);
+            for (int k = c * s->chunk; k < k1; k++) enqueue_sweep(s, k, true);
+        }
         HIPCHECK(hipMemcpyAsync(&s->b.hostFlag[c & 1], &s->b.ctl->doneAt, sizeof(int32_t),
                                 hipMemcpyDeviceToHost, s->stream));
         HIPCHECK(hipEventRecord(s->chunkEv[c & 1], s->stream));
@@ -564,8 +614,8 @@ static int adaptive_substep(yb_sim* s, int mode) {
     };
     int rc = launch_chunk(0);
     if (rc != YB_OK) return rc;
-    for (int c = 0; c < s->numChunks; c++) {
-        if (launched < s->numChunks && launched == c + 1) {
+    for (int c = 0; c < numChunks; c++) {
+        if (launched < numChunks && launched == c + 1) {
             rc = launch_chunk(c + 1);
             if (rc != YB_OK) return rc;
         }
@@ -575,7 +625,11 @@ static int adaptive_substep(yb_sim* s, int mode) {
             break;
         }
     }
-    HIPCHECK(hipGraphLaunch(s->endGraph, s->stream));
+    if (s->useGraphs)
+        HIPCHECK(hipGraphLaunch(s->endGraph, s->stream));
+    else
+        enqueue_end(s, 0, true);
+    HIPCHECK(hipGetLastError());
     s->warmSubsteps[start]++;
     update_chooser(s, start, done);
     return YB_OK;
@@ -636,14 +690,14 @@ extern "C" int yb_advance(yb_sim* s, yb_params* p, float h, float maxH, int* war
     p->h = h / (float)steps;
     int rc = yb_validate_params(p);
     if (rc != YB_OK) return rc;
-    if (s->tol <= 0) {
+    if (s->tol <= 0 && s->useGraphs) {
         rc = rebuild_graphs(s, p->numItr);
         if (rc != YB_OK) return rc;
     }
     rc = upload_params(s, p);
     if (rc != YB_OK) return rc;
     const bool adaptive = s->tol > 0;
-    if (adaptive) {
+    if (adaptive && s->useGraphs) {
         rc = rebuild_adaptive_graphs(s);
         if (rc != YB_OK) return rc;
     }
@@ -652,8 +706,13 @@ extern "C" int yb_advance(yb_sim* s, yb_params* p, float h, float maxH, int* war
         if (adaptive) {
             rc = adaptive_substep(s, mode);
             if (rc != YB_OK) return rc;
-        } else {
+        } else if (s->useGraphs) {
             HIPCHECK(hipGraphLaunch(s->graphs[mode], s->stream));
+        } else {
+            enqueue_prologue(s, mode, false, s->warmStart);
+            for (int k = 0; k < p->numItr; k++) enqueue_sweep(s, k, false);
+            enqueue_end(s, p->numItr & 1, false);
+            HIPCHECK(hipGetLastError());
         }
     }
     // Interpenetration check of the final state (the step limit checks every
@@ -832,13 +891,23 @@ extern "C" int yb_profile(yb_sim* s, yb_params* p, float h, int substeps, double
     const bool adaptive = s->tol > 0;
     ProfState ps;
     s->prof = &ps;
+    // detach and free the markers on every exit path, early error returns included
+    struct Detach {
+        yb_sim* s;
+        ProfState* ps;
+        ~Detach() {
+            s->prof = nullptr;
+            for (auto& e : ps->ev) (void)hipEventDestroy(e);
+        }
+    } detach{s, &ps};
     auto harvest = [&]() -> int {
         for (int k = 1; k < ps.used; k++) {
             float t = 0.f;
             if (ps.stage[k] < 0) continue;  // gap marker: host latency, not a kernel
+            if (ps.tag[k] > ps.doneAt) continue;  // speculative launches after convergence
             HIPCHECK(hipEventElapsedTime(&t, ps.ev[k - 1], ps.ev[k]));
             ms[ps.stage[k]] += t;
-            launches[ps.stage[k]] += 1;
+            launches[ps.stage[k]] += ps.count[k];
         }
         return YB_OK;
     };
@@ -848,20 +917,38 @@ extern "C" int yb_profile(yb_sim* s, yb_params* p, float h, int substeps, double
         mark(s, -1);
         enqueue_prologue(s, mode, adaptive, s->warmStart);
         int fin = p->numItr & 1;
+        ps.doneAt = 0x7fffffff;
         if (adaptive) {
-            // one sweep at a time so that no early-exit launch is timed
-            for (int k = 0; k < s->maxSweeps; k++) {
-                mark(s, -1);
-                enqueue_sweep(s, k, true);
-                if (!checks_residual(s, k)) continue;
-                int32_t done = 0;
-                HIPCHECK(hipMemcpyAsync(&done, &s->b.ctl->doneAt, 4, hipMemcpyDeviceToHost,
-                                        s->stream));
-                HIPCHECK(hipStreamSynchronize(s->stream));
-                if (done != 0x7fffffff) break;
+            // As in the stepper, the next group of sweeps is queued before the
+            // host waits for the current group's residual, so the device does
+            // not idle between groups; the groups queued after convergence
+            // (early-exit launches) are left out of the harvest.
+            const int ce = s->checkEvery;
+            const int groups = (s->maxSweeps + ce - 1) / ce;
+            auto enqueue_group = [&](int g) -> int {
+                ps.curTag = g * ce;
+                const int k1 = std::min((g + 1) * ce, s->maxSweeps);
+                for (int k = g * ce; k < k1; k++) enqueue_sweep(s, k, true);
+                ps.curTag = -1;
+                HIPCHECK(hipMemcpyAsync(&s->b.hostFlag[g & 1], &s->b.ctl->doneAt, sizeof(int32_t),
+                                        hipMemcpyDeviceToHost, s->stream));
+                HIPCHECK(hipEventRecord(s->chunkEv[g & 1], s->stream));
+                mark(s, -1);  // the readback belongs to no stage
+                return YB_OK;
+            };
+            rc = enqueue_group(0);
+            for (int g = 0; g < groups && rc == YB_OK; g++) {
+                if (g + 1 < groups) rc = enqueue_group(g + 1);
+                if (rc != YB_OK) break;
+                HIPCHECK(hipEventSynchronize(s->chunkEv[g & 1]));
+                if (s->b.hostFlag[g & 1] != 0x7fffffff) {
+                    ps.doneAt = s->b.hostFlag[g & 1];
+                    break;
+                }
             }
+            if (rc != YB_OK) break;
         } else {
-            for (int k = 0; k < p->numItr; k++) enqueue_sweep(s, k, false);
+            for (int k = 0; k < p->numItr; k++) enqueue_sweep(s, k, false, k == p->numItr - 1);
         }
         mark(s, -1);
         enqueue_end(s, fin, adaptive);
@@ -869,8 +956,6 @@ extern "C" int yb_profile(yb_sim* s, yb_params* p, float h, int substeps, double
         HIPCHECK(hipStreamSynchronize(s->stream));
         rc = harvest();
     }
-    s->prof = nullptr;
-    for (auto& e : ps.ev) (void)hipEventDestroy(e);
     if (rc != YB_OK) return rc;
     p->time += h * substeps;
     return check_errors(s, nullptr);

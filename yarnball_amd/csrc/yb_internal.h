@@ -161,6 +161,7 @@ struct yb_sim {
     // captured substep graphs: [detect mode 0=none,1=refit,2=rebuild]
     hipGraphExec_t graphs[3] = {nullptr, nullptr, nullptr};
     int graphItr = -1;
+    int useGraphs = 1;  // 0: every kernel enqueued directly on the stream
     // residual-targeted mode (tol > 0): prologue graphs per detection mode,
     // one graph per chunk of sweeps, one end graph
     float tol = 0.f;

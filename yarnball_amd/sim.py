@@ -169,6 +169,12 @@ class Sim:
         check(self._lib.yb_solver_stats(self._h, ctypes.byref(st), int(bool(reset))))
         return {f: getattr(st, f) for f, _ in st._fields_}
 
+    def setGraphs(self, on=True):
+        """Run substeps as captured HIP graphs (default) or enqueue every
+        kernel directly (same results; for tools that cannot trace graphs)."""
+        self._need_handle()
+        check(self._lib.yb_set_graphs(self._h, int(bool(on))))
+
     def minSeparation(self, reset=False):
         """Smallest contact distance / (2 radius) since the last reset (below 1:
         interpenetration, reported as WARNING_SEGMENT_INTERPENETRATION)."""
