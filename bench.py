@@ -94,6 +94,141 @@ def cpu_baseline(sc, substeps, threads, residual, max_sweeps):
     return sc.n * substeps / dt, used, dt, sweeps
 
 
+def bench_batched(args):
+    """Batched uneven scenes: many random-walk strands (sizes log-uniform in
+    [256, 65536], kBend log-uniform in [1e-9, 1e-7]) in one handle, every
+    kernel launched once over all scenes, against the same scenes advanced
+    one handle at a time.  Fixed sweep schedule (numItr), in which batched
+    and per-scene runs are bit-identical."""
+    from yarnball_amd import Sim, scenes
+    from yarnball_amd._lib import STAGES, check, ptr
+
+    rng = np.random.default_rng(0)
+    sizes = np.exp(rng.uniform(np.log(256), np.log(65536), args.scenes)).astype(np.int64)
+    t0 = time.perf_counter()
+    scs = scenes.random_walks(sizes, seed=0, numItr=args.numItr)
+    big, offs = scenes.batch(scs, name=f"random_walks_{args.scenes}")
+    print(f"[bench] {args.scenes} scenes, {big.n} vertices, built in "
+          f"{time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+    frame = np.float32(big.frame_dt)
+    spf = int(np.ceil(frame / np.float32(big.maxH)))
+    if args.residual > 0:
+        print("[bench] batched scenes run the fixed sweep schedule (numItr)", file=sys.stderr)
+
+    b = Sim(big.n)
+    # the random walks cross themselves here and there at t = 0: reported as
+    # min_separation below instead of a warning per advance
+    b.printErrors = False
+    scenes.load_into(b, big)
+    b.setScenes(offs)
+    b.minSeparation(reset=True)
+    if args.no_graphs:
+        b.setGraphs(False)
+    for w in range(args.warmup):
+        b.advance(frame)
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        b.advance(frame)
+        print(f"[bench] batched: frame {k} done", file=sys.stderr, flush=True)
+    el_b = time.perf_counter() - t0
+    value = big.n * spf * args.steps / el_b
+    min_sep = b.minSeparation()
+
+    # live per-stage profile of the batched handle
+    ms = np.zeros(len(STAGES), np.float64)
+    nl = np.zeros(len(STAGES), np.int32)
+    p = b.meta
+    check(b._lib.yb_profile(b.handle, ctypes.byref(p), np.float32(frame / spf),
+                            args.profile_substeps, ptr(ms), ptr(nl)))
+    counts = b.collisionCounts()
+    segs = (b.verts["flags"] & 2) != 0
+    c = float(counts[segs].mean()) if segs.any() else 0.0
+    i_sw = STAGES.index("sweep")
+    sweep_us = ms[i_sw] / max(nl[i_sw], 1) * 1e3
+    for s_ in STAGES:
+        i = STAGES.index(s_)
+        if nl[i]:
+            print(f"[bench] stage {s_:11s} launches {nl[i]:5d} avg {ms[i] / nl[i] * 1e3:9.1f} us",
+                  file=sys.stderr)
+    b.close()
+
+    per = None
+    if not args.no_per_scene:
+        sims = []
+        for sc in scs:
+            s_ = Sim(sc.n)
+            s_.printErrors = False
+            scenes.load_into(s_, sc)
+            sims.append(s_)
+        for w in range(args.warmup):
+            for s_ in sims:
+                s_.advance(frame)
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            for s_ in sims:
+                s_.advance(frame)
+            print(f"[bench] per-scene: frame {k} done", file=sys.stderr, flush=True)
+        el_p = time.perf_counter() - t0
+        for s_ in sims:
+            s_.close()
+        per = {"value": round(big.n * spf * args.steps / el_p, 1),
+               "ms_per_step": round(el_p / args.steps * 1e3, 3)}
+        print(f"[bench] per-scene launches: {per['value']:.4e} vertex-substeps/s", file=sys.stderr)
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import Oracle, load
+        lib = load()
+        if args.cpu_threads > 0:
+            lib.orc_set_threads(args.cpu_threads)
+        sub = scs[:32]
+        orcs = [(Oracle(sc), sc.params.copy(), sc) for sc in sub]
+        nv = sum(sc.n for sc in sub)
+        rounds = 0
+        t0 = time.perf_counter()
+        while rounds == 0 or time.perf_counter() - t0 < 10.0:
+            for o, pp, sc in orcs:
+                o.advance(pp, sc.maxH, sc.maxH)
+            rounds += 1
+        dt = time.perf_counter() - t0
+        for o, _, _ in orcs:
+            o.close()
+        cpu = {"value": round(nv * rounds / dt, 1), "unit": "vertex-substeps/s",
+               "cores": int(lib.orc_get_threads()), "kind": "port",
+               "sample": f"FP64 OpenMP oracle, the first 32 scenes ({nv} vertices) one substep "
+                         f"({args.numItr} sweeps) each in turn, {rounds} rounds in {dt:.1f}s"}
+
+    alg = (124 + 4 * c) * big.n
+    out = {
+        "metric": "vertex-substeps/s (batched uneven scenes)",
+        "value": round(value, 1),
+        "unit": "vertex-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(el_b / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": big.name, "scenes": int(args.scenes), "vertices": int(big.n),
+                   "scene_vertices": [int(sizes.min()), int(sizes.max())],
+                   "substeps_per_step": spf, "numItr": int(args.numItr),
+                   "launch": "eager" if args.no_graphs else "hip graphs",
+                   "min_separation_x2r": round(min_sep, 3)},
+        "per_scene_launches": per,
+        "batched_vs_per_scene": round(value / per["value"], 3) if per else None,
+        "roofline": {"kernel": "sweep", "bound": "hbm",
+                     "achieved": round(alg / (sweep_us * 1e-6) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(alg / (sweep_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": None, "alg_bytes_per_launch": int(alg),
+                     "avg_launch_us": round(sweep_us, 2), "mean_contacts_per_segment": round(c, 3)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
 def run_method(sc, args, method, model):
     """Warm-up frames, then the timed frames, of one warm-start method on a
     fresh simulation; returns the open simulation and the measurements."""
@@ -162,11 +297,16 @@ def main():
                     help="enqueue kernels directly instead of HIP graphs (profiler runs)")
     ap.add_argument("--model", default=os.path.join(ROOT, "yarnball_amd", "models",
                                                     "warmstart.npy"))
+    ap.add_argument("--scenes", type=int, default=512, help="batched_walks: number of scenes")
+    ap.add_argument("--no-per-scene", action="store_true",
+                    help="batched_walks: skip the one-handle-per-scene comparison")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the stepper runs on a single MI355X "
               "(multi-GPU sharding is design-only)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batched_walks":
+        return bench_batched(args)
 
     from yarnball_amd._lib import STAGES, check, ptr
 

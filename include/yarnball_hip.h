@@ -204,6 +204,23 @@ typedef struct yb_solver_stats_t {
 int yb_set_solver(yb_sim* s, float residualTol, int maxSweeps, int chunk, int checkEvery);
 int yb_solver_stats(yb_sim* s, yb_solver_stats_t* out, int reset);
 
+/* ---- batched scenes ---------------------------------------------------------
+ * No counterpart in the reference (one Sim per scene).  Many independent
+ * scenes share one handle: their vertices are concatenated, scene k owning
+ * [offsets[k], offsets[k+1]) with offsets[0] = 0 and offsets[numScenes] = n;
+ * the last vertex of every scene must not have YB_FLAG_HAS_NEXT.  Every kernel
+ * then covers all scenes in one launch.  Contact detection stays inside each
+ * scene: per-scene bounding boxes and Morton codes, scene-major sorted order,
+ * one subtree per scene whose shape equals a standalone run's, and queries
+ * that start at their own scene's subtree root.  The scenes share one
+ * yb_params (gravity, radii, stiffness scales, schedule); per-vertex
+ * stiffness and rest shapes are per scene.  With the fixed sweep schedule each
+ * scene's state evolves bit-identically to a standalone handle of that scene;
+ * the residual-targeted schedule uses one residual over all scenes.
+ * numScenes = 1 restores the single-scene layout.  Call after yb_create
+ * (before or after yb_upload). */
+int yb_set_scenes(yb_sim* s, int numScenes, const int32_t* offsets);
+
 /* Launch mode.  on = 1 (default): each substep's kernels run as captured HIP
  * graphs (step.cpp:36-59 likewise captures the sweep chain).  on = 0: the same
  * kernels are enqueued one by one on the handle's stream -- same results,

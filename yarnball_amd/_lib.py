@@ -123,6 +123,7 @@ SIGNATURES = {
     "yb_mlp_counters": (_I, [_P, _P, _I]),
     "yb_warm_start_substeps": (_I, [_P, _P, _I]),
     "yb_set_graphs": (_I, [_P, _I]),
+    "yb_set_scenes": (_I, [_P, _I, _P]),
     "yb_record": (_I, [_P, ctypes.c_int64]),
     "yb_record_enable": (_I, [_P, _I]),
     "yb_record_fetch": (_I, [_P, _P, _P, ctypes.POINTER(ctypes.c_int64)]),

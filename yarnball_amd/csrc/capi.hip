@@ -49,7 +49,8 @@ static void free_all(yb_sim* s) {
                     b.kids,    b.strad,   b.stradCount, b.leafNeed, b.sufBox, b.preBox, b.blkTab,
                     b.err,     b.params,  b.stage, b.ctl, b.partial, b.minSep,
                     b.mlpPrm,  b.mlpTiles, b.mlpBias, b.featB, b.corrPrev, b.dxPlain, b.predC,
-                    b.mlpCount, b.recX, b.recT, b.recCount};
+                    b.mlpCount, b.recX, b.recT, b.recCount, b.sceneOf, b.sceneOff,
+                    b.sceneBoxes, b.sceneRoot, b.sceneEndFlags};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (b.hostFlag) (void)hipHostFree(b.hostFlag);
@@ -227,6 +228,13 @@ extern "C" int yb_upload(yb_sim* s, const yb_vertex* verts, const float* vels, c
             return YB_ERR_INVALID_ARGUMENT;
         }
     }
+    for (int k = 0; k + 1 < (int)s->sceneOffHost.size(); k++) {
+        if (verts[s->sceneOffHost[k + 1] - 1].flags & YB_FLAG_HAS_NEXT) {
+            set_message("yb_upload: the last vertex of scene " + std::to_string(k) +
+                        " has a next vertex (a segment would cross into the next scene)");
+            return YB_ERR_INVALID_ARGUMENT;
+        }
+    }
     char* st = (char*)s->b.stage;
     HIPCHECK(hipMemcpyAsync(st, verts, sizeof(yb_vertex) * N, hipMemcpyHostToDevice, s->stream));
     st += sizeof(yb_vertex) * N;
@@ -251,6 +259,73 @@ extern "C" int yb_upload(yb_sim* s, const yb_vertex* verts, const float* vels, c
     HIPCHECK(hipMemcpyAsync(&s->b.ctl->denom, &s->denomHost, sizeof(double),
                             hipMemcpyHostToDevice, s->stream));
     HIPCHECK(hipStreamSynchronize(s->stream));
+    s->uploaded = true;
+    return YB_OK;
+}
+
+extern "C" int yb_set_scenes(yb_sim* s, int numScenes, const int32_t* offsets) {
+    if (!s || numScenes < 1 || !offsets) {
+        set_message("yb_set_scenes: invalid argument");
+        return YB_ERR_INVALID_ARGUMENT;
+    }
+    if (numScenes > 65536) {
+        set_message("yb_set_scenes: at most 65536 scenes");
+        return YB_ERR_INVALID_ARGUMENT;
+    }
+    if (offsets[0] != 0 || offsets[numScenes] != s->n) {
+        set_message("yb_set_scenes: offsets must start at 0 and end at the vertex count");
+        return YB_ERR_INVALID_ARGUMENT;
+    }
+    for (int k = 0; k < numScenes; k++)
+        if (offsets[k + 1] - offsets[k] < 2) {
+            set_message("yb_set_scenes: scene " + std::to_string(k) + " has fewer than 2 vertices");
+            return YB_ERR_INVALID_ARGUMENT;
+        }
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    Buffers& b = s->b;
+    for (void* p : {(void*)b.sceneOf, (void*)b.sceneOff, (void*)b.sceneBoxes, (void*)b.sceneRoot,
+                    (void*)b.sceneEndFlags})
+        if (p) HIPCHECK(hipFree(p));
+    b.sceneOf = b.sceneOff = b.sceneRoot = b.sceneEndFlags = nullptr;
+    b.sceneBoxes = nullptr;
+    s->numScenes = 1;
+    s->sceneOffHost.clear();
+    if (numScenes > 1) {
+        const size_t N = (size_t)s->n;
+        HIPCHECK(hipMalloc(&b.sceneOf, sizeof(int32_t) * N));
+        HIPCHECK(hipMalloc(&b.sceneOff, sizeof(int32_t) * (numScenes + 1)));
+        HIPCHECK(hipMalloc(&b.sceneBoxes, sizeof(float4) * 2 * numScenes));
+        HIPCHECK(hipMalloc(&b.sceneRoot, sizeof(int32_t) * numScenes));
+        HIPCHECK(hipMalloc(&b.sceneEndFlags, sizeof(int32_t) * numScenes));
+        std::vector<int32_t> of(N);
+        for (int k = 0; k < numScenes; k++)
+            for (int v = offsets[k]; v < offsets[k + 1]; v++) of[v] = k;
+        HIPCHECK(hipMemcpy(b.sceneOf, of.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(b.sceneOff, offsets, sizeof(int32_t) * (numScenes + 1),
+                           hipMemcpyHostToDevice));
+        s->numScenes = numScenes;
+        s->sceneOffHost.assign(offsets, offsets + numScenes + 1);
+        if (s->uploaded) {
+            launch_scene_end_flags(s);
+            HIPCHECK(hipGetLastError());
+            std::vector<int32_t> fl(numScenes);
+            HIPCHECK(hipMemcpyAsync(fl.data(), b.sceneEndFlags, sizeof(int32_t) * numScenes,
+                                    hipMemcpyDeviceToHost, s->stream));
+            HIPCHECK(hipStreamSynchronize(s->stream));
+            for (int k = 0; k < numScenes; k++)
+                if (fl[k] & YB_FLAG_HAS_NEXT) {
+                    set_message("yb_set_scenes: the last vertex of scene " + std::to_string(k) +
+                                " has a next vertex (a segment would cross into the next scene)");
+                    return yb_set_scenes(s, 1, std::vector<int32_t>{0, s->n}.data()) == YB_OK
+                               ? YB_ERR_INVALID_ARGUMENT
+                               : YB_ERR_HIP;
+                }
+        }
+    }
+    // the tree and the captured graphs depend on the scene layout
+    s->treeBuilt = false;
+    s->graphItr = -1;
+    s->adaptiveReady = false;
     return YB_OK;
 }
 

@@ -144,14 +144,46 @@ __global__ __launch_bounds__(kBoxThreads) void k_box_final(int nb, const float4*
     }
 }
 
+// Batched scenes: one workgroup per scene reduces that scene's boxes
+// [off[s], off[s+1]) into sceneBoxes[2s], [2s+1] (the same exact min/max as a
+// standalone run of the scene).
+__global__ __launch_bounds__(kBoxThreads) void k_box_scenes(const int32_t* __restrict__ off,
+                                                            const float4* __restrict__ aabb,
+                                                            float4* __restrict__ sceneBoxes) {
+    __shared__ float sm[6][kBoxThreads];
+    const int sc = blockIdx.x;
+    v3 lo = v3{INFINITY, INFINITY, INFINITY};
+    v3 hi = v3{-INFINITY, -INFINITY, -INFINITY};
+    for (int v = off[sc] + threadIdx.x; v < off[sc + 1]; v += blockDim.x) {
+        lo = vmin(lo, xyz(aabb[2 * v]));
+        hi = vmax(hi, xyz(aabb[2 * v + 1]));
+    }
+    const int t = threadIdx.x;
+    sm[0][t] = lo.x; sm[1][t] = lo.y; sm[2][t] = lo.z;
+    sm[3][t] = hi.x; sm[4][t] = hi.y; sm[5][t] = hi.z;
+    __syncthreads();
+    for (int st = kBoxThreads / 2; st > 0; st >>= 1) {
+        if (t < st) {
+            for (int k = 0; k < 3; k++) sm[k][t] = fminf(sm[k][t], sm[k][t + st]);
+            for (int k = 3; k < 6; k++) sm[k][t] = fmaxf(sm[k][t], sm[k][t + st]);
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        sceneBoxes[2 * sc] = make_float4(sm[0][0], sm[1][0], sm[2][0], 0.f);
+        sceneBoxes[2 * sc + 1] = make_float4(sm[3][0], sm[4][0], sm[5][0], 0.f);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Morton codes: box centre quantised to 1024 cubic cells spanning the largest
 // extent of the scene box (isotropic, so a long thin scene does not get
 // sliver cells), bits interleaved x-major.  Non-segments get 0xFFFFFFFF and
-// sort last.
+// sort last.  Batched scenes (sceneOf != null) quantise in their own box.
 __global__ __launch_bounds__(256) void k_morton(int n, const float4* __restrict__ prm,
                                                 const float4* __restrict__ aabb,
                                                 const float4* __restrict__ sceneBox,
+                                                const int32_t* __restrict__ sceneOf,
                                                 uint32_t* __restrict__ morton,
                                                 uint32_t* __restrict__ keys,
                                                 int32_t* __restrict__ vals) {
@@ -159,8 +191,9 @@ __global__ __launch_bounds__(256) void k_morton(int n, const float4* __restrict_
     if (v >= n) return;
     uint32_t code = 0xFFFFFFFFu;
     if (__float_as_uint(prm[v].w) & YB_FLAG_HAS_NEXT) {
-        const v3 slo = xyz(sceneBox[0]);
-        const v3 shi = xyz(sceneBox[1]);
+        const int sc = sceneOf ? sceneOf[v] : 0;
+        const v3 slo = xyz(sceneBox[2 * sc]);
+        const v3 shi = xyz(sceneBox[2 * sc + 1]);
         const v3 e3 = shi - slo;
         const float ext = fmaxf(fmaxf(e3.x, e3.y), e3.z);
         const v3 lo = xyz(aabb[2 * v]);
@@ -182,9 +215,19 @@ __global__ __launch_bounds__(256) void k_morton(int n, const float4* __restrict_
 // digit-major [256 x tiles] table -> scatter.  The scatter ranks keys of equal
 // digit in index order: within a 64-lane wave by ballot matching of the 8
 // digit bits, across the block's 4 waves through LDS counters, across strips
-// by a running per-digit count.
+// by a running per-digit count.  With sceneOf != null the digit is taken from
+// the scene of the value instead of the key: two such passes after the four
+// key passes order batched scenes scene-major, Morton order within a scene.
+__device__ __forceinline__ uint32_t sort_digit(uint32_t key, int32_t val, int shift,
+                                               const int32_t* __restrict__ sceneOf) {
+    const uint32_t x = sceneOf ? (uint32_t)sceneOf[val] : key;
+    return (x >> shift) & (kSortBins - 1);
+}
+
 __global__ __launch_bounds__(kSortThreads) void k_sort_hist(int n, int shift, int tiles,
                                                             const uint32_t* __restrict__ keys,
+                                                            const int32_t* __restrict__ vals,
+                                                            const int32_t* __restrict__ sceneOf,
                                                             uint32_t* __restrict__ hist) {
     __shared__ uint32_t lh[kSortBins];
     const int t = threadIdx.x;
@@ -194,7 +237,8 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_hist(int n, int shift, in
 #pragma unroll
     for (int k = 0; k < kSortItems; k++) {
         const int idx = base + k * kSortThreads + t;
-        if (idx < n) atomicAdd(&lh[(keys[idx] >> shift) & (kSortBins - 1)], 1u);
+        if (idx < n)
+            atomicAdd(&lh[sort_digit(keys[idx], sceneOf ? vals[idx] : 0, shift, sceneOf)], 1u);
     }
     __syncthreads();
     hist[t * tiles + blockIdx.x] = lh[t];
@@ -231,6 +275,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(int n, int shift,
                                                                const int32_t* __restrict__ vin,
                                                                uint32_t* __restrict__ kout,
                                                                int32_t* __restrict__ vout,
+                                                               const int32_t* __restrict__ sceneOf,
                                                                const uint32_t* __restrict__ hist) {
     constexpr int kWaves = kSortThreads / 64;
     __shared__ uint32_t run[kSortBins];
@@ -253,7 +298,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(int n, int shift,
             key = kin[idx];
             val = vin[idx];
         }
-        const uint32_t d = (key >> shift) & (kSortBins - 1);
+        const uint32_t d = valid ? sort_digit(key, val, shift, sceneOf) : 0u;
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int bit = 0; bit < kSortBits; bit++) {
@@ -286,19 +331,39 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(int n, int shift,
 
 // ---------------------------------------------------------------------------
 // Karras (2012) radix tree.  Keys are made unique by the sorted position:
-// delta(i,j) = clz(k_i ^ k_j), or 32 + clz(i ^ j) for equal codes.
+// delta(i,j) = clz(k_i ^ k_j), or 32 + clz(i ^ j) for equal codes.  Batched
+// scenes (sceneOf != null) extend the key to (scene, code, position within
+// the scene): delta = clz(s_i ^ s_j) across scenes, 32 + the single-scene
+// delta within one, so every scene is one subtree shaped exactly as in a
+// standalone run of that scene; k_karras records that subtree's root.
 // Node i (internal, 0..n-2) stores its children's boxes inline:
 // nodes[4i+0] = (lo_left, child code), nodes[4i+1] = (hi_left, leaf pad),
 // nodes[4i+2] = (lo_right, child code), nodes[4i+3] = (hi_right, leaf pad),
 // child code = internal index (>= 0) or ~segment for a leaf; "leaf pad" is the
 // leaf's signed velocity pad (see k_aabb).  Also written per node: its sorted
 // leaf range rng[i] = (first, last) and kids[i] with leaves as ~sorted position.
-__device__ __forceinline__ int kdelta(const uint32_t* __restrict__ keys, int n, int a, int b) {
+struct SceneKeys {
+    const int32_t* sceneOf;   // scene of each vertex = of each sorted position (scene-major)
+    const int32_t* sceneOff;  // first vertex of each scene, numScenes + 1 entries
+    int32_t* sceneRoot;       // out: root node of each scene's subtree
+};
+
+__device__ __forceinline__ int kdelta(const uint32_t* __restrict__ keys, int n, int a, int b,
+                                      const SceneKeys& sk) {
     if (b < 0 || b > n - 1) return -1;
+    int la = a, lb = b;
+    int base = 0;
+    if (sk.sceneOf) {
+        const int sa = sk.sceneOf[a], sb = sk.sceneOf[b];
+        if (sa != sb) return __clz((uint32_t)(sa ^ sb));
+        la -= sk.sceneOff[sa];
+        lb -= sk.sceneOff[sa];
+        base = 32;
+    }
     const uint32_t ka = keys[a];
     const uint32_t kb = keys[b];
-    if (ka == kb) return 32 + __clz((uint32_t)(a ^ b));
-    return __clz(ka ^ kb);
+    if (ka == kb) return base + 32 + __clz((uint32_t)(la ^ lb));
+    return base + __clz(ka ^ kb);
 }
 
 __global__ __launch_bounds__(256) void k_karras(int n, const uint32_t* __restrict__ keys,
@@ -308,23 +373,23 @@ __global__ __launch_bounds__(256) void k_karras(int n, const uint32_t* __restric
                                                 int2* __restrict__ rng, int2* __restrict__ kids,
                                                 int32_t* __restrict__ strad,
                                                 int32_t* __restrict__ stradCount,
-                                                int32_t* __restrict__ leafNeed) {
+                                                int32_t* __restrict__ leafNeed, SceneKeys sk) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n - 1) return;
-    const int d = (kdelta(keys, n, i, i + 1) - kdelta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
-    const int dmin = kdelta(keys, n, i, i - d);
+    const int d = (kdelta(keys, n, i, i + 1, sk) - kdelta(keys, n, i, i - 1, sk)) >= 0 ? 1 : -1;
+    const int dmin = kdelta(keys, n, i, i - d, sk);
     int lmax = 2;
-    while (kdelta(keys, n, i, i + lmax * d) > dmin) lmax <<= 1;
+    while (kdelta(keys, n, i, i + lmax * d, sk) > dmin) lmax <<= 1;
     int l = 0;
     for (int t = lmax >> 1; t >= 1; t >>= 1)
-        if (kdelta(keys, n, i, i + (l + t) * d) > dmin) l += t;
+        if (kdelta(keys, n, i, i + (l + t) * d, sk) > dmin) l += t;
     const int j = i + l * d;
-    const int dnode = kdelta(keys, n, i, j);
+    const int dnode = kdelta(keys, n, i, j, sk);
     int s = 0;
     int t = l;
     do {
         t = (t + 1) >> 1;
-        if (kdelta(keys, n, i, i + (s + t) * d) > dnode) s += t;
+        if (kdelta(keys, n, i, i + (s + t) * d, sk) > dnode) s += t;
     } while (t > 1);
     const int gamma = i + s * d + min(d, 0);
     const int lo = min(i, j);
@@ -351,6 +416,10 @@ __global__ __launch_bounds__(256) void k_karras(int n, const uint32_t* __restric
     rng[i] = make_int2(lo, hi);
     kids[i] = make_int2(kl, kr);
     if (i == 0) parent[0] = -1;
+    if (sk.sceneOf) {
+        const int sc = sk.sceneOf[lo];
+        if (lo == sk.sceneOff[sc] && hi == sk.sceneOff[sc + 1] - 1) sk.sceneRoot[sc] = i;
+    }
     // Nodes whose leaf range crosses a refit-block boundary are finished by
     // the range-union pass; their end leaves need suffix / prefix unions.
     if (lo / kRefitLeaves != hi / kRefitLeaves) {
@@ -638,7 +707,9 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
                                                          const float4* __restrict__ prm,
                                                          int32_t* __restrict__ numCols,
                                                          int32_t* __restrict__ cols,
-                                                         int32_t* __restrict__ err) {
+                                                         int32_t* __restrict__ err,
+                                                         const int32_t* __restrict__ sceneOf,
+                                                         const int32_t* __restrict__ sceneRoot) {
     __shared__ int32_t stk[kStack][kQueryThreads];
     const int n = P->n;
     // Segments in index order: consecutive segments of a strand are spatial
@@ -663,7 +734,8 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
     si.glued = signbit(qhi4.w);
     const int t = threadIdx.x;
     int sp = 0;
-    int node = 0;
+    // batched scenes: traverse only this segment's own scene subtree
+    int node = sceneOf ? sceneRoot[sceneOf[i]] : 0;
     int cnt = 0;
     bool overflowStack = false;
     while (true) {
@@ -719,6 +791,13 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
     numCols[i] = cnt;
 }
 
+// Flags of each scene's last vertex (which must not start a segment).
+__global__ void k_scene_end_flags(int numScenes, const int32_t* __restrict__ off,
+                                  const float4* __restrict__ prm, int32_t* __restrict__ out) {
+    const int sc = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sc < numScenes) out[sc] = (int32_t)__float_as_uint(prm[off[sc + 1] - 1].w);
+}
+
 static inline int blocks(int n, int b) { return (n + b - 1) / b; }
 
 // ---- launchers -------------------------------------------------------------
@@ -734,30 +813,49 @@ void launch_aabb(yb_sim* s) {
 
 void launch_rebuild(yb_sim* s) {
     const int n = s->n;
-    hipLaunchKernelGGL(k_box_partial, dim3(s->boxBlocks), dim3(kBoxThreads), 0, s->stream, n,
-                       s->b.aabb, s->b.blockBox);
-    hipLaunchKernelGGL(k_box_final, dim3(1), dim3(kBoxThreads), 0, s->stream, s->boxBlocks,
-                       s->b.blockBox, s->b.sceneBox);
+    const bool batched = s->numScenes > 1;
+    const float4* boxes = s->b.sceneBox;
+    if (batched) {
+        hipLaunchKernelGGL(k_box_scenes, dim3(s->numScenes), dim3(kBoxThreads), 0, s->stream,
+                           s->b.sceneOff, s->b.aabb, s->b.sceneBoxes);
+        boxes = s->b.sceneBoxes;
+    } else {
+        hipLaunchKernelGGL(k_box_partial, dim3(s->boxBlocks), dim3(kBoxThreads), 0, s->stream, n,
+                           s->b.aabb, s->b.blockBox);
+        hipLaunchKernelGGL(k_box_final, dim3(1), dim3(kBoxThreads), 0, s->stream, s->boxBlocks,
+                           s->b.blockBox, s->b.sceneBox);
+    }
     hipLaunchKernelGGL(k_morton, dim3(blocks(n, 256)), dim3(256), 0, s->stream, n, s->b.prm,
-                       s->b.aabb, s->b.sceneBox, s->b.morton, s->b.keys[0], s->b.vals[0]);
+                       s->b.aabb, boxes, batched ? s->b.sceneOf : nullptr, s->b.morton,
+                       s->b.keys[0], s->b.vals[0]);
     const int tiles = s->sortTiles;
-    for (int pass = 0; pass < 4; pass++) {
+    // four 8-bit passes over the Morton code, then (batched) two over the scene
+    // index: an even count, so the sorted pairs end in keys[0] / vals[0] == order
+    const int passes = batched ? 6 : 4;
+    for (int pass = 0; pass < passes; pass++) {
         const int in = pass & 1;
-        const int shift = pass * kSortBits;
+        const bool scenePass = pass >= 4;
+        const int shift = (scenePass ? pass - 4 : pass) * kSortBits;
+        const int32_t* sof = scenePass ? s->b.sceneOf : nullptr;
         hipLaunchKernelGGL(k_sort_hist, dim3(tiles), dim3(kSortThreads), 0, s->stream, n, shift,
-                           tiles, s->b.keys[in], s->b.hist);
+                           tiles, s->b.keys[in], s->b.vals[in], sof, s->b.hist);
         hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(kScanThreads), 0, s->stream,
                            tiles * kSortBins, s->b.hist);
         hipLaunchKernelGGL(k_sort_scatter, dim3(tiles), dim3(kSortThreads), 0, s->stream, n, shift,
                            tiles, s->b.keys[in], s->b.vals[in], s->b.keys[in ^ 1],
-                           s->b.vals[in ^ 1], s->b.hist);
+                           s->b.vals[in ^ 1], sof, s->b.hist);
     }
-    // After 4 passes the sorted pairs are back in keys[0] / vals[0] == order.
     launch_fill_i32(s, s->b.stradCount, 0, 1);
     launch_fill_i32(s, s->b.leafNeed, 0, n);
+    SceneKeys sk{batched ? s->b.sceneOf : nullptr, s->b.sceneOff, s->b.sceneRoot};
     hipLaunchKernelGGL(k_karras, dim3(blocks(n - 1, 256)), dim3(256), 0, s->stream, n,
                        s->b.keys[0], s->b.order, s->b.nodes, s->b.parent, s->b.rng, s->b.kids,
-                       s->b.strad, s->b.stradCount, s->b.leafNeed);
+                       s->b.strad, s->b.stradCount, s->b.leafNeed, sk);
+}
+
+void launch_scene_end_flags(yb_sim* s) {
+    hipLaunchKernelGGL(k_scene_end_flags, dim3(blocks(s->numScenes, 256)), dim3(256), 0, s->stream,
+                       s->numScenes, s->b.sceneOff, s->b.prm, s->b.sceneEndFlags);
 }
 
 void launch_refit(yb_sim* s) {
@@ -777,7 +875,8 @@ void launch_refit(yb_sim* s) {
 void launch_query(yb_sim* s) {
     hipLaunchKernelGGL(k_query, dim3(blocks(s->n, kQueryThreads)), dim3(kQueryThreads), 0,
                        s->stream, s->b.params, s->b.order, s->b.aabb, s->b.nodes, s->b.pos,
-                       s->b.prm, s->b.numCols, s->b.cols, s->b.err);
+                       s->b.prm, s->b.numCols, s->b.cols, s->b.err,
+                       s->numScenes > 1 ? s->b.sceneOf : nullptr, s->b.sceneRoot);
 }
 
 }  // namespace yb

@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/yarnball_hip.h"
 
@@ -105,6 +106,13 @@ struct Buffers {
     // broadphase
     float4* aabb = nullptr;       // 2n: lo (xyz), hi (xyz) per segment
     float4* sceneBox = nullptr;   // 2 float4: union of all segment boxes
+    // batched scenes (numScenes > 1): scene of each vertex, first vertex of each
+    // scene (+ n), per-scene boxes (2 per scene), root node of each scene's subtree
+    int32_t* sceneOf = nullptr;
+    int32_t* sceneOff = nullptr;
+    float4* sceneBoxes = nullptr;
+    int32_t* sceneRoot = nullptr;
+    int32_t* sceneEndFlags = nullptr;
     float4* blockBox = nullptr;   // 2 per reduction block
     uint32_t* morton = nullptr;   // n, indexed by segment
     uint32_t* keys[2] = {nullptr, nullptr};
@@ -156,6 +164,9 @@ struct yb_sim {
     int refitLevels = 0;
     int refitTopBlocks = 0;
     bool treeBuilt = false;
+    bool uploaded = false;
+    int numScenes = 1;
+    std::vector<int32_t> sceneOffHost;  // first vertex of each scene, + n
     float lastBVHRebuild = INFINITY;  // sim time since the last full rebuild
     int64_t stepCounter = 0;
     // captured substep graphs: [detect mode 0=none,1=refit,2=rebuild]
@@ -211,6 +222,7 @@ void launch_separation(yb_sim* s);              // contact distances of the curr
 // kernel honours SolverCtl (early exit after convergence, residual partials).
 void launch_sweep(yb_sim* s, int sweep, bool adaptive, bool partials);  // cosserat.cu:20-253
 void launch_residual(yb_sim* s, int sweep);
+void launch_scene_end_flags(yb_sim* s);  // flags of each scene's last vertex -> sceneEndFlags
 void launch_ctl_reset(yb_sim* s);
 void launch_frame_refresh(yb_sim* s);  // closed-form frame update from dx[0]
 void launch_end(yb_sim* s, int fin, bool adaptive);         // iteration.cu:48-61

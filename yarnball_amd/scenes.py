@@ -190,9 +190,9 @@ def knit_patch(rows=256, per=4096, yarn_r=1e-4, seg_len=2e-4, seed=0, numItr=8):
     return sc
 
 
-def random_walk(nverts, seed=0, yarn_r=1e-4, seg_len=2e-4, kBend=1e-8, numItr=8):
-    """One strand whose tangent turns by N(0, 0.3) rad per segment (curvature
-    ~ N(0, 0.3/segLen)); first two vertices pinned."""
+def random_walk(nverts, seed=0, yarn_r=1e-4, seg_len=2e-4, kBend=1e-8, numItr=8, turn=0.3):
+    """One strand whose tangent turns by N(0, turn) rad per segment (curvature
+    ~ N(0, turn/segLen)); first two vertices pinned."""
     rng = np.random.default_rng(seed)
     t = np.array([1.0, 0.0, 0.0])
     pos = np.zeros((nverts, 3))
@@ -200,14 +200,81 @@ def random_walk(nverts, seed=0, yarn_r=1e-4, seg_len=2e-4, kBend=1e-8, numItr=8)
         axis = rng.normal(size=3)
         axis -= axis.dot(t) * t
         axis /= max(np.linalg.norm(axis), 1e-12)
-        ang = rng.normal(0.0, 0.3)
+        ang = rng.normal(0.0, turn)
         t = t * np.cos(ang) + np.cross(axis, t) * np.sin(ang)
         t /= np.linalg.norm(t)
         pos[i] = pos[i - 1] + seg_len * t
+    return _walk_scene(pos, seed, yarn_r, kBend, numItr)
+
+
+def _walk_scene(pos, seed, yarn_r, kBend, numItr):
+    nverts = len(pos)
     radius, barrier = _curve_radius(yarn_r)
     return _configure(f"random_walk_{nverts}_{seed}", pos.astype(np.float32), [nverts - 1],
                       [0, 1], kCollision=1e-7, kBend=kBend, kStretch=2e-2, maxH=1e-3,
                       radius=radius, barrier=barrier, numItr=numItr)
+
+
+def random_walks(sizes, seed=0, yarn_r=1e-4, seg_len=2e-4, kbend=(1e-9, 1e-7), numItr=8,
+                 turn=0.3):
+    """Many random-walk strands (as random_walk), generated together: one
+    seeded stream, the tangent update vectorised across the strands, kBend
+    log-uniform in kbend.  Returns a list of configured scenes."""
+    rng = np.random.default_rng(seed)
+    sizes = np.asarray(sizes, np.int64)
+    m = len(sizes)
+    order = np.argsort(-sizes, kind="stable")  # active strands form a prefix
+    ssz = sizes[order]
+    # step-major storage: step i holds the positions of the strands longer than i
+    active = np.searchsorted(-ssz, -np.arange(int(ssz[0])), side="left")
+    base = np.concatenate([[0], np.cumsum(active)])
+    flat = np.zeros((int(base[-1]), 3))
+    t = np.tile(np.array([1.0, 0.0, 0.0]), (m, 1))
+    cur = np.zeros((m, 3))
+    for i in range(1, int(ssz[0])):
+        a = int(active[i])
+        tt = t[:a]
+        axis = rng.normal(size=(a, 3))
+        axis -= np.sum(axis * tt, axis=1, keepdims=True) * tt
+        axis /= np.maximum(np.linalg.norm(axis, axis=1, keepdims=True), 1e-12)
+        ang = rng.normal(0.0, turn, size=(a, 1))
+        tt = tt * np.cos(ang) + np.cross(axis, tt) * np.sin(ang)
+        tt /= np.linalg.norm(tt, axis=1, keepdims=True)
+        t[:a] = tt
+        cur[:a] += seg_len * tt
+        flat[base[i]:base[i] + a] = cur[:a]
+    walks = [flat[base[:int(ssz[r])] + r] for r in range(m)]
+    kb = np.exp(rng.uniform(np.log(kbend[0]), np.log(kbend[1]), size=m))
+    out = [None] * m
+    for r, k in enumerate(order):
+        out[k] = _walk_scene(walks[r], seed * 100000 + int(k), yarn_r, float(kb[k]), numItr)
+    return out
+
+
+def batch(scenes_, name="batch"):
+    """Concatenate configured scenes into one HostScene plus scene offsets
+    (for Sim.setScenes).  The scenes must share the global parameters
+    (gravity, radius, barrier, kCollision, maxH, schedule); the batch takes
+    the first scene's, with the smallest minSegLen and largest maxSegLen.
+    Glue indices (connectionIndex >= 0) are shifted to the batch's indexing."""
+    offs = np.zeros(len(scenes_) + 1, np.int32)
+    offs[1:] = np.cumsum([sc.n for sc in scenes_])
+    verts = np.concatenate([sc.verts for sc in scenes_])
+    cid = verts["connectionIndex"]
+    for k, sc in enumerate(scenes_):
+        sl = slice(offs[k], offs[k + 1])
+        cid[sl] = np.where(cid[sl] >= 0, cid[sl] + offs[k], cid[sl])
+    p = scenes_[0].params.copy()
+    p.numVerts = int(offs[-1])
+    p.minSegLen = min(float(sc.params.minSegLen) for sc in scenes_)
+    p.maxSegLen = max(float(sc.params.maxSegLen) for sc in scenes_)
+    out = HostScene(name, verts, np.concatenate([sc.vels for sc in scenes_]),
+                    np.concatenate([sc.qs for sc in scenes_]),
+                    np.concatenate([sc.qRests for sc in scenes_]), p, scenes_[0].maxH,
+                    np.concatenate([sc.initialInvMasses for sc in scenes_]),
+                    frame_dt=scenes_[0].frame_dt, frames=scenes_[0].frames,
+                    init_pos=np.concatenate([sc.init_pos for sc in scenes_]))
+    return out, offs
 
 
 def drive_frame(sc, pos, vels, t0, dt):

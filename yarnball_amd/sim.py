@@ -169,6 +169,12 @@ class Sim:
         check(self._lib.yb_solver_stats(self._h, ctypes.byref(st), int(bool(reset))))
         return {f: getattr(st, f) for f, _ in st._fields_}
 
+    def setScenes(self, offsets):
+        """Batched scenes: scene k owns vertices [offsets[k], offsets[k+1])."""
+        self._need_handle()
+        off = np.ascontiguousarray(offsets, np.int32)
+        check(self._lib.yb_set_scenes(self._h, len(off) - 1, ptr(off)), "setScenes")
+
     def setGraphs(self, on=True):
         """Run substeps as captured HIP graphs (default) or enqueue every
         kernel directly (same results; for tools that cannot trace graphs)."""
