@@ -115,6 +115,13 @@ def bench_batched(args):
     if args.residual > 0:
         print("[bench] batched scenes run the fixed sweep schedule (numItr)", file=sys.stderr)
 
+    tuned = None
+    if not args.no_autotune:
+        from yarnball_amd import autotune
+        tuned = autotune.tune(big, offsets=offs, substeps=2,
+                              log=lambda m: print(m, file=sys.stderr, flush=True))
+        args.launch_config = (tuned["sweep_threads"], tuned["query_threads"])
+
     b = Sim(big.n)
     # the random walks cross themselves here and there at t = 0: reported as
     # min_separation below instead of a warning per advance
@@ -124,6 +131,8 @@ def bench_batched(args):
     b.minSeparation(reset=True)
     if args.no_graphs:
         b.setGraphs(False)
+    if args.launch_config:
+        b.setLaunchConfig(*args.launch_config)
     for w in range(args.warmup):
         b.advance(frame)
     t0 = time.perf_counter()
@@ -159,6 +168,8 @@ def bench_batched(args):
             s_ = Sim(sc.n)
             s_.printErrors = False
             scenes.load_into(s_, sc)
+            if args.launch_config:
+                s_.setLaunchConfig(*args.launch_config)
             sims.append(s_)
         for w in range(args.warmup):
             for s_ in sims:
@@ -218,6 +229,9 @@ def bench_batched(args):
                    "launch": "eager" if args.no_graphs else "hip graphs",
                    "min_separation_x2r": round(min_sep, 3)},
         "per_scene_launches": per,
+        "launch_config": {"sweep_threads": (args.launch_config or (256, 128))[0],
+                          "query_threads": (args.launch_config or (256, 128))[1],
+                          "autotune": tuned},
         "batched_vs_per_scene": round(value / per["value"], 3) if per else None,
         "roofline": {"kernel": "sweep", "bound": "hbm",
                      "achieved": round(alg / (sweep_us * 1e-6) / 1e9, 1), "peak": HBM_PEAK_GBS,
@@ -242,6 +256,8 @@ def run_method(sc, args, method, model):
         learned.use(sim, model)
     if args.no_graphs:
         sim.setGraphs(False)
+    if args.launch_config:
+        sim.setLaunchConfig(*args.launch_config)
     frame = np.float32(sc.frame_dt)
     substeps_per_frame = int(np.ceil(frame / np.float32(sc.maxH)))
     for w in range(args.warmup):
@@ -298,6 +314,8 @@ def main():
     ap.add_argument("--model", default=os.path.join(ROOT, "yarnball_amd", "models",
                                                     "warmstart.npy"))
     ap.add_argument("--scenes", type=int, default=512, help="batched_walks: number of scenes")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default block sizes instead of timing the candidates")
     ap.add_argument("--no-per-scene", action="store_true",
                     help="batched_walks: skip the one-handle-per-scene comparison")
     args = ap.parse_args()
@@ -305,6 +323,7 @@ def main():
         print("bench.py measures one GPU: the stepper runs on a single MI355X "
               "(multi-GPU sharding is design-only)", file=sys.stderr)
         sys.exit(2)
+    args.launch_config = None
     if args.workload == "batched_walks":
         return bench_batched(args)
 
@@ -315,6 +334,12 @@ def main():
     print(f"[bench] workload {sc.name}: {sc.n} vertices, built in "
           f"{time.perf_counter() - t_build:.1f}s", file=sys.stderr, flush=True)
     methods = ["plain", "learned"] if args.method == "both" else [args.method]
+    tuned = None
+    if not args.no_autotune:
+        from yarnball_amd import autotune
+        tuned = autotune.tune(sc, residual=args.residual, max_sweeps=args.max_sweeps, substeps=1,
+                              log=lambda m: print(m, file=sys.stderr, flush=True))
+        args.launch_config = (tuned["sweep_threads"], tuned["query_threads"])
     model = None
     if "learned" in methods:
         model = np.load(args.model, allow_pickle=False)
@@ -434,6 +459,9 @@ def main():
         },
         "methods": method_rows,
         "learned_vs_plain": speedup,
+        "launch_config": {"sweep_threads": sim.launchConfig()[0],
+                          "query_threads": sim.launchConfig()[1],
+                          "autotune": tuned},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -221,6 +221,15 @@ int yb_solver_stats(yb_sim* s, yb_solver_stats_t* out, int reset);
  * (before or after yb_upload). */
 int yb_set_scenes(yb_sim* s, int numScenes, const int32_t* offsets);
 
+/* Launch shape (no counterpart in the reference): threads per block of the
+ * Newton sweep (64, 128, 256 or 512; default 256) and of the contact query
+ * (64, 128 or 256; default 128).  Results do not depend on it -- every
+ * kernel's output per vertex or segment is a function of its inputs only, and
+ * the residual partials are per 62-vertex wave tile -- only the speed does;
+ * yarnball_amd/autotune.py picks it by timing. */
+int yb_set_launch_config(yb_sim* s, int sweepThreads, int queryThreads);
+int yb_get_launch_config(yb_sim* s, int* sweepThreads, int* queryThreads);
+
 /* Launch mode.  on = 1 (default): each substep's kernels run as captured HIP
  * graphs (step.cpp:36-59 likewise captures the sweep chain).  on = 0: the same
  * kernels are enqueued one by one on the handle's stream -- same results,

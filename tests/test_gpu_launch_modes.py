@@ -1,18 +1,21 @@
 """Eager launches (every kernel enqueued directly) and captured HIP graphs run
-the same kernels in the same order: bit-identical states, in the fixed-sweep
-and in the residual-targeted schedule."""
+the same kernels in the same order, and any block sizes the autotuner may
+pick compute the same values: bit-identical states, in the fixed-sweep and
+in the residual-targeted schedule."""
 import numpy as np
 import pytest
 
 from yarnball_amd import Sim, scenes
 
 
-def _state(sc, graphs, residual):
+def _state(sc, graphs, residual, config=None):
     sim = Sim(sc.n)
     scenes.load_into(sim, sc)
     if residual > 0:
         sim.setSolver(residual, maxSweeps=4000, chunk=16, checkEvery=4)
     sim.setGraphs(graphs)
+    if config:
+        sim.setLaunchConfig(*config)
     sim.advance(np.float32(6 * sc.maxH))
     sim.download()
     st = sim.solverStats() if residual > 0 else None
@@ -32,3 +35,17 @@ def test_eager_equals_graphs(residual):
     if residual > 0:
         assert a[3]["sweeps"] == b[3]["sweeps"] and a[3]["substeps"] == b[3]["substeps"] == 6
         print(f"residual {residual}: {a[3]['sweeps']} sweeps in both launch modes")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("residual", [0.0, 1e-5])
+def test_launch_shape_does_not_change_results(residual):
+    """The autotuner's candidates: block sizes of the sweep and the query."""
+    sc = scenes.knit_patch(rows=8, per=256)
+    ref = _state(sc, True, residual)
+    for cfg in [(64, 64), (128, 256), (512, 128)]:
+        got = _state(sc, True, residual, cfg)
+        for x, y in zip(ref[:3], got[:3]):
+            assert np.array_equal(x, y), cfg
+        if residual > 0:
+            assert ref[3]["sweeps"] == got[3]["sweeps"], cfg

@@ -124,6 +124,8 @@ SIGNATURES = {
     "yb_warm_start_substeps": (_I, [_P, _P, _I]),
     "yb_set_graphs": (_I, [_P, _I]),
     "yb_set_scenes": (_I, [_P, _I, _P]),
+    "yb_set_launch_config": (_I, [_P, _I, _I]),
+    "yb_get_launch_config": (_I, [_P, _P, _P]),
     "yb_record": (_I, [_P, ctypes.c_int64]),
     "yb_record_enable": (_I, [_P, _I]),
     "yb_record_fetch": (_I, [_P, _P, _P, ctypes.POINTER(ctypes.c_int64)]),

@@ -161,7 +161,7 @@ extern "C" int yb_create(int n, yb_sim** out) {
     ALLOC(b.predC, N);
     ALLOC(b.mlpCount, 2);
     ALLOC(b.recCount, 2);
-    ALLOC(b.partial, (size_t)sweep_blocks(n));
+    ALLOC(b.partial, (size_t)sweep_waves(n));
 #undef ALLOC
     if (e == hipSuccess) e = hipHostMalloc((void**)&b.hostFlag, 2 * sizeof(int32_t));
     if (e == hipSuccess) e = hipEventCreateWithFlags(&s->chunkEv[0], hipEventDisableTiming);
@@ -598,6 +598,30 @@ extern "C" int yb_set_solver(yb_sim* s, float residualTol, int maxSweeps, int ch
     s->maxSweeps = maxSweeps;
     s->chunk = chunk;
     s->checkEvery = checkEvery;
+    return YB_OK;
+}
+
+extern "C" int yb_set_launch_config(yb_sim* s, int sweepThreads, int queryThreads) {
+    if (!s) return YB_ERR_INVALID_ARGUMENT;
+    const bool okS = sweepThreads == 64 || sweepThreads == 128 || sweepThreads == 256 ||
+                     sweepThreads == 512;
+    const bool okQ = queryThreads == 64 || queryThreads == 128 || queryThreads == 256;
+    if (!okS || !okQ) {
+        set_message("yb_set_launch_config: sweep threads 64/128/256/512, query threads 64/128/256");
+        return YB_ERR_INVALID_ARGUMENT;
+    }
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    s->sweepThreads = sweepThreads;
+    s->queryThreads = queryThreads;
+    s->graphItr = -1;
+    s->adaptiveReady = false;
+    return YB_OK;
+}
+
+extern "C" int yb_get_launch_config(yb_sim* s, int* sweepThreads, int* queryThreads) {
+    if (!s || !sweepThreads || !queryThreads) return YB_ERR_INVALID_ARGUMENT;
+    *sweepThreads = s->sweepThreads;
+    *queryThreads = s->queryThreads;
     return YB_OK;
 }
 

@@ -655,7 +655,6 @@ __global__ __launch_bounds__(256) void k_refit_top(int n, int nb,
 // segments' largest warm-start displacement.  Because the test is symmetric,
 // segment i lists j exactly when j lists i -- the set the reference builds
 // with two atomicAdds per pair.
-constexpr int kQueryThreads = 128;
 constexpr int kStack = 64;
 
 __device__ __forceinline__ bool overlap(v3 qlo, v3 qhi, float4 lo, float4 hi) {
@@ -699,6 +698,7 @@ __device__ __forceinline__ bool pair_accepted(const DevParams* __restrict__ P,
     return d2 < r * r;
 }
 
+template <int kQueryThreads>
 __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __restrict__ P,
                                                          const int32_t* __restrict__ order,
                                                          const float4* __restrict__ aabb,
@@ -872,11 +872,21 @@ void launch_refit(yb_sim* s) {
     }
 }
 
+template <int T>
+static void launch_query_t(yb_sim* s) {
+    hipLaunchKernelGGL(k_query<T>, dim3(blocks(s->n, T)), dim3(T), 0, s->stream, s->b.params,
+                       s->b.order, s->b.aabb, s->b.nodes, s->b.pos, s->b.prm, s->b.numCols,
+                       s->b.cols, s->b.err, s->numScenes > 1 ? s->b.sceneOf : nullptr,
+                       s->b.sceneRoot);
+}
+
 void launch_query(yb_sim* s) {
-    hipLaunchKernelGGL(k_query, dim3(blocks(s->n, kQueryThreads)), dim3(kQueryThreads), 0,
-                       s->stream, s->b.params, s->b.order, s->b.aabb, s->b.nodes, s->b.pos,
-                       s->b.prm, s->b.numCols, s->b.cols, s->b.err,
-                       s->numScenes > 1 ? s->b.sceneOf : nullptr, s->b.sceneRoot);
+    if (s->queryThreads == 64)
+        launch_query_t<64>(s);
+    else if (s->queryThreads == 256)
+        launch_query_t<256>(s);
+    else
+        launch_query_t<128>(s);
 }
 
 }  // namespace yb

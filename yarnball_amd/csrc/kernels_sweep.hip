@@ -79,7 +79,7 @@ __global__ __launch_bounds__(256) void k_step_limit(const DevParams* __restrict_
     reduce_min_sep(minContact / (2.0f * P->radius), minSep);
 }
 
-__device__ __forceinline__ void block_residual(float num, double* partial, int tile);
+__device__ __forceinline__ void wave_residual(float num, double* partial, int wave, int n);
 
 // Closest points of segment e = (0 -> e) and (B0 -> B1), specialised from
 // closest_uv (Common.h:318-334) with the segment's |e|^2 and 1/|e|^2 hoisted
@@ -116,7 +116,7 @@ __device__ __forceinline__ void closest_e(v3 e, float a, float inva, v3 B0, v3 B
 // tile's first vertex without writing it, so that lane 62's frame update has
 // dx'(i+1); it also supplies q(i+1).  All exchange is cross-lane shifts --
 // no LDS, no barrier (the reference uses a 2-sector LDS reduction).
-__global__ __launch_bounds__(256) void k_sweep(const DevParams* __restrict__ P,
+__global__ __launch_bounds__(512) void k_sweep(const DevParams* __restrict__ P,
                                                const float4* __restrict__ pos,
                                                const float4* __restrict__ prm,
                                                const float4* __restrict__ yv,
@@ -325,26 +325,19 @@ __global__ __launch_bounds__(256) void k_sweep(const DevParams* __restrict__ P,
         }
         qOut[v] = f4r(q0);
     }
-    if (partial) block_residual(owner ? num : 0.0f, partial, tile);
+    if (partial) wave_residual(owner ? num : 0.0f, partial, wave, n);
 }
 
 // Per-sweep relative Newton residual
 //   r = sqrt( sum_free |dx_k+1 - dx_k|^2 / sum_free l_i^2 ),
 // the RMS change of the iterate in units of the rest length (l_i: the rest
 // length of vertex i's own segment, or of the previous one at a strand end).
-// Per-block partial sums here, one small reduction in k_residual.
-__device__ __forceinline__ void block_residual(float num, double* partial, int tile) {
-    __shared__ float sn[4];
+// One partial sum per wave tile of 62 vertices (so the partials, and the
+// residual, do not depend on the block size), one small reduction in k_residual.
+__device__ __forceinline__ void wave_residual(float num, double* partial, int wave, int n) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) num += __shfl_xor(num, off, 64);
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) sn[w] = num;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double a = 0.0;
-        for (int k = 0; k < (int)(blockDim.x >> 6); k++) a += sn[k];
-        partial[tile] = a;
-    }
+    if ((threadIdx.x & 63) == 0 && wave < (n + 61) / 62) partial[wave] = (double)num;
 }
 
 __global__ __launch_bounds__(1024) void k_residual(SolverCtl* __restrict__ ctl,
@@ -567,12 +560,15 @@ void launch_separation(yb_sim* s) {
                        s->b.err, s->b.minSep);
 }
 
-int sweep_blocks(int n) { return blocks(blocks(n, 62) * 64, 256); }
+// wave tiles of 62 vertices (a 64-lane wave with a one-vertex halo each side)
+int sweep_waves(int n) { return blocks(n, 62); }
 
 void launch_sweep(yb_sim* s, int sweep, bool adaptive, bool partials) {
     const int in = sweep & 1;
     const int out = in ^ 1;
-    hipLaunchKernelGGL(k_sweep, dim3(sweep_blocks(s->n)), dim3(256), 0, s->stream, s->b.params,
+    const int threads = s->sweepThreads;
+    hipLaunchKernelGGL(k_sweep, dim3(blocks(sweep_waves(s->n) * 64, threads)), dim3(threads), 0,
+                       s->stream, s->b.params,
                        s->b.pos, s->b.prm, s->b.yv, s->b.q[in], s->b.q[out], s->b.qRest,
                        s->b.dx[in], s->b.dx[out], s->b.numCols, s->b.cols,
                        adaptive ? s->b.ctl : nullptr, sweep, partials ? s->b.partial : nullptr);
@@ -580,7 +576,7 @@ void launch_sweep(yb_sim* s, int sweep, bool adaptive, bool partials) {
 
 void launch_residual(yb_sim* s, int sweep) {
     hipLaunchKernelGGL(k_residual, dim3(1), dim3(1024), 0, s->stream, s->b.ctl, s->b.partial,
-                       sweep_blocks(s->n), sweep, s->tol, s->maxSweeps);
+                       sweep_waves(s->n), sweep, s->tol, s->maxSweeps);
 }
 
 void launch_frame_refresh(yb_sim* s) {

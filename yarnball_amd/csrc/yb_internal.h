@@ -173,6 +173,8 @@ struct yb_sim {
     hipGraphExec_t graphs[3] = {nullptr, nullptr, nullptr};
     int graphItr = -1;
     int useGraphs = 1;  // 0: every kernel enqueued directly on the stream
+    int sweepThreads = 256;  // block size of k_sweep (64, 128, 256 or 512)
+    int queryThreads = 128;  // block size of k_query (64, 128 or 256)
     // residual-targeted mode (tol > 0): prologue graphs per detection mode,
     // one graph per chunk of sweeps, one end graph
     float tol = 0.f;
@@ -226,7 +228,7 @@ void launch_scene_end_flags(yb_sim* s);  // flags of each scene's last vertex ->
 void launch_ctl_reset(yb_sim* s);
 void launch_frame_refresh(yb_sim* s);  // closed-form frame update from dx[0]
 void launch_end(yb_sim* s, int fin, bool adaptive);         // iteration.cu:48-61
-int sweep_blocks(int n);
+int sweep_waves(int n);
 // learned warm start (mlp.hip)
 void launch_mlp_pack(yb_sim* s);
 void launch_features(yb_sim* s, bool infer, bool record);

@@ -175,6 +175,27 @@ class Sim:
         off = np.ascontiguousarray(offsets, np.int32)
         check(self._lib.yb_set_scenes(self._h, len(off) - 1, ptr(off)), "setScenes")
 
+    def setLaunchConfig(self, sweepThreads=256, queryThreads=128):
+        """Threads per block of the sweep and the contact query (speed only)."""
+        self._need_handle()
+        check(self._lib.yb_set_launch_config(self._h, int(sweepThreads), int(queryThreads)))
+
+    def launchConfig(self):
+        self._need_handle()
+        a, b = ctypes.c_int(0), ctypes.c_int(0)
+        check(self._lib.yb_get_launch_config(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def profile(self, substeps=2):
+        """Run `substeps` substeps of maxH with HIP events around every stage
+        (yb_profile); returns {stage: (milliseconds, launches)}."""
+        self._need_handle()
+        ms = np.zeros(len(_lib.STAGES), np.float64)
+        nl = np.zeros(len(_lib.STAGES), np.int32)
+        check(self._lib.yb_profile(self._h, ctypes.byref(self.meta), float(np.float32(self.maxH)),
+                                   int(substeps), ptr(ms), ptr(nl)), "profile")
+        return {st: (float(ms[i]), int(nl[i])) for i, st in enumerate(_lib.STAGES)}
+
     def setGraphs(self, on=True):
         """Run substeps as captured HIP graphs (default) or enqueue every
         kernel directly (same results; for tools that cannot trace graphs)."""
