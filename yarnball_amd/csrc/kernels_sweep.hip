@@ -81,19 +81,18 @@ __global__ __launch_bounds__(256) void k_step_limit(const DevParams* __restrict_
 
 __device__ __forceinline__ void wave_residual(float num, double* partial, int wave, int n);
 
-// Closest points of segment e = (0 -> e) and (B0 -> B1), specialised from
+// Closest points of segment e = (0 -> e) and (B0 -> B0 + A1), specialised from
 // closest_uv (Common.h:318-334) with the segment's |e|^2 and 1/|e|^2 hoisted
 // out of the contact loop.
-__device__ __forceinline__ void closest_e(v3 e, float a, float inva, v3 B0, v3 B1, float& u,
+__device__ __forceinline__ void closest_e(v3 e, float a, float inva, v3 B0, v3 A1, float& u,
                                          float& w) {
-    const v3 A1 = B1 - B0;
     const float b = dot(e, A1);
     const float c = len2(A1);
     float vy = b * dot(e, B0) - a * dot(A1, B0);
     vy = vy / (a * c - b * b);
     if (!isfinite(vy)) vy = 0.5f;
     vy = clampf(vy, 0.f, 1.f);
-    const float ux = clampf(dot(mixv(B0, B1, vy), e) * inva, 0.f, 1.f);
+    const float ux = clampf(dot(B0 + A1 * vy, e) * inva, 0.f, 1.f);
     vy = clampf(dot(e * ux - B0, A1) / c, 0.f, 1.f);
     u = ux;
     w = vy;
@@ -216,19 +215,22 @@ __global__ __launch_bounds__(512) void k_sweep(const DevParams* __restrict__ P,
                     const v3 db0 = xyz(dxIn[o]);
                     const v3 db1 = xyz(dxIn[o + 1]);
                     o = oNext;
+                    // partner segment relative to this vertex, in the
+                    // current configuration x + dx
+                    const v3 B0 = (b0 - x0) + (db0 - dx);
+                    const v3 A1 = ((b1 - x0) + (db1 - dx)) - B0;
                     float u, w;
-                    closest_e(e, ea, inva, (b0 - x0) + (db0 - dx), (b1 - x0) + (db1 - dx), u, w);
-                    const v3 dpos = mixv(x0, p1, u) - mixv(b0, b1, w);
-                    const v3 ddpos = mixv(dx, p1dx, u) - mixv(db0, db1, w);
-                    v3 nrm = dpos + ddpos;
+                    closest_e(e, ea, inva, B0, A1, u, w);
+                    // closest-point difference (pos + dx of this segment minus
+                    // the partner's) and its length
+                    v3 nrm = e * u - (B0 + A1 * w);
                     const float dist = len(nrm);
-                    nrm *= 1.0f / dist;
-                    const float w1 = u;
-                    const float w0 = 1.0f - u;
-
                     float dd = (dist - radius2) * invb;
                     if (dd > 1.0f) continue;  // outside the barrier
                     dd = fmaxf(dd, 1e-3f);
+                    nrm *= 1.0f / dist;
+                    const float w1 = u;
+                    const float w0 = 1.0f - u;
 
                     // b(d) = -(d-1)^2 ln d; ff = -kCol b'(d), dH = kCol/b b''(d)
                     const float invd = 1.0f / dd;
@@ -238,25 +240,25 @@ __global__ __launch_bounds__(512) void k_sweep(const DevParams* __restrict__ P,
                     f += (ff * w0 - damping * dH * w0 * w0 * dot(nrm, dx)) * nrm;
                     f2 += (ff * w1 - damping * dH * w1 * w1 * dot(nrm, p1dx)) * nrm;
                     dH *= 1.0f + damping;
-                    const sym3 op = sym_outer(nrm);
-                    sym_axpy(H, op, dH * w0 * w0);
-                    sym_axpy(H2, op, dH * w1 * w1);
 
-                    // Lagged friction on the tangential relative displacement.
+                    // Lagged friction on the tangential relative displacement;
+                    // its Hessian f1 w^2 (I - n n^T) is merged with the
+                    // barrier's dH w^2 n n^T into one rank-1 update + diagonal.
+                    const v3 ddpos = mixv(dx, p1dx, u) - mixv(db0, db1, w);
                     const v3 uu = ddpos - dot(nrm, ddpos) * nrm;
                     const float ul = len(uu);
+                    float f1 = 0.0f;
                     if (ul > 0.0f) {
-                        const float f1 = fminf(fricK, fricMu * ff / ul);
-                        const float s0 = w0 * w0 * f1;
-                        const float s1 = w1 * w1 * f1;
+                        f1 = fminf(fricK, fricMu * ff / ul);
                         f -= (f1 * w0) * uu;
                         f2 -= (f1 * w1) * uu;
-                        // H -= (nn^T - I) s
-                        sym_axpy(H, op, -s0);
-                        sym_add_diag(H, s0);
-                        sym_axpy(H2, op, -s1);
-                        sym_add_diag(H2, s1);
                     }
+                    const sym3 op = sym_outer(nrm);
+                    const float s0 = w0 * w0, s1 = w1 * w1;
+                    sym_axpy(H, op, (dH - f1) * s0);
+                    sym_add_diag(H, f1 * s0);
+                    sym_axpy(H2, op, (dH - f1) * s1);
+                    sym_add_diag(H2, f1 * s1);
                 }
             }
         }
