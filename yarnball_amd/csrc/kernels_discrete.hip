@@ -656,6 +656,7 @@ __global__ __launch_bounds__(256) void k_refit_top(int n, int nb,
 // segment i lists j exactly when j lists i -- the set the reference builds
 // with two atomicAdds per pair.
 constexpr int kStack = 64;
+constexpr int kCand = 16;  // candidate leaves buffered per thread before the narrowphase
 
 __device__ __forceinline__ bool overlap(v3 qlo, v3 qhi, float4 lo, float4 hi) {
     return (qlo.x <= hi.x) && (lo.x <= qhi.x) && (qlo.y <= hi.y) && (lo.y <= qhi.y) &&
@@ -711,6 +712,8 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
                                                          const int32_t* __restrict__ sceneOf,
                                                          const int32_t* __restrict__ sceneRoot) {
     __shared__ int32_t stk[kStack][kQueryThreads];
+    __shared__ int32_t candJ[kCand][kQueryThreads];
+    __shared__ float candPad[kCand][kQueryThreads];
     const int n = P->n;
     // Segments in index order: consecutive segments of a strand are spatial
     // neighbours (coherent traversals) and the slot-major list writes
@@ -738,6 +741,29 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
     int node = sceneOf ? sceneRoot[sceneOf[i]] : 0;
     int cnt = 0;
     bool overflowStack = false;
+    // The traversal only collects overlapping leaves, in visit order; the
+    // exact pair test runs over the buffered candidates when some thread of
+    // the wave has nearly filled its buffer, so the wave evaluates pair tests
+    // together instead of once per node visit at which any thread met a leaf.
+    // Candidates keep their visit order, hence so do the contact lists.
+    int nb = 0;
+    auto narrowphase = [&]() {
+#pragma unroll 1
+        for (int k = 0; k < nb; k++) {
+            const int j = candJ[k][t];
+            const float pw = candPad[k][t];
+            SegInfo sj;
+            sj.p0 = xyz(pos[j]);
+            sj.p1 = xyz(pos[j + 1]);
+            sj.vpad = fabsf(pw);
+            sj.glued = signbit(pw);
+            if (pair_accepted(P, prm, i, si, j, sj)) {
+                if (cnt < kMaxCols) cols[i + n * cnt] = j;
+                cnt++;
+            }
+        }
+        nb = 0;
+    };
     while (true) {
         const float4 l0 = nodes[4 * node + 0];
         const float4 l1 = nodes[4 * node + 1];
@@ -747,24 +773,18 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
         const int cr = __float_as_int(r0.w);
         const bool hl = overlap(qlo, qhi, l0, l1);
         const bool hr = overlap(qlo, qhi, r0, r1);
-#pragma unroll
-        for (int side = 0; side < 2; side++) {
-            const bool hit = side ? hr : hl;
-            const int c = side ? cr : cl;
-            if (hit && c < 0 && ~c != i) {
-                const int j = ~c;
-                const float pw = side ? r1.w : l1.w;
-                SegInfo sj;
-                sj.p0 = xyz(pos[j]);
-                sj.p1 = xyz(pos[j + 1]);
-                sj.vpad = fabsf(pw);
-                sj.glued = signbit(pw);
-                if (pair_accepted(P, prm, i, si, j, sj)) {
-                    if (cnt < kMaxCols) cols[i + n * cnt] = j;
-                    cnt++;
-                }
-            }
+        if (hl && cl < 0 && ~cl != i) {
+            candJ[nb][t] = ~cl;
+            candPad[nb][t] = l1.w;
+            nb++;
         }
+        if (hr && cr < 0 && ~cr != i) {
+            candJ[nb][t] = ~cr;
+            candPad[nb][t] = r1.w;
+            nb++;
+        }
+        // at most two leaves per visit: flush before a buffer could overflow
+        if (__any(nb > kCand - 2)) narrowphase();
         const bool tl = hl && cl >= 0;
         const bool tr = hr && cr >= 0;
         if (tl || tr) {
@@ -783,6 +803,7 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
             node = stk[sp][t];
         }
     }
+    narrowphase();
     if (cnt > kMaxCols) {
         err[0] = YB_ERROR_MAX_COLLISIONS_PER_SEGMENT_EXCEEDED;
         cnt = kMaxCols;
