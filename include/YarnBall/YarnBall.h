@@ -133,6 +133,18 @@ struct MetaData {
 };
 static_assert(sizeof(MetaData) == sizeof(yb_params), "MetaData mirrors yb_params");
 
+// Column-major 4x4 matrix (glm::mat4 layout: m[column][row]), for the
+// transform argument of the curve readers.
+struct mat4 {
+    vec4 m[4];
+    explicit mat4(float d = 1.f) {
+        for (int c = 0; c < 4; c++)
+            for (int r = 0; r < 4; r++) m[c][r] = c == r ? d : 0.f;
+    }
+    vec4& operator[](int c) { return m[c]; }
+    const vec4& operator[](int c) const { return m[c]; }
+};
+
 class Sim {
 public:
     enum {
@@ -178,6 +190,15 @@ public:
     void printCollisionStats();
     Kit::LBVH::aabb bounds();
 
+    // Scene files (io/fileExport.cpp, io/jsonBuilder.cpp:174-212).
+    void exportToBCC(std::string path, bool exportAsPolyline = false);
+    void exportToOBJ(std::string path);
+    void glueEndpoints(float searchRadius);
+
+    // A Sim holding a yb_scene's arrays (configured: also created on the
+    // device and uploaded, as buildFromJSON returns it).
+    static Sim* fromScene(yb_scene* scene, bool configured);
+
     // Device handle, for callers that need the C ABI directly.
     yb_sim* deviceHandle() { return handle; }
 
@@ -185,5 +206,14 @@ private:
     yb_sim* handle = nullptr;
     void raise(int code, const char* where);
 };
+
+// Scene readers (io/reader.cpp:50-295, io/jsonBuilder.cpp:8-170).
+Sim* readFromOBJ(std::string path, float targetSegLen, mat4 transform,
+                 bool breakUpClosedCurves = false, bool allowResample = true);
+Sim* readFromBCC(std::string path, float targetSegLen, mat4 transform,
+                 bool breakUpClosedCurves = false, bool allowResample = true);
+Sim* readFromPoly(std::string path, float targetSegLen, mat4 transform,
+                  bool breakUpClosedCurves = false, bool allowResample = true);
+Sim* buildFromJSON(std::string path);
 
 }  // namespace YarnBall

@@ -153,6 +153,40 @@ int yb_min_separation(yb_sim* s, float* out, int reset);
 /* Last error message (thread-local), for negative status codes. */
 const char* yb_last_message(void);
 
+/* ---- scene files (host side, no device needed) ------------------------------
+ * A yb_scene holds what Sim(numVerts) plus the caller's edits hold before
+ * upload(): yb_params, maxH, the AoS vertices, velocities, frames, rest
+ * frames and the initial inverse masses.  Read it out with yb_scene_get
+ * (null pointers skip an array) and free it with yb_scene_free. */
+typedef struct yb_scene yb_scene;
+/* buildFromJSON (jsonBuilder.cpp:8-170): curve file + resampling, curveRadius,
+ * the "simulation" block, configure(density), setKStretch/setKBend,
+ * glueEndpoints, fixBorders, fixVertex.  Comments and trailing commas, as in
+ * the shipped configs, are accepted.  A relative curveFile is looked up in the
+ * working directory, then beside the JSON file and one directory above it. */
+int yb_build_from_json(const char* path, yb_scene** out);
+/* readFromPoly / readFromOBJ / readFromBCC (reader.cpp:50-295), chosen by the
+ * file extension (.poly, .obj, else BCC): curves -> vertices with strand ends
+ * and closed-curve connections (createFromCurves, reader.cpp:13-48), Catmull-
+ * Rom resampling to targetSegLen (resample.h).  transform: 12 floats, a
+ * row-major 3x4 affine map applied to every point, or null for the reference's
+ * default 0.01 scale.  The scene is not configured. */
+int yb_read_curves(const char* path, float targetSegLen, const float* transform,
+                   int breakUpClosedCurves, int allowResample, yb_scene** out);
+int yb_scene_size(const yb_scene* s, int* numVerts);
+int yb_scene_get(const yb_scene* s, yb_params* p, float* maxH, yb_vertex* verts, float* vels,
+                 float* qs, float* qRests, float* initialInvMasses);
+void yb_scene_free(yb_scene* s);
+/* Sim::glueEndpoints (jsonBuilder.cpp:174-212) on host vertices: each strand
+ * endpoint, in index order, whose nearest vertex more than 2 indices away is
+ * within searchRadius is connected to it (connectionIndex both ways) unless
+ * either is connected already, and both move to their midpoint. */
+int yb_glue_endpoints(int n, yb_vertex* verts, float searchRadius);
+/* Sim::exportToBCC / exportToOBJ (fileExport.cpp:6-100) of host vertices
+ * (call yb_download first, as the reference's methods do). */
+int yb_export_bcc(const char* path, int n, const yb_vertex* verts, int asPolyline);
+int yb_export_obj(const char* path, int n, const yb_vertex* verts);
+
 /* ---- stage-level access used by the parity tests -------------------------
  * These expose intermediate device state so that each stage can be compared
  * with the CPU oracle in isolation.  They have no counterpart in the reference

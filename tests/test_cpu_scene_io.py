@@ -1,0 +1,218 @@
+"""Scene files on the host: the lenient JSON scene description and its
+fields, Catmull-Rom resampling (closed forms: a straight line resamples to
+uniform spacing, a circle stays on the circle), endpoint gluing against a
+brute-force restatement of the reference loop (jsonBuilder.cpp:174-212), and
+BCC / OBJ round trips."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from yarnball_amd import scene_io
+from yarnball_amd._lib import VERTEX_DTYPE
+
+IDENTITY = [[1, 0, 0, 0], [0, 1, 0, 0], [0, 0, 1, 0]]
+
+
+def write_poly(path, curves):
+    """curves: list of (points (k,3), closed) -> .poly with 1-based ids."""
+    pts, segs = [], []
+    for p, closed in curves:
+        base = len(pts)
+        pts.extend(p.tolist())
+        k = len(p)
+        segs.extend((base + a, base + a + 1) for a in range(k - 1))
+        if closed:
+            segs.append((base + k - 1, base))
+    with open(path, "w") as f:
+        f.write("POINTS\n")
+        for i, q in enumerate(pts):
+            f.write(f"{i + 1}: {q[0]!r} {q[1]!r} {q[2]!r}\n")
+        f.write("POLYS\n")
+        for i, (a, b) in enumerate(segs):
+            f.write(f"{i + 1}: {a + 1} {b + 1}\n")
+        f.write("END\n")
+
+
+def seg_lengths(v):
+    seg = (v["flags"] & 2) != 0
+    d = np.linalg.norm(np.diff(v["pos"].astype(np.float64), axis=0), axis=1)
+    return d[seg[:-1]]
+
+
+def test_straight_line_resamples_uniformly(tmp_path):
+    # uniform Catmull-Rom through collinear, evenly spaced points is the line
+    # itself, linear in the parameter: marching gives exactly even spacing
+    cp = np.stack([np.arange(50.0), np.zeros(50), np.zeros(50)], axis=1)
+    write_poly(tmp_path / "line.poly", [(cp, False)])
+    sc = scene_io.read_curves(tmp_path / "line.poly", 0.47, transform=IDENTITY)
+    L = seg_lengths(sc.verts)
+    assert sc.n == 101 and len(L) == 100
+    assert np.allclose(L, 0.47, rtol=1e-5)
+    p = sc.verts["pos"]
+    assert np.allclose(p[0], [1, 0, 0]) and np.allclose(p[-1], [48, 0, 0])  # t from 1 to n-2
+    assert sc.verts["flags"][-1] == 0
+
+
+def test_circle_resampling_stays_on_the_circle(tmp_path):
+    a = np.linspace(0, 2 * np.pi, 200, endpoint=False)
+    cp = np.stack([np.cos(a), np.sin(a), np.zeros_like(a)], axis=1)
+    write_poly(tmp_path / "ring.poly", [(cp, True)])
+    sc = scene_io.read_curves(tmp_path / "ring.poly", 0.05, transform=IDENTITY)
+    p = sc.verts["pos"].astype(np.float64)
+    L = seg_lengths(sc.verts)
+    assert abs(L.mean() - 0.05) < 0.05 * 0.02 and L.max() - L.min() < 0.05 * 0.02
+    assert np.max(np.abs(np.linalg.norm(p[:, :2], axis=1) - 1.0)) < 1e-4
+    # closed curve: ends connected to each other
+    assert sc.verts["connectionIndex"][0] == sc.n - 1 and sc.verts["connectionIndex"][-1] == 0
+
+
+def test_json_scene_fields(tmp_path):
+    t = np.linspace(0, 4 * np.pi, 120)
+    helix = np.stack([np.cos(t), np.sin(t), 0.2 * t], axis=1) * 10.0  # file units
+    a = np.linspace(0, 2 * np.pi, 80, endpoint=False)
+    ring = np.stack([3 * np.cos(a), 3 * np.sin(a), np.full_like(a, 5.0)], axis=1) * 10.0
+    write_poly(tmp_path / "shape.poly", [(helix, False), (ring, True)])
+    text = """{
+      // comments and trailing commas, as in the shipped configs
+      "curveFile": "shape.poly",
+      "resampleLength": 0.01,
+      "breakUpClosedCurves": false,
+      "curveRadius": 0.002,
+      "fixVertex": [[0.1, 0.0, 0.0], [0.3, 0.0, 0.5, 0.02], 5,],
+      "simulation": {
+        "density": 2e-3, "frictionCoeff": 0.3, "kStretch": 0.1, "kBend": 1e-7,
+        "kCollision": 1e-4, "kFriction": 3, "damping": 1e-6,
+        "gravity": [0, 0, -9.8], "drag": 0.5,
+        "maxTimeStep": 5e-4, "numIterations": 6, "detectionPeriod": 2,
+        "detectionScaler": 1.2, "stepLimit": false, "velStepLimit": true,
+        "bvhRebuildPeriod": 0.05,
+      },
+    }"""
+    (tmp_path / "scene.json").write_text(text)
+    sc = scene_io.build_from_json(tmp_path / "scene.json")
+    p = sc.params
+    assert abs(p.radius - 0.05 * 0.002) < 1e-9 and abs(p.barrierThickness - 2 * 0.95 * 0.002) < 1e-9
+    assert np.isclose(p.frictionCoeff, 0.3) and np.isclose(p.kCollision, 1e-4)
+    assert np.isclose(p.kFriction, 3) and np.isclose(p.damping, 1e-6) and np.isclose(p.drag, 0.5)
+    assert list(p.gravity) == pytest.approx([0, 0, -9.8])
+    assert np.isclose(sc.maxH, 5e-4) and p.numItr == 6 and p.detectionPeriod == 2
+    assert np.isclose(p.detectionScaler, 1.2) and p.useStepSizeLimit == 0
+    assert p.useVelocityRadius == 1 and np.isclose(p.bvhRebuildPeriod, 0.05)
+    v = sc.verts
+    # default transform: 0.01 scale; open curve first, then the closed one
+    L = seg_lengths(v)
+    assert abs(L.mean() - 0.01) < 2e-4
+    ends = np.nonzero((v["flags"] & 2) == 0)[0]  # vertices without a next one
+    assert len(ends) == 2
+    first_ring = ends[0] + 1
+    assert v["connectionIndex"][first_ring] == sc.n - 1  # ring closed through the glue link
+    # configure + setKStretch: kStretch = k * lRest
+    seg = (v["flags"] & 2) != 0
+    assert np.allclose(v["kStretch"][seg], 0.1 * v["lRest"][seg], rtol=1e-6)
+    # fixVertex: the closest vertex to (0.1, 0, 0), all within 0.02 of (0.3, 0, 0.5) (none on
+    # the shapes), and vertex index 5
+    pos = v["pos"].astype(np.float64)
+    closest = int(np.argmin(np.sum((pos - [0.1, 0, 0]) ** 2, axis=1)))
+    pinned = set(np.nonzero(v["invMass"] == 0)[0].tolist())
+    assert closest in pinned and 5 in pinned
+    near = np.nonzero(np.sum((pos - [0.3, 0, 0.5]) ** 2, axis=1) < 0.02 ** 2)[0]
+    assert set(near.tolist()) <= pinned
+    assert len(pinned) == 2 + len(near)
+
+
+def brute_glue(verts, radius):
+    """The reference loop (jsonBuilder.cpp:174-212), endpoints in index order."""
+    v = verts.copy()
+    n = len(v)
+    fl = v["flags"]
+    for i in range(n):
+        has_prev = i > 0 and (fl[i - 1] & 2) != 0
+        has_next = (fl[i] & 2) != 0
+        if has_prev == has_next:
+            continue
+        p = v["pos"]
+        best, closest = np.float32(np.inf), -1
+        for j in range(n):
+            if abs(i - j) <= 2:
+                continue
+            d = p[i] - p[j]
+            dist = np.sqrt((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2], dtype=np.float32)
+            if dist < best:
+                best, closest = dist, j
+        if closest >= 0 and best <= np.float32(radius):
+            if v["connectionIndex"][closest] < 0 and v["connectionIndex"][i] < 0:
+                v["connectionIndex"][closest] = i
+                v["connectionIndex"][i] = closest
+                mid = np.float32(0.5) * (p[i] + p[closest])
+                v["pos"][closest] = mid
+                v["pos"][i] = mid
+    return v
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_glue_endpoints_matches_the_reference_loop(seed):
+    rng = np.random.default_rng(seed)
+    strands, per = 12, 15
+    v = np.zeros(strands * per, VERTEX_DTYPE)
+    for s in range(strands):
+        start = rng.uniform(0, 0.05, 3)
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        for k in range(per):
+            v["pos"][s * per + k] = start + 0.002 * k * d + rng.normal(0, 2e-4, 3)
+    v["flags"] = 2
+    v["flags"][per - 1::per] = 0
+    v["connectionIndex"] = -1
+    v["invMass"] = 1
+    want = brute_glue(v, 0.006)
+    got = v.copy()
+    scene_io.glue_endpoints(got, 0.006)
+    assert np.array_equal(got["connectionIndex"], want["connectionIndex"])
+    assert np.array_equal(got["pos"], want["pos"])
+    assert (got["connectionIndex"] >= 0).sum() > 0
+
+
+def test_bcc_and_obj_round_trip(tmp_path):
+    t = np.linspace(0, 3, 40)
+    a = np.stack([t, np.sin(t), np.cos(t)], axis=1) * 0.1
+    b = np.stack([np.cos(t), t, np.sin(t)], axis=1) * 0.1
+    write_poly(tmp_path / "two.poly", [(a, False), (b, False)])
+    sc = scene_io.read_curves(tmp_path / "two.poly", 0.01, transform=IDENTITY)
+    scene_io.export_bcc(tmp_path / "out.bcc", sc.verts, asPolyline=True)
+    back = scene_io.read_curves(tmp_path / "out.bcc", 0.01, transform=IDENTITY, allowResample=False)
+    assert np.array_equal(back.verts["pos"], sc.verts["pos"])
+    assert np.array_equal(back.verts["flags"] == 0, sc.verts["flags"] == 0)
+    raw = (tmp_path / "out.bcc").read_bytes()
+    assert raw[:3] == b"BCC" and raw[3] == 0x44 and raw[4:6] == b"PL" and raw[6] == 3
+    scene_io.export_obj(tmp_path / "out.obj", sc.verts)
+    back = scene_io.read_curves(tmp_path / "out.obj", 0.01, transform=IDENTITY, allowResample=False)
+    assert np.array_equal(back.verts["pos"], sc.verts["pos"])
+    lines = [l for l in (tmp_path / "out.obj").read_text().splitlines() if l.startswith("l ")]
+    assert len(lines) == 2
+
+
+def test_bad_inputs_raise(tmp_path):
+    (tmp_path / "bad.json").write_text('{"curveFile": "x.poly", "simulation": {')
+    with pytest.raises(Exception):
+        scene_io.build_from_json(tmp_path / "bad.json")
+    # a vertex with three neighbours is a graph, not a curve
+    write_poly(tmp_path / "g.poly", [(np.zeros((0, 3)), False)])
+    (tmp_path / "g.poly").write_text("POINTS\n1: 0 0 0\n2: 1 0 0\n3: 0 1 0\n4: 0 0 1\n"
+                                     "POLYS\n1: 1 2\n2: 1 3\n3: 1 4\nEND\n")
+    with pytest.raises(Exception):
+        scene_io.read_curves(tmp_path / "g.poly", 0.1, transform=IDENTITY)
+
+
+REF_CONFIG = "/root/reference/KittenEngine/configs/letterA.json"
+
+
+@pytest.mark.skipif(not os.path.exists(REF_CONFIG), reason="reference configs not present")
+def test_reference_letter_config_builds():
+    sc = scene_io.build_from_json(REF_CONFIG)
+    v = sc.verts
+    L = seg_lengths(v)
+    assert sc.n > 30000 and abs(L.mean() - 2e-3) < 2e-5
+    assert int((v["invMass"] == 0).sum()) == 6       # six fixVertex entries
+    assert np.isclose(sc.params.radius, 0.05 * 0.001) and np.isclose(sc.maxH, 4e-4)

@@ -125,6 +125,14 @@ SIGNATURES = {
     "yb_set_graphs": (_I, [_P, _I]),
     "yb_set_scenes": (_I, [_P, _I, _P]),
     "yb_set_launch_config": (_I, [_P, _I, _I]),
+    "yb_build_from_json": (_I, [ctypes.c_char_p, _P]),
+    "yb_read_curves": (_I, [ctypes.c_char_p, _F, _P, _I, _I, _P]),
+    "yb_scene_size": (_I, [_P, _P]),
+    "yb_scene_get": (_I, [_P, ctypes.POINTER(Params), _P, _P, _P, _P, _P, _P]),
+    "yb_scene_free": (None, [_P]),
+    "yb_glue_endpoints": (_I, [_I, _P, _F]),
+    "yb_export_bcc": (_I, [ctypes.c_char_p, _I, _P, _I]),
+    "yb_export_obj": (_I, [ctypes.c_char_p, _I, _P]),
     "yb_get_launch_config": (_I, [_P, _P, _P]),
     "yb_record": (_I, [_P, ctypes.c_int64]),
     "yb_record_enable": (_I, [_P, _I]),

@@ -142,4 +142,77 @@ void Sim::printCollisionStats() {
     printf("Segments with collisions: %d/%d\n", numSegWithCols, meta.numVerts);
 }
 
+// ---- scene files ----------------------------------------------------------------
+void Sim::exportToBCC(std::string path, bool exportAsPolyline) {
+    download();
+    if (yb_export_bcc(path.c_str(), meta.numVerts, reinterpret_cast<const yb_vertex*>(verts),
+                      exportAsPolyline ? 1 : 0) != YB_OK)
+        fprintf(stderr, "Error opening file\n");
+}
+
+void Sim::exportToOBJ(std::string path) {
+    download();
+    if (yb_export_obj(path.c_str(), meta.numVerts, reinterpret_cast<const yb_vertex*>(verts)) !=
+        YB_OK)
+        fprintf(stderr, "Error opening file\n");
+}
+
+void Sim::glueEndpoints(float searchRadius) {
+    yb_glue_endpoints(meta.numVerts, reinterpret_cast<yb_vertex*>(verts), searchRadius);
+}
+
+Sim* Sim::fromScene(yb_scene* scene, bool configured) {
+    int n = 0;
+    yb_scene_size(scene, &n);
+    Sim* sim = new Sim(n);
+    sim->initialInvMasses.resize(n);
+    yb_scene_get(scene, reinterpret_cast<yb_params*>(&sim->meta), &sim->maxH,
+                 reinterpret_cast<yb_vertex*>(sim->verts), &sim->vels[0].x, &sim->qs[0].x,
+                 &sim->qRests[0].x, sim->initialInvMasses.data());
+    yb_scene_free(scene);
+    if (configured) {
+        int rc = yb_create(n, &sim->handle);
+        if (rc == YB_OK) rc = yb_validate_params(reinterpret_cast<yb_params*>(&sim->meta));
+        if (rc != YB_OK) {
+            std::string msg = yb_last_message();
+            delete sim;
+            throw std::runtime_error(msg);
+        }
+        sim->upload();
+    } else {
+        sim->initialInvMasses.clear();
+    }
+    return sim;
+}
+
+static Sim* read_curves(const std::string& path, float segLen, const mat4& t, bool breakUp,
+                        bool resample) {
+    float tf[12];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 4; j++) tf[4 * i + j] = t[j][i];
+    yb_scene* sc = nullptr;
+    if (yb_read_curves(path.c_str(), segLen, tf, breakUp ? 1 : 0, resample ? 1 : 0, &sc) != YB_OK)
+        throw std::runtime_error(yb_last_message());
+    return Sim::fromScene(sc, false);
+}
+
+Sim* readFromOBJ(std::string path, float targetSegLen, mat4 transform, bool breakUpClosedCurves,
+                 bool allowResample) {
+    return read_curves(path, targetSegLen, transform, breakUpClosedCurves, allowResample);
+}
+Sim* readFromBCC(std::string path, float targetSegLen, mat4 transform, bool breakUpClosedCurves,
+                 bool allowResample) {
+    return read_curves(path, targetSegLen, transform, breakUpClosedCurves, allowResample);
+}
+Sim* readFromPoly(std::string path, float targetSegLen, mat4 transform, bool breakUpClosedCurves,
+                  bool allowResample) {
+    return read_curves(path, targetSegLen, transform, breakUpClosedCurves, allowResample);
+}
+
+Sim* buildFromJSON(std::string path) {
+    yb_scene* sc = nullptr;
+    if (yb_build_from_json(path.c_str(), &sc) != YB_OK) throw std::runtime_error(yb_last_message());
+    return Sim::fromScene(sc, true);
+}
+
 }  // namespace YarnBall
