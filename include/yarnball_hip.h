@@ -153,6 +153,39 @@ int yb_min_separation(yb_sim* s, float* out, int reset);
 /* Last error message (thread-local), for negative status codes. */
 const char* yb_last_message(void);
 
+/* ---- kinematic drivers on the device ---------------------------------------
+ * The reference's scenario animations (main.cpp:54-121) move the pinned
+ * vertices on the host between download() and upload() every frame; here one
+ * kernel applies the same per-frame update to the device state.
+ *   twist: every pinned vertex on the chosen side of center.y is placed at
+ *          center + rotateY(initial - center, angle(t)) with velocity
+ *          (position(t + dt) - position(t)) / dt, angle(t) = speed *
+ *          clamp(t - start, 0, duration); after t > release it gets its
+ *          initial inverse mass back (release < 0: never).
+ *   pull:  gravity becomes (0, 0, -9.8) in *p; every pinned vertex gets
+ *          y = initial.y + x(t) below center.y and initial.y - x(t) above, with
+ *          x(t) = -speed * clamp(t - start, 0, duration), and the matching
+ *          y velocity.
+ * yb_driver_default fills main.cpp's constants (twist: pi rad/s from t = 2 s
+ * for 12 s, released at 15 s, the vertices below the centre; pull: 0.25 m/s
+ * from 2 s for 6 s).  yb_driver_init records the current device positions as
+ * the initial ones, the initial inverse masses (null: the current ones), and
+ * returns the centre of their bounding box (main.cpp:273-280). */
+#define YB_DRIVE_TWIST 1
+#define YB_DRIVE_PULL 2
+typedef struct yb_driver {
+    int32_t kind;
+    float center[3];
+    float speed;    /* rad/s (twist) or m/s (pull) */
+    float start;    /* s */
+    float duration; /* s */
+    float release;  /* s, twist only; < 0: never */
+    int32_t side;   /* twist: -1 below center.y, +1 above, 0 every pinned vertex */
+} yb_driver;
+int yb_driver_default(int kind, yb_driver* d);
+int yb_driver_init(yb_sim* s, const float* initialInvMasses, float center[3]);
+int yb_driver_frame(yb_sim* s, yb_params* p, const yb_driver* d, float t, float dt);
+
 /* ---- scene files (host side, no device needed) ------------------------------
  * A yb_scene holds what Sim(numVerts) plus the caller's edits hold before
  * upload(): yb_params, maxH, the AoS vertices, velocities, frames, rest

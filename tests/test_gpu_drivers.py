@@ -1,0 +1,116 @@
+"""Scenario drivers on the device (main.cpp:54-121): one frame of the twist
+and of the pull matches a restatement of the reference's host loops on the
+same state; the twist releases its vertices after the release time; and the
+benchmark cable driven on the device tracks the same cable driven from the
+host (download, move the pinned vertices, upload)."""
+import numpy as np
+import pytest
+
+from yarnball_amd import Sim, scenes
+from yarnball_amd._lib import DRIVE_PULL, DRIVE_TWIST
+
+
+def ref_twist(pos, vel, inv, init, inv0, center, t, dt, d):
+    """main.cpp:54-87 on host arrays (float32, in place)."""
+    f = np.float32
+    a0 = f(d.speed) * f(min(max(f(t) - f(d.start), 0.0), d.duration))
+    a1 = f(d.speed) * f(min(max(f(t) + f(dt) - f(d.start), 0.0), d.duration))
+    for i in range(len(pos)):
+        if not (inv[i] == 0 and ((pos[i, 1] < center[1]) == (d.side < 0) or d.side == 0)):
+            continue
+        r = init[i] - center
+        def at(a):
+            c, s = np.cos(a, dtype=np.float32), np.sin(a, dtype=np.float32)
+            return center + np.array([c * r[0] - s * r[2], r[1], s * r[0] + c * r[2]], np.float32)
+        p0, p1 = at(a0), at(a1)
+        pos[i] = p0
+        vel[i] = (p1 - p0) / f(dt)
+        if d.release >= 0 and t > d.release:
+            inv[i] = inv0[i]
+
+
+def ref_pull(pos, vel, inv, init, center, t, dt, d):
+    """main.cpp:90-121 on host arrays (float32, in place)."""
+    f = np.float32
+    x0 = -f(d.speed) * f(min(max(f(t) - f(d.start), 0.0), d.duration))
+    x1 = -f(d.speed) * f(min(max(f(t) + f(dt) - f(d.start), 0.0), d.duration))
+    for i in range(len(pos)):
+        if inv[i] != 0:
+            continue
+        below = pos[i, 1] < center[1]
+        y0 = init[i, 1] + (x0 if below else -x0)
+        y1 = init[i, 1] + (x1 if below else -x1)
+        pos[i, 1] = y0
+        vel[i, 1] = (y1 - y0) / f(dt)
+
+
+def _sim(sc):
+    s = Sim(sc.n)
+    s.printErrors = False
+    scenes.load_into(s, sc)
+    return s
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", [DRIVE_TWIST, DRIVE_PULL])
+def test_driver_frame_matches_reference_loop(kind):
+    sc = scenes.twisted_cable(strands=7, per=64) if kind == DRIVE_TWIST else \
+        scenes.knit_patch(rows=4, per=64)
+    s = _sim(sc)
+    # initial inverse masses as configure() computes them before the scenario
+    # pins vertices (here the pins predate configure, so use unit masses)
+    inv0 = np.where(sc.initialInvMasses == 0, np.float32(1.0), sc.initialInvMasses)
+    center = s.driverInit(inv0)
+    init = sc.verts["pos"].copy()
+    d = Sim.defaultDriver(kind)
+    d.center[0], d.center[1], d.center[2] = center
+    dt = np.float32(1.0 / 30.0)
+    checked = 0
+    for t in (1.0, 2.5, 9.0, 16.0):          # before the start, moving, later, released
+        s.advance(dt)
+        s.download()
+        pos, vel, inv = s.verts["pos"].copy(), s.vels.copy(), s.verts["invMass"].copy()
+        if kind == DRIVE_TWIST:
+            ref_twist(pos, vel, inv, init, inv0, center, t, dt, d)
+        else:
+            ref_pull(pos, vel, inv, init, center, t, dt, d)
+        s.driveFrame(d, t, dt)
+        s.download()
+        assert np.allclose(s.verts["pos"], pos, rtol=0, atol=1e-7), t
+        assert np.allclose(s.vels, vel, rtol=1e-4, atol=1e-6), t
+        assert np.array_equal(s.verts["invMass"], inv), t
+        checked += int((sc.initialInvMasses == 0).sum())
+    if kind == DRIVE_PULL:
+        assert list(s.meta.gravity) == pytest.approx([0, 0, -9.8])
+    else:  # released after 15 s: the driven side has its inverse masses back
+        assert (s.verts["invMass"] == 0).sum() < (sc.verts["invMass"] == 0).sum()
+    s.close()
+
+
+@pytest.mark.gpu
+def test_device_twist_tracks_host_twist_on_the_cable():
+    sc = scenes.twisted_cable(strands=7, per=256)
+    frame = np.float32(sc.frame_dt)
+    host = _sim(sc)
+    dev = _sim(sc)
+    dev.driverInit()
+    d = Sim.defaultDriver(DRIVE_TWIST)   # the cable's driver: 1 rev/s about the y axis,
+    d.speed, d.start, d.duration, d.release, d.side = 2 * np.pi, 0.0, 1e9, -1.0, 1  # top end
+    d.center[0], d.center[1], d.center[2] = 0.0, float(np.mean(sc.verts["pos"][:, 1])), 0.0
+    t = 0.0
+    for f in range(4):
+        host.download()
+        scenes.drive_frame(sc, host.verts["pos"], host.vels, t, float(frame))
+        host.upload()
+        host.advance(frame)
+        dev.driveFrame(d, t, float(frame))
+        dev.advance(frame)
+        t += float(frame)
+    host.download()
+    dev.download()
+    scale = np.max(np.abs(host.verts["pos"]))
+    err = np.max(np.abs(dev.verts["pos"] - host.verts["pos"])) / scale
+    print(f"device vs host twist after 4 frames: rel {err:.2e}")
+    assert err < 1e-5
+    host.close()
+    dev.close()

@@ -49,6 +49,16 @@ class SolverStats(ctypes.Structure):
     ]
 
 
+class Driver(ctypes.Structure):
+    """yb_driver: scenario animation of the pinned vertices (main.cpp:54-121)."""
+    _fields_ = [("kind", ctypes.c_int32), ("center", ctypes.c_float * 3), ("speed", ctypes.c_float),
+                ("start", ctypes.c_float), ("duration", ctypes.c_float),
+                ("release", ctypes.c_float), ("side", ctypes.c_int32)]
+
+
+DRIVE_TWIST, DRIVE_PULL = 1, 2
+
+
 class Params(ctypes.Structure):
     """yb_params: the scalar fields of MetaData (YarnBall.h:63-114)."""
 
@@ -125,6 +135,9 @@ SIGNATURES = {
     "yb_set_graphs": (_I, [_P, _I]),
     "yb_set_scenes": (_I, [_P, _I, _P]),
     "yb_set_launch_config": (_I, [_P, _I, _I]),
+    "yb_driver_default": (_I, [_I, _P]),
+    "yb_driver_init": (_I, [_P, _P, _P]),
+    "yb_driver_frame": (_I, [_P, ctypes.POINTER(Params), _P, _F, _F]),
     "yb_build_from_json": (_I, [ctypes.c_char_p, _P]),
     "yb_read_curves": (_I, [ctypes.c_char_p, _F, _P, _I, _I, _P]),
     "yb_scene_size": (_I, [_P, _P]),

@@ -50,7 +50,7 @@ static void free_all(yb_sim* s) {
                     b.err,     b.params,  b.stage, b.ctl, b.partial, b.minSep,
                     b.mlpPrm,  b.mlpTiles, b.mlpBias, b.featB, b.corrPrev, b.dxPlain, b.predC,
                     b.mlpCount, b.recX, b.recT, b.recCount, b.sceneOf, b.sceneOff,
-                    b.sceneBoxes, b.sceneRoot, b.sceneEndFlags};
+                    b.sceneBoxes, b.sceneRoot, b.sceneEndFlags, b.drvInit};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (b.hostFlag) (void)hipHostFree(b.hostFlag);
@@ -260,6 +260,70 @@ extern "C" int yb_upload(yb_sim* s, const yb_vertex* verts, const float* vels, c
                             hipMemcpyHostToDevice, s->stream));
     HIPCHECK(hipStreamSynchronize(s->stream));
     s->uploaded = true;
+    return YB_OK;
+}
+
+extern "C" int yb_driver_default(int kind, yb_driver* d) {
+    if (!d || (kind != YB_DRIVE_TWIST && kind != YB_DRIVE_PULL)) return YB_ERR_INVALID_ARGUMENT;
+    memset(d, 0, sizeof(*d));
+    d->kind = kind;
+    if (kind == YB_DRIVE_TWIST) {  // main.cpp:58-61, 73
+        d->speed = 0.5f * 2.0f * (float)M_PI;
+        d->start = 2.0f;
+        d->duration = 12.0f;
+        d->release = 12.0f + 3.0f;
+        d->side = -1;
+    } else {  // main.cpp:95-97
+        d->speed = 0.25f;
+        d->start = 2.0f;
+        d->duration = 6.0f;
+        d->release = -1.0f;
+    }
+    return YB_OK;
+}
+
+extern "C" int yb_driver_init(yb_sim* s, const float* initialInvMasses, float center[3]) {
+    if (!s) return YB_ERR_INVALID_ARGUMENT;
+    if (!s->uploaded) {
+        set_message("yb_driver_init: upload the scene first");
+        return YB_ERR_INVALID_ARGUMENT;
+    }
+    const size_t N = (size_t)s->n;
+    if (!s->b.drvInit) HIPCHECK(hipMalloc(&s->b.drvInit, sizeof(float4) * N));
+    std::vector<float4> p(N);
+    HIPCHECK(hipMemcpyAsync(p.data(), s->b.pos, sizeof(float4) * N, hipMemcpyDeviceToHost,
+                            s->stream));
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (size_t i = 0; i < N; i++) {
+        const float* q = &p[i].x;
+        for (int k = 0; k < 3; k++) {
+            lo[k] = fminf(lo[k], q[k]);
+            hi[k] = fmaxf(hi[k], q[k]);
+        }
+        if (initialInvMasses) p[i].w = initialInvMasses[i];
+    }
+    HIPCHECK(hipMemcpy(s->b.drvInit, p.data(), sizeof(float4) * N, hipMemcpyHostToDevice));
+    if (center)
+        for (int k = 0; k < 3; k++) center[k] = 0.5f * (hi[k] + lo[k]);
+    return YB_OK;
+}
+
+extern "C" int yb_driver_frame(yb_sim* s, yb_params* p, const yb_driver* d, float t, float dt) {
+    if (!s || !p || !d || !(dt > 0.f)) return YB_ERR_INVALID_ARGUMENT;
+    if (!s->b.drvInit) {
+        set_message("yb_driver_frame: call yb_driver_init first");
+        return YB_ERR_INVALID_ARGUMENT;
+    }
+    if (d->kind != YB_DRIVE_TWIST && d->kind != YB_DRIVE_PULL) return YB_ERR_INVALID_ARGUMENT;
+    if (d->kind == YB_DRIVE_PULL) {  // main.cpp:93
+        p->gravity[0] = 0.f;
+        p->gravity[1] = 0.f;
+        p->gravity[2] = -9.8f;
+    }
+    launch_drive(s, *d, t, dt);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(s->stream));
     return YB_OK;
 }
 

@@ -511,6 +511,45 @@ __global__ void k_zero_vel(int n, float4* __restrict__ vel, float4* __restrict__
     lastVel[v] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
+// Scenario drivers (main.cpp:54-121) on the device state.
+__global__ void k_drive(int n, yb_driver d, float t, float dt, float4* __restrict__ pos,
+                        float4* __restrict__ vel, const float4* __restrict__ init) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    float4 p = pos[v];
+    if (p.w != 0.0f) return;  // only pinned vertices are driven
+    const float4 q = init[v];
+    const float cx = d.center[0], cy = d.center[1], cz = d.center[2];
+    const bool below = p.y < cy;
+    if (d.kind == YB_DRIVE_TWIST) {
+        if (d.side != 0 && (d.side < 0) != below) return;
+        const float a0 = d.speed * fminf(fmaxf(t - d.start, 0.0f), d.duration);
+        const float a1 = d.speed * fminf(fmaxf(t + dt - d.start, 0.0f), d.duration);
+        const float rx = q.x - cx, rz = q.z - cz;
+        const float c0 = cosf(a0), s0 = sinf(a0), c1 = cosf(a1), s1 = sinf(a1);
+        const float x0 = cx + (c0 * rx - s0 * rz), z0 = cz + (s0 * rx + c0 * rz);
+        const float x1 = cx + (c1 * rx - s1 * rz), z1 = cz + (s1 * rx + c1 * rz);
+        const float y0 = cy + (q.y - cy);
+        p.x = x0;
+        p.y = y0;
+        p.z = z0;
+        const float4 u = vel[v];
+        vel[v] = make_float4((x1 - x0) / dt, ((cy + (q.y - cy)) - y0) / dt, (z1 - z0) / dt, u.w);
+        if (d.release >= 0.0f && t > d.release) p.w = q.w;
+        pos[v] = p;
+    } else if (d.kind == YB_DRIVE_PULL) {
+        const float x0 = -d.speed * fminf(fmaxf(t - d.start, 0.0f), d.duration);
+        const float x1 = -d.speed * fminf(fmaxf(t + dt - d.start, 0.0f), d.duration);
+        const float y0 = q.y + (below ? x0 : -x0);
+        const float y1 = q.y + (below ? x1 : -x1);
+        p.y = y0;
+        pos[v] = p;
+        float4 u = vel[v];
+        u.y = (y1 - y0) / dt;
+        vel[v] = u;
+    }
+}
+
 __global__ void k_fill_i32(int32_t* __restrict__ p, int32_t val, int count) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < count) p[i] = val;
@@ -538,6 +577,11 @@ void launch_download(yb_sim* s) {
     float* sq = svel + 3 * (size_t)n;
     hipLaunchKernelGGL(k_download, dim3(blocks(n, 256)), dim3(256), 0, s->stream, n, sv, svel, sq,
                        s->b.pos, s->b.prm, s->b.vel, s->b.q[0]);
+}
+
+void launch_drive(yb_sim* s, const yb_driver& d, float t, float dt) {
+    hipLaunchKernelGGL(k_drive, dim3(blocks(s->n, 256)), dim3(256), 0, s->stream, s->n, d, t, dt,
+                       s->b.pos, s->b.vel, s->b.drvInit);
 }
 
 void launch_zero_vel(yb_sim* s) {

@@ -113,6 +113,7 @@ struct Buffers {
     float4* sceneBoxes = nullptr;
     int32_t* sceneRoot = nullptr;
     int32_t* sceneEndFlags = nullptr;
+    float4* drvInit = nullptr;  // scenario driver: initial position xyz, initial inverse mass
     float4* blockBox = nullptr;   // 2 per reduction block
     uint32_t* morton = nullptr;   // n, indexed by segment
     uint32_t* keys[2] = {nullptr, nullptr};
@@ -217,6 +218,7 @@ void set_message(const std::string& m);
 void launch_upload(yb_sim* s);     // staging AoS -> SoA
 void launch_download(yb_sim* s);   // SoA -> staging AoS
 void launch_zero_vel(yb_sim* s);
+void launch_drive(yb_sim* s, const yb_driver& d, float t, float dt);
 void launch_init(yb_sim* s);                    // iteration.cu:7-41
 void launch_step_limit(yb_sim* s);              // collision.cu:121-172
 void launch_separation(yb_sim* s);              // contact distances of the current state

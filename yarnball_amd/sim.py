@@ -196,6 +196,27 @@ class Sim:
                                    int(substeps), ptr(ms), ptr(nl)), "profile")
         return {st: (float(ms[i]), int(nl[i])) for i, st in enumerate(_lib.STAGES)}
 
+    def driverInit(self, initialInvMasses=None):
+        """Record the current device positions (and initial inverse masses) for
+        the scenario drivers; returns the centre of their bounding box."""
+        self._need_handle()
+        c = (ctypes.c_float * 3)()
+        inv = None if initialInvMasses is None else np.ascontiguousarray(initialInvMasses, np.float32)
+        check(self._lib.yb_driver_init(self._h, None if inv is None else ptr(inv), c))
+        return np.array(list(c), np.float32)
+
+    def driveFrame(self, driver, t, dt):
+        """One frame of a scenario driver on the device (before advance(dt))."""
+        self._need_handle()
+        check(self._lib.yb_driver_frame(self._h, ctypes.byref(self.meta), ctypes.byref(driver),
+                                        float(t), float(dt)))
+
+    @staticmethod
+    def defaultDriver(kind):
+        d = _lib.Driver()
+        check(_lib.load().yb_driver_default(int(kind), ctypes.byref(d)))
+        return d
+
     def setGraphs(self, on=True):
         """Run substeps as captured HIP graphs (default) or enqueue every
         kernel directly (same results; for tools that cannot trace graphs)."""
