@@ -327,6 +327,22 @@ extern "C" int yb_driver_frame(yb_sim* s, yb_params* p, const yb_driver* d, floa
     return YB_OK;
 }
 
+// Back to the single-scene layout (frees the batched-scene buffers).
+static int clear_scenes(yb_sim* s) {
+    Buffers& b = s->b;
+    for (void* p : {(void*)b.sceneOf, (void*)b.sceneOff, (void*)b.sceneBoxes, (void*)b.sceneRoot,
+                    (void*)b.sceneEndFlags})
+        if (p) HIPCHECK(hipFree(p));
+    b.sceneOf = b.sceneOff = b.sceneRoot = b.sceneEndFlags = nullptr;
+    b.sceneBoxes = nullptr;
+    s->numScenes = 1;
+    s->sceneOffHost.clear();
+    s->treeBuilt = false;
+    s->graphItr = -1;
+    s->adaptiveReady = false;
+    return YB_OK;
+}
+
 extern "C" int yb_set_scenes(yb_sim* s, int numScenes, const int32_t* offsets) {
     if (!s || numScenes < 1 || !offsets) {
         set_message("yb_set_scenes: invalid argument");
@@ -346,45 +362,37 @@ extern "C" int yb_set_scenes(yb_sim* s, int numScenes, const int32_t* offsets) {
             return YB_ERR_INVALID_ARGUMENT;
         }
     HIPCHECK(hipStreamSynchronize(s->stream));
+    int rc = clear_scenes(s);
+    if (rc != YB_OK || numScenes == 1) return rc;
     Buffers& b = s->b;
-    for (void* p : {(void*)b.sceneOf, (void*)b.sceneOff, (void*)b.sceneBoxes, (void*)b.sceneRoot,
-                    (void*)b.sceneEndFlags})
-        if (p) HIPCHECK(hipFree(p));
-    b.sceneOf = b.sceneOff = b.sceneRoot = b.sceneEndFlags = nullptr;
-    b.sceneBoxes = nullptr;
-    s->numScenes = 1;
-    s->sceneOffHost.clear();
-    if (numScenes > 1) {
-        const size_t N = (size_t)s->n;
-        HIPCHECK(hipMalloc(&b.sceneOf, sizeof(int32_t) * N));
-        HIPCHECK(hipMalloc(&b.sceneOff, sizeof(int32_t) * (numScenes + 1)));
-        HIPCHECK(hipMalloc(&b.sceneBoxes, sizeof(float4) * 2 * numScenes));
-        HIPCHECK(hipMalloc(&b.sceneRoot, sizeof(int32_t) * numScenes));
-        HIPCHECK(hipMalloc(&b.sceneEndFlags, sizeof(int32_t) * numScenes));
-        std::vector<int32_t> of(N);
+    const size_t N = (size_t)s->n;
+    HIPCHECK(hipMalloc(&b.sceneOf, sizeof(int32_t) * N));
+    HIPCHECK(hipMalloc(&b.sceneOff, sizeof(int32_t) * (numScenes + 1)));
+    HIPCHECK(hipMalloc(&b.sceneBoxes, sizeof(float4) * 2 * numScenes));
+    HIPCHECK(hipMalloc(&b.sceneRoot, sizeof(int32_t) * numScenes));
+    HIPCHECK(hipMalloc(&b.sceneEndFlags, sizeof(int32_t) * numScenes));
+    std::vector<int32_t> of(N);
+    for (int k = 0; k < numScenes; k++)
+        for (int v = offsets[k]; v < offsets[k + 1]; v++) of[v] = k;
+    HIPCHECK(hipMemcpy(b.sceneOf, of.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(b.sceneOff, offsets, sizeof(int32_t) * (numScenes + 1),
+                       hipMemcpyHostToDevice));
+    s->numScenes = numScenes;
+    s->sceneOffHost.assign(offsets, offsets + numScenes + 1);
+    if (s->uploaded) {
+        launch_scene_end_flags(s);
+        HIPCHECK(hipGetLastError());
+        std::vector<int32_t> fl(numScenes);
+        HIPCHECK(hipMemcpyAsync(fl.data(), b.sceneEndFlags, sizeof(int32_t) * numScenes,
+                                hipMemcpyDeviceToHost, s->stream));
+        HIPCHECK(hipStreamSynchronize(s->stream));
         for (int k = 0; k < numScenes; k++)
-            for (int v = offsets[k]; v < offsets[k + 1]; v++) of[v] = k;
-        HIPCHECK(hipMemcpy(b.sceneOf, of.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice));
-        HIPCHECK(hipMemcpy(b.sceneOff, offsets, sizeof(int32_t) * (numScenes + 1),
-                           hipMemcpyHostToDevice));
-        s->numScenes = numScenes;
-        s->sceneOffHost.assign(offsets, offsets + numScenes + 1);
-        if (s->uploaded) {
-            launch_scene_end_flags(s);
-            HIPCHECK(hipGetLastError());
-            std::vector<int32_t> fl(numScenes);
-            HIPCHECK(hipMemcpyAsync(fl.data(), b.sceneEndFlags, sizeof(int32_t) * numScenes,
-                                    hipMemcpyDeviceToHost, s->stream));
-            HIPCHECK(hipStreamSynchronize(s->stream));
-            for (int k = 0; k < numScenes; k++)
-                if (fl[k] & YB_FLAG_HAS_NEXT) {
-                    set_message("yb_set_scenes: the last vertex of scene " + std::to_string(k) +
-                                " has a next vertex (a segment would cross into the next scene)");
-                    return yb_set_scenes(s, 1, std::vector<int32_t>{0, s->n}.data()) == YB_OK
-                               ? YB_ERR_INVALID_ARGUMENT
-                               : YB_ERR_HIP;
-                }
-        }
+            if (fl[k] & YB_FLAG_HAS_NEXT) {
+                rc = clear_scenes(s);
+                set_message("yb_set_scenes: the last vertex of scene " + std::to_string(k) +
+                            " has a next vertex (a segment would cross into the next scene)");
+                return rc == YB_OK ? YB_ERR_INVALID_ARGUMENT : rc;
+            }
     }
     // the tree and the captured graphs depend on the scene layout
     s->treeBuilt = false;

@@ -6,9 +6,9 @@
 // c = (dx* - dx0) / L between the converged iterate dx* and the plain warm
 // start dx0, in units of the local rest length L, as a residual on top of the
 // previous substep's converged correction c_prev (also one of its inputs):
-// c = c_prev + MLP(features).  The prediction replaces dx0 before the first
-// sweep; the residual-targeted sweeps then run to the
-// same tolerance as without it, so the plain solver remains the judge of the
+// c = c_prev + MLP(features).  The prediction, low-passed along each strand,
+// replaces dx0 before the first sweep; the residual-targeted sweeps then run
+// to the same tolerance as without it, so the plain solver still decides the
 // final state.  A prediction that is not finite keeps dx0 (counted as a
 // fallback); every prediction is clamped to the step limit like a sweep.
 //
