@@ -24,6 +24,10 @@ extern "C" {
 #endif
 
 #define YB_MAX_COLLISIONS_PER_SEGMENT 128 /* YarnBall.h:20 */
+/* Largest vertex count of one handle (all scenes together): device byte
+ * offsets of per-vertex float4 arrays stay below 2^32.  At this size the
+ * device buffers take about 180 GiB, most of it the 128-slot contact lists. */
+#define YB_MAX_VERTICES (1 << 28)
 
 enum {
     YB_OK = 0,
@@ -112,7 +116,8 @@ int yb_validate_params(yb_params* p);
 /* ---- device lifecycle ---------------------------------------------------- */
 
 /* Device allocation part of Sim::configure (YarnBall.cu:166-198): SoA buffers,
- * one HIP stream, contact lists, BVH storage.  n >= 3 (YarnBall.cu:12). */
+ * one HIP stream, contact lists, BVH storage.  3 <= n <= YB_MAX_VERTICES
+ * (the lower bound is YarnBall.cu:12). */
 int yb_create(int n, yb_sim** out);
 /* Sim::~Sim (YarnBall.cu:53-85). */
 void yb_destroy(yb_sim* s);

@@ -68,6 +68,16 @@ def test_errors_like_the_reference():
         s.configure()
 
 
+def test_create_rejects_sizes_outside_the_limits():
+    # Both checks come before any device call, so they hold without a GPU.
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    for n, msg in ((2, "Too little vertices"), (_lib.MAX_VERTICES + 1, "YB_MAX_VERTICES")):
+        assert lib.yb_create(n, ctypes.byref(h)) == _lib.YB_ERR_INVALID_ARGUMENT
+        assert not h.value
+        assert msg in lib.yb_last_message().decode()
+
+
 def configured(pos, breaks=(), pinned=(), density=1e-3):
     lib = _lib.load()
     n = len(pos)

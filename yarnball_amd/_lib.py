@@ -13,10 +13,16 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libyarnball_hip.so")
 
 MAX_COLLISIONS_PER_SEGMENT = 128
+MAX_VERTICES = 1 << 28
 
 YB_OK = 0
 YB_ERROR_MAX_COLLISIONS_PER_SEGMENT_EXCEEDED = 1
 YB_WARNING_SEGMENT_INTERPENETRATION = 2
+YB_ERR_INVALID_ARGUMENT = -1
+YB_ERR_NOT_CONFIGURED = -2
+YB_ERR_HIP = -3
+YB_ERR_VALIDATION = -4
+YB_ERR_NO_DEVICE = -5
 
 FLAG_HAS_PREV = 1
 FLAG_HAS_NEXT = 2

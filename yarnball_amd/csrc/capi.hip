@@ -97,6 +97,10 @@ extern "C" int yb_create(int n, yb_sim** out) {
         set_message("Too little vertices");  // YarnBall.cu:12
         return YB_ERR_INVALID_ARGUMENT;
     }
+    if (n > YB_MAX_VERTICES) {
+        set_message("yb_create: more than YB_MAX_VERTICES (268435456) vertices");
+        return YB_ERR_INVALID_ARGUMENT;
+    }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
         set_message("yb_create: no HIP device visible");

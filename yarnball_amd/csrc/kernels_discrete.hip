@@ -753,12 +753,12 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
             const int j = candJ[k][t];
             const float pw = candPad[k][t];
             SegInfo sj;
-            sj.p0 = xyz(pos[j]);
-            sj.p1 = xyz(pos[j + 1]);
+            sj.p0 = xyz(gather4(pos, j));
+            sj.p1 = xyz(gather4<true>(pos, j));
             sj.vpad = fabsf(pw);
             sj.glued = signbit(pw);
             if (pair_accepted(P, prm, i, si, j, sj)) {
-                if (cnt < kMaxCols) cols[i + n * cnt] = j;
+                if (cnt < kMaxCols) cols[i + (size_t)n * cnt] = j;
                 cnt++;
             }
         }

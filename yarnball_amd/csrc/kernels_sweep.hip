@@ -23,6 +23,32 @@ __device__ __forceinline__ rot asrot(float4 a) { return rot{a.x, a.y, a.z, a.w};
 __device__ __forceinline__ float4 f4(v3 a, float w) { return make_float4(a.x, a.y, a.z, w); }
 __device__ __forceinline__ float4 f4r(rot a) { return make_float4(a.x, a.y, a.z, a.w); }
 
+// Hardware reciprocal, reciprocal square root and logarithm (1 ulp).  The
+// compiler's fast divide adds a range-scaling step for denominators above
+// 2^96 and its logf a compensated log2 -> ln product; neither is needed here:
+// the denominators are lengths, determinants and masses far below 2^96, and
+// ln is taken only of the barrier distance in [1e-3, 1].  A zero denominator
+// still gives inf, as the division did.
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fln(float x) { return __builtin_amdgcn_logf(x) * 0.693147181f; }
+__device__ __forceinline__ rot rnormalize_fast(rot a) {
+    return rscale(a, __builtin_amdgcn_rsqf(rdot(a, a)));
+}
+// H^{-1} f through the adjugate (sym_solve with the hardware reciprocal).
+__device__ __forceinline__ v3 sym_solve_fast(const sym3& m, v3 f) {
+    const float a = m.a[0], b = m.a[1], c = m.a[2], d = m.a[3], e = m.a[4], g = m.a[5];
+    const float A00 = b * c - g * g;
+    const float A01 = e * g - d * c;
+    const float A02 = d * g - b * e;
+    const float A11 = a * c - e * e;
+    const float A12 = d * e - a * g;
+    const float A22 = a * b - d * d;
+    const float inv = frcp((a * A00 + d * A01) + e * A02);
+    return v3{((A00 * f.x + A01 * f.y) + A02 * f.z) * inv,
+              ((A01 * f.x + A11 * f.y) + A12 * f.z) * inv,
+              ((A02 * f.x + A12 * f.y) + A22 * f.z) * inv};
+}
+
 // ---------------------------------------------------------------------------
 // Step limit (collision.cu:121-159): the largest move for which the padded
 // broadphase query still sees every possible contact, and at most 45% of the
@@ -60,7 +86,7 @@ __global__ __launch_bounds__(256) void k_step_limit(const DevParams* __restrict_
             const v3 a1 = xyz(pos[v + 1]) - a0;
             const int nc = numCols[v];
             for (int i = 0; i < nc; i++) {
-                const int o = cols[v + n * i];
+                const int o = cols[v + (size_t)n * i];
                 const v3 b0 = xyz(pos[o]) - a0;
                 const v3 b1 = xyz(pos[o + 1]) - a0;
                 float u, w;
@@ -83,17 +109,20 @@ __device__ __forceinline__ void wave_residual(float num, double* partial, int wa
 
 // Closest points of segment e = (0 -> e) and (B0 -> B0 + A1), specialised from
 // closest_uv (Common.h:318-334) with the segment's |e|^2 and 1/|e|^2 hoisted
-// out of the contact loop.
+// out of the contact loop.  The two reprojections are the same as in
+// closest_uv, with their dot products expanded over the ones the line solve
+// already has:  (B0 + A1 v).e = e.B0 + v (e.A1),  (e u - B0).A1 = u (e.A1) - A1.B0.
 __device__ __forceinline__ void closest_e(v3 e, float a, float inva, v3 B0, v3 A1, float& u,
                                          float& w) {
     const float b = dot(e, A1);
     const float c = len2(A1);
-    float vy = b * dot(e, B0) - a * dot(A1, B0);
-    vy = vy / (a * c - b * b);
+    const float eB = dot(e, B0);
+    const float AB = dot(A1, B0);
+    float vy = (b * eB - a * AB) * frcp(a * c - b * b);
     if (!isfinite(vy)) vy = 0.5f;
     vy = clampf(vy, 0.f, 1.f);
-    const float ux = clampf(dot(B0 + A1 * vy, e) * inva, 0.f, 1.f);
-    vy = clampf(dot(e * ux - B0, A1) / c, 0.f, 1.f);
+    const float ux = clampf((eB + vy * b) * inva, 0.f, 1.f);
+    vy = clampf((ux * b - AB) * frcp(c), 0.f, 1.f);
     u = ux;
     w = vy;
 }
@@ -151,6 +180,7 @@ __global__ __launch_bounds__(512) void k_sweep(const DevParams* __restrict__ P,
     float4 q4 = make_float4(0.f, 0.f, 0.f, 1.f);
     float4 qr4 = make_float4(0.f, 0.f, 0.f, 0.f);
     v3 p1 = v3{0.f, 0.f, 0.f};
+    float invl = 0.0f;  // 1 / rest length of segment i (set when it exists)
     uint32_t flags = 0;
 
     if (in) {
@@ -158,14 +188,13 @@ __global__ __launch_bounds__(512) void k_sweep(const DevParams* __restrict__ P,
         r4 = prm[v];
         d4 = dxIn[v];
         q4 = qIn[v];
-        qr4 = qRest[v];
         flags = __float_as_uint(r4.w);
         const int cid = __float_as_int(r4.z);
         const v3 x0 = xyz(p4);
         const v3 dx = xyz(d4);
 
         // Inertia (cosserat.cu:40-44); yv holds y - x.
-        const float mh2 = 1.0f / (p4.w * h * h);
+        const float mh2 = frcp(p4.w * h * h);
         H = sym_diag(mh2);
         f = mh2 * (xyz(yv[v]) - dx);
 
@@ -184,7 +213,7 @@ __global__ __launch_bounds__(512) void k_sweep(const DevParams* __restrict__ P,
             segLimit = d4.w;
 
             // Cosserat stretch-shear (cosserat.cu:67-81).
-            const float invl = 1.0f / r4.x;
+            invl = frcp(r4.x);
             const v3 e = (p1 - x0) + (p1dx - dx);
             const v3 c = e * invl - rrotate_e1(asrot(q4));
             const float k = r4.y * invl;
@@ -205,15 +234,19 @@ __global__ __launch_bounds__(512) void k_sweep(const DevParams* __restrict__ P,
                 const float fricMu = P->frictionCoeff;
                 const float kCol = P->kCollision * invb;
                 const float ea = len2(e);
-                const float inva = 1.0f / ea;
+                const float inva = frcp(ea);
+                // friction's isotropic Hessian part f1 w^2 I, summed as scalars
+                float diag0 = 0.0f, diag1 = 0.0f;
+                const v3 dseg = p1dx - dx;  // this segment's relative displacement
                 int o = cols[v];
                 for (int i = 0; i < nc; i++) {
-                    // next partner id fetched one iteration ahead
-                    const int oNext = (i + 1 < nc) ? cols[v + n * (i + 1)] : o;
-                    const v3 b0 = xyz(pos[o]);
-                    const v3 b1 = xyz(pos[o + 1]);
-                    const v3 db0 = xyz(dxIn[o]);
-                    const v3 db1 = xyz(dxIn[o + 1]);
+                    // next partner id fetched one iteration ahead; slot-major
+                    // lists span n * 128 entries, past the int range
+                    const int oNext = (i + 1 < nc) ? cols[v + (size_t)n * (i + 1)] : o;
+                    const v3 b0 = xyz(gather4(pos, o));
+                    const v3 b1 = xyz(gather4<true>(pos, o));
+                    const v3 db0 = xyz(gather4(dxIn, o));
+                    const v3 db1 = xyz(gather4<true>(dxIn, o));
                     o = oNext;
                     // partner segment relative to this vertex, in the
                     // current configuration x + dx
@@ -227,14 +260,14 @@ __global__ __launch_bounds__(512) void k_sweep(const DevParams* __restrict__ P,
                     const float dist = len(nrm);
                     float dd = (dist - radius2) * invb;
                     if (dd > 1.0f) continue;  // outside the barrier
-                    dd = fmaxf(dd, 1e-3f);
-                    nrm *= 1.0f / dist;
+                    dd = __builtin_amdgcn_fmed3f(dd, 1e-3f, 1.0f);  // max(dd, 1e-3), dd <= 1
+                    nrm *= frcp(dist);
                     const float w1 = u;
                     const float w0 = 1.0f - u;
 
                     // b(d) = -(d-1)^2 ln d; ff = -kCol b'(d), dH = kCol/b b''(d)
-                    const float invd = 1.0f / dd;
-                    const float logd = __logf(dd);
+                    const float invd = frcp(dd);
+                    const float logd = fln(dd);
                     float dH = (-3.0f + (2.0f + invd) * invd - 2.0f * logd) * kCol * invb;
                     const float ff = -(1.0f - dd) * (dd - 1.0f + 2.0f * dd * logd) * invd * kCol;
                     f += (ff * w0 - damping * dH * w0 * w0 * dot(nrm, dx)) * nrm;
@@ -244,25 +277,31 @@ __global__ __launch_bounds__(512) void k_sweep(const DevParams* __restrict__ P,
                     // Lagged friction on the tangential relative displacement;
                     // its Hessian f1 w^2 (I - n n^T) is merged with the
                     // barrier's dH w^2 n n^T into one rank-1 update + diagonal.
-                    const v3 ddpos = mixv(dx, p1dx, u) - mixv(db0, db1, w);
+                    const v3 ddpos = (dx + dseg * u) - (db0 + (db1 - db0) * w);
                     const v3 uu = ddpos - dot(nrm, ddpos) * nrm;
                     const float ul = len(uu);
                     float f1 = 0.0f;
                     if (ul > 0.0f) {
-                        f1 = fminf(fricK, fricMu * ff / ul);
+                        f1 = fminf(fricK, fricMu * ff * frcp(ul));
                         f -= (f1 * w0) * uu;
                         f2 -= (f1 * w1) * uu;
                     }
                     const sym3 op = sym_outer(nrm);
                     const float s0 = w0 * w0, s1 = w1 * w1;
                     sym_axpy(H, op, (dH - f1) * s0);
-                    sym_add_diag(H, f1 * s0);
                     sym_axpy(H2, op, (dH - f1) * s1);
-                    sym_add_diag(H2, f1 * s1);
+                    diag0 += f1 * s0;
+                    diag1 += f1 * s1;
                 }
+                sym_add_diag(H, diag0);
+                sym_add_diag(H2, diag1);
             }
         }
     }
+
+    // The rest rotor is needed only from here on: loading it after the contact
+    // loop keeps it out of the loop's register budget.
+    if (in) qr4 = qRest[v];
 
     // Segment (i-1)'s endpoint-1 terms, frame and rest rotor (cosserat.cu:175-193).
     const float pf0 = __shfl_up(f2.x, 1, 64);
@@ -290,7 +329,7 @@ __global__ __launch_bounds__(512) void k_sweep(const DevParams* __restrict__ P,
         }
         if (p4.w != 0.0f) {
             // Local 3x3 solve (cosserat.cu:195-205).
-            dxNew += P->accelerationRatio * sym_solve(H, f);
+            dxNew += P->accelerationRatio * sym_solve_fast(H, f);
             const float l = len(dxNew);
             if (l > limit && l > 0.0f) dxNew *= limit / l;
             num = len2(dxNew - xyz(d4));
@@ -309,8 +348,7 @@ __global__ __launch_bounds__(512) void k_sweep(const DevParams* __restrict__ P,
         rot q0 = asrot(q4);
         if (!(flags & YB_FLAG_FIX_ORIENTATION) && (flags & YB_FLAG_HAS_NEXT)) {
             const v3 e = (p1 - xyz(p4)) + (dxNext - dxNew);
-            v3 vv = v3{e.x / r4.x, e.y / r4.x, e.z / r4.x};
-            vv *= -2.0f * r4.y;
+            const v3 vv = e * (-2.0f * r4.y * invl);
             rot b = rot{0.f, 0.f, 0.f, 0.f};
             if (flags & YB_FLAG_HAS_PREV) {
                 const float sg = rdot(rmul(rinv(qPrev), q0), qrPrev) > 0.0f ? 1.0f : -1.0f;
@@ -323,7 +361,7 @@ __global__ __launch_bounds__(512) void k_sweep(const DevParams* __restrict__ P,
             }
             const float lambda = len(vv) + rlen(b);
             const rot t = rmul(rmul(mkrot(vv, 0.0f), b), rot{1.f, 0.f, 0.f, 0.f});
-            q0 = rnormalize(radd(t, rscale(b, lambda)));
+            q0 = rnormalize_fast(radd(t, rscale(b, lambda)));
         }
         qOut[v] = f4r(q0);
     }

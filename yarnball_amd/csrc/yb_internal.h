@@ -63,6 +63,17 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
     return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
 }
 
+// Element i (+ 1 when `next`) of a per-vertex float4 array through a 32-bit
+// byte offset: yb_create caps n at YB_MAX_VERTICES = 2^28, so every offset fits,
+// and a data-dependent gather becomes one SGPR-base + VGPR-offset load with no
+// 64-bit address arithmetic.
+template <bool next = false>
+__device__ __forceinline__ float4 gather4(const float4* __restrict__ a, int i) {
+    const uint32_t off = (uint32_t)i << 4;
+    return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(a) + off +
+                                            (next ? 16 : 0));
+}
+
 // Learned warm start: parameter vector layout (floats), see yarnball_hip.h.
 constexpr int kMlpIn = YB_MLP_INPUTS;
 constexpr int kMlpHid = YB_MLP_HIDDEN;
