@@ -89,28 +89,45 @@ def test_driver_frame_matches_reference_loop(kind):
 
 @pytest.mark.gpu
 def test_device_twist_tracks_host_twist_on_the_cable():
+    """The device driver and the host loop place the driven vertices alike
+    (float32 sin/cos on the device, float64 on the host: a few ulp apart),
+    so the driven vertices agree every frame to 1e-6 of the scene size.  The
+    free vertices then differ by what the dynamics make of ulp-sized input
+    differences over 4 frames (136 substeps); the bound is set by measuring
+    that: the same host-driven run started from free positions moved by one
+    ulp."""
     sc = scenes.twisted_cable(strands=7, per=256)
     frame = np.float32(sc.frame_dt)
-    host = _sim(sc)
-    dev = _sim(sc)
+    host, noisy, dev = _sim(sc), _sim(sc), _sim(sc)
+    free = sc.verts["invMass"] != 0
+    noisy.verts["pos"][free] = np.nextafter(noisy.verts["pos"][free], np.float32(np.inf))
+    noisy.upload()
     dev.driverInit()
     d = Sim.defaultDriver(DRIVE_TWIST)   # the cable's driver: 1 rev/s about the y axis,
     d.speed, d.start, d.duration, d.release, d.side = 2 * np.pi, 0.0, 1e9, -1.0, 1  # top end
     d.center[0], d.center[1], d.center[2] = 0.0, float(np.mean(sc.verts["pos"][:, 1])), 0.0
+    scale = float(np.max(np.abs(sc.verts["pos"])))
+    ids = sc.drive_ids
+    assert len(ids) > 0
     t = 0.0
     for f in range(4):
-        host.download()
-        scenes.drive_frame(sc, host.verts["pos"], host.vels, t, float(frame))
-        host.upload()
-        host.advance(frame)
+        for s in (host, noisy):
+            s.download()
+            scenes.drive_frame(sc, s.verts["pos"], s.vels, t, float(frame))
+            s.upload()
+            s.advance(frame)
         dev.driveFrame(d, t, float(frame))
         dev.advance(frame)
         t += float(frame)
-    host.download()
-    dev.download()
-    scale = np.max(np.abs(host.verts["pos"]))
+        host.download()
+        dev.download()
+        pin = np.max(np.abs(dev.verts["pos"][ids] - host.verts["pos"][ids])) / scale
+        assert pin < 1e-6, (f, pin)
+    noisy.download()
     err = np.max(np.abs(dev.verts["pos"] - host.verts["pos"])) / scale
-    print(f"device vs host twist after 4 frames: rel {err:.2e}")
-    assert err < 1e-5
-    host.close()
-    dev.close()
+    floor = np.max(np.abs(noisy.verts["pos"] - host.verts["pos"])) / scale
+    print(f"device vs host twist after 4 frames: rel {err:.2e}; one-ulp start: {floor:.2e}")
+    assert floor > 0
+    assert err < floor
+    for s in (host, noisy, dev):
+        s.close()

@@ -34,6 +34,21 @@ __device__ __forceinline__ float fln(float x) { return __builtin_amdgcn_logf(x) 
 __device__ __forceinline__ rot rnormalize_fast(rot a) {
     return rscale(a, __builtin_amdgcn_rsqf(rdot(a, a)));
 }
+// rotate(r, e1) (Rotor.h:96-105 with v = (1,0,0)) written out, so that no
+// product with a constant 0 is left for the compiler (it may not drop x * 0).
+__device__ __forceinline__ v3 rotate_e1(rot r) {
+    return v3{(r.w * r.w + r.x * r.x) - (r.y * r.y + r.z * r.z), 2.0f * (r.x * r.y + r.z * r.w),
+              2.0f * (r.x * r.z - r.y * r.w)};
+}
+// (V b) e1 for the pure rotor V = (v, 0) and e1 = Rotor(1) = (1,0,0 | 0): the two
+// rotor products (Rotor.h:185-188) written out.  With m = V b,
+//   m = (b.w v + v x b.xyz | -v . b.xyz),   m e1 = (m.w, m.z, -m.y | -m.x).
+__device__ __forceinline__ rot vb_e1(v3 v, rot b) {
+    const v3 bq = rq(b);
+    const v3 mq = b.w * v + cross(v, bq);
+    return rot{-dot(v, bq), mq.z, -mq.y, -mq.x};
+}
+
 // H^{-1} f through the adjugate (sym_solve with the hardware reciprocal).
 __device__ __forceinline__ v3 sym_solve_fast(const sym3& m, v3 f) {
     const float a = m.a[0], b = m.a[1], c = m.a[2], d = m.a[3], e = m.a[4], g = m.a[5];
@@ -169,133 +184,128 @@ __global__ __launch_bounds__(512) void k_sweep(const DevParams* __restrict__ P,
     const float h = P->h;
     const float damping = P->dampingH;
 
-    v3 f = v3{0.f, 0.f, 0.f};
     v3 f2 = v3{0.f, 0.f, 0.f};
-    sym3 H = sym_zero();
     sym3 H2 = sym_zero();
     float segLimit = INFINITY;
-    float4 p4 = make_float4(0.f, 0.f, 0.f, 0.f);
-    float4 r4 = make_float4(1.f, 0.f, 0.f, 0.f);
-    float4 d4 = make_float4(0.f, 0.f, 0.f, 0.f);
-    float4 q4 = make_float4(0.f, 0.f, 0.f, 1.f);
     float4 qr4 = make_float4(0.f, 0.f, 0.f, 0.f);
     v3 p1 = v3{0.f, 0.f, 0.f};
     float invl = 0.0f;  // 1 / rest length of segment i (set when it exists)
-    uint32_t flags = 0;
 
-    if (in) {
-        p4 = pos[v];
-        r4 = prm[v];
-        d4 = dxIn[v];
-        q4 = qIn[v];
-        flags = __float_as_uint(r4.w);
-        const int cid = __float_as_int(r4.z);
-        const v3 x0 = xyz(p4);
-        const v3 dx = xyz(d4);
+    // Lanes outside [0, n) (the halo before vertex 0, the tail) read a clamped
+    // vertex with its flags cleared: they add no segment terms to their
+    // neighbours and write nothing, and no code branches on the range.
+    const int vc = min(max(v, 0), n - 1);
+    const float4 p4 = pos[vc];
+    const float4 r4 = prm[vc];
+    const float4 d4 = dxIn[vc];
+    const float4 q4 = qIn[vc];
+    const uint32_t flags = in ? __float_as_uint(r4.w) : 0u;
+    const int cid = in ? __float_as_int(r4.z) : -1;
+    const v3 x0 = xyz(p4);
+    const v3 dx = xyz(d4);
 
-        // Inertia (cosserat.cu:40-44); yv holds y - x.
-        const float mh2 = frcp(p4.w * h * h);
-        H = sym_diag(mh2);
-        f = mh2 * (xyz(yv[v]) - dx);
+    // Inertia (cosserat.cu:40-44); yv holds y - x.
+    const float mh2 = frcp(p4.w * h * h);
+    sym3 H = sym_diag(mh2);
+    v3 f = mh2 * (xyz(yv[vc]) - dx);
 
-        // Glue spring to a connected vertex (cosserat.cu:47-53).
-        if (cid >= 0) {
-            const float stiffness = 40.0f;
-            const v3 pc = xyz(pos[cid]);
-            const v3 dc = xyz(dxIn[cid]);
-            f -= stiffness * (((x0 - pc) + (dx - dc)) + damping * dx);
-            sym_add_diag(H, stiffness * (1.0f + damping));
-        }
+    // Glue spring to a connected vertex (cosserat.cu:47-53).
+    if (cid >= 0) {
+        const float stiffness = 40.0f;
+        const v3 pc = xyz(pos[cid]);
+        const v3 dc = xyz(dxIn[cid]);
+        f -= stiffness * (((x0 - pc) + (dx - dc)) + damping * dx);
+        sym_add_diag(H, stiffness * (1.0f + damping));
+    }
 
-        if (flags & YB_FLAG_HAS_NEXT) {
-            p1 = xyz(pos[v + 1]);
-            const v3 p1dx = xyz(dxIn[v + 1]);
-            segLimit = d4.w;
+    if (flags & YB_FLAG_HAS_NEXT) {
+        p1 = xyz(pos[vc + 1]);
+        const v3 p1dx = xyz(dxIn[vc + 1]);
+        segLimit = d4.w;
 
-            // Cosserat stretch-shear (cosserat.cu:67-81).
-            invl = frcp(r4.x);
-            const v3 e = (p1 - x0) + (p1dx - dx);
-            const v3 c = e * invl - rrotate_e1(asrot(q4));
-            const float k = r4.y * invl;
-            float d = k * invl;
-            f += k * c - (damping * d) * dx;
-            f2 = -k * c - (damping * d) * p1dx;
-            d *= 1.0f + damping;
-            sym_add_diag(H, d);
-            sym_add_diag(H2, d);
+        // Cosserat stretch-shear (cosserat.cu:67-81).
+        invl = frcp(r4.x);
+        const v3 e = (p1 - x0) + (p1dx - dx);
+        const v3 c = e * invl - rotate_e1(asrot(q4));
+        const float k = r4.y * invl;
+        float d = k * invl;
+        f += k * c - (damping * d) * dx;
+        f2 = -k * c - (damping * d) * p1dx;
+        d *= 1.0f + damping;
+        sym_add_diag(H, d);
+        sym_add_diag(H2, d);
 
-            // IPC barrier + lagged friction over this segment's contacts
-            // (cosserat.cu:83-148).
-            const int nc = numCols[v];
-            if (nc > 0) {
-                const float fricK = P->kFriction;
-                const float invb = 1.0f / P->barrierThickness;
-                const float radius2 = 2.0f * P->radius;
-                const float fricMu = P->frictionCoeff;
-                const float kCol = P->kCollision * invb;
-                const float ea = len2(e);
-                const float inva = frcp(ea);
-                // friction's isotropic Hessian part f1 w^2 I, summed as scalars
-                float diag0 = 0.0f, diag1 = 0.0f;
-                const v3 dseg = p1dx - dx;  // this segment's relative displacement
-                int o = cols[v];
-                for (int i = 0; i < nc; i++) {
-                    // next partner id fetched one iteration ahead; slot-major
-                    // lists span n * 128 entries, past the int range
-                    const int oNext = (i + 1 < nc) ? cols[v + (size_t)n * (i + 1)] : o;
-                    const v3 b0 = xyz(gather4(pos, o));
-                    const v3 b1 = xyz(gather4<true>(pos, o));
-                    const v3 db0 = xyz(gather4(dxIn, o));
-                    const v3 db1 = xyz(gather4<true>(dxIn, o));
-                    o = oNext;
-                    // partner segment relative to this vertex, in the
-                    // current configuration x + dx
-                    const v3 B0 = (b0 - x0) + (db0 - dx);
-                    const v3 A1 = ((b1 - x0) + (db1 - dx)) - B0;
-                    float u, w;
-                    closest_e(e, ea, inva, B0, A1, u, w);
-                    // closest-point difference (pos + dx of this segment minus
-                    // the partner's) and its length
-                    v3 nrm = e * u - (B0 + A1 * w);
-                    const float dist = len(nrm);
-                    float dd = (dist - radius2) * invb;
-                    if (dd > 1.0f) continue;  // outside the barrier
-                    dd = __builtin_amdgcn_fmed3f(dd, 1e-3f, 1.0f);  // max(dd, 1e-3), dd <= 1
-                    nrm *= frcp(dist);
-                    const float w1 = u;
-                    const float w0 = 1.0f - u;
+        // IPC barrier + lagged friction over this segment's contacts
+        // (cosserat.cu:83-148).
+        const int nc = numCols[vc];
+        if (nc > 0) {
+            const float fricK = P->kFriction;
+            const float invb = 1.0f / P->barrierThickness;
+            const float radius2 = 2.0f * P->radius;
+            const float fricMu = P->frictionCoeff;
+            const float kCol = P->kCollision * invb;
+            const float ea = len2(e);
+            const float inva = frcp(ea);
+            // friction's isotropic Hessian part f1 w^2 I, summed as scalars
+            float diag0 = 0.0f, diag1 = 0.0f;
+            const v3 dseg = p1dx - dx;  // this segment's relative displacement
+            int o = cols[vc];
+            for (int i = 0; i < nc; i++) {
+                // next partner id fetched one iteration ahead; slot-major
+                // lists span n * 128 entries, past the int range
+                const int oNext = (i + 1 < nc) ? cols[vc + (size_t)n * (i + 1)] : o;
+                const v3 b0 = xyz(gather4(pos, o));
+                const v3 b1 = xyz(gather4<true>(pos, o));
+                const v3 db0 = xyz(gather4(dxIn, o));
+                const v3 db1 = xyz(gather4<true>(dxIn, o));
+                o = oNext;
+                // partner segment relative to this vertex, in the
+                // current configuration x + dx
+                const v3 B0 = (b0 - x0) + (db0 - dx);
+                const v3 A1 = ((b1 - x0) + (db1 - dx)) - B0;
+                float u, w;
+                closest_e(e, ea, inva, B0, A1, u, w);
+                // closest-point difference (pos + dx of this segment minus
+                // the partner's) and its length
+                v3 nrm = e * u - (B0 + A1 * w);
+                const float dist = len(nrm);
+                float dd = (dist - radius2) * invb;
+                if (dd > 1.0f) continue;  // outside the barrier
+                dd = __builtin_amdgcn_fmed3f(dd, 1e-3f, 1.0f);  // max(dd, 1e-3), dd <= 1
+                nrm *= frcp(dist);
+                const float w1 = u;
+                const float w0 = 1.0f - u;
 
-                    // b(d) = -(d-1)^2 ln d; ff = -kCol b'(d), dH = kCol/b b''(d)
-                    const float invd = frcp(dd);
-                    const float logd = fln(dd);
-                    float dH = (-3.0f + (2.0f + invd) * invd - 2.0f * logd) * kCol * invb;
-                    const float ff = -(1.0f - dd) * (dd - 1.0f + 2.0f * dd * logd) * invd * kCol;
-                    f += (ff * w0 - damping * dH * w0 * w0 * dot(nrm, dx)) * nrm;
-                    f2 += (ff * w1 - damping * dH * w1 * w1 * dot(nrm, p1dx)) * nrm;
-                    dH *= 1.0f + damping;
+                // b(d) = -(d-1)^2 ln d; ff = -kCol b'(d), dH = kCol/b b''(d)
+                const float invd = frcp(dd);
+                const float logd = fln(dd);
+                float dH = (-3.0f + (2.0f + invd) * invd - 2.0f * logd) * kCol * invb;
+                const float ff = -(1.0f - dd) * (dd - 1.0f + 2.0f * dd * logd) * invd * kCol;
+                f += (ff * w0 - damping * dH * w0 * w0 * dot(nrm, dx)) * nrm;
+                f2 += (ff * w1 - damping * dH * w1 * w1 * dot(nrm, p1dx)) * nrm;
+                dH *= 1.0f + damping;
 
-                    // Lagged friction on the tangential relative displacement;
-                    // its Hessian f1 w^2 (I - n n^T) is merged with the
-                    // barrier's dH w^2 n n^T into one rank-1 update + diagonal.
-                    const v3 ddpos = (dx + dseg * u) - (db0 + (db1 - db0) * w);
-                    const v3 uu = ddpos - dot(nrm, ddpos) * nrm;
-                    const float ul = len(uu);
-                    float f1 = 0.0f;
-                    if (ul > 0.0f) {
-                        f1 = fminf(fricK, fricMu * ff * frcp(ul));
-                        f -= (f1 * w0) * uu;
-                        f2 -= (f1 * w1) * uu;
-                    }
-                    const sym3 op = sym_outer(nrm);
-                    const float s0 = w0 * w0, s1 = w1 * w1;
-                    sym_axpy(H, op, (dH - f1) * s0);
-                    sym_axpy(H2, op, (dH - f1) * s1);
-                    diag0 += f1 * s0;
-                    diag1 += f1 * s1;
+                // Lagged friction on the tangential relative displacement;
+                // its Hessian f1 w^2 (I - n n^T) is merged with the
+                // barrier's dH w^2 n n^T into one rank-1 update + diagonal.
+                const v3 ddpos = (dx + dseg * u) - (db0 + (db1 - db0) * w);
+                const v3 uu = ddpos - dot(nrm, ddpos) * nrm;
+                const float ul = len(uu);
+                float f1 = 0.0f;
+                if (ul > 0.0f) {
+                    f1 = fminf(fricK, fricMu * ff * frcp(ul));
+                    f -= (f1 * w0) * uu;
+                    f2 -= (f1 * w1) * uu;
                 }
-                sym_add_diag(H, diag0);
-                sym_add_diag(H2, diag1);
+                const sym3 op = sym_outer(nrm);
+                const float s0 = w0 * w0, s1 = w1 * w1;
+                sym_axpy(H, op, (dH - f1) * s0);
+                sym_axpy(H2, op, (dH - f1) * s1);
+                diag0 += f1 * s0;
+                diag1 += f1 * s1;
             }
+            sym_add_diag(H, diag0);
+            sym_add_diag(H2, diag1);
         }
     }
 
@@ -352,7 +362,7 @@ __global__ __launch_bounds__(512) void k_sweep(const DevParams* __restrict__ P,
             rot b = rot{0.f, 0.f, 0.f, 0.f};
             if (flags & YB_FLAG_HAS_PREV) {
                 const float sg = rdot(rmul(rinv(qPrev), q0), qrPrev) > 0.0f ? 1.0f : -1.0f;
-                b = radd(b, rscale(rmul(qPrev, qrPrev), sg));
+                b = rscale(rmul(qPrev, qrPrev), sg);
             }
             if (flags & YB_FLAG_HAS_NEXT_ORIENTATION) {
                 const rot qr = asrot(qr4);
@@ -360,8 +370,7 @@ __global__ __launch_bounds__(512) void k_sweep(const DevParams* __restrict__ P,
                 b = radd(b, rscale(rmul(qNext, rinv(qr)), sg));
             }
             const float lambda = len(vv) + rlen(b);
-            const rot t = rmul(rmul(mkrot(vv, 0.0f), b), rot{1.f, 0.f, 0.f, 0.f});
-            q0 = rnormalize_fast(radd(t, rscale(b, lambda)));
+            q0 = rnormalize_fast(radd(vb_e1(vv, b), rscale(b, lambda)));
         }
         qOut[v] = f4r(q0);
     }
@@ -439,7 +448,7 @@ __global__ __launch_bounds__(256) void k_frames(const DevParams* __restrict__ P,
             const rot qr = asrot(qRest[v - 1]);
             const rot qq = asrot(qIn[v - 1]);
             const float sg = rdot(rmul(rinv(qq), q0), qr) > 0.0f ? 1.0f : -1.0f;
-            b = radd(b, rscale(rmul(qq, qr), sg));
+            b = rscale(rmul(qq, qr), sg);
         }
         if (flags & YB_FLAG_HAS_NEXT_ORIENTATION) {
             const rot qr = asrot(qRest[v]);
@@ -448,8 +457,7 @@ __global__ __launch_bounds__(256) void k_frames(const DevParams* __restrict__ P,
             b = radd(b, rscale(rmul(qq, rinv(qr)), sg));
         }
         const float lambda = len(vv) + rlen(b);
-        const rot t = rmul(rmul(mkrot(vv, 0.0f), b), rot{1.f, 0.f, 0.f, 0.f});
-        q0 = rnormalize(radd(t, rscale(b, lambda)));
+        q0 = rnormalize(radd(vb_e1(vv, b), rscale(b, lambda)));
     }
     qOut[v] = f4r(q0);
 }
