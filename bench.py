@@ -52,7 +52,7 @@ def stage_bytes(stage, c):
 # HBM bytes per vertex of one k_sweep launch on the knit patch (256 x 4096,
 # residual 1e-6, c = 4.42 contacts per segment), from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE scaled by a copy kernel's counts (profiles/sweep_knit1M_hbm_traffic.md).
-MEASURED_TRAFFIC = {("knit_patch", "sweep"): 160.9}
+MEASURED_TRAFFIC = {("knit_patch", "sweep"): 160.6}
 
 
 def build_workload(name, numItr):

@@ -393,18 +393,32 @@ __global__ __launch_bounds__(1024) void k_residual(SolverCtl* __restrict__ ctl,
                                                    const double* __restrict__ partial, int nb,
                                                    int sweep, float tol, int maxSweeps) {
     if (ctl->doneAt < sweep) return;
-    __shared__ double sa[1024];
-    double a = 0.0;
-    for (int k = threadIdx.x; k < nb; k += blockDim.x) a += partial[k];
-    sa[threadIdx.x] = a;
-    __syncthreads();
-    for (int st = 512; st > 0; st >>= 1) {
-        if ((int)threadIdx.x < st) sa[threadIdx.x] += sa[threadIdx.x + st];
-        __syncthreads();
+    // Four independent running sums per thread keep four loads in flight; the
+    // block then reduces by wave shuffles and one wave over the 16 wave sums.
+    // The order of additions is fixed, so the residual is reproducible.
+    const int t = threadIdx.x;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int k = t;
+    for (; k + 3 * 1024 < nb; k += 4 * 1024) {
+        a0 += partial[k];
+        a1 += partial[k + 1024];
+        a2 += partial[k + 2048];
+        a3 += partial[k + 3072];
     }
-    if (threadIdx.x == 0) {
+    for (; k < nb; k += 1024) a0 += partial[k];
+    double a = (a0 + a1) + (a2 + a3);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+    __shared__ double sw[16];
+    if ((t & 63) == 0) sw[t >> 6] = a;
+    __syncthreads();
+    if (t >= 64) return;
+    a = t < 16 ? sw[t] : 0.0;
+#pragma unroll
+    for (int off = 8; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+    if (t == 0) {
         const double den = ctl->denom;
-        const float r = den > 0.0 ? (float)sqrt(sa[0] / den) : 0.0f;
+        const float r = den > 0.0 ? (float)sqrt(a / den) : 0.0f;
         ctl->resid = r;
         if (r <= tol || sweep == maxSweeps - 1) {
             ctl->doneAt = sweep;
