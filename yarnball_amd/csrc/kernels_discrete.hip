@@ -442,7 +442,9 @@ __global__ __launch_bounds__(256) void k_karras(int n, const uint32_t* __restric
 // union of a suffix, at most two overlapping sparse-table windows and a
 // prefix -- computed directly, so no node waits on another.  Every step is a
 // min/max, hence exact and independent of scheduling.
-constexpr int kRefitThreads = 256;
+// One thread per leaf: each climb is a chain of dependent loads, and with a
+// leaf per thread all of a block's climbs are in flight at once.
+constexpr int kRefitThreads = 1024;
 constexpr int kRefitPer = kRefitLeaves / kRefitThreads;
 
 __device__ __forceinline__ void box_union(float4& lo, float4& hi, float4 lo2, float4 hi2) {
@@ -490,7 +492,11 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_local(
         bool leaf = true;
         while (slot >= 0) {
             const int p = slot >> 1;
+            // rng, kids and parent are read-only here: fetch all three at once,
+            // so a level of the climb waits for one memory round trip
             const int2 pr = rng[p];
+            const int2 kc = kids[p];
+            const int up = parent[p];
             if (pr.x < base || pr.y >= end) {
                 // p straddles: publish this child's box in p's slot for traversal.
                 const int side = slot & 1;
@@ -502,7 +508,6 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_local(
             const int old = atomicAdd(&cnt[p - base], 1);
             if (old == 0) break;
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-            const int2 kc = kids[p];
             float4 lo1, hi1;
             if (kc.x < 0) { lo = lbox[~kc.x - base][0]; hi = lbox[~kc.x - base][1]; }
             else { lo = nbox[kc.x - base][0]; hi = nbox[kc.x - base][1]; }
@@ -512,7 +517,7 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_local(
             nbox[p - base][0] = lo;
             nbox[p - base][1] = hi;
             leaf = false;
-            slot = parent[p];
+            slot = up;
         }
     }
     __syncthreads();

@@ -102,8 +102,8 @@ __global__ __launch_bounds__(256) void k_step_limit(const DevParams* __restrict_
             const int nc = numCols[v];
             for (int i = 0; i < nc; i++) {
                 const int o = cols[v + (size_t)n * i];
-                const v3 b0 = xyz(pos[o]) - a0;
-                const v3 b1 = xyz(pos[o + 1]) - a0;
+                const v3 b0 = xyz(gather4(pos, o)) - a0;
+                const v3 b1 = xyz(gather4<true>(pos, o)) - a0;
                 float u, w;
                 closest_uv(v3{0.f, 0.f, 0.f}, a1, b0, b1, u, w);
                 minContact = fminf(minContact, len(u * a1 - mixv(b0, b1, w)));
