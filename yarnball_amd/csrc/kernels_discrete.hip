@@ -812,7 +812,14 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
         } else {
             if (sp == 0) break;
             sp--;
-            node = sp < kStackL ? stk[sp][t] : deep[sp - kStackL];
+            if (sp < kStackL) {
+                node = stk[sp][t];
+            } else {
+                // keeps the compiler from merging the two pops into one generic
+                // (flat) load, which would make every pop a slow volatile one
+                asm volatile("" ::: "memory");
+                node = deep[sp - kStackL];
+            }
         }
     }
     narrowphase();
