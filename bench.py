@@ -67,14 +67,16 @@ def build_workload(name, numItr):
     raise SystemExit(f"unknown workload {name}")
 
 
-def cpu_baseline(sc, substeps, threads, residual, max_sweeps):
+def cpu_baseline(sc, substeps, threads, residual, max_sweeps, fp32=False):
+    """The CPU oracle (FP64; fp32=True: its float build) on `substeps` substeps
+    of the scene: (vertex-substeps/s, threads, seconds, sweeps)."""
     from oracle import Oracle, load
 
-    lib = load()
+    lib = load(fp32)
     if threads > 0:
         lib.orc_set_threads(threads)
     used = lib.orc_get_threads()
-    orc = Oracle(sc)
+    orc = Oracle(sc, fp32=fp32)
     if residual > 0:
         orc.set_solver(residual, max_sweeps)
     p = sc.params.copy()
@@ -189,25 +191,33 @@ def bench_batched(args):
     cpu = None
     if not args.no_cpu:
         from oracle import Oracle, load
-        lib = load()
-        if args.cpu_threads > 0:
-            lib.orc_set_threads(args.cpu_threads)
         sub = scs[:32]
-        orcs = [(Oracle(sc), sc.params.copy(), sc) for sc in sub]
         nv = sum(sc.n for sc in sub)
-        rounds = 0
-        t0 = time.perf_counter()
-        while rounds == 0 or time.perf_counter() - t0 < 10.0:
-            for o, pp, sc in orcs:
-                o.advance(pp, sc.maxH, sc.maxH)
-            rounds += 1
-        dt = time.perf_counter() - t0
-        for o, _, _ in orcs:
-            o.close()
-        cpu = {"value": round(nv * rounds / dt, 1), "unit": "vertex-substeps/s",
-               "cores": int(lib.orc_get_threads()), "kind": "port",
+
+        def per_scene_oracle(fp32, seconds):
+            lib = load(fp32)
+            if args.cpu_threads > 0:
+                lib.orc_set_threads(args.cpu_threads)
+            orcs = [(Oracle(sc, fp32=fp32), sc.params.copy(), sc) for sc in sub]
+            rounds = 0
+            t0 = time.perf_counter()
+            while rounds == 0 or time.perf_counter() - t0 < seconds:
+                for o, pp, sc in orcs:
+                    o.advance(pp, sc.maxH, sc.maxH)
+                rounds += 1
+            dt = time.perf_counter() - t0
+            for o, _, _ in orcs:
+                o.close()
+            return nv * rounds / dt, rounds, dt, int(lib.orc_get_threads())
+
+        v, rounds, dt, used = per_scene_oracle(False, 10.0)
+        cpu = {"value": round(v, 1), "unit": "vertex-substeps/s", "cores": used, "kind": "port",
                "sample": f"FP64 OpenMP oracle, the first 32 scenes ({nv} vertices) one substep "
                          f"({args.numItr} sweeps) each in turn, {rounds} rounds in {dt:.1f}s"}
+        v32, rounds32, dt32, _ = per_scene_oracle(True, 5.0)
+        cpu["fp32"] = {"value": round(v32, 1),
+                       "sample": f"the same stepper built in float (liboracle32.so), same scenes: "
+                                 f"{rounds32} rounds in {dt32:.1f}s"}
 
     alg = (124 + 4 * c) * big.n
     out = {
@@ -411,6 +421,12 @@ def main():
                          f"sweeps capped at the plain GPU run's mean per substep), {dt:.1f}s"}
         print(f"[bench] cpu baseline {v:.3e} vertex-substeps/s on {used} threads",
               file=sys.stderr)
+        v32, _, dt32, sw32 = cpu_baseline(sc, args.cpu_substeps, args.cpu_threads, args.residual,
+                                          cap, fp32=True)
+        cpu["fp32"] = {"value": round(v32, 1),
+                       "sample": f"the same stepper built in float (liboracle32.so), same sample: "
+                                 f"{sw32} sweeps, {dt32:.1f}s"}
+        print(f"[bench] cpu baseline fp32 {v32:.3e} vertex-substeps/s", file=sys.stderr)
 
     method_rows = {}
     for m, r in runs.items():

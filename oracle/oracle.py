@@ -47,21 +47,28 @@ SIGS = {
     "orc_from_to": (None, [_P, _P, _P]),
 }
 
-_lib = None
+# liboracle32.so: the same stepper built with float arithmetic (-DORC_FP32), a
+# CPU timing baseline only; it has no test helpers.
+LIB32_PATH = os.path.join(_HERE, "liboracle32.so")
+_TEST_ONLY = {"orc_closest", "orc_barrier", "orc_solve3", "orc_configure", "orc_rotor_mul",
+              "orc_rotor_rotate", "orc_from_to"}
+_libs = {}
 
 
-def load():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise RuntimeError(f"{LIB_PATH} missing: run `make -C oracle`")
-        lib = ctypes.CDLL(LIB_PATH)
+def load(fp32=False):
+    path = LIB32_PATH if fp32 else LIB_PATH
+    if path not in _libs:
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} missing: run `make -C oracle`")
+        lib = ctypes.CDLL(path)
         for k, (r, a) in SIGS.items():
+            if fp32 and k in _TEST_ONLY:
+                continue
             f = getattr(lib, k)
             f.restype = r
             f.argtypes = a
-        _lib = lib
-    return _lib
+        _libs[path] = lib
+    return _libs[path]
 
 
 def _f64(a, shape):
@@ -69,10 +76,10 @@ def _f64(a, shape):
 
 
 class Oracle:
-    """FP64 oracle holding one scene's state."""
+    """FP64 oracle holding one scene's state (fp32=True: the float build)."""
 
-    def __init__(self, scene=None, n=None):
-        self.lib = load()
+    def __init__(self, scene=None, n=None, fp32=False):
+        self.lib = load(fp32)
         self.n = n if n is not None else scene.n
         self.o = self.lib.orc_new(self.n)
         if scene is not None:

@@ -14,6 +14,15 @@
 #include <cstdint>
 #include <vector>
 
+// Arithmetic type of the stepper: double for the oracle.  A second build with
+// -DORC_FP32 (liboracle32.so) is an FP32 CPU timing baseline only; the test
+// helpers at the end of this file exist in the double build alone.
+#ifdef ORC_FP32
+using real = float;
+#else
+using real = double;
+#endif
+
 namespace {
 
 // ---------------------------------------------------------------------------
@@ -116,10 +125,10 @@ template <class T> inline R4<T> from_to(V3<T> from, V3<T> to) {
 
 // Packed symmetric 3x3 {xx, yy, zz, xy, xz, yz} (SymMat.h:8-107).
 struct S3 {
-    double a[6];
+    real a[6];
 };
-inline void s3_diag(S3& m, double d) { m.a[0] += d; m.a[1] += d; m.a[2] += d; }
-inline void s3_outer_axpy(S3& m, V3<double> v, double s) {
+inline void s3_diag(S3& m, real d) { m.a[0] += d; m.a[1] += d; m.a[2] += d; }
+inline void s3_outer_axpy(S3& m, V3<real> v, real s) {
     m.a[0] += v.x * v.x * s;
     m.a[1] += v.y * v.y * s;
     m.a[2] += v.z * v.z * s;
@@ -129,8 +138,8 @@ inline void s3_outer_axpy(S3& m, V3<double> v, double s) {
 }
 // 3x3 solve by Gaussian elimination with partial pivoting (FP64 textbook route;
 // the device uses the adjugate).
-inline V3<double> s3_solve(const S3& m, V3<double> f) {
-    double A[3][4] = {{m.a[0], m.a[3], m.a[4], f.x},
+inline V3<real> s3_solve(const S3& m, V3<real> f) {
+    real A[3][4] = {{m.a[0], m.a[3], m.a[4], f.x},
                       {m.a[3], m.a[1], m.a[5], f.y},
                       {m.a[4], m.a[5], m.a[2], f.z}};
     for (int c = 0; c < 3; c++) {
@@ -140,22 +149,22 @@ inline V3<double> s3_solve(const S3& m, V3<double> f) {
         if (p != c)
             for (int k = 0; k < 4; k++) std::swap(A[c][k], A[p][k]);
         for (int r = c + 1; r < 3; r++) {
-            double fct = A[r][c] / A[c][c];
+            real fct = A[r][c] / A[c][c];
             for (int k = c; k < 4; k++) A[r][k] -= fct * A[c][k];
         }
     }
-    double x[3];
+    real x[3];
     for (int r = 2; r >= 0; r--) {
-        double s = A[r][3];
+        real s = A[r][3];
         for (int k = r + 1; k < 3; k++) s -= A[r][k] * x[k];
         x[r] = s / A[r][r];
     }
     return {x[0], x[1], x[2]};
 }
 
-using D3 = V3<double>;
+using D3 = V3<real>;
 using F3 = V3<float>;
-using DR = R4<double>;
+using DR = R4<real>;
 
 inline bool has(uint32_t f, int b) { return (f & (uint32_t)b) != 0; }
 
@@ -380,11 +389,11 @@ extern "C" int orc32_contacts(int n, const yb_params* p, const float* pos3,
 // FP64 stepper
 struct orc_sim {
     int n = 0;
-    std::vector<double> pos, vel, lastVel, q, qRest, invMass, lRest, kStretch;
+    std::vector<real> pos, vel, lastVel, q, qRest, invMass, lRest, kStretch;
     std::vector<int32_t> cid;
     std::vector<uint32_t> flags;
     std::vector<int32_t> numCols, cols;  // slot-major, n * 128
-    std::vector<double> y, dx, dxNext, qNext, stepLimit, dxWarm;
+    std::vector<real> y, dx, dxNext, qNext, stepLimit, dxWarm;
     int64_t stepCounter = 0;
     double lastResidual = 0.0;
     double tol = 0.0;     // > 0: sweep until the relative residual <= tol
@@ -455,35 +464,39 @@ extern "C" void orc_load(orc_sim* o, const yb_vertex* verts, const float* vels,
 extern "C" void orc_load64(orc_sim* o, const double* pos, const double* vels,
                            const double* lastVels, const double* qs) {
     const size_t N = (size_t)o->n;
-    if (pos) memcpy(o->pos.data(), pos, sizeof(double) * 3 * N);
-    if (vels) memcpy(o->vel.data(), vels, sizeof(double) * 3 * N);
-    if (lastVels) memcpy(o->lastVel.data(), lastVels, sizeof(double) * 3 * N);
-    if (qs) memcpy(o->q.data(), qs, sizeof(double) * 4 * N);
+    for (size_t k = 0; k < 3 * N; k++) {
+        if (pos) o->pos[k] = (real)pos[k];
+        if (vels) o->vel[k] = (real)vels[k];
+        if (lastVels) o->lastVel[k] = (real)lastVels[k];
+    }
+    for (size_t k = 0; qs && k < 4 * N; k++) o->q[k] = (real)qs[k];
 }
 
 extern "C" void orc_store64(const orc_sim* o, double* pos, double* vels, double* lastVels,
                             double* qs) {
     const size_t N = (size_t)o->n;
-    if (pos) memcpy(pos, o->pos.data(), sizeof(double) * 3 * N);
-    if (vels) memcpy(vels, o->vel.data(), sizeof(double) * 3 * N);
-    if (lastVels) memcpy(lastVels, o->lastVel.data(), sizeof(double) * 3 * N);
-    if (qs) memcpy(qs, o->q.data(), sizeof(double) * 4 * N);
+    for (size_t k = 0; k < 3 * N; k++) {
+        if (pos) pos[k] = o->pos[k];
+        if (vels) vels[k] = o->vel[k];
+        if (lastVels) lastVels[k] = o->lastVel[k];
+    }
+    for (size_t k = 0; qs && k < 4 * N; k++) qs[k] = o->q[k];
 }
 
 extern "C" void orc_set_step_counter(orc_sim* o, int64_t c) { o->stepCounter = c; }
 
 namespace {
 
-inline D3 g3(const std::vector<double>& a, int i) { return {a[3 * i], a[3 * i + 1], a[3 * i + 2]}; }
-inline void s3(std::vector<double>& a, int i, D3 v) {
+inline D3 g3(const std::vector<real>& a, int i) { return {a[3 * i], a[3 * i + 1], a[3 * i + 2]}; }
+inline void s3(std::vector<real>& a, int i, D3 v) {
     a[3 * i] = v.x;
     a[3 * i + 1] = v.y;
     a[3 * i + 2] = v.z;
 }
-inline DR g4(const std::vector<double>& a, int i) {
+inline DR g4(const std::vector<real>& a, int i) {
     return {a[4 * i], a[4 * i + 1], a[4 * i + 2], a[4 * i + 3]};
 }
-inline void s4(std::vector<double>& a, int i, DR v) {
+inline void s4(std::vector<real>& a, int i, DR v) {
     a[4 * i] = v.x;
     a[4 * i + 1] = v.y;
     a[4 * i + 2] = v.z;
@@ -508,60 +521,60 @@ struct SegTerms {
     S3 H0, H1;
 };
 
-SegTerms seg_terms(const orc_sim* o, const yb_params* p, double damping, int s,
-                   const std::vector<double>& dx) {
+SegTerms seg_terms(const orc_sim* o, const yb_params* p, real damping, int s,
+                   const std::vector<real>& dx) {
     SegTerms t{};
     const int n = o->n;
     const D3 x0 = g3(o->pos, s);
     const D3 p1 = g3(o->pos, s + 1);
     const D3 d0 = g3(dx, s);
     const D3 d1 = g3(dx, s + 1);
-    const double invl = 1.0 / o->lRest[s];
+    const real invl = 1.0 / o->lRest[s];
     const D3 e = add(sub(p1, x0), sub(d1, d0));
     const D3 c = sub(mulr(e, invl), rrot(g4(o->q, s), D3{1, 0, 0}));
-    const double k = o->kStretch[s] * invl;
-    double d = k * invl;
+    const real k = o->kStretch[s] * invl;
+    real d = k * invl;
     t.f0 = sub(mul(k, c), mul(damping * d, d0));
     t.f1 = sub(mul(-k, c), mul(damping * d, d1));
     d *= 1.0 + damping;
     s3_diag(t.H0, d);
     s3_diag(t.H1, d);
 
-    const double fricK = p->kFriction;
-    const double invb = 1.0 / (double)p->barrierThickness;
-    const double radius2 = 2.0 * (double)p->radius;
-    const double mu = p->frictionCoeff;
-    const double kCol = (double)p->kCollision * invb;
+    const real fricK = p->kFriction;
+    const real invb = 1.0 / (real)p->barrierThickness;
+    const real radius2 = 2.0 * (real)p->radius;
+    const real mu = p->frictionCoeff;
+    const real kCol = (real)p->kCollision * invb;
     const int nc = o->numCols[s];
     for (int i = 0; i < nc; i++) {
         const int oid = o->cols[s + (size_t)n * i];
         const D3 b0 = g3(o->pos, oid), b1 = g3(o->pos, oid + 1);
         const D3 db0 = g3(dx, oid), db1 = g3(dx, oid + 1);
-        double u, w;
-        closest<double>(D3{0, 0, 0}, e, add(sub(b0, x0), sub(db0, d0)),
+        real u, w;
+        closest<real>(D3{0, 0, 0}, e, add(sub(b0, x0), sub(db0, d0)),
                         add(sub(b1, x0), sub(db1, d0)), u, w);
         const D3 dpos = sub(mix(x0, p1, u), mix(b0, b1, w));
         const D3 ddpos = sub(mix(d0, d1, u), mix(db0, db1, w));
         D3 nrm = add(dpos, ddpos);
-        const double dist = len(nrm);
-        nrm = mulr(nrm, 1.0 / dist);
-        const double w1 = u, w0 = 1.0 - u;
-        double dd = (dist - radius2) * invb;
+        const real dist = len(nrm);
+        nrm = mulr(nrm, real(1.0) / dist);
+        const real w1 = u, w0 = 1.0 - u;
+        real dd = (dist - radius2) * invb;
         if (dd > 1.0) continue;
-        dd = std::max(dd, 1e-3);
-        const double invd = 1.0 / dd;
-        const double logd = std::log(dd);
-        double dH = (-3.0 + (2.0 + invd) * invd - 2.0 * logd) * kCol * invb;
-        const double ff = -(1.0 - dd) * (dd - 1.0 + 2.0 * dd * logd) * invd * kCol;
+        dd = std::max(dd, real(1e-3));
+        const real invd = 1.0 / dd;
+        const real logd = std::log(dd);
+        real dH = (-3.0 + (2.0 + invd) * invd - 2.0 * logd) * kCol * invb;
+        const real ff = -(1.0 - dd) * (dd - 1.0 + 2.0 * dd * logd) * invd * kCol;
         t.f0 = add(t.f0, mul(ff * w0 - damping * dH * w0 * w0 * dot(nrm, d0), nrm));
         t.f1 = add(t.f1, mul(ff * w1 - damping * dH * w1 * w1 * dot(nrm, d1), nrm));
         dH *= 1.0 + damping;
         s3_outer_axpy(t.H0, nrm, dH * w0 * w0);
         s3_outer_axpy(t.H1, nrm, dH * w1 * w1);
         const D3 uu = sub(ddpos, mul(dot(nrm, ddpos), nrm));
-        const double ul = len(uu);
+        const real ul = len(uu);
         if (ul > 0) {
-            const double f1 = std::min(fricK, mu * ff / ul);
+            const real f1 = std::min(fricK, mu * ff / ul);
             t.f0 = sub(t.f0, mul(f1 * w0, uu));
             t.f1 = sub(t.f1, mul(f1 * w1, uu));
             // H -= (nn^T - I) w^2 f1
@@ -579,8 +592,8 @@ SegTerms seg_terms(const orc_sim* o, const yb_params* p, double damping, int s,
 // One substep (step.cpp:111-116 + the captured graph of step.cpp:79-86).
 extern "C" int orc_substep(orc_sim* o, const yb_params* p, int doDetect) {
     const int n = o->n;
-    const double h = p->h;
-    const double damping = (double)p->damping / h;
+    const real h = p->h;
+    const real damping = (real)p->damping / h;
     const D3 g = {p->gravity[0], p->gravity[1], p->gravity[2]};
     int err = 0;
 
@@ -606,11 +619,11 @@ extern "C" int orc_substep(orc_sim* o, const yb_params* p, int doDetect) {
         s3(o->lastVel, v, u);
         if (o->invMass[v] != 0.0) {
             s3(o->y, v, add(dx, mul(h * h, g)));
-            const double g2 = len2(g);
+            const real g2 = len2(g);
             if (g2 > 0) {
-                const double lh = p->lastH;
+                const real lh = p->lastH;
                 const D3 a = {(u.x - lv.x) / lh, (u.y - lv.y) / lh, (u.z - lv.z) / lh};
-                const double s = clamp01(dot(a, g) / g2);
+                const real s = clamp01(dot(a, g) / g2);
                 dx = add(dx, mul(h * h * s, g));
             }
         } else {
@@ -624,23 +637,23 @@ extern "C" int orc_substep(orc_sim* o, const yb_params* p, int doDetect) {
     if (doDetect) err = detect(o, p, dxw32);
 
     // recomputeStepLimit (collision.cu:121-159), FP64.
-    const double detR = (double)p->radius + 0.5 * (double)p->barrierThickness;
+    const real detR = (real)p->radius + 0.5 * (real)p->barrierThickness;
 #pragma omp parallel for schedule(static)
     for (int v = 0; v < n; v++) {
-        double minDist = INFINITY;
+        real minDist = INFINITY;
         if (p->useStepSizeLimit && has(o->flags[v], YB_FLAG_HAS_NEXT)) {
             const D3 a0 = g3(o->pos, v);
             const D3 a1 = sub(g3(o->pos, v + 1), a0);
-            minDist = detR * ((double)p->detectionScaler - 1.0);
+            minDist = detR * ((real)p->detectionScaler - 1.0);
             if (p->useVelocityRadius) minDist += len(g3(o->dx, v));
             for (int i = 0; i < o->numCols[v]; i++) {
                 const int oid = o->cols[v + (size_t)n * i];
                 const D3 b0 = sub(g3(o->pos, oid), a0);
                 const D3 b1 = sub(g3(o->pos, oid + 1), a0);
-                double u, w;
-                closest<double>(D3{0, 0, 0}, a1, b0, b1, u, w);
+                real u, w;
+                closest<real>(D3{0, 0, 0}, a1, b0, b1, u, w);
                 const D3 nrm = sub(mul(u, a1), mix(b0, b1, w));
-                minDist = std::min(minDist, 0.45 * len(nrm));
+                minDist = std::min(minDist, real(0.45) * len(nrm));
             }
         }
         o->stepLimit[v] = minDist;
@@ -666,19 +679,19 @@ extern "C" int orc_substep(orc_sim* o, const yb_params* p, int doDetect) {
                 s3(o->dxNext, v, d);
                 continue;
             }
-            const double mh2 = 1.0 / (o->invMass[v] * h * h);
+            const real mh2 = 1.0 / (o->invMass[v] * h * h);
             S3 H{};
             s3_diag(H, mh2);
             D3 f = mul(mh2, sub(g3(o->y, v), d));
             const int c = o->cid[v];
             if (c >= 0) {
-                const double st = 40.0;
+                const real st = 40.0;
                 f = sub(f, mul(st, add(add(sub(g3(o->pos, v), g3(o->pos, c)),
                                            sub(d, g3(o->dx, c))),
                                        mul(damping, d))));
                 s3_diag(H, st * (1.0 + damping));
             }
-            double limit = INFINITY;
+            real limit = INFINITY;
             if (has(o->flags[v], YB_FLAG_HAS_NEXT)) {
                 f = add(f, seg[v].f0);
                 for (int k = 0; k < 6; k++) H.a[k] += seg[v].H0.a[k];
@@ -689,12 +702,12 @@ extern "C" int orc_substep(orc_sim* o, const yb_params* p, int doDetect) {
                 for (int k = 0; k < 6; k++) H.a[k] += seg[v - 1].H1.a[k];
                 limit = std::min(limit, o->stepLimit[v - 1]);
             }
-            D3 nd = add(d, mul((double)p->accelerationRatio, s3_solve(H, f)));
-            const double l = len(nd);
+            D3 nd = add(d, mul((real)p->accelerationRatio, s3_solve(H, f)));
+            const real l = len(nd);
             if (l > limit && l > 0) nd = mulr(nd, limit / l);
             s3(o->dxNext, v, nd);
             num += len2(sub(nd, d));
-            double li = o->lRest[v];
+            real li = o->lRest[v];
             if (!has(o->flags[v], YB_FLAG_HAS_NEXT) && has(o->flags[v], YB_FLAG_HAS_PREV))
                 li = o->lRest[v - 1];
             den += li * li;
@@ -710,24 +723,24 @@ extern "C" int orc_substep(orc_sim* o, const yb_params* p, int doDetect) {
                 const D3 e = add(sub(g3(o->pos, v + 1), g3(o->pos, v)),
                                  sub(g3(o->dx, v + 1), g3(o->dx, v)));
                 D3 vv = {e.x / o->lRest[v], e.y / o->lRest[v], e.z / o->lRest[v]};
-                vv = mulr(vv, -2.0 * o->kStretch[v]);
+                vv = mulr(vv, real(-2.0) * o->kStretch[v]);
                 DR b = {0, 0, 0, 0};
                 if (has(fl, YB_FLAG_HAS_PREV)) {
                     const DR qr = g4(o->qRest, v - 1);
                     const DR qq = g4(o->q, v - 1);
-                    const double sg = rdot(rmul(rinv(qq), q0), qr) > 0 ? 1.0 : -1.0;
+                    const real sg = rdot(rmul(rinv(qq), q0), qr) > 0 ? 1.0 : -1.0;
                     b = radd(b, rscale(rmul(qq, qr), sg));
                 }
                 if (has(fl, YB_FLAG_HAS_NEXT_ORIENTATION)) {
                     const DR qr = g4(o->qRest, v);
                     const DR qq = g4(o->q, v + 1);
-                    const double sg = rdot(rmul(rinv(q0), qq), qr) > 0 ? 1.0 : -1.0;
+                    const real sg = rdot(rmul(rinv(q0), qq), qr) > 0 ? 1.0 : -1.0;
                     b = radd(b, rscale(rmul(qq, rinv(qr)), sg));
                 }
-                const double lambda = len(vv) + rlen(b);
-                const DR t = rmul(rmul(mkr(vv, 0.0), b), DR{1, 0, 0, 0});
+                const real lambda = len(vv) + rlen(b);
+                const DR t = rmul(rmul(mkr(vv, real(0.0)), b), DR{1, 0, 0, 0});
                 const DR r = radd(t, rscale(b, lambda));
-                q0 = rscale(r, 1.0 / rlen(r));
+                q0 = rscale(r, real(1.0) / rlen(r));
             }
             s4(o->qNext, v, q0);
         }
@@ -741,11 +754,11 @@ extern "C" int orc_substep(orc_sim* o, const yb_params* p, int doDetect) {
     o->totalSweeps += used;
 
     // endItr (iteration.cu:48-61).
-    const double s = 1.0 - (double)p->drag * h;
+    const real s = 1.0 - (real)p->drag * h;
 #pragma omp parallel for schedule(static)
     for (int v = 0; v < n; v++) {
         const D3 d = g3(o->dx, v);
-        if (o->invMass[v] != 0.0) s3(o->vel, v, mulr(mulr(d, 1.0 / h), s));
+        if (o->invMass[v] != 0.0) s3(o->vel, v, mulr(mulr(d, real(1.0) / h), s));
         s3(o->pos, v, add(g3(o->pos, v), d));
     }
     return err;
@@ -776,12 +789,13 @@ extern "C" void orc_contacts(const orc_sim* o, int32_t* numCols, int32_t* cols) 
 
 extern "C" void orc_last_prologue(const orc_sim* o, double* stepLimit, double* dxWarm) {
     const size_t N = (size_t)o->n;
-    if (stepLimit) memcpy(stepLimit, o->stepLimit.data(), sizeof(double) * N);
-    if (dxWarm) memcpy(dxWarm, o->dxWarm.data(), sizeof(double) * 3 * N);
+    for (size_t k = 0; stepLimit && k < N; k++) stepLimit[k] = o->stepLimit[k];
+    for (size_t k = 0; dxWarm && k < 3 * N; k++) dxWarm[k] = o->dxWarm[k];
 }
 
 extern "C" double orc_last_residual(const orc_sim* o) { return o->lastResidual; }
 
+#ifndef ORC_FP32
 // ============================================================================
 // FP64 pieces for closed-form tests.
 extern "C" void orc_closest(const double* a0, const double* a1, const double* b0,
@@ -880,3 +894,4 @@ extern "C" int orc_configure(int n, yb_vertex* verts, double* qs, double* qRests
     }
     return 0;
 }
+#endif  // ORC_FP32
