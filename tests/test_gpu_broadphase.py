@@ -147,3 +147,69 @@ def test_karras_tree_invariants(name, make):
     assert np.array_equal(order, g["order"]), "refit must keep the sorted order"
     check_tree(sim, n, order, aabb)
     sim.close()
+
+
+def starburst(strands=32, per=64, yarn_r=1e-4, seg_len=4e-4, seed=3):
+    """Straight strands in random directions, all passing within 1e-4 of the
+    origin: near the centre every segment's box overlaps those of every other
+    strand, so the tree walk keeps many subtrees pending at once."""
+    rng = np.random.default_rng(seed)
+    pos = []
+    for _ in range(strands):
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        c = rng.uniform(-1e-4, 1e-4, 3)
+        t = (np.arange(per) - (per - 1) / 2.0) * seg_len
+        pos.append(c[None, :] + t[:, None] * d[None, :])
+    pos = np.concatenate(pos).astype(np.float32)
+    radius, barrier = scenes._curve_radius(yarn_r)
+    ends = [k * per + per - 1 for k in range(strands)]
+    return scenes._configure("starburst", pos, ends, [], kCollision=1e-7, kBend=1e-8,
+                             kStretch=2e-2, maxH=1e-3, radius=radius, barrier=barrier)
+
+
+def max_pending(child, box, aabb, flags):
+    """Largest number of pending subtrees any query holds, walking the tree as
+    k_query does (left child first, the right one pushed when both overlap)."""
+    best = 0
+    for i in range(len(flags)):
+        if not flags[i] & 2:
+            continue
+        qlo, qhi = aabb[i, 0:3], aabb[i, 4:7]
+        node, stack = 0, []
+        while True:
+            hit = [bool(np.all(qlo <= box[node, 2 * s + 1, :3]) and np.all(box[node, 2 * s, :3] <= qhi))
+                   for s in range(2)]
+            cl, cr = int(child[node, 0]), int(child[node, 1])
+            tl, tr = hit[0] and cl >= 0, hit[1] and cr >= 0
+            if tl or tr:
+                node = cl if tl else cr
+                if tl and tr:
+                    stack.append(cr)
+                    best = max(best, len(stack))
+            elif stack:
+                node = stack.pop()
+            else:
+                break
+    return best
+
+
+@pytest.mark.gpu
+def test_deep_traversal_contact_sets_bit_exact():
+    """Walks deeper than the 8 stack entries the query keeps in LDS use its
+    scratch spill; the contact sets must still equal the FP32 twin's."""
+    sim, sc, p, lv, g = prologue(starburst())
+    n = sc.n
+    child = np.zeros(2 * (n - 1), np.int32)
+    parent = np.zeros(2 * n - 1, np.int32)
+    box = np.zeros(16 * (n - 1), np.float32)
+    check(sim._lib.yb_debug_tree(sim.handle, ptr(child), ptr(parent), ptr(box)))
+    depth = max_pending(child.reshape(n - 1, 2), box.reshape(n - 1, 4, 4), g["aabb"],
+                        sc.verts["flags"])
+    t = twin(sc, p, lv)
+    print(f"starburst: deepest walk keeps {depth} subtrees pending, contacts {int(t['nc'].sum())}, "
+          f"max per segment {int(t['nc'].max())}")
+    assert depth > 8, "the scene must drive the walk past the LDS part of the stack"
+    assert t["err"] == 0 and int(t["nc"].max()) < 128
+    assert contact_sets(g["nc"], g["cols"]) == contact_sets(t["nc"], t["cols"]), "contact sets"
+    sim.close()
