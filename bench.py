@@ -328,12 +328,17 @@ def main():
                     help="keep the default block sizes instead of timing the candidates")
     ap.add_argument("--no-per-scene", action="store_true",
                     help="batched_walks: skip the one-handle-per-scene comparison")
+    ap.add_argument("--launch-config", type=int, nargs=2, default=None,
+                    metavar=("SWEEP_THREADS", "QUERY_THREADS"),
+                    help="fixed block sizes (skips the autotuner)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the stepper runs on a single MI355X "
               "(multi-GPU sharding is design-only)", file=sys.stderr)
         sys.exit(2)
-    args.launch_config = None
+    if args.launch_config:
+        args.launch_config = tuple(args.launch_config)
+        args.no_autotune = True
     if args.workload == "batched_walks":
         return bench_batched(args)
 

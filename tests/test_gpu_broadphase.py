@@ -213,3 +213,58 @@ def test_deep_traversal_contact_sets_bit_exact():
     assert t["err"] == 0 and int(t["nc"].max()) < 128
     assert contact_sets(g["nc"], g["cols"]) == contact_sets(t["nc"], t["cols"]), "contact sets"
     sim.close()
+
+
+@pytest.mark.gpu
+def test_refit_top_nodes_of_a_tree_over_2048_refit_blocks():
+    """Past 2048 refit blocks of 1024 leaves, the upper levels of the table of
+    block unions are built by one launch each.  The top nodes of the tree span
+    many blocks, so their boxes come from those levels; each must equal the
+    union of its leaves' boxes, bit for bit."""
+    sc = scenes.knit_patch(rows=512, per=4200)
+    n = sc.n
+    assert n > 2048 * 1024
+    sim = gpu_sim(sc)
+    for _ in range(2):
+        sim.advance(sim.maxH)
+    p = sim.meta.copy()
+    check(sim._lib.yb_debug_prologue(sim.handle, ctypes.byref(p), np.float32(sim.maxH), 1, 0))
+    aabb = np.zeros((n, 8), np.float32)
+    order = np.zeros(n, np.int32)
+    check(sim._lib.yb_debug_read(sim.handle, None, ptr(aabb), None, ptr(order), None, None))
+    child = np.zeros(2 * (n - 1), np.int32)
+    parent = np.zeros(2 * n - 1, np.int32)
+    box = np.zeros(16 * (n - 1), np.float32)
+    check(sim._lib.yb_debug_tree(sim.handle, ptr(child), ptr(parent), ptr(box)))
+    child = child.reshape(n - 1, 2)
+    box = box.reshape(n - 1, 4, 4)
+    rank = np.empty(n, np.int64)
+    rank[order] = np.arange(n)
+    lo_sorted = aabb[order, 0:3]
+    hi_sorted = aabb[order, 4:7]
+
+    def end_leaf(c, side):
+        while c >= 0:
+            c = int(child[c, side])
+        return int(rank[~c])
+
+    checked, level = 0, [0]
+    for depth in range(5):
+        nxt = []
+        for node in level:
+            for side in range(2):
+                c = int(child[node, side])
+                a, b = (end_leaf(c, 0), end_leaf(c, 1)) if c >= 0 else (rank[~c], rank[~c])
+                lo = lo_sorted[a:b + 1].min(axis=0)
+                hi = hi_sorted[a:b + 1].max(axis=0)
+                assert np.array_equal(box[node, 2 * side, :3], lo), (depth, node, side)
+                assert np.array_equal(box[node, 2 * side + 1, :3], hi), (depth, node, side)
+                checked += 1
+                if c >= 0:
+                    nxt.append(c)
+        level = nxt
+    root = np.concatenate(sim.bounds())
+    assert np.array_equal(root[:3], lo_sorted.min(axis=0))
+    assert np.array_equal(root[3:], hi_sorted.max(axis=0))
+    print(f"{n} leaves, {(n + 1023) // 1024} refit blocks: {checked} top child boxes exact")
+    sim.close()

@@ -545,72 +545,117 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_local(
         }
     }
 
-    // Inclusive prefix and suffix unions over the block's leaves (Hillis-Steele
-    // in LDS, reusing nbox for the suffix), for the ends of straddling ranges.
-    for (int off = 1; off < kRefitLeaves; off <<= 1) {
-        float4 pl[kRefitPer][2], sl[kRefitPer][2];
-        for (int r = 0; r < kRefitPer; r++) {
-            const int m = t + kRefitThreads * r;
-            pl[r][0] = lbox[m][0];
-            pl[r][1] = lbox[m][1];
-            if (off == 1) {
-                sl[r][0] = lbox[m][0];
-                sl[r][1] = lbox[m][1];
-            } else {
-                sl[r][0] = nbox[m][0];
-                sl[r][1] = nbox[m][1];
-            }
-            if (m - off >= 0 && m < cntLeaves) box_union(pl[r][0], pl[r][1], lbox[m - off][0], lbox[m - off][1]);
-            const float4* src = (off == 1) ? &lbox[0][0] : &nbox[0][0];
-            if (m + off < cntLeaves && m < cntLeaves)
-                box_union(sl[r][0], sl[r][1], src[2 * (m + off)], src[2 * (m + off) + 1]);
-        }
-        __syncthreads();
-        for (int r = 0; r < kRefitPer; r++) {
-            const int m = t + kRefitThreads * r;
-            lbox[m][0] = pl[r][0];
-            lbox[m][1] = pl[r][1];
-            nbox[m][0] = sl[r][0];
-            nbox[m][1] = sl[r][1];
-        }
-        __syncthreads();
+    // Inclusive prefix and suffix unions over the block's leaves, for the ends
+    // of straddling ranges: scans within each wave by shuffles, then across
+    // the block's waves through LDS.  Leaves past the block's end are empty
+    // boxes.  Min/max make every order of combination give the same box.
+    static_assert(kRefitPer == 1 && kRefitThreads % 64 == 0, "one leaf per thread");
+    constexpr int kWaves = kRefitThreads / 64;
+    __shared__ float4 wpre[kWaves][2], wsuf[kWaves][2];
+    const int lane = t & 63, w = t >> 6;
+    float4 plo = make_float4(INFINITY, INFINITY, INFINITY, 0.f);
+    float4 phi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.f);
+    if (t < cntLeaves) {
+        plo = lbox[t][0];
+        phi = lbox[t][1];
     }
-    // lbox now holds prefix unions, nbox suffix unions.
-    for (int r = 0; r < kRefitPer; r++) {
-        const int k = base + t + kRefitThreads * r;
-        if (k >= end) continue;
-        const int need = leafNeed[k];
-        if (need & 1) {
-            sufBox[2 * k] = nbox[k - base][0];
-            sufBox[2 * k + 1] = nbox[k - base][1];
-        }
-        if (need & 2) {
-            preBox[2 * k] = lbox[k - base][0];
-            preBox[2 * k + 1] = lbox[k - base][1];
-        }
-        if (k == base) {  // block union = prefix at the last leaf
-            blkBox[2 * blockIdx.x] = lbox[cntLeaves - 1][0];
-            blkBox[2 * blockIdx.x + 1] = lbox[cntLeaves - 1][1];
+    float4 slo = plo, shi = phi;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const float4 ulo = make_float4(__shfl_up(plo.x, off, 64), __shfl_up(plo.y, off, 64),
+                                       __shfl_up(plo.z, off, 64), 0.f);
+        const float4 uhi = make_float4(__shfl_up(phi.x, off, 64), __shfl_up(phi.y, off, 64),
+                                       __shfl_up(phi.z, off, 64), 0.f);
+        const float4 dlo = make_float4(__shfl_down(slo.x, off, 64), __shfl_down(slo.y, off, 64),
+                                       __shfl_down(slo.z, off, 64), 0.f);
+        const float4 dhi = make_float4(__shfl_down(shi.x, off, 64), __shfl_down(shi.y, off, 64),
+                                       __shfl_down(shi.z, off, 64), 0.f);
+        if (lane >= off) box_union(plo, phi, ulo, uhi);
+        if (lane + off < 64) box_union(slo, shi, dlo, dhi);
+    }
+    if (lane == 63) {
+        wpre[w][0] = plo;
+        wpre[w][1] = phi;
+    }
+    if (lane == 0) {
+        wsuf[w][0] = slo;
+        wsuf[w][1] = shi;
+    }
+    __syncthreads();
+    for (int k = 0; k < w; k++) box_union(plo, phi, wpre[k][0], wpre[k][1]);
+    for (int k = w + 1; k < kWaves; k++) box_union(slo, shi, wsuf[k][0], wsuf[k][1]);
+    {
+        const int k = base + t;
+        if (k < end) {
+            const int need = leafNeed[k];
+            if (need & 1) {
+                sufBox[2 * k] = slo;
+                sufBox[2 * k + 1] = shi;
+            }
+            if (need & 2) {
+                preBox[2 * k] = plo;
+                preBox[2 * k + 1] = phi;
+            }
+            if (t == cntLeaves - 1) {  // block union = prefix at the last leaf
+                blkBox[2 * blockIdx.x] = plo;
+                blkBox[2 * blockIdx.x + 1] = phi;
+            }
         }
     }
 }
 
 // Sparse table over block unions: level L, entry b = union of blocks
-// [b, b + 2^L).  One workgroup; levels separated by barriers.
+// [b, b + 2^L) (clipped at nb).  Levels 1..kTabLocal: one workgroup per 1024
+// entries, working in LDS on a 2048-entry window (entry b at level L needs
+// level L-1 at b + 2^(L-1), at most 512 past the workgroup's last entry).
+// Higher levels (more than 2048 blocks): one launch each over all entries.
+constexpr int kTabLocal = 10;
 __global__ __launch_bounds__(1024) void k_refit_table(int nb, int levels,
                                                       float4* __restrict__ tab) {
-    for (int L = 1; L < levels; L++) {
-        __syncthreads();
+    __shared__ float4 wlo[2048], whi[2048];
+    const int t = threadIdx.x;
+    const int b0 = blockIdx.x * 1024;
+    for (int j = t; j < 2048; j += 1024) {
+        const int b = b0 + j;
+        wlo[j] = b < nb ? tab[2 * b] : make_float4(INFINITY, INFINITY, INFINITY, 0.f);
+        whi[j] = b < nb ? tab[2 * b + 1] : make_float4(-INFINITY, -INFINITY, -INFINITY, 0.f);
+    }
+    const int top = min(levels - 1, kTabLocal);
+    for (int L = 1; L <= top; L++) {
         const int half = 1 << (L - 1);
-        const float4* src = tab + (size_t)2 * nb * (L - 1);
-        float4* dst = tab + (size_t)2 * nb * L;
-        for (int b = threadIdx.x; b < nb; b += blockDim.x) {
-            float4 lo = src[2 * b], hi = src[2 * b + 1];
-            if (b + half < nb) box_union(lo, hi, src[2 * (b + half)], src[2 * (b + half) + 1]);
-            dst[2 * b] = lo;
-            dst[2 * b + 1] = hi;
+        __syncthreads();
+        float4 lo[2], hi[2];
+        for (int r = 0; r < 2; r++) {
+            const int j = t + 1024 * r;
+            lo[r] = wlo[j];
+            hi[r] = whi[j];
+            if (j + half < 2048) box_union(lo[r], hi[r], wlo[j + half], whi[j + half]);
+        }
+        __syncthreads();
+        for (int r = 0; r < 2; r++) {
+            wlo[t + 1024 * r] = lo[r];
+            whi[t + 1024 * r] = hi[r];
+        }
+        const int b = b0 + t;
+        if (b < nb) {
+            float4* dst = tab + (size_t)2 * nb * L;
+            dst[2 * b] = lo[0];
+            dst[2 * b + 1] = hi[0];
         }
     }
+}
+
+__global__ __launch_bounds__(256) void k_refit_table_level(int nb, int L,
+                                                           float4* __restrict__ tab) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    const int half = 1 << (L - 1);
+    const float4* src = tab + (size_t)2 * nb * (L - 1);
+    float4* dst = tab + (size_t)2 * nb * L;
+    float4 lo = src[2 * b], hi = src[2 * b + 1];
+    if (b + half < nb) box_union(lo, hi, src[2 * (b + half)], src[2 * (b + half) + 1]);
+    dst[2 * b] = lo;
+    dst[2 * b + 1] = hi;
 }
 
 __global__ __launch_bounds__(256) void k_refit_top(int n, int nb,
@@ -904,8 +949,12 @@ void launch_refit(yb_sim* s) {
                        s->b.order, s->b.aabb, s->b.nodes, s->b.parent, s->b.rng, s->b.kids,
                        s->b.leafNeed, s->b.sufBox, s->b.preBox, s->b.blkTab, s->b.sceneBox);
     if (s->refitBlocks > 1) {
-        hipLaunchKernelGGL(k_refit_table, dim3(1), dim3(1024), 0, s->stream, s->refitBlocks,
+        const int nb = s->refitBlocks;
+        hipLaunchKernelGGL(k_refit_table, dim3(blocks(nb, 1024)), dim3(1024), 0, s->stream, nb,
                            s->refitLevels, s->b.blkTab);
+        for (int L = kTabLocal + 1; L < s->refitLevels; L++)
+            hipLaunchKernelGGL(k_refit_table_level, dim3(blocks(nb, 256)), dim3(256), 0, s->stream,
+                               nb, L, s->b.blkTab);
         hipLaunchKernelGGL(k_refit_top, dim3(s->refitTopBlocks), dim3(256), 0, s->stream, n,
                            s->refitBlocks, s->b.strad, s->b.stradCount, s->b.rng, s->b.parent,
                            s->b.sufBox, s->b.preBox, s->b.blkTab, s->b.nodes, s->b.sceneBox);
