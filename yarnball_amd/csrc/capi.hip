@@ -45,7 +45,7 @@ static void free_all(yb_sim* s) {
     void* ptrs[] = {b.pos,     b.prm,     b.vel,      b.lastVel, b.yv,      b.dx[0],
                     b.dx[1],   b.q[0],    b.q[1],     b.qRest,   b.numCols, b.cols,
                     b.aabb,    b.sceneBox, b.blockBox, b.morton, b.keys[0], b.keys[1],
-                    b.vals[0], b.vals[1], b.hist,     b.nodes,   b.parent,  b.rng,
+                    b.vals[0], b.vals[1], b.hist, b.scanPart, b.nodes,   b.parent,  b.rng,
                     b.kids,    b.strad,   b.stradCount, b.leafNeed, b.sufBox, b.preBox, b.blkTab,
                     b.err,     b.params,  b.stage, b.ctl, b.partial, b.minSep,
                     b.mlpPrm,  b.mlpTiles, b.mlpBias, b.featB, b.corrPrev, b.dxPlain, b.predC,
@@ -142,6 +142,7 @@ extern "C" int yb_create(int n, yb_sim** out) {
     ALLOC(b.vals[0], N);
     ALLOC(b.vals[1], N);
     ALLOC(b.hist, (size_t)kSortBins * s->sortTiles);
+    ALLOC(b.scanPart, ((size_t)kSortBins * s->sortTiles + kScanBlock - 1) / kScanBlock);
     ALLOC(b.nodes, 4 * (N - 1));
     ALLOC(b.parent, 2 * N - 1);
     ALLOC(b.rng, N - 1);

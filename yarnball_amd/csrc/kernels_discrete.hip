@@ -244,29 +244,83 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_hist(int n, int shift, in
     hist[t * tiles + blockIdx.x] = lh[t];
 }
 
-constexpr int kScanThreads = 1024;
-__global__ __launch_bounds__(kScanThreads) void k_sort_scan(int total, uint32_t* __restrict__ hist) {
-    __shared__ uint32_t part[kScanThreads];
-    const int t = threadIdx.x;
-    const int chunk = (total + kScanThreads - 1) / kScanThreads;
-    const int b = t * chunk;
-    const int e = min(b + chunk, total);
-    uint32_t sum = 0;
-    for (int i = b; i < e; i++) sum += hist[i];
-    part[t] = sum;
-    __syncthreads();
-    // Hillis-Steele inclusive scan of the per-thread sums.
-    for (int off = 1; off < kScanThreads; off <<= 1) {
-        uint32_t x = (t >= off) ? part[t - off] : 0u;
-        __syncthreads();
-        part[t] += x;
-        __syncthreads();
+// Exclusive scan of the digit-major histogram (hist[digit * tiles + tile]), in
+// three steps: per-workgroup sums of kScanBlock counters, a scan of those sums,
+// and a scan within each workgroup's counters from its offset.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
     }
-    uint32_t run = part[t] - sum;
+    return x;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(int total,
+                                                              const uint32_t* __restrict__ hist,
+                                                              uint32_t* __restrict__ part) {
+    __shared__ uint32_t ws[kScanThreads / 64];
+    const int t = threadIdx.x;
+    const int base = blockIdx.x * kScanBlock;
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        const int i = base + k * kScanThreads + t;
+        if (i < total) sum += hist[i];
+    }
+    sum = wave_incl_scan(sum);
+    if ((t & 63) == 63) ws[t >> 6] = sum;
+    __syncthreads();
+    if (t == 0) {
+        uint32_t tot = 0;
+        for (int w = 0; w < kScanThreads / 64; w++) tot += ws[w];
+        part[blockIdx.x] = tot;
+    }
+}
+
+// One workgroup: exclusive scan of the per-workgroup sums, in place.
+__global__ __launch_bounds__(1024) void k_scan_parts(int parts, uint32_t* __restrict__ part) {
+    __shared__ uint32_t ws[16];
+    const int t = threadIdx.x;
+    const int per = (parts + 1023) / 1024;
+    const int b = t * per, e = min(b + per, parts);
+    uint32_t sum = 0;
+    for (int i = b; i < e; i++) sum += part[i];
+    const uint32_t inc = wave_incl_scan(sum);
+    if ((t & 63) == 63) ws[t >> 6] = inc;
+    __syncthreads();
+    uint32_t run = inc - sum;
+    for (int w = 0; w < (t >> 6); w++) run += ws[w];
     for (int i = b; i < e; i++) {
-        const uint32_t c = hist[i];
-        hist[i] = run;
+        const uint32_t c = part[i];
+        part[i] = run;
         run += c;
+    }
+}
+
+// Each thread scans kScanItems consecutive counters.
+__global__ __launch_bounds__(kScanThreads) void k_scan_down(int total, uint32_t* __restrict__ hist,
+                                                            const uint32_t* __restrict__ part) {
+    __shared__ uint32_t ws[kScanThreads / 64];
+    const int t = threadIdx.x;
+    const int base = blockIdx.x * kScanBlock + t * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        v[k] = base + k < total ? hist[base + k] : 0u;
+        sum += v[k];
+    }
+    const uint32_t inc = wave_incl_scan(sum);
+    if ((t & 63) == 63) ws[t >> 6] = inc;
+    __syncthreads();
+    uint32_t run = part[blockIdx.x] + inc - sum;
+    for (int w = 0; w < (t >> 6); w++) run += ws[w];
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        if (base + k < total) hist[base + k] = run;
+        run += v[k];
     }
 }
 
@@ -924,8 +978,13 @@ void launch_rebuild(yb_sim* s) {
         const int32_t* sof = scenePass ? s->b.sceneOf : nullptr;
         hipLaunchKernelGGL(k_sort_hist, dim3(tiles), dim3(kSortThreads), 0, s->stream, n, shift,
                            tiles, s->b.keys[in], s->b.vals[in], sof, s->b.hist);
-        hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(kScanThreads), 0, s->stream,
-                           tiles * kSortBins, s->b.hist);
+        const int total = tiles * kSortBins;
+        const int parts = blocks(total, kScanBlock);
+        hipLaunchKernelGGL(k_scan_reduce, dim3(parts), dim3(kScanThreads), 0, s->stream, total,
+                           s->b.hist, s->b.scanPart);
+        hipLaunchKernelGGL(k_scan_parts, dim3(1), dim3(1024), 0, s->stream, parts, s->b.scanPart);
+        hipLaunchKernelGGL(k_scan_down, dim3(parts), dim3(kScanThreads), 0, s->stream, total,
+                           s->b.hist, s->b.scanPart);
         hipLaunchKernelGGL(k_sort_scatter, dim3(tiles), dim3(kSortThreads), 0, s->stream, n, shift,
                            tiles, s->b.keys[in], s->b.vals[in], s->b.keys[in ^ 1],
                            s->b.vals[in ^ 1], sof, s->b.hist);

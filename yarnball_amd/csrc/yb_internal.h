@@ -96,6 +96,9 @@ constexpr int kSortThreads = 256;
 constexpr int kSortItems = 8;
 constexpr int kSortTile = kSortThreads * kSortItems;
 constexpr int kRefitLeaves = 1024;  // sorted leaves per refit workgroup
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanBlock = kScanThreads * kScanItems;  // histogram counters per scan workgroup
 
 // SoA device state.  Every per-vertex array is float4 (16 B, one dwordx4 per
 // lane, fully coalesced):
@@ -130,6 +133,7 @@ struct Buffers {
     uint32_t* keys[2] = {nullptr, nullptr};
     int32_t* vals[2] = {nullptr, nullptr};
     uint32_t* hist = nullptr;     // kSortBins * numTiles
+    uint32_t* scanPart = nullptr; // per-workgroup sums of the histogram scan
     int32_t* order = nullptr;     // sorted leaf -> segment
     float4* nodes = nullptr;      // 4 per internal node: lo0, hi0, lo1, hi1 (child ids in .w)
     int32_t* parent = nullptr;    // 2n-1: (parent << 1) | side
