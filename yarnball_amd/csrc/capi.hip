@@ -997,6 +997,9 @@ extern "C" int yb_debug_tree(yb_sim* s, int32_t* child, int32_t* parent, float* 
         set_message(std::string("yb_debug_tree: ") + hipGetErrorString(e));
         return YB_ERR_HIP;
     }
+    if (parent)  // drop the refit-block flag: plain (node << 1) | side slots
+        for (size_t i = 0; i < 2 * N - 1; i++)
+            if (parent[i] >= 0) parent[i] &= ~kParentStrad;
     for (size_t i = 0; i + 1 < N; i++) {
         if (child) {
             int32_t cl, cr;

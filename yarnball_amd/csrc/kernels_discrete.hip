@@ -448,22 +448,24 @@ __global__ __launch_bounds__(256) void k_karras(int n, const uint32_t* __restric
     const int gamma = i + s * d + min(d, 0);
     const int lo = min(i, j);
     const int hi = max(i, j);
+    // the children's parent slots carry whether this node straddles refit blocks
+    const int32_t sf = (lo / kRefitLeaves != hi / kRefitLeaves) ? kParentStrad : 0;
     int cl, cr, kl, kr;
     if (lo == gamma) {
         cl = ~order[gamma];
         kl = ~gamma;
-        parent[(n - 1) + gamma] = (i << 1) | 0;
+        parent[(n - 1) + gamma] = (i << 1) | 0 | sf;
     } else {
         cl = kl = gamma;
-        parent[gamma] = (i << 1) | 0;
+        parent[gamma] = (i << 1) | 0 | sf;
     }
     if (hi == gamma + 1) {
         cr = ~order[gamma + 1];
         kr = ~(gamma + 1);
-        parent[(n - 1) + gamma + 1] = (i << 1) | 1;
+        parent[(n - 1) + gamma + 1] = (i << 1) | 1 | sf;
     } else {
         cr = kr = gamma + 1;
-        parent[gamma + 1] = (i << 1) | 1;
+        parent[gamma + 1] = (i << 1) | 1 | sf;
     }
     nodes[4 * i + 0].w = __int_as_float(cl);
     nodes[4 * i + 2].w = __int_as_float(cr);
@@ -520,58 +522,62 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_local(
     float4* __restrict__ sceneBox) {
     __shared__ float4 lbox[kRefitLeaves][2];
     __shared__ float4 nbox[kRefitLeaves][2];
-    __shared__ int32_t cnt[kRefitLeaves];
+    __shared__ int2 lkids[kRefitLeaves];          // kids[] of nodes base..end-1
+    __shared__ int32_t lpar[kRefitLeaves];        // parent[] of nodes base..end-1
+    __shared__ uint32_t arrived[kRefitLeaves / 32];  // first child done, per node
     const int t = threadIdx.x;
     const int base = blockIdx.x * kRefitLeaves;
     const int end = min(base + kRefitLeaves, n);
     const int cntLeaves = end - base;
-    for (int r = 0; r < kRefitPer; r++) {
-        const int k = base + t + kRefitThreads * r;
-        cnt[t + kRefitThreads * r] = 0;
-        if (k < end) {
-            const int seg = order[k];
-            lbox[k - base][0] = aabb[2 * seg];
-            lbox[k - base][1] = aabb[2 * seg + 1];
+    static_assert(kRefitPer == 1, "one leaf per thread");
+    // A Karras node's range starts or ends at its own index, so every node
+    // whose range lies in [base, end) has its index there too: its topology
+    // is loaded here once, with coalesced reads.
+    if (t < kRefitLeaves / 32) arrived[t] = 0u;
+    if (base + t < end) {
+        const int seg = order[base + t];
+        lbox[t][0] = aabb[2 * seg];
+        lbox[t][1] = aabb[2 * seg + 1];
+        if (base + t < n - 1) {
+            lkids[t] = kids[base + t];
+            lpar[t] = parent[base + t];
         }
     }
     __syncthreads();
 
-    // Climb through the nodes whose range lies in this block, in LDS only.
-    for (int r = 0; r < kRefitPer; r++) {
-        const int k = base + t + kRefitThreads * r;
-        if (k >= end) continue;
-        float4 lo = lbox[k - base][0];
-        float4 hi = lbox[k - base][1];
-        int slot = parent[(n - 1) + k];
+    // Climb from each leaf through the nodes whose range lies in this block,
+    // in LDS only; the second child to arrive at a node forms its box.
+    if (base + t < end) {
+        float4 lo = lbox[t][0];
+        float4 hi = lbox[t][1];
+        int slot = parent[(n - 1) + base + t];
         bool leaf = true;
         while (slot >= 0) {
-            const int p = slot >> 1;
-            // rng, kids and parent are read-only here: fetch all three at once,
-            // so a level of the climb waits for one memory round trip
-            const int2 pr = rng[p];
-            const int2 kc = kids[p];
-            const int up = parent[p];
-            if (pr.x < base || pr.y >= end) {
+            const int p = (slot & ~kParentStrad) >> 1;
+            if (slot & kParentStrad) {
                 // p straddles: publish this child's box in p's slot for traversal.
                 const int side = slot & 1;
                 store_xyz(&nodes[4 * p + 2 * side], lo);
                 nodes[4 * p + 2 * side + 1] = make_float4(hi.x, hi.y, hi.z, leaf ? hi.w : 0.f);
                 break;
             }
+            const int q = p - base;
+            const uint32_t bit = 1u << (q & 31);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-            const int old = atomicAdd(&cnt[p - base], 1);
-            if (old == 0) break;
+            const uint32_t old = atomicOr(&arrived[q >> 5], bit);
+            if (!(old & bit)) break;
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            const int2 kc = lkids[q];
             float4 lo1, hi1;
             if (kc.x < 0) { lo = lbox[~kc.x - base][0]; hi = lbox[~kc.x - base][1]; }
             else { lo = nbox[kc.x - base][0]; hi = nbox[kc.x - base][1]; }
             if (kc.y < 0) { lo1 = lbox[~kc.y - base][0]; hi1 = lbox[~kc.y - base][1]; }
             else { lo1 = nbox[kc.y - base][0]; hi1 = nbox[kc.y - base][1]; }
             box_union(lo, hi, lo1, hi1);
-            nbox[p - base][0] = lo;
-            nbox[p - base][1] = hi;
+            nbox[q][0] = lo;
+            nbox[q][1] = hi;
             leaf = false;
-            slot = up;
+            slot = lpar[q];
         }
     }
     __syncthreads();
@@ -582,7 +588,7 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_local(
         if (p >= end || p >= n - 1) continue;
         const int2 pr = rng[p];
         if (pr.x < base || pr.y >= end) continue;
-        const int2 kc = kids[p];
+        const int2 kc = lkids[t];
         float4 lo0, hi0, lo1, hi1;
         if (kc.x < 0) { lo0 = lbox[~kc.x - base][0]; hi0 = lbox[~kc.x - base][1]; }
         else { lo0 = nbox[kc.x - base][0]; hi0 = nbox[kc.x - base][1]; }
@@ -742,7 +748,7 @@ __global__ __launch_bounds__(256) void k_refit_top(int n, int nb,
             sceneBox[0] = lo;
             sceneBox[1] = hi;
         } else {
-            const int slot = parent[i];
+            const int slot = parent[i] & ~kParentStrad;
             const int p = slot >> 1;
             const int side = slot & 1;
             store_xyz(&nodes[4 * p + 2 * side], lo);

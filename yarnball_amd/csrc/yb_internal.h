@@ -96,6 +96,9 @@ constexpr int kSortThreads = 256;
 constexpr int kSortItems = 8;
 constexpr int kSortTile = kSortThreads * kSortItems;
 constexpr int kRefitLeaves = 1024;  // sorted leaves per refit workgroup
+// Flag in a parent slot ((node << 1) | side, node < 2^28): set when that parent
+// node's leaf range crosses a boundary between refit workgroups.
+constexpr int32_t kParentStrad = 1 << 30;
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 16;
 constexpr int kScanBlock = kScanThreads * kScanItems;  // histogram counters per scan workgroup
