@@ -49,10 +49,11 @@ def stage_bytes(stage, c):
     }.get(stage)
 
 
-# HBM bytes per vertex of one k_sweep launch on the knit patch (256 x 4096,
-# residual 1e-6, c = 4.42 contacts per segment), from rocprofv3 FETCH_SIZE and
-# WRITE_SIZE scaled by a copy kernel's counts (profiles/sweep_knit1M_hbm_traffic.md).
-MEASURED_TRAFFIC = {("knit_patch", "sweep"): 160.6}
+# HBM bytes per vertex of one k_sweep launch, from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE scaled by a copy kernel's counts: the knit patch (256 x 4096,
+# residual 1e-6, c = 4.42 contacts per segment; profiles/sweep_knit1M_hbm_traffic.md)
+# and the 512-scene batch (8 sweeps, c = 0.24; profiles/sweep_batched512_hbm_traffic.md).
+MEASURED_TRAFFIC = {("knit_patch", "sweep"): 160.6, ("batched_walks", "sweep"): 131.3}
 
 
 def build_workload(name, numItr):
@@ -246,7 +247,8 @@ def bench_batched(args):
         "roofline": {"kernel": "sweep", "bound": "hbm",
                      "achieved": round(alg / (sweep_us * 1e-6) / 1e9, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(alg / (sweep_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
-                     "traffic": None, "alg_bytes_per_launch": int(alg),
+                     "traffic": round(MEASURED_TRAFFIC[("batched_walks", "sweep")] * big.n),
+                     "alg_bytes_per_launch": int(alg),
                      "avg_launch_us": round(sweep_us, 2), "mean_contacts_per_segment": round(c, 3)},
         "cpu_baseline": cpu,
     }
