@@ -767,9 +767,11 @@ __global__ __launch_bounds__(256) void k_refit_top(int n, int nb,
 // with two atomicAdds per pair.
 constexpr int kStack = 64;  // traversal depth limit (pending right children)
 
+// All six comparisons are evaluated (no short circuit): six compares and five
+// mask ANDs, with no branches.
 __device__ __forceinline__ bool overlap(v3 qlo, v3 qhi, float4 lo, float4 hi) {
-    return (qlo.x <= hi.x) && (lo.x <= qhi.x) && (qlo.y <= hi.y) && (lo.y <= qhi.y) &&
-           (qlo.z <= hi.z) && (lo.z <= qhi.z);
+    return (qlo.x <= hi.x) & (lo.x <= qhi.x) & (qlo.y <= hi.y) & (lo.y <= qhi.y) &
+           (qlo.z <= hi.z) & (lo.z <= qhi.z);
 }
 
 struct SegInfo {

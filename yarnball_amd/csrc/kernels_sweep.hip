@@ -260,9 +260,10 @@ __global__ __launch_bounds__(512) void k_sweep(const DevParams* __restrict__ P,
                 const v3 db1 = xyz(gather4<true>(dxIn, o));
                 o = oNext;
                 // partner segment relative to this vertex, in the
-                // current configuration x + dx
+                // current configuration x + dx: start B0, direction A1
+                const v3 ddb = db1 - db0;
                 const v3 B0 = (b0 - x0) + (db0 - dx);
-                const v3 A1 = ((b1 - x0) + (db1 - dx)) - B0;
+                const v3 A1 = (b1 - b0) + ddb;
                 float u, w;
                 closest_e(e, ea, inva, B0, A1, u, w);
                 // closest-point difference (pos + dx of this segment minus
@@ -288,7 +289,7 @@ __global__ __launch_bounds__(512) void k_sweep(const DevParams* __restrict__ P,
                 // Lagged friction on the tangential relative displacement;
                 // its Hessian f1 w^2 (I - n n^T) is merged with the
                 // barrier's dH w^2 n n^T into one rank-1 update + diagonal.
-                const v3 ddpos = (dx + dseg * u) - (db0 + (db1 - db0) * w);
+                const v3 ddpos = (dx + dseg * u) - (db0 + ddb * w);
                 const v3 uu = ddpos - dot(nrm, ddpos) * nrm;
                 const float ul = len(uu);
                 float f1 = 0.0f;
