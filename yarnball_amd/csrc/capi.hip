@@ -164,7 +164,7 @@ extern "C" int yb_create(int n, yb_sim** out) {
     ALLOC(b.corrPrev, N);
     ALLOC(b.dxPlain, N);
     ALLOC(b.predC, N);
-    ALLOC(b.mlpCount, 2);
+    ALLOC(b.mlpCount, 2 * ((N + 63) / 64));
     ALLOC(b.recCount, 2);
     ALLOC(b.partial, (size_t)sweep_waves(n));
 #undef ALLOC
@@ -194,7 +194,8 @@ extern "C" int yb_create(int n, yb_sim** out) {
     if (e == hipSuccess) e = hipMemset(b.err, 0, sizeof(int32_t) * 2);
     if (e == hipSuccess) e = hipMemset(b.nodes, 0, sizeof(float4) * 4 * (N - 1));
     if (e == hipSuccess) e = hipMemset(b.corrPrev, 0, sizeof(float4) * N);
-    if (e == hipSuccess) e = hipMemset(b.mlpCount, 0, sizeof(unsigned long long) * 2);
+    if (e == hipSuccess)
+        e = hipMemset(b.mlpCount, 0, sizeof(unsigned long long) * 2 * ((N + 63) / 64));
     if (e == hipSuccess) e = hipMemset(b.recCount, 0, 2 * sizeof(int64_t));
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {

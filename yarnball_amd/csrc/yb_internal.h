@@ -162,7 +162,7 @@ struct Buffers {
     float4* corrPrev = nullptr;       // n: last converged correction (dx* - dx0)/L
     float4* dxPlain = nullptr;        // n: plain warm start of the current substep
     float4* predC = nullptr;          // n: predicted correction before smoothing
-    unsigned long long* mlpCount = nullptr;  // [applied, fallback]
+    unsigned long long* mlpCount = nullptr;  // [applied, fallback] per 64-vertex wave
     float* recX = nullptr;            // recording: cap x 48 raw features
     float* recT = nullptr;            // recording: cap x 3 targets
     int64_t* recCount = nullptr;      // [rows stored, recording enabled]

@@ -9,11 +9,16 @@ recorded by the residual-targeted plain solver, seeded and deterministic:
     1 rev/s by the per-frame driver pattern of main.cpp:54-87), every 4th
     substep of 8 frames, a seeded vertex subsample of each;
   * a knit patch of the evaluation size (256 rows x 4096 vertices) with a
-    different jitter seed (7), every 2nd substep of 4 frames, a seeded vertex
+    different jitter seed (7), every substep of frames 1-4 (the first frame,
+    the settling transient, is run but not recorded), a seeded vertex
     subsample of each.  The cable has no contacts, so without a knit the model
     never sees a contact state; a smaller knit does not do, because the
     dominant part of the correction is the patch-wide sag/tension mode, which
-    depends on the span.
+    depends on the span.  The transient is left out because its corrections
+    are far larger than the steady state's: in a squared loss they crowd out
+    the small steady-state corrections, and the model then starts the steady
+    state worse than the plain warm start (1.01x plain over frames 1-2 with the
+    transient recorded, 1.3x without it).
 The evaluation patch (256 x 4096, jitter seed 0) is never recorded.
 
     python -m yarnball_amd.train_warmstart --out yarnball_amd/models/warmstart.npy
@@ -32,16 +37,19 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_MODEL = os.path.join(HERE, "models", "warmstart.npy")
 
 
-def record_scene(sc, frames, stride, keep, tol=1e-6, max_sweeps=6000, seed=0, log=print):
+def record_scene(sc, frames, stride, keep, tol=1e-6, max_sweeps=6000, seed=0, log=print,
+                 first_frame=0):
     """Run `frames` frames of a scene with the plain solver, substep by substep,
-    recording every `stride`-th substep; keep a seeded subsample of `keep`
-    vertices of each recorded substep (all of them when keep >= n)."""
+    recording every `stride`-th substep from frame `first_frame` on; keep a
+    seeded subsample of `keep` vertices of each recorded substep (all of them
+    when keep >= n)."""
     n = sc.n
     frame = np.float32(sc.frame_dt)
     sub = int(np.ceil(frame / np.float32(sc.maxH)))
     h = np.float32(frame / np.float32(sub))
     total = frames * sub
-    nrec = sum(1 for k in range(total) if k % stride == stride - 1)
+    rec = lambda k: k >= first_frame * sub and k % stride == stride - 1
+    nrec = sum(1 for k in range(total) if rec(k))
     sim = Sim(n)
     scenes.load_into(sim, sc)
     sim.setSolver(tol, maxSweeps=max_sweeps, chunk=16, checkEvery=4)
@@ -53,7 +61,7 @@ def record_scene(sc, frames, stride, keep, tol=1e-6, max_sweeps=6000, seed=0, lo
             scenes.drive_frame(sc, sim.verts["pos"], sim.vels, t, float(frame))
             sim.upload()
         for k in range(sub):
-            learned.record_enable(sim, (f * sub + k) % stride == stride - 1)
+            learned.record_enable(sim, rec(f * sub + k))
             sim.advance(h)
         t += float(frame)
     learned.record_enable(sim, False)
@@ -74,8 +82,8 @@ def generate(tol=1e-6, max_sweeps=6000, log=print):
         _readme(tol, max_sweeps, log),
         record_scene(scenes.twisted_cable(64, 2048), 8, 4, 8192, tol, max_sweeps, seed=1,
                      log=log),
-        record_scene(scenes.knit_patch(256, 4096, seed=7), 4, 2, 24576, tol, max_sweeps, seed=2,
-                     log=log),
+        record_scene(scenes.knit_patch(256, 4096, seed=7), 5, 1, 24576, tol, max_sweeps, seed=2,
+                     log=log, first_frame=1),
     ]
     X = np.concatenate([p[0] for p in parts])
     C = np.concatenate([p[1] for p in parts])
@@ -89,7 +97,7 @@ def _readme(tol, max_sweeps, log, seconds=0.5):
     return record_scene(sc, 1, 1, sc.n, tol, max_sweeps, log=log)
 
 
-def fit(X, C, epochs=40, batch=16384, lr=1e-3, seed=0, log=print):
+def fit(X, C, epochs=400, batch=16384, lr=1e-3, seed=0, log=print):
     """Train on recorded (features X, converged corrections C)."""
     T = learned.residual_targets(X, C)
     p0 = learned.init_params(X, T, seed=seed)
@@ -104,7 +112,7 @@ def fit(X, C, epochs=40, batch=16384, lr=1e-3, seed=0, log=print):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=DEFAULT_MODEL)
-    ap.add_argument("--epochs", type=int, default=40)
+    ap.add_argument("--epochs", type=int, default=400)
     ap.add_argument("--tol", type=float, default=1e-6)
     args = ap.parse_args()
     log = lambda m: print(m, file=sys.stderr, flush=True)
