@@ -166,7 +166,7 @@ extern "C" int yb_create(int n, yb_sim** out) {
     ALLOC(b.predC, N);
     ALLOC(b.mlpCount, 2 * ((N + 63) / 64));
     ALLOC(b.recCount, 2);
-    ALLOC(b.partial, (size_t)sweep_waves(n));
+    ALLOC(b.partial, 2 * (size_t)sweep_waves(n));
 #undef ALLOC
     if (e == hipSuccess) e = hipHostMalloc((void**)&b.hostFlag, 2 * sizeof(int32_t));
     if (e == hipSuccess) e = hipEventCreateWithFlags(&s->chunkEv[0], hipEventDisableTiming);
@@ -565,14 +565,18 @@ static bool checks_residual(const yb_sim* s, int k) {
 }
 
 // last: the final sweep of a fixed schedule (ends the run of sweeps in a profile)
+// The residual of a check sweep is reduced, and convergence decided, by the
+// next sweep's first wave; only the last sweep a substep may run is checked by
+// a launch of its own.
 static void enqueue_sweep(yb_sim* s, int k, bool adaptive, bool last = false) {
     const bool check = adaptive && checks_residual(s, k);
-    launch_sweep(s, k, adaptive, check);
+    const bool checkPrev = adaptive && k > 0 && checks_residual(s, k - 1);
+    launch_sweep(s, k, adaptive, check, checkPrev);
     if (check || last)
         mark(s, YB_STAGE_SWEEP);
     else
         hold(s, YB_STAGE_SWEEP);
-    if (check) {
+    if (check && k == s->maxSweeps - 1) {
         launch_residual(s, k);
         mark(s, YB_STAGE_RESIDUAL);
     }

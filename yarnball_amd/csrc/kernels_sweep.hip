@@ -121,6 +121,9 @@ __global__ __launch_bounds__(256) void k_step_limit(const DevParams* __restrict_
 }
 
 __device__ __forceinline__ void wave_residual(float num, double* partial, int wave, int n);
+__device__ __forceinline__ void residual_wave(SolverCtl* __restrict__ ctl,
+                                              const double* __restrict__ partial, int nb,
+                                              int sweep, float tol, int maxSweeps);
 
 // Closest points of segment e = (0 -> e) and (B0 -> B0 + A1), specialised from
 // closest_uv (Common.h:318-334) with the segment's |e|^2 and 1/|e|^2 hoisted
@@ -170,11 +173,20 @@ __global__ __launch_bounds__(512) void k_sweep(const DevParams* __restrict__ P,
                                                float4* __restrict__ dxOut,
                                                const int32_t* __restrict__ numCols,
                                                const int32_t* __restrict__ cols,
-                                               const SolverCtl* __restrict__ ctl, int sweep,
-                                               double* __restrict__ partial) {
+                                               SolverCtl* __restrict__ ctl, int sweep,
+                                               double* __restrict__ partial,
+                                               const double* __restrict__ partialPrev,
+                                               float tol, int maxSweeps) {
     // Residual-targeted mode: sweeps after convergence do nothing.
     if (ctl && ctl->doneAt < sweep) return;
     const int n = P->n;
+    // The previous sweep was a check sweep: its residual and the stopping
+    // decision are made here, by the first wave of the launch, while the rest
+    // of this sweep runs.  A sweep that converged is thus recognised one sweep
+    // later (this sweep's result is not used; the next one exits at once), and
+    // no separate reduction launch sits between two sweeps.
+    if (partialPrev && blockIdx.x == 0 && threadIdx.x < 64)
+        residual_wave(ctl, partialPrev, (n + 61) / 62, sweep - 1, tol, maxSweeps);
     const int lane = threadIdx.x & 63;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int wave = (int)((tile * blockDim.x + threadIdx.x) >> 6);
@@ -390,34 +402,27 @@ __device__ __forceinline__ void wave_residual(float num, double* partial, int wa
     if ((threadIdx.x & 63) == 0 && wave < (n + 61) / 62) partial[wave] = (double)num;
 }
 
-__global__ __launch_bounds__(1024) void k_residual(SolverCtl* __restrict__ ctl,
-                                                   const double* __restrict__ partial, int nb,
-                                                   int sweep, float tol, int maxSweeps) {
-    if (ctl->doneAt < sweep) return;
-    // Four independent running sums per thread keep four loads in flight; the
-    // block then reduces by wave shuffles and one wave over the 16 wave sums.
-    // The order of additions is fixed, so the residual is reproducible.
-    const int t = threadIdx.x;
+// Residual of sweep `sweep` from its per-wave partials, and the stopping
+// decision, by one wave (64 lanes) in a fixed order of additions, so the value
+// does not depend on how the work is launched.  Four running sums per lane
+// keep four loads in flight.
+__device__ __forceinline__ void residual_wave(SolverCtl* __restrict__ ctl,
+                                              const double* __restrict__ partial, int nb,
+                                              int sweep, float tol, int maxSweeps) {
+    const int lane = threadIdx.x & 63;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-    int k = t;
-    for (; k + 3 * 1024 < nb; k += 4 * 1024) {
+    int k = lane;
+    for (; k + 192 < nb; k += 256) {
         a0 += partial[k];
-        a1 += partial[k + 1024];
-        a2 += partial[k + 2048];
-        a3 += partial[k + 3072];
+        a1 += partial[k + 64];
+        a2 += partial[k + 128];
+        a3 += partial[k + 192];
     }
-    for (; k < nb; k += 1024) a0 += partial[k];
+    for (; k < nb; k += 64) a0 += partial[k];
     double a = (a0 + a1) + (a2 + a3);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
-    __shared__ double sw[16];
-    if ((t & 63) == 0) sw[t >> 6] = a;
-    __syncthreads();
-    if (t >= 64) return;
-    a = t < 16 ? sw[t] : 0.0;
-#pragma unroll
-    for (int off = 8; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
-    if (t == 0) {
+    if (lane == 0) {
         const double den = ctl->denom;
         const float r = den > 0.0 ? (float)sqrt(a / den) : 0.0f;
         ctl->resid = r;
@@ -431,6 +436,15 @@ __global__ __launch_bounds__(1024) void k_residual(SolverCtl* __restrict__ ctl,
             if (r > ctl->maxResid) ctl->maxResid = r;
         }
     }
+}
+
+// Stand-alone check, used only for the last sweep a substep may run (every
+// other check is made by the following sweep's first wave, in k_sweep).
+__global__ __launch_bounds__(64) void k_residual(SolverCtl* __restrict__ ctl,
+                                                 const double* __restrict__ partial, int nb,
+                                                 int sweep, float tol, int maxSweeps) {
+    if (ctl->doneAt < sweep) return;
+    residual_wave(ctl, partial, nb, sweep, tol, maxSweeps);
 }
 
 __global__ void k_ctl_reset(SolverCtl* __restrict__ ctl) {
@@ -670,20 +684,27 @@ void launch_separation(yb_sim* s) {
 // wave tiles of 62 vertices (a 64-lane wave with a one-vertex halo each side)
 int sweep_waves(int n) { return blocks(n, 62); }
 
-void launch_sweep(yb_sim* s, int sweep, bool adaptive, bool partials) {
+void launch_sweep(yb_sim* s, int sweep, bool adaptive, bool partials, bool checkPrev) {
     const int in = sweep & 1;
     const int out = in ^ 1;
     const int threads = s->sweepThreads;
+    const size_t nbw = (size_t)sweep_waves(s->n);
+    // partials of sweep k go to half k & 1, so a sweep can reduce its
+    // predecessor's while writing its own
+    double* part = s->b.partial + nbw * (sweep & 1);
+    const double* prev = s->b.partial + nbw * ((sweep & 1) ^ 1);
     hipLaunchKernelGGL(k_sweep, dim3(blocks(sweep_waves(s->n) * 64, threads)), dim3(threads), 0,
                        s->stream, s->b.params,
                        s->b.pos, s->b.prm, s->b.yv, s->b.q[in], s->b.q[out], s->b.qRest,
                        s->b.dx[in], s->b.dx[out], s->b.numCols, s->b.cols,
-                       adaptive ? s->b.ctl : nullptr, sweep, partials ? s->b.partial : nullptr);
+                       adaptive ? s->b.ctl : nullptr, sweep, partials ? part : nullptr,
+                       checkPrev ? prev : nullptr, s->tol, s->maxSweeps);
 }
 
 void launch_residual(yb_sim* s, int sweep) {
-    hipLaunchKernelGGL(k_residual, dim3(1), dim3(1024), 0, s->stream, s->b.ctl, s->b.partial,
-                       sweep_waves(s->n), sweep, s->tol, s->maxSweeps);
+    const size_t nbw = (size_t)sweep_waves(s->n);
+    hipLaunchKernelGGL(k_residual, dim3(1), dim3(64), 0, s->stream, s->b.ctl,
+                       s->b.partial + nbw * (sweep & 1), (int)nbw, sweep, s->tol, s->maxSweeps);
 }
 
 void launch_frame_refresh(yb_sim* s) {

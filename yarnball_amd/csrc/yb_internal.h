@@ -152,7 +152,7 @@ struct Buffers {
     float* minSep = nullptr;      // smallest contact distance / (2 radius) seen
     DevParams* params = nullptr;
     SolverCtl* ctl = nullptr;
-    double* partial = nullptr;    // per-block residual partial sums
+    double* partial = nullptr;    // residual partial sums per sweep wave, two halves
     int32_t* hostFlag = nullptr;  // pinned: doneAt read back after each chunk
     // learned warm start
     float* mlpPrm = nullptr;          // kMlpParams floats
@@ -242,7 +242,9 @@ void launch_step_limit(yb_sim* s);              // collision.cu:121-172
 void launch_separation(yb_sim* s);              // contact distances of the current state
 // Sweep k reads dx[k&1], q[k&1] and writes the other buffer.  adaptive: the
 // kernel honours SolverCtl (early exit after convergence, residual partials).
-void launch_sweep(yb_sim* s, int sweep, bool adaptive, bool partials);  // cosserat.cu:20-253
+// cosserat.cu:20-253; partials: write this sweep's residual partials;
+// checkPrev: also decide convergence of the previous sweep (a check sweep)
+void launch_sweep(yb_sim* s, int sweep, bool adaptive, bool partials, bool checkPrev);
 void launch_residual(yb_sim* s, int sweep);
 void launch_scene_end_flags(yb_sim* s);  // flags of each scene's last vertex -> sceneEndFlags
 void launch_ctl_reset(yb_sim* s);
