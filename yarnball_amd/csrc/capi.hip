@@ -477,7 +477,8 @@ struct ProfState {
     std::vector<int> tag;    // first sweep index of a run of sweeps / residuals, else -1
     int used = 0;
     int curTag = -1;
-    int doneAt = 0x7fffffff;  // runs tagged after the converged sweep are not harvested
+    int doneAt = 0x7fffffff;  // runs reaching past the converged sweep are not harvested
+    int runLen = 1;           // sweeps per tagged run
     int held = 0, heldStage = -1;
 };
 
@@ -568,15 +569,17 @@ static bool checks_residual(const yb_sim* s, int k) {
 // The residual of a check sweep is reduced, and convergence decided, by the
 // next sweep's first wave; only the last sweep a substep may run is checked by
 // a launch of its own.
+// last: the final sweep of a run that yb_profile times as one interval.
 static void enqueue_sweep(yb_sim* s, int k, bool adaptive, bool last = false) {
     const bool check = adaptive && checks_residual(s, k);
     const bool checkPrev = adaptive && k > 0 && checks_residual(s, k - 1);
     launch_sweep(s, k, adaptive, check, checkPrev);
-    if (check || last)
+    const bool standalone = check && k == s->maxSweeps - 1;
+    if (last || standalone)
         mark(s, YB_STAGE_SWEEP);
     else
         hold(s, YB_STAGE_SWEEP);
-    if (check && k == s->maxSweeps - 1) {
+    if (standalone) {
         launch_residual(s, k);
         mark(s, YB_STAGE_RESIDUAL);
     }
@@ -1088,7 +1091,8 @@ extern "C" int yb_profile(yb_sim* s, yb_params* p, float h, int substeps, double
         for (int k = 1; k < ps.used; k++) {
             float t = 0.f;
             if (ps.stage[k] < 0) continue;  // gap marker: host latency, not a kernel
-            if (ps.tag[k] > ps.doneAt) continue;  // speculative launches after convergence
+            // runs that reach past the converged sweep hold early-exit launches
+            if (ps.tag[k] >= 0 && ps.tag[k] + ps.runLen - 1 > ps.doneAt) continue;
             HIPCHECK(hipEventElapsedTime(&t, ps.ev[k - 1], ps.ev[k]));
             ms[ps.stage[k]] += t;
             launches[ps.stage[k]] += ps.count[k];
@@ -1103,16 +1107,18 @@ extern "C" int yb_profile(yb_sim* s, yb_params* p, float h, int substeps, double
         int fin = p->numItr & 1;
         ps.doneAt = 0x7fffffff;
         if (adaptive) {
-            // As in the stepper, the next group of sweeps is queued before the
-            // host waits for the current group's residual, so the device does
-            // not idle between groups; the groups queued after convergence
-            // (early-exit launches) are left out of the harvest.
-            const int ce = s->checkEvery;
+            // As in the stepper, sweeps run in chunks of s->chunk and the next
+            // chunk is queued before the host waits for the current one's
+            // flag, so the device does not idle between chunks.  One event
+            // per chunk; the chunks that reach past the converged sweep (they
+            // hold early-exit launches) are left out of the harvest.
+            const int ce = s->chunk;
+            ps.runLen = ce;
             const int groups = (s->maxSweeps + ce - 1) / ce;
             auto enqueue_group = [&](int g) -> int {
                 ps.curTag = g * ce;
                 const int k1 = std::min((g + 1) * ce, s->maxSweeps);
-                for (int k = g * ce; k < k1; k++) enqueue_sweep(s, k, true);
+                for (int k = g * ce; k < k1; k++) enqueue_sweep(s, k, true, k == k1 - 1);
                 ps.curTag = -1;
                 HIPCHECK(hipMemcpyAsync(&s->b.hostFlag[g & 1], &s->b.ctl->doneAt, sizeof(int32_t),
                                         hipMemcpyDeviceToHost, s->stream));
