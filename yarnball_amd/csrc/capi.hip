@@ -48,7 +48,7 @@ static void free_all(yb_sim* s) {
                     b.vals[0], b.vals[1], b.hist, b.scanPart, b.nodes,   b.parent,  b.rng,
                     b.kids,    b.strad,   b.stradCount, b.leafNeed, b.sufBox, b.preBox, b.blkTab,
                     b.err,     b.params,  b.stage, b.ctl, b.partial, b.minSep,
-                    b.mlpPrm,  b.mlpTiles, b.mlpBias, b.featB, b.corrPrev, b.dxPlain, b.predC,
+                    b.mlpPrm,  b.mlpTiles, b.mlpBias, b.corrPrev, b.dxPlain, b.predC,
                     b.mlpCount, b.recX, b.recT, b.recCount, b.sceneOf, b.sceneOff,
                     b.sceneBoxes, b.sceneRoot, b.sceneEndFlags, b.drvInit};
     for (void* p : ptrs)
@@ -158,9 +158,8 @@ extern "C" int yb_create(int n, yb_sim** out) {
     ALLOC(b.ctl, 1);
     ALLOC(b.minSep, 1);
     ALLOC(b.mlpPrm, kMlpParams);
-    ALLOC(b.mlpTiles, 168 * 256);
+    ALLOC(b.mlpTiles, kMlpTileBufElems);
     ALLOC(b.mlpBias, 2 * kMlpHid + 16);
-    ALLOC(b.featB, N * kMlpIn);
     ALLOC(b.corrPrev, N);
     ALLOC(b.dxPlain, N);
     ALLOC(b.predC, N);
@@ -542,7 +541,9 @@ static void enqueue_prologue(yb_sim* s, int mode, bool adaptive, int warm) {
     mark(s, YB_STAGE_STEP_LIMIT);
     const bool learned = warm == 1, record = s->recCap > 0;
     if (learned || record || s->warmStart == 1) {
-        launch_features(s, learned, record);
+        // the inference kernel computes its own features; the recording
+        // kernel runs for training data and for the plain warm start alone
+        if (record || !learned) launch_features(s, record);
         if (learned) launch_mlp_apply(s);
         mark(s, YB_STAGE_MLP);
     } else if (warm == 2) {

@@ -90,6 +90,9 @@ constexpr int kMlpIStd = kMlpMean + kMlpIn;
 constexpr int kMlpOScale = kMlpIStd + kMlpIn;
 constexpr int kMlpParams = kMlpOScale + kMlpOut;
 static_assert(kMlpParams == YB_MLP_PARAMS, "parameter layout");
+// bf16 weight tiles: 168 training tiles of 256 elements (K = 16 operands),
+// then 52 inference tiles of 512 elements (K = 32 operands).
+constexpr int kMlpTileBufElems = 168 * 256 + 52 * 512;
 constexpr int kSortBits = 8;
 constexpr int kSortBins = 1 << kSortBits;
 constexpr int kSortThreads = 256;
@@ -158,7 +161,6 @@ struct Buffers {
     float* mlpPrm = nullptr;          // kMlpParams floats
     uint16_t* mlpTiles = nullptr;     // packed bf16 weight tiles
     float* mlpBias = nullptr;         // 2*hidden + 16
-    uint16_t* featB = nullptr;        // n x 48 normalised bf16 features
     float4* corrPrev = nullptr;       // n: last converged correction (dx* - dx0)/L
     float4* dxPlain = nullptr;        // n: plain warm start of the current substep
     float4* predC = nullptr;          // n: predicted correction before smoothing
@@ -213,6 +215,7 @@ struct yb_sim {
     int warmStart = 0;     // 0 plain (iteration.cu:18-34), 1 learned, 2 temporal
     int frameRefresh = 1;  // refresh frames after a non-plain warm start
     int mlpSmooth = 1;     // [1 2 1]/4 passes on the predicted correction (0 or 1)
+    int mlpGrid = 0;       // resident blocks of the inference kernel (set by yb_mlp_load)
     // Substep-level fallback of a learned session in residual-targeted mode:
     // the chooser runs the warm start whose sweep-count average is lower and
     // probes the other one from time to time.
@@ -253,7 +256,7 @@ void launch_end(yb_sim* s, int fin, bool adaptive);         // iteration.cu:48-6
 int sweep_waves(int n);
 // learned warm start (mlp.hip)
 void launch_mlp_pack(yb_sim* s);
-void launch_features(yb_sim* s, bool infer, bool record);
+void launch_features(yb_sim* s, bool record);
 void launch_mlp_apply(yb_sim* s);
 void launch_temporal(yb_sim* s);
 void launch_correction(yb_sim* s, int fin, bool adaptive, bool record);
