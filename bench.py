@@ -7,11 +7,14 @@ opposite edge pulled at 5 cm/s, gravity -z), 1/30 s frames, maxH = 1e-3
 (34 substeps per frame), detection every substep, numItr sweeps per substep.
 
 A step is one frame: Sim.advance(1/30) over the whole patch, state resident
-in HBM, solved every substep to the relative Newton residual 1e-6.  The plain
-warm start and the learned one (the committed model, with the substep-level
-plain fallback) run the same frames from the same initial state; value =
-vertices x substeps / second of the learned run over the timed frames, with
-the plain run's rate and the ratio beside it.
+in HBM, solved every substep to the relative Newton residual 1e-6.  Four
+methods run the same frames from the same initial state: the plain warm
+start, the learned one (the committed model, with the substep-level plain
+fallback), and each of them followed by the strand solve (a tridiagonal
+Newton step along the strands after the warm start and before every 6th
+sweep).  value = vertices x substeps / second over the timed frames of the
+fastest of the non-plain methods; every method's rate, sweeps per substep and
+ratio to the plain one are beside it.
 
 Also reported:
   roofline     the dominant kernel's algorithmic bytes per launch / its average
@@ -284,6 +287,11 @@ def run_method(sc, args, method, model):
         sim.setSolver(args.residual, maxSweeps=args.max_sweeps, chunk=16, checkEvery=4)
     if method == "learned":
         learned.use(sim, model)
+    elif method == "strand":
+        learned.strand(sim, every=args.strand_every)
+    elif method == "learned+strand":
+        learned.use(sim, model, strand=True)
+        learned.strand_schedule(sim, every=args.strand_every)
     if args.no_graphs:
         sim.setGraphs(False)
     if args.launch_config:
@@ -295,7 +303,7 @@ def run_method(sc, args, method, model):
         print(f"[bench] {method}: warmup frame {w} done", file=sys.stderr, flush=True)
     if args.residual > 0:
         sim.solverStats(reset=True)
-    if method == "learned":
+    if method.startswith("learned"):
         learned.warm_substeps(sim, reset=True)
         learned.counters(sim, reset=True)
     sim.minSeparation(reset=True)
@@ -314,7 +322,7 @@ def run_method(sc, args, method, model):
         print(f"[bench] {method}: {r['sweeps_per_substep']:.1f} sweeps/substep, max "
               f"{st['maxSweepsUsed']}, not converged {st['notConverged']}, max final residual "
               f"{st['maxResidual']:.2e}", file=sys.stderr, flush=True)
-    if method == "learned":
+    if method.startswith("learned"):
         r["warm_substeps"] = learned.warm_substeps(sim)
         r["vertex_counters"] = learned.counters(sim)
     # live per-stage timing right after the timed frames, while the GPU runs at
@@ -340,12 +348,17 @@ def main():
     ap.add_argument("--residual", type=float, default=1e-6,
                     help="relative Newton residual target; <= 0 runs numItr sweeps")
     ap.add_argument("--max-sweeps", type=int, default=6000)
-    ap.add_argument("--method", choices=["plain", "learned", "both"], default="both",
-                    help="warm start(s) to run; value is the last one's rate")
+    ap.add_argument("--method", choices=["plain", "learned", "both", "strand", "learned+strand",
+                                         "all"],
+                    default="all",
+                    help="warm start(s) to run; value is the fastest non-plain method's rate")
     ap.add_argument("--no-graphs", action="store_true",
                     help="enqueue kernels directly instead of HIP graphs (profiler runs)")
     ap.add_argument("--model", default=os.path.join(ROOT, "yarnball_amd", "models",
                                                     "warmstart.npy"))
+    ap.add_argument("--strand-every", type=int, default=6,
+                    help="strand methods: also run the strand step before every k-th sweep "
+                         "(0: only after the warm start)")
     ap.add_argument("--scenes", type=int, default=512, help="batched_walks: number of scenes")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default block sizes instead of timing the candidates")
@@ -371,7 +384,9 @@ def main():
     sc = build_workload(args.workload, args.numItr)
     print(f"[bench] workload {sc.name}: {sc.n} vertices, built in "
           f"{time.perf_counter() - t_build:.1f}s", file=sys.stderr, flush=True)
-    methods = ["plain", "learned"] if args.method == "both" else [args.method]
+    methods = {"both": ["plain", "learned"],
+               "all": ["plain", "learned", "strand", "learned+strand"]}.get(args.method,
+                                                                             [args.method])
     tuned = None
     if not args.no_autotune:
         from yarnball_amd import autotune
@@ -379,12 +394,15 @@ def main():
                               log=lambda m: print(m, file=sys.stderr, flush=True))
         args.launch_config = (tuned["sweep_threads"], tuned["query_threads"])
     model = None
-    if "learned" in methods:
+    if any(m.startswith("learned") for m in methods):
         model = np.load(args.model, allow_pickle=False)
     runs = {}
     for m in methods:
         runs[m] = run_method(sc, args, m, model)
-    main_m = methods[-1]
+    # headline: the fastest method at the common residual (the plain method
+    # when it runs alone)
+    main_m = max((m for m in methods if m != "plain"), key=lambda m: runs[m]["value"],
+                 default=methods[-1])
     ref = runs["plain" if "plain" in runs else main_m]
     for m in methods:
         if runs[m] is not ref:
@@ -458,15 +476,24 @@ def main():
             row["sweeps_per_substep"] = round(r["sweeps_per_substep"], 1)
             row["not_converged_substeps"] = int(r["stats"]["notConverged"])
             row["max_final_residual"] = float(r["stats"]["maxResidual"])
-        if m == "learned":
+        if m.startswith("learned"):
             ws = r["warm_substeps"]
             row["substeps_learned_start"] = ws["learned"]
             row["substeps_plain_fallback"] = ws["plain"]
             row["fallback_rate"] = round(ws["plain"] / max(ws["learned"] + ws["plain"], 1), 3)
             row["vertex_fallbacks"] = r["vertex_counters"]["fallback"]
         method_rows[m] = row
-    speedup = (round(runs["learned"]["value"] / runs["plain"]["value"], 3)
-               if "learned" in runs and "plain" in runs else None)
+    def vs_plain(m):
+        return (round(runs[m]["value"] / runs["plain"]["value"], 3)
+                if m in runs and "plain" in runs else None)
+    method_names = {
+        "plain": "plain warm start",
+        "learned": "learned warm start with substep-level plain fallback",
+        "strand": f"plain warm start + strand solve (after the warm start and before every "
+                  f"{args.strand_every}th sweep)",
+        "learned+strand": f"learned warm start with substep-level plain fallback + strand solve "
+                          f"(after the warm start and before every {args.strand_every}th sweep)",
+    }
 
     out = {
         "metric": "vertex-substeps/s (plain vs learned warm-start, equal residual) + HBM GB/s",
@@ -490,13 +517,14 @@ def main():
             "mean_sweeps_per_substep": (round(stats["sweeps"] / max(stats["substeps"], 1), 1)
                                         if stats else int(sim.meta.numItr)),
             "not_converged_substeps": stats["notConverged"] if stats else None,
-            "method": ("learned warm start with substep-level plain fallback"
-                       if main_m == "learned" else "plain warm start"),
+            "method": method_names[main_m],
             "detection_period": int(sim.meta.detectionPeriod),
             "launch": "eager" if args.no_graphs else "hip graphs",
         },
         "methods": method_rows,
-        "learned_vs_plain": speedup,
+        "learned_vs_plain": vs_plain("learned"),
+        "strand_vs_plain": vs_plain("strand"),
+        "learned_strand_vs_plain": vs_plain("learned+strand"),
         "launch_config": {"sweep_threads": sim.launchConfig()[0],
                           "query_threads": sim.launchConfig()[1],
                           "autotune": tuned},

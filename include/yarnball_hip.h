@@ -251,6 +251,11 @@ int yb_debug_tree(yb_sim* s, int32_t* child, int32_t* parent, float* box);
 /* Plain warm start of the last prologue (4n floats) and the first n recorded
  * feature rows (48n floats) -- for the learned-warm-start tests. */
 int yb_debug_read_mlp(yb_sim* s, float* dxPlain, float* features);
+/* One strand step (see YB_WARM_STRAND_SOLVE) on the current iterate dx, with
+ * the handle's pair-iteration count and no frame refresh; system (5n floats,
+ * may be null): each vertex's row -- diagonal, coupling to the next vertex,
+ * right-hand side xyz -- for checking the solve. */
+int yb_debug_strand(yb_sim* s, float* system);
 /* Overwrite the device warm-start/iterate dx (4n floats) and lastVels (3n). */
 int yb_debug_write_dx(yb_sim* s, const float* dx);
 int yb_debug_read_last_vels(yb_sim* s, float* lastVels);
@@ -341,7 +346,32 @@ int yb_mlp_load(yb_sim* s, const float* params, int count);
  * other one to keep its average current.  The plain solver still decides
  * convergence either way.  YB_WARM_NO_FALLBACK always uses the learned start. */
 #define YB_WARM_NO_FALLBACK 0x400
+/* Strand solve: after the warm start (any mode), one Newton step of the
+ * linearised system restricted to the strands -- the isotropic part of each
+ * vertex's local Hessian on the diagonal, the stretch stiffness of each
+ * segment between its free ends -- solved along whole strands: exactly
+ * inside each 62-vertex sweep tile (cyclic reduction over the wave's lanes),
+ * then by iterations over the vertex pairs that face each other across tile
+ * boundaries (two by default, see yb_set_strand_schedule).  The correction is
+ * clamped to the step limit like a sweep step, and the frames are refreshed
+ * unless YB_WARM_NO_FRAME_REFRESH is set.  Block-Jacobi sweeps propagate a
+ * correction one vertex per sweep and damp smooth modes along stiff strands
+ * slowly; this step removes them.  It changes only the iterate the sweeps
+ * start from: they still run to the residual tolerance.  No counterpart in
+ * the reference. */
+#define YB_WARM_STRAND_SOLVE 0x800
 int yb_set_warm_start(yb_sim* s, int mode);
+/* Strand-solve schedule (with YB_WARM_STRAND_SOLVE): in residual-targeted
+ * mode, also run the strand step before every `every`-th sweep (0: only after
+ * the warm start; a multiple of 4 keeps the residual checks where they are),
+ * on the current iterate, with no frame refresh (the next sweep updates the
+ * frames); pairIters boundary-pair iterations per step (default 2).  The
+ * residual of the sweep before such a step is reduced before it, and a
+ * converged iterate is never changed.  At most 16 such steps per substep:
+ * one that has not converged by then finishes with plain sweeps.  Vertices
+ * held at their step limit take no part in a strand step (they stay where the
+ * sweeps put them).  No counterpart in the reference. */
+int yb_set_strand_schedule(yb_sim* s, int every, int pairIters);
 /* Vertex level: out[0] predictions applied, out[1] vertices that kept the
  * plain warm start because the prediction was not finite. */
 int yb_mlp_counters(yb_sim* s, int64_t out[2], int reset);

@@ -128,6 +128,7 @@ SIGNATURES = {
     "yb_debug_read": (_I, [_P, _P, _P, _P, _P, _P, _P]),
     "yb_debug_tree": (_I, [_P, _P, _P, _P]),
     "yb_debug_write_dx": (_I, [_P, _P]),
+    "yb_debug_strand": (_I, [_P, _P]),
     "yb_debug_read_last_vels": (_I, [_P, _P]),
     "yb_debug_write_last_vels": (_I, [_P, _P]),
     "yb_debug_read_mlp": (_I, [_P, _P, _P]),
@@ -136,6 +137,7 @@ SIGNATURES = {
     "yb_min_separation": (_I, [_P, _P, _I]),
     "yb_mlp_load": (_I, [_P, _P, _I]),
     "yb_set_warm_start": (_I, [_P, _I]),
+    "yb_set_strand_schedule": (_I, [_P, _I, _I]),
     "yb_mlp_counters": (_I, [_P, _P, _I]),
     "yb_warm_start_substeps": (_I, [_P, _P, _I]),
     "yb_set_graphs": (_I, [_P, _I]),

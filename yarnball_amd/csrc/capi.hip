@@ -49,6 +49,7 @@ static void free_all(yb_sim* s) {
                     b.kids,    b.strad,   b.stradCount, b.leafNeed, b.sufBox, b.preBox, b.blkTab,
                     b.err,     b.params,  b.stage, b.ctl, b.partial, b.minSep,
                     b.mlpPrm,  b.mlpTiles, b.mlpBias, b.corrPrev, b.dxPlain, b.predC,
+                    b.lineG,   b.lineQL,  b.lineE[0], b.lineE[1], b.lineF[0], b.lineF[1],
                     b.mlpCount, b.recX, b.recT, b.recCount, b.sceneOf, b.sceneOff,
                     b.sceneBoxes, b.sceneRoot, b.sceneEndFlags, b.drvInit};
     for (void* p : ptrs)
@@ -166,6 +167,12 @@ extern "C" int yb_create(int n, yb_sim** out) {
     ALLOC(b.mlpCount, 2 * ((N + 63) / 64));
     ALLOC(b.recCount, 2);
     ALLOC(b.partial, 2 * (size_t)sweep_waves(n));
+    ALLOC(b.lineG, N);
+    ALLOC(b.lineQL, N);
+    for (int k = 0; k < 2; k++) {
+        ALLOC(b.lineE[k], (size_t)sweep_waves(n));
+        ALLOC(b.lineF[k], (size_t)sweep_waves(n));
+    }
 #undef ALLOC
     if (e == hipSuccess) e = hipHostMalloc((void**)&b.hostFlag, 2 * sizeof(int32_t));
     if (e == hipSuccess) e = hipEventCreateWithFlags(&s->chunkEv[0], hipEventDisableTiming);
@@ -495,6 +502,13 @@ static void hold(yb_sim* s, int stage) {
     ps->held++;
 }
 
+static void mark(yb_sim* s, int stage);
+// Closes a run of held launches with an event of their own stage.
+static void flush_held(yb_sim* s) {
+    ProfState* ps = (ProfState*)s->prof;
+    if (ps && ps->held) mark(s, ps->heldStage);
+}
+
 static void mark(yb_sim* s, int stage) {
     ProfState* ps = (ProfState*)s->prof;
     if (!ps) return;
@@ -550,7 +564,11 @@ static void enqueue_prologue(yb_sim* s, int mode, bool adaptive, int warm) {
         launch_temporal(s);
         mark(s, YB_STAGE_MLP);
     }
-    if (warm != 0 && s->frameRefresh) {
+    if (s->strandSolve) {
+        launch_strand_solve(s);
+        mark(s, YB_STAGE_MLP);
+    }
+    if ((warm != 0 || s->strandSolve) && s->frameRefresh) {
         launch_frame_refresh(s);
         mark(s, YB_STAGE_MLP);
     }
@@ -571,9 +589,30 @@ static bool checks_residual(const yb_sim* s, int k) {
 // next sweep's first wave; only the last sweep a substep may run is checked by
 // a launch of its own.
 // last: the final sweep of a run that yb_profile times as one interval.
+// A strand step between sweeps k - 1 and k (residual mode, every
+// s->strandEvery sweeps) works on sweep k's input buffers.  The decision about
+// sweep k - 1 is made before it, by a launch of its own, and every kernel of
+// the step does nothing once the solve has converged: a converged iterate is
+// never changed.
+// At most kStrandSteps such steps per substep: a substep that has not
+// converged by then finishes with plain sweeps.
+constexpr int kStrandSteps = 16;
 static void enqueue_sweep(yb_sim* s, int k, bool adaptive, bool last = false) {
     const bool check = adaptive && checks_residual(s, k);
-    const bool checkPrev = adaptive && k > 0 && checks_residual(s, k - 1);
+    bool checkPrev = adaptive && k > 0 && checks_residual(s, k - 1);
+    if (adaptive && s->strandEvery > 0 && k > 0 && k % s->strandEvery == 0 &&
+        k / s->strandEvery <= kStrandSteps) {
+        flush_held(s);
+        if (checkPrev) {
+            launch_residual(s, k - 1);
+            mark(s, YB_STAGE_RESIDUAL);
+            checkPrev = false;
+        }
+        // no frame refresh: the sweep that follows updates the frames from the
+        // new dx (a refresh here changed no sweep count, only the time)
+        launch_strand_solve(s, k & 1, s->b.ctl, k);
+        mark(s, YB_STAGE_MLP);
+    }
     launch_sweep(s, k, adaptive, check, checkPrev);
     const bool standalone = check && k == s->maxSweeps - 1;
     if (last || standalone)
@@ -684,6 +723,20 @@ extern "C" int yb_set_solver(yb_sim* s, float residualTol, int maxSweeps, int ch
     s->maxSweeps = maxSweeps;
     s->chunk = chunk;
     s->checkEvery = checkEvery;
+    return YB_OK;
+}
+
+extern "C" int yb_set_strand_schedule(yb_sim* s, int every, int pairIters) {
+    if (!s || every < 0 || pairIters < 1 || pairIters > 64) {
+        set_message("yb_set_strand_schedule: every >= 0, 1 <= pairIters <= 64");
+        return YB_ERR_INVALID_ARGUMENT;
+    }
+    if (every != s->strandEvery || pairIters != s->strandIters) {
+        s->strandEvery = every;
+        s->strandIters = pairIters;
+        s->graphItr = -1;
+        s->adaptiveReady = false;
+    }
     return YB_OK;
 }
 
@@ -1042,6 +1095,21 @@ extern "C" int yb_debug_read_last_vels(yb_sim* s, float* lastVels) {
         lastVels[3 * i + 2] = tmp[i].z;
     }
     delete[] tmp;
+    HIPCHECK(e);
+    return YB_OK;
+}
+
+extern "C" int yb_debug_strand(yb_sim* s, float* system) {
+    if (!s) return YB_ERR_INVALID_ARGUMENT;
+    float* d = nullptr;
+    if (system) HIPCHECK(hipMalloc(&d, sizeof(float) * 5 * (size_t)s->n));
+    launch_strand_solve(s, 0, nullptr, 0, d);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && system)
+        e = hipMemcpyAsync(system, d, sizeof(float) * 5 * (size_t)s->n, hipMemcpyDeviceToHost,
+                           s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (d) (void)hipFree(d);
     HIPCHECK(e);
     return YB_OK;
 }

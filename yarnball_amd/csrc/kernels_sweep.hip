@@ -162,7 +162,49 @@ __device__ __forceinline__ void closest_e(v3 e, float a, float inva, v3 B0, v3 A
 // tile's first vertex without writing it, so that lane 62's frame update has
 // dx'(i+1); it also supplies q(i+1).  All exchange is cross-lane shifts --
 // no LDS, no barrier (the reference uses a 2-sector LDS reduction).
-__global__ __launch_bounds__(512) void k_sweep(const DevParams* __restrict__ P,
+// A strand step between sweeps does nothing once the substep has converged
+// (before `sweep`).  Every kernel of one step reads the same control word, so
+// they agree.
+__device__ __forceinline__ bool strand_skip(const SolverCtl* ctl, int sweep) {
+    return ctl && ctl->doneAt < sweep;
+}
+
+// Outputs of the strand solve's tile kernel (line mode of the sweep body).
+struct LineOut {
+    float4* g;   // per vertex: particular solution xyz, left-boundary factor p
+    float2* ql;  // per vertex: right-boundary factor q, step limit (-1: not a row)
+    float* sys;  // optional (tests): per vertex diagonal, coupling to the next, rhs xyz
+};
+
+// Parallel cyclic reduction over the 64 rows of a wave, one row per lane:
+//   A x_l - Lo x_(l-1) - Up x_(l+1) = B_l      (five right-hand sides)
+// Rows outside the wave count as identity rows.  After strides 1..32 every
+// row is decoupled and x_l = B_l / A.  Each step substitutes the equations of
+// rows l - s and l + s into row l; the system is diagonally dominant, so no
+// pivoting is needed.
+__device__ __forceinline__ void pcr64(float& A, float& Lo, float& Up, float B[5]) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int s = 1; s < 64; s <<= 1) {
+        const bool hl = lane >= s, hr = lane + s < 64;
+        const float Am = __shfl_up(A, s, 64), Lm = __shfl_up(Lo, s, 64), Um = __shfl_up(Up, s, 64);
+        const float Ap = __shfl_down(A, s, 64), Lp = __shfl_down(Lo, s, 64),
+                    Upp = __shfl_down(Up, s, 64);
+        const float al = hl ? Lo / Am : 0.0f;
+        const float be = hr ? Up / Ap : 0.0f;
+        A = A - al * (hl ? Um : 0.0f) - be * (hr ? Lp : 0.0f);
+        Lo = hl ? al * Lm : 0.0f;
+        Up = hr ? be * Upp : 0.0f;
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+            const float bm = __shfl_up(B[k], s, 64), bp = __shfl_down(B[k], s, 64);
+            B[k] += al * bm + be * bp;
+        }
+    }
+}
+
+template <bool kLine>
+__device__ __forceinline__ void sweep_body(const DevParams* __restrict__ P,
                                                const float4* __restrict__ pos,
                                                const float4* __restrict__ prm,
                                                const float4* __restrict__ yv,
@@ -176,7 +218,7 @@ __global__ __launch_bounds__(512) void k_sweep(const DevParams* __restrict__ P,
                                                SolverCtl* __restrict__ ctl, int sweep,
                                                double* __restrict__ partial,
                                                const double* __restrict__ partialPrev,
-                                               float tol, int maxSweeps) {
+                                               float tol, int maxSweeps, LineOut lo) {
     // Residual-targeted mode: sweeps after convergence do nothing.
     if (ctl && ctl->doneAt < sweep) return;
     const int n = P->n;
@@ -342,21 +384,63 @@ __global__ __launch_bounds__(512) void k_sweep(const DevParams* __restrict__ P,
     // Newton step.
     v3 dxNew = xyz(d4);
     float num = 0.0f;  // |dx' - dx|^2 for the residual
+    float limit = segLimit;
     if (in && lane != 0) {
-        float limit = segLimit;
         if (flags & YB_FLAG_HAS_PREV) {
             limit = fminf(limit, pl);
             f += v3{pf0, pf1, pf2};
 #pragma unroll
             for (int k = 0; k < 6; k++) H.a[k] += ph[k];
         }
-        if (p4.w != 0.0f) {
+        if (!kLine && p4.w != 0.0f) {
             // Local 3x3 solve (cosserat.cu:195-205).
             dxNew += P->accelerationRatio * sym_solve_fast(H, f);
             const float l = len(dxNew);
             if (l > limit && l > 0.0f) dxNew *= limit / l;
             num = len2(dxNew - xyz(d4));
         }
+    }
+
+    if constexpr (kLine) {
+        // Strand solve, tile part: the tile's owned vertices (lanes 1..62)
+        // form a tridiagonal system along their strands -- the isotropic part
+        // trace(H)/3 of each local Hessian on the diagonal and the stretch
+        // stiffness k/l^2 of each segment between its two free ends -- with
+        // the vertices just outside the tile (lanes 0 and 63) as unknown
+        // boundary values.  Per vertex: x = g + p x_left + q x_right.
+        // A vertex the step limit holds (a sweep clamped it) keeps a nonzero
+        // force at the sweeps' fixed point; as a row it would push its
+        // neighbours every strand step and the sweeps would push them back.
+        // It is held fixed instead, like a pinned vertex, so the converged
+        // state of the sweeps is also a fixed point of the strand step.
+        const bool free = in && p4.w != 0.0f && len2(dx) < limit * limit * (1.0f - 2e-5f);
+        const bool row = free && lane >= 1 && lane <= 62;
+        const float nextFree = __shfl_down(free ? 1.0f : 0.0f, 1, 64);
+        const float cpl = (free && (flags & YB_FLAG_HAS_NEXT) && nextFree != 0.0f)
+                              ? r4.y * invl * invl
+                              : 0.0f;
+        const float cPrev = __shfl_up(cpl, 1, 64);
+        float A = row ? (H.a[0] + H.a[1] + H.a[2]) * (1.0f / 3.0f) : 1.0f;
+        float Lo = (row && lane >= 2) ? cPrev : 0.0f;
+        float Up = (row && lane <= 61) ? cpl : 0.0f;
+        float B[5] = {row ? f.x : 0.0f, row ? f.y : 0.0f, row ? f.z : 0.0f,
+                      (row && lane == 1) ? cPrev : 0.0f, (row && lane == 62) ? cpl : 0.0f};
+        pcr64(A, Lo, Up, B);
+        if (lo.sys && in && lane >= 1 && lane <= 62) {
+            float* o = lo.sys + 5 * (size_t)v;
+            o[0] = row ? (H.a[0] + H.a[1] + H.a[2]) * (1.0f / 3.0f) : 1.0f;
+            o[1] = cpl;
+            o[2] = row ? f.x : 0.0f;
+            o[3] = row ? f.y : 0.0f;
+            o[4] = row ? f.z : 0.0f;
+        }
+        if (in && lane >= 1 && lane <= 62) {
+            const float ia = 1.0f / A;
+            lo.g[v] = row ? make_float4(B[0] * ia, B[1] * ia, B[2] * ia, B[3] * ia)
+                          : make_float4(0.f, 0.f, 0.f, 0.f);
+            lo.ql[v] = make_float2(row ? B[4] * ia : 0.0f, row ? limit : -1.0f);
+        }
+        return;
     }
 
     // Next vertex's new dx and the next segment's frame.
@@ -388,6 +472,99 @@ __global__ __launch_bounds__(512) void k_sweep(const DevParams* __restrict__ P,
         qOut[v] = f4r(q0);
     }
     if (partial) wave_residual(owner ? num : 0.0f, partial, wave, n);
+}
+
+__global__ __launch_bounds__(512) void k_sweep(const DevParams* __restrict__ P,
+                                               const float4* __restrict__ pos,
+                                               const float4* __restrict__ prm,
+                                               const float4* __restrict__ yv,
+                                               const float4* __restrict__ qIn,
+                                               float4* __restrict__ qOut,
+                                               const float4* __restrict__ qRest,
+                                               const float4* __restrict__ dxIn,
+                                               float4* __restrict__ dxOut,
+                                               const int32_t* __restrict__ numCols,
+                                               const int32_t* __restrict__ cols,
+                                               SolverCtl* __restrict__ ctl, int sweep,
+                                               double* __restrict__ partial,
+                                               const double* __restrict__ partialPrev,
+                                               float tol, int maxSweeps) {
+    sweep_body<false>(P, pos, prm, yv, qIn, qOut, qRest, dxIn, dxOut, numCols, cols, ctl, sweep,
+                      partial, partialPrev, tol, maxSweeps, LineOut{nullptr, nullptr, nullptr});
+}
+
+// Strand solve, tile part (see the line mode of sweep_body).
+__global__ __launch_bounds__(256) void k_line_tile(const DevParams* __restrict__ P,
+                                                   const float4* __restrict__ pos,
+                                                   const float4* __restrict__ prm,
+                                                   const float4* __restrict__ yv,
+                                                   const float4* __restrict__ qIn,
+                                                   const float4* __restrict__ qRest,
+                                                   const float4* __restrict__ dxIn,
+                                                   const int32_t* __restrict__ numCols,
+                                                   const int32_t* __restrict__ cols, LineOut lo,
+                                                   const SolverCtl* __restrict__ ctl, int sweep) {
+    if (strand_skip(ctl, sweep)) return;
+    sweep_body<true>(P, pos, prm, yv, qIn, nullptr, qRest, dxIn, nullptr, numCols, cols, nullptr, 0,
+                     nullptr, nullptr, 0.0f, 0, lo);
+}
+
+// Strand solve, boundary part: one iteration over the pairs of vertices that
+// face each other across a tile boundary.  E_w is the last owned vertex of
+// tile w, F_(w+1) the first of tile w + 1; their tile solutions
+//   E_w = gE + pE E_(w-1) + qE F_(w+1),   F_(w+1) = gF + pF E_w + qF F_(w+2)
+// are solved exactly for the pair, with E_(w-1) and F_(w+2) -- each a whole
+// tile away, so weakly coupled -- from the previous iteration (zero in the
+// first).
+__global__ void k_line_pairs(const float4* __restrict__ g, const float2* __restrict__ ql,
+                             int tiles, const float4* __restrict__ Eold,
+                             const float4* __restrict__ Fold, float4* __restrict__ Enew,
+                             float4* __restrict__ Fnew, int first,
+                             const SolverCtl* __restrict__ ctl, int sweep) {
+    if (strand_skip(ctl, sweep)) return;
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= tiles - 1) return;
+    const int vE = 62 * w + 61, vF = 62 * (w + 1);  // both < n: tile w + 1 exists
+    const float4 GE = g[vE], GF = g[vF];
+    const float qE = ql[vE].x, qF = ql[vF].x;
+    v3 eo = v3{0.f, 0.f, 0.f}, fo = v3{0.f, 0.f, 0.f};
+    if (!first && w >= 1) eo = xyz(Eold[w - 1]);
+    if (!first && w + 2 < tiles) fo = xyz(Fold[w + 2]);
+    const v3 a = xyz(GE) + GE.w * eo;
+    const v3 b = xyz(GF) + qF * fo;
+    const float pF = GF.w;
+    const v3 E = (a + qE * b) * (1.0f / (1.0f - qE * pF));
+    const v3 F = b + pF * E;
+    Enew[w] = f4(E, 0.0f);
+    Fnew[w + 1] = f4(F, 0.0f);
+}
+
+// Strand solve, apply: x = g + p E_(w-1) + q F_(w+1) for every vertex of tile
+// w, added to dx and clamped to the step limit like a sweep's step.  Vertices
+// that are not rows (pinned) and non-finite results keep dx.
+__global__ __launch_bounds__(256) void k_line_apply(int n, int tiles,
+                                                    const float4* __restrict__ g,
+                                                    const float2* __restrict__ ql,
+                                                    const float4* __restrict__ E,
+                                                    const float4* __restrict__ F,
+                                                    float4* __restrict__ dx,
+                                                    const SolverCtl* __restrict__ ctl, int sweep) {
+    if (strand_skip(ctl, sweep)) return;
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    const float2 QL = ql[v];
+    if (QL.y < 0.0f) return;
+    const int w = v / 62;
+    const float4 G = g[v];
+    v3 d = xyz(G);
+    if (w >= 1) d += G.w * xyz(E[w - 1]);
+    if (w + 1 < tiles) d += QL.x * xyz(F[w + 1]);
+    if (!(isfinite(d.x) && isfinite(d.y) && isfinite(d.z))) return;
+    const float4 d4 = dx[v];
+    v3 nd = xyz(d4) + d;
+    const float l = len(nd);
+    if (l > QL.y && l > 0.0f) nd *= QL.y / l;
+    dx[v] = f4(nd, d4.w);
 }
 
 // Per-sweep relative Newton residual
@@ -712,6 +889,25 @@ void launch_frame_refresh(yb_sim* s) {
                        s->b.pos, s->b.prm, s->b.dx[0], s->b.q[0], s->b.q[1], s->b.qRest);
     hipLaunchKernelGGL(k_copy4, dim3(blocks(s->n, 256)), dim3(256), 0, s->stream, s->b.q[1],
                        s->b.q[0], s->n);
+}
+
+void launch_strand_solve(yb_sim* s, int buf, SolverCtl* ctl, int sweep, float* sys) {
+    const int tiles = sweep_waves(s->n);
+    const LineOut lo{s->b.lineG, s->b.lineQL, sys};
+    hipLaunchKernelGGL(k_line_tile, dim3(blocks(tiles * 64, 256)), dim3(256), 0, s->stream,
+                       s->b.params, s->b.pos, s->b.prm, s->b.yv, s->b.q[buf], s->b.qRest,
+                       s->b.dx[buf], s->b.numCols, s->b.cols, lo, ctl, sweep);
+    int cur = 0;
+    for (int it = 0; it < s->strandIters; it++) {
+        hipLaunchKernelGGL(k_line_pairs, dim3(blocks(std::max(tiles - 1, 1), 256)), dim3(256), 0,
+                           s->stream, s->b.lineG, s->b.lineQL, tiles, s->b.lineE[cur],
+                           s->b.lineF[cur], s->b.lineE[cur ^ 1], s->b.lineF[cur ^ 1], it == 0 ? 1 : 0,
+                           ctl, sweep);
+        cur ^= 1;
+    }
+    hipLaunchKernelGGL(k_line_apply, dim3(blocks(s->n, 256)), dim3(256), 0, s->stream, s->n, tiles,
+                       s->b.lineG, s->b.lineQL, s->b.lineE[cur], s->b.lineF[cur], s->b.dx[buf], ctl,
+                       sweep);
 }
 
 void launch_ctl_reset(yb_sim* s) {

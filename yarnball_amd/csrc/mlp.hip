@@ -1103,11 +1103,14 @@ extern "C" int yb_set_warm_start(yb_sim* s, int mode) {
     const int refresh = (mode & YB_WARM_NO_FRAME_REFRESH) ? 0 : 1;
     const int smooth = (mode & YB_WARM_NO_SMOOTHING) ? 0 : 1;
     const int fallback = (mode & YB_WARM_NO_FALLBACK) ? 0 : 1;
+    const int strand = (mode & YB_WARM_STRAND_SOLVE) ? 1 : 0;
     mode &= 0xff;
-    if (refresh != s->frameRefresh || smooth != s->mlpSmooth || fallback != s->fallback) {
+    if (refresh != s->frameRefresh || smooth != s->mlpSmooth || fallback != s->fallback ||
+        strand != s->strandSolve) {
         s->frameRefresh = refresh;
         s->mlpSmooth = smooth;
         s->fallback = fallback;
+        s->strandSolve = strand;
         s->graphItr = -1;
         s->adaptiveReady = false;
     }

@@ -164,6 +164,11 @@ struct Buffers {
     float4* corrPrev = nullptr;       // n: last converged correction (dx* - dx0)/L
     float4* dxPlain = nullptr;        // n: plain warm start of the current substep
     float4* predC = nullptr;          // n: predicted correction before smoothing
+    // strand solve (warm-start option)
+    float4* lineG = nullptr;   // n: tile solution xyz + left-boundary factor
+    float2* lineQL = nullptr;  // n: right-boundary factor + step limit
+    float4* lineE[2] = {nullptr, nullptr};  // per tile: last owned vertex's value, two iterations
+    float4* lineF[2] = {nullptr, nullptr};  // per tile: first owned vertex's value
     unsigned long long* mlpCount = nullptr;  // [applied, fallback] per 64-vertex wave
     float* recX = nullptr;            // recording: cap x 48 raw features
     float* recT = nullptr;            // recording: cap x 3 targets
@@ -214,6 +219,9 @@ struct yb_sim {
     double denomHost = 0.0;
     int warmStart = 0;     // 0 plain (iteration.cu:18-34), 1 learned, 2 temporal
     int frameRefresh = 1;  // refresh frames after a non-plain warm start
+    int strandSolve = 0;   // one Newton step along the strands after the warm start
+    int strandIters = 2;   // boundary-pair iterations of the strand solve
+    int strandEvery = 0;   // residual mode: also before every strandEvery-th sweep (0: never)
     int mlpSmooth = 1;     // [1 2 1]/4 passes on the predicted correction (0 or 1)
     int mlpGrid = 0;       // resident blocks of the inference kernel (set by yb_mlp_load)
     // Substep-level fallback of a learned session in residual-targeted mode:
@@ -252,6 +260,12 @@ void launch_residual(yb_sim* s, int sweep);
 void launch_scene_end_flags(yb_sim* s);  // flags of each scene's last vertex -> sceneEndFlags
 void launch_ctl_reset(yb_sim* s);
 void launch_frame_refresh(yb_sim* s);  // closed-form frame update from dx[0]
+// Tridiagonal Newton step along the strands on dx[buf] and q[buf]; with ctl
+// (between sweeps), nothing once the substep converged before `sweep`.
+// sys (tests): also write each vertex's system row (diagonal, coupling to the
+// next vertex, right-hand side xyz).
+void launch_strand_solve(yb_sim* s, int buf = 0, SolverCtl* ctl = nullptr, int sweep = 0,
+                         float* sys = nullptr);
 void launch_end(yb_sim* s, int fin, bool adaptive);         // iteration.cu:48-61
 int sweep_waves(int n);
 // learned warm start (mlp.hip)

@@ -108,18 +108,38 @@ def grad(X, T, params):
 NO_FRAME_REFRESH = 0x100
 NO_SMOOTHING = 0x200
 NO_FALLBACK = 0x400
+STRAND_SOLVE = 0x800
 
 
-def use(sim, params, frame_refresh=True, smooth=True, fallback=True):
+def use(sim, params, frame_refresh=True, smooth=True, fallback=True, strand=False):
     """Load a model into a simulation and switch its warm start to learned.
 
     fallback: in residual-targeted mode, start each substep from whichever of
-    the learned and the plain warm start has needed fewer sweeps lately."""
+    the learned and the plain warm start has needed fewer sweeps lately.
+    strand: follow the warm start with the strand solve (see strand())."""
     p = np.ascontiguousarray(params, np.float32)
     check(sim._lib.yb_mlp_load(sim.handle, ptr(p), MLP_PARAMS))
     mode = (1 | (0 if frame_refresh else NO_FRAME_REFRESH) | (0 if smooth else NO_SMOOTHING)
-            | (0 if fallback else NO_FALLBACK))
+            | (0 if fallback else NO_FALLBACK) | (STRAND_SOLVE if strand else 0))
     check(sim._lib.yb_set_warm_start(sim.handle, mode))
+
+
+def strand_schedule(sim, every=0, pair_iters=2):
+    """Strand-step schedule: every > 0 (residual mode) repeats the step before
+    every `every`-th sweep, at most 16 times per substep; pair_iters
+    boundary-pair iterations per step."""
+    check(sim._lib.yb_set_strand_schedule(sim.handle, int(every), int(pair_iters)))
+
+
+def strand(sim, frame_refresh=True, every=0, pair_iters=2):
+    """Plain warm start followed by one Newton step solved along whole strands
+    (the isotropic part of each local Hessian, stretch coupling between
+    neighbours): it removes the smooth error modes that block-Jacobi sweeps
+    damp slowly on long stiff strands.  No network.  every > 0 (residual
+    mode): repeat the step before every `every`-th sweep."""
+    strand_schedule(sim, every, pair_iters)
+    check(sim._lib.yb_set_warm_start(
+        sim.handle, STRAND_SOLVE | (0 if frame_refresh else NO_FRAME_REFRESH)))
 
 
 def warm_substeps(sim, reset=False):
