@@ -276,7 +276,7 @@ def live_profile(sim, sc, args, substeps_per_frame):
     return ms, nl, c
 
 
-def run_method(sc, args, method, model):
+def run_method(sc, args, method, model, strand_model=None):
     """Warm-up frames, then the timed frames, of one warm-start method on a
     fresh simulation; returns the open simulation and the measurements."""
     from yarnball_amd import Sim, learned, scenes
@@ -290,7 +290,7 @@ def run_method(sc, args, method, model):
     elif method == "strand":
         learned.strand(sim, every=args.strand_every)
     elif method == "learned+strand":
-        learned.use(sim, model, strand=True)
+        learned.use(sim, strand_model, strand=True)
         learned.strand_schedule(sim, every=args.strand_every)
     if args.no_graphs:
         sim.setGraphs(False)
@@ -356,6 +356,10 @@ def main():
                     help="enqueue kernels directly instead of HIP graphs (profiler runs)")
     ap.add_argument("--model", default=os.path.join(ROOT, "yarnball_amd", "models",
                                                     "warmstart.npy"))
+    ap.add_argument("--strand-model", default=os.path.join(ROOT, "yarnball_amd", "models",
+                                                           "warmstart_strand.npy"),
+                    help="model of the learned + strand method (trained on that solver's "
+                         "converged corrections)")
     ap.add_argument("--strand-every", type=int, default=6,
                     help="strand methods: also run the strand step before every k-th sweep "
                          "(0: only after the warm start)")
@@ -393,12 +397,14 @@ def main():
         tuned = autotune.tune(sc, residual=args.residual, max_sweeps=args.max_sweeps, substeps=1,
                               log=lambda m: print(m, file=sys.stderr, flush=True))
         args.launch_config = (tuned["sweep_threads"], tuned["query_threads"])
-    model = None
-    if any(m.startswith("learned") for m in methods):
+    model = strand_model = None
+    if "learned" in methods:
         model = np.load(args.model, allow_pickle=False)
+    if "learned+strand" in methods:
+        strand_model = np.load(args.strand_model, allow_pickle=False)
     runs = {}
     for m in methods:
-        runs[m] = run_method(sc, args, m, model)
+        runs[m] = run_method(sc, args, m, model, strand_model)
     # headline: the fastest method at the common residual (the plain method
     # when it runs alone)
     main_m = max((m for m in methods if m != "plain"), key=lambda m: runs[m]["value"],

@@ -22,6 +22,12 @@ recorded by the residual-targeted plain solver, seeded and deterministic:
 The evaluation patch (256 x 4096, jitter seed 0) is never recorded.
 
     python -m yarnball_amd.train_warmstart --out yarnball_amd/models/warmstart.npy
+
+The model for the learned warm start followed by the strand solve is trained
+the same way on the corrections that solver converges to:
+
+    python -m yarnball_amd.train_warmstart --strand-every 6 \
+        --out yarnball_amd/models/warmstart_strand.npy
 """
 import argparse
 import os
@@ -38,11 +44,13 @@ DEFAULT_MODEL = os.path.join(HERE, "models", "warmstart.npy")
 
 
 def record_scene(sc, frames, stride, keep, tol=1e-6, max_sweeps=6000, seed=0, log=print,
-                 first_frame=0):
+                 first_frame=0, strand_every=None):
     """Run `frames` frames of a scene with the plain solver, substep by substep,
     recording every `stride`-th substep from frame `first_frame` on; keep a
     seeded subsample of `keep` vertices of each recorded substep (all of them
-    when keep >= n)."""
+    when keep >= n).  strand_every (not None): the solver runs with the strand
+    solve after the plain warm start and before every strand_every-th sweep,
+    so the targets are the corrections that solver converges to."""
     n = sc.n
     frame = np.float32(sc.frame_dt)
     sub = int(np.ceil(frame / np.float32(sc.maxH)))
@@ -53,6 +61,8 @@ def record_scene(sc, frames, stride, keep, tol=1e-6, max_sweeps=6000, seed=0, lo
     sim = Sim(n)
     scenes.load_into(sim, sc)
     sim.setSolver(tol, maxSweeps=max_sweeps, chunk=16, checkEvery=4)
+    if strand_every is not None:
+        learned.strand(sim, every=strand_every)
     learned.record(sim, nrec * n)
     t = 0.0
     for f in range(frames):
@@ -76,14 +86,14 @@ def record_scene(sc, frames, stride, keep, tol=1e-6, max_sweeps=6000, seed=0, lo
     return X, C
 
 
-def generate(tol=1e-6, max_sweeps=6000, log=print):
+def generate(tol=1e-6, max_sweeps=6000, log=print, strand_every=None):
     t0 = time.perf_counter()
     parts = [
-        _readme(tol, max_sweeps, log),
+        _readme(tol, max_sweeps, log, strand_every=strand_every),
         record_scene(scenes.twisted_cable(64, 2048), 8, 4, 8192, tol, max_sweeps, seed=1,
-                     log=log),
+                     log=log, strand_every=strand_every),
         record_scene(scenes.knit_patch(256, 4096, seed=7), 5, 1, 24576, tol, max_sweeps, seed=2,
-                     log=log, first_frame=1),
+                     log=log, first_frame=1, strand_every=strand_every),
     ]
     X = np.concatenate([p[0] for p in parts])
     C = np.concatenate([p[1] for p in parts])
@@ -91,10 +101,10 @@ def generate(tol=1e-6, max_sweeps=6000, log=print):
     return X, C
 
 
-def _readme(tol, max_sweeps, log, seconds=0.5):
+def _readme(tol, max_sweeps, log, seconds=0.5, strand_every=None):
     sc = scenes.readme_scene()
     sc.frame_dt = seconds
-    return record_scene(sc, 1, 1, sc.n, tol, max_sweeps, log=log)
+    return record_scene(sc, 1, 1, sc.n, tol, max_sweeps, log=log, strand_every=strand_every)
 
 
 def fit(X, C, epochs=400, batch=16384, lr=1e-3, seed=0, log=print):
@@ -114,9 +124,12 @@ def main():
     ap.add_argument("--out", default=DEFAULT_MODEL)
     ap.add_argument("--epochs", type=int, default=400)
     ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--strand-every", type=int, default=None,
+                    help="record with the strand solve (after the warm start and before every "
+                         "k-th sweep): the model for learned + strand")
     args = ap.parse_args()
     log = lambda m: print(m, file=sys.stderr, flush=True)
-    X, T = generate(args.tol, log=log)
+    X, T = generate(args.tol, log=log, strand_every=args.strand_every)
     p, hist = fit(X, T, epochs=args.epochs, log=log)
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
     np.save(args.out, p)
