@@ -666,6 +666,8 @@ static int rebuild_graphs(yb_sim* s, int numItr) {
 
 // probe gaps of the substep-level warm-start chooser (choose_start below)
 constexpr int kProbeMin = 8, kProbeMax = 64;
+// sweeps of margin on the convergence estimate before a look-ahead chunk is skipped
+constexpr double kLookAheadMargin = 4.0;
 
 // Residual-targeted mode: prologue graphs, chunk graphs of s->chunk sweeps
 // with the residual reduction after each dx sweep, and the end graph.
@@ -685,6 +687,7 @@ static int rebuild_adaptive_graphs(yb_sim* s) {
             if (rc != YB_OK) return rc;
         }
     s->avgSweeps[0] = s->avgSweeps[1] = 0.0;
+    s->sweepEstimate = 0.0;
     s->chooserStep = 0;
     s->probeGap = kProbeMin;
     s->nextProbe = kProbeMin;
@@ -850,10 +853,18 @@ static int adaptive_substep(yb_sim* s, int mode) {
         launched = c + 1;
         return YB_OK;
     };
+    // The next chunk is queued before the host waits for the current one, so
+    // the device does not idle between chunks -- unless the recent substeps
+    // say this one converges within the current chunk: a chunk queued after
+    // convergence is a run of launches that exit at once (about 5 us each for
+    // a sweep).  If the estimate was wrong, the next chunk is queued after the
+    // wait.  This changes only when launches are queued, never the results.
     int rc = launch_chunk(0);
     if (rc != YB_OK) return rc;
     for (int c = 0; c < numChunks; c++) {
-        if (launched < numChunks && launched == c + 1) {
+        const bool expectDone =
+            s->sweepEstimate > 0.0 && s->sweepEstimate + kLookAheadMargin <= (c + 1) * s->chunk;
+        if (launched < numChunks && launched == c + 1 && !expectDone) {
             rc = launch_chunk(c + 1);
             if (rc != YB_OK) return rc;
         }
@@ -862,7 +873,13 @@ static int adaptive_substep(yb_sim* s, int mode) {
             done = s->b.hostFlag[c & 1] + 1;
             break;
         }
+        if (launched < numChunks && launched == c + 1) {
+            rc = launch_chunk(c + 1);
+            if (rc != YB_OK) return rc;
+        }
     }
+    // rises at once, decays slowly
+    s->sweepEstimate = done > s->sweepEstimate ? (double)done : 0.75 * s->sweepEstimate + 0.25 * done;
     if (s->useGraphs)
         HIPCHECK(hipGraphLaunch(s->endGraph, s->stream));
     else

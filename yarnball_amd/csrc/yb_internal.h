@@ -221,6 +221,7 @@ struct yb_sim {
     int frameRefresh = 1;  // refresh frames after a non-plain warm start
     int strandSolve = 0;   // one Newton step along the strands after the warm start
     int strandIters = 2;   // boundary-pair iterations of the strand solve
+    double sweepEstimate = 0.0;  // residual mode: recent sweeps to convergence (look-ahead)
     int strandEvery = 0;   // residual mode: also before every strandEvery-th sweep (0: never)
     int mlpSmooth = 1;     // [1 2 1]/4 passes on the predicted correction (0 or 1)
     int mlpGrid = 0;       // resident blocks of the inference kernel (set by yb_mlp_load)
