@@ -1,18 +1,21 @@
 """Eager launches (every kernel enqueued directly) and captured HIP graphs run
 the same kernels in the same order, and any block sizes the autotuner may
 pick compute the same values: bit-identical states, in the fixed-sweep and
-in the residual-targeted schedule."""
+in the residual-targeted schedule, with and without the strand solve (its
+tiles are the sweep's 62-vertex wave tiles, whatever the block size)."""
 import numpy as np
 import pytest
 
-from yarnball_amd import Sim, scenes
+from yarnball_amd import Sim, learned, scenes
 
 
-def _state(sc, graphs, residual, config=None):
+def _state(sc, graphs, residual, config=None, strand=False):
     sim = Sim(sc.n)
     scenes.load_into(sim, sc)
     if residual > 0:
         sim.setSolver(residual, maxSweeps=4000, chunk=16, checkEvery=4)
+    if strand:
+        learned.strand(sim, every=6)
     sim.setGraphs(graphs)
     if config:
         sim.setLaunchConfig(*config)
@@ -49,3 +52,15 @@ def test_launch_shape_does_not_change_results(residual):
             assert np.array_equal(x, y), cfg
         if residual > 0:
             assert ref[3]["sweeps"] == got[3]["sweeps"], cfg
+
+
+@pytest.mark.gpu
+def test_strand_solve_is_independent_of_launch_mode_and_shape():
+    sc = scenes.knit_patch(rows=8, per=256)
+    ref = _state(sc, True, 1e-5, strand=True)
+    for graphs, cfg in [(False, None), (True, (64, 64)), (True, (512, 128))]:
+        got = _state(sc, graphs, 1e-5, cfg, strand=True)
+        for x, y in zip(ref[:3], got[:3]):
+            assert np.array_equal(x, y), (graphs, cfg)
+        assert ref[3]["sweeps"] == got[3]["sweeps"], (graphs, cfg)
+    print(f"strand solve: {ref[3]['sweeps']} sweeps in every launch mode and shape")
