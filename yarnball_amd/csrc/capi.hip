@@ -600,7 +600,7 @@ constexpr int kStrandSteps = 16;
 static void enqueue_sweep(yb_sim* s, int k, bool adaptive, bool last = false) {
     const bool check = adaptive && checks_residual(s, k);
     bool checkPrev = adaptive && k > 0 && checks_residual(s, k - 1);
-    if (adaptive && s->strandEvery > 0 && k > 0 && k % s->strandEvery == 0 &&
+    if (adaptive && s->strandSolve && s->strandEvery > 0 && k > 0 && k % s->strandEvery == 0 &&
         k / s->strandEvery <= kStrandSteps) {
         flush_held(s);
         if (checkPrev) {

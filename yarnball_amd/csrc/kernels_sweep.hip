@@ -145,23 +145,6 @@ __device__ __forceinline__ void closest_e(v3 e, float a, float inva, v3 B0, v3 A
     w = vy;
 }
 
-// ---------------------------------------------------------------------------
-// One block-Jacobi sweep: Newton step on dx (cosserat.cu:20-207) followed by
-// the closed-form frame update (cosserat.cu:209-253), fused.
-//
-// Vertex i:  H = m/h^2 I + glue + stretch(seg i-1, seg i) + barrier/friction
-//            f = m/h^2 ((y-x) - dx) + ...
-//            dx' = dx + accelerationRatio H^{-1} f, clamped to the step limit.
-// Segment i: v = -2 kStretch e'/l with e' from dx', b = neighbour frames
-//            rotated by the rest Darboux rotors (sign-corrected), then
-//            q' = normalize((v b e1) + (|v| + |b|) b).
-//
-// Wave tile: lane l works on vertex 62 w + l - 1.  Lanes 1..62 own their
-// vertex (write dx' and q').  Lane 0 is a halo that only hands its segment's
-// endpoint-1 terms and its frame to lane 1.  Lane 63 computes dx' of the next
-// tile's first vertex without writing it, so that lane 62's frame update has
-// dx'(i+1); it also supplies q(i+1).  All exchange is cross-lane shifts --
-// no LDS, no barrier (the reference uses a 2-sector LDS reduction).
 // A strand step between sweeps does nothing once the substep has converged
 // (before `sweep`).  Every kernel of one step reads the same control word, so
 // they agree.
@@ -203,22 +186,43 @@ __device__ __forceinline__ void pcr64(float& A, float& Lo, float& Up, float B[5]
     }
 }
 
+// ---------------------------------------------------------------------------
+// One block-Jacobi sweep: Newton step on dx (cosserat.cu:20-207) followed by
+// the closed-form frame update (cosserat.cu:209-253), fused.
+//
+// Vertex i:  H = m/h^2 I + glue + stretch(seg i-1, seg i) + barrier/friction
+//            f = m/h^2 ((y-x) - dx) + ...
+//            dx' = dx + accelerationRatio H^{-1} f, clamped to the step limit.
+// Segment i: v = -2 kStretch e'/l with e' from dx', b = neighbour frames
+//            rotated by the rest Darboux rotors (sign-corrected), then
+//            q' = normalize((v b e1) + (|v| + |b|) b).
+//
+// Wave tile: lane l works on vertex 62 w + l - 1.  Lanes 1..62 own their
+// vertex (write dx' and q').  Lane 0 is a halo that only hands its segment's
+// endpoint-1 terms and its frame to lane 1.  Lane 63 computes dx' of the next
+// tile's first vertex without writing it, so that lane 62's frame update has
+// dx'(i+1); it also supplies q(i+1).  All exchange is cross-lane shifts --
+// no LDS, no barrier (the reference uses a 2-sector LDS reduction).
+//
+// kLine: the strand solve's tile part.  The same local systems are assembled,
+// then solved along the tile's strands instead of vertex by vertex (the line
+// tail below); nothing of the sweep's own output is written.
 template <bool kLine>
 __device__ __forceinline__ void sweep_body(const DevParams* __restrict__ P,
-                                               const float4* __restrict__ pos,
-                                               const float4* __restrict__ prm,
-                                               const float4* __restrict__ yv,
-                                               const float4* __restrict__ qIn,
-                                               float4* __restrict__ qOut,
-                                               const float4* __restrict__ qRest,
-                                               const float4* __restrict__ dxIn,
-                                               float4* __restrict__ dxOut,
-                                               const int32_t* __restrict__ numCols,
-                                               const int32_t* __restrict__ cols,
-                                               SolverCtl* __restrict__ ctl, int sweep,
-                                               double* __restrict__ partial,
-                                               const double* __restrict__ partialPrev,
-                                               float tol, int maxSweeps, LineOut lo) {
+                                           const float4* __restrict__ pos,
+                                           const float4* __restrict__ prm,
+                                           const float4* __restrict__ yv,
+                                           const float4* __restrict__ qIn,
+                                           float4* __restrict__ qOut,
+                                           const float4* __restrict__ qRest,
+                                           const float4* __restrict__ dxIn,
+                                           float4* __restrict__ dxOut,
+                                           const int32_t* __restrict__ numCols,
+                                           const int32_t* __restrict__ cols,
+                                           SolverCtl* __restrict__ ctl, int sweep,
+                                           double* __restrict__ partial,
+                                           const double* __restrict__ partialPrev,
+                                           float tol, int maxSweeps, LineOut lo) {
     // Residual-targeted mode: sweeps after convergence do nothing.
     if (ctl && ctl->doneAt < sweep) return;
     const int n = P->n;
