@@ -24,10 +24,16 @@ The evaluation patch (256 x 4096, jitter seed 0) is never recorded.
     python -m yarnball_amd.train_warmstart --out yarnball_amd/models/warmstart.npy
 
 The model for the learned warm start followed by the strand solve is trained
-the same way on the corrections that solver converges to:
+the same way on the corrections that solver converges to, from more of the
+knit (frames 1-8: 6.7M samples), for 800 epochs:
 
-    python -m yarnball_amd.train_warmstart --strand-every 6 \
-        --out yarnball_amd/models/warmstart_strand.npy
+    python -m yarnball_amd.train_warmstart --strand-every 6 --knit-frames 9 \
+        --epochs 800 --out yarnball_amd/models/warmstart_strand.npy
+
+With frames 1-4 (and 400 or 800 epochs) it needed 25.9 / 26.6 sweeps per
+substep in the learned + strand method on the evaluation patch; with frames
+1-8, 23.8, and the plain fallback ran in 24% of substeps instead of about
+half.
 """
 import argparse
 import os
@@ -86,14 +92,16 @@ def record_scene(sc, frames, stride, keep, tol=1e-6, max_sweeps=6000, seed=0, lo
     return X, C
 
 
-def generate(tol=1e-6, max_sweeps=6000, log=print, strand_every=None):
+def generate(tol=1e-6, max_sweeps=6000, log=print, strand_every=None, knit_frames=5):
+    """knit_frames: frames of the knit run (the first, the transient, is not
+    recorded)."""
     t0 = time.perf_counter()
     parts = [
         _readme(tol, max_sweeps, log, strand_every=strand_every),
         record_scene(scenes.twisted_cable(64, 2048), 8, 4, 8192, tol, max_sweeps, seed=1,
                      log=log, strand_every=strand_every),
-        record_scene(scenes.knit_patch(256, 4096, seed=7), 5, 1, 24576, tol, max_sweeps, seed=2,
-                     log=log, first_frame=1, strand_every=strand_every),
+        record_scene(scenes.knit_patch(256, 4096, seed=7), knit_frames, 1, 24576, tol,
+                     max_sweeps, seed=2, log=log, first_frame=1, strand_every=strand_every),
     ]
     X = np.concatenate([p[0] for p in parts])
     C = np.concatenate([p[1] for p in parts])
@@ -127,9 +135,12 @@ def main():
     ap.add_argument("--strand-every", type=int, default=None,
                     help="record with the strand solve (after the warm start and before every "
                          "k-th sweep): the model for learned + strand")
+    ap.add_argument("--knit-frames", type=int, default=5,
+                    help="frames of the training knit (frame 0 is run, not recorded)")
     args = ap.parse_args()
     log = lambda m: print(m, file=sys.stderr, flush=True)
-    X, T = generate(args.tol, log=log, strand_every=args.strand_every)
+    X, T = generate(args.tol, log=log, strand_every=args.strand_every,
+                    knit_frames=args.knit_frames)
     p, hist = fit(X, T, epochs=args.epochs, log=log)
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
     np.save(args.out, p)
