@@ -11,7 +11,7 @@ in HBM, solved every substep to the relative Newton residual 1e-6.  Four
 methods run the same frames from the same initial state: the plain warm
 start, the learned one (the committed model, with the substep-level plain
 fallback), and each of them followed by the strand solve (a tridiagonal
-Newton step along the strands after the warm start and before every 6th
+Newton step along the strands after the warm start and before every 7th
 sweep).  value = vertices x substeps / second over the timed frames of the
 fastest of the non-plain methods; every method's rate, sweeps per substep and
 ratio to the plain one are beside it.
@@ -360,7 +360,7 @@ def main():
                                                            "warmstart_strand.npy"),
                     help="model of the learned + strand method (trained on that solver's "
                          "converged corrections)")
-    ap.add_argument("--strand-every", type=int, default=6,
+    ap.add_argument("--strand-every", type=int, default=7,
                     help="strand methods: also run the strand step before every k-th sweep "
                          "(0: only after the warm start)")
     ap.add_argument("--scenes", type=int, default=512, help="batched_walks: number of scenes")
