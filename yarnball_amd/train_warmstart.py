@@ -25,15 +25,15 @@ The evaluation patch (256 x 4096, jitter seed 0) is never recorded.
 
 The model for the learned warm start followed by the strand solve is trained
 the same way on the corrections that solver converges to, from more of the
-knit (frames 1-8: 6.7M samples), for 800 epochs:
+knit (frames 1-12: 10M samples), for 800 epochs:
 
-    python -m yarnball_amd.train_warmstart --strand-every 6 --knit-frames 9 \
+    python -m yarnball_amd.train_warmstart --strand-every 7 --knit-frames 13 \
         --epochs 800 --out yarnball_amd/models/warmstart_strand.npy
 
-With frames 1-4 (and 400 or 800 epochs) it needed 25.9 / 26.6 sweeps per
-substep in the learned + strand method on the evaluation patch; with frames
-1-8, 23.8, and the plain fallback ran in 24% of substeps instead of about
-half.
+On the evaluation patch the learned + strand method then needs 22.5 sweeps per
+substep (the plain fallback runs in 18% of substeps).  More of the knit was
+the lever: frames 1-4 gave 25.9-26.6 and a fallback in about half of the
+substeps, frames 1-8 23.8 (steps every 6).
 """
 import argparse
 import os
