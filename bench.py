@@ -287,10 +287,12 @@ def run_method(sc, args, method, model, strand_model=None):
         sim.setSolver(args.residual, maxSweeps=args.max_sweeps, chunk=16, checkEvery=4)
     if method == "learned":
         learned.use(sim, model)
+    # the strand methods skip the frame refresh after the warm start: with the
+    # strand steps it saves no sweeps
     elif method == "strand":
-        learned.strand(sim, every=args.strand_every)
+        learned.strand(sim, every=args.strand_every, frame_refresh=False)
     elif method == "learned+strand":
-        learned.use(sim, strand_model, strand=True)
+        learned.use(sim, strand_model, strand=True, frame_refresh=False)
         learned.strand_schedule(sim, every=args.strand_every)
     if args.no_graphs:
         sim.setGraphs(False)
