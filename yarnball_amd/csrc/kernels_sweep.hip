@@ -520,6 +520,25 @@ __global__ __launch_bounds__(256) void k_line_tile(const DevParams* __restrict__
 // are solved exactly for the pair, with E_(w-1) and F_(w+2) -- each a whole
 // tile away, so weakly coupled -- from the previous iteration (zero in the
 // first).
+// The pair (E, F) of boundary b from its two rows and the outer neighbours'
+// values eo = E_(b-1), fo = F_(b+1) of the previous iteration.
+struct PairRows {
+    float4 GE, GF;  // tile solutions (xyz) and left factors (w) of the two rows
+    float qE, qF;   // their right factors
+};
+__device__ __forceinline__ PairRows pair_rows(const float4* __restrict__ g,
+                                              const float2* __restrict__ ql, int b) {
+    const int vE = 62 * b + 61, vF = 62 * (b + 1);  // both < n: tile b + 1 exists
+    return PairRows{g[vE], g[vF], ql[vE].x, ql[vF].x};
+}
+__device__ __forceinline__ void pair_solve(const PairRows& r, v3 eo, v3 fo, v3& E, v3& F) {
+    const v3 a = xyz(r.GE) + r.GE.w * eo;
+    const v3 b = xyz(r.GF) + r.qF * fo;
+    const float pF = r.GF.w;
+    E = (a + r.qE * b) * (1.0f / (1.0f - r.qE * pF));
+    F = b + pF * E;
+}
+
 __global__ void k_line_pairs(const float4* __restrict__ g, const float2* __restrict__ ql,
                              int tiles, const float4* __restrict__ Eold,
                              const float4* __restrict__ Fold, float4* __restrict__ Enew,
@@ -528,19 +547,66 @@ __global__ void k_line_pairs(const float4* __restrict__ g, const float2* __restr
     if (strand_skip(ctl, sweep)) return;
     const int w = blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= tiles - 1) return;
-    const int vE = 62 * w + 61, vF = 62 * (w + 1);  // both < n: tile w + 1 exists
-    const float4 GE = g[vE], GF = g[vF];
-    const float qE = ql[vE].x, qF = ql[vF].x;
     v3 eo = v3{0.f, 0.f, 0.f}, fo = v3{0.f, 0.f, 0.f};
     if (!first && w >= 1) eo = xyz(Eold[w - 1]);
     if (!first && w + 2 < tiles) fo = xyz(Fold[w + 2]);
-    const v3 a = xyz(GE) + GE.w * eo;
-    const v3 b = xyz(GF) + qF * fo;
-    const float pF = GF.w;
-    const v3 E = (a + qE * b) * (1.0f / (1.0f - qE * pF));
-    const v3 F = b + pF * E;
+    v3 E, F;
+    pair_solve(pair_rows(g, ql, w), eo, fo, E, F);
     Enew[w] = f4(E, 0.0f);
     Fnew[w + 1] = f4(F, 0.0f);
+}
+
+// One or two boundary-pair iterations and the apply, fused: one wave per
+// 62-vertex tile (lane l: vertex 62 w + l - 1, as in the sweep).  Tile w needs
+// E_(w-1) and F_w.  Lanes 0..3 solve the first iteration of boundaries
+// w-2 .. w+1 (the outer values start at zero); with two iterations, lanes 1
+// and 2 solve boundaries w-1 and w again from their neighbours' results.  The
+// same arithmetic as k_line_pairs, so the result equals that path's.
+__global__ __launch_bounds__(256) void k_line_pairs_apply(int n, int tiles, int iters,
+                                                          const float4* __restrict__ g,
+                                                          const float2* __restrict__ ql,
+                                                          float4* __restrict__ dx,
+                                                          const SolverCtl* __restrict__ ctl,
+                                                          int sweep) {
+    if (strand_skip(ctl, sweep)) return;
+    const int lane = threadIdx.x & 63;
+    const int w = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6);
+    if (w >= tiles) return;
+    // first iteration: lane l < 4 takes boundary w - 2 + l
+    v3 E1 = v3{0.f, 0.f, 0.f}, F1 = v3{0.f, 0.f, 0.f};
+    PairRows rows{};
+    const int b = w - 2 + lane;
+    const bool has = lane < 4 && b >= 0 && b <= tiles - 2;
+    if (has) {
+        rows = pair_rows(g, ql, b);
+        pair_solve(rows, v3{0.f, 0.f, 0.f}, v3{0.f, 0.f, 0.f}, E1, F1);
+    }
+    v3 E = E1, F = F1;
+    if (iters == 2) {
+        // lane l needs E of lane l - 1 and F of lane l + 1
+        const v3 eo = v3{__shfl_up(E1.x, 1, 64), __shfl_up(E1.y, 1, 64), __shfl_up(E1.z, 1, 64)};
+        const v3 fo = v3{__shfl_down(F1.x, 1, 64), __shfl_down(F1.y, 1, 64),
+                         __shfl_down(F1.z, 1, 64)};
+        if (has && (lane == 1 || lane == 2)) pair_solve(rows, eo, fo, E, F);
+    }
+    // E_(w-1) is boundary w - 1's (lane 1), F_w boundary w's (lane 2); a missing
+    // boundary left zeros there
+    const v3 Ew = v3{__shfl(E.x, 1, 64), __shfl(E.y, 1, 64), __shfl(E.z, 1, 64)};
+    const v3 Fw = v3{__shfl(F.x, 2, 64), __shfl(F.y, 2, 64), __shfl(F.z, 2, 64)};
+    const int v = 62 * w + lane - 1;
+    if (lane < 1 || lane > 62 || v >= n) return;
+    const float2 QL = ql[v];
+    if (QL.y < 0.0f) return;
+    const float4 G = g[v];
+    v3 d = xyz(G);
+    if (w >= 1) d += G.w * Ew;
+    if (w + 1 < tiles) d += QL.x * Fw;
+    if (!(isfinite(d.x) && isfinite(d.y) && isfinite(d.z))) return;
+    const float4 d4 = dx[v];
+    v3 nd = xyz(d4) + d;
+    const float l = len(nd);
+    if (l > QL.y && l > 0.0f) nd *= QL.y / l;
+    dx[v] = f4(nd, d4.w);
 }
 
 // Strand solve, apply: x = g + p E_(w-1) + q F_(w+1) for every vertex of tile
@@ -900,6 +966,12 @@ void launch_strand_solve(yb_sim* s, int buf, SolverCtl* ctl, int sweep, float* s
     hipLaunchKernelGGL(k_line_tile, dim3(blocks(tiles * 64, 256)), dim3(256), 0, s->stream,
                        s->b.params, s->b.pos, s->b.prm, s->b.yv, s->b.q[buf], s->b.qRest,
                        s->b.dx[buf], s->b.numCols, s->b.cols, lo, ctl, sweep);
+    if (s->strandIters <= 2) {
+        hipLaunchKernelGGL(k_line_pairs_apply, dim3(blocks(tiles * 64, 256)), dim3(256), 0,
+                           s->stream, s->n, tiles, s->strandIters, s->b.lineG, s->b.lineQL,
+                           s->b.dx[buf], ctl, sweep);
+        return;
+    }
     int cur = 0;
     for (int it = 0; it < s->strandIters; it++) {
         hipLaunchKernelGGL(k_line_pairs, dim3(blocks(std::max(tiles - 1, 1), 256)), dim3(256), 0,
