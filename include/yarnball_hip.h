@@ -415,7 +415,8 @@ enum {
     YB_STAGE_QUERY = 4,       /* LBVH query + collision.cu:39-92 */
     YB_STAGE_STEP_LIMIT = 5,  /* collision.cu:121-159 */
     YB_STAGE_SWEEP = 6,       /* cosserat.cu:20-253 (dx step + frame update, fused) */
-    YB_STAGE_MLP = 7,         /* learned warm start: features + inference */
+    YB_STAGE_MLP = 7,         /* warm starts: learned features + inference, temporal,
+                                 frame refresh, strand steps */
     YB_STAGE_END = 8,         /* iteration.cu:48-61 */
     YB_STAGE_RESIDUAL = 9,    /* residual reduction / solver control */
     YB_NUM_STAGES = 10
