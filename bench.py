@@ -11,8 +11,8 @@ in HBM, solved every substep to the relative Newton residual 1e-6.  Four
 methods run the same frames from the same initial state: the plain warm
 start, the learned one (the committed model, with the substep-level plain
 fallback), and each of them followed by the strand solve (a tridiagonal
-Newton step along the strands after the warm start and before every 7th
-sweep).  value = vertices x substeps / second over the timed frames of the
+Newton step along the strands after the warm start and before sweeps 3, 10,
+17, ...).  value = vertices x substeps / second over the timed frames of the
 fastest of the non-plain methods; every method's rate, sweeps per substep and
 ratio to the plain one are beside it.
 
@@ -290,10 +290,10 @@ def run_method(sc, args, method, model, strand_model=None):
     # the strand methods skip the frame refresh after the warm start: with the
     # strand steps it saves no sweeps
     elif method == "strand":
-        learned.strand(sim, every=args.strand_every, frame_refresh=False)
+        learned.strand(sim, every=args.strand_every, first=args.strand_first, frame_refresh=False)
     elif method == "learned+strand":
         learned.use(sim, strand_model, strand=True, frame_refresh=False)
-        learned.strand_schedule(sim, every=args.strand_every)
+        learned.strand_schedule(sim, every=args.strand_every, first=args.strand_first)
     if args.no_graphs:
         sim.setGraphs(False)
     if args.launch_config:
@@ -365,6 +365,9 @@ def main():
     ap.add_argument("--strand-every", type=int, default=7,
                     help="strand methods: also run the strand step before every k-th sweep "
                          "(0: only after the warm start)")
+    ap.add_argument("--strand-first", type=int, default=3,
+                    help="strand methods: the sweep before which the first such step runs "
+                         "(0: the strand-every-th)")
     ap.add_argument("--scenes", type=int, default=512, help="batched_walks: number of scenes")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default block sizes instead of timing the candidates")
@@ -497,10 +500,11 @@ def main():
     method_names = {
         "plain": "plain warm start",
         "learned": "learned warm start with substep-level plain fallback",
-        "strand": f"plain warm start + strand solve (after the warm start and before every "
-                  f"{args.strand_every}th sweep)",
+        "strand": f"plain warm start + strand solve (after the warm start and before sweeps "
+                  f"{args.strand_first}, {args.strand_first}+{args.strand_every}k)",
         "learned+strand": f"learned warm start with substep-level plain fallback + strand solve "
-                          f"(after the warm start and before every {args.strand_every}th sweep)",
+                          f"(after the warm start and before sweeps {args.strand_first}, "
+                          f"{args.strand_first}+{args.strand_every}k)",
     }
 
     out = {

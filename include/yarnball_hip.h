@@ -362,16 +362,16 @@ int yb_mlp_load(yb_sim* s, const float* params, int count);
 #define YB_WARM_STRAND_SOLVE 0x800
 int yb_set_warm_start(yb_sim* s, int mode);
 /* Strand-solve schedule (with YB_WARM_STRAND_SOLVE): in residual-targeted
- * mode, also run the strand step before every `every`-th sweep (0: only after
- * the warm start; a multiple of 4 keeps the residual checks where they are),
- * on the current iterate, with no frame refresh (the next sweep updates the
- * frames); pairIters boundary-pair iterations per step (default 2).  The
+ * mode, also run the strand step before sweeps first, first + every,
+ * first + 2 every, ... (every 0: only after the warm start; first 0: first =
+ * every), on the current iterate, with no frame refresh (the next sweep
+ * updates the frames); pairIters boundary-pair iterations per step (default 2).  The
  * residual of the sweep before such a step is reduced before it, and a
  * converged iterate is never changed.  At most 16 such steps per substep:
  * one that has not converged by then finishes with plain sweeps.  Vertices
  * held at their step limit take no part in a strand step (they stay where the
  * sweeps put them).  No counterpart in the reference. */
-int yb_set_strand_schedule(yb_sim* s, int every, int pairIters);
+int yb_set_strand_schedule(yb_sim* s, int every, int pairIters, int first);
 /* Vertex level: out[0] predictions applied, out[1] vertices that kept the
  * plain warm start because the prediction was not finite. */
 int yb_mlp_counters(yb_sim* s, int64_t out[2], int reset);

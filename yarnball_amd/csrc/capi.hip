@@ -594,14 +594,15 @@ static bool checks_residual(const yb_sim* s, int k) {
 // sweep k - 1 is made before it, by a launch of its own, and every kernel of
 // the step does nothing once the solve has converged: a converged iterate is
 // never changed.
-// At most kStrandSteps such steps per substep: a substep that has not
-// converged by then finishes with plain sweeps.
+// Steps at iterations first, first + every, ...; at most kStrandSteps per
+// substep: a substep that has not converged by then finishes with plain sweeps.
 constexpr int kStrandSteps = 16;
 static void enqueue_sweep(yb_sim* s, int k, bool adaptive, bool last = false) {
     const bool check = adaptive && checks_residual(s, k);
     bool checkPrev = adaptive && k > 0 && checks_residual(s, k - 1);
-    if (adaptive && s->strandSolve && s->strandEvery > 0 && k > 0 && k % s->strandEvery == 0 &&
-        k / s->strandEvery <= kStrandSteps) {
+    const int first = s->strandFirst > 0 ? s->strandFirst : s->strandEvery;
+    if (adaptive && s->strandSolve && s->strandEvery > 0 && k >= first &&
+        (k - first) % s->strandEvery == 0 && (k - first) / s->strandEvery < kStrandSteps) {
         flush_held(s);
         if (checkPrev) {
             launch_residual(s, k - 1);
@@ -729,14 +730,15 @@ extern "C" int yb_set_solver(yb_sim* s, float residualTol, int maxSweeps, int ch
     return YB_OK;
 }
 
-extern "C" int yb_set_strand_schedule(yb_sim* s, int every, int pairIters) {
-    if (!s || every < 0 || pairIters < 1 || pairIters > 64) {
-        set_message("yb_set_strand_schedule: every >= 0, 1 <= pairIters <= 64");
+extern "C" int yb_set_strand_schedule(yb_sim* s, int every, int pairIters, int first) {
+    if (!s || every < 0 || pairIters < 1 || pairIters > 64 || first < 0) {
+        set_message("yb_set_strand_schedule: every >= 0, 1 <= pairIters <= 64, first >= 0");
         return YB_ERR_INVALID_ARGUMENT;
     }
-    if (every != s->strandEvery || pairIters != s->strandIters) {
+    if (every != s->strandEvery || pairIters != s->strandIters || first != s->strandFirst) {
         s->strandEvery = every;
         s->strandIters = pairIters;
+        s->strandFirst = first;
         s->graphItr = -1;
         s->adaptiveReady = false;
     }

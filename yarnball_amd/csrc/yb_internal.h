@@ -223,6 +223,7 @@ struct yb_sim {
     int strandIters = 2;   // boundary-pair iterations of the strand solve
     double sweepEstimate = 0.0;  // residual mode: recent sweeps to convergence (look-ahead)
     int strandEvery = 0;   // residual mode: also before every strandEvery-th sweep (0: never)
+    int strandFirst = 0;   // ... starting at this sweep (0: at strandEvery)
     int mlpSmooth = 1;     // [1 2 1]/4 passes on the predicted correction (0 or 1)
     int mlpGrid = 0;       // resident blocks of the inference kernel (set by yb_mlp_load)
     // Substep-level fallback of a learned session in residual-targeted mode:

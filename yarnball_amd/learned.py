@@ -124,20 +124,20 @@ def use(sim, params, frame_refresh=True, smooth=True, fallback=True, strand=Fals
     check(sim._lib.yb_set_warm_start(sim.handle, mode))
 
 
-def strand_schedule(sim, every=0, pair_iters=2):
+def strand_schedule(sim, every=0, pair_iters=2, first=0):
     """Strand-step schedule: every > 0 (residual mode) repeats the step before
-    every `every`-th sweep, at most 16 times per substep; pair_iters
-    boundary-pair iterations per step."""
-    check(sim._lib.yb_set_strand_schedule(sim.handle, int(every), int(pair_iters)))
+    sweeps first, first + every, ... (first 0: first = every), at most 16 times
+    per substep; pair_iters boundary-pair iterations per step."""
+    check(sim._lib.yb_set_strand_schedule(sim.handle, int(every), int(pair_iters), int(first)))
 
 
-def strand(sim, frame_refresh=True, every=0, pair_iters=2):
+def strand(sim, frame_refresh=True, every=0, pair_iters=2, first=0):
     """Plain warm start followed by one Newton step solved along whole strands
     (the isotropic part of each local Hessian, stretch coupling between
     neighbours): it removes the smooth error modes that block-Jacobi sweeps
     damp slowly on long stiff strands.  No network.  every > 0 (residual
     mode): repeat the step before every `every`-th sweep."""
-    strand_schedule(sim, every, pair_iters)
+    strand_schedule(sim, every, pair_iters, first)
     check(sim._lib.yb_set_warm_start(
         sim.handle, STRAND_SOLVE | (0 if frame_refresh else NO_FRAME_REFRESH)))
 
