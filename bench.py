@@ -284,7 +284,7 @@ def run_method(sc, args, method, model, strand_model=None):
     sim = Sim(sc.n)
     scenes.load_into(sim, sc)
     if args.residual > 0:
-        sim.setSolver(args.residual, maxSweeps=args.max_sweeps, chunk=args.chunk, checkEvery=4)
+        sim.setSolver(args.residual, maxSweeps=args.max_sweeps, chunk=args.chunk, checkEvery=args.check_every)
     if method == "learned":
         learned.use(sim, model)
     # the strand methods skip the frame refresh after the warm start: with the
@@ -352,6 +352,8 @@ def main():
     ap.add_argument("--max-sweeps", type=int, default=6000)
     ap.add_argument("--chunk", type=int, default=16,
                     help="sweeps per chunk graph in residual mode")
+    ap.add_argument("--check-every", type=int, default=4,
+                    help="residual mode: sweeps between residual evaluations")
     ap.add_argument("--method", choices=["plain", "learned", "both", "strand", "learned+strand",
                                          "all"],
                     default="all",

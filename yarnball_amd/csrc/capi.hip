@@ -584,7 +584,6 @@ static bool checks_residual(const yb_sim* s, int k) {
     return (k + 1) % s->checkEvery == 0 || k == s->maxSweeps - 1;
 }
 
-// last: the final sweep of a fixed schedule (ends the run of sweeps in a profile)
 // The residual of a check sweep is reduced, and convergence decided, by the
 // next sweep's first wave; only the last sweep a substep may run is checked by
 // a launch of its own.
