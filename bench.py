@@ -11,8 +11,8 @@ in HBM, solved every substep to the relative Newton residual 1e-6.  Four
 methods run the same frames from the same initial state: the plain warm
 start, the learned one (the committed model, with the substep-level plain
 fallback), and each of them followed by the strand solve (a tridiagonal
-Newton step along the strands after the warm start and before sweeps 3, 10,
-17, ...).  value = vertices x substeps / second over the timed frames of the
+Newton step along the strands after the warm start and before sweeps 1, 5,
+9, ...).  value = vertices x substeps / second over the timed frames of the
 fastest of the non-plain methods; every method's rate, sweeps per substep and
 ratio to the plain one are beside it.
 
@@ -366,10 +366,10 @@ def main():
                                                            "warmstart_strand.npy"),
                     help="model of the learned + strand method (trained on that solver's "
                          "converged corrections)")
-    ap.add_argument("--strand-every", type=int, default=7,
+    ap.add_argument("--strand-every", type=int, default=4,
                     help="strand methods: also run the strand step before every k-th sweep "
                          "(0: only after the warm start)")
-    ap.add_argument("--strand-first", type=int, default=3,
+    ap.add_argument("--strand-first", type=int, default=1,
                     help="strand methods: the sweep before which the first such step runs "
                          "(0: the strand-every-th)")
     ap.add_argument("--scenes", type=int, default=512, help="batched_walks: number of scenes")

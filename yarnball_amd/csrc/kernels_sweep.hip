@@ -407,11 +407,18 @@ __device__ __forceinline__ void sweep_body(const DevParams* __restrict__ P,
 
     if constexpr (kLine) {
         // Strand solve, tile part: the tile's owned vertices (lanes 1..62)
-        // form a tridiagonal system along their strands -- the isotropic part
-        // trace(H)/3 of each local Hessian on the diagonal and the stretch
-        // stiffness k/l^2 of each segment between its two free ends -- with
-        // the vertices just outside the tile (lanes 0 and 63) as unknown
+        // form a tridiagonal system along their strands -- on the diagonal a
+        // scalar stiffness of each local Hessian, between neighbours the
+        // stretch stiffness k/l^2 of each segment between its two free ends --
+        // with the vertices just outside the tile (lanes 0 and 63) as unknown
         // boundary values.  Per vertex: x = g + p x_left + q x_right.
+        // The scalar is the curvature along the vertex's force, f.H.f / f.f,
+        // with which x = f / a is the exact minimiser of the local quadratic
+        // along f.  The isotropic trace(H)/3 (used when f vanishes) is smaller
+        // wherever the contact stiffness sits in the force direction: the
+        // step then overshoots there and the sweeps that follow must take the
+        // excess back, so steps could not come closer together than about 7
+        // sweeps without losing ground.
         // A vertex the step limit holds (a sweep clamped it) keeps a nonzero
         // force at the sweeps' fixed point; as a row it would push its
         // neighbours every strand step and the sweeps would push them back.
@@ -424,7 +431,11 @@ __device__ __forceinline__ void sweep_body(const DevParams* __restrict__ P,
                               ? r4.y * invl * invl
                               : 0.0f;
         const float cPrev = __shfl_up(cpl, 1, 64);
-        float A = row ? (H.a[0] + H.a[1] + H.a[2]) * (1.0f / 3.0f) : 1.0f;
+        const float ff = len2(f);
+        const float fHf = H.a[0] * f.x * f.x + H.a[1] * f.y * f.y + H.a[2] * f.z * f.z +
+                          2.0f * (H.a[3] * f.x * f.y + H.a[4] * f.x * f.z + H.a[5] * f.y * f.z);
+        const float diag = ff > 1e-30f ? fHf / ff : (H.a[0] + H.a[1] + H.a[2]) * (1.0f / 3.0f);
+        float A = row ? diag : 1.0f;
         float Lo = (row && lane >= 2) ? cPrev : 0.0f;
         float Up = (row && lane <= 61) ? cpl : 0.0f;
         float B[5] = {row ? f.x : 0.0f, row ? f.y : 0.0f, row ? f.z : 0.0f,
@@ -432,7 +443,7 @@ __device__ __forceinline__ void sweep_body(const DevParams* __restrict__ P,
         pcr64(A, Lo, Up, B);
         if (lo.sys && in && lane >= 1 && lane <= 62) {
             float* o = lo.sys + 5 * (size_t)v;
-            o[0] = row ? (H.a[0] + H.a[1] + H.a[2]) * (1.0f / 3.0f) : 1.0f;
+            o[0] = row ? diag : 1.0f;
             o[1] = cpl;
             o[2] = row ? f.x : 0.0f;
             o[3] = row ? f.y : 0.0f;
