@@ -347,9 +347,10 @@ int yb_mlp_load(yb_sim* s, const float* params, int count);
  * convergence either way.  YB_WARM_NO_FALLBACK always uses the learned start. */
 #define YB_WARM_NO_FALLBACK 0x400
 /* Strand solve: after the warm start (any mode), one Newton step of the
- * linearised system restricted to the strands -- the isotropic part of each
- * vertex's local Hessian on the diagonal, the stretch stiffness of each
- * segment between its free ends -- solved along whole strands: exactly
+ * linearised system restricted to the strands -- on the diagonal the
+ * curvature of each vertex's local Hessian H along its force f, f.H.f / f.f
+ * (trace(H)/3 where f vanishes), the stretch stiffness of each segment
+ * between its free ends off the diagonal -- solved along whole strands: exactly
  * inside each 62-vertex sweep tile (cyclic reduction over the wave's lanes),
  * then by iterations over the vertex pairs that face each other across tile
  * boundaries (two by default, see yb_set_strand_schedule).  The correction is

@@ -133,10 +133,11 @@ def strand_schedule(sim, every=0, pair_iters=2, first=0):
 
 def strand(sim, frame_refresh=True, every=0, pair_iters=2, first=0):
     """Plain warm start followed by one Newton step solved along whole strands
-    (the isotropic part of each local Hessian, stretch coupling between
-    neighbours): it removes the smooth error modes that block-Jacobi sweeps
-    damp slowly on long stiff strands.  No network.  every > 0 (residual
-    mode): repeat the step before every `every`-th sweep."""
+    (each local Hessian's curvature along the vertex's force on the
+    diagonal, stretch coupling between neighbours): it removes the smooth
+    error modes that block-Jacobi sweeps damp slowly on long stiff strands.
+    No network.  every > 0 (residual mode): repeat the step before sweeps
+    first, first + every, ... (the 1M knit: every=4, first=1)."""
     strand_schedule(sim, every, pair_iters, first)
     check(sim._lib.yb_set_warm_start(
         sim.handle, STRAND_SOLVE | (0 if frame_refresh else NO_FRAME_REFRESH)))
