@@ -30,10 +30,12 @@ knit (frames 1-12: 10M samples), for 800 epochs:
     python -m yarnball_amd.train_warmstart --strand-every 7 --knit-frames 13 \
         --epochs 800 --out yarnball_amd/models/warmstart_strand.npy
 
-On the evaluation patch the learned + strand method then needs 22.5 sweeps per
-substep (the plain fallback runs in 18% of substeps).  More of the knit was
-the lever: frames 1-4 gave 25.9-26.6 and a fallback in about half of the
-substeps, frames 1-8 23.8 (steps every 6).
+On the evaluation patch the learned + strand method then needs 13.2 sweeps per
+substep with steps before sweeps 1, 5, 9, ...  More of the knit was the
+lever: frames 1-4 gave 25.9-26.6 and a fallback in about half of the
+substeps, frames 1-8 23.8 (steps every 6, an older diagonal).  Recording with
+the bench's own schedule (--strand-every 4 --strand-first 1) gave a worse
+model: 18.5 sweeps per substep.
 """
 import argparse
 import os
@@ -50,13 +52,14 @@ DEFAULT_MODEL = os.path.join(HERE, "models", "warmstart.npy")
 
 
 def record_scene(sc, frames, stride, keep, tol=1e-6, max_sweeps=6000, seed=0, log=print,
-                 first_frame=0, strand_every=None):
+                 first_frame=0, strand_every=None, strand_first=0):
     """Run `frames` frames of a scene with the plain solver, substep by substep,
     recording every `stride`-th substep from frame `first_frame` on; keep a
     seeded subsample of `keep` vertices of each recorded substep (all of them
     when keep >= n).  strand_every (not None): the solver runs with the strand
-    solve after the plain warm start and before every strand_every-th sweep,
-    so the targets are the corrections that solver converges to."""
+    solve after the plain warm start and before sweeps strand_first,
+    strand_first + strand_every, ... (strand_first 0: strand_every), so the
+    targets are the corrections that solver converges to."""
     n = sc.n
     frame = np.float32(sc.frame_dt)
     sub = int(np.ceil(frame / np.float32(sc.maxH)))
@@ -68,7 +71,7 @@ def record_scene(sc, frames, stride, keep, tol=1e-6, max_sweeps=6000, seed=0, lo
     scenes.load_into(sim, sc)
     sim.setSolver(tol, maxSweeps=max_sweeps, chunk=16, checkEvery=4)
     if strand_every is not None:
-        learned.strand(sim, every=strand_every)
+        learned.strand(sim, every=strand_every, first=strand_first)
     learned.record(sim, nrec * n)
     t = 0.0
     for f in range(frames):
@@ -92,16 +95,18 @@ def record_scene(sc, frames, stride, keep, tol=1e-6, max_sweeps=6000, seed=0, lo
     return X, C
 
 
-def generate(tol=1e-6, max_sweeps=6000, log=print, strand_every=None, knit_frames=5):
+def generate(tol=1e-6, max_sweeps=6000, log=print, strand_every=None, knit_frames=5,
+             strand_first=0):
     """knit_frames: frames of the knit run (the first, the transient, is not
     recorded)."""
     t0 = time.perf_counter()
+    st = dict(strand_every=strand_every, strand_first=strand_first)
     parts = [
-        _readme(tol, max_sweeps, log, strand_every=strand_every),
+        _readme(tol, max_sweeps, log, **st),
         record_scene(scenes.twisted_cable(64, 2048), 8, 4, 8192, tol, max_sweeps, seed=1,
-                     log=log, strand_every=strand_every),
+                     log=log, **st),
         record_scene(scenes.knit_patch(256, 4096, seed=7), knit_frames, 1, 24576, tol,
-                     max_sweeps, seed=2, log=log, first_frame=1, strand_every=strand_every),
+                     max_sweeps, seed=2, log=log, first_frame=1, **st),
     ]
     X = np.concatenate([p[0] for p in parts])
     C = np.concatenate([p[1] for p in parts])
@@ -109,10 +114,11 @@ def generate(tol=1e-6, max_sweeps=6000, log=print, strand_every=None, knit_frame
     return X, C
 
 
-def _readme(tol, max_sweeps, log, seconds=0.5, strand_every=None):
+def _readme(tol, max_sweeps, log, seconds=0.5, strand_every=None, strand_first=0):
     sc = scenes.readme_scene()
     sc.frame_dt = seconds
-    return record_scene(sc, 1, 1, sc.n, tol, max_sweeps, log=log, strand_every=strand_every)
+    return record_scene(sc, 1, 1, sc.n, tol, max_sweeps, log=log, strand_every=strand_every,
+                        strand_first=strand_first)
 
 
 def fit(X, C, epochs=400, batch=16384, lr=1e-3, seed=0, log=print):
@@ -135,12 +141,15 @@ def main():
     ap.add_argument("--strand-every", type=int, default=None,
                     help="record with the strand solve (after the warm start and before every "
                          "k-th sweep): the model for learned + strand")
+    ap.add_argument("--strand-first", type=int, default=0,
+                    help="with --strand-every: the sweep before which the first such step "
+                         "runs (0: the strand-every-th)")
     ap.add_argument("--knit-frames", type=int, default=5,
                     help="frames of the training knit (frame 0 is run, not recorded)")
     args = ap.parse_args()
     log = lambda m: print(m, file=sys.stderr, flush=True)
     X, T = generate(args.tol, log=log, strand_every=args.strand_every,
-                    knit_frames=args.knit_frames)
+                    knit_frames=args.knit_frames, strand_first=args.strand_first)
     p, hist = fit(X, T, epochs=args.epochs, log=log)
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
     np.save(args.out, p)
