@@ -69,6 +69,15 @@ def test_strand_step_solves_its_system():
         assert np.array_equal(after[~free], before[~free]), "pinned vertices must keep dx"
         # the pinned rows are identity rows with no coupling
         assert np.all(rows[~free, 0] == 1) and np.all(rows[~free, 2:] == 0)
+        # The rows are diagonally dominant, which the unpivoted cyclic
+        # reduction relies on: the diagonal (the local Hessian's curvature
+        # along the force) holds the stretch stiffness of both segments.
+        c = rows[:, 1].astype(np.float64)
+        off = c + np.concatenate([[0.0], c[:-1]])
+        a = rows[:, 0].astype(np.float64)
+        margin = (a[m] - off[m]) / a[m]
+        print(f"pair iterations {iters}: min diagonal-dominance margin {margin.min():.3e}")
+        assert margin.min() > -1e-6
     print("strand step vs exact banded solve, relative max error by pair iterations:", errs)
     # two pair iterations already reach the FP32 accuracy of the tile solves
     assert errs[2] < 1e-5 and errs[8] < 1e-5
