@@ -583,6 +583,17 @@ __global__ __launch_bounds__(256) void k_line_pairs_apply(int n, int tiles, int 
     const int lane = threadIdx.x & 63;
     const int w = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6);
     if (w >= tiles) return;
+    // The apply's own rows are loaded first, so that their latency overlaps
+    // the pair solves (which depend only on the boundary rows).
+    const int v = 62 * w + lane - 1;
+    const bool own = lane >= 1 && lane <= 62 && v < n;
+    float2 QL = make_float2(0.0f, -1.0f);
+    float4 G = make_float4(0.f, 0.f, 0.f, 0.f), d4 = G;
+    if (own) {
+        QL = ql[v];
+        G = g[v];
+        d4 = dx[v];
+    }
     // first iteration: lane l < 4 takes boundary w - 2 + l
     v3 E1 = v3{0.f, 0.f, 0.f}, F1 = v3{0.f, 0.f, 0.f};
     PairRows rows{};
@@ -604,16 +615,11 @@ __global__ __launch_bounds__(256) void k_line_pairs_apply(int n, int tiles, int 
     // boundary left zeros there
     const v3 Ew = v3{__shfl(E.x, 1, 64), __shfl(E.y, 1, 64), __shfl(E.z, 1, 64)};
     const v3 Fw = v3{__shfl(F.x, 2, 64), __shfl(F.y, 2, 64), __shfl(F.z, 2, 64)};
-    const int v = 62 * w + lane - 1;
-    if (lane < 1 || lane > 62 || v >= n) return;
-    const float2 QL = ql[v];
-    if (QL.y < 0.0f) return;
-    const float4 G = g[v];
+    if (!own || QL.y < 0.0f) return;
     v3 d = xyz(G);
     if (w >= 1) d += G.w * Ew;
     if (w + 1 < tiles) d += QL.x * Fw;
     if (!(isfinite(d.x) && isfinite(d.y) && isfinite(d.z))) return;
-    const float4 d4 = dx[v];
     v3 nd = xyz(d4) + d;
     const float l = len(nd);
     if (l > QL.y && l > 0.0f) nd *= QL.y / l;
