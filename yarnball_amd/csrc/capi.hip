@@ -503,10 +503,17 @@ static void hold(yb_sim* s, int stage) {
 }
 
 static void mark(yb_sim* s, int stage);
-// Closes a run of held launches with an event of their own stage.
+// Closes a run of held launches with an event of their own stage.  It is
+// called before the launches of sweep curTag, so the run ends at sweep
+// curTag - 1; mark() counts one launch of its own, which a flush does not add.
 static void flush_held(yb_sim* s) {
     ProfState* ps = (ProfState*)s->prof;
-    if (ps && ps->held) mark(s, ps->heldStage);
+    if (!ps || !ps->held) return;
+    const int tag = ps->curTag;
+    if (tag >= 0) ps->curTag = tag - 1;
+    ps->held--;
+    mark(s, ps->heldStage);
+    ps->curTag = tag;
 }
 
 static void mark(yb_sim* s, int stage) {
@@ -1199,16 +1206,20 @@ extern "C" int yb_profile(yb_sim* s, yb_params* p, float h, int substeps, double
         if (adaptive) {
             // As in the stepper, sweeps run in chunks of s->chunk and the next
             // chunk is queued before the host waits for the current one's
-            // flag, so the device does not idle between chunks.  One event
-            // per chunk; the chunks that reach past the converged sweep (they
-            // hold early-exit launches) are left out of the harvest.
+            // flag, so the device does not idle between chunks.  Each event
+            // is tagged with the sweep its interval ends at (one per chunk,
+            // plus the strand steps'); the intervals that reach past the
+            // converged sweep (they hold early-exit launches) are left out of
+            // the harvest.
             const int ce = s->chunk;
-            ps.runLen = ce;
+            ps.runLen = 1;
             const int groups = (s->maxSweeps + ce - 1) / ce;
             auto enqueue_group = [&](int g) -> int {
-                ps.curTag = g * ce;
                 const int k1 = std::min((g + 1) * ce, s->maxSweeps);
-                for (int k = g * ce; k < k1; k++) enqueue_sweep(s, k, true, k == k1 - 1);
+                for (int k = g * ce; k < k1; k++) {
+                    ps.curTag = k;
+                    enqueue_sweep(s, k, true, k == k1 - 1);
+                }
                 ps.curTag = -1;
                 HIPCHECK(hipMemcpyAsync(&s->b.hostFlag[g & 1], &s->b.ctl->doneAt, sizeof(int32_t),
                                         hipMemcpyDeviceToHost, s->stream));
