@@ -62,7 +62,7 @@ typedef struct yb_vertex {
 /* Simulation parameters: the scalar part of YarnBall::MetaData (YarnBall.h:63-114)
  * without its device pointers.  h, lastH, time, detectionRadius and
  * scaledDetectionRadius are written by yb_advance(), as Sim::advance and
- * Sim::uploadMeta do (step.cpp:101-118, YarnBall.cu:226-228). */
+ * Sim::uploadMeta do (step.cpp:36-53, YarnBall.cu:226-228). */
 typedef struct yb_params {
     float gravity[3];
     int32_t numItr;
@@ -132,7 +132,7 @@ int yb_download(yb_sim* s, yb_vertex* verts, float* vels, float* qs);
 /* Sim::zeroVelocities (YarnBall.cu:263-274). */
 int yb_zero_velocities(yb_sim* s);
 
-/* Sim::advance (step.cpp:101-120) + Sim::checkErrors (YarnBall.cu:276-304).
+/* Sim::advance (step.cpp:36-55) + Sim::checkErrors (YarnBall.cu:276-304).
  * Validates *p like uploadMeta, splits h into ceil(h/maxH) substeps, runs them on
  * the handle's stream and blocks until done.  Updates p->h, p->lastH, p->time and
  * the derived radii.  Returns 0, 1 (contact-list overflow; state may be corrupt),
@@ -263,7 +263,7 @@ int yb_debug_write_last_vels(yb_sim* s, const float* lastVels);
 
 /* ---- residual-targeted solving ---------------------------------------------
  * No counterpart in the reference, which always runs meta.numItr sweeps
- * (step.cpp:83-84).  With residualTol > 0 every substep runs sweeps until the
+ * (step.cpp:18-19).  With residualTol > 0 every substep runs sweeps until the
  * relative Newton residual
  *     r = sqrt( sum_free |dx_{k+1} - dx_k|^2 / sum_free l_i^2 )
  * -- the RMS change of the iterate per sweep in units of the rest length, l_i
@@ -308,7 +308,7 @@ int yb_set_launch_config(yb_sim* s, int sweepThreads, int queryThreads);
 int yb_get_launch_config(yb_sim* s, int* sweepThreads, int* queryThreads);
 
 /* Launch mode.  on = 1 (default): each substep's kernels run as captured HIP
- * graphs (step.cpp:36-59 likewise captures the sweep chain).  on = 0: the same
+ * graphs (step.cpp:5-34 likewise captures the sweep chain).  on = 0: the same
  * kernels are enqueued one by one on the handle's stream -- same results,
  * more host launch overhead; for tools that cannot trace graph launches. */
 int yb_set_graphs(yb_sim* s, int on);

@@ -589,7 +589,7 @@ SegTerms seg_terms(const orc_sim* o, const yb_params* p, real damping, int s,
 
 }  // namespace
 
-// One substep (step.cpp:111-116 + the captured graph of step.cpp:79-86).
+// One substep (step.cpp:46-51 + the captured graph of step.cpp:14-21).
 extern "C" int orc_substep(orc_sim* o, const yb_params* p, int doDetect) {
     const int n = o->n;
     const real h = p->h;

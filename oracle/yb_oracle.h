@@ -13,7 +13,7 @@
  *     cosseratItr           KittenEngine/YarnBall/sim/cosserat.cu:20-207
  *     quaternionLambdaItr   KittenEngine/YarnBall/sim/cosserat.cu:209-253
  *     endItr                KittenEngine/YarnBall/sim/iteration.cu:48-61
- *     advance schedule      KittenEngine/YarnBall/sim/step.cpp:101-120
+ *     advance schedule      KittenEngine/YarnBall/sim/step.cpp:36-55
  *     configure/setK*       KittenEngine/YarnBall/YarnBall.cu:91-224
  *   discrete stages in FP32, same operation order as the device (bit-exact):
  *     warm start, padded boxes (collision.cu:10-37), Morton codes, stable sort,

@@ -1,7 +1,7 @@
 // capi.hip — extern "C" entry points of libyarnball_hip.so: device lifecycle,
 // substep scheduling with captured HIP graphs, error reporting.
 //
-// Scheduling mirrors Sim::advance (step.cpp:101-120): a frame of length h is
+// Scheduling mirrors Sim::advance (step.cpp:36-55): a frame of length h is
 // split into ceil(h / maxH) substeps; each substep is initItr, optional contact
 // detection every detectionPeriod substeps (full tree rebuild when
 // bvhRebuildPeriod of simulated time has passed, refit otherwise,
@@ -539,7 +539,7 @@ static void mark(yb_sim* s, int stage) {
 }
 
 // Substep prologue in detection mode m (0 none, 1 refit, 2 rebuild):
-// initItr, detectCollisions, recomputeStepLimit (step.cpp:111-116, 80-81).
+// initItr, detectCollisions, recomputeStepLimit (step.cpp:46-51, 15-16).
 // warm: the warm start of this prologue (0 plain, 1 learned, 2 temporal); a
 // plain prologue of a learned session still writes the plain start for the
 // end-of-substep correction that the next learned substep reads.
@@ -654,7 +654,7 @@ static int capture(yb_sim* s, hipGraphExec_t* out, const std::function<void()>& 
 }
 
 // Fixed-sweep mode: one graph per detection mode holds the whole substep
-// (the reference's stepGraph plus initItr and detection, step.cpp:70-99).
+// (the reference's stepGraph plus initItr and detection, step.cpp:5-34).
 static int rebuild_graphs(yb_sim* s, int numItr) {
     if (s->graphItr == numItr && s->graphs[0]) return YB_OK;
     s->graphItr = -1;
@@ -923,7 +923,7 @@ static int check_errors(yb_sim* s, int* warning) {
 }
 
 // Host-side decision of the detection mode for the current substep
-// (step.cpp:113-114, collision.cu:101-108).
+// (step.cpp:48-49, collision.cu:101-108).
 static int next_mode(yb_sim* s, const yb_params* p) {
     if (!(p->detectionPeriod > 0 && s->stepCounter % p->detectionPeriod == 0)) return 0;
     if (!s->treeBuilt || s->lastBVHRebuild >= p->bvhRebuildPeriod) {
@@ -941,7 +941,7 @@ extern "C" int yb_advance(yb_sim* s, yb_params* p, float h, float maxH, int* war
         set_message("yb_advance: null argument");
         return YB_ERR_INVALID_ARGUMENT;
     }
-    if (!(h > 0)) return YB_OK;  // step.cpp:102
+    if (!(h > 0)) return YB_OK;  // step.cpp:37
     if (!(maxH > 0)) {
         set_message("yb_advance: maxH must be positive");
         return YB_ERR_INVALID_ARGUMENT;

@@ -93,7 +93,7 @@ void Sim::zeroVelocities() {
     if (rc != YB_OK) raise(rc, "zeroVelocities");
 }
 
-// Sim::advance (step.cpp:101-120) + checkErrors (YarnBall.cu:276-304).
+// Sim::advance (step.cpp:36-55) + checkErrors (YarnBall.cu:276-304).
 // Returns the simulated time advanced.
 float Sim::advance(float h) {
     if (h <= 0) return 0;
@@ -122,7 +122,7 @@ float Sim::advance(float h) {
     return h;
 }
 
-// Sim::step (step.cpp:122-124): one maxH step; dt is ignored as in the reference.
+// Sim::step (step.cpp:57-59): one maxH step; dt is ignored as in the reference.
 void Sim::step(float) { advance(maxH); }
 
 // Sim::printCollisionStats (statistics.cpp:5-24).

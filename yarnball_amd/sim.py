@@ -106,7 +106,7 @@ class Sim:
 
     # -- stepping --------------------------------------------------------------
     def advance(self, dt):
-        """Sim::advance (step.cpp:101-120) + checkErrors (YarnBall.cu:276-304).
+        """Sim::advance (step.cpp:36-55) + checkErrors (YarnBall.cu:276-304).
 
         Returns the simulated time advanced."""
         if dt <= 0:
@@ -129,7 +129,7 @@ class Sim:
         return float(dt)
 
     def step(self, dt=None):
-        """Sim::step (step.cpp:122-124): one maxH step."""
+        """Sim::step (step.cpp:57-59): one maxH step."""
         self.advance(self.maxH)
 
     def stepCount(self):
