@@ -32,6 +32,15 @@ SIGS = {
     "orc_last_residual": (ctypes.c_double, [_P]),
     "orc_set_solver": (None, [_P, ctypes.c_double, _I]),
     "orc_last_sweeps": (_I, [_P]),
+    "orc_last_force_norm0": (ctypes.c_double, [_P]),
+    "orc_last_newton_residual": (ctypes.c_double, [_P]),
+    "orc_last_update_residual": (ctypes.c_double, [_P]),
+    "orc_set_residual_kind": (None, [_P, _I]),
+    "orc_set_contacts": (None, [_P, _P, _P]),
+    "orc_set_iterate": (None, [_P, _P, _P, _P]),
+    "orc_seg_terms": (None, [_P, _PP, _I, _P, _P]),
+    "orc_vertex_terms": (None, [_P, _PP, _I, _P, _P]),
+    "orc_force_norm2": (ctypes.c_double, [_P, _PP, _P]),
     "orc32_warm_start": (None, [_I, _PP, _P, _P, _P, _P, _P]),
     "orc32_aabb": (None, [_I, _PP, _P, _P, _P, _P]),
     "orc32_scene_box": (None, [_I, _P, _P]),
@@ -75,6 +84,12 @@ def _f64(a, shape):
     return np.ascontiguousarray(np.asarray(a, np.float64).reshape(shape))
 
 
+def _sym(a6):
+    """Packed symmetric {xx, yy, zz, xy, xz, yz} -> 3x3."""
+    xx, yy, zz, xy, xz, yz = a6
+    return np.array([[xx, xy, xz], [xy, yy, yz], [xz, yz, zz]])
+
+
 class Oracle:
     """FP64 oracle holding one scene's state (fp32=True: the float build)."""
 
@@ -109,6 +124,12 @@ class Oracle:
         self.lib.orc_store64(self.o, ptr(pos), ptr(vel), ptr(lv), ptr(q))
         return pos, vel, lv, q
 
+    def load64(self, pos=None, vels=None, lastVels=None, qs=None):
+        """Overwrite parts of the FP64 state (None keeps an array)."""
+        arrs = [None if a is None else _f64(a, (self.n, k))
+                for a, k in ((pos, 3), (vels, 3), (lastVels, 3), (qs, 4))]
+        self.lib.orc_load64(self.o, *[None if a is None else ptr(a) for a in arrs])
+
     def set_step_counter(self, c):
         self.lib.orc_set_step_counter(self.o, int(c))
 
@@ -137,7 +158,48 @@ class Oracle:
         return int(self.lib.orc_last_sweeps(self.o))
 
     def residual(self):
+        """Stopping residual of the last sweep of the last substep."""
         return float(self.lib.orc_last_residual(self.o))
+
+    def newton_residual(self):
+        """|f(dx_k)| / |f(dx_0)| of the last sweep's input iterate."""
+        return float(self.lib.orc_last_newton_residual(self.o))
+
+    def update_residual(self):
+        return float(self.lib.orc_last_update_residual(self.o))
+
+    def set_residual_kind(self, kind):
+        """0: stop on the update residual, 1: on the Newton residual."""
+        self.lib.orc_set_residual_kind(self.o, int(kind))
+
+    # -- the sweep's own functions (closed-form and finite-difference tests) --
+    def set_contacts(self, numCols, cols):
+        """numCols (n,), cols (128, n) slot-major."""
+        self.lib.orc_set_contacts(self.o, ptr(np.ascontiguousarray(numCols, np.int32)),
+                                  ptr(np.ascontiguousarray(cols, np.int32)))
+
+    def set_iterate(self, dx=None, y=None, step_limit=None):
+        def f(a, shape):
+            return None if a is None else ptr(_f64(a, shape))
+        self.lib.orc_set_iterate(self.o, f(dx, (self.n, 3)), f(y, (self.n, 3)),
+                                 f(step_limit, (self.n,)))
+
+    def seg_terms(self, params, s, dx):
+        out = np.zeros(18)
+        self.lib.orc_seg_terms(self.o, ctypes.byref(params), int(s), ptr(_f64(dx, (self.n, 3))),
+                               ptr(out))
+        return {"f0": out[0:3], "f1": out[3:6], "H0": _sym(out[6:12]), "H1": _sym(out[12:18])}
+
+    def vertex_terms(self, params, v, dx):
+        out = np.zeros(16)
+        self.lib.orc_vertex_terms(self.o, ctypes.byref(params), int(v),
+                                  ptr(_f64(dx, (self.n, 3))), ptr(out))
+        return {"f": out[0:3], "H": _sym(out[3:9]), "limit": out[9], "next": out[10:13],
+                "resid": out[13:16]}
+
+    def force_norm2(self, params, dx):
+        return float(self.lib.orc_force_norm2(self.o, ctypes.byref(params),
+                                              ptr(_f64(dx, (self.n, 3)))))
 
     def close(self):
         if self.o:

@@ -396,6 +396,9 @@ struct orc_sim {
     std::vector<real> y, dx, dxNext, qNext, stepLimit, dxWarm;
     int64_t stepCounter = 0;
     double lastResidual = 0.0;
+    double forceNorm0 = 0.0;  // |f|^2 at the plain warm start of the last substep
+    double lastForceResidual = 0.0, lastUpdateResidual = 0.0;
+    int kind = 0;  // stopping residual: 0 update, 1 Newton
     double tol = 0.0;     // > 0: sweep until the relative residual <= tol
     int maxSweeps = 0;
     int lastSweeps = 0;
@@ -587,6 +590,86 @@ SegTerms seg_terms(const orc_sim* o, const yb_params* p, real damping, int s,
     return t;
 }
 
+// Newton terms of vertex v at iterate dx: inertia (cosserat.cu:40-44), glue
+// spring (cosserat.cu:47-53), endpoint-0 terms of segment v and endpoint-1
+// terms of segment v-1 (cosserat.cu:63-193), and the step limit of the two
+// segments (cosserat.cu:69,190).  seg holds seg_terms of every segment.
+struct VertTerms {
+    D3 f;
+    S3 H;
+    real limit;
+};
+
+VertTerms vertex_terms(const orc_sim* o, real h, real damping, int v,
+                       const std::vector<SegTerms>& seg, const std::vector<real>& dx) {
+    VertTerms t{};
+    const D3 d = g3(dx, v);
+    const real mh2 = 1.0 / (o->invMass[v] * h * h);
+    s3_diag(t.H, mh2);
+    t.f = mul(mh2, sub(g3(o->y, v), d));
+    const int c = o->cid[v];
+    if (c >= 0) {
+        const real st = 40.0;
+        t.f = sub(t.f, mul(st, add(add(sub(g3(o->pos, v), g3(o->pos, c)), sub(d, g3(dx, c))),
+                                   mul(damping, d))));
+        s3_diag(t.H, st * (1.0 + damping));
+    }
+    t.limit = INFINITY;
+    if (has(o->flags[v], YB_FLAG_HAS_NEXT)) {
+        t.f = add(t.f, seg[v].f0);
+        for (int k = 0; k < 6; k++) t.H.a[k] += seg[v].H0.a[k];
+        t.limit = o->stepLimit[v];
+    }
+    if (has(o->flags[v], YB_FLAG_HAS_PREV)) {
+        t.f = add(t.f, seg[v - 1].f1);
+        for (int k = 0; k < 6; k++) t.H.a[k] += seg[v - 1].H1.a[k];
+        t.limit = std::min(t.limit, o->stepLimit[v - 1]);
+    }
+    return t;
+}
+
+inline D3 s3_apply(const S3& m, D3 x) {
+    return {m.a[0] * x.x + m.a[3] * x.y + m.a[4] * x.z, m.a[3] * x.x + m.a[1] * x.y + m.a[5] * x.z,
+            m.a[4] * x.x + m.a[5] * x.y + m.a[2] * x.z};
+}
+
+// Local Newton step of vertex v (cosserat.cu:195-205): the new iterate, and
+// the vertex's residual force -- f itself, or, when the step limit shortens
+// the step, H (dx' - dx) / accelerationRatio: the part of f the limit does
+// not absorb, which vanishes at a clamped fixed point of the sweeps.
+inline D3 local_step(const VertTerms& t, D3 d, real accel, D3& resid) {
+    D3 nd = add(d, mul(accel, s3_solve(t.H, t.f)));
+    const real l = len(nd);
+    resid = t.f;
+    if (l > t.limit && l > 0) {
+        nd = mulr(nd, t.limit / l);
+        resid = mulr(s3_apply(t.H, sub(nd, d)), real(1.0) / accel);
+    }
+    return nd;
+}
+
+// Sum over free vertices of |residual force|^2 at iterate dx (contacts, y and
+// the step limits of the current substep).
+double force_norm2(const orc_sim* o, const yb_params* p, const std::vector<real>& dx) {
+    const int n = o->n;
+    const real h = p->h;
+    const real damping = (real)p->damping / h;
+    std::vector<SegTerms> seg(n);
+#pragma omp parallel for schedule(static)
+    for (int s = 0; s < n; s++)
+        if (has(o->flags[s], YB_FLAG_HAS_NEXT)) seg[s] = seg_terms(o, p, damping, s, dx);
+    double acc = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : acc)
+    for (int v = 0; v < n; v++) {
+        if (o->invMass[v] == 0.0) continue;
+        const VertTerms t = vertex_terms(o, h, damping, v, seg, dx);
+        D3 r;
+        local_step(t, g3(dx, v), (real)p->accelerationRatio, r);
+        acc += (double)len2(r);
+    }
+    return acc;
+}
+
 }  // namespace
 
 // One substep (step.cpp:46-51 + the captured graph of step.cpp:14-21).
@@ -660,58 +743,45 @@ extern "C" int orc_substep(orc_sim* o, const yb_params* p, int doDetect) {
     }
 
     // numItr strict-Jacobi sweeps: dx then frames (cosserat.cu:20-253).
+    // Sweep k measures two residuals:
+    //   update  r_U = sqrt( sum_free |dx_k+1 - dx_k|^2 / sum_free l_i^2 ),
+    //   Newton  r_F = |f(dx_k)| / |f(dx_0)| (2-norms over free vertices, dx_0
+    //           the plain warm start, f the residual force of local_step);
+    // with a tolerance the substep stops after the first sweep whose stopping
+    // residual (o->kind) is <= tol; that sweep's output is kept.
     std::vector<SegTerms> seg(n);
-    double num = 0.0, den = 0.0;
+    double numU = 0.0, denU = 0.0, numF = 0.0;
     const bool adaptive = o->tol > 0;
     const int itMax = adaptive ? o->maxSweeps : p->numItr;
+    const real accel = (real)p->accelerationRatio;
     int used = 0;
+    o->lastResidual = 0.0;
+    o->lastForceResidual = 0.0;
     for (int it = 0; it < itMax; it++) {
         used = it + 1;
 #pragma omp parallel for schedule(static)
         for (int s = 0; s < n; s++)
             if (has(o->flags[s], YB_FLAG_HAS_NEXT)) seg[s] = seg_terms(o, p, damping, s, o->dx);
-        num = 0.0;
-        den = 0.0;
-#pragma omp parallel for schedule(static) reduction(+ : num, den)
+        numU = denU = numF = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : numU, denU, numF)
         for (int v = 0; v < n; v++) {
             const D3 d = g3(o->dx, v);
             if (o->invMass[v] == 0.0) {
                 s3(o->dxNext, v, d);
                 continue;
             }
-            const real mh2 = 1.0 / (o->invMass[v] * h * h);
-            S3 H{};
-            s3_diag(H, mh2);
-            D3 f = mul(mh2, sub(g3(o->y, v), d));
-            const int c = o->cid[v];
-            if (c >= 0) {
-                const real st = 40.0;
-                f = sub(f, mul(st, add(add(sub(g3(o->pos, v), g3(o->pos, c)),
-                                           sub(d, g3(o->dx, c))),
-                                       mul(damping, d))));
-                s3_diag(H, st * (1.0 + damping));
-            }
-            real limit = INFINITY;
-            if (has(o->flags[v], YB_FLAG_HAS_NEXT)) {
-                f = add(f, seg[v].f0);
-                for (int k = 0; k < 6; k++) H.a[k] += seg[v].H0.a[k];
-                limit = o->stepLimit[v];
-            }
-            if (has(o->flags[v], YB_FLAG_HAS_PREV)) {
-                f = add(f, seg[v - 1].f1);
-                for (int k = 0; k < 6; k++) H.a[k] += seg[v - 1].H1.a[k];
-                limit = std::min(limit, o->stepLimit[v - 1]);
-            }
-            D3 nd = add(d, mul((real)p->accelerationRatio, s3_solve(H, f)));
-            const real l = len(nd);
-            if (l > limit && l > 0) nd = mulr(nd, limit / l);
+            const VertTerms t = vertex_terms(o, h, damping, v, seg, o->dx);
+            D3 r;
+            const D3 nd = local_step(t, d, accel, r);
             s3(o->dxNext, v, nd);
-            num += len2(sub(nd, d));
+            numU += (double)len2(sub(nd, d));
+            numF += (double)len2(r);
             real li = o->lRest[v];
             if (!has(o->flags[v], YB_FLAG_HAS_NEXT) && has(o->flags[v], YB_FLAG_HAS_PREV))
                 li = o->lRest[v - 1];
-            den += li * li;
+            denU += (double)(li * li);
         }
+        if (it == 0) o->forceNorm0 = numF;
         std::swap(o->dx, o->dxNext);
 
         // quaternionLambdaItr with the new dx and the previous frames.
@@ -745,9 +815,9 @@ extern "C" int orc_substep(orc_sim* o, const yb_params* p, int doDetect) {
             s4(o->qNext, v, q0);
         }
         std::swap(o->q, o->qNext);
-        // Relative Newton residual of this sweep: RMS |dx_k+1 - dx_k| over free
-        // vertices in units of the rest length.
-        o->lastResidual = den > 0 ? std::sqrt(num / den) : 0.0;
+        o->lastForceResidual = o->forceNorm0 > 0 ? std::sqrt(numF / o->forceNorm0) : 0.0;
+        o->lastUpdateResidual = denU > 0 ? std::sqrt(numU / denU) : 0.0;
+        o->lastResidual = o->kind ? o->lastForceResidual : o->lastUpdateResidual;
         if (adaptive && o->lastResidual <= o->tol) break;
     }
     o->lastSweeps = used;
@@ -794,6 +864,81 @@ extern "C" void orc_last_prologue(const orc_sim* o, double* stepLimit, double* d
 }
 
 extern "C" double orc_last_residual(const orc_sim* o) { return o->lastResidual; }
+extern "C" double orc_last_force_norm0(const orc_sim* o) { return o->forceNorm0; }
+extern "C" double orc_last_newton_residual(const orc_sim* o) { return o->lastForceResidual; }
+extern "C" double orc_last_update_residual(const orc_sim* o) { return o->lastUpdateResidual; }
+extern "C" void orc_set_residual_kind(orc_sim* o, int kind) { o->kind = kind ? 1 : 0; }
+
+// ---- access to the sweep's own functions, for the closed-form tests ----------
+extern "C" void orc_set_contacts(orc_sim* o, const int32_t* numCols, const int32_t* cols) {
+    const size_t N = (size_t)o->n;
+    for (size_t i = 0; i < N; i++) {
+        o->numCols[i] = std::min(std::max(numCols[i], 0), YB_MAX_COLLISIONS_PER_SEGMENT);
+        for (int k = 0; k < o->numCols[i]; k++) o->cols[i + N * k] = cols[i + N * k];
+    }
+}
+
+static std::vector<real> vec_of(const double* a, size_t count) {
+    std::vector<real> v(count);
+    for (size_t k = 0; k < count; k++) v[k] = (real)a[k];
+    return v;
+}
+
+extern "C" void orc_set_iterate(orc_sim* o, const double* dx, const double* y,
+                                const double* stepLimit) {
+    const size_t N = (size_t)o->n;
+    if (dx) o->dx = vec_of(dx, 3 * N);
+    if (y) o->y = vec_of(y, 3 * N);
+    if (stepLimit) o->stepLimit = vec_of(stepLimit, N);
+}
+
+// out: f0[3], f1[3], H0[6], H1[6] of segment s at iterate dx (3n).
+extern "C" void orc_seg_terms(const orc_sim* o, const yb_params* p, int s, const double* dx,
+                              double* out) {
+    const std::vector<real> d = vec_of(dx, 3 * (size_t)o->n);
+    const SegTerms t = seg_terms(o, p, (real)p->damping / (real)p->h, s, d);
+    const D3 v[2] = {t.f0, t.f1};
+    for (int k = 0; k < 2; k++) {
+        out[3 * k] = v[k].x;
+        out[3 * k + 1] = v[k].y;
+        out[3 * k + 2] = v[k].z;
+    }
+    for (int k = 0; k < 6; k++) {
+        out[6 + k] = t.H0.a[k];
+        out[12 + k] = t.H1.a[k];
+    }
+}
+
+// out: f[3], H[6], step limit, new iterate[3], residual force[3] of vertex v
+// (free) at iterate dx (3n), with the contacts, y and step limits set.
+extern "C" void orc_vertex_terms(const orc_sim* o, const yb_params* p, int v, const double* dx,
+                                 double* out) {
+    const int n = o->n;
+    const std::vector<real> d = vec_of(dx, 3 * (size_t)n);
+    const real damping = (real)p->damping / (real)p->h;
+    std::vector<SegTerms> seg(n);
+    for (int s = std::max(v - 1, 0); s <= v; s++)
+        if (has(o->flags[s], YB_FLAG_HAS_NEXT)) seg[s] = seg_terms(o, p, damping, s, d);
+    const VertTerms t = vertex_terms(o, (real)p->h, damping, v, seg, d);
+    D3 r;
+    const D3 nd = local_step(t, g3(d, v), (real)p->accelerationRatio, r);
+    const D3 w[4] = {t.f, nd, r, nd};
+    out[0] = t.f.x;
+    out[1] = t.f.y;
+    out[2] = t.f.z;
+    for (int k = 0; k < 6; k++) out[3 + k] = t.H.a[k];
+    out[9] = t.limit;
+    for (int k = 1; k < 3; k++) {
+        out[10 + 3 * (k - 1)] = w[k].x;
+        out[11 + 3 * (k - 1)] = w[k].y;
+        out[12 + 3 * (k - 1)] = w[k].z;
+    }
+}
+
+// Sum over free vertices of |residual force|^2 at iterate dx (3n).
+extern "C" double orc_force_norm2(const orc_sim* o, const yb_params* p, const double* dx) {
+    return force_norm2(o, p, vec_of(dx, 3 * (size_t)o->n));
+}
 
 #ifndef ORC_FP32
 // ============================================================================

@@ -55,13 +55,39 @@ int orc_advance(orc_sim* o, yb_params* p, float h, float maxH);
 void orc_contacts(const orc_sim* o, int32_t* numCols, int32_t* cols);
 /* Step limits (per segment) and warm-start dx (3n) of the last substep. */
 void orc_last_prologue(const orc_sim* o, double* stepLimit, double* dxWarm);
-/* Residual-targeted mode (tol > 0): sweeps until
- * sqrt(sum_free |dx_k+1 - dx_k|^2 / sum_free l_i^2) <= tol or maxSweeps;
- * tol <= 0 runs p->numItr sweeps. */
+/* Residual-targeted mode (tol > 0): sweeps until the stopping residual of a
+ * sweep is <= tol (that sweep's output is kept) or maxSweeps sweeps ran;
+ * tol <= 0 runs p->numItr sweeps.  Residuals of sweep k:
+ *   update (kind 0, default)  sqrt(sum_free |dx_k+1 - dx_k|^2 / sum_free l_i^2),
+ *   Newton (kind 1)  |f(dx_k)| / |f(dx_0)|, 2-norms over free vertices, dx_0 the
+ *     plain warm start, f the local Newton step's force -- at a vertex the step
+ *     limit holds, H (dx' - dx) / accelerationRatio instead, the part of f the
+ *     limit does not absorb. */
 void orc_set_solver(orc_sim* o, double tol, int maxSweeps);
+void orc_set_residual_kind(orc_sim* o, int kind);
 int orc_last_sweeps(const orc_sim* o);
-/* Relative Newton residual of the last substep's final iterate (see .cpp). */
+/* Residuals of the last sweep of the last substep (stopping one, update,
+ * Newton), and |f(dx_0)|^2 of that substep. */
 double orc_last_residual(const orc_sim* o);
+double orc_last_update_residual(const orc_sim* o);
+double orc_last_newton_residual(const orc_sim* o);
+double orc_last_force_norm0(const orc_sim* o);
+
+/* ---- the sweep's own functions, for closed-form and finite-difference tests ----
+ * orc_set_contacts: contact lists (slot-major, numCols n, cols 128n).
+ * orc_set_iterate: iterate dx (3n), inertial target y - x (3n), step limits (n);
+ *   null skips an array.
+ * orc_seg_terms: segment s's endpoint terms at dx (3n): f0[3] f1[3] H0[6] H1[6]
+ *   (H packed xx yy zz xy xz yz).
+ * orc_vertex_terms: free vertex v's assembled f[3], H[6], step limit, new
+ *   iterate[3] and residual force[3] at dx.
+ * orc_force_norm2: sum over free vertices of |residual force|^2 at dx. */
+void orc_set_contacts(orc_sim* o, const int32_t* numCols, const int32_t* cols);
+void orc_set_iterate(orc_sim* o, const double* dx, const double* y, const double* stepLimit);
+void orc_seg_terms(const orc_sim* o, const yb_params* p, int s, const double* dx, double* out);
+void orc_vertex_terms(const orc_sim* o, const yb_params* p, int v, const double* dx,
+                      double* out);
+double orc_force_norm2(const orc_sim* o, const yb_params* p, const double* dx);
 
 /* ---- FP32 twin of the discrete stages (bit-exact with the device) ---- */
 void orc32_warm_start(int n, const yb_params* p, const float* invMass, const float* vel3,

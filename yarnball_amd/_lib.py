@@ -52,6 +52,9 @@ class SolverStats(ctypes.Structure):
         ("notConverged", ctypes.c_int32),
         ("lastResidual", ctypes.c_float),
         ("maxResidual", ctypes.c_float),
+        ("lastNewtonResidual", ctypes.c_float),
+        ("maxNewtonResidual", ctypes.c_float),
+        ("sumNewtonResidual", ctypes.c_double),
     ]
 
 
@@ -134,6 +137,7 @@ SIGNATURES = {
     "yb_debug_read_mlp": (_I, [_P, _P, _P]),
     "yb_profile": (_I, [_P, ctypes.POINTER(Params), _F, _I, _P, _P]),
     "yb_set_solver": (_I, [_P, _F, _I, _I, _I]),
+    "yb_set_residual_kind": (_I, [_P, _I]),
     "yb_min_separation": (_I, [_P, _P, _I]),
     "yb_mlp_load": (_I, [_P, _P, _I]),
     "yb_set_warm_start": (_I, [_P, _I]),

@@ -257,8 +257,8 @@ extern "C" int yb_upload(yb_sim* s, const yb_vertex* verts, const float* vels, c
     HIPCHECK(hipMemcpyAsync(st, qRests, sizeof(float) * 4 * N, hipMemcpyHostToDevice, s->stream));
     launch_upload(s);
     HIPCHECK(hipGetLastError());
-    // Residual normaliser: sum over free vertices of l_i^2, l_i the rest length
-    // of the vertex's own segment (or of the previous one at a strand end).
+    // Update-residual normaliser: sum over free vertices of l_i^2, l_i the rest
+    // length of the vertex's own segment (or of the previous one at a strand end).
     double denom = 0.0;
     for (size_t i = 0; i < N; i++) {
         if (verts[i].invMass == 0.f) continue;
@@ -560,6 +560,17 @@ static void enqueue_prologue(yb_sim* s, int mode, bool adaptive, int warm) {
     }
     launch_step_limit(s);
     mark(s, YB_STAGE_STEP_LIMIT);
+    // Residual-targeted mode: the relative residual's denominator is the
+    // force at the plain warm start (now in dx[0]) for every warm start, so
+    // that all of them are held to the same target.  A plain start without a
+    // strand step gets it from sweep 0; otherwise it is taken here, before the
+    // start is changed (a strand step on the plain start computes the same
+    // forces and writes the partials itself).
+    if (adaptive) {
+        launch_ctl_reset(s);
+        if (warm != 0) launch_residual_denom(s, false);
+        mark(s, YB_STAGE_RESIDUAL);
+    }
     const bool learned = warm == 1, record = s->recCap > 0;
     if (learned || record || s->warmStart == 1) {
         // the inference kernel computes its own features; the recording
@@ -572,23 +583,22 @@ static void enqueue_prologue(yb_sim* s, int mode, bool adaptive, int warm) {
         mark(s, YB_STAGE_MLP);
     }
     if (s->strandSolve) {
-        launch_strand_solve(s);
+        const bool denom = adaptive && warm == 0;
+        launch_strand_solve(s, 0, nullptr, 0, nullptr, denom);
+        if (denom) launch_residual_denom(s, true);
         mark(s, YB_STAGE_MLP);
     }
     if ((warm != 0 || s->strandSolve) && s->frameRefresh) {
         launch_frame_refresh(s);
         mark(s, YB_STAGE_MLP);
     }
-    if (adaptive) {
-        launch_ctl_reset(s);
-        mark(s, YB_STAGE_RESIDUAL);
-    }
 }
 
 // In residual-targeted mode the residual is evaluated after every
-// s->checkEvery-th sweep (and after the last allowed one).
+// s->checkEvery-th sweep, after the last allowed one, and after sweep 0 (whose
+// input is the plain warm start of a plain substep: the denominator).
 static bool checks_residual(const yb_sim* s, int k) {
-    return (k + 1) % s->checkEvery == 0 || k == s->maxSweeps - 1;
+    return k == 0 || (k + 1) % s->checkEvery == 0 || k == s->maxSweeps - 1;
 }
 
 // The residual of a check sweep is reduced, and convergence decided, by the
@@ -736,6 +746,18 @@ extern "C" int yb_set_solver(yb_sim* s, float residualTol, int maxSweeps, int ch
     return YB_OK;
 }
 
+extern "C" int yb_set_residual_kind(yb_sim* s, int kind) {
+    if (!s || (kind != YB_RESIDUAL_UPDATE && kind != YB_RESIDUAL_NEWTON)) {
+        set_message("yb_set_residual_kind: YB_RESIDUAL_UPDATE or YB_RESIDUAL_NEWTON");
+        return YB_ERR_INVALID_ARGUMENT;
+    }
+    s->residualKind = kind;
+    const int32_t k = kind;
+    HIPCHECK(hipMemcpyAsync(&s->b.ctl->kind, &k, sizeof(k), hipMemcpyHostToDevice, s->stream));
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    return YB_OK;
+}
+
 extern "C" int yb_set_strand_schedule(yb_sim* s, int every, int pairIters, int first) {
     if (!s || every < 0 || pairIters < 1 || pairIters > 64 || first < 0) {
         set_message("yb_set_strand_schedule: every >= 0, 1 <= pairIters <= 64, first >= 0");
@@ -795,11 +817,16 @@ extern "C" int yb_solver_stats(yb_sim* s, yb_solver_stats_t* out, int reset) {
     out->notConverged = c.notConverged;
     out->lastResidual = c.resid;
     out->maxResidual = c.maxResid;
+    out->lastNewtonResidual = c.fres;
+    out->maxNewtonResidual = c.maxFres;
+    out->sumNewtonResidual = c.sumFres;
     if (reset) {
         SolverCtl z;
         memset(&z, 0, sizeof(z));
         z.doneAt = 0x7fffffff;
         z.denom = s->denomHost;
+        z.denomF = -1.0;
+        z.kind = s->residualKind;
         HIPCHECK(hipMemcpyAsync(s->b.ctl, &z, sizeof(z), hipMemcpyHostToDevice, s->stream));
         HIPCHECK(hipStreamSynchronize(s->stream));
     }

@@ -120,9 +120,10 @@ __global__ __launch_bounds__(256) void k_step_limit(const DevParams* __restrict_
     reduce_min_sep(minContact / (2.0f * P->radius), minSep);
 }
 
-__device__ __forceinline__ void wave_residual(float num, double* partial, int wave, int n);
+__device__ __forceinline__ void wave_residual(float numU, float numF, double2* partial, int wave,
+                                              int n);
 __device__ __forceinline__ void residual_wave(SolverCtl* __restrict__ ctl,
-                                              const double* __restrict__ partial, int nb,
+                                              const double2* __restrict__ partial, int nb,
                                               int sweep, float tol, int maxSweeps);
 
 // Closest points of segment e = (0 -> e) and (B0 -> B0 + A1), specialised from
@@ -204,10 +205,15 @@ __device__ __forceinline__ void pcr64(float& A, float& Lo, float& Up, float B[5]
 // dx'(i+1); it also supplies q(i+1).  All exchange is cross-lane shifts --
 // no LDS, no barrier (the reference uses a 2-sector LDS reduction).
 //
-// kLine: the strand solve's tile part.  The same local systems are assembled,
-// then solved along the tile's strands instead of vertex by vertex (the line
-// tail below); nothing of the sweep's own output is written.
-template <bool kLine>
+// Modes: kSweep -- the sweep.  kLine -- the strand solve's tile part: the
+// same local systems are assembled, then solved along the tile's strands
+// instead of vertex by vertex (the line tail below); nothing of the sweep's own
+// output is written.  kEval -- only the residual partials of the input iterate.
+// Every mode writes, when `partial` is set, the per-wave sums of the squared
+// update |dx' - dx|^2 (kSweep) and of the squared residual force |r|^2 at its
+// input iterate (r: see the Newton step below).
+constexpr int kSweep = 0, kLine = 1, kEval = 2;
+template <int kMode>
 __device__ __forceinline__ void sweep_body(const DevParams* __restrict__ P,
                                            const float4* __restrict__ pos,
                                            const float4* __restrict__ prm,
@@ -220,8 +226,8 @@ __device__ __forceinline__ void sweep_body(const DevParams* __restrict__ P,
                                            const int32_t* __restrict__ numCols,
                                            const int32_t* __restrict__ cols,
                                            SolverCtl* __restrict__ ctl, int sweep,
-                                           double* __restrict__ partial,
-                                           const double* __restrict__ partialPrev,
+                                           double2* __restrict__ partial,
+                                           const double2* __restrict__ partialPrev,
                                            float tol, int maxSweeps, LineOut lo) {
     // Residual-targeted mode: sweeps after convergence do nothing.
     if (ctl && ctl->doneAt < sweep) return;
@@ -387,8 +393,10 @@ __device__ __forceinline__ void sweep_body(const DevParams* __restrict__ P,
 
     // Newton step.
     v3 dxNew = xyz(d4);
-    float num = 0.0f;  // |dx' - dx|^2 for the residual
+    float num = 0.0f;   // |dx' - dx|^2
+    float numF = 0.0f;  // |residual force|^2 of this vertex at the input iterate
     float limit = segLimit;
+    const bool owner = in && lane >= 1 && lane <= 62;
     if (in && lane != 0) {
         if (flags & YB_FLAG_HAS_PREV) {
             limit = fminf(limit, pl);
@@ -396,16 +404,28 @@ __device__ __forceinline__ void sweep_body(const DevParams* __restrict__ P,
 #pragma unroll
             for (int k = 0; k < 6; k++) H.a[k] += ph[k];
         }
-        if (!kLine && p4.w != 0.0f) {
-            // Local 3x3 solve (cosserat.cu:195-205).
+        if (p4.w != 0.0f && (kMode == kSweep || partial)) {
+            // Local 3x3 solve (cosserat.cu:195-205).  The residual force is f,
+            // or, where the step limit shortens the step, H (dx' - dx) /
+            // accelerationRatio: the part of f the limit does not absorb, which
+            // vanishes at a clamped fixed point of the sweeps.
             dxNew += P->accelerationRatio * sym_solve_fast(H, f);
             const float l = len(dxNew);
-            if (l > limit && l > 0.0f) dxNew *= limit / l;
+            v3 r = f;
+            if (l > limit && l > 0.0f) {
+                dxNew *= limit / l;
+                r = sym_apply(H, dxNew - xyz(d4)) * frcp(P->accelerationRatio);
+            }
             num = len2(dxNew - xyz(d4));
+            numF = len2(r);
         }
     }
+    if constexpr (kMode == kEval) {
+        wave_residual(0.0f, owner ? numF : 0.0f, partial, wave, n);
+        return;
+    }
 
-    if constexpr (kLine) {
+    if constexpr (kMode == kLine) {
         // Strand solve, tile part: the tile's owned vertices (lanes 1..62)
         // form a tridiagonal system along their strands -- on the diagonal a
         // scalar stiffness of each local Hessian, between neighbours the
@@ -455,6 +475,7 @@ __device__ __forceinline__ void sweep_body(const DevParams* __restrict__ P,
                           : make_float4(0.f, 0.f, 0.f, 0.f);
             lo.ql[v] = make_float2(row ? B[4] * ia : 0.0f, row ? limit : -1.0f);
         }
+        if (partial) wave_residual(0.0f, owner ? numF : 0.0f, partial, wave, n);
         return;
     }
 
@@ -464,7 +485,6 @@ __device__ __forceinline__ void sweep_body(const DevParams* __restrict__ P,
     const rot qNext = rot{__shfl_down(q4.x, 1, 64), __shfl_down(q4.y, 1, 64),
                           __shfl_down(q4.z, 1, 64), __shfl_down(q4.w, 1, 64)};
 
-    const bool owner = in && lane >= 1 && lane <= 62;
     if (owner) {
         dxOut[v] = f4(dxNew, d4.w);
         rot q0 = asrot(q4);
@@ -486,7 +506,7 @@ __device__ __forceinline__ void sweep_body(const DevParams* __restrict__ P,
         }
         qOut[v] = f4r(q0);
     }
-    if (partial) wave_residual(owner ? num : 0.0f, partial, wave, n);
+    if (partial) wave_residual(owner ? num : 0.0f, owner ? numF : 0.0f, partial, wave, n);
 }
 
 __global__ __launch_bounds__(512) void k_sweep(const DevParams* __restrict__ P,
@@ -501,11 +521,11 @@ __global__ __launch_bounds__(512) void k_sweep(const DevParams* __restrict__ P,
                                                const int32_t* __restrict__ numCols,
                                                const int32_t* __restrict__ cols,
                                                SolverCtl* __restrict__ ctl, int sweep,
-                                               double* __restrict__ partial,
-                                               const double* __restrict__ partialPrev,
+                                               double2* __restrict__ partial,
+                                               const double2* __restrict__ partialPrev,
                                                float tol, int maxSweeps) {
-    sweep_body<false>(P, pos, prm, yv, qIn, qOut, qRest, dxIn, dxOut, numCols, cols, ctl, sweep,
-                      partial, partialPrev, tol, maxSweeps, LineOut{nullptr, nullptr, nullptr});
+    sweep_body<kSweep>(P, pos, prm, yv, qIn, qOut, qRest, dxIn, dxOut, numCols, cols, ctl, sweep,
+                       partial, partialPrev, tol, maxSweeps, LineOut{nullptr, nullptr, nullptr});
 }
 
 // Strand solve, tile part (see the line mode of sweep_body).
@@ -518,10 +538,28 @@ __global__ __launch_bounds__(256) void k_line_tile(const DevParams* __restrict__
                                                    const float4* __restrict__ dxIn,
                                                    const int32_t* __restrict__ numCols,
                                                    const int32_t* __restrict__ cols, LineOut lo,
-                                                   const SolverCtl* __restrict__ ctl, int sweep) {
+                                                   const SolverCtl* __restrict__ ctl, int sweep,
+                                                   double2* __restrict__ partial) {
     if (strand_skip(ctl, sweep)) return;
-    sweep_body<true>(P, pos, prm, yv, qIn, nullptr, qRest, dxIn, nullptr, numCols, cols, nullptr, 0,
-                     nullptr, nullptr, 0.0f, 0, lo);
+    sweep_body<kLine>(P, pos, prm, yv, qIn, nullptr, qRest, dxIn, nullptr, numCols, cols, nullptr,
+                      0, partial, nullptr, 0.0f, 0, lo);
+}
+
+// Residual partials of an iterate without changing it: the denominator |f(dx0)|
+// of the relative residual when the sweeps do not start from the plain warm
+// start.
+__global__ __launch_bounds__(256) void k_force_eval(const DevParams* __restrict__ P,
+                                                    const float4* __restrict__ pos,
+                                                    const float4* __restrict__ prm,
+                                                    const float4* __restrict__ yv,
+                                                    const float4* __restrict__ qIn,
+                                                    const float4* __restrict__ qRest,
+                                                    const float4* __restrict__ dxIn,
+                                                    const int32_t* __restrict__ numCols,
+                                                    const int32_t* __restrict__ cols,
+                                                    double2* __restrict__ partial) {
+    sweep_body<kEval>(P, pos, prm, yv, qIn, nullptr, qRest, dxIn, nullptr, numCols, cols, nullptr,
+                      0, partial, nullptr, 0.0f, 0, LineOut{nullptr, nullptr, nullptr});
 }
 
 // Strand solve, boundary part: one iteration over the pairs of vertices that
@@ -654,41 +692,79 @@ __global__ __launch_bounds__(256) void k_line_apply(int n, int tiles,
     dx[v] = f4(nd, d4.w);
 }
 
-// Per-sweep relative Newton residual
-//   r = sqrt( sum_free |dx_k+1 - dx_k|^2 / sum_free l_i^2 ),
-// the RMS change of the iterate in units of the rest length (l_i: the rest
-// length of vertex i's own segment, or of the previous one at a strand end).
-// One partial sum per wave tile of 62 vertices (so the partials, and the
-// residual, do not depend on the block size), one small reduction in k_residual.
-__device__ __forceinline__ void wave_residual(float num, double* partial, int wave, int n) {
+// Residuals of a sweep, both from per-wave partial sums (one per wave tile of
+// 62 vertices, so the partials and the residuals do not depend on the block
+// size), reduced in a fixed order by one wave:
+//   update residual  r_U = sqrt( sum_free |dx_k+1 - dx_k|^2 / sum_free l_i^2 ),
+//     the RMS change of the iterate in units of the rest length (l_i: the rest
+//     length of vertex i's own segment, or of the previous one at a strand end);
+//   Newton residual  r_F = |f(dx_k)| / |f(dx_0)|, the residual force of the
+//     sweep's input iterate relative to that of the plain warm start dx_0.
+// The stopping test uses one of them (SolverCtl::kind); both are recorded.
+__device__ __forceinline__ void wave_residual(float numU, float numF, double2* partial, int wave,
+                                              int n) {
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) num += __shfl_xor(num, off, 64);
-    if ((threadIdx.x & 63) == 0 && wave < (n + 61) / 62) partial[wave] = (double)num;
+    for (int off = 32; off > 0; off >>= 1) {
+        numU += __shfl_xor(numU, off, 64);
+        numF += __shfl_xor(numF, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0 && wave < (n + 61) / 62)
+        partial[wave] = make_double2((double)numU, (double)numF);
 }
 
-// Residual of sweep `sweep` from its per-wave partials, and the stopping
-// decision, by one wave (64 lanes) in a fixed order of additions, so the value
-// does not depend on how the work is launched.  Eight running sums per lane
-// keep eight loads in flight (the reduction is latency-bound: one wave reads
-// 17k partials on the 1M-vertex knit).
-__device__ __forceinline__ void residual_wave(SolverCtl* __restrict__ ctl,
-                                              const double* __restrict__ partial, int nb,
-                                              int sweep, float tol, int maxSweeps) {
+// Sum of nb partials by one wave in a fixed order of additions.  Eight running
+// sums per lane keep eight loads in flight (the reduction is latency-bound: one
+// wave reads 17k partials on the 1M-vertex knit).
+__device__ __forceinline__ double2 reduce_partials(const double2* __restrict__ partial, int nb) {
     const int lane = threadIdx.x & 63;
-    double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double2 acc[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) acc[j] = make_double2(0.0, 0.0);
     int k = lane;
     for (; k + 7 * 64 < nb; k += 8 * 64) {
 #pragma unroll
-        for (int j = 0; j < 8; j++) acc[j] += partial[k + 64 * j];
+        for (int j = 0; j < 8; j++) {
+            const double2 t = partial[k + 64 * j];
+            acc[j].x += t.x;
+            acc[j].y += t.y;
+        }
     }
-    for (; k < nb; k += 64) acc[0] += partial[k];
-    double a = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+    for (; k < nb; k += 64) {
+        const double2 t = partial[k];
+        acc[0].x += t.x;
+        acc[0].y += t.y;
+    }
+    double a = ((acc[0].x + acc[1].x) + (acc[2].x + acc[3].x)) +
+               ((acc[4].x + acc[5].x) + (acc[6].x + acc[7].x));
+    double b = ((acc[0].y + acc[1].y) + (acc[2].y + acc[3].y)) +
+               ((acc[4].y + acc[5].y) + (acc[6].y + acc[7].y));
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
-    if (lane == 0) {
-        const double den = ctl->denom;
-        const float r = den > 0.0 ? (float)sqrt(a / den) : 0.0f;
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off, 64);
+        b += __shfl_xor(b, off, 64);
+    }
+    return make_double2(a, b);
+}
+
+// Residuals of sweep `sweep` and the stopping decision, by one wave.
+__device__ __forceinline__ void residual_wave(SolverCtl* __restrict__ ctl,
+                                              const double2* __restrict__ partial, int nb,
+                                              int sweep, float tol, int maxSweeps) {
+    const double2 a = reduce_partials(partial, nb);
+    if ((threadIdx.x & 63) == 0) {
+        // no Newton-residual denominator yet: this is sweep 0 of a plain start,
+        // whose input is the plain warm start itself
+        double denF = ctl->denomF;
+        if (denF < 0.0) {
+            denF = a.y;
+            ctl->denomF = a.y;
+        }
+        const double denU = ctl->denom;
+        const float rU = denU > 0.0 ? (float)sqrt(a.x / denU) : 0.0f;
+        const float rF = denF > 0.0 ? (float)sqrt(a.y / denF) : 0.0f;
+        const float r = ctl->kind ? rF : rU;
         ctl->resid = r;
+        ctl->fres = rF;
         if (r <= tol || sweep == maxSweeps - 1) {
             ctl->doneAt = sweep;
             ctl->fin = (sweep & 1) ^ 1;
@@ -697,6 +773,8 @@ __device__ __forceinline__ void residual_wave(SolverCtl* __restrict__ ctl,
             if (sweep + 1 > ctl->maxUsed) ctl->maxUsed = sweep + 1;
             if (r > tol) ctl->notConverged += 1;
             if (r > ctl->maxResid) ctl->maxResid = r;
+            if (rF > ctl->maxFres) ctl->maxFres = rF;
+            ctl->sumFres += (double)rF;
         }
     }
 }
@@ -704,14 +782,26 @@ __device__ __forceinline__ void residual_wave(SolverCtl* __restrict__ ctl,
 // Stand-alone check, used only for the last sweep a substep may run (every
 // other check is made by the following sweep's first wave, in k_sweep).
 __global__ __launch_bounds__(64) void k_residual(SolverCtl* __restrict__ ctl,
-                                                 const double* __restrict__ partial, int nb,
+                                                 const double2* __restrict__ partial, int nb,
                                                  int sweep, float tol, int maxSweeps) {
     if (ctl->doneAt < sweep) return;
     residual_wave(ctl, partial, nb, sweep, tol, maxSweeps);
 }
 
 __global__ void k_ctl_reset(SolverCtl* __restrict__ ctl) {
-    if (threadIdx.x == 0) ctl->doneAt = 0x7fffffff;
+    if (threadIdx.x == 0) {
+        ctl->doneAt = 0x7fffffff;
+        ctl->denomF = -1.0;
+    }
+}
+
+// Newton-residual denominator |f(dx_0)|^2 from the partials of a pass over the
+// plain warm start.
+__global__ __launch_bounds__(64) void k_residual_denom(SolverCtl* __restrict__ ctl,
+                                                       const double2* __restrict__ partial,
+                                                       int nb) {
+    const double2 a = reduce_partials(partial, nb);
+    if ((threadIdx.x & 63) == 0) ctl->denomF = a.y;
 }
 
 // ---------------------------------------------------------------------------
@@ -954,8 +1044,8 @@ void launch_sweep(yb_sim* s, int sweep, bool adaptive, bool partials, bool check
     const size_t nbw = (size_t)sweep_waves(s->n);
     // partials of sweep k go to half k & 1, so a sweep can reduce its
     // predecessor's while writing its own
-    double* part = s->b.partial + nbw * (sweep & 1);
-    const double* prev = s->b.partial + nbw * ((sweep & 1) ^ 1);
+    double2* part = s->b.partial + nbw * (sweep & 1);
+    const double2* prev = s->b.partial + nbw * ((sweep & 1) ^ 1);
     hipLaunchKernelGGL(k_sweep, dim3(blocks(sweep_waves(s->n) * 64, threads)), dim3(threads), 0,
                        s->stream, s->b.params,
                        s->b.pos, s->b.prm, s->b.yv, s->b.q[in], s->b.q[out], s->b.qRest,
@@ -977,12 +1067,14 @@ void launch_frame_refresh(yb_sim* s) {
                        s->b.q[0], s->n);
 }
 
-void launch_strand_solve(yb_sim* s, int buf, SolverCtl* ctl, int sweep, float* sys) {
+void launch_strand_solve(yb_sim* s, int buf, SolverCtl* ctl, int sweep, float* sys,
+                         bool denomPartials) {
     const int tiles = sweep_waves(s->n);
     const LineOut lo{s->b.lineG, s->b.lineQL, sys};
     hipLaunchKernelGGL(k_line_tile, dim3(blocks(tiles * 64, 256)), dim3(256), 0, s->stream,
                        s->b.params, s->b.pos, s->b.prm, s->b.yv, s->b.q[buf], s->b.qRest,
-                       s->b.dx[buf], s->b.numCols, s->b.cols, lo, ctl, sweep);
+                       s->b.dx[buf], s->b.numCols, s->b.cols, lo, ctl, sweep,
+                       denomPartials ? s->b.partial + (size_t)tiles : nullptr);
     if (s->strandIters <= 2) {
         hipLaunchKernelGGL(k_line_pairs_apply, dim3(blocks(tiles * 64, 256)), dim3(256), 0,
                            s->stream, s->n, tiles, s->strandIters, s->b.lineG, s->b.lineQL,
@@ -1004,6 +1096,20 @@ void launch_strand_solve(yb_sim* s, int buf, SolverCtl* ctl, int sweep, float* s
 
 void launch_ctl_reset(yb_sim* s) {
     hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(64), 0, s->stream, s->b.ctl);
+}
+
+// The residual's denominator from the plain warm start in dx[0]: a force
+// evaluation (skipped when `partialsWritten`, i.e. a strand step already wrote
+// them from the same iterate), then the reduction.  The partials use the
+// second half of the buffer; sweep 0 writes the first.
+void launch_residual_denom(yb_sim* s, bool partialsWritten) {
+    const size_t nbw = (size_t)sweep_waves(s->n);
+    double2* part = s->b.partial + nbw;
+    if (!partialsWritten)
+        hipLaunchKernelGGL(k_force_eval, dim3(blocks(sweep_waves(s->n) * 64, 256)), dim3(256), 0,
+                           s->stream, s->b.params, s->b.pos, s->b.prm, s->b.yv, s->b.q[0],
+                           s->b.qRest, s->b.dx[0], s->b.numCols, s->b.cols, part);
+    hipLaunchKernelGGL(k_residual_denom, dim3(1), dim3(64), 0, s->stream, s->b.ctl, part, (int)nbw);
 }
 
 void launch_end(yb_sim* s, int fin, bool adaptive) {

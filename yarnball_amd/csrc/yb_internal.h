@@ -46,8 +46,14 @@ struct SolverCtl {
     unsigned long long sweeps;    // sweeps run, summed over substeps
     unsigned long long substeps;  // substeps run in this mode
     int32_t notConverged;         // substeps that hit maxSweeps above the tolerance
-    float maxResid;               // largest final residual
-    double denom;                 // sum over free vertices of l_i^2 (set at upload)
+    float maxResid;               // largest final residual (of the stopping test)
+    double denom;                 // update residual: sum over free vertices of l_i^2 (upload)
+    double denomF;                // Newton residual: |f(dx_0)|^2 of the substep (< 0: unknown)
+    double sumFres;               // final Newton residuals summed over substeps
+    float fres;                   // Newton residual of the last evaluated sweep
+    float maxFres;                // largest final Newton residual
+    int32_t kind;                 // stopping test: 0 update residual, 1 Newton residual
+    int32_t pad_;
 };
 
 constexpr int kMaxCols = YB_MAX_COLLISIONS_PER_SEGMENT;
@@ -155,7 +161,7 @@ struct Buffers {
     float* minSep = nullptr;      // smallest contact distance / (2 radius) seen
     DevParams* params = nullptr;
     SolverCtl* ctl = nullptr;
-    double* partial = nullptr;    // residual partial sums per sweep wave, two halves
+    double2* partial = nullptr;   // residual partial sums (update, force) per sweep wave, two halves
     int32_t* hostFlag = nullptr;  // pinned: doneAt read back after each chunk
     // learned warm start
     float* mlpPrm = nullptr;          // kMlpParams floats
@@ -216,7 +222,8 @@ struct yb_sim {
     bool adaptiveReady = false;
     hipEvent_t chunkEv[2] = {nullptr, nullptr};
     void* prof = nullptr;  // eager-profiling event log (capi.hip)
-    double denomHost = 0.0;
+    double denomHost = 0.0;  // update-residual normaliser (sum of l_i^2 over free vertices)
+    int residualKind = 0;    // stopping test: YB_RESIDUAL_UPDATE or YB_RESIDUAL_NEWTON
     int warmStart = 0;     // 0 plain (iteration.cu:18-34), 1 learned, 2 temporal
     int frameRefresh = 1;  // refresh frames after a non-plain warm start
     int strandSolve = 0;   // one Newton step along the strands after the warm start
@@ -266,8 +273,13 @@ void launch_frame_refresh(yb_sim* s);  // closed-form frame update from dx[0]
 // (between sweeps), nothing once the substep converged before `sweep`.
 // sys (tests): also write each vertex's system row (diagonal, coupling to the
 // next vertex, right-hand side xyz).
+// denomPartials: also write the residual partials of the input iterate to the
+// buffer launch_residual_denom reads (a strand step on the plain warm start).
 void launch_strand_solve(yb_sim* s, int buf = 0, SolverCtl* ctl = nullptr, int sweep = 0,
-                         float* sys = nullptr);
+                         float* sys = nullptr, bool denomPartials = false);
+// Relative-residual denominator |f(dx_0)|^2 of the substep from the plain warm
+// start in dx[0] (after launch_ctl_reset; before the warm start is changed).
+void launch_residual_denom(yb_sim* s, bool partialsWritten);
 void launch_end(yb_sim* s, int fin, bool adaptive);         // iteration.cu:48-61
 int sweep_waves(int n);
 // learned warm start (mlp.hip)

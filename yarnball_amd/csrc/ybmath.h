@@ -141,6 +141,11 @@ YB_HD void sym_add(sym3& m, const sym3& o) {
 YB_UNROLL
     for (int i = 0; i < 6; i++) m.a[i] += o.a[i];
 }
+YB_HD v3 sym_apply(const sym3& m, v3 x) {
+    return v3{(m.a[0] * x.x + m.a[3] * x.y) + m.a[4] * x.z,
+              (m.a[3] * x.x + m.a[1] * x.y) + m.a[5] * x.z,
+              (m.a[4] * x.x + m.a[5] * x.y) + m.a[2] * x.z};
+}
 // H^{-1} f through the adjugate of the symmetric matrix.
 YB_HD v3 sym_solve(const sym3& m, v3 f) {
     float a = m.a[0], b = m.a[1], c = m.a[2], d = m.a[3], e = m.a[4], g = m.a[5];

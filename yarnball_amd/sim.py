@@ -155,19 +155,28 @@ class Sim:
         print(f"Segments with collisions: {int((c > 0).sum())}/{self.numVerts}")
 
     # -- residual-targeted solving (no counterpart in the reference) -----------
-    def setSolver(self, residualTol, maxSweeps=512, chunk=8, checkEvery=1):
-        """Sweep every substep until the relative Newton residual
-        RMS |dx_k+1 - dx_k| / rest length <= residualTol (or maxSweeps); residualTol <= 0
-        restores the fixed meta.numItr schedule."""
+    def setSolver(self, residualTol, maxSweeps=512, chunk=8, checkEvery=1, kind=None):
+        """Sweep every substep until the stopping residual <= residualTol (or
+        maxSweeps); residualTol <= 0 restores the fixed meta.numItr schedule.
+        kind: "update" -- RMS |dx_k+1 - dx_k| / rest length (the default);
+        "newton" -- |f(dx_k)| / |f(dx_0)| over free vertices, dx_0 the plain warm
+        start.  Both are measured and reported by solverStats()."""
         self._need_handle()
         check(self._lib.yb_set_solver(self._h, float(residualTol), int(maxSweeps), int(chunk),
                                       int(checkEvery)))
+        if kind is not None:
+            k = {"update": 0, "newton": 1}[kind]
+            check(self._lib.yb_set_residual_kind(self._h, k))
 
     def solverStats(self, reset=False):
+        """Counters of the residual-targeted mode since the last reset;
+        meanNewtonResidual is the mean over substeps of the final |f| / |f0|."""
         st = _lib.SolverStats()
         self._need_handle()
         check(self._lib.yb_solver_stats(self._h, ctypes.byref(st), int(bool(reset))))
-        return {f: getattr(st, f) for f, _ in st._fields_}
+        out = {f: getattr(st, f) for f, _ in st._fields_}
+        out["meanNewtonResidual"] = out["sumNewtonResidual"] / max(out["substeps"], 1)
+        return out
 
     def setScenes(self, offsets):
         """Batched scenes: scene k owns vertices [offsets[k], offsets[k+1])."""
