@@ -7,22 +7,28 @@ opposite edge pulled at 5 cm/s, gravity -z), 1/30 s frames, maxH = 1e-3
 (34 substeps per frame), detection every substep, numItr sweeps per substep.
 
 A step is one frame: Sim.advance(1/30) over the whole patch, state resident
-in HBM, solved every substep to the relative Newton residual 1e-6.  Four
-methods run the same frames from the same initial state: the plain warm
-start, the learned one (the committed model, with the substep-level plain
-fallback), and each of them followed by the strand solve (a tridiagonal
-Newton step along the strands after the warm start and before sweeps 1, 5,
-9, ...).  value = vertices x substeps / second over the timed frames of the
-fastest of the non-plain methods; every method's rate, sweeps per substep and
-ratio to the plain one are beside it.
+in HBM, every substep solved until the update residual (RMS change of the
+iterate per sweep in rest lengths) is <= 1e-6; the Newton residual |f|/|f0|
+each method ends at is reported beside it.  Four methods run the same frames
+from the same initial state: the plain warm start, the learned one (the
+committed model, with the substep-level plain fallback), and each of them
+followed by the strand solve (a tridiagonal Newton step along the strands
+after the warm start and before sweeps 1, 5, 9, ...).  value = vertices x
+substeps / second over the timed frames of the learned warm start (the
+faster of its two forms); every method's rate, sweeps per substep, residuals
+and ratio to the plain one are beside it.
 
 Also reported:
   roofline     the dominant kernel's algorithmic bytes per launch / its average
                launch time, measured live with HIP events on the stepper's
                stream (yb_profile), against MI355X's 8 TB/s HBM peak;
+  broadphase   the same for the contact query (p candidates and k contacts per
+               segment counted on the device) and the tree refit;
   cpu_baseline the CPU oracle (FP64, OpenMP) on a bounded sample of the same
                workload, on this host's cores: one substep with its sweeps
-               capped at the plain GPU run's mean per substep.
+               capped at the plain GPU run's mean per substep;
+  batched      the batched workload: 512 uneven random-walk scenes in one
+               handle (fixed 8-sweep schedule), against one handle per scene.
 """
 import argparse
 import ctypes
@@ -101,11 +107,17 @@ def cpu_baseline(sc, substeps, threads, residual, max_sweeps, fp32=False):
 
 
 def bench_batched(args):
+    print(json.dumps(measure_batched(args, args.steps, args.warmup, args.steps,
+                                     not args.no_cpu)))
+
+
+def measure_batched(args, steps, warmup, per_scene_steps, with_cpu):
     """Batched uneven scenes: many random-walk strands (sizes log-uniform in
     [256, 65536], kBend log-uniform in [1e-9, 1e-7]) in one handle, every
     kernel launched once over all scenes, against the same scenes advanced
-    one handle at a time.  Fixed sweep schedule (numItr), in which batched
-    and per-scene runs are bit-identical."""
+    one handle at a time (per_scene_steps frames).  Fixed sweep schedule
+    (numItr), in which batched and per-scene runs are bit-identical.  Returns
+    the JSON record."""
     from yarnball_amd import Sim, scenes
     from yarnball_amd._lib import STAGES, check, ptr
 
@@ -122,11 +134,12 @@ def bench_batched(args):
         print("[bench] batched scenes run the fixed sweep schedule (numItr)", file=sys.stderr)
 
     tuned = None
+    launch_config = args.launch_config
     if not args.no_autotune:
         from yarnball_amd import autotune
         tuned = autotune.tune(big, offsets=offs, substeps=2,
                               log=lambda m: print(m, file=sys.stderr, flush=True))
-        args.launch_config = (tuned["sweep_threads"], tuned["query_threads"])
+        launch_config = (tuned["sweep_threads"], tuned["query_threads"])
 
     b = Sim(big.n)
     # the random walks cross themselves here and there at t = 0: reported as
@@ -137,16 +150,16 @@ def bench_batched(args):
     b.minSeparation(reset=True)
     if args.no_graphs:
         b.setGraphs(False)
-    if args.launch_config:
-        b.setLaunchConfig(*args.launch_config)
-    for w in range(args.warmup):
+    if launch_config:
+        b.setLaunchConfig(*launch_config)
+    for w in range(warmup):
         b.advance(frame)
     t0 = time.perf_counter()
-    for k in range(args.steps):
+    for k in range(steps):
         b.advance(frame)
         print(f"[bench] batched: frame {k} done", file=sys.stderr, flush=True)
     el_b = time.perf_counter() - t0
-    value = big.n * spf * args.steps / el_b
+    value = big.n * spf * steps / el_b
     min_sep = b.minSeparation()
 
     # live per-stage profile of the batched handle
@@ -168,32 +181,32 @@ def bench_batched(args):
     b.close()
 
     per = None
-    if not args.no_per_scene:
+    if not args.no_per_scene and per_scene_steps > 0:
         sims = []
         for sc in scs:
             s_ = Sim(sc.n)
             s_.printErrors = False
             scenes.load_into(s_, sc)
-            if args.launch_config:
-                s_.setLaunchConfig(*args.launch_config)
+            if launch_config:
+                s_.setLaunchConfig(*launch_config)
             sims.append(s_)
-        for w in range(args.warmup):
+        for w in range(min(warmup, 1)):
             for s_ in sims:
                 s_.advance(frame)
         t0 = time.perf_counter()
-        for k in range(args.steps):
+        for k in range(per_scene_steps):
             for s_ in sims:
                 s_.advance(frame)
             print(f"[bench] per-scene: frame {k} done", file=sys.stderr, flush=True)
         el_p = time.perf_counter() - t0
         for s_ in sims:
             s_.close()
-        per = {"value": round(big.n * spf * args.steps / el_p, 1),
-               "ms_per_step": round(el_p / args.steps * 1e3, 3)}
+        per = {"value": round(big.n * spf * per_scene_steps / el_p, 1),
+               "ms_per_step": round(el_p / per_scene_steps * 1e3, 3), "steps": per_scene_steps}
         print(f"[bench] per-scene launches: {per['value']:.4e} vertex-substeps/s", file=sys.stderr)
 
     cpu = None
-    if not args.no_cpu:
+    if with_cpu:
         from oracle import Oracle, load
         sub = scs[:32]
         nv = sum(sc.n for sc in sub)
@@ -229,9 +242,9 @@ def bench_batched(args):
         "value": round(value, 1),
         "unit": "vertex-substeps/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(el_b / args.steps * 1e3, 3),
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": round(el_b / steps * 1e3, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -243,8 +256,8 @@ def bench_batched(args):
                    "launch": "eager" if args.no_graphs else "hip graphs",
                    "min_separation_x2r": round(min_sep, 3)},
         "per_scene_launches": per,
-        "launch_config": {"sweep_threads": (args.launch_config or (256, 128))[0],
-                          "query_threads": (args.launch_config or (256, 128))[1],
+        "launch_config": {"sweep_threads": (launch_config or (256, 128))[0],
+                          "query_threads": (launch_config or (256, 128))[1],
                           "autotune": tuned},
         "batched_vs_per_scene": round(value / per["value"], 3) if per else None,
         "roofline": {"kernel": "sweep", "bound": "hbm",
@@ -255,7 +268,7 @@ def bench_batched(args):
                      "avg_launch_us": round(sweep_us, 2), "mean_contacts_per_segment": round(c, 3)},
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out))
+    return out
 
 
 def live_profile(sim, sc, args, substeps_per_frame):
@@ -287,12 +300,12 @@ def run_method(sc, args, method, model, strand_model=None):
         sim.setSolver(args.residual, maxSweeps=args.max_sweeps, chunk=args.chunk, checkEvery=args.check_every)
     if method == "learned":
         learned.use(sim, model)
-    # the strand methods skip the frame refresh after the warm start: with the
-    # strand steps it saves no sweeps
+    # the strand methods refresh the frames from the warm start before the
+    # strand step (held frames would undo the rotation the warm start predicts)
     elif method == "strand":
-        learned.strand(sim, every=args.strand_every, first=args.strand_first, frame_refresh=False)
+        learned.strand(sim, every=args.strand_every, first=args.strand_first)
     elif method == "learned+strand":
-        learned.use(sim, strand_model, strand=True, frame_refresh=False)
+        learned.use(sim, strand_model, strand=True)
         learned.strand_schedule(sim, every=args.strand_every, first=args.strand_first)
     if args.no_graphs:
         sim.setGraphs(False)
@@ -323,7 +336,9 @@ def run_method(sc, args, method, model, strand_model=None):
         r["sweeps_per_substep"] = st["sweeps"] / max(st["substeps"], 1)
         print(f"[bench] {method}: {r['sweeps_per_substep']:.1f} sweeps/substep, max "
               f"{st['maxSweepsUsed']}, not converged {st['notConverged']}, max final residual "
-              f"{st['maxResidual']:.2e}", file=sys.stderr, flush=True)
+              f"{st['maxResidual']:.2e}, final Newton residual mean "
+              f"{st['meanNewtonResidual']:.2e} max {st['maxNewtonResidual']:.2e}",
+              file=sys.stderr, flush=True)
     if method.startswith("learned"):
         r["warm_substeps"] = learned.warm_substeps(sim)
         r["vertex_counters"] = learned.counters(sim)
@@ -377,6 +392,8 @@ def main():
                     help="keep the default block sizes instead of timing the candidates")
     ap.add_argument("--no-per-scene", action="store_true",
                     help="batched_walks: skip the one-handle-per-scene comparison")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="default workload: skip the batched-scenes record")
     ap.add_argument("--launch-config", type=int, nargs=2, default=None,
                     metavar=("SWEEP_THREADS", "QUERY_THREADS"),
                     help="fixed block sizes (skips the autotuner)")
@@ -414,11 +431,22 @@ def main():
     runs = {}
     for m in methods:
         runs[m] = run_method(sc, args, m, model, strand_model)
-    # headline: the fastest method at the common residual (the plain method
-    # when it runs alone)
-    main_m = max((m for m in methods if m != "plain"), key=lambda m: runs[m]["value"],
-                 default=methods[-1])
+    # headline: the learned warm start the metric names -- the faster of its
+    # two forms (alone, or refined by the strand step) -- at the common
+    # residual; another method only when no learned one ran
+    learned_ms = [m for m in methods if m.startswith("learned")]
+    main_m = max(learned_ms or [m for m in methods if m != "plain"] or methods,
+                 key=lambda m: runs[m]["value"])
     ref = runs["plain" if "plain" in runs else main_m]
+    # broadphase counts on the headline method's final state: candidate leaves
+    # p and accepted contacts k per segment, for the query's byte model
+    qsim = runs[main_m]["sim"]
+    qc = np.zeros(3, np.int64)
+    qp = qsim.meta.copy()
+    check(qsim._lib.yb_debug_query_counts(qsim.handle, ctypes.byref(qp), np.float32(sc.maxH),
+                                          ptr(qc)))
+    segs_q = max(int(qc[0]), 1)
+    p_cand, k_acc = qc[1] / segs_q, qc[2] / segs_q
     for m in methods:
         if runs[m] is not ref:
             runs[m]["sim"].close()
@@ -462,6 +490,27 @@ def main():
         "avg_launch_us": round(per_launch[dominant] * 1e3, 2),
         "mean_contacts_per_segment": round(c, 3),
     }
+    # Broadphase: the query (traversal 24 + 8p, filter 8p + 8k + 4 bytes per
+    # segment) and the refit (52 bytes per segment), p candidate leaves and k
+    # accepted contacts per segment as counted on the device.
+    broad = {}
+    for st_, b_seg in (("query", 28.0 + 16.0 * p_cand + 8.0 * k_acc), ("refit", 52.0)):
+        i = STAGES.index(st_)
+        if not nl[i]:
+            continue
+        us = per_launch[st_] * 1e3
+        ach = b_seg * segs_q / (us * 1e-6) / 1e9
+        broad[st_] = {"kernel": st_, "bound": "hbm", "achieved": round(ach, 1),
+                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                      "traffic": (round(MEASURED_TRAFFIC[(args.workload, st_)] * sc.n)
+                                  if (args.workload, st_) in MEASURED_TRAFFIC else None),
+                      "alg_bytes_per_launch": int(b_seg * segs_q), "avg_launch_us": round(us, 2)}
+    if "query" in broad:
+        broad["query"]["candidates_per_segment"] = round(float(p_cand), 3)
+        broad["query"]["accepted_per_segment"] = round(float(k_acc), 3)
+    print(f"[bench] broadphase: p {p_cand:.2f} candidates, k {k_acc:.2f} contacts per segment; "
+          + ", ".join(f"{k_} {v['avg_launch_us']:.1f} us {v['frac']:.3f} of peak"
+                      for k_, v in broad.items()), file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu:
@@ -491,6 +540,8 @@ def main():
             row["sweeps_per_substep"] = round(r["sweeps_per_substep"], 1)
             row["not_converged_substeps"] = int(r["stats"]["notConverged"])
             row["max_final_residual"] = float(r["stats"]["maxResidual"])
+            row["newton_residual_mean"] = float(r["stats"]["meanNewtonResidual"])
+            row["newton_residual_max"] = float(r["stats"]["maxNewtonResidual"])
         if m.startswith("learned"):
             ws = r["warm_substeps"]
             row["substeps_learned_start"] = ws["learned"]
@@ -511,6 +562,12 @@ def main():
                           f"{args.strand_first}+{args.strand_every}k)",
     }
 
+    sim.close()
+    # the batched uneven-scene workload (512 random-walk scenes) in the same run
+    batched = None
+    if not args.no_batched:
+        batched = measure_batched(args, args.steps, min(args.warmup, 2), 2, False)
+
     out = {
         "metric": "vertex-substeps/s (plain vs learned warm-start, equal residual) + HBM GB/s",
         "value": round(value, 1),
@@ -528,26 +585,29 @@ def main():
             "workload": sc.name,
             "vertices": sc.n,
             "substeps_per_step": substeps_per_frame,
-            "numItr": int(sim.meta.numItr) if args.residual <= 0 else None,
+            "numItr": int(args.numItr) if args.residual <= 0 else None,
             "residual_tol": args.residual if args.residual > 0 else None,
+            "residual_kind": "update (RMS sweep change / rest length)" if args.residual > 0 else None,
             "mean_sweeps_per_substep": (round(stats["sweeps"] / max(stats["substeps"], 1), 1)
-                                        if stats else int(sim.meta.numItr)),
+                                        if stats else int(args.numItr)),
             "not_converged_substeps": stats["notConverged"] if stats else None,
             "method": method_names[main_m],
-            "detection_period": int(sim.meta.detectionPeriod),
+            "detection_period": int(sc.params.detectionPeriod),
             "launch": "eager" if args.no_graphs else "hip graphs",
         },
         "methods": method_rows,
         "learned_vs_plain": vs_plain("learned"),
         "strand_vs_plain": vs_plain("strand"),
         "learned_strand_vs_plain": vs_plain("learned+strand"),
-        "launch_config": {"sweep_threads": sim.launchConfig()[0],
-                          "query_threads": sim.launchConfig()[1],
+        "launch_config": {"sweep_threads": (args.launch_config or (256, 128))[0],
+                          "query_threads": (args.launch_config or (256, 128))[1],
                           "autotune": tuned},
         "roofline": roofline,
+        "roofline_broadphase": broad.get("query"),
+        "broadphase": broad,
         "cpu_baseline": cpu,
+        "batched": batched,
     }
-    sim.close()
     print(json.dumps(out))
 
 

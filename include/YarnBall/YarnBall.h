@@ -150,7 +150,8 @@ public:
     enum {
         ERROR_NONE = 0,
         ERROR_MAX_COLLISIONS_PER_SEGMENT_EXCEEDED,
-        WARNING_SEGMENT_INTERPENETRATION
+        WARNING_SEGMENT_INTERPENETRATION,
+        ERROR_TRAVERSAL_STACK_EXCEEDED  // no counterpart in the reference (its LBVH is external)
     };
 
     Vertex* verts;

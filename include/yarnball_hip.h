@@ -12,6 +12,8 @@
  *
  * Status codes: 0 = ok; 1 = YB_ERROR_MAX_COLLISIONS_PER_SEGMENT_EXCEEDED
  * (YarnBall.h:120); 2 = YB_WARNING_SEGMENT_INTERPENETRATION (YarnBall.h:121);
+ * 3 = YB_ERROR_TRAVERSAL_STACK_EXCEEDED (a broadphase query ran out of stack;
+ * the reference's LBVH is not part of its sources, so this has no counterpart);
  * negative values are API / HIP errors, described by yb_last_message().
  */
 #ifndef YARNBALL_HIP_H
@@ -24,6 +26,8 @@ extern "C" {
 #endif
 
 #define YB_MAX_COLLISIONS_PER_SEGMENT 128 /* YarnBall.h:20 */
+/* Pending subtrees one broadphase query may hold (its traversal stack). */
+#define YB_QUERY_STACK_DEPTH 64
 /* Largest vertex count of one handle (all scenes together): device byte
  * offsets of per-vertex float4 arrays stay below 2^32.  At this size the
  * device buffers take about 180 GiB, most of it the 128-slot contact lists. */
@@ -33,6 +37,7 @@ enum {
     YB_OK = 0,
     YB_ERROR_MAX_COLLISIONS_PER_SEGMENT_EXCEEDED = 1, /* YarnBall.h:120 */
     YB_WARNING_SEGMENT_INTERPENETRATION = 2,          /* YarnBall.h:121 */
+    YB_ERROR_TRAVERSAL_STACK_EXCEEDED = 3,            /* no counterpart */
     YB_ERR_INVALID_ARGUMENT = -1,
     YB_ERR_NOT_CONFIGURED = -2,
     YB_ERR_HIP = -3,
@@ -234,6 +239,12 @@ int yb_export_obj(const char* path, int n, const yb_vertex* verts);
  * the broadphase + narrowphase of detectCollisions (collision.cu:94-119) and
  * the step limit (collision.cu:121-172).  Uses p and h as yb_advance would. */
 int yb_debug_prologue(yb_sim* s, yb_params* p, float h, int detect, int rebuild);
+/* Broadphase counts for the query's byte model: runs yb_debug_prologue(s, p,
+ * h, detect 1, refit) once more with counting on (so it changes the state as
+ * that call does) and returns out[0] segments, out[1] candidate leaves whose
+ * boxes overlap a query box (summed over segments), out[2] accepted contacts
+ * (the summed list lengths). */
+int yb_debug_query_counts(yb_sim* s, yb_params* p, float h, int64_t out[3]);
 /* Copy out intermediate arrays (any pointer may be null):
  * dx: 4n floats (xyz + step limit), aabb: 8n floats (lo xyz _, hi xyz _),
  * morton: n u32 indexed by segment, order: n i32 (sorted leaf -> segment),
@@ -256,6 +267,11 @@ int yb_debug_read_mlp(yb_sim* s, float* dxPlain, float* features);
  * may be null): each vertex's row -- diagonal, coupling to the next vertex,
  * right-hand side xyz -- for checking the solve. */
 int yb_debug_strand(yb_sim* s, float* system);
+/* Traversal stack entries a broadphase query may use, 1..YB_QUERY_STACK_DEPTH
+ * (the default).  Lowering it forces the YB_ERROR_TRAVERSAL_STACK_EXCEEDED
+ * path on ordinary scenes (tests); a query that runs out skips the subtree it
+ * cannot push. */
+int yb_debug_set_query_stack(yb_sim* s, int entries);
 /* Overwrite the device warm-start/iterate dx (4n floats) and lastVels (3n). */
 int yb_debug_write_dx(yb_sim* s, const float* dx);
 int yb_debug_read_last_vels(yb_sim* s, float* lastVels);

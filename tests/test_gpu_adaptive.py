@@ -64,3 +64,39 @@ def test_adaptive_matches_fixed_when_capped():
     assert st["notConverged"] == 5 and st["sweeps"] == 5 * sc.params.numItr
     a.close()
     b.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,make,tol", [
+    ("readme", scenes.readme_scene, 3e-2),
+    ("knit_8x256", lambda: scenes.knit_patch(rows=8, per=256), 3e-2),
+])
+def test_newton_residual_stop_matches_the_oracle(name, make, tol):
+    """Stopping on the Newton residual |f(dx_k)| / |f(dx_0)| (tolerance well
+    above the FP32 floor): the device and the FP64 oracle, from the same
+    state, stop after nearly the same number of sweeps and report nearly the
+    same residual; both residuals are reported whichever one stops."""
+    sc = make()
+    sim = gpu_sim(sc)
+    sim.setSolver(tol, maxSweeps=4000, chunk=8, kind="newton")
+    orc = Oracle(n=sc.n)
+    orc.set_solver(tol, 4000)
+    orc.set_residual_kind(1)
+    for k in range(6):
+        verts, vels, qs, lv = gpu_state(sim)
+        counter = sim.stepCount()
+        sim.advance(sim.maxH)
+        st = sim.solverStats(reset=True)
+        orc.load_state(verts, vels, qs, lv, sc.qRests if k == 0 else None)
+        orc.set_step_counter(counter)
+        p = sim.meta.copy()
+        orc.substep(p, p.detectionPeriod > 0 and counter % p.detectionPeriod == 0)
+        print(f"{name} substep {k}: gpu sweeps {st['sweeps']} Newton {st['lastNewtonResidual']:.3e} "
+              f"update {st['lastResidual']:.2e} | oracle sweeps {orc.last_sweeps()} "
+              f"Newton {orc.newton_residual():.3e} update {orc.update_residual():.2e}")
+        assert st["notConverged"] == 0 and st["lastNewtonResidual"] <= tol
+        assert st["lastResidual"] == st["lastNewtonResidual"]  # the stopping one
+        assert abs(st["sweeps"] - orc.last_sweeps()) <= max(3, 0.05 * orc.last_sweeps())
+        if st["sweeps"] == orc.last_sweeps():
+            assert st["lastNewtonResidual"] == pytest.approx(orc.newton_residual(), rel=2e-2)
+    sim.close()

@@ -69,3 +69,101 @@ def test_set_scenes_rejects_bad_layouts():
     with pytest.raises(Exception):
         s.setScenes(offs)               # checked against the uploaded flags
     s.close()
+
+
+def _walk_sizes(count, lo, hi, seed):
+    rng = np.random.default_rng(seed)
+    return np.exp(rng.uniform(np.log(lo), np.log(hi), count)).astype(np.int64)
+
+
+@pytest.mark.gpu
+def test_random_walk_batch_against_the_oracle():
+    """Eight uneven random-walk scenes (the batched workload's scene type) in
+    one handle, substep by substep against the FP64 oracle run on each scene
+    alone from the same state: positions and frames <= 1e-5, contact sets
+    bit-exact, and no contact across scenes."""
+    from oracle import Oracle
+
+    from helpers import gpu_state, rel_err
+
+    scs = scenes.random_walks(_walk_sizes(8, 256, 4096, 5), seed=5, turn=0.8)
+    big, offs = scenes.batch(scs)
+    b = Sim(big.n)
+    b.printErrors = False
+    scenes.load_into(b, big)
+    b.setScenes(offs)
+    orcs = [Oracle(n=sc.n) for sc in scs]
+    worst_p = worst_q = 0.0
+    contacts = 0
+    for k in range(10):
+        verts, vels, qs, lv = gpu_state(b)
+        counter = b.stepCount()
+        b.advance(b.maxH)
+        v1, _, q1, _ = gpu_state(b)
+        bnc, bcols = gpu_contacts(b)
+        sets = contact_sets(bnc, bcols)
+        p = b.meta.copy()
+        for s, (sc, o) in enumerate(zip(scs, orcs)):
+            sl = slice(offs[s], offs[s + 1])
+            cid = verts["connectionIndex"][sl]
+            assert np.all(cid < 0)
+            o.load_state(verts[sl], vels[sl], qs[sl], lv[sl], sc.qRests if k == 0 else None)
+            o.set_step_counter(counter)
+            ps = p.copy()
+            ps.numVerts = sc.n
+            o.substep(ps, True)
+            pos, _, _, q = o.state()
+            worst_p = max(worst_p, rel_err(v1["pos"][sl], pos))
+            worst_q = max(worst_q, float(np.max(np.abs(q1[sl].astype(np.float64) - q))))
+            onc, ocols = o.contacts()
+            mine = [tuple(j - offs[s] for j in c) for c in sets[offs[s]:offs[s + 1]]]
+            assert all(0 <= j < sc.n for c in mine for j in c)  # within the scene
+            assert mine == contact_sets(onc, ocols), f"substep {k} scene {s}"
+            contacts += int(onc.sum())
+    print(f"8 random walks ({big.n} vertices): worst pos rel {worst_p:.2e}, frames {worst_q:.2e}, "
+          f"{contacts} contact entries")
+    assert contacts > 0
+    assert worst_p <= 1e-5 and worst_q <= 1e-5
+    b.close()
+
+
+@pytest.mark.gpu
+def test_full_512_scene_batch_properties():
+    """The batched workload at full size (512 scenes, sizes log-uniform in
+    [256, 65536]) for one frame: every value finite, the pinned vertices
+    unmoved, contacts only inside each scene, and three scenes (smallest,
+    median, largest) bit-identical to standalone runs of themselves."""
+    sizes = _walk_sizes(512, 256, 65536, 0)
+    scs = scenes.random_walks(sizes, seed=0)
+    big, offs = scenes.batch(scs)
+    frame = np.float32(big.frame_dt)
+    b = Sim(big.n)
+    b.printErrors = False
+    scenes.load_into(b, big)
+    b.setScenes(offs)
+    b.advance(frame)
+    b.download()
+    assert np.all(np.isfinite(b.verts["pos"])) and np.all(np.isfinite(b.qs))
+    pinned = big.verts["invMass"] == 0
+    assert np.array_equal(b.verts["pos"][pinned], big.verts["pos"][pinned])
+    nc, cols = gpu_contacts(b)
+    scene_of = np.repeat(np.arange(len(scs)), np.diff(offs))
+    seg = np.flatnonzero(nc > 0)
+    for i in seg[:: max(1, len(seg) // 20000)]:
+        part = cols[: nc[i], i]
+        assert np.all(scene_of[part] == scene_of[i])
+    order = np.argsort(sizes)
+    for s in (order[0], order[len(order) // 2], order[-1]):
+        one = Sim(scs[s].n)
+        one.printErrors = False
+        scenes.load_into(one, scs[s])
+        one.advance(frame)
+        one.download()
+        sl = slice(offs[s], offs[s + 1])
+        assert np.array_equal(b.verts["pos"][sl], one.verts["pos"]), f"scene {s}"
+        assert np.array_equal(b.qs[sl], one.qs), f"scene {s}"
+        one.close()
+    print(f"512 scenes, {big.n} vertices, one frame: {int(nc.sum())} contact entries, "
+          f"all inside their scenes; scenes {order[0]}, {order[len(order) // 2]}, {order[-1]} "
+          f"equal their standalone runs")
+    b.close()

@@ -17,6 +17,7 @@ MAX_VERTICES = 1 << 28
 
 YB_OK = 0
 YB_ERROR_MAX_COLLISIONS_PER_SEGMENT_EXCEEDED = 1
+YB_ERROR_TRAVERSAL_STACK_EXCEEDED = 3
 YB_WARNING_SEGMENT_INTERPENETRATION = 2
 YB_ERR_INVALID_ARGUMENT = -1
 YB_ERR_NOT_CONFIGURED = -2
@@ -138,6 +139,8 @@ SIGNATURES = {
     "yb_profile": (_I, [_P, ctypes.POINTER(Params), _F, _I, _P, _P]),
     "yb_set_solver": (_I, [_P, _F, _I, _I, _I]),
     "yb_set_residual_kind": (_I, [_P, _I]),
+    "yb_debug_set_query_stack": (_I, [_P, _I]),
+    "yb_debug_query_counts": (_I, [_P, ctypes.POINTER(Params), _F, _P]),
     "yb_min_separation": (_I, [_P, _P, _I]),
     "yb_mlp_load": (_I, [_P, _P, _I]),
     "yb_set_warm_start": (_I, [_P, _I]),

@@ -51,7 +51,7 @@ static void free_all(yb_sim* s) {
                     b.mlpPrm,  b.mlpTiles, b.mlpBias, b.corrPrev, b.dxPlain, b.predC,
                     b.lineG,   b.lineQL,  b.lineE[0], b.lineE[1], b.lineF[0], b.lineF[1],
                     b.mlpCount, b.recX, b.recT, b.recCount, b.sceneOf, b.sceneOff,
-                    b.sceneBoxes, b.sceneRoot, b.sceneEndFlags, b.drvInit};
+                    b.sceneBoxes, b.sceneRoot, b.sceneEndFlags, b.drvInit, b.candCount};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (b.hostFlag) (void)hipHostFree(b.hostFlag);
@@ -560,15 +560,14 @@ static void enqueue_prologue(yb_sim* s, int mode, bool adaptive, int warm) {
     }
     launch_step_limit(s);
     mark(s, YB_STAGE_STEP_LIMIT);
-    // Residual-targeted mode: the relative residual's denominator is the
-    // force at the plain warm start (now in dx[0]) for every warm start, so
-    // that all of them are held to the same target.  A plain start without a
-    // strand step gets it from sweep 0; otherwise it is taken here, before the
-    // start is changed (a strand step on the plain start computes the same
-    // forces and writes the partials itself).
+    // Residual-targeted mode: the Newton residual's denominator is the force
+    // at the plain warm start (now in dx[0], with the previous substep's
+    // frames) for every warm start, so that all of them are held to the same
+    // target.  A plain session gets it from sweep 0; any other session takes
+    // it here, before the start is changed.
     if (adaptive) {
         launch_ctl_reset(s);
-        if (warm != 0) launch_residual_denom(s, false);
+        if (s->warmStart != 0 || s->strandSolve) launch_residual_denom(s);
         mark(s, YB_STAGE_RESIDUAL);
     }
     const bool learned = warm == 1, record = s->recCap > 0;
@@ -582,23 +581,28 @@ static void enqueue_prologue(yb_sim* s, int mode, bool adaptive, int warm) {
         launch_temporal(s);
         mark(s, YB_STAGE_MLP);
     }
-    if (s->strandSolve) {
-        const bool denom = adaptive && warm == 0;
-        launch_strand_solve(s, 0, nullptr, 0, nullptr, denom);
-        if (denom) launch_residual_denom(s, true);
-        mark(s, YB_STAGE_MLP);
-    }
+    // The frames are brought in line with the new warm start before a strand
+    // step: the step solves for dx with the frames held, so frames left at
+    // the previous substep would hold every segment to its old direction and
+    // undo the rotation the warm start predicts (the sweeps restore it only
+    // slowly where bending is soft).
     if ((warm != 0 || s->strandSolve) && s->frameRefresh) {
         launch_frame_refresh(s);
+        mark(s, YB_STAGE_MLP);
+    }
+    if (s->strandSolve) {
+        launch_strand_solve(s);
         mark(s, YB_STAGE_MLP);
     }
 }
 
 // In residual-targeted mode the residual is evaluated after every
-// s->checkEvery-th sweep, after the last allowed one, and after sweep 0 (whose
-// input is the plain warm start of a plain substep: the denominator).
+// s->checkEvery-th sweep, after the last allowed one, and -- in a plain
+// session, whose prologue leaves the Newton-residual denominator to it --
+// after sweep 0, whose input is the plain warm start.
 static bool checks_residual(const yb_sim* s, int k) {
-    return k == 0 || (k + 1) % s->checkEvery == 0 || k == s->maxSweeps - 1;
+    const bool plainStart = s->warmStart == 0 && !s->strandSolve;
+    return (k == 0 && plainStart) || (k + 1) % s->checkEvery == 0 || k == s->maxSweeps - 1;
 }
 
 // The residual of a check sweep is reduced, and convergence decided, by the
@@ -743,6 +747,18 @@ extern "C" int yb_set_solver(yb_sim* s, float residualTol, int maxSweeps, int ch
     s->maxSweeps = maxSweeps;
     s->chunk = chunk;
     s->checkEvery = checkEvery;
+    return YB_OK;
+}
+
+extern "C" int yb_debug_set_query_stack(yb_sim* s, int entries) {
+    if (!s || entries < 1 || entries > YB_QUERY_STACK_DEPTH) {
+        set_message("yb_debug_set_query_stack: 1 <= entries <= YB_QUERY_STACK_DEPTH");
+        return YB_ERR_INVALID_ARGUMENT;
+    }
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    s->queryStack = entries;
+    s->graphItr = -1;  // the captured queries carry the old limit
+    s->adaptiveReady = false;
     return YB_OK;
 }
 
@@ -938,12 +954,20 @@ static int check_errors(yb_sim* s, int* warning) {
         HIPCHECK(hipStreamSynchronize(s->stream));
     }
     if (warning) *warning = err[1];
-    if (err[0] == YB_ERROR_MAX_COLLISIONS_PER_SEGMENT_EXCEEDED) {
-        set_message("MAX_COLLISIONS_PER_SEGMENT exceeded");
+    // the reference's error (YarnBall.cu:283-286) takes precedence
+    if (err[0] & kErrBitContacts) {
+        set_message((err[0] & kErrBitStack)
+                        ? "MAX_COLLISIONS_PER_SEGMENT exceeded (and a broadphase query ran out "
+                          "of traversal stack)"
+                        : "MAX_COLLISIONS_PER_SEGMENT exceeded");
         return YB_ERROR_MAX_COLLISIONS_PER_SEGMENT_EXCEEDED;
     }
+    if (err[0] & kErrBitStack) {
+        set_message("broadphase traversal stack exceeded: contacts may be missing");
+        return YB_ERROR_TRAVERSAL_STACK_EXCEEDED;
+    }
     if (err[0] != 0) {
-        set_message("Indescript error");
+        set_message("Indescript error");  // YarnBall.cu:289
         return YB_ERR_HIP;
     }
     return YB_OK;
@@ -1077,6 +1101,33 @@ extern "C" int yb_debug_prologue(yb_sim* s, yb_params* p, float h, int detect, i
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(s->stream));
     return check_errors(s, nullptr);
+}
+
+extern "C" int yb_debug_query_counts(yb_sim* s, yb_params* p, float h, int64_t out[3]) {
+    if (!s || !p || !out) return YB_ERR_INVALID_ARGUMENT;
+    const size_t N = (size_t)s->n;
+    if (!s->b.candCount) HIPCHECK(hipMalloc(&s->b.candCount, sizeof(int32_t) * N));
+    s->countCandidates = true;
+    int rc = yb_debug_prologue(s, p, h, 1, 0);
+    s->countCandidates = false;
+    if (rc != YB_OK) return rc;
+    std::vector<int32_t> cand(N), nc(N);
+    std::vector<float4> prm(N);
+    HIPCHECK(hipMemcpy(cand.data(), s->b.candCount, sizeof(int32_t) * N, hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(nc.data(), s->b.numCols, sizeof(int32_t) * N, hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(prm.data(), s->b.prm, sizeof(float4) * N, hipMemcpyDeviceToHost));
+    int64_t segs = 0, c = 0, k = 0;
+    for (size_t i = 0; i < N; i++) {
+        uint32_t fl;
+        memcpy(&fl, &prm[i].w, sizeof(fl));
+        if (fl & YB_FLAG_HAS_NEXT) segs++;
+        c += cand[i];
+        k += nc[i];
+    }
+    out[0] = segs;
+    out[1] = c;
+    out[2] = k;
+    return YB_OK;
 }
 
 extern "C" int yb_debug_read(yb_sim* s, float* dx, float* aabb, uint32_t* morton, int32_t* order,

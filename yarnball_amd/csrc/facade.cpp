@@ -113,6 +113,10 @@ float Sim::advance(float h) {
                 "corrupted!\n");
         throw std::runtime_error("MAX_COLLISIONS_PER_SEGMENT exceeded");
     }
+    if (rc == YB_ERROR_TRAVERSAL_STACK_EXCEEDED) {
+        lastErrorCode = rc;
+        throw std::runtime_error("broadphase traversal stack exceeded");
+    }
     if (rc != YB_OK) raise(rc, "advance");
     float out[6];
     if (yb_bounds(handle, out) == YB_OK) {

@@ -765,7 +765,13 @@ __global__ __launch_bounds__(256) void k_refit_top(int n, int nb,
 // segments' largest warm-start displacement.  Because the test is symmetric,
 // segment i lists j exactly when j lists i -- the set the reference builds
 // with two atomicAdds per pair.
-constexpr int kStack = 64;  // traversal depth limit (pending right children)
+// Traversal stack: pending right children.  The pending entries lie on one
+// root-to-leaf path, so their number is below the tree depth: at most the 30
+// Morton bits plus the index bits that split equal codes (28 at the largest
+// handle) plus 16 scene bits of a batch -- 64 holds any tree whose levels do
+// not all split in two.  A query that still runs out skips the subtree it
+// cannot push and reports YB_ERROR_TRAVERSAL_STACK_EXCEEDED.
+constexpr int kStack = YB_QUERY_STACK_DEPTH;
 
 // All six comparisons are evaluated (no short circuit): six compares and five
 // mask ANDs, with no branches.
@@ -825,7 +831,9 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
                                                          int32_t* __restrict__ cols,
                                                          int32_t* __restrict__ err,
                                                          const int32_t* __restrict__ sceneOf,
-                                                         const int32_t* __restrict__ sceneRoot) {
+                                                         const int32_t* __restrict__ sceneRoot,
+                                                         int stackLimit,
+                                                         int32_t* __restrict__ candCount) {
     __shared__ int32_t stk[kStackL][kQueryThreads];
     __shared__ int32_t candJ[kCand][kQueryThreads];
     __shared__ float candPad[kCand][kQueryThreads];
@@ -839,6 +847,7 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
     if (i >= n) return;
     if (!(__float_as_uint(prm[i].w) & YB_FLAG_HAS_NEXT)) {
         numCols[i] = 0;
+        if (candCount) candCount[i] = 0;
         return;
     }
     const float4 qlo4 = aabb[2 * i];
@@ -863,7 +872,9 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
     // together instead of once per node visit at which any thread met a leaf.
     // Candidates keep their visit order, hence so do the contact lists.
     int nb = 0;
+    int cand = 0;  // leaves whose boxes overlap the query box (measurement only)
     auto narrowphase = [&]() {
+        cand += nb;
 #pragma unroll 1
         for (int k = 0; k < nb; k++) {
             const int j = candJ[k][t];
@@ -906,14 +917,16 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
         if (tl || tr) {
             node = tl ? cl : cr;
             if (tl && tr) {
-                if (sp < kStackL) {
+                // stackLimit <= kStack (host-checked); tests lower it to force
+                // the overflow path
+                if (sp >= stackLimit) {
+                    overflowStack = true;
+                } else if (sp < kStackL) {
                     stk[sp][t] = cr;
                     sp++;
-                } else if (sp < kStack) {
+                } else {
                     deep[sp - kStackL] = cr;
                     sp++;
-                } else {
-                    overflowStack = true;
                 }
             }
         } else {
@@ -930,12 +943,14 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
         }
     }
     narrowphase();
+    // err[0] collects one bit per failure kind (kErrBit*), read by the host
     if (cnt > kMaxCols) {
-        err[0] = YB_ERROR_MAX_COLLISIONS_PER_SEGMENT_EXCEEDED;
+        atomicOr(&err[0], kErrBitContacts);
         cnt = kMaxCols;
     }
-    if (overflowStack) err[0] = YB_ERROR_MAX_COLLISIONS_PER_SEGMENT_EXCEEDED;
+    if (overflowStack) atomicOr(&err[0], kErrBitStack);
     numCols[i] = cnt;
+    if (candCount) candCount[i] = cand;
 }
 
 // Flags of each scene's last vertex (which must not start a segment).
@@ -1039,7 +1054,7 @@ static void launch_query_t(yb_sim* s) {
     hipLaunchKernelGGL((k_query<T, SL, C>), dim3(blocks(s->n, T)), dim3(T), 0, s->stream, s->b.params,
                        s->b.order, s->b.aabb, s->b.nodes, s->b.pos, s->b.prm, s->b.numCols,
                        s->b.cols, s->b.err, s->numScenes > 1 ? s->b.sceneOf : nullptr,
-                       s->b.sceneRoot);
+                       s->b.sceneRoot, s->queryStack, s->countCandidates ? s->b.candCount : nullptr);
 }
 
 void launch_query(yb_sim* s) {

@@ -209,9 +209,9 @@ __device__ __forceinline__ void pcr64(float& A, float& Lo, float& Up, float B[5]
 // same local systems are assembled, then solved along the tile's strands
 // instead of vertex by vertex (the line tail below); nothing of the sweep's own
 // output is written.  kEval -- only the residual partials of the input iterate.
-// Every mode writes, when `partial` is set, the per-wave sums of the squared
-// update |dx' - dx|^2 (kSweep) and of the squared residual force |r|^2 at its
-// input iterate (r: see the Newton step below).
+// kSweep and kEval write, when `partial` is set, the per-wave sums of the
+// squared update |dx' - dx|^2 (kSweep) and of the squared residual force |r|^2
+// at the input iterate (r: see the Newton step below).
 constexpr int kSweep = 0, kLine = 1, kEval = 2;
 template <int kMode>
 __device__ __forceinline__ void sweep_body(const DevParams* __restrict__ P,
@@ -475,7 +475,6 @@ __device__ __forceinline__ void sweep_body(const DevParams* __restrict__ P,
                           : make_float4(0.f, 0.f, 0.f, 0.f);
             lo.ql[v] = make_float2(row ? B[4] * ia : 0.0f, row ? limit : -1.0f);
         }
-        if (partial) wave_residual(0.0f, owner ? numF : 0.0f, partial, wave, n);
         return;
     }
 
@@ -538,11 +537,10 @@ __global__ __launch_bounds__(256) void k_line_tile(const DevParams* __restrict__
                                                    const float4* __restrict__ dxIn,
                                                    const int32_t* __restrict__ numCols,
                                                    const int32_t* __restrict__ cols, LineOut lo,
-                                                   const SolverCtl* __restrict__ ctl, int sweep,
-                                                   double2* __restrict__ partial) {
+                                                   const SolverCtl* __restrict__ ctl, int sweep) {
     if (strand_skip(ctl, sweep)) return;
     sweep_body<kLine>(P, pos, prm, yv, qIn, nullptr, qRest, dxIn, nullptr, numCols, cols, nullptr,
-                      0, partial, nullptr, 0.0f, 0, lo);
+                      0, nullptr, nullptr, 0.0f, 0, lo);
 }
 
 // Residual partials of an iterate without changing it: the denominator |f(dx0)|
@@ -1067,14 +1065,12 @@ void launch_frame_refresh(yb_sim* s) {
                        s->b.q[0], s->n);
 }
 
-void launch_strand_solve(yb_sim* s, int buf, SolverCtl* ctl, int sweep, float* sys,
-                         bool denomPartials) {
+void launch_strand_solve(yb_sim* s, int buf, SolverCtl* ctl, int sweep, float* sys) {
     const int tiles = sweep_waves(s->n);
     const LineOut lo{s->b.lineG, s->b.lineQL, sys};
     hipLaunchKernelGGL(k_line_tile, dim3(blocks(tiles * 64, 256)), dim3(256), 0, s->stream,
                        s->b.params, s->b.pos, s->b.prm, s->b.yv, s->b.q[buf], s->b.qRest,
-                       s->b.dx[buf], s->b.numCols, s->b.cols, lo, ctl, sweep,
-                       denomPartials ? s->b.partial + (size_t)tiles : nullptr);
+                       s->b.dx[buf], s->b.numCols, s->b.cols, lo, ctl, sweep);
     if (s->strandIters <= 2) {
         hipLaunchKernelGGL(k_line_pairs_apply, dim3(blocks(tiles * 64, 256)), dim3(256), 0,
                            s->stream, s->n, tiles, s->strandIters, s->b.lineG, s->b.lineQL,
@@ -1098,17 +1094,15 @@ void launch_ctl_reset(yb_sim* s) {
     hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(64), 0, s->stream, s->b.ctl);
 }
 
-// The residual's denominator from the plain warm start in dx[0]: a force
-// evaluation (skipped when `partialsWritten`, i.e. a strand step already wrote
-// them from the same iterate), then the reduction.  The partials use the
-// second half of the buffer; sweep 0 writes the first.
-void launch_residual_denom(yb_sim* s, bool partialsWritten) {
+// The Newton residual's denominator from the plain warm start in dx[0]: a
+// force evaluation, then the reduction.  The partials use the second half of
+// the buffer; sweep 0 writes the first.
+void launch_residual_denom(yb_sim* s) {
     const size_t nbw = (size_t)sweep_waves(s->n);
     double2* part = s->b.partial + nbw;
-    if (!partialsWritten)
-        hipLaunchKernelGGL(k_force_eval, dim3(blocks(sweep_waves(s->n) * 64, 256)), dim3(256), 0,
-                           s->stream, s->b.params, s->b.pos, s->b.prm, s->b.yv, s->b.q[0],
-                           s->b.qRest, s->b.dx[0], s->b.numCols, s->b.cols, part);
+    hipLaunchKernelGGL(k_force_eval, dim3(blocks(sweep_waves(s->n) * 64, 256)), dim3(256), 0,
+                       s->stream, s->b.params, s->b.pos, s->b.prm, s->b.yv, s->b.q[0],
+                       s->b.qRest, s->b.dx[0], s->b.numCols, s->b.cols, part);
     hipLaunchKernelGGL(k_residual_denom, dim3(1), dim3(64), 0, s->stream, s->b.ctl, part, (int)nbw);
 }
 

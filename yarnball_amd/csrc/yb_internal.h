@@ -57,6 +57,9 @@ struct SolverCtl {
 };
 
 constexpr int kMaxCols = YB_MAX_COLLISIONS_PER_SEGMENT;
+// Bits of the device error word err[0] (one per failure kind).
+constexpr int32_t kErrBitContacts = 1;  // a segment had more than kMaxCols contacts
+constexpr int32_t kErrBitStack = 2;     // a broadphase query ran out of traversal stack
 
 // The 8 XCDs each have their own L2, and block b of a launch runs on XCD b % 8.
 // Tile index of block b of nb such that every XCD works through one contiguous
@@ -157,7 +160,8 @@ struct Buffers {
     float4* sufBox = nullptr;     // 2n
     float4* preBox = nullptr;     // 2n
     float4* blkTab = nullptr;     // sparse table of refit-block unions, 2*nb per level
-    int32_t* err = nullptr;       // [0] error code, [1] warning code
+    int32_t* candCount = nullptr; // n: broadphase candidates per segment (measurement, lazy)
+    int32_t* err = nullptr;       // [0] error bits, [1] warning code
     float* minSep = nullptr;      // smallest contact distance / (2 radius) seen
     DevParams* params = nullptr;
     SolverCtl* ctl = nullptr;
@@ -207,6 +211,8 @@ struct yb_sim {
     int useGraphs = 1;  // 0: every kernel enqueued directly on the stream
     int sweepThreads = 256;  // block size of k_sweep (64, 128, 256 or 512)
     int queryThreads = 128;  // block size of k_query (64, 128 or 256)
+    int queryStack = YB_QUERY_STACK_DEPTH;  // traversal stack entries a query may use
+    bool countCandidates = false;           // k_query writes b.candCount (measurement)
     // residual-targeted mode (tol > 0): prologue graphs per detection mode,
     // one graph per chunk of sweeps, one end graph
     float tol = 0.f;
@@ -273,13 +279,11 @@ void launch_frame_refresh(yb_sim* s);  // closed-form frame update from dx[0]
 // (between sweeps), nothing once the substep converged before `sweep`.
 // sys (tests): also write each vertex's system row (diagonal, coupling to the
 // next vertex, right-hand side xyz).
-// denomPartials: also write the residual partials of the input iterate to the
-// buffer launch_residual_denom reads (a strand step on the plain warm start).
 void launch_strand_solve(yb_sim* s, int buf = 0, SolverCtl* ctl = nullptr, int sweep = 0,
-                         float* sys = nullptr, bool denomPartials = false);
-// Relative-residual denominator |f(dx_0)|^2 of the substep from the plain warm
+                         float* sys = nullptr);
+// Newton-residual denominator |f(dx_0)|^2 of the substep from the plain warm
 // start in dx[0] (after launch_ctl_reset; before the warm start is changed).
-void launch_residual_denom(yb_sim* s, bool partialsWritten);
+void launch_residual_denom(yb_sim* s);
 void launch_end(yb_sim* s, int fin, bool adaptive);         // iteration.cu:48-61
 int sweep_waves(int n);
 // learned warm start (mlp.hip)

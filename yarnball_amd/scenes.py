@@ -8,6 +8,7 @@ the device and the CPU oracle start from byte-identical state.
                      top end rotated (the twist driver pattern of main.cpp:54-87)
 * ``knit_patch``     interlocked sinusoidal rows, one edge pinned, the opposite
                      edge pulled (the pull driver pattern of main.cpp:90-121)
+* ``glued_rings``    a chain of interlocked closed rings, ends glued (glue paths)
 * ``random_walk``    one strand as a seeded random walk (batched scenes)
 """
 import ctypes
@@ -187,6 +188,40 @@ def knit_patch(rows=256, per=4096, yarn_r=1e-4, seg_len=2e-4, seed=0, numItr=8):
     sc.drive_rate = 0.05
     # The pull is a constant velocity: set it once.
     sc.vels[right, 0] = np.float32(0.05)
+    return sc
+
+
+def glued_rings(rings=4, per=64, yarn_r=1e-4, ring_r=2e-3, seed=0, numItr=8):
+    """A chain of interlocked closed rings: ring k is centred on the x axis at
+    k (2 R - 1.8 r) and lies in the x-y plane (even k) or the x-z plane (odd
+    k), so each ring passes through its neighbours' holes and touches each of
+    them at two points where their segments cross, 1.8 yarn radii apart.  Each
+    ring is one strand whose last vertex sits on its first and is glued to it
+    (connectionIndex both ways, as the reference connects closed curves,
+    reader.cpp:38-42), with its ends a quarter turn from the contacts; the
+    vertex opposite the ends is pinned and gravity pulls along -y, so the glue
+    spring carries load."""
+    rng = np.random.default_rng(seed)
+    n = rings * per
+    pos = np.zeros((n, 3))
+    spacing = 2.0 * ring_r - 1.8 * yarn_r
+    ang = np.pi / 2 + 2 * np.pi * np.arange(per) / (per - 1)
+    for r in range(rings):
+        sl = slice(r * per, (r + 1) * per)
+        pos[sl, 0] = r * spacing + ring_r * np.cos(ang)
+        pos[sl, 1 if r % 2 == 0 else 2] = ring_r * np.sin(ang)
+        pos[r * per + 1: (r + 1) * per - 1] += rng.normal(0.0, 0.02 * yarn_r, (per - 2, 3))
+        pos[(r + 1) * per - 1] = pos[r * per]
+    breaks = [r * per + per - 1 for r in range(rings)]
+    pinned = [r * per + per // 2 for r in range(rings)]
+    radius, barrier = _curve_radius(yarn_r)
+    sc = _configure(f"glued_rings_{n}", pos.astype(np.float32), breaks, pinned, kCollision=1e-7,
+                    kBend=1e-8, kStretch=2e-2, maxH=1e-3, radius=radius, barrier=barrier,
+                    numItr=numItr)
+    cid = sc.verts["connectionIndex"]
+    for r in range(rings):
+        cid[r * per] = r * per + per - 1
+        cid[r * per + per - 1] = r * per
     return sc
 
 

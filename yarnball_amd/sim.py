@@ -125,6 +125,9 @@ class Sim:
             print("ERROR: MAX_COLLISIONS_PER_SEGMENT exceeded. Current simulation state may be "
                   "corrupted!", file=sys.stderr)
             raise YarnBallError(rc, "MAX_COLLISIONS_PER_SEGMENT exceeded")
+        if rc == _lib.YB_ERROR_TRAVERSAL_STACK_EXCEEDED:
+            self.lastErrorCode = rc
+            raise YarnBallError(rc, "broadphase traversal stack exceeded")
         check(rc, "advance")
         return float(dt)
 
