@@ -441,6 +441,19 @@ int yb_mlp_grad(const float* X, const float* T, int64_t n, const float* params, 
                 float* loss);
 /* Forward pass: Y (n x 3) in normalised output units. */
 int yb_mlp_forward(const float* X, int64_t n, const float* params, float* Y);
+/* The same forward pass without allocating: X (n x 48 floats), params
+ * (YB_MLP_PARAMS floats) and Y (n x 3 floats) are device pointers; the work is
+ * enqueued on `stream` (a hipStream_t; null: the default stream) and the call
+ * returns without waiting.  No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops).  Its scratch -- packed weights and the normalised inputs, about
+ * 6.4 MB -- is one static array per GPU, held until the process ends, so
+ * n <= YB_MLP_NOALLOC_MAX_ROWS (larger n: YB_ERR_INVALID_ARGUMENT; split the
+ * batch), and calls that overlap on one GPU race.  Y equals yb_mlp_forward's
+ * bit for bit. */
+#define YB_MLP_NOALLOC_MAX_ROWS 65536
+int yb_mlp_forward_device(const float* X, int64_t n, const float* params, float* Y,
+                          void* stream);
 
 /* ---- live per-stage timing (measurement harness) --------------------------
  * Runs `substeps` substeps of length h with the yb_advance schedule, enqueued

@@ -24,6 +24,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <numeric>
 #include <random>
 #include <string>
@@ -1010,6 +1011,75 @@ extern "C" int yb_mlp_forward(const float* X, int64_t n, const float* params, fl
     if (e == hipSuccess) e = hipStreamSynchronize(tr.st);
     (void)hipFree(dY);
     HIPCHECK(e);
+    return YB_OK;
+}
+
+// ---- allocation-free forward pass -------------------------------------------------
+// Scratch of yb_mlp_forward_device: the packed bf16 weight tiles, the biases and
+// the normalised bf16 inputs of up to YB_MLP_NOALLOC_MAX_ROWS rows.  One copy
+// per GPU, held until the process ends; every call on a GPU uses it, so calls
+// that overlap on one GPU race.  Each call rewrites every byte it reads (the
+// pack writes all tiles and biases, the normalisation every row < n), so
+// nothing is carried over from the previous call.
+constexpr size_t kScratchTiles = 0;
+constexpr size_t kScratchBias =
+    (kScratchTiles + sizeof(uint16_t) * kMlpTileBufElems + 255) / 256 * 256;
+constexpr size_t kScratchX = (kScratchBias + sizeof(float) * (2 * kMlpHid + 16) + 255) / 256 * 256;
+constexpr size_t kScratchBytes =
+    kScratchX + sizeof(uint16_t) * (size_t)kMlpIn * YB_MLP_NOALLOC_MAX_ROWS;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchLock;
+unsigned char* g_scratchPtr[kMaxDevices] = {};  // device address of `scratch` per device
+
+// The device address of this GPU's scratch, looked up once per device (with
+// that device current) on its first call.
+int scratch_for_current_device(unsigned char** out) {
+    int dev = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_message("yb_mlp_forward_device: device index beyond 63");
+        return YB_ERR_INVALID_ARGUMENT;
+    }
+    std::lock_guard<std::mutex> lock(g_scratchLock);
+    if (!g_scratchPtr[dev]) {
+        void* p = nullptr;
+        HIPCHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratchPtr[dev] = static_cast<unsigned char*>(p);
+    }
+    *out = g_scratchPtr[dev];
+    return YB_OK;
+}
+}  // namespace
+
+extern "C" int yb_mlp_forward_device(const float* dX, int64_t n, const float* dParams, float* dY,
+                                     void* stream) {
+    if (!dX || !dParams || !dY || n < 1) {
+        set_message("yb_mlp_forward_device: null pointer or n < 1");
+        return YB_ERR_INVALID_ARGUMENT;
+    }
+    if (n > YB_MLP_NOALLOC_MAX_ROWS) {
+        set_message("yb_mlp_forward_device: n = " + std::to_string(n) +
+                    " rows exceeds YB_MLP_NOALLOC_MAX_ROWS (65536); split the batch");
+        return YB_ERR_INVALID_ARGUMENT;
+    }
+    unsigned char* base = nullptr;
+    const int rc = scratch_for_current_device(&base);
+    if (rc != YB_OK) return rc;
+    uint16_t* tiles = reinterpret_cast<uint16_t*>(base + kScratchTiles);
+    float* bias = reinterpret_cast<float*>(base + kScratchBias);
+    uint16_t* Xb = reinterpret_cast<uint16_t*>(base + kScratchX);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    // the kernels of yb_mlp_forward, on the caller's buffers and stream
+    hipLaunchKernelGGL(k_norm_data, dim3(nblk(n * kMlpIn, 256)), dim3(256), 0, st, dX,
+                       (const float*)nullptr, n, dParams, Xb, (float*)nullptr);
+    hipLaunchKernelGGL(k_mlp_pack, dim3(nblk(kAllTiles * 64, 256)), dim3(256), 0, st, dParams,
+                       tiles, bias);
+    hipLaunchKernelGGL(k_forward_only, dim3(nblk((n + 63) / 64, 4)), dim3(256), 0, st, tiles, bias,
+                       Xb, (int)n, dY);
+    HIPCHECK(hipGetLastError());
     return YB_OK;
 }
 
