@@ -441,7 +441,7 @@ def main():
     # broadphase counts on the headline method's final state: candidate leaves
     # p and accepted contacts k per segment, for the query's byte model
     qsim = runs[main_m]["sim"]
-    qc = np.zeros(3, np.int64)
+    qc = np.zeros(6, np.int64)
     qp = qsim.meta.copy()
     check(qsim._lib.yb_debug_query_counts(qsim.handle, ctypes.byref(qp), np.float32(sc.maxH),
                                           ptr(qc)))
@@ -508,6 +508,9 @@ def main():
     if "query" in broad:
         broad["query"]["candidates_per_segment"] = round(float(p_cand), 3)
         broad["query"]["accepted_per_segment"] = round(float(k_acc), 3)
+        broad["query"]["node_visits_per_segment"] = round(float(qc[3] / segs_q), 2)
+        broad["query"]["stack_past_lds_fraction"] = round(float(qc[4] / segs_q), 4)
+        broad["query"]["max_stack_depth"] = int(qc[5])
     print(f"[bench] broadphase: p {p_cand:.2f} candidates, k {k_acc:.2f} contacts per segment; "
           + ", ".join(f"{k_} {v['avg_launch_us']:.1f} us {v['frac']:.3f} of peak"
                       for k_, v in broad.items()), file=sys.stderr, flush=True)

@@ -1103,30 +1103,37 @@ extern "C" int yb_debug_prologue(yb_sim* s, yb_params* p, float h, int detect, i
     return check_errors(s, nullptr);
 }
 
-extern "C" int yb_debug_query_counts(yb_sim* s, yb_params* p, float h, int64_t out[3]) {
+extern "C" int yb_debug_query_counts(yb_sim* s, yb_params* p, float h, int64_t out[6]) {
     if (!s || !p || !out) return YB_ERR_INVALID_ARGUMENT;
     const size_t N = (size_t)s->n;
-    if (!s->b.candCount) HIPCHECK(hipMalloc(&s->b.candCount, sizeof(int32_t) * N));
+    if (!s->b.candCount) HIPCHECK(hipMalloc(&s->b.candCount, sizeof(int32_t) * 3 * N));
     s->countCandidates = true;
     int rc = yb_debug_prologue(s, p, h, 1, 0);
     s->countCandidates = false;
     if (rc != YB_OK) return rc;
-    std::vector<int32_t> cand(N), nc(N);
+    std::vector<int32_t> cand(3 * N), nc(N);
     std::vector<float4> prm(N);
-    HIPCHECK(hipMemcpy(cand.data(), s->b.candCount, sizeof(int32_t) * N, hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(cand.data(), s->b.candCount, sizeof(int32_t) * 3 * N,
+                       hipMemcpyDeviceToHost));
     HIPCHECK(hipMemcpy(nc.data(), s->b.numCols, sizeof(int32_t) * N, hipMemcpyDeviceToHost));
     HIPCHECK(hipMemcpy(prm.data(), s->b.prm, sizeof(float4) * N, hipMemcpyDeviceToHost));
-    int64_t segs = 0, c = 0, k = 0;
+    int64_t segs = 0, c = 0, k = 0, visits = 0, deep = 0, maxDepth = 0;
     for (size_t i = 0; i < N; i++) {
         uint32_t fl;
         memcpy(&fl, &prm[i].w, sizeof(fl));
         if (fl & YB_FLAG_HAS_NEXT) segs++;
         c += cand[i];
         k += nc[i];
+        visits += cand[N + i];
+        deep += cand[2 * N + i] > 8 ? 1 : 0;
+        maxDepth = std::max<int64_t>(maxDepth, cand[2 * N + i]);
     }
     out[0] = segs;
     out[1] = c;
     out[2] = k;
+    out[3] = visits;
+    out[4] = deep;
+    out[5] = maxDepth;
     return YB_OK;
 }
 

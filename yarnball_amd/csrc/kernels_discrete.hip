@@ -847,7 +847,11 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
     if (i >= n) return;
     if (!(__float_as_uint(prm[i].w) & YB_FLAG_HAS_NEXT)) {
         numCols[i] = 0;
-        if (candCount) candCount[i] = 0;
+        if (candCount) {
+            candCount[i] = 0;
+            candCount[n + i] = 0;
+            candCount[2 * n + i] = 0;
+        }
         return;
     }
     const float4 qlo4 = aabb[2 * i];
@@ -872,7 +876,9 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
     // together instead of once per node visit at which any thread met a leaf.
     // Candidates keep their visit order, hence so do the contact lists.
     int nb = 0;
-    int cand = 0;  // leaves whose boxes overlap the query box (measurement only)
+    // measurement only: leaves whose boxes overlap the query box, node visits,
+    // deepest stack
+    int cand = 0, visits = 0, maxSp = 0;
     auto narrowphase = [&]() {
         cand += nb;
 #pragma unroll 1
@@ -892,6 +898,10 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
         nb = 0;
     };
     while (true) {
+        if (candCount) {
+            visits++;
+            maxSp = max(maxSp, sp);
+        }
         const float4 l0 = nodes[4 * node + 0];
         const float4 l1 = nodes[4 * node + 1];
         const float4 r0 = nodes[4 * node + 2];
@@ -950,7 +960,11 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
     }
     if (overflowStack) atomicOr(&err[0], kErrBitStack);
     numCols[i] = cnt;
-    if (candCount) candCount[i] = cand;
+    if (candCount) {
+        candCount[i] = cand;
+        candCount[n + i] = visits;
+        candCount[2 * n + i] = maxSp;
+    }
 }
 
 // Flags of each scene's last vertex (which must not start a segment).
