@@ -242,7 +242,8 @@ int yb_debug_prologue(yb_sim* s, yb_params* p, float h, int detect, int rebuild)
 /* Broadphase counts for the query's byte model: runs yb_debug_prologue(s, p,
  * h, detect 1, refit) once more with counting on (so it changes the state as
  * that call does) and returns, summed over segments: out[0] segments, out[1]
- * candidate leaves whose boxes overlap a query box, out[2] accepted contacts
+ * candidate leaves whose boxes overlap a query box (leaves within two indices,
+ * always exempt, not counted), out[2] accepted contacts
  * (the list lengths), out[3] tree nodes visited; out[4] the segments whose
  * traversal stack grew past 8 entries, out[5] the deepest stack. */
 int yb_debug_query_counts(yb_sim* s, yb_params* p, float h, int64_t out[6]);

@@ -876,9 +876,15 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
     // together instead of once per node visit at which any thread met a leaf.
     // Candidates keep their visit order, hence so do the contact lists.
     int nb = 0;
-    // measurement only: leaves whose boxes overlap the query box, node visits,
-    // deepest stack
+    // measurement only: leaves more than two indices away whose boxes overlap
+    // the query box, node visits, deepest stack
     int cand = 0, visits = 0, maxSp = 0;
+    // The first four partners stay in registers and are stored at the end,
+    // slot by slot, by the whole wave at once (one coalesced 256-byte store
+    // per slot); storing each partner when it is found left every slot's lines
+    // partly written at different times, and the cache wrote them back several
+    // times over.  Later partners are stored as they are found.
+    int32_t r0 = 0, r1 = 0, r2 = 0, r3 = 0;
     auto narrowphase = [&]() {
         cand += nb;
 #pragma unroll 1
@@ -891,7 +897,11 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
             sj.vpad = fabsf(pw);
             sj.glued = signbit(pw);
             if (pair_accepted(P, prm, i, si, j, sj)) {
-                if (cnt < kMaxCols) cols[i + (size_t)n * cnt] = j;
+                r0 = cnt == 0 ? j : r0;
+                r1 = cnt == 1 ? j : r1;
+                r2 = cnt == 2 ? j : r2;
+                r3 = cnt == 3 ? j : r3;
+                if (cnt >= 4 && cnt < kMaxCols) cols[i + (size_t)n * cnt] = j;
                 cnt++;
             }
         }
@@ -910,12 +920,15 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
         const int cr = __float_as_int(r0.w);
         const bool hl = overlap(qlo, qhi, l0, l1);
         const bool hr = overlap(qlo, qhi, r0, r1);
-        if (hl && cl < 0 && ~cl != i) {
+        // leaves within two indices of i (itself and its strand neighbours,
+        // which always overlap) are exempt from the pair test (collision.cu:54):
+        // they are dropped here, before their positions are gathered
+        if (hl && cl < 0 && abs(~cl - i) > 2) {
             candJ[nb][t] = ~cl;
             candPad[nb][t] = l1.w;
             nb++;
         }
-        if (hr && cr < 0 && ~cr != i) {
+        if (hr && cr < 0 && abs(~cr - i) > 2) {
             candJ[nb][t] = ~cr;
             candPad[nb][t] = r1.w;
             nb++;
@@ -953,6 +966,10 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
         }
     }
     narrowphase();
+    if (cnt > 0) cols[i] = r0;
+    if (cnt > 1) cols[i + (size_t)n] = r1;
+    if (cnt > 2) cols[i + 2 * (size_t)n] = r2;
+    if (cnt > 3) cols[i + 3 * (size_t)n] = r3;
     // err[0] collects one bit per failure kind (kErrBit*), read by the host
     if (cnt > kMaxCols) {
         atomicOr(&err[0], kErrBitContacts);
