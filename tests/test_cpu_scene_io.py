@@ -1,6 +1,8 @@
 """Scene files on the host: the lenient JSON scene description and its
 fields, Catmull-Rom resampling (closed forms: a straight line resamples to
-uniform spacing, a circle stays on the circle), endpoint gluing against a
+uniform spacing, a circle to equal chords 2 sin(theta / 2m) on the circle, a
+helix to equal chords with exact ends, a loop narrower than a segment does
+not break the spacing), endpoint gluing against a
 brute-force restatement of the reference loop (jsonBuilder.cpp:174-212), and
 BCC / OBJ round trips."""
 import json
@@ -64,8 +66,44 @@ def test_circle_resampling_stays_on_the_circle(tmp_path):
     L = seg_lengths(sc.verts)
     assert abs(L.mean() - 0.05) < 0.05 * 0.02 and L.max() - L.min() < 0.05 * 0.02
     assert np.max(np.abs(np.linalg.norm(p[:, :2], axis=1) - 1.0)) < 1e-4
+    # the spline runs from control point 1 to n-2; m equal chords of the arc
+    # between them are 2 sin(theta / 2m) long, m = round(chord length / 0.05)
+    theta, m = a[-2] - a[1], sc.n - 1
+    assert m == round(2 * np.sin(theta / (2 * m)) * m / 0.05)
+    assert np.allclose(L, 2 * np.sin(theta / (2 * m)), rtol=1e-5)
+    assert np.allclose(p[0], cp[1], atol=1e-6) and np.allclose(p[-1], cp[-2], atol=1e-6)
     # closed curve: ends connected to each other
     assert sc.verts["connectionIndex"][0] == sc.n - 1 and sc.verts["connectionIndex"][-1] == 0
+
+
+def test_helix_resamples_to_equal_chords(tmp_path):
+    t = np.linspace(0, 6 * np.pi, 150)
+    cp = np.stack([np.cos(t), np.sin(t), 0.3 * t], axis=1)
+    write_poly(tmp_path / "helix.poly", [(cp, False)])
+    sc = scene_io.read_curves(tmp_path / "helix.poly", 0.07, transform=IDENTITY)
+    p = sc.verts["pos"].astype(np.float64)
+    L = seg_lengths(sc.verts)
+    m = sc.n - 1
+    assert np.allclose(L, L.mean(), rtol=1e-5)           # equal chords (FP32 positions)
+    assert abs(L.mean() - 0.07) <= 0.07 / (2 * m) + 1e-7  # m = round(length / 0.07)
+    assert np.allclose(p[0], cp[1], atol=1e-6) and np.allclose(p[-1], cp[-2], atol=1e-6)
+
+
+def test_loop_narrower_than_a_segment(tmp_path):
+    # a straight run with a loop of diameter 0.06 in it, resampled at 0.1:
+    # the spacing stays even and the ends exact
+    s = np.linspace(0, 3, 60)
+    line = np.stack([s, 0 * s, 0 * s], axis=1)
+    th = np.linspace(0, 2 * np.pi, 40)[1:-1]
+    loop = np.stack([1.5 + 0.03 * np.sin(th), 0.03 - 0.03 * np.cos(th), 0 * th], axis=1)
+    cp = np.concatenate([line[s < 1.5], loop, line[s >= 1.5]])
+    write_poly(tmp_path / "loop.poly", [(cp, False)])
+    sc = scene_io.read_curves(tmp_path / "loop.poly", 0.1, transform=IDENTITY)
+    p = sc.verts["pos"].astype(np.float64)
+    L = seg_lengths(sc.verts)
+    assert np.isfinite(p).all() and sc.n > 25
+    assert np.allclose(L, L.mean(), rtol=1e-4) and abs(L.mean() - 0.1) < 0.1 / sc.n
+    assert np.allclose(p[0], cp[1], atol=1e-6) and np.allclose(p[-1], cp[-2], atol=1e-6)
 
 
 def test_json_scene_fields(tmp_path):

@@ -191,7 +191,7 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(
-            f"{LIB_PATH} is missing: build it with `python -m yarnball_amd.build` "
+            f"{LIB_PATH} is missing: build it with `make -C yarnball_amd/csrc` "
             "(the stepper has no CPU fallback)"
         )
     lib = ctypes.CDLL(LIB_PATH)

@@ -2,13 +2,13 @@
 // the JSON scene description, the curve file readers with Catmull-Rom
 // resampling, endpoint gluing, and the BCC / OBJ writers.
 //
-// Reference behaviour restated here (no code shared):
+// Reference behaviour restated here:
 //   buildFromJSON            KittenEngine/YarnBall/io/jsonBuilder.cpp:8-170
 //   Sim::glueEndpoints       jsonBuilder.cpp:174-212
 //   createFromCurves         KittenEngine/YarnBall/io/reader.cpp:13-48
 //   readFromBCC / OBJ / Poly reader.cpp:50-295
-//   resampleCMR and helpers  KittenEngine/YarnBall/io/resample.h:10-188
-//   cmrSpline                KittenEngine/includes/modules/Common.h:681-694
+//   resampleCMR (contract)   KittenEngine/YarnBall/io/resample.h:10-188
+//   cmrSpline (the curve)    KittenEngine/includes/modules/Common.h:681-694
 //   exportToBCC / OBJ        KittenEngine/YarnBall/io/fileExport.cpp:6-100
 //   BCCHeader                KittenEngine/YarnBall/YarnBall.h:23-32
 //
@@ -18,7 +18,12 @@
 // first segment listed for it (the reference takes whatever its hash map
 // yields first); glueEndpoints visits endpoints in index order and finds the
 // nearest vertex through a uniform grid instead of an O(E N) scan (the
-// reference's OpenMP loop makes the order of competing glues a race).
+// reference's OpenMP loop makes the order of competing glues a race); the
+// resampler solves for equal chords over the whole curve at once (see
+// equalize_chords) where the reference marches chords from both ends,
+// splices the two marches and blends the seams, so vertex positions differ
+// from the reference's by a fraction of a segment while counts and spacing
+// agree.
 #include <float.h>
 #include <math.h>
 #include <stdint.h>
@@ -260,8 +265,6 @@ inline d3 operator*(d3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
 inline double dlen2(d3 a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
 inline double dlen(d3 a) { return sqrt(dlen2(a)); }
 inline d3 tod(f3 p) { return {(double)p.x, (double)p.y, (double)p.z}; }
-// x (1 - a) + y a, the interpolation of glm::mix for floating-point vectors
-inline d3 dmix(d3 x, d3 y, double a) { return x * (1.0 - a) + y * a; }
 
 // 3x4 affine map, row-major: p'_i = sum_j m[i][j] p_j + m[i][3], evaluated
 // in float column by column like a 4x4 matrix times (p, 1).
@@ -275,147 +278,246 @@ struct Affine {
     }
 };
 
-// Uniform Catmull-Rom segment between p1 and p2, t in [0, 1], as the
-// pyramid of linear interpolations of Common.h:681-694.
-inline d3 cmr(d3 p0, d3 p1, d3 p2, d3 p3, double t) {
-    const d3 a0 = dmix(p0, p1, t + 1.0);
-    const d3 a1 = dmix(p1, p2, t);
-    const d3 a2 = dmix(p2, p3, t - 1.0);
-    const d3 b0 = dmix(a0, a1, (t + 1.0) * 0.5);
-    const d3 b1 = dmix(a1, a2, t * 0.5);
-    return dmix(b0, b1, t);
-}
-
 // ---------------------------------------------------------------------------
-// Resampling of a Catmull-Rom spline through control points cp to points a
-// target distance apart (resample.h:10-188).  The curve parameter t runs
-// over [1, n - 2]; segment floor(t) uses control points floor(t)-1 .. +2.
-class Resampler {
+// Uniform Catmull-Rom spline through control points cp[0 .. n-1].  The curve
+// runs from cp[1] (u = 1) to cp[n-2] (u = n-2); span k, k <= u < k+1,
+// interpolates cp[k] -> cp[k+1] with end tangents (cp[k+1] - cp[k-1]) / 2
+// and (cp[k+2] - cp[k]) / 2 (the curve and parameter range the reference
+// samples, resample.h:10-18 over Common.h:681-694).
+class CmrCurve {
 public:
-    explicit Resampler(const std::vector<f3>& cp) : cp_(cp) {}
+    explicit CmrCurve(const std::vector<f3>& cp) : cp_(cp), n_((int)cp.size()) {}
+    double first() const { return 1.0; }
+    double last() const { return n_ - 2.0; }
+    int spans() const { return n_ - 3; }
 
-    d3 at(double t) const {
-        const int n = (int)cp_.size();
-        const int si = std::min(std::max((int)floor(t), 1), n - 3);
-        return cmr(tod(cp_[si - 1]), tod(cp_[si]), tod(cp_[si + 1]), tod(cp_[si + 2]), t - si);
+    d3 at(double u) const {
+        const int k = std::min(std::max((int)std::floor(u), 1), n_ - 3);
+        const double s = u - k, s2 = s * s, s3 = s2 * s;
+        const d3 p0 = tod(cp_[k - 1]), p1 = tod(cp_[k]), p2 = tod(cp_[k + 1]), p3 = tod(cp_[k + 2]);
+        // cubic Hermite basis with the Catmull-Rom tangents
+        return p1 * (2 * s3 - 3 * s2 + 1) + (p2 - p0) * (0.5 * (s3 - 2 * s2 + s)) +
+               p2 * (3 * s2 - 2 * s3) + (p3 - p1) * (0.5 * (s3 - s2));
     }
 
-    // Parameters of points spaced segLen apart from start towards end: march
-    // in steps of 0.2, and on each overshoot place the point by eight secant
-    // steps; afterwards pin the last point to `end` and spread the change
-    // over the last eight intervals.
-    std::vector<double> march(double start, double end, double segLen) const {
-        std::vector<double> ts;
-        const double step = end > start ? 0.2 : -0.2;
-        const double r2 = segLen * segLen;
-        double t = start;
-        d3 last = at(t);
-        ts.push_back(t);
-        while (end > start ? t < end : t > end) {
-            t += step;
-            double l0 = dlen2(at(t) - last);
-            if (l0 > r2) {
-                l0 = sqrt(l0);
-                double lo = t - step, hi = t;
-                if (lo > hi) std::swap(lo, hi);
-                const double t0 = t;
-                t -= 0.5 * step;
-                for (int k = 0; k < 8; k++) {
-                    const double d = dlen(at(t) - last);
-                    t += (t - t0) * (segLen - d) / (d - l0);
-                }
-                t = std::min(std::max(t, lo), hi);
-                ts.push_back(t);
-                last = at(t);
-            }
-        }
-        if (ts.size() == 1) {
-            ts.push_back(end);
-        } else {
-            const double gap = dlen(at(ts.back()) - at(end));
-            double dt = ts.back() - ts[ts.size() - 2];
-            if (gap > 0.5f * segLen)
-                ts.push_back(end);
-            else
-                ts.back() = end;
-            dt -= ts.back() - ts[ts.size() - 2];
-            const int m = std::min((int)ts.size(), 8);
-            for (int k = 0; k < m; k++) ts[ts.size() - 2 - k] -= dt * (m - k) / (double)m;
-        }
-        return ts;
+    // dx/du, the derivative of the Hermite form above
+    d3 tangent(double u) const {
+        const int k = std::min(std::max((int)std::floor(u), 1), n_ - 3);
+        const double s = u - k, s2 = s * s;
+        const d3 p0 = tod(cp_[k - 1]), p1 = tod(cp_[k]), p2 = tod(cp_[k + 1]), p3 = tod(cp_[k + 2]);
+        return (p2 - p1) * (6 * (s - s2)) + (p2 - p0) * (0.5 * (3 * s2 - 4 * s + 1)) +
+               (p3 - p1) * (0.5 * (3 * s2 - 2 * s));
     }
 
-    // March from both ends and splice the two where their parameters agree
-    // best, blending eight points on either side of the splice.
-    std::vector<double> march2(double start, double end, double segLen) const {
-        const std::vector<double> fw = march(start, end, segLen);
-        const std::vector<double> bw = march(end, start, segLen);
-        constexpr int kBorder = 8;
-        if (!((int)fw.size() > 4 * kBorder && (int)bw.size() > 4 * kBorder)) return fw;
-        double best = INFINITY;
-        int sx = 2 * kBorder, sy = 2 * kBorder;
-        for (int i = kBorder, j = (int)bw.size() - 1 - kBorder; i < (int)fw.size() - 8; i++) {
-            const double t = fw[i];
-            while (j > 0 && bw[j] < t) j--;
-            if (best > fabs(t - bw[j + 1])) {
-                best = fabs(t - bw[j + 1]);
-                sx = i;
-                sy = j + 1;
-            }
-            if (best > fabs(bw[j] - t)) {
-                best = fabs(bw[j] - t);
-                sx = i;
-                sy = j;
-            }
-        }
-        std::vector<double> out(fw.begin(), fw.begin() + sx);
-        const double mid = 0.5 * (fw[sx] + bw[sy]);
-        out.push_back(mid);
-        for (int k = sy - 1; k >= 0; k--) out.push_back(bw[k]);
-        double dt = fw[sx] - mid;
-        for (int k = 0; k < kBorder; k++) out[sx - k - 1] -= dt * (kBorder - k) / (1.0 + kBorder);
-        dt = bw[sy] - mid;
-        for (int k = 0; k < kBorder; k++) out[sx + k + 1] -= dt * (kBorder - k) / (1.0 + kBorder);
-        return out;
-    }
-
-    // Two passes: the target length, then the first pass's mean length;
-    // keep whichever has the smaller worst deviation from its target.
-    std::vector<f3> resample(double start, double end, double segLen) const {
-        auto stats = [&](const std::vector<double>& ts, double target, double& total) {
-            double err = 0.0;
-            total = 0.0;
-            d3 last = at(ts[0]);
-            for (size_t k = 1; k < ts.size(); k++) {
-                const d3 p = at(ts[k]);
-                const double l = dlen(p - last);
-                err = std::max(err, fabs(l - target));
-                total += l;
-                last = p;
-            }
-            return err;
-        };
-        const std::vector<double> ts0 = march2(start, end, segLen);
-        double total0 = 0.0, total1 = 0.0;
-        const double err0 = stats(ts0, segLen, total0);
-        const double avg = total0 / (double)(ts0.size() - 1);
-        const std::vector<double> ts1 = march2(start, end, avg);
-        const double err1 = stats(ts1, avg, total1);
-        const std::vector<double>& ts = err0 < err1 ? ts0 : ts1;
-        std::vector<f3> out(ts.size());
-        for (size_t k = 0; k < ts.size(); k++) {
-            const d3 p = at(ts[k]);
-            out[k] = f3{(float)p.x, (float)p.y, (float)p.z};
-        }
-        return out;
+    // Length of the Bezier control polygon of span k (points p1,
+    // p1 + (p2 - p0) / 6, p2 - (p3 - p1) / 6, p2): an upper bound of the
+    // span's arc length.
+    double span_bound(int k) const {
+        const d3 p0 = tod(cp_[k - 1]), p1 = tod(cp_[k]), p2 = tod(cp_[k + 1]), p3 = tod(cp_[k + 2]);
+        const d3 b1 = p1 + (p2 - p0) * (1.0 / 6.0), b2 = p2 - (p3 - p1) * (1.0 / 6.0);
+        return dlen(b1 - p1) + dlen(b2 - b1) + dlen(p2 - b2);
     }
 
 private:
     const std::vector<f3>& cp_;
+    int n_;
 };
 
+// Arc length of the curve: parameters tabulated at most segLen / 8 of arc
+// apart (by the control-polygon bound), the arc between neighbours by
+// 5-point Gauss-Legendre quadrature of |dx/du|, accumulated.
+class ArcTable {
+public:
+    ArcTable(const CmrCurve& cv, double segLen) : cv_(cv) {
+        for (int k = 1; k <= cv.spans(); k++) {
+            const int m = (int)std::min(std::max(std::ceil(8.0 * cv.span_bound(k) / segLen), 1.0), 65536.0);
+            for (int j = 0; j < m; j++) us_.push_back(k + j / (double)m);
+        }
+        us_.push_back(cv.last());
+        s_.assign(us_.size(), 0.0);
+        for (size_t i = 1; i < us_.size(); i++) s_[i] = s_[i - 1] + arc(us_[i - 1], us_[i]);
+    }
+
+    double length() const { return s_.back(); }
+
+    // arc length from u0 to u1 inside one table interval
+    double arc(double u0, double u1) const {
+        static const double x[5] = {-0.9061798459386640, -0.5384693101056831, 0.0,
+                                    0.5384693101056831, 0.9061798459386640};
+        static const double w[5] = {0.2369268850561891, 0.4786286704993665, 0.5688888888888889,
+                                    0.4786286704993665, 0.2369268850561891};
+        const double h = 0.5 * (u1 - u0), c = 0.5 * (u0 + u1);
+        double sum = 0.0;
+        for (int q = 0; q < 5; q++) sum += w[q] * dlen(cv_.tangent(c + h * x[q]));
+        return sum * h;
+    }
+
+    // The parameter at arc length s from the start: the table interval
+    // holding s, then bisection on the arc inside it.
+    double param_at(double s) const {
+        size_t i = (size_t)(std::upper_bound(s_.begin(), s_.end(), s) - s_.begin());
+        i = std::min(std::max(i, (size_t)1), s_.size() - 1) - 1;
+        double lo = us_[i], hi = us_[i + 1];
+        const double rest = s - s_[i];
+        for (int b = 0; b < 60; b++) {
+            const double mid = 0.5 * (lo + hi);
+            if (mid <= lo || mid >= hi) break;
+            (arc(us_[i], mid) < rest ? lo : hi) = mid;
+        }
+        return 0.5 * (lo + hi);
+    }
+
+private:
+    const CmrCurve& cv_;
+    std::vector<double> us_, s_;
+};
+
+// Solves for parameters u_1 .. u_{m-1} (u_0 and u_m fixed) and a chord
+// length c so that every chord |x(u_{k+1}) - x(u_k)| equals c: Gauss-Newton
+// on the residuals r_k = |x(u_{k+1}) - x(u_k)| - c with Levenberg-Marquardt
+// damping.  Each residual touches two neighbouring parameters and c, so the
+// normal equations are tridiagonal with one dense border row and column,
+// solved by two Thomas sweeps and a Schur complement for c.  Steps that
+// would reorder the parameters, or do not lower the sum of squares, are
+// retried with more damping.  Where the curve has a loop narrower than c no
+// placement on it has equal chords; the solve then settles on the least-
+// squares compromise (points stay on the loop, chords there a little short).
+// Returns the largest |r_k| / c of the last iterate checked.
+double equalize_chords(const CmrCurve& cv, std::vector<double>& u) {
+    const int m = (int)u.size() - 1;
+    std::vector<d3> x(m + 1), t(m + 1);
+    std::vector<double> r(m), a(m), b(m);
+    auto chords = [&](const std::vector<double>& uu, double& sum) {
+        for (int k = 0; k <= m; k++) x[k] = cv.at(uu[k]);
+        sum = 0.0;
+        for (int k = 0; k < m; k++) sum += dlen(x[k + 1] - x[k]);
+    };
+    auto cost = [&](const std::vector<double>& uu, double cc) {
+        double sq = 0.0;
+        for (int k = 0; k < m; k++) {
+            const double l = dlen(cv.at(uu[k + 1]) - cv.at(uu[k])) - cc;
+            sq += l * l;
+        }
+        return sq;
+    };
+    double total = 0.0;
+    chords(u, total);
+    double c = total / m;
+    if (m < 2 || c <= 0.0) return 0.0;
+    const int nu = m - 1;  // unknown parameters u_1 .. u_{m-1}
+    std::vector<double> diag(nu), off(nu), w(nu), g(nu), y(nu), z(nu), cp(nu), du(nu);
+    std::vector<double> trial(m + 1);
+    double lambda = 1e-4, worst = INFINITY;
+    for (int it = 0; it < 100; it++) {
+        double sq = 0.0;
+        worst = 0.0;
+        for (int k = 0; k <= m; k++) {
+            x[k] = cv.at(u[k]);
+            t[k] = cv.tangent(u[k]);
+        }
+        for (int k = 0; k < m; k++) {
+            const d3 d = x[k + 1] - x[k];
+            const double l = dlen(d);
+            const d3 e = l > 0.0 ? d * (1.0 / l) : d3{0, 0, 0};
+            r[k] = l - c;
+            a[k] = -(e.x * t[k].x + e.y * t[k].y + e.z * t[k].z);       // d r_k / d u_k
+            b[k] = e.x * t[k + 1].x + e.y * t[k + 1].y + e.z * t[k + 1].z;  // d r_k / d u_{k+1}
+            sq += r[k] * r[k];
+            worst = std::max(worst, fabs(r[k]));
+        }
+        worst /= c;
+        if (worst <= 1e-10) break;
+        // normal equations: unknown j (0-based) is u_{j+1}, touched by rows j+1 (as u_k) and j (as u_{k+1})
+        double gc = 0.0, sc = (double)m;
+        for (int k = 0; k < m; k++) gc -= r[k];
+        for (int j = 0; j < nu; j++) {
+            const double ak = a[j + 1], bk = b[j];
+            diag[j] = ak * ak + bk * bk;
+            off[j] = j + 1 < nu ? a[j + 1] * b[j + 1] : 0.0;  // (u_{j+1}, u_{j+2}) through row j+1
+            w[j] = -ak - bk;
+            g[j] = ak * r[j + 1] + bk * r[j];
+        }
+        bool accepted = false;
+        for (int tries = 0; tries < 30 && !accepted; tries++) {
+            // Thomas sweeps on T = tridiag(off, diag (1 + lambda), off) for -g and w
+            double prevC = 0.0, prevY = 0.0, prevZ = 0.0;
+            for (int j = 0; j < nu; j++) {
+                const double dj = diag[j] * (1.0 + lambda) + 1e-300;
+                const double lo = j > 0 ? off[j - 1] : 0.0;
+                const double den = dj - lo * prevC;
+                cp[j] = off[j] / den;
+                y[j] = (-g[j] - lo * prevY) / den;
+                z[j] = (w[j] - lo * prevZ) / den;
+                prevC = cp[j];
+                prevY = y[j];
+                prevZ = z[j];
+            }
+            for (int j = nu - 2; j >= 0; j--) {
+                y[j] -= cp[j] * y[j + 1];
+                z[j] -= cp[j] * z[j + 1];
+            }
+            double wy = 0.0, wz = 0.0;
+            for (int j = 0; j < nu; j++) {
+                wy += w[j] * y[j];
+                wz += w[j] * z[j];
+            }
+            const double dc = (-gc - wy) / (sc * (1.0 + lambda) - wz);
+            bool ordered = true;
+            trial[0] = u[0];
+            trial[m] = u[m];
+            for (int j = 0; j < nu; j++) {
+                du[j] = y[j] - z[j] * dc;
+                trial[j + 1] = u[j + 1] + du[j];
+            }
+            for (int k = 0; k < m && ordered; k++) ordered = trial[k + 1] > trial[k];
+            const double sqNew = ordered ? cost(trial, c + dc) : INFINITY;
+            if (sqNew < sq) {
+                u.swap(trial);
+                c += dc;
+                lambda = std::max(lambda * 0.3, 1e-12);
+                accepted = true;
+                // a loop narrower than c leaves a least-squares remainder: stop once it settles
+                if (sq - sqNew <= 1e-9 * sq) return worst;
+            } else {
+                lambda *= 10.0;
+            }
+        }
+        if (!accepted) break;
+    }
+    return worst;
+}
+
+// Resamples the Catmull-Rom spline through cp to points about segLen apart,
+// from cp[1] to cp[n-2] (resampleCMR, resample.h:149-188, has the same
+// contract: consecutive points one straight length apart, both ends on the
+// curve's ends).  The placement is this file's own: a first placement at
+// equal arc length about segLen apart gives the sum S of its chords; the
+// curve gets m = round(S / segLen) segments, starts from equal arc length
+// L / m, and equalize_chords makes every chord the same length c ~ segLen.
 std::vector<f3> resample_cmr(const std::vector<f3>& cp, double segLen) {
-    return Resampler(cp).resample(1.0, (double)cp.size() - 2.0, segLen);
+    const CmrCurve cv(cp);
+    const ArcTable arc(cv, segLen);
+    const double L = arc.length();
+    auto equal_arc = [&](int m) {
+        std::vector<double> u(m + 1);
+        u[0] = cv.first();
+        for (int k = 1; k < m; k++) u[k] = arc.param_at(L * k / m);
+        u[m] = cv.last();
+        return u;
+    };
+    const int m0 = std::max(1, (int)std::lround(L / segLen));
+    const std::vector<double> u0 = equal_arc(m0);
+    double chordSum = 0.0;
+    for (int k = 0; k < m0; k++) chordSum += dlen(cv.at(u0[k + 1]) - cv.at(u0[k]));
+    const int m = std::max(1, (int)std::lround(chordSum / segLen));
+    std::vector<double> u = equal_arc(m);
+    equalize_chords(cv, u);
+    std::vector<f3> out(u.size());
+    for (size_t k = 0; k < u.size(); k++) {
+        const d3 p = cv.at(u[k]);
+        out[k] = f3{(float)p.x, (float)p.y, (float)p.z};
+    }
+    return out;
 }
 
 // ---------------------------------------------------------------------------

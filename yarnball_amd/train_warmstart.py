@@ -96,7 +96,7 @@ def record_scene(sc, frames, stride, keep, tol=1e-6, max_sweeps=6000, seed=0, lo
 
 
 def generate(tol=1e-6, max_sweeps=6000, log=print, strand_every=None, knit_frames=5,
-             strand_first=0):
+             strand_first=0, knit_stride=1):
     """knit_frames: frames of the knit run (the first, the transient, is not
     recorded)."""
     t0 = time.perf_counter()
@@ -105,7 +105,7 @@ def generate(tol=1e-6, max_sweeps=6000, log=print, strand_every=None, knit_frame
         _readme(tol, max_sweeps, log, **st),
         record_scene(scenes.twisted_cable(64, 2048), 8, 4, 8192, tol, max_sweeps, seed=1,
                      log=log, **st),
-        record_scene(scenes.knit_patch(256, 4096, seed=7), knit_frames, 1, 24576, tol,
+        record_scene(scenes.knit_patch(256, 4096, seed=7), knit_frames, knit_stride, 24576, tol,
                      max_sweeps, seed=2, log=log, first_frame=1, **st),
     ]
     X = np.concatenate([p[0] for p in parts])
@@ -146,10 +146,14 @@ def main():
                          "runs (0: the strand-every-th)")
     ap.add_argument("--knit-frames", type=int, default=5,
                     help="frames of the training knit (frame 0 is run, not recorded)")
+    ap.add_argument("--knit-stride", type=int, default=1,
+                    help="record every k-th substep of the training knit (device memory: "
+                         "each recorded substep holds all 1M vertices' rows)")
     args = ap.parse_args()
     log = lambda m: print(m, file=sys.stderr, flush=True)
     X, T = generate(args.tol, log=log, strand_every=args.strand_every,
-                    knit_frames=args.knit_frames, strand_first=args.strand_first)
+                    knit_frames=args.knit_frames, strand_first=args.strand_first,
+                    knit_stride=args.knit_stride)
     p, hist = fit(X, T, epochs=args.epochs, log=log)
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
     np.save(args.out, p)
