@@ -291,6 +291,26 @@ def live_profile(sim, sc, args, substeps_per_frame):
     return ms, nl, c
 
 
+def substep_timeline(sim, frame, substeps_per_frame):
+    """One more frame (untimed) with the residual-mode substep timeline on:
+    mean microseconds per substep of GPU idle before it, prologue, sweeps up
+    to the chunk that converged, chunks queued after that one, and the end
+    kernels, from HIP events between the graph launches; wall = the frame's
+    host clock per substep."""
+    sim.startTimeline(substeps_per_frame)
+    t0 = time.perf_counter()
+    sim.advance(frame)
+    wall = (time.perf_counter() - t0) / substeps_per_frame * 1e6
+    tl = sim.timeline()
+    names = ("idle_us", "prologue_us", "converge_us", "tail_us", "end_us")
+    out = {k: round(float(tl[1:, i].mean()) if len(tl) > 1 else 0.0, 1)
+           for i, k in enumerate(names)}
+    out["wall_us"] = round(wall, 1)
+    out["substeps"] = int(len(tl))
+    out["mean_chunks"] = round(float(tl[:, 6].mean()), 2) if len(tl) else 0.0
+    return out
+
+
 def run_method(sc, args, method, model, strand_model=None):
     """Warm-up frames, then the timed frames, of one warm-start method on a
     fresh simulation; returns the open simulation and the measurements."""
@@ -344,6 +364,13 @@ def run_method(sc, args, method, model, strand_model=None):
     if method.startswith("learned"):
         r["warm_substeps"] = learned.warm_substeps(sim)
         r["vertex_counters"] = learned.counters(sim)
+    if args.residual > 0:
+        r["timeline"] = substep_timeline(sim, frame, substeps_per_frame)
+        t = r["timeline"]
+        print(f"[bench] {method}: substep timeline (us): idle {t['idle_us']}, prologue "
+              f"{t['prologue_us']}, sweeps to the converged chunk {t['converge_us']}, chunks "
+              f"after it {t['tail_us']}, end {t['end_us']}, wall {t['wall_us']}",
+              file=sys.stderr, flush=True)
     # live per-stage timing right after the timed frames, while the GPU runs at
     # its working clocks (a handle left idle first starts with slower kernels)
     r["profile"] = live_profile(sim, sc, args, substeps_per_frame)
@@ -547,6 +574,7 @@ def main():
             row["max_final_residual"] = float(r["stats"]["maxResidual"])
             row["newton_residual_mean"] = float(r["stats"]["meanNewtonResidual"])
             row["newton_residual_max"] = float(r["stats"]["maxNewtonResidual"])
+            row["substep_timeline"] = r.get("timeline")
         if m.startswith("learned"):
             ws = r["warm_substeps"]
             row["substeps_learned_start"] = ws["learned"]

@@ -247,6 +247,20 @@ int yb_debug_prologue(yb_sim* s, yb_params* p, float h, int detect, int rebuild)
  * (the list lengths), out[3] tree nodes visited; out[4] the segments whose
  * traversal stack grew past 8 entries, out[5] the deepest stack. */
 int yb_debug_query_counts(yb_sim* s, yb_params* p, float h, int64_t out[6]);
+/* Substep timeline of the residual-targeted schedule (tol > 0), from HIP
+ * events recorded on the handle's stream between its launches: one before
+ * the prologue, one after it, one after each chunk of sweeps, one after the
+ * end kernels.  yb_debug_timeline(s, m) starts recording the next m substeps
+ * (0 stops and frees the events); yb_debug_timeline_read writes 8 floats per
+ * recorded substep: [0] GPU idle before the substep in us (time from the
+ * previous substep's end event to this one's first; 0 for the first row),
+ * [1] prologue us, [2] us from the prologue to the end of the chunk that held
+ * the converged sweep, [3] us of the chunks queued after it, [4] end kernels
+ * us, [5] sweeps to convergence, [6] chunks launched, [7] the converged chunk
+ * (-1: not converged).  Recording changes no results; each event adds a
+ * little time of its own. */
+int yb_debug_timeline(yb_sim* s, int maxSubsteps);
+int yb_debug_timeline_read(yb_sim* s, float* out, int maxRows, int* rows);
 /* Copy out intermediate arrays (any pointer may be null):
  * dx: 4n floats (xyz + step limit), aabb: 8n floats (lo xyz _, hi xyz _),
  * morton: n u32 indexed by segment, order: n i32 (sorted leaf -> segment),

@@ -100,3 +100,32 @@ def test_newton_residual_stop_matches_the_oracle(name, make, tol):
         if st["sweeps"] == orc.last_sweeps():
             assert st["lastNewtonResidual"] == pytest.approx(orc.newton_residual(), rel=2e-2)
     sim.close()
+
+
+@pytest.mark.gpu
+def test_substep_timeline_records_each_substep():
+    """The residual-mode timeline (yb_debug_timeline): one row per substep,
+    non-negative durations, the converged chunk consistent with the sweep
+    count, and the recorded sweeps equal to the solver's own count."""
+    sc = scenes.knit_patch(rows=8, per=256)
+    sim = gpu_sim(sc)
+    chunk = 8
+    sim.setSolver(1e-5, maxSweeps=4000, chunk=chunk)
+    sim.advance(sim.maxH)  # graphs built, tree built
+    sim.solverStats(reset=True)
+    sim.startTimeline(4)
+    for _ in range(4):
+        sim.advance(sim.maxH)
+    tl = sim.timeline()
+    st = sim.solverStats()
+    print(tl)
+    assert tl.shape == (4, 8)
+    assert np.all(tl[:, :5] >= 0.0) and np.all(tl[:, 1] > 0.0) and np.all(tl[:, 4] > 0.0)
+    sweeps = tl[:, 5].astype(int)
+    assert np.all(sweeps >= 1) and int(sweeps.sum()) == st["sweeps"]
+    assert np.array_equal(tl[:, 7].astype(int), (sweeps - 1) // chunk)
+    assert np.all(tl[:, 6] >= tl[:, 7] + 1)
+    # recording stopped: nothing more is kept
+    sim.advance(sim.maxH)
+    assert sim.timeline().shape == (0, 8)
+    sim.close()

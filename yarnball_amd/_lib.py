@@ -141,6 +141,8 @@ SIGNATURES = {
     "yb_set_residual_kind": (_I, [_P, _I]),
     "yb_debug_set_query_stack": (_I, [_P, _I]),
     "yb_debug_query_counts": (_I, [_P, ctypes.POINTER(Params), _F, _P]),
+    "yb_debug_timeline": (_I, [_P, _I]),
+    "yb_debug_timeline_read": (_I, [_P, _P, _I, _P]),
     "yb_mlp_forward_device": (_I, [_P, ctypes.c_int64, _P, _P, _P]),
     "yb_min_separation": (_I, [_P, _P, _I]),
     "yb_mlp_load": (_I, [_P, _P, _I]),

@@ -40,7 +40,46 @@ using namespace yb;
 
 extern "C" const char* yb_last_message(void) { return g_msg.c_str(); }
 
+// Substep timeline of the residual-targeted schedule (yb_debug_timeline): a
+// HIP event before the prologue, after it, after every chunk and after the
+// end kernels, all recorded between graph launches on the handle's stream.
+struct TimelineRow {
+    int first = 0;   // index of the row's first event in ev
+    int chunks = 0;  // chunk events (launched chunks)
+    int done = 0;    // sweeps to convergence (maxSweeps if not converged)
+    int conv = -1;   // chunk holding the converged sweep, -1 if none
+};
+struct Timeline {
+    int cap = 0;
+    std::vector<hipEvent_t> ev;
+    size_t used = 0;
+    std::vector<TimelineRow> rows;
+    bool open = false;  // a row is being recorded
+};
+
+static void timeline_free(yb_sim* s) {
+    Timeline* t = (Timeline*)s->timeline;
+    if (!t) return;
+    for (hipEvent_t e : t->ev) (void)hipEventDestroy(e);
+    delete t;
+    s->timeline = nullptr;
+}
+
+// Records the next timeline event of the open row (no-op without one).
+static int timeline_mark(yb_sim* s) {
+    Timeline* t = (Timeline*)s->timeline;
+    if (!t || !t->open) return YB_OK;
+    if (t->used == t->ev.size()) {
+        hipEvent_t e;
+        HIPCHECK(hipEventCreate(&e));
+        t->ev.push_back(e);
+    }
+    HIPCHECK(hipEventRecord(t->ev[t->used++], s->stream));
+    return YB_OK;
+}
+
 static void free_all(yb_sim* s) {
+    timeline_free(s);
     Buffers& b = s->b;
     void* ptrs[] = {b.pos,     b.prm,     b.vel,      b.lastVel, b.yv,      b.dx[0],
                     b.dx[1],   b.q[0],    b.q[1],     b.qRest,   b.numCols, b.cols,
@@ -762,6 +801,57 @@ extern "C" int yb_debug_set_query_stack(yb_sim* s, int entries) {
     return YB_OK;
 }
 
+extern "C" int yb_debug_timeline(yb_sim* s, int maxSubsteps) {
+    if (!s || maxSubsteps < 0) {
+        set_message("yb_debug_timeline: maxSubsteps >= 0");
+        return YB_ERR_INVALID_ARGUMENT;
+    }
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    timeline_free(s);
+    if (maxSubsteps > 0) {
+        Timeline* t = new Timeline();
+        t->cap = maxSubsteps;
+        s->timeline = t;
+    }
+    return YB_OK;
+}
+
+extern "C" int yb_debug_timeline_read(yb_sim* s, float* out, int maxRows, int* rows) {
+    if (!s || !out || !rows || maxRows < 0) {
+        set_message("yb_debug_timeline_read: null argument");
+        return YB_ERR_INVALID_ARGUMENT;
+    }
+    *rows = 0;
+    Timeline* t = (Timeline*)s->timeline;
+    if (!t) return YB_OK;
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    auto dt = [&](int a, int b, float* us) -> int {
+        float ms = 0.f;
+        HIPCHECK(hipEventElapsedTime(&ms, t->ev[a], t->ev[b]));
+        *us = 1e3f * ms;
+        return YB_OK;
+    };
+    const int count = std::min((int)t->rows.size() - (t->open ? 1 : 0), maxRows);
+    for (int r = 0; r < count; r++) {
+        const TimelineRow& row = t->rows[r];
+        const int e0 = row.first, e1 = e0 + 1, eLast = e1 + row.chunks, eEnd = eLast + 1;
+        const int eConv = row.conv >= 0 ? e1 + 1 + row.conv : eLast;
+        float* o = out + 8 * (size_t)r;
+        o[0] = 0.f;
+        int rc = r > 0 ? dt(t->rows[r - 1].first + t->rows[r - 1].chunks + 2, e0, &o[0]) : YB_OK;
+        if (rc == YB_OK) rc = dt(e0, e1, &o[1]);
+        if (rc == YB_OK) rc = dt(e1, eConv, &o[2]);
+        if (rc == YB_OK) rc = dt(eConv, eLast, &o[3]);
+        if (rc == YB_OK) rc = dt(eLast, eEnd, &o[4]);
+        if (rc != YB_OK) return rc;
+        o[5] = (float)row.done;
+        o[6] = (float)row.chunks;
+        o[7] = (float)row.conv;
+    }
+    *rows = count;
+    return YB_OK;
+}
+
 extern "C" int yb_set_residual_kind(yb_sim* s, int kind) {
     if (!s || (kind != YB_RESIDUAL_UPDATE && kind != YB_RESIDUAL_NEWTON)) {
         set_message("yb_set_residual_kind: YB_RESIDUAL_UPDATE or YB_RESIDUAL_NEWTON");
@@ -884,13 +974,24 @@ static void update_chooser(yb_sim* s, int start, int sweeps) {
 
 static int adaptive_substep(yb_sim* s, int mode) {
     const int start = choose_start(s);
+    Timeline* tl = (Timeline*)s->timeline;
+    if (tl && (int)tl->rows.size() < tl->cap) {
+        tl->rows.emplace_back();
+        tl->rows.back().first = (int)tl->used;
+        tl->open = true;
+    }
+    int rc = timeline_mark(s);
+    if (rc != YB_OK) return rc;
     if (s->useGraphs)
         HIPCHECK(hipGraphLaunch(s->proGraphs[start][mode], s->stream));
     else
         enqueue_prologue(s, mode, true, start == 0 ? s->warmStart : 0);
+    rc = timeline_mark(s);
+    if (rc != YB_OK) return rc;
     const int numChunks = (s->maxSweeps + s->chunk - 1) / s->chunk;
     int launched = 0;
     int done = s->maxSweeps;  // sweeps of this substep (all of them if not converged)
+    bool converged = false;
     auto launch_chunk = [&](int c) -> int {
         if (s->useGraphs) {
             HIPCHECK(hipGraphLaunch(s->chunkGraphs[c], s->stream));
@@ -902,7 +1003,7 @@ static int adaptive_substep(yb_sim* s, int mode) {
                                 hipMemcpyDeviceToHost, s->stream));
         HIPCHECK(hipEventRecord(s->chunkEv[c & 1], s->stream));
         launched = c + 1;
-        return YB_OK;
+        return timeline_mark(s);
     };
     // The next chunk is queued before the host waits for the current one, so
     // the device does not idle between chunks -- unless the recent substeps
@@ -910,7 +1011,7 @@ static int adaptive_substep(yb_sim* s, int mode) {
     // convergence is a run of launches that exit at once (about 5 us each for
     // a sweep).  If the estimate was wrong, the next chunk is queued after the
     // wait.  This changes only when launches are queued, never the results.
-    int rc = launch_chunk(0);
+    rc = launch_chunk(0);
     if (rc != YB_OK) return rc;
     for (int c = 0; c < numChunks; c++) {
         const bool expectDone =
@@ -922,6 +1023,7 @@ static int adaptive_substep(yb_sim* s, int mode) {
         HIPCHECK(hipEventSynchronize(s->chunkEv[c & 1]));
         if (s->b.hostFlag[c & 1] != 0x7fffffff) {
             done = s->b.hostFlag[c & 1] + 1;
+            converged = true;
             break;
         }
         if (launched < numChunks && launched == c + 1) {
@@ -939,6 +1041,15 @@ static int adaptive_substep(yb_sim* s, int mode) {
     else
         enqueue_end(s, 0, true);
     HIPCHECK(hipGetLastError());
+    rc = timeline_mark(s);
+    if (rc != YB_OK) return rc;
+    if (tl && tl->open) {
+        TimelineRow& row = tl->rows.back();
+        row.chunks = launched;
+        row.done = done;
+        row.conv = converged ? (done - 1) / s->chunk : -1;
+        tl->open = false;
+    }
     s->warmSubsteps[start]++;
     update_chooser(s, start, done);
     return YB_OK;

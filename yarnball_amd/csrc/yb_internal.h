@@ -228,6 +228,7 @@ struct yb_sim {
     bool adaptiveReady = false;
     hipEvent_t chunkEv[2] = {nullptr, nullptr};
     void* prof = nullptr;  // eager-profiling event log (capi.hip)
+    void* timeline = nullptr;  // residual-mode substep timeline (capi.hip)
     double denomHost = 0.0;  // update-residual normaliser (sum of l_i^2 over free vertices)
     int residualKind = 0;    // stopping test: YB_RESIDUAL_UPDATE or YB_RESIDUAL_NEWTON
     int warmStart = 0;     // 0 plain (iteration.cu:18-34), 1 learned, 2 temporal

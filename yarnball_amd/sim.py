@@ -208,6 +208,27 @@ class Sim:
                                    int(substeps), ptr(ms), ptr(nl)), "profile")
         return {st: (float(ms[i]), int(nl[i])) for i, st in enumerate(_lib.STAGES)}
 
+    TIMELINE_FIELDS = ("idle_us", "prologue_us", "converge_us", "tail_us", "end_us",
+                       "sweeps", "chunks", "converged_chunk")
+
+    def startTimeline(self, substeps):
+        """Record a GPU timeline of the next `substeps` residual-mode substeps
+        (yb_debug_timeline); read it with timeline()."""
+        self._need_handle()
+        check(self._lib.yb_debug_timeline(self._h, int(substeps)), "timeline")
+
+    def timeline(self, stop=True):
+        """Recorded substeps as an (rows, 8) array with columns TIMELINE_FIELDS."""
+        self._need_handle()
+        cap = 1 << 16
+        out = np.zeros((cap, 8), np.float32)
+        rows = ctypes.c_int(0)
+        check(self._lib.yb_debug_timeline_read(self._h, ptr(out), cap, ctypes.byref(rows)),
+              "timeline")
+        if stop:
+            check(self._lib.yb_debug_timeline(self._h, 0), "timeline")
+        return out[:rows.value].copy()
+
     def driverInit(self, initialInvMasses=None):
         """Record the current device positions (and initial inverse masses) for
         the scenario drivers; returns the centre of their bounding box."""
