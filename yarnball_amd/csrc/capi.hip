@@ -205,7 +205,7 @@ extern "C" int yb_create(int n, yb_sim** out) {
     ALLOC(b.predC, N);
     ALLOC(b.mlpCount, 2 * ((N + 63) / 64));
     ALLOC(b.recCount, 2);
-    ALLOC(b.partial, 2 * (size_t)sweep_waves(n));
+    ALLOC(b.partial, 3 * (size_t)sweep_waves(n));
     ALLOC(b.lineG, N);
     ALLOC(b.lineQL, N);
     for (int k = 0; k < 2; k++) {
@@ -604,9 +604,14 @@ static void enqueue_prologue(yb_sim* s, int mode, bool adaptive, int warm) {
     // frames) for every warm start, so that all of them are held to the same
     // target.  A plain session gets it from sweep 0; any other session takes
     // it here, before the start is changed.
+    // With a plain start this substep, the frame refresh below is made by the
+    // same pass (both read the plain warm start and the previous frames).
+    const bool refresh = (warm != 0 || s->strandSolve) && s->frameRefresh;
+    const bool evalDenom = adaptive && (s->warmStart != 0 || s->strandSolve);
+    const bool fused = evalDenom && refresh && warm == 0;
     if (adaptive) {
         launch_ctl_reset(s);
-        if (s->warmStart != 0 || s->strandSolve) launch_residual_denom(s);
+        if (evalDenom) launch_residual_denom(s, fused);
         mark(s, YB_STAGE_RESIDUAL);
     }
     const bool learned = warm == 1, record = s->recCap > 0;
@@ -625,7 +630,7 @@ static void enqueue_prologue(yb_sim* s, int mode, bool adaptive, int warm) {
     // the previous substep would hold every segment to its old direction and
     // undo the rotation the warm start predicts (the sweeps restore it only
     // slowly where bending is soft).
-    if ((warm != 0 || s->strandSolve) && s->frameRefresh) {
+    if (refresh && !fused) {
         launch_frame_refresh(s);
         mark(s, YB_STAGE_MLP);
     }

@@ -123,7 +123,8 @@ __global__ __launch_bounds__(256) void k_step_limit(const DevParams* __restrict_
 __device__ __forceinline__ void wave_residual(float numU, float numF, double2* partial, int wave,
                                               int n);
 __device__ __forceinline__ void residual_wave(SolverCtl* __restrict__ ctl,
-                                              const double2* __restrict__ partial, int nb,
+                                              const double2* __restrict__ partial,
+                                              const double2* __restrict__ denomPart, int nb,
                                               int sweep, float tol, int maxSweeps);
 
 // Closest points of segment e = (0 -> e) and (B0 -> B0 + A1), specialised from
@@ -228,6 +229,7 @@ __device__ __forceinline__ void sweep_body(const DevParams* __restrict__ P,
                                            SolverCtl* __restrict__ ctl, int sweep,
                                            double2* __restrict__ partial,
                                            const double2* __restrict__ partialPrev,
+                                           const double2* __restrict__ denomPart,
                                            float tol, int maxSweeps, LineOut lo) {
     // Residual-targeted mode: sweeps after convergence do nothing.
     if (ctl && ctl->doneAt < sweep) return;
@@ -238,7 +240,7 @@ __device__ __forceinline__ void sweep_body(const DevParams* __restrict__ P,
     // later (this sweep's result is not used; the next one exits at once), and
     // no separate reduction launch sits between two sweeps.
     if (partialPrev && blockIdx.x == 0 && threadIdx.x < 64)
-        residual_wave(ctl, partialPrev, (n + 61) / 62, sweep - 1, tol, maxSweeps);
+        residual_wave(ctl, partialPrev, denomPart, (n + 61) / 62, sweep - 1, tol, maxSweeps);
     const int lane = threadIdx.x & 63;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int wave = (int)((tile * blockDim.x + threadIdx.x) >> 6);
@@ -422,7 +424,10 @@ __device__ __forceinline__ void sweep_body(const DevParams* __restrict__ P,
     }
     if constexpr (kMode == kEval) {
         wave_residual(0.0f, owner ? numF : 0.0f, partial, wave, n);
-        return;
+        if (!qOut) return;
+        // frame refresh from the input iterate itself (the closed-form frame
+        // update below, with dx' = dx)
+        dxNew = dx;
     }
 
     if constexpr (kMode == kLine) {
@@ -485,7 +490,7 @@ __device__ __forceinline__ void sweep_body(const DevParams* __restrict__ P,
                           __shfl_down(q4.z, 1, 64), __shfl_down(q4.w, 1, 64)};
 
     if (owner) {
-        dxOut[v] = f4(dxNew, d4.w);
+        if constexpr (kMode == kSweep) dxOut[v] = f4(dxNew, d4.w);
         rot q0 = asrot(q4);
         if (!(flags & YB_FLAG_FIX_ORIENTATION) && (flags & YB_FLAG_HAS_NEXT)) {
             const v3 e = (p1 - xyz(p4)) + (dxNext - dxNew);
@@ -505,7 +510,8 @@ __device__ __forceinline__ void sweep_body(const DevParams* __restrict__ P,
         }
         qOut[v] = f4r(q0);
     }
-    if (partial) wave_residual(owner ? num : 0.0f, owner ? numF : 0.0f, partial, wave, n);
+    if constexpr (kMode == kSweep)
+        if (partial) wave_residual(owner ? num : 0.0f, owner ? numF : 0.0f, partial, wave, n);
 }
 
 __global__ __launch_bounds__(512) void k_sweep(const DevParams* __restrict__ P,
@@ -522,9 +528,11 @@ __global__ __launch_bounds__(512) void k_sweep(const DevParams* __restrict__ P,
                                                SolverCtl* __restrict__ ctl, int sweep,
                                                double2* __restrict__ partial,
                                                const double2* __restrict__ partialPrev,
+                                               const double2* __restrict__ denomPart,
                                                float tol, int maxSweeps) {
     sweep_body<kSweep>(P, pos, prm, yv, qIn, qOut, qRest, dxIn, dxOut, numCols, cols, ctl, sweep,
-                       partial, partialPrev, tol, maxSweeps, LineOut{nullptr, nullptr, nullptr});
+                       partial, partialPrev, denomPart, tol, maxSweeps,
+                       LineOut{nullptr, nullptr, nullptr});
 }
 
 // Strand solve, tile part (see the line mode of sweep_body).
@@ -540,24 +548,26 @@ __global__ __launch_bounds__(256) void k_line_tile(const DevParams* __restrict__
                                                    const SolverCtl* __restrict__ ctl, int sweep) {
     if (strand_skip(ctl, sweep)) return;
     sweep_body<kLine>(P, pos, prm, yv, qIn, nullptr, qRest, dxIn, nullptr, numCols, cols, nullptr,
-                      0, nullptr, nullptr, 0.0f, 0, lo);
+                      0, nullptr, nullptr, nullptr, 0.0f, 0, lo);
 }
 
 // Residual partials of an iterate without changing it: the denominator |f(dx0)|
 // of the relative residual when the sweeps do not start from the plain warm
-// start.
+// start.  With qOut set, also the frames refreshed from that iterate (the
+// sweep's closed-form frame update at dx' = dx), written to qOut.
 __global__ __launch_bounds__(256) void k_force_eval(const DevParams* __restrict__ P,
                                                     const float4* __restrict__ pos,
                                                     const float4* __restrict__ prm,
                                                     const float4* __restrict__ yv,
                                                     const float4* __restrict__ qIn,
+                                                    float4* __restrict__ qOut,
                                                     const float4* __restrict__ qRest,
                                                     const float4* __restrict__ dxIn,
                                                     const int32_t* __restrict__ numCols,
                                                     const int32_t* __restrict__ cols,
                                                     double2* __restrict__ partial) {
-    sweep_body<kEval>(P, pos, prm, yv, qIn, nullptr, qRest, dxIn, nullptr, numCols, cols, nullptr,
-                      0, partial, nullptr, 0.0f, 0, LineOut{nullptr, nullptr, nullptr});
+    sweep_body<kEval>(P, pos, prm, yv, qIn, qOut, qRest, dxIn, nullptr, numCols, cols, nullptr,
+                      0, partial, nullptr, nullptr, 0.0f, 0, LineOut{nullptr, nullptr, nullptr});
 }
 
 // Strand solve, boundary part: one iteration over the pairs of vertices that
@@ -745,18 +755,19 @@ __device__ __forceinline__ double2 reduce_partials(const double2* __restrict__ p
 }
 
 // Residuals of sweep `sweep` and the stopping decision, by one wave.
+// No Newton-residual denominator yet (the substep's first check): it is the
+// sum of denomPart, the partials of the prologue's pass over the plain warm
+// start, or -- in a plain session, whose first check is sweep 0 and whose
+// sweep 0 input is the plain warm start itself -- this check's own sum.
 __device__ __forceinline__ void residual_wave(SolverCtl* __restrict__ ctl,
-                                              const double2* __restrict__ partial, int nb,
+                                              const double2* __restrict__ partial,
+                                              const double2* __restrict__ denomPart, int nb,
                                               int sweep, float tol, int maxSweeps) {
     const double2 a = reduce_partials(partial, nb);
+    double denF = ctl->denomF;  // uniform across the wave
+    if (denF < 0.0) denF = denomPart ? reduce_partials(denomPart, nb).y : a.y;
     if ((threadIdx.x & 63) == 0) {
-        // no Newton-residual denominator yet: this is sweep 0 of a plain start,
-        // whose input is the plain warm start itself
-        double denF = ctl->denomF;
-        if (denF < 0.0) {
-            denF = a.y;
-            ctl->denomF = a.y;
-        }
+        ctl->denomF = denF;
         const double denU = ctl->denom;
         const float rU = denU > 0.0 ? (float)sqrt(a.x / denU) : 0.0f;
         const float rF = denF > 0.0 ? (float)sqrt(a.y / denF) : 0.0f;
@@ -780,10 +791,11 @@ __device__ __forceinline__ void residual_wave(SolverCtl* __restrict__ ctl,
 // Stand-alone check, used only for the last sweep a substep may run (every
 // other check is made by the following sweep's first wave, in k_sweep).
 __global__ __launch_bounds__(64) void k_residual(SolverCtl* __restrict__ ctl,
-                                                 const double2* __restrict__ partial, int nb,
+                                                 const double2* __restrict__ partial,
+                                                 const double2* __restrict__ denomPart, int nb,
                                                  int sweep, float tol, int maxSweeps) {
     if (ctl->doneAt < sweep) return;
-    residual_wave(ctl, partial, nb, sweep, tol, maxSweeps);
+    residual_wave(ctl, partial, denomPart, nb, sweep, tol, maxSweeps);
 }
 
 __global__ void k_ctl_reset(SolverCtl* __restrict__ ctl) {
@@ -793,14 +805,6 @@ __global__ void k_ctl_reset(SolverCtl* __restrict__ ctl) {
     }
 }
 
-// Newton-residual denominator |f(dx_0)|^2 from the partials of a pass over the
-// plain warm start.
-__global__ __launch_bounds__(64) void k_residual_denom(SolverCtl* __restrict__ ctl,
-                                                       const double2* __restrict__ partial,
-                                                       int nb) {
-    const double2 a = reduce_partials(partial, nb);
-    if ((threadIdx.x & 63) == 0) ctl->denomF = a.y;
-}
 
 // ---------------------------------------------------------------------------
 // Frame refresh after a non-plain warm start: one closed-form frame update
@@ -1049,13 +1053,14 @@ void launch_sweep(yb_sim* s, int sweep, bool adaptive, bool partials, bool check
                        s->b.pos, s->b.prm, s->b.yv, s->b.q[in], s->b.q[out], s->b.qRest,
                        s->b.dx[in], s->b.dx[out], s->b.numCols, s->b.cols,
                        adaptive ? s->b.ctl : nullptr, sweep, partials ? part : nullptr,
-                       checkPrev ? prev : nullptr, s->tol, s->maxSweeps);
+                       checkPrev ? prev : nullptr, denom_partials(s), s->tol, s->maxSweeps);
 }
 
 void launch_residual(yb_sim* s, int sweep) {
     const size_t nbw = (size_t)sweep_waves(s->n);
     hipLaunchKernelGGL(k_residual, dim3(1), dim3(64), 0, s->stream, s->b.ctl,
-                       s->b.partial + nbw * (sweep & 1), (int)nbw, sweep, s->tol, s->maxSweeps);
+                       s->b.partial + nbw * (sweep & 1), denom_partials(s), (int)nbw, sweep,
+                       s->tol, s->maxSweeps);
 }
 
 void launch_frame_refresh(yb_sim* s) {
@@ -1097,13 +1102,24 @@ void launch_ctl_reset(yb_sim* s) {
 // The Newton residual's denominator from the plain warm start in dx[0]: a
 // force evaluation, then the reduction.  The partials use the second half of
 // the buffer; sweep 0 writes the first.
-void launch_residual_denom(yb_sim* s) {
-    const size_t nbw = (size_t)sweep_waves(s->n);
-    double2* part = s->b.partial + nbw;
+// The Newton-residual denominator's partials: the third part of b.partial,
+// written by the prologue's force evaluation in sessions whose sweeps do not
+// start from the plain warm start; null in plain sessions.
+const double2* denom_partials(const yb_sim* s) {
+    return (s->warmStart != 0 || s->strandSolve)
+               ? s->b.partial + 2 * (size_t)sweep_waves(s->n)
+               : nullptr;
+}
+
+void launch_residual_denom(yb_sim* s, bool frames) {
+    double2* part = s->b.partial + 2 * (size_t)sweep_waves(s->n);
     hipLaunchKernelGGL(k_force_eval, dim3(blocks(sweep_waves(s->n) * 64, 256)), dim3(256), 0,
                        s->stream, s->b.params, s->b.pos, s->b.prm, s->b.yv, s->b.q[0],
-                       s->b.qRest, s->b.dx[0], s->b.numCols, s->b.cols, part);
-    hipLaunchKernelGGL(k_residual_denom, dim3(1), dim3(64), 0, s->stream, s->b.ctl, part, (int)nbw);
+                       frames ? s->b.q[1] : nullptr, s->b.qRest, s->b.dx[0], s->b.numCols,
+                       s->b.cols, part);
+    if (frames)
+        hipLaunchKernelGGL(k_copy4, dim3(blocks(s->n, 256)), dim3(256), 0, s->stream, s->b.q[1],
+                           s->b.q[0], s->n);
 }
 
 void launch_end(yb_sim* s, int fin, bool adaptive) {

@@ -282,9 +282,12 @@ void launch_frame_refresh(yb_sim* s);  // closed-form frame update from dx[0]
 // next vertex, right-hand side xyz).
 void launch_strand_solve(yb_sim* s, int buf = 0, SolverCtl* ctl = nullptr, int sweep = 0,
                          float* sys = nullptr);
-// Newton-residual denominator |f(dx_0)|^2 of the substep from the plain warm
-// start in dx[0] (after launch_ctl_reset; before the warm start is changed).
-void launch_residual_denom(yb_sim* s);
+// Newton-residual denominator of the substep: the partials of |f(dx_0)|^2 at
+// the plain warm start in dx[0] (after launch_ctl_reset; before the warm start
+// is changed), reduced by the substep's first residual check.  frames: also
+// refresh the frames from dx[0] (q[0] updated in place through q[1]).
+void launch_residual_denom(yb_sim* s, bool frames);
+const double2* denom_partials(const yb_sim* s);
 void launch_end(yb_sim* s, int fin, bool adaptive);         // iteration.cu:48-61
 int sweep_waves(int n);
 // learned warm start (mlp.hip)
