@@ -137,11 +137,13 @@ def measure_batched(args, steps, warmup, per_scene_steps, with_cpu):
 
     tuned = None
     launch_config = args.launch_config
+    tile_config = args.tile_config
     if not args.no_autotune:
         from yarnball_amd import autotune
         tuned = autotune.tune(big, offsets=offs, substeps=2,
                               log=lambda m: print(m, file=sys.stderr, flush=True))
         launch_config = (tuned["sweep_threads"], tuned["query_threads"])
+        tile_config = (tuned["refit_leaves"], *tuned["query_shape"])
 
     b = Sim(big.n)
     # the random walks cross themselves here and there at t = 0: reported as
@@ -154,6 +156,8 @@ def measure_batched(args, steps, warmup, per_scene_steps, with_cpu):
         b.setGraphs(False)
     if launch_config:
         b.setLaunchConfig(*launch_config)
+    if tile_config:
+        b.setTileConfig(*tile_config)
     for w in range(warmup):
         b.advance(frame)
     t0 = time.perf_counter()
@@ -191,6 +195,8 @@ def measure_batched(args, steps, warmup, per_scene_steps, with_cpu):
             scenes.load_into(s_, sc)
             if launch_config:
                 s_.setLaunchConfig(*launch_config)
+            if tile_config:
+                s_.setTileConfig(*tile_config)
             sims.append(s_)
         for w in range(min(warmup, 1)):
             for s_ in sims:
@@ -260,6 +266,8 @@ def measure_batched(args, steps, warmup, per_scene_steps, with_cpu):
         "per_scene_launches": per,
         "launch_config": {"sweep_threads": (launch_config or (256, 128))[0],
                           "query_threads": (launch_config or (256, 128))[1],
+                          "refit_leaves": (tile_config or (1024, 8, 6))[0],
+                          "query_shape": list((tile_config or (1024, 8, 6))[1:]),
                           "autotune": tuned},
         "batched_vs_per_scene": round(value / per["value"], 3) if per else None,
         "roofline": {"kernel": "sweep", "bound": "hbm",
@@ -333,6 +341,8 @@ def run_method(sc, args, method, model, strand_model=None):
         sim.setGraphs(False)
     if args.launch_config:
         sim.setLaunchConfig(*args.launch_config)
+    if args.tile_config:
+        sim.setTileConfig(*args.tile_config)
     frame = np.float32(sc.frame_dt)
     substeps_per_frame = int(np.ceil(frame / np.float32(sc.maxH)))
     for w in range(args.warmup):
@@ -426,6 +436,10 @@ def main():
     ap.add_argument("--launch-config", type=int, nargs=2, default=None,
                     metavar=("SWEEP_THREADS", "QUERY_THREADS"),
                     help="fixed block sizes (skips the autotuner)")
+    ap.add_argument("--tile-config", type=int, nargs=3, default=None,
+                    metavar=("REFIT_LEAVES", "QUERY_STACK_LDS", "QUERY_CANDIDATES"),
+                    help="fixed refit tile and query shape (default when the autotuner is "
+                         "skipped: 1024 8 6)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the stepper runs on a single MI355X "
@@ -434,6 +448,8 @@ def main():
     if args.launch_config:
         args.launch_config = tuple(args.launch_config)
         args.no_autotune = True
+    if args.tile_config:
+        args.tile_config = tuple(args.tile_config)
     if args.workload == "batched_walks":
         return bench_batched(args)
 
@@ -452,6 +468,7 @@ def main():
         tuned = autotune.tune(sc, residual=args.residual, max_sweeps=args.max_sweeps, substeps=1,
                               log=lambda m: print(m, file=sys.stderr, flush=True))
         args.launch_config = (tuned["sweep_threads"], tuned["query_threads"])
+        args.tile_config = (tuned["refit_leaves"], *tuned["query_shape"])
     model = strand_model = None
     if "learned" in methods:
         model = np.load(args.model, allow_pickle=False)
@@ -634,6 +651,8 @@ def main():
         "learned_strand_vs_plain": vs_plain("learned+strand"),
         "launch_config": {"sweep_threads": (args.launch_config or (256, 128))[0],
                           "query_threads": (args.launch_config or (256, 128))[1],
+                          "refit_leaves": (args.tile_config or (1024, 8, 6))[0],
+                          "query_shape": list((args.tile_config or (1024, 8, 6))[1:]),
                           "autotune": tuned},
         "roofline": roofline,
         "roofline_broadphase": broad.get("query"),

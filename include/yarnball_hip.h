@@ -364,6 +364,17 @@ int yb_set_scenes(yb_sim* s, int numScenes, const int32_t* offsets);
 int yb_set_launch_config(yb_sim* s, int sweepThreads, int queryThreads);
 int yb_get_launch_config(yb_sim* s, int* sweepThreads, int* queryThreads);
 
+/* Tile shapes (no counterpart in the reference): the refit tile -- sorted
+ * leaves per refit workgroup, 256, 512 or 1024 (default 1024) -- and the
+ * contact query's LDS shape -- traversal-stack entries kept in LDS and
+ * candidate leaves buffered per thread, one of (4,4) (8,6) (8,8) (16,8)
+ * (default (8,6)).  Refit boxes are min/max unions and the query's contact
+ * lists keep their order for every shape, so only the speed depends on them;
+ * a new refit tile rebuilds the tree at the next detection.
+ * yarnball_amd/autotune.py picks them by timing. */
+int yb_set_tile_config(yb_sim* s, int refitLeaves, int queryStackLds, int queryCandidates);
+int yb_get_tile_config(yb_sim* s, int* refitLeaves, int* queryStackLds, int* queryCandidates);
+
 /* Launch mode.  on = 1 (default): each substep's kernels run as captured HIP
  * graphs (step.cpp:5-34 likewise captures the sweep chain).  on = 0: the same
  * kernels are enqueued one by one on the handle's stream -- same results,

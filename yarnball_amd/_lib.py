@@ -165,6 +165,8 @@ SIGNATURES = {
     "yb_export_bcc": (_I, [ctypes.c_char_p, _I, _P, _I]),
     "yb_export_obj": (_I, [ctypes.c_char_p, _I, _P]),
     "yb_get_launch_config": (_I, [_P, _P, _P]),
+    "yb_set_tile_config": (_I, [_P, _I, _I, _I]),
+    "yb_get_tile_config": (_I, [_P, _P, _P, _P]),
     "yb_record": (_I, [_P, ctypes.c_int64]),
     "yb_record_enable": (_I, [_P, _I]),
     "yb_record_fetch": (_I, [_P, _P, _P, ctypes.POINTER(ctypes.c_int64)]),

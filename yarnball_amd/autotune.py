@@ -1,12 +1,17 @@
-"""Launch-shape autotuner.
+"""Launch-shape and tile autotuner.
 
-Picks the threads per block of the Newton sweep (k_sweep) and of the contact
-query (k_query) by timing each candidate, with HIP events on the stepper's
-stream (Sim.profile), on a scratch handle loaded with the caller's workload.
+Picks, by timing each candidate with HIP events on the stepper's stream
+(Sim.profile) on a scratch handle loaded with the caller's workload:
+  - the threads per block of the Newton sweep (k_sweep) and of the contact
+    query (k_query);
+  - the refit tile: sorted leaves per refit workgroup (256, 512, 1024);
+  - the query's LDS shape: traversal-stack entries kept in LDS and candidate
+    leaves buffered per thread, which together set the waves a CU can hold.
 The caller's simulation is never touched: apply the result with
-Sim.setLaunchConfig.  Every kernel's output per vertex or segment is a
-function of its inputs only, so the choice changes the speed and never the
-results (checked by tests/test_gpu_launch_modes.py).
+Sim.setLaunchConfig and Sim.setTileConfig.  Every kernel's output per vertex
+or segment is a function of its inputs only, refit boxes are min/max unions
+and contact lists keep their order, so the choice changes the speed and never
+the results (checked by tests/test_gpu_launch_modes.py).
 """
 import numpy as np
 
@@ -15,6 +20,8 @@ from .sim import Sim
 
 SWEEP_THREADS = (64, 128, 256, 512)
 QUERY_THREADS = (64, 128, 256)
+REFIT_LEAVES = (256, 512, 1024)
+QUERY_SHAPES = ((4, 4), (8, 6), (8, 8), (16, 8))
 
 
 def _stage_us(prof, stage):
@@ -46,11 +53,33 @@ def tune(scene, offsets=None, residual=0.0, max_sweeps=6000, substeps=2, rounds=
             sim.setLaunchConfig(best_sweep, t)
             query[t] = min(query[t], _stage_us(sim.profile(substeps), "query"))
     best_query = min(query, key=query.get)
+    sim.setLaunchConfig(best_sweep, best_query)
+    # refit tile: the first substep after a change rebuilds the tree, so each
+    # candidate is timed over substeps + 1 and the first one dropped
+    refit = {r: float("inf") for r in REFIT_LEAVES}
+    for _ in range(rounds):
+        for r in REFIT_LEAVES:
+            sim.setTileConfig(r, 8, 6)
+            sim.profile(1)
+            refit[r] = min(refit[r], _stage_us(sim.profile(substeps), "refit"))
+    best_refit = min(refit, key=refit.get)
+    sim.setTileConfig(best_refit, 8, 6)
+    sim.profile(1)
+    shape = {c: float("inf") for c in QUERY_SHAPES}
+    for _ in range(rounds):
+        for c in QUERY_SHAPES:
+            sim.setTileConfig(best_refit, *c)
+            shape[c] = min(shape[c], _stage_us(sim.profile(substeps), "query"))
+    best_shape = min(shape, key=shape.get)
     sim.close()
     out = {"sweep_threads": best_sweep, "query_threads": best_query,
+           "refit_leaves": best_refit, "query_shape": list(best_shape),
            "sweep_us": {str(k): round(v, 2) for k, v in sweep.items()},
-           "query_us": {str(k): round(v, 2) for k, v in query.items()}}
+           "query_us": {str(k): round(v, 2) for k, v in query.items()},
+           "refit_us": {str(k): round(v, 2) for k, v in refit.items()},
+           "query_shape_us": {f"{a}x{b}": round(v, 2) for (a, b), v in shape.items()}}
     if log:
         log(f"[autotune] sweep {out['sweep_us']} -> {best_sweep}; query {out['query_us']} -> "
-            f"{best_query}")
+            f"{best_query}; refit tile {out['refit_us']} -> {best_refit}; query shape "
+            f"{out['query_shape_us']} -> {best_shape}")
     return out

@@ -127,6 +127,19 @@ static hipError_t dalloc(T** p, size_t count) {
     return hipMalloc((void**)p, sizeof(T) * (count ? count : 1));
 }
 
+// Refit blocking for a tile of `leaves` sorted leaves per workgroup: block
+// count, sparse-table levels and the grid of the range-union pass.
+static void set_refit_geometry(yb_sim* s, int leaves) {
+    const int n = s->n;
+    s->refitLeaves = leaves;
+    s->refitShift = 31 - __builtin_clz((unsigned)leaves);
+    s->refitBlocks = (n + leaves - 1) / leaves;
+    s->refitLevels = 1;
+    while ((1 << s->refitLevels) <= s->refitBlocks) s->refitLevels++;
+    s->refitTopBlocks = (n / leaves) * 64 / 256 + 1;
+    if (s->refitTopBlocks > 2048) s->refitTopBlocks = 2048;
+}
+
 extern "C" int yb_create(int n, yb_sim** out) {
     if (!out) {
         set_message("yb_create: null output pointer");
@@ -151,11 +164,9 @@ extern "C" int yb_create(int n, yb_sim** out) {
     s->sortTiles = (n + kSortTile - 1) / kSortTile;
     s->boxBlocks = (n + 255) / 256;
     if (s->boxBlocks > 1024) s->boxBlocks = 1024;
-    s->refitBlocks = (n + kRefitLeaves - 1) / kRefitLeaves;
-    s->refitLevels = 1;
-    while ((1 << s->refitLevels) <= s->refitBlocks) s->refitLevels++;
-    s->refitTopBlocks = (n / kRefitLeaves) * 64 / 256 + 1;
-    if (s->refitTopBlocks > 2048) s->refitTopBlocks = 2048;
+    set_refit_geometry(s, kRefitLeavesMin);  // sizes the table for the smallest tile
+    const size_t tabEntries = 2 * (size_t)s->refitBlocks * s->refitLevels;
+    set_refit_geometry(s, kRefitLeavesMax);
     Buffers& b = s->b;
     const size_t N = (size_t)n;
     hipError_t e = hipSuccess;
@@ -192,7 +203,7 @@ extern "C" int yb_create(int n, yb_sim** out) {
     ALLOC(b.leafNeed, N);
     ALLOC(b.sufBox, 2 * N);
     ALLOC(b.preBox, 2 * N);
-    ALLOC(b.blkTab, 2 * (size_t)s->refitBlocks * s->refitLevels);
+    ALLOC(b.blkTab, tabEntries);
     ALLOC(b.err, 2);
     ALLOC(b.params, 1);
     ALLOC(b.ctl, 1);
@@ -905,6 +916,38 @@ extern "C" int yb_get_launch_config(yb_sim* s, int* sweepThreads, int* queryThre
     if (!s || !sweepThreads || !queryThreads) return YB_ERR_INVALID_ARGUMENT;
     *sweepThreads = s->sweepThreads;
     *queryThreads = s->queryThreads;
+    return YB_OK;
+}
+
+extern "C" int yb_set_tile_config(yb_sim* s, int refitLeaves, int queryStackLds,
+                                  int queryCandidates) {
+    if (!s) return YB_ERR_INVALID_ARGUMENT;
+    const bool okR = refitLeaves == 256 || refitLeaves == 512 || refitLeaves == 1024;
+    const bool okQ = query_shape_supported(queryStackLds, queryCandidates);
+    if (!okR || !okQ) {
+        set_message("yb_set_tile_config: refit leaves 256/512/1024; query (LDS stack, "
+                    "candidates) one of (4,4) (8,6) (8,8) (16,8)");
+        return YB_ERR_INVALID_ARGUMENT;
+    }
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    if (refitLeaves != s->refitLeaves) {
+        set_refit_geometry(s, refitLeaves);
+        s->treeBuilt = false;  // the tree's straddle flags follow the refit tile
+    }
+    s->queryStackLds = queryStackLds;
+    s->queryCandidates = queryCandidates;
+    s->graphItr = -1;
+    s->adaptiveReady = false;
+    return YB_OK;
+}
+
+extern "C" int yb_get_tile_config(yb_sim* s, int* refitLeaves, int* queryStackLds,
+                                  int* queryCandidates) {
+    if (!s || !refitLeaves || !queryStackLds || !queryCandidates)
+        return YB_ERR_INVALID_ARGUMENT;
+    *refitLeaves = s->refitLeaves;
+    *queryStackLds = s->queryStackLds;
+    *queryCandidates = s->queryCandidates;
     return YB_OK;
 }
 

@@ -427,7 +427,8 @@ __global__ __launch_bounds__(256) void k_karras(int n, const uint32_t* __restric
                                                 int2* __restrict__ rng, int2* __restrict__ kids,
                                                 int32_t* __restrict__ strad,
                                                 int32_t* __restrict__ stradCount,
-                                                int32_t* __restrict__ leafNeed, SceneKeys sk) {
+                                                int32_t* __restrict__ leafNeed, SceneKeys sk,
+                                                int leafShift) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n - 1) return;
     const int d = (kdelta(keys, n, i, i + 1, sk) - kdelta(keys, n, i, i - 1, sk)) >= 0 ? 1 : -1;
@@ -449,7 +450,8 @@ __global__ __launch_bounds__(256) void k_karras(int n, const uint32_t* __restric
     const int lo = min(i, j);
     const int hi = max(i, j);
     // the children's parent slots carry whether this node straddles refit blocks
-    const int32_t sf = (lo / kRefitLeaves != hi / kRefitLeaves) ? kParentStrad : 0;
+    const bool straddles = (lo >> leafShift) != (hi >> leafShift);
+    const int32_t sf = straddles ? kParentStrad : 0;
     int cl, cr, kl, kr;
     if (lo == gamma) {
         cl = ~order[gamma];
@@ -478,7 +480,7 @@ __global__ __launch_bounds__(256) void k_karras(int n, const uint32_t* __restric
     }
     // Nodes whose leaf range crosses a refit-block boundary are finished by
     // the range-union pass; their end leaves need suffix / prefix unions.
-    if (lo / kRefitLeaves != hi / kRefitLeaves) {
+    if (straddles) {
         const int slot = atomicAdd(stradCount, 1);
         strad[slot] = i;
         atomicOr(&leafNeed[lo], 1);
@@ -488,7 +490,7 @@ __global__ __launch_bounds__(256) void k_karras(int n, const uint32_t* __restric
 
 // ---------------------------------------------------------------------------
 // Refit without device-scope fences.  Sorted leaves are split into blocks of
-// kRefitLeaves.  (1) k_refit_local: one workgroup per block climbs from every
+// kLeaves (the refit tile: 256, 512 or 1024, chosen by timing).  (1) k_refit_local: one workgroup per block climbs from every
 // leaf through the nodes whose range lies inside the block, with LDS arrival
 // counters and LDS boxes (the second arrival at a node forms the union); it
 // writes those nodes' child boxes, the box of every child whose parent
@@ -499,9 +501,8 @@ __global__ __launch_bounds__(256) void k_karras(int n, const uint32_t* __restric
 // prefix -- computed directly, so no node waits on another.  Every step is a
 // min/max, hence exact and independent of scheduling.
 // One thread per leaf: each climb is a chain of dependent loads, and with a
-// leaf per thread all of a block's climbs are in flight at once.
-constexpr int kRefitThreads = 1024;
-constexpr int kRefitPer = kRefitLeaves / kRefitThreads;
+// leaf per thread all of a block's climbs are in flight at once.  Every global
+// input of a leaf is loaded up front, so a block waits on memory once.
 
 __device__ __forceinline__ void box_union(float4& lo, float4& hi, float4 lo2, float4 hi2) {
     lo = make_float4(fminf(lo.x, lo2.x), fminf(lo.y, lo2.y), fminf(lo.z, lo2.z), 0.f);
@@ -514,34 +515,40 @@ __device__ __forceinline__ void store_xyz(float4* p, float4 v) {
     f[2] = v.z;
 }
 
-__global__ __launch_bounds__(kRefitThreads) void k_refit_local(
+template <int kLeaves>
+__global__ __launch_bounds__(kLeaves) void k_refit_local(
     int n, const int32_t* __restrict__ order, const float4* __restrict__ aabb,
     float4* __restrict__ nodes, const int32_t* __restrict__ parent, const int2* __restrict__ rng,
     const int2* __restrict__ kids, const int32_t* __restrict__ leafNeed,
     float4* __restrict__ sufBox, float4* __restrict__ preBox, float4* __restrict__ blkBox,
     float4* __restrict__ sceneBox) {
-    __shared__ float4 lbox[kRefitLeaves][2];
-    __shared__ float4 nbox[kRefitLeaves][2];
-    __shared__ int2 lkids[kRefitLeaves];          // kids[] of nodes base..end-1
-    __shared__ int32_t lpar[kRefitLeaves];        // parent[] of nodes base..end-1
-    __shared__ uint32_t arrived[kRefitLeaves / 32];  // first child done, per node
+    __shared__ float4 lbox[kLeaves][2];
+    __shared__ float4 nbox[kLeaves][2];
+    __shared__ int2 lkids[kLeaves];          // kids[] of nodes base..end-1
+    __shared__ int32_t lpar[kLeaves];        // parent[] of nodes base..end-1
+    __shared__ uint32_t arrived[kLeaves / 32];  // first child done, per node
     const int t = threadIdx.x;
-    const int base = blockIdx.x * kRefitLeaves;
-    const int end = min(base + kRefitLeaves, n);
+    const int base = blockIdx.x * kLeaves;
+    const int end = min(base + kLeaves, n);
     const int cntLeaves = end - base;
-    static_assert(kRefitPer == 1, "one leaf per thread");
     // A Karras node's range starts or ends at its own index, so every node
     // whose range lies in [base, end) has its index there too: its topology
-    // is loaded here once, with coalesced reads.
-    if (t < kRefitLeaves / 32) arrived[t] = 0u;
+    // is loaded here once, with coalesced reads, together with the leaf's
+    // parent slot, the node's range and the leaf's suffix / prefix needs.
+    if (t < kLeaves / 32) arrived[t] = 0u;
+    int slot = -1, need = 0;
+    int2 pr = make_int2(-1, -1);
     if (base + t < end) {
         const int seg = order[base + t];
-        lbox[t][0] = aabb[2 * seg];
-        lbox[t][1] = aabb[2 * seg + 1];
+        slot = parent[(n - 1) + base + t];
+        need = leafNeed[base + t];
         if (base + t < n - 1) {
             lkids[t] = kids[base + t];
             lpar[t] = parent[base + t];
+            pr = rng[base + t];
         }
+        lbox[t][0] = aabb[2 * seg];
+        lbox[t][1] = aabb[2 * seg + 1];
     }
     __syncthreads();
 
@@ -550,7 +557,6 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_local(
     if (base + t < end) {
         float4 lo = lbox[t][0];
         float4 hi = lbox[t][1];
-        int slot = parent[(n - 1) + base + t];
         bool leaf = true;
         while (slot >= 0) {
             const int p = (slot & ~kParentStrad) >> 1;
@@ -583,11 +589,8 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_local(
     __syncthreads();
 
     // Child slots of every node completed in this block (coalesced pass).
-    for (int r = 0; r < kRefitPer; r++) {
-        const int p = base + t + kRefitThreads * r;
-        if (p >= end || p >= n - 1) continue;
-        const int2 pr = rng[p];
-        if (pr.x < base || pr.y >= end) continue;
+    const int p = base + t;
+    if (p < end && p < n - 1 && pr.x >= base && pr.y < end) {
         const int2 kc = lkids[t];
         float4 lo0, hi0, lo1, hi1;
         if (kc.x < 0) { lo0 = lbox[~kc.x - base][0]; hi0 = lbox[~kc.x - base][1]; }
@@ -609,8 +612,8 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_local(
     // of straddling ranges: scans within each wave by shuffles, then across
     // the block's waves through LDS.  Leaves past the block's end are empty
     // boxes.  Min/max make every order of combination give the same box.
-    static_assert(kRefitPer == 1 && kRefitThreads % 64 == 0, "one leaf per thread");
-    constexpr int kWaves = kRefitThreads / 64;
+    static_assert(kLeaves % 64 == 0, "whole waves");
+    constexpr int kWaves = kLeaves / 64;
     __shared__ float4 wpre[kWaves][2], wsuf[kWaves][2];
     const int lane = t & 63, w = t >> 6;
     float4 plo = make_float4(INFINITY, INFINITY, INFINITY, 0.f);
@@ -647,7 +650,6 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_local(
     {
         const int k = base + t;
         if (k < end) {
-            const int need = leafNeed[k];
             if (need & 1) {
                 sufBox[2 * k] = slo;
                 sufBox[2 * k + 1] = shi;
@@ -718,7 +720,7 @@ __global__ __launch_bounds__(256) void k_refit_table_level(int nb, int L,
     dst[2 * b + 1] = hi;
 }
 
-__global__ __launch_bounds__(256) void k_refit_top(int n, int nb,
+__global__ __launch_bounds__(256) void k_refit_top(int n, int nb, int leafShift,
                                                    const int32_t* __restrict__ strad,
                                                    const int32_t* __restrict__ stradCount,
                                                    const int2* __restrict__ rng,
@@ -734,8 +736,8 @@ __global__ __launch_bounds__(256) void k_refit_top(int n, int nb,
         const int2 r = rng[i];
         float4 lo = sufBox[2 * r.x], hi = sufBox[2 * r.x + 1];
         box_union(lo, hi, preBox[2 * r.y], preBox[2 * r.y + 1]);
-        const int b0 = r.x / kRefitLeaves + 1;
-        const int b1 = r.y / kRefitLeaves - 1;
+        const int b0 = (r.x >> leafShift) + 1;
+        const int b1 = (r.y >> leafShift) - 1;
         if (b1 >= b0) {
             const int span = b1 - b0 + 1;
             const int L = 31 - __clz(span);
@@ -1048,7 +1050,7 @@ void launch_rebuild(yb_sim* s) {
     SceneKeys sk{batched ? s->b.sceneOf : nullptr, s->b.sceneOff, s->b.sceneRoot};
     hipLaunchKernelGGL(k_karras, dim3(blocks(n - 1, 256)), dim3(256), 0, s->stream, n,
                        s->b.keys[0], s->b.order, s->b.nodes, s->b.parent, s->b.rng, s->b.kids,
-                       s->b.strad, s->b.stradCount, s->b.leafNeed, sk);
+                       s->b.strad, s->b.stradCount, s->b.leafNeed, sk, s->refitShift);
 }
 
 void launch_scene_end_flags(yb_sim* s) {
@@ -1056,11 +1058,21 @@ void launch_scene_end_flags(yb_sim* s) {
                        s->numScenes, s->b.sceneOff, s->b.prm, s->b.sceneEndFlags);
 }
 
+template <int kLeaves>
+static void launch_refit_local(yb_sim* s) {
+    hipLaunchKernelGGL(k_refit_local<kLeaves>, dim3(s->refitBlocks), dim3(kLeaves), 0, s->stream,
+                       s->n, s->b.order, s->b.aabb, s->b.nodes, s->b.parent, s->b.rng, s->b.kids,
+                       s->b.leafNeed, s->b.sufBox, s->b.preBox, s->b.blkTab, s->b.sceneBox);
+}
+
 void launch_refit(yb_sim* s) {
     const int n = s->n;
-    hipLaunchKernelGGL(k_refit_local, dim3(s->refitBlocks), dim3(kRefitThreads), 0, s->stream, n,
-                       s->b.order, s->b.aabb, s->b.nodes, s->b.parent, s->b.rng, s->b.kids,
-                       s->b.leafNeed, s->b.sufBox, s->b.preBox, s->b.blkTab, s->b.sceneBox);
+    if (s->refitLeaves == 256)
+        launch_refit_local<256>(s);
+    else if (s->refitLeaves == 512)
+        launch_refit_local<512>(s);
+    else
+        launch_refit_local<1024>(s);
     if (s->refitBlocks > 1) {
         const int nb = s->refitBlocks;
         hipLaunchKernelGGL(k_refit_table, dim3(blocks(nb, 1024)), dim3(1024), 0, s->stream, nb,
@@ -1069,18 +1081,19 @@ void launch_refit(yb_sim* s) {
             hipLaunchKernelGGL(k_refit_table_level, dim3(blocks(nb, 256)), dim3(256), 0, s->stream,
                                nb, L, s->b.blkTab);
         hipLaunchKernelGGL(k_refit_top, dim3(s->refitTopBlocks), dim3(256), 0, s->stream, n,
-                           s->refitBlocks, s->b.strad, s->b.stradCount, s->b.rng, s->b.parent,
+                           s->refitBlocks, s->refitShift, s->b.strad, s->b.stradCount, s->b.rng, s->b.parent,
                            s->b.sufBox, s->b.preBox, s->b.blkTab, s->b.nodes, s->b.sceneBox);
     }
 }
 
-// 8 stack entries in LDS and 6 buffered candidates: 80 B of LDS per thread,
-// which lets a CU hold the full 8 waves per SIMD.  On the 1M-vertex knit and
+// The default shape, 8 stack entries in LDS and 6 buffered candidates: 80 B of
+// LDS per thread, which lets a CU hold the full 8 waves per SIMD.  On the
+// 1M-vertex knit and
 // the 512-scene batch this ran the query 1.8x and 2.9x faster than a 64-entry
 // LDS stack with 16 candidates (384 B per thread, 1.5 waves per SIMD): the
 // traversal waits little on memory, but a SIMD needs several waves to
 // interleave its VALU, scalar, branch and LDS instructions.
-template <int T, int SL = 8, int C = 6>
+template <int T, int SL, int C>
 static void launch_query_t(yb_sim* s) {
     hipLaunchKernelGGL((k_query<T, SL, C>), dim3(blocks(s->n, T)), dim3(T), 0, s->stream, s->b.params,
                        s->b.order, s->b.aabb, s->b.nodes, s->b.pos, s->b.prm, s->b.numCols,
@@ -1088,13 +1101,33 @@ static void launch_query_t(yb_sim* s) {
                        s->b.sceneRoot, s->queryStack, s->countCandidates ? s->b.candCount : nullptr);
 }
 
-void launch_query(yb_sim* s) {
+// Query shapes (LDS stack entries, buffered candidates) the autotuner picks
+// from: 48, 80, 96 and 128 B of LDS per thread.
+bool query_shape_supported(int stackLds, int candidates) {
+    return (stackLds == 4 && candidates == 4) || (stackLds == 8 && candidates == 6) ||
+           (stackLds == 8 && candidates == 8) || (stackLds == 16 && candidates == 8);
+}
+
+template <int SL, int C>
+static void launch_query_shape(yb_sim* s) {
     if (s->queryThreads == 64)
-        launch_query_t<64>(s);
+        launch_query_t<64, SL, C>(s);
     else if (s->queryThreads == 256)
-        launch_query_t<256>(s);
+        launch_query_t<256, SL, C>(s);
     else
-        launch_query_t<128>(s);
+        launch_query_t<128, SL, C>(s);
+}
+
+void launch_query(yb_sim* s) {
+    const int sl = s->queryStackLds, c = s->queryCandidates;
+    if (sl == 4 && c == 4)
+        launch_query_shape<4, 4>(s);
+    else if (sl == 8 && c == 8)
+        launch_query_shape<8, 8>(s);
+    else if (sl == 16 && c == 8)
+        launch_query_shape<16, 8>(s);
+    else
+        launch_query_shape<8, 6>(s);
 }
 
 }  // namespace yb

@@ -107,7 +107,8 @@ constexpr int kSortBins = 1 << kSortBits;
 constexpr int kSortThreads = 256;
 constexpr int kSortItems = 8;
 constexpr int kSortTile = kSortThreads * kSortItems;
-constexpr int kRefitLeaves = 1024;  // sorted leaves per refit workgroup
+// sorted leaves per refit workgroup (the refit tile): 256, 512 or 1024
+constexpr int kRefitLeavesMin = 256, kRefitLeavesMax = 1024;
 // Flag in a parent slot ((node << 1) | side, node < 2^28): set when that parent
 // node's leaf range crosses a boundary between refit workgroups.
 constexpr int32_t kParentStrad = 1 << 30;
@@ -196,9 +197,13 @@ struct yb_sim {
     yb::Buffers b;
     int sortTiles = 0;
     int boxBlocks = 0;
+    int refitLeaves = yb::kRefitLeavesMax;  // refit tile (yb_set_tile_config)
+    int refitShift = 10;                // log2(refitLeaves)
     int refitBlocks = 0;
     int refitLevels = 0;
     int refitTopBlocks = 0;
+    int queryStackLds = 8;   // k_query: traversal-stack entries in LDS
+    int queryCandidates = 6; // k_query: candidate leaves buffered per thread
     bool treeBuilt = false;
     bool uploaded = false;
     int numScenes = 1;
@@ -302,5 +307,6 @@ void launch_aabb(yb_sim* s);
 void launch_rebuild(yb_sim* s);    // scene box, Morton, sort, Karras
 void launch_refit(yb_sim* s);
 void launch_query(yb_sim* s);      // traversal + narrowphase -> contact lists
+bool query_shape_supported(int stackLds, int candidates);
 
 }  // namespace yb

@@ -198,6 +198,20 @@ class Sim:
         check(self._lib.yb_get_launch_config(self._h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
 
+    def setTileConfig(self, refitLeaves=1024, queryStackLds=8, queryCandidates=6):
+        """Refit tile (leaves per workgroup) and the contact query's LDS shape
+        (speed only; a new refit tile rebuilds the tree at the next detection)."""
+        self._need_handle()
+        check(self._lib.yb_set_tile_config(self._h, int(refitLeaves), int(queryStackLds),
+                                           int(queryCandidates)), "setTileConfig")
+
+    def tileConfig(self):
+        self._need_handle()
+        a, b, c = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+        check(self._lib.yb_get_tile_config(self._h, ctypes.byref(a), ctypes.byref(b),
+                                           ctypes.byref(c)))
+        return a.value, b.value, c.value
+
     def profile(self, substeps=2):
         """Run `substeps` substeps of maxH with HIP events around every stage
         (yb_profile); returns {stage: (milliseconds, launches)}."""
