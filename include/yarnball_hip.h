@@ -330,7 +330,11 @@ typedef struct yb_solver_stats_t {
     float maxResidual;    /* largest final stopping residual of a substep */
     float lastNewtonResidual;  /* r_F of the last check */
     float maxNewtonResidual;   /* largest final r_F of a substep */
-    double sumNewtonResidual;  /* final r_F summed over substeps */
+    double sumNewtonResidual;  /* final r_F summed over substeps (over scene-substeps
+                                  in a batch) */
+    int64_t sceneSubsteps;     /* batched scenes: substeps x scenes (else = substeps) */
+    int64_t sceneSweeps;       /* batched scenes: each scene's own sweeps, summed
+                                  (else = sweeps) */
 } yb_solver_stats_t;
 int yb_set_solver(yb_sim* s, float residualTol, int maxSweeps, int chunk, int checkEvery);
 /* Stopping test of the residual-targeted mode: YB_RESIDUAL_UPDATE (default) or
@@ -349,8 +353,13 @@ int yb_solver_stats(yb_sim* s, yb_solver_stats_t* out, int reset);
  * that start at their own scene's subtree root.  The scenes share one
  * yb_params (gravity, radii, stiffness scales, schedule); per-vertex
  * stiffness and rest shapes are per scene.  With the fixed sweep schedule each
- * scene's state evolves bit-identically to a standalone handle of that scene;
- * the residual-targeted schedule uses one residual over all scenes.
+ * scene's state evolves bit-identically to a standalone handle of that scene.
+ * In the residual-targeted schedule every scene has its own residual and
+ * stops at its own sweep (converged scenes are left alone while the others
+ * run; the substep ends when the last scene has converged), with the strand
+ * solve too; this needs scenes of at least 64 vertices (yb_advance fails
+ * with YB_ERR_INVALID_ARGUMENT otherwise).  Sweep counts per scene are in
+ * yb_solver_stats (sceneSweeps).
  * numScenes = 1 restores the single-scene layout.  Call after yb_create
  * (before or after yb_upload). */
 int yb_set_scenes(yb_sim* s, int numScenes, const int32_t* offsets);

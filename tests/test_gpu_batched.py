@@ -167,3 +167,133 @@ def test_full_512_scene_batch_properties():
           f"all inside their scenes; scenes {order[0]}, {order[len(order) // 2]}, {order[-1]} "
           f"equal their standalone runs")
     b.close()
+
+
+def _residual_runs(scs, tol, strand, steps):
+    """The batch and a standalone handle per scene, each in the residual-
+    targeted schedule (optionally with the strand solve), advanced by `steps`
+    substeps; returns the downloaded batch, its stats and the standalone
+    handles' states and stats."""
+    from yarnball_amd import learned
+
+    def setup(sim):
+        sim.printErrors = False
+        sim.setSolver(tol, maxSweeps=2000, chunk=8, checkEvery=4)
+        if strand:
+            learned.strand(sim, every=4, first=1)
+
+    big, offs = scenes.batch(scs)
+    b = Sim(big.n)
+    scenes.load_into(b, big)
+    b.setScenes(offs)
+    setup(b)
+    b.advance(np.float32(steps * big.maxH))
+    b.download()
+    bst = b.solverStats()
+    alone = []
+    for sc in scs:
+        s = Sim(sc.n)
+        scenes.load_into(s, sc)
+        setup(s)
+        s.advance(np.float32(steps * sc.maxH))
+        s.download()
+        alone.append((s.verts["pos"].copy(), s.vels.copy(), s.qs.copy(), s.solverStats()))
+        s.close()
+    return b, offs, bst, alone
+
+
+@pytest.mark.gpu
+def test_batched_residual_schedule_converges_per_scene():
+    """Each scene of a batch stops at its own sweep: with the plain start the
+    scene's state equals a standalone run's bit for bit (the per-scene
+    residual sums the same per-vertex terms as the standalone residual, in
+    tiles that may straddle two scenes), and the scenes' own sweep counts add
+    up to the standalone runs' totals, below the batch's sweeps x scenes."""
+    scs = _scenes()
+    steps = 8
+    b, offs, bst, alone = _residual_runs(scs, 1e-5, False, steps)
+    total = 0
+    for k, (pos, vel, q, st) in enumerate(alone):
+        sl = slice(offs[k], offs[k + 1])
+        assert st["notConverged"] == 0
+        assert np.array_equal(b.verts["pos"][sl], pos), f"scene {k} positions"
+        assert np.array_equal(b.vels[sl], vel), f"scene {k} velocities"
+        assert np.array_equal(b.qs[sl], q), f"scene {k} frames"
+        total += st["sweeps"]
+        print(f"scene {k}: {st['sweeps'] / steps:.1f} sweeps per substep standalone")
+    assert bst["notConverged"] == 0 and bst["substeps"] == steps
+    assert bst["sceneSubsteps"] == steps * len(scs)
+    assert bst["sceneSweeps"] == total
+    assert bst["sceneSweeps"] < bst["sweeps"] * len(scs)  # scenes stopped apart
+    print(f"batch: {bst['sweeps'] / steps:.1f} sweeps per substep, scenes "
+          f"{bst['meanSceneSweeps']:.1f} on average")
+    b.close()
+
+
+@pytest.mark.gpu
+def test_batched_strand_solve_on_tile_aligned_scenes():
+    """Scenes whose sizes are multiples of 62 start on a wave-tile boundary,
+    so the batch's strand tiles are the standalone runs' tiles: with the
+    strand solve every scene's state equals its standalone run's bit for bit,
+    each scene stopping at its own sweep."""
+    scs = scenes.random_walks([310, 992, 248, 2480, 620, 1054], seed=3, turn=0.8)
+    for sc in scs:
+        sc.params.bvhRebuildPeriod = 0.01
+    steps = 8
+    b, offs, bst, alone = _residual_runs(scs, 1e-5, True, steps)
+    total = 0
+    for k, (pos, vel, q, st) in enumerate(alone):
+        sl = slice(offs[k], offs[k + 1])
+        assert st["notConverged"] == 0
+        assert np.array_equal(b.verts["pos"][sl], pos), f"scene {k} positions"
+        assert np.array_equal(b.qs[sl], q), f"scene {k} frames"
+        total += st["sweeps"]
+    assert bst["notConverged"] == 0 and bst["sceneSweeps"] == total
+    assert bst["sceneSweeps"] < bst["sweeps"] * len(scs)
+    print(f"aligned strand batch: {bst['sweeps'] / steps:.1f} sweeps per substep, scenes "
+          f"{bst['meanSceneSweeps']:.1f}")
+    b.close()
+
+
+@pytest.mark.gpu
+def test_batched_strand_solve_on_unaligned_scenes():
+    """Scenes that start inside a wave tile share strand tiles with their
+    neighbours, whose tile-boundary solves differ from a standalone run's: the
+    states then agree to the solver's accuracy -- no further from the
+    standalone strand runs than those are from the plain solver at the same
+    tolerance -- and every scene converges at its own sweep."""
+    from helpers import rel_err
+
+    scs = _scenes()
+    steps = 8
+    b, offs, bst, alone = _residual_runs(scs, 1e-5, True, steps)
+    _, _, _, plain = _residual_runs(scs, 1e-5, False, steps)
+    worst = scale = 0.0
+    for k, ((pos, _, _, st), (ppos, _, _, _)) in enumerate(zip(alone, plain)):
+        sl = slice(offs[k], offs[k + 1])
+        assert st["notConverged"] == 0
+        assert np.isfinite(b.verts["pos"][sl]).all()
+        worst = max(worst, rel_err(b.verts["pos"][sl], pos))
+        scale = max(scale, rel_err(ppos, pos))
+    print(f"unaligned strand batch: worst position error {worst:.2e} (standalone plain vs "
+          f"strand {scale:.2e}), {bst['sweeps'] / steps:.1f} sweeps per substep, scenes "
+          f"{bst['meanSceneSweeps']:.1f}")
+    assert bst["notConverged"] == 0 and worst <= 2.0 * scale
+    assert bst["sceneSweeps"] < bst["sweeps"] * len(scs)
+    b.close()
+
+
+@pytest.mark.gpu
+def test_batched_residual_schedule_needs_scenes_of_64_vertices():
+    scs = scenes.random_walks([50, 300], seed=2)
+    big, offs = scenes.batch(scs)
+    b = Sim(big.n)
+    b.printErrors = False
+    scenes.load_into(b, big)
+    b.setScenes(offs)
+    b.setSolver(1e-5, maxSweeps=100, chunk=8, checkEvery=4)
+    with pytest.raises(Exception, match="64 vertices"):
+        b.advance(np.float32(big.maxH))
+    b.setSolver(0.0, maxSweeps=100, chunk=8, checkEvery=4)  # the fixed schedule runs
+    b.advance(np.float32(big.maxH))
+    b.close()

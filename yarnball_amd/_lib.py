@@ -56,6 +56,8 @@ class SolverStats(ctypes.Structure):
         ("lastNewtonResidual", ctypes.c_float),
         ("maxNewtonResidual", ctypes.c_float),
         ("sumNewtonResidual", ctypes.c_double),
+        ("sceneSubsteps", ctypes.c_int64),
+        ("sceneSweeps", ctypes.c_int64),
     ]
 
 

@@ -78,6 +78,8 @@ static int timeline_mark(yb_sim* s) {
     return YB_OK;
 }
 
+static int reset_scene_ctl(yb_sim* s);
+
 static void free_all(yb_sim* s) {
     timeline_free(s);
     Buffers& b = s->b;
@@ -310,19 +312,21 @@ extern "C" int yb_upload(yb_sim* s, const yb_vertex* verts, const float* vels, c
     // Update-residual normaliser: sum over free vertices of l_i^2, l_i the rest
     // length of the vertex's own segment (or of the previous one at a strand end).
     double denom = 0.0;
+    s->lsqHost.assign(N, 0.0);
     for (size_t i = 0; i < N; i++) {
         if (verts[i].invMass == 0.f) continue;
         double l = verts[i].lRest;
         if (!(verts[i].flags & YB_FLAG_HAS_NEXT) && (verts[i].flags & YB_FLAG_HAS_PREV) && i > 0)
             l = verts[i - 1].lRest;
         denom += l * l;
+        s->lsqHost[i] = l * l;
     }
     s->denomHost = denom;
     HIPCHECK(hipMemcpyAsync(&s->b.ctl->denom, &s->denomHost, sizeof(double),
                             hipMemcpyHostToDevice, s->stream));
     HIPCHECK(hipStreamSynchronize(s->stream));
     s->uploaded = true;
-    return YB_OK;
+    return reset_scene_ctl(s);
 }
 
 extern "C" int yb_driver_default(int kind, yb_driver* d) {
@@ -389,14 +393,37 @@ extern "C" int yb_driver_frame(yb_sim* s, yb_params* p, const yb_driver* d, floa
     return YB_OK;
 }
 
+// Per-scene solver state of a batch, reset: update-residual normalisers from
+// the uploaded vertices, everything else cleared (yb_set_scenes, yb_upload,
+// yb_solver_stats with reset).
+static int reset_scene_ctl(yb_sim* s) {
+    if (s->numScenes <= 1 || !s->b.sceneCtl) return YB_OK;
+    std::vector<SceneCtl> c(s->numScenes);
+    for (int k = 0; k < s->numScenes; k++) {
+        memset(&c[k], 0, sizeof(SceneCtl));
+        c[k].doneAt = 0x7fffffff;
+        c[k].denomF = -1.0;
+        double d = 0.0;
+        if (!s->lsqHost.empty())
+            for (int v = s->sceneOffHost[k]; v < s->sceneOffHost[k + 1]; v++) d += s->lsqHost[v];
+        c[k].denom = d;
+    }
+    HIPCHECK(hipMemcpyAsync(s->b.sceneCtl, c.data(), sizeof(SceneCtl) * c.size(),
+                            hipMemcpyHostToDevice, s->stream));
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    return YB_OK;
+}
+
 // Back to the single-scene layout (frees the batched-scene buffers).
 static int clear_scenes(yb_sim* s) {
     Buffers& b = s->b;
     for (void* p : {(void*)b.sceneOf, (void*)b.sceneOff, (void*)b.sceneBoxes, (void*)b.sceneRoot,
-                    (void*)b.sceneEndFlags})
+                    (void*)b.sceneEndFlags, (void*)b.sceneCtl, (void*)b.scenePart})
         if (p) HIPCHECK(hipFree(p));
     b.sceneOf = b.sceneOff = b.sceneRoot = b.sceneEndFlags = nullptr;
     b.sceneBoxes = nullptr;
+    b.sceneCtl = nullptr;
+    b.scenePart = nullptr;
     s->numScenes = 1;
     s->sceneOffHost.clear();
     s->treeBuilt = false;
@@ -433,6 +460,8 @@ extern "C" int yb_set_scenes(yb_sim* s, int numScenes, const int32_t* offsets) {
     HIPCHECK(hipMalloc(&b.sceneBoxes, sizeof(float4) * 2 * numScenes));
     HIPCHECK(hipMalloc(&b.sceneRoot, sizeof(int32_t) * numScenes));
     HIPCHECK(hipMalloc(&b.sceneEndFlags, sizeof(int32_t) * numScenes));
+    HIPCHECK(hipMalloc(&b.sceneCtl, sizeof(SceneCtl) * numScenes));
+    HIPCHECK(hipMalloc(&b.scenePart, sizeof(double2) * 6 * (size_t)sweep_waves(s->n)));
     std::vector<int32_t> of(N);
     for (int k = 0; k < numScenes; k++)
         for (int v = offsets[k]; v < offsets[k + 1]; v++) of[v] = k;
@@ -441,6 +470,8 @@ extern "C" int yb_set_scenes(yb_sim* s, int numScenes, const int32_t* offsets) {
                        hipMemcpyHostToDevice));
     s->numScenes = numScenes;
     s->sceneOffHost.assign(offsets, offsets + numScenes + 1);
+    rc = reset_scene_ctl(s);
+    if (rc != YB_OK) return rc;
     if (s->uploaded) {
         launch_scene_end_flags(s);
         HIPCHECK(hipGetLastError());
@@ -673,8 +704,11 @@ static bool checks_residual(const yb_sim* s, int k) {
 // substep: a substep that has not converged by then finishes with plain sweeps.
 constexpr int kStrandSteps = 16;
 static void enqueue_sweep(yb_sim* s, int k, bool adaptive, bool last = false) {
+    // batched scenes decide each scene's convergence in a launch of their own
+    // right after every check sweep (no in-sweep check of the previous one)
+    const bool perScene = adaptive && s->numScenes > 1;
     const bool check = adaptive && checks_residual(s, k);
-    bool checkPrev = adaptive && k > 0 && checks_residual(s, k - 1);
+    bool checkPrev = !perScene && adaptive && k > 0 && checks_residual(s, k - 1);
     const int first = s->strandFirst > 0 ? s->strandFirst : s->strandEvery;
     if (adaptive && s->strandSolve && s->strandEvery > 0 && k >= first &&
         (k - first) % s->strandEvery == 0 && (k - first) / s->strandEvery < kStrandSteps) {
@@ -690,7 +724,7 @@ static void enqueue_sweep(yb_sim* s, int k, bool adaptive, bool last = false) {
         mark(s, YB_STAGE_MLP);
     }
     launch_sweep(s, k, adaptive, check, checkPrev);
-    const bool standalone = check && k == s->maxSweeps - 1;
+    const bool standalone = check && (perScene || k == s->maxSweeps - 1);
     if (last || standalone)
         mark(s, YB_STAGE_SWEEP);
     else
@@ -737,6 +771,19 @@ static int rebuild_graphs(yb_sim* s, int numItr) {
         if (rc != YB_OK) return rc;
     }
     s->graphItr = numItr;
+    return YB_OK;
+}
+
+// The residual-targeted schedule on a batch keeps each tile's residual
+// partials for at most two scenes.
+static int check_scene_sizes(const yb_sim* s) {
+    for (int k = 0; k < s->numScenes && s->numScenes > 1; k++)
+        if (s->sceneOffHost[k + 1] - s->sceneOffHost[k] < kMinSceneForResidual) {
+            set_message("residual-targeted batches need scenes of at least 64 vertices (scene " +
+                        std::to_string(k) + " has " +
+                        std::to_string(s->sceneOffHost[k + 1] - s->sceneOffHost[k]) + ")");
+            return YB_ERR_INVALID_ARGUMENT;
+        }
     return YB_OK;
 }
 
@@ -974,6 +1021,24 @@ extern "C" int yb_solver_stats(yb_sim* s, yb_solver_stats_t* out, int reset) {
     out->lastNewtonResidual = c.fres;
     out->maxNewtonResidual = c.maxFres;
     out->sumNewtonResidual = c.sumFres;
+    out->sceneSubsteps = out->substeps;
+    out->sceneSweeps = out->sweeps;
+    if (s->numScenes > 1 && s->b.sceneCtl) {
+        // a batch: every scene converges once per substep, at its own sweep
+        std::vector<SceneCtl> sc(s->numScenes);
+        HIPCHECK(hipMemcpyAsync(sc.data(), s->b.sceneCtl, sizeof(SceneCtl) * sc.size(),
+                                hipMemcpyDeviceToHost, s->stream));
+        HIPCHECK(hipStreamSynchronize(s->stream));
+        int64_t sw = 0;
+        double fr = 0.0;
+        for (const SceneCtl& k : sc) {
+            sw += (int64_t)k.sweeps;
+            fr += k.sumFres;
+        }
+        out->sceneSubsteps = out->substeps * s->numScenes;
+        out->sceneSweeps = sw;
+        out->sumNewtonResidual = fr;
+    }
     if (reset) {
         SolverCtl z;
         memset(&z, 0, sizeof(z));
@@ -983,6 +1048,8 @@ extern "C" int yb_solver_stats(yb_sim* s, yb_solver_stats_t* out, int reset) {
         z.kind = s->residualKind;
         HIPCHECK(hipMemcpyAsync(s->b.ctl, &z, sizeof(z), hipMemcpyHostToDevice, s->stream));
         HIPCHECK(hipStreamSynchronize(s->stream));
+        const int rc = reset_scene_ctl(s);
+        if (rc != YB_OK) return rc;
     }
     return YB_OK;
 }
@@ -1173,6 +1240,10 @@ extern "C" int yb_advance(yb_sim* s, yb_params* p, float h, float maxH, int* war
     rc = upload_params(s, p);
     if (rc != YB_OK) return rc;
     const bool adaptive = s->tol > 0;
+    if (adaptive) {
+        rc = check_scene_sizes(s);
+        if (rc != YB_OK) return rc;
+    }
     if (adaptive && s->useGraphs) {
         rc = rebuild_adaptive_graphs(s);
         if (rc != YB_OK) return rc;
@@ -1417,6 +1488,10 @@ extern "C" int yb_profile(yb_sim* s, yb_params* p, float h, int substeps, double
     rc = upload_params(s, p);
     if (rc != YB_OK) return rc;
     const bool adaptive = s->tol > 0;
+    if (adaptive) {
+        const int rcs = check_scene_sizes(s);
+        if (rcs != YB_OK) return rcs;
+    }
     ProfState ps;
     s->prof = &ps;
     // detach and free the markers on every exit path, early error returns included

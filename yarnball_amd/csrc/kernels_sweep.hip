@@ -214,7 +214,34 @@ __device__ __forceinline__ void pcr64(float& A, float& Lo, float& Up, float B[5]
 // squared update |dx' - dx|^2 (kSweep) and of the squared residual force |r|^2
 // at the input iterate (r: see the Newton step below).
 constexpr int kSweep = 0, kLine = 1, kEval = 2;
-template <int kMode>
+
+// Batched scenes in the residual-targeted schedule (kBatched): a wave tile
+// holds at most two scenes (A: the scene of its first owned vertex, B: of its
+// last); a lane of a scene that has converged computes but writes nothing,
+// and a tile whose scenes have both converged exits at once (a strand tile
+// then writes empty rows, so that its neighbours read finite values).  The
+// residual partials are kept per scene side: sums over the lanes of scene A
+// at partial[wave], of scene B at partial[tiles + wave].
+
+__device__ __forceinline__ void wave_residual_scenes(float numU, float numF, bool sideA,
+                                                     double2* partial, int wave, int n) {
+    float uA = sideA ? numU : 0.0f, fA = sideA ? numF : 0.0f;
+    float uB = sideA ? 0.0f : numU, fB = sideA ? 0.0f : numF;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        uA += __shfl_xor(uA, off, 64);
+        fA += __shfl_xor(fA, off, 64);
+        uB += __shfl_xor(uB, off, 64);
+        fB += __shfl_xor(fB, off, 64);
+    }
+    const int tiles = (n + 61) / 62;
+    if ((threadIdx.x & 63) == 0 && wave < tiles) {
+        partial[wave] = make_double2((double)uA, (double)fA);
+        partial[tiles + wave] = make_double2((double)uB, (double)fB);
+    }
+}
+
+template <int kMode, bool kBatched = false>
 __device__ __forceinline__ void sweep_body(const DevParams* __restrict__ P,
                                            const float4* __restrict__ pos,
                                            const float4* __restrict__ prm,
@@ -230,7 +257,8 @@ __device__ __forceinline__ void sweep_body(const DevParams* __restrict__ P,
                                            double2* __restrict__ partial,
                                            const double2* __restrict__ partialPrev,
                                            const double2* __restrict__ denomPart,
-                                           float tol, int maxSweeps, LineOut lo) {
+                                           float tol, int maxSweeps, LineOut lo,
+                                           SceneArgs sa = SceneArgs{nullptr, nullptr}) {
     // Residual-targeted mode: sweeps after convergence do nothing.
     if (ctl && ctl->doneAt < sweep) return;
     const int n = P->n;
@@ -239,13 +267,33 @@ __device__ __forceinline__ void sweep_body(const DevParams* __restrict__ P,
     // of this sweep runs.  A sweep that converged is thus recognised one sweep
     // later (this sweep's result is not used; the next one exits at once), and
     // no separate reduction launch sits between two sweeps.
-    if (partialPrev && blockIdx.x == 0 && threadIdx.x < 64)
+    if (!kBatched && partialPrev && blockIdx.x == 0 && threadIdx.x < 64)
         residual_wave(ctl, partialPrev, denomPart, (n + 61) / 62, sweep - 1, tol, maxSweeps);
     const int lane = threadIdx.x & 63;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int wave = (int)((tile * blockDim.x + threadIdx.x) >> 6);
     const int v = wave * 62 + lane - 1;
     const bool in = (v >= 0) && (v < n);
+
+    // batched: the tile's scenes, whether each still runs, this lane's side
+    bool live = true, sideA = true;
+    if constexpr (kBatched) {
+        const int scA = sa.sceneOf[min(wave * 62, n - 1)];
+        const int scB = sa.sceneOf[min(wave * 62 + 61, n - 1)];
+        const bool liveA = sa.ctl[scA].doneAt >= sweep;
+        const bool liveB = sa.ctl[scB].doneAt >= sweep;
+        if (!liveA && !liveB) {
+            if constexpr (kMode == kLine) {
+                if (in && lane >= 1 && lane <= 62) {
+                    lo.g[v] = make_float4(0.f, 0.f, 0.f, 0.f);
+                    lo.ql[v] = make_float2(0.0f, -1.0f);
+                }
+            }
+            return;
+        }
+        sideA = sa.sceneOf[min(max(v, 0), n - 1)] == scA;
+        live = sideA ? liveA : liveB;
+    }
 
     const float h = P->h;
     const float damping = P->dampingH;
@@ -423,7 +471,10 @@ __device__ __forceinline__ void sweep_body(const DevParams* __restrict__ P,
         }
     }
     if constexpr (kMode == kEval) {
-        wave_residual(0.0f, owner ? numF : 0.0f, partial, wave, n);
+        if constexpr (kBatched)
+            wave_residual_scenes(0.0f, owner ? numF : 0.0f, sideA, partial, wave, n);
+        else
+            wave_residual(0.0f, owner ? numF : 0.0f, partial, wave, n);
         if (!qOut) return;
         // frame refresh from the input iterate itself (the closed-form frame
         // update below, with dx' = dx)
@@ -489,7 +540,7 @@ __device__ __forceinline__ void sweep_body(const DevParams* __restrict__ P,
     const rot qNext = rot{__shfl_down(q4.x, 1, 64), __shfl_down(q4.y, 1, 64),
                           __shfl_down(q4.z, 1, 64), __shfl_down(q4.w, 1, 64)};
 
-    if (owner) {
+    if (owner && live) {
         if constexpr (kMode == kSweep) dxOut[v] = f4(dxNew, d4.w);
         rot q0 = asrot(q4);
         if (!(flags & YB_FLAG_FIX_ORIENTATION) && (flags & YB_FLAG_HAS_NEXT)) {
@@ -510,10 +561,18 @@ __device__ __forceinline__ void sweep_body(const DevParams* __restrict__ P,
         }
         qOut[v] = f4r(q0);
     }
-    if constexpr (kMode == kSweep)
-        if (partial) wave_residual(owner ? num : 0.0f, owner ? numF : 0.0f, partial, wave, n);
+    if constexpr (kMode == kSweep) {
+        if (partial) {
+            if constexpr (kBatched)
+                wave_residual_scenes(owner ? num : 0.0f, owner ? numF : 0.0f, sideA, partial,
+                                     wave, n);
+            else
+                wave_residual(owner ? num : 0.0f, owner ? numF : 0.0f, partial, wave, n);
+        }
+    }
 }
 
+template <bool kBatched>
 __global__ __launch_bounds__(512) void k_sweep(const DevParams* __restrict__ P,
                                                const float4* __restrict__ pos,
                                                const float4* __restrict__ prm,
@@ -529,13 +588,14 @@ __global__ __launch_bounds__(512) void k_sweep(const DevParams* __restrict__ P,
                                                double2* __restrict__ partial,
                                                const double2* __restrict__ partialPrev,
                                                const double2* __restrict__ denomPart,
-                                               float tol, int maxSweeps) {
-    sweep_body<kSweep>(P, pos, prm, yv, qIn, qOut, qRest, dxIn, dxOut, numCols, cols, ctl, sweep,
-                       partial, partialPrev, denomPart, tol, maxSweeps,
-                       LineOut{nullptr, nullptr, nullptr});
+                                               float tol, int maxSweeps, SceneArgs sa) {
+    sweep_body<kSweep, kBatched>(P, pos, prm, yv, qIn, qOut, qRest, dxIn, dxOut, numCols, cols,
+                                 ctl, sweep, partial, partialPrev, denomPart, tol, maxSweeps,
+                                 LineOut{nullptr, nullptr, nullptr}, sa);
 }
 
 // Strand solve, tile part (see the line mode of sweep_body).
+template <bool kBatched>
 __global__ __launch_bounds__(256) void k_line_tile(const DevParams* __restrict__ P,
                                                    const float4* __restrict__ pos,
                                                    const float4* __restrict__ prm,
@@ -545,16 +605,18 @@ __global__ __launch_bounds__(256) void k_line_tile(const DevParams* __restrict__
                                                    const float4* __restrict__ dxIn,
                                                    const int32_t* __restrict__ numCols,
                                                    const int32_t* __restrict__ cols, LineOut lo,
-                                                   const SolverCtl* __restrict__ ctl, int sweep) {
+                                                   const SolverCtl* __restrict__ ctl, int sweep,
+                                                   SceneArgs sa) {
     if (strand_skip(ctl, sweep)) return;
-    sweep_body<kLine>(P, pos, prm, yv, qIn, nullptr, qRest, dxIn, nullptr, numCols, cols, nullptr,
-                      0, nullptr, nullptr, nullptr, 0.0f, 0, lo);
+    sweep_body<kLine, kBatched>(P, pos, prm, yv, qIn, nullptr, qRest, dxIn, nullptr, numCols,
+                                cols, nullptr, sweep, nullptr, nullptr, nullptr, 0.0f, 0, lo, sa);
 }
 
 // Residual partials of an iterate without changing it: the denominator |f(dx0)|
 // of the relative residual when the sweeps do not start from the plain warm
 // start.  With qOut set, also the frames refreshed from that iterate (the
 // sweep's closed-form frame update at dx' = dx), written to qOut.
+template <bool kBatched>
 __global__ __launch_bounds__(256) void k_force_eval(const DevParams* __restrict__ P,
                                                     const float4* __restrict__ pos,
                                                     const float4* __restrict__ prm,
@@ -565,9 +627,11 @@ __global__ __launch_bounds__(256) void k_force_eval(const DevParams* __restrict_
                                                     const float4* __restrict__ dxIn,
                                                     const int32_t* __restrict__ numCols,
                                                     const int32_t* __restrict__ cols,
-                                                    double2* __restrict__ partial) {
-    sweep_body<kEval>(P, pos, prm, yv, qIn, qOut, qRest, dxIn, nullptr, numCols, cols, nullptr,
-                      0, partial, nullptr, nullptr, 0.0f, 0, LineOut{nullptr, nullptr, nullptr});
+                                                    double2* __restrict__ partial,
+                                                    SceneArgs sa) {
+    sweep_body<kEval, kBatched>(P, pos, prm, yv, qIn, qOut, qRest, dxIn, nullptr, numCols, cols,
+                                nullptr, 0, partial, nullptr, nullptr, 0.0f, 0,
+                                LineOut{nullptr, nullptr, nullptr}, sa);
 }
 
 // Strand solve, boundary part: one iteration over the pairs of vertices that
@@ -624,7 +688,7 @@ __global__ __launch_bounds__(256) void k_line_pairs_apply(int n, int tiles, int 
                                                           const float2* __restrict__ ql,
                                                           float4* __restrict__ dx,
                                                           const SolverCtl* __restrict__ ctl,
-                                                          int sweep) {
+                                                          int sweep, SceneArgs sa) {
     if (strand_skip(ctl, sweep)) return;
     const int lane = threadIdx.x & 63;
     const int w = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6);
@@ -662,6 +726,7 @@ __global__ __launch_bounds__(256) void k_line_pairs_apply(int n, int tiles, int 
     const v3 Ew = v3{__shfl(E.x, 1, 64), __shfl(E.y, 1, 64), __shfl(E.z, 1, 64)};
     const v3 Fw = v3{__shfl(F.x, 2, 64), __shfl(F.y, 2, 64), __shfl(F.z, 2, 64)};
     if (!own || QL.y < 0.0f) return;
+    if (sa.ctl && sa.ctl[sa.sceneOf[v]].doneAt < sweep) return;  // its scene has converged
     v3 d = xyz(G);
     if (w >= 1) d += G.w * Ew;
     if (w + 1 < tiles) d += QL.x * Fw;
@@ -681,12 +746,14 @@ __global__ __launch_bounds__(256) void k_line_apply(int n, int tiles,
                                                     const float4* __restrict__ E,
                                                     const float4* __restrict__ F,
                                                     float4* __restrict__ dx,
-                                                    const SolverCtl* __restrict__ ctl, int sweep) {
+                                                    const SolverCtl* __restrict__ ctl, int sweep,
+                                                    SceneArgs sa) {
     if (strand_skip(ctl, sweep)) return;
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n) return;
     const float2 QL = ql[v];
     if (QL.y < 0.0f) return;
+    if (sa.ctl && sa.ctl[sa.sceneOf[v]].doneAt < sweep) return;  // its scene has converged
     const int w = v / 62;
     const float4 G = g[v];
     v3 d = xyz(G);
@@ -798,10 +865,104 @@ __global__ __launch_bounds__(64) void k_residual(SolverCtl* __restrict__ ctl,
     residual_wave(ctl, partial, denomPart, nb, sweep, tol, maxSweeps);
 }
 
-__global__ void k_ctl_reset(SolverCtl* __restrict__ ctl) {
-    if (threadIdx.x == 0) {
+__global__ void k_ctl_reset(SolverCtl* __restrict__ ctl, SceneCtl* __restrict__ sceneCtl,
+                            int numScenes) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k == 0) {
         ctl->doneAt = 0x7fffffff;
         ctl->denomF = -1.0;
+        ctl->scenesDone = 0;
+    }
+    if (sceneCtl && k < numScenes) {
+        sceneCtl[k].doneAt = 0x7fffffff;
+        sceneCtl[k].denomF = -1.0;
+    }
+}
+
+// Fixed-order sum of the partials [a, b) by one wave (as reduce_partials).
+__device__ __forceinline__ double2 reduce_range(const double2* __restrict__ p, int a, int b) {
+    const int lane = threadIdx.x & 63;
+    double2 acc[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc[j] = make_double2(0.0, 0.0);
+    int k = a + lane;
+    for (; k + 3 * 64 < b; k += 4 * 64) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const double2 t = p[k + 64 * j];
+            acc[j].x += t.x;
+            acc[j].y += t.y;
+        }
+    }
+    for (; k < b; k += 64) {
+        const double2 t = p[k];
+        acc[0].x += t.x;
+        acc[0].y += t.y;
+    }
+    double x = (acc[0].x + acc[1].x) + (acc[2].x + acc[3].x);
+    double y = (acc[0].y + acc[1].y) + (acc[2].y + acc[3].y);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        x += __shfl_xor(x, off, 64);
+        y += __shfl_xor(y, off, 64);
+    }
+    return make_double2(x, y);
+}
+
+// Sum over scene sc of per-scene-side tile partials (wave_residual_scenes):
+// its first tile contributes the side that holds it, every later tile its
+// side A.
+__device__ __forceinline__ double2 scene_sum(const double2* __restrict__ part,
+                                             const int32_t* __restrict__ sceneOff,
+                                             const int32_t* __restrict__ sceneOf, int sc,
+                                             int tiles) {
+    const int v0 = sceneOff[sc], v1 = sceneOff[sc + 1] - 1;
+    const int t0 = v0 / 62, t1 = v1 / 62;
+    const double2 rest = reduce_range(part, t0 + 1, t1 + 1);
+    const double2 first = sceneOf[62 * t0] == sc ? part[t0] : part[tiles + t0];
+    return make_double2(first.x + rest.x, first.y + rest.y);
+}
+
+// Residual check of sweep `sweep` for batched scenes: one wave per scene
+// decides that scene's convergence (the test of residual_wave on the scene's
+// own sums); the last scene to converge ends the substep for the batch.
+__global__ __launch_bounds__(64) void k_residual_scenes(SolverCtl* __restrict__ ctl,
+                                                        SceneCtl* __restrict__ sc,
+                                                        const int32_t* __restrict__ sceneOff,
+                                                        const int32_t* __restrict__ sceneOf,
+                                                        const double2* __restrict__ part,
+                                                        const double2* __restrict__ denomPart,
+                                                        int tiles, int numScenes, int sweep,
+                                                        float tol, int maxSweeps) {
+    const int k = blockIdx.x;
+    if (sc[k].doneAt < sweep) return;
+    const double2 a = scene_sum(part, sceneOff, sceneOf, k, tiles);
+    double denF = sc[k].denomF;
+    if (denF < 0.0) denF = denomPart ? scene_sum(denomPart, sceneOff, sceneOf, k, tiles).y : a.y;
+    if (threadIdx.x != 0) return;
+    sc[k].denomF = denF;
+    const double denU = sc[k].denom;
+    const float rU = denU > 0.0 ? (float)sqrt(a.x / denU) : 0.0f;
+    const float rF = denF > 0.0 ? (float)sqrt(a.y / denF) : 0.0f;
+    const float r = ctl->kind ? rF : rU;
+    if (!(r <= tol || sweep == maxSweeps - 1)) return;
+    sc[k].doneAt = sweep;
+    sc[k].fin = (sweep & 1) ^ 1;
+    sc[k].sweeps += (unsigned long long)(sweep + 1);
+    sc[k].sumFres += (double)rF;
+    // batch statistics: counts and maxima of non-negative floats (as integers)
+    // are independent of the order in which scenes finish
+    if (r > tol) atomicAdd(&ctl->notConverged, 1);
+    atomicMax(reinterpret_cast<int*>(&ctl->maxResid), __float_as_int(r));
+    atomicMax(reinterpret_cast<int*>(&ctl->maxFres), __float_as_int(rF));
+    __threadfence();
+    if (atomicAdd(&ctl->scenesDone, 1) == numScenes - 1) {
+        ctl->doneAt = sweep;
+        ctl->resid = r;
+        ctl->fres = rF;
+        ctl->sweeps += (unsigned long long)(sweep + 1);
+        ctl->substeps += 1ull;
+        if (sweep + 1 > ctl->maxUsed) ctl->maxUsed = sweep + 1;
     }
 }
 
@@ -863,11 +1024,13 @@ __global__ __launch_bounds__(256) void k_end(const DevParams* __restrict__ P,
                                              float4* __restrict__ q0,
                                              const SolverCtl* __restrict__ ctl,
                                              const float4* __restrict__ dxAlt,
-                                             const float4* __restrict__ qAlt) {
+                                             const float4* __restrict__ qAlt, SceneArgs sa) {
     const int n = P->n;
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n) return;
-    if (ctl && ctl->fin == 1) {  // residual-targeted mode: final iterate in buffer 1
+    // residual-targeted mode: final iterate in buffer 1 (per scene in a batch)
+    const int fin = sa.ctl ? sa.ctl[sa.sceneOf[v]].fin : (ctl ? ctl->fin : 0);
+    if (fin == 1) {
         dxFin = dxAlt;
         qFin = qAlt;
     }
@@ -1039,28 +1202,58 @@ void launch_separation(yb_sim* s) {
 // wave tiles of 62 vertices (a 64-lane wave with a one-vertex halo each side)
 int sweep_waves(int n) { return blocks(n, 62); }
 
+// Batched scenes in the residual-targeted schedule: per-scene convergence.
+static bool per_scene(const yb_sim* s, bool adaptive) {
+    return adaptive && s->numScenes > 1 && s->b.sceneCtl;
+}
+SceneArgs scene_args(const yb_sim* s, bool adaptive) {
+    return per_scene(s, adaptive) ? SceneArgs{s->b.sceneOf, s->b.sceneCtl}
+                                  : SceneArgs{nullptr, nullptr};
+}
+// Residual partials of sweep k: half k & 1 of b.partial (one double2 per
+// tile), or of b.scenePart in a batch (two per tile: the tile's scene sides).
+static double2* sweep_partials(yb_sim* s, int sweep, bool batched) {
+    const size_t nbw = (size_t)sweep_waves(s->n);
+    return batched ? s->b.scenePart + 2 * nbw * (sweep & 1) : s->b.partial + nbw * (sweep & 1);
+}
+
 void launch_sweep(yb_sim* s, int sweep, bool adaptive, bool partials, bool checkPrev) {
     const int in = sweep & 1;
     const int out = in ^ 1;
     const int threads = s->sweepThreads;
-    const size_t nbw = (size_t)sweep_waves(s->n);
+    const bool batched = per_scene(s, adaptive);
     // partials of sweep k go to half k & 1, so a sweep can reduce its
     // predecessor's while writing its own
-    double2* part = s->b.partial + nbw * (sweep & 1);
-    const double2* prev = s->b.partial + nbw * ((sweep & 1) ^ 1);
-    hipLaunchKernelGGL(k_sweep, dim3(blocks(sweep_waves(s->n) * 64, threads)), dim3(threads), 0,
-                       s->stream, s->b.params,
-                       s->b.pos, s->b.prm, s->b.yv, s->b.q[in], s->b.q[out], s->b.qRest,
-                       s->b.dx[in], s->b.dx[out], s->b.numCols, s->b.cols,
-                       adaptive ? s->b.ctl : nullptr, sweep, partials ? part : nullptr,
-                       checkPrev ? prev : nullptr, denom_partials(s), s->tol, s->maxSweeps);
+    double2* part = sweep_partials(s, sweep, batched);
+    const double2* prev = sweep_partials(s, sweep ^ 1, batched);
+    const dim3 grid(blocks(sweep_waves(s->n) * 64, threads));
+    if (batched)
+        hipLaunchKernelGGL(k_sweep<true>, grid, dim3(threads), 0, s->stream, s->b.params,
+                           s->b.pos, s->b.prm, s->b.yv, s->b.q[in], s->b.q[out], s->b.qRest,
+                           s->b.dx[in], s->b.dx[out], s->b.numCols, s->b.cols, s->b.ctl, sweep,
+                           partials ? part : nullptr, nullptr, nullptr, s->tol, s->maxSweeps,
+                           scene_args(s, adaptive));
+    else
+        hipLaunchKernelGGL(k_sweep<false>, grid, dim3(threads), 0, s->stream, s->b.params,
+                           s->b.pos, s->b.prm, s->b.yv, s->b.q[in], s->b.q[out], s->b.qRest,
+                           s->b.dx[in], s->b.dx[out], s->b.numCols, s->b.cols,
+                           adaptive ? s->b.ctl : nullptr, sweep, partials ? part : nullptr,
+                           checkPrev ? prev : nullptr, denom_partials(s), s->tol, s->maxSweeps,
+                           SceneArgs{nullptr, nullptr});
 }
 
 void launch_residual(yb_sim* s, int sweep) {
-    const size_t nbw = (size_t)sweep_waves(s->n);
+    const int nbw = sweep_waves(s->n);
+    if (per_scene(s, true)) {
+        hipLaunchKernelGGL(k_residual_scenes, dim3(s->numScenes), dim3(64), 0, s->stream, s->b.ctl,
+                           s->b.sceneCtl, s->b.sceneOff, s->b.sceneOf,
+                           sweep_partials(s, sweep, true), denom_partials(s), nbw, s->numScenes,
+                           sweep, s->tol, s->maxSweeps);
+        return;
+    }
     hipLaunchKernelGGL(k_residual, dim3(1), dim3(64), 0, s->stream, s->b.ctl,
-                       s->b.partial + nbw * (sweep & 1), denom_partials(s), (int)nbw, sweep,
-                       s->tol, s->maxSweeps);
+                       sweep_partials(s, sweep, false), denom_partials(s), nbw, sweep, s->tol,
+                       s->maxSweeps);
 }
 
 void launch_frame_refresh(yb_sim* s) {
@@ -1073,13 +1266,20 @@ void launch_frame_refresh(yb_sim* s) {
 void launch_strand_solve(yb_sim* s, int buf, SolverCtl* ctl, int sweep, float* sys) {
     const int tiles = sweep_waves(s->n);
     const LineOut lo{s->b.lineG, s->b.lineQL, sys};
-    hipLaunchKernelGGL(k_line_tile, dim3(blocks(tiles * 64, 256)), dim3(256), 0, s->stream,
-                       s->b.params, s->b.pos, s->b.prm, s->b.yv, s->b.q[buf], s->b.qRest,
-                       s->b.dx[buf], s->b.numCols, s->b.cols, lo, ctl, sweep);
+    // in a batch, converged scenes are left alone (before sweep `sweep`)
+    const SceneArgs sa = scene_args(s, ctl != nullptr);
+    if (sa.ctl)
+        hipLaunchKernelGGL(k_line_tile<true>, dim3(blocks(tiles * 64, 256)), dim3(256), 0,
+                           s->stream, s->b.params, s->b.pos, s->b.prm, s->b.yv, s->b.q[buf],
+                           s->b.qRest, s->b.dx[buf], s->b.numCols, s->b.cols, lo, ctl, sweep, sa);
+    else
+        hipLaunchKernelGGL(k_line_tile<false>, dim3(blocks(tiles * 64, 256)), dim3(256), 0,
+                           s->stream, s->b.params, s->b.pos, s->b.prm, s->b.yv, s->b.q[buf],
+                           s->b.qRest, s->b.dx[buf], s->b.numCols, s->b.cols, lo, ctl, sweep, sa);
     if (s->strandIters <= 2) {
         hipLaunchKernelGGL(k_line_pairs_apply, dim3(blocks(tiles * 64, 256)), dim3(256), 0,
                            s->stream, s->n, tiles, s->strandIters, s->b.lineG, s->b.lineQL,
-                           s->b.dx[buf], ctl, sweep);
+                           s->b.dx[buf], ctl, sweep, sa);
         return;
     }
     int cur = 0;
@@ -1092,31 +1292,39 @@ void launch_strand_solve(yb_sim* s, int buf, SolverCtl* ctl, int sweep, float* s
     }
     hipLaunchKernelGGL(k_line_apply, dim3(blocks(s->n, 256)), dim3(256), 0, s->stream, s->n, tiles,
                        s->b.lineG, s->b.lineQL, s->b.lineE[cur], s->b.lineF[cur], s->b.dx[buf], ctl,
-                       sweep);
+                       sweep, sa);
 }
 
 void launch_ctl_reset(yb_sim* s) {
-    hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(64), 0, s->stream, s->b.ctl);
+    const bool batched = per_scene(s, true);
+    hipLaunchKernelGGL(k_ctl_reset, dim3(batched ? blocks(s->numScenes, 256) : 1),
+                       dim3(batched ? 256 : 64), 0, s->stream, s->b.ctl,
+                       batched ? s->b.sceneCtl : nullptr, s->numScenes);
 }
 
-// The Newton residual's denominator from the plain warm start in dx[0]: a
-// force evaluation, then the reduction.  The partials use the second half of
-// the buffer; sweep 0 writes the first.
-// The Newton-residual denominator's partials: the third part of b.partial,
-// written by the prologue's force evaluation in sessions whose sweeps do not
-// start from the plain warm start; null in plain sessions.
+// The Newton-residual denominator's partials: the third part of b.partial
+// (of b.scenePart in a batch), written by the prologue's force evaluation in
+// sessions whose sweeps do not start from the plain warm start; null in
+// plain sessions.
 const double2* denom_partials(const yb_sim* s) {
-    return (s->warmStart != 0 || s->strandSolve)
-               ? s->b.partial + 2 * (size_t)sweep_waves(s->n)
-               : nullptr;
+    if (!(s->warmStart != 0 || s->strandSolve)) return nullptr;
+    const size_t nbw = (size_t)sweep_waves(s->n);
+    return per_scene(s, true) ? s->b.scenePart + 4 * nbw : s->b.partial + 2 * nbw;
 }
 
 void launch_residual_denom(yb_sim* s, bool frames) {
-    double2* part = s->b.partial + 2 * (size_t)sweep_waves(s->n);
-    hipLaunchKernelGGL(k_force_eval, dim3(blocks(sweep_waves(s->n) * 64, 256)), dim3(256), 0,
-                       s->stream, s->b.params, s->b.pos, s->b.prm, s->b.yv, s->b.q[0],
-                       frames ? s->b.q[1] : nullptr, s->b.qRest, s->b.dx[0], s->b.numCols,
-                       s->b.cols, part);
+    double2* part = const_cast<double2*>(denom_partials(s));
+    const dim3 grid(blocks(sweep_waves(s->n) * 64, 256));
+    if (per_scene(s, true))
+        hipLaunchKernelGGL(k_force_eval<true>, grid, dim3(256), 0, s->stream, s->b.params,
+                           s->b.pos, s->b.prm, s->b.yv, s->b.q[0], frames ? s->b.q[1] : nullptr,
+                           s->b.qRest, s->b.dx[0], s->b.numCols, s->b.cols, part,
+                           scene_args(s, true));
+    else
+        hipLaunchKernelGGL(k_force_eval<false>, grid, dim3(256), 0, s->stream, s->b.params,
+                           s->b.pos, s->b.prm, s->b.yv, s->b.q[0], frames ? s->b.q[1] : nullptr,
+                           s->b.qRest, s->b.dx[0], s->b.numCols, s->b.cols, part,
+                           SceneArgs{nullptr, nullptr});
     if (frames)
         hipLaunchKernelGGL(k_copy4, dim3(blocks(s->n, 256)), dim3(256), 0, s->stream, s->b.q[1],
                            s->b.q[0], s->n);
@@ -1125,7 +1333,8 @@ void launch_residual_denom(yb_sim* s, bool frames) {
 void launch_end(yb_sim* s, int fin, bool adaptive) {
     hipLaunchKernelGGL(k_end, dim3(blocks(s->n, 256)), dim3(256), 0, s->stream, s->b.params,
                        s->b.pos, s->b.vel, s->b.dx[fin], s->b.q[fin], s->b.q[0],
-                       adaptive ? s->b.ctl : nullptr, s->b.dx[1], s->b.q[1]);
+                       adaptive ? s->b.ctl : nullptr, s->b.dx[1], s->b.q[1],
+                       scene_args(s, adaptive));
 }
 
 }  // namespace yb

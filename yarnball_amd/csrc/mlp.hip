@@ -621,11 +621,14 @@ __global__ __launch_bounds__(256) void k_correction(const DevParams* __restrict_
                                                     float4* __restrict__ corrPrev,
                                                     float* __restrict__ recT,
                                                     const int64_t* __restrict__ recCount,
-                                                    int64_t recCap) {
+                                                    int64_t recCap, SceneArgs sa) {
     const int n = P->n;
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n) return;
-    if (ctl) fin = ctl->fin;
+    if (sa.ctl)
+        fin = sa.ctl[sa.sceneOf[v]].fin;  // a batch: the vertex's scene's final buffer
+    else if (ctl)
+        fin = ctl->fin;
     const float4 d = (fin ? dxB : dxA)[v];
     const float4 p = dxPlain[v];
     const uint32_t fl = __float_as_uint(prm[v].w);
@@ -893,7 +896,7 @@ void launch_correction(yb_sim* s, int fin, bool adaptive, bool record) {
     hipLaunchKernelGGL(k_correction, dim3(nblk(s->n, 256)), dim3(256), 0, s->stream, s->b.params,
                        s->b.prm, s->b.dx[0], s->b.dx[1], adaptive ? s->b.ctl : nullptr, fin,
                        s->b.dxPlain, s->b.corrPrev, record ? s->b.recT : nullptr, s->b.recCount,
-                       s->recCap);
+                       s->recCap, scene_args(s, adaptive));
     if (record)
         hipLaunchKernelGGL(k_rec_advance, dim3(1), dim3(64), 0, s->stream, s->b.recCount, s->n,
                            s->recCap);

@@ -53,8 +53,30 @@ struct SolverCtl {
     float fres;                   // Newton residual of the last evaluated sweep
     float maxFres;                // largest final Newton residual
     int32_t kind;                 // stopping test: 0 update residual, 1 Newton residual
-    int32_t pad_;
+    int32_t scenesDone;           // batched scenes: scenes converged in this substep
 };
+
+// Per-scene solver state of a batch in the residual-targeted schedule: each
+// scene stops at its own sweep and keeps its final iterate in its own buffer.
+struct SceneCtl {
+    int32_t doneAt;  // sweep at which the scene converged (INT_MAX: running)
+    int32_t fin;     // buffer (0/1) holding the scene's final iterate
+    double denom;    // update residual: sum over the scene's free vertices of l_i^2
+    double denomF;   // Newton residual: |f(dx_0)|^2 of the scene (< 0: unknown)
+    double sumFres;  // final Newton residuals summed over substeps
+    unsigned long long sweeps;  // the scene's own sweeps summed over substeps
+};
+// Per-scene arguments of the kernels that honour per-scene convergence (both
+// null outside a residual-targeted batch).
+struct SceneArgs {
+    const int32_t* sceneOf;  // scene of each vertex
+    SceneCtl* ctl;           // per-scene solver state
+};
+
+// Residual-targeted batches need scenes of at least this many vertices: a
+// 62-vertex wave tile then holds at most two scenes, whose residual partials
+// it keeps apart.
+constexpr int kMinSceneForResidual = 64;
 
 constexpr int kMaxCols = YB_MAX_COLLISIONS_PER_SEGMENT;
 // Bits of the device error word err[0] (one per failure kind).
@@ -143,6 +165,8 @@ struct Buffers {
     float4* sceneBoxes = nullptr;
     int32_t* sceneRoot = nullptr;
     int32_t* sceneEndFlags = nullptr;
+    SceneCtl* sceneCtl = nullptr;  // per-scene solver state (residual-targeted batches)
+    double2* scenePart = nullptr;  // 3 parts x 2 sides x tiles: residual partials per scene
     float4* drvInit = nullptr;  // scenario driver: initial position xyz, initial inverse mass
     float4* blockBox = nullptr;   // 2 per reduction block
     uint32_t* morton = nullptr;   // n, indexed by segment
@@ -208,6 +232,7 @@ struct yb_sim {
     bool uploaded = false;
     int numScenes = 1;
     std::vector<int32_t> sceneOffHost;  // first vertex of each scene, + n
+    std::vector<double> lsqHost;  // per vertex: l_i^2 of a free vertex, 0 otherwise (upload)
     float lastBVHRebuild = INFINITY;  // sim time since the last full rebuild
     int64_t stepCounter = 0;
     // captured substep graphs: [detect mode 0=none,1=refit,2=rebuild]
@@ -294,6 +319,7 @@ void launch_strand_solve(yb_sim* s, int buf = 0, SolverCtl* ctl = nullptr, int s
 void launch_residual_denom(yb_sim* s, bool frames);
 const double2* denom_partials(const yb_sim* s);
 void launch_end(yb_sim* s, int fin, bool adaptive);         // iteration.cu:48-61
+SceneArgs scene_args(const yb_sim* s, bool adaptive);
 int sweep_waves(int n);
 // learned warm start (mlp.hip)
 void launch_mlp_pack(yb_sim* s);

@@ -173,12 +173,15 @@ class Sim:
 
     def solverStats(self, reset=False):
         """Counters of the residual-targeted mode since the last reset;
-        meanNewtonResidual is the mean over substeps of the final |f| / |f0|."""
+        meanNewtonResidual is the mean over substeps (scene-substeps in a
+        batch) of the final |f| / |f0|; meanSceneSweeps the mean sweeps a
+        scene ran per substep (= sweeps per substep without batching)."""
         st = _lib.SolverStats()
         self._need_handle()
         check(self._lib.yb_solver_stats(self._h, ctypes.byref(st), int(bool(reset))))
         out = {f: getattr(st, f) for f, _ in st._fields_}
-        out["meanNewtonResidual"] = out["sumNewtonResidual"] / max(out["substeps"], 1)
+        out["meanNewtonResidual"] = out["sumNewtonResidual"] / max(out["sceneSubsteps"], 1)
+        out["meanSceneSweeps"] = out["sceneSweeps"] / max(out["sceneSubsteps"], 1)
         return out
 
     def setScenes(self, offsets):
