@@ -186,6 +186,51 @@ def measure_batched(args, steps, warmup, per_scene_steps, with_cpu):
                   file=sys.stderr)
     b.close()
 
+    # The same batch in the residual-targeted schedule with the strand solve:
+    # every scene stops at its own sweep (per-scene convergence), timed over a
+    # few substeps after one untimed substep.
+    res = None
+    if args.batched_residual > 0 and args.batched_residual_substeps > 0:
+        from yarnball_amd import learned
+        r = Sim(big.n)
+        r.printErrors = False
+        scenes.load_into(r, big)
+        r.setScenes(offs)
+        r.setSolver(args.batched_residual, maxSweeps=args.max_sweeps, chunk=args.chunk,
+                    checkEvery=args.check_every)
+        learned.strand(r, every=args.strand_every, first=args.strand_first)
+        if args.no_graphs:
+            r.setGraphs(False)
+        if launch_config:
+            r.setLaunchConfig(*launch_config)
+        if tile_config:
+            r.setTileConfig(*tile_config)
+        h = np.float32(big.maxH)
+        r.advance(h)
+        r.solverStats(reset=True)
+        nsub = args.batched_residual_substeps
+        t0 = time.perf_counter()
+        for k in range(nsub):
+            r.advance(h)
+            print(f"[bench] batched residual: substep {k} done", file=sys.stderr, flush=True)
+        el_r = time.perf_counter() - t0
+        st = r.solverStats()
+        r.close()
+        res = {"residual_tol": args.batched_residual,
+               "residual_kind": "update (RMS sweep change / rest length), per scene",
+               "method": f"plain warm start + strand solve (after the warm start and before "
+                         f"sweeps {args.strand_first}, {args.strand_first}+{args.strand_every}k)",
+               "substeps": nsub, "value": round(big.n * nsub / el_r, 1),
+               "ms_per_substep": round(el_r / nsub * 1e3, 3),
+               "batch_sweeps_per_substep": round(st["sweeps"] / max(st["substeps"], 1), 1),
+               "scene_sweeps_per_substep": round(st["meanSceneSweeps"], 1),
+               "not_converged_scene_substeps": int(st["notConverged"]),
+               "newton_residual_mean": float(st["meanNewtonResidual"])}
+        print(f"[bench] batched residual {args.batched_residual:g}: {res['value']:.4e} "
+              f"vertex-substeps/s, batch {res['batch_sweeps_per_substep']} sweeps per substep, "
+              f"scenes {res['scene_sweeps_per_substep']}, not converged "
+              f"{res['not_converged_scene_substeps']}", file=sys.stderr, flush=True)
+
     per = None
     if not args.no_per_scene and per_scene_steps > 0:
         sims = []
@@ -270,6 +315,7 @@ def measure_batched(args, steps, warmup, per_scene_steps, with_cpu):
                           "query_shape": list((tile_config or (1024, 8, 6))[1:]),
                           "autotune": tuned},
         "batched_vs_per_scene": round(value / per["value"], 3) if per else None,
+        "residual_schedule": res,
         "roofline": {"kernel": "sweep", "bound": "hbm",
                      "achieved": round(alg / (sweep_us * 1e-6) / 1e9, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(alg / (sweep_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
@@ -433,6 +479,11 @@ def main():
                     help="batched_walks: skip the one-handle-per-scene comparison")
     ap.add_argument("--no-batched", action="store_true",
                     help="default workload: skip the batched-scenes record")
+    ap.add_argument("--batched-residual", type=float, default=1e-5,
+                    help="batched scenes: residual target of the per-scene convergence record "
+                         "(<= 0 skips it)")
+    ap.add_argument("--batched-residual-substeps", type=int, default=4,
+                    help="batched scenes: timed substeps of the per-scene convergence record")
     ap.add_argument("--launch-config", type=int, nargs=2, default=None,
                     metavar=("SWEEP_THREADS", "QUERY_THREADS"),
                     help="fixed block sizes (skips the autotuner)")
