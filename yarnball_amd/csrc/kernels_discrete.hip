@@ -924,47 +924,42 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
         const bool hr = overlap(qlo, qhi, r0, r1);
         // leaves within two indices of i (itself and its strand neighbours,
         // which always overlap) are exempt from the pair test (collision.cu:54):
-        // they are dropped here, before their positions are gathered
-        if (hl && cl < 0 && abs(~cl - i) > 2) {
-            candJ[nb][t] = ~cl;
-            candPad[nb][t] = l1.w;
-            nb++;
-        }
-        if (hr && cr < 0 && abs(~cr - i) > 2) {
-            candJ[nb][t] = ~cr;
-            candPad[nb][t] = r1.w;
-            nb++;
-        }
+        // they are dropped here, before their positions are gathered.  Both
+        // leaves are written without a branch: slot nb and the one after it
+        // are free (nb <= kCand - 2 at every visit), and nb advances only for
+        // an accepted leaf.
+        const bool addL = hl && cl < 0 && abs(~cl - i) > 2;
+        const bool addR = hr && cr < 0 && abs(~cr - i) > 2;
+        candJ[nb][t] = ~cl;
+        candPad[nb][t] = l1.w;
+        nb += addL ? 1 : 0;
+        candJ[nb][t] = ~cr;
+        candPad[nb][t] = r1.w;
+        nb += addR ? 1 : 0;
         // at most two leaves per visit: flush before a buffer could overflow
         if (__any(nb > kCand - 2)) narrowphase();
         const bool tl = hl && cl >= 0;
         const bool tr = hr && cr >= 0;
-        if (tl || tr) {
-            node = tl ? cl : cr;
-            if (tl && tr) {
-                // stackLimit <= kStack (host-checked); tests lower it to force
-                // the overflow path
-                if (sp >= stackLimit) {
-                    overflowStack = true;
-                } else if (sp < kStackL) {
-                    stk[sp][t] = cr;
-                    sp++;
-                } else {
-                    deep[sp - kStackL] = cr;
-                    sp++;
-                }
-            }
-        } else {
-            if (sp == 0) break;
-            sp--;
-            if (sp < kStackL) {
-                node = stk[sp][t];
-            } else {
-                // keeps the compiler from merging the two pops into one generic
-                // (flat) load, which would make every pop a slow volatile one
-                asm volatile("" ::: "memory");
-                node = deep[sp - kStackL];
-            }
+        const bool descend = tl || tr;
+        if (!descend && sp == 0) break;
+        // Next node: descend left first (the right child is pushed when both
+        // overlap), else pop.  stackLimit <= kStack (host-checked); tests lower
+        // it to force the overflow path.  The LDS part of the stack is used
+        // without branches: the slot above the top is free, so the write is
+        // harmless when nothing is pushed, and the pop reads the top entry.
+        const bool both = tl && tr;
+        const bool push = both && sp < stackLimit;
+        overflowStack = overflowStack || (both && !push);
+        if (sp < kStackL) stk[sp][t] = cr;
+        const int popped = stk[min(max(sp - 1, 0), kStackL - 1)][t];
+        if (push && sp >= kStackL) deep[sp - kStackL] = cr;
+        node = descend ? (tl ? cl : cr) : popped;
+        sp = descend ? sp + (push ? 1 : 0) : sp - 1;
+        if (!descend && sp >= kStackL) {
+            // keeps the compiler from merging the two pops into one generic
+            // (flat) load, which would make every pop a slow volatile one
+            asm volatile("" ::: "memory");
+            node = deep[sp - kStackL];
         }
     }
     narrowphase();
