@@ -163,6 +163,13 @@ extern "C" int yb_create(int n, yb_sim** out) {
     }
     yb_sim* s = new yb_sim();
     s->n = n;
+    {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+            cus > 0)
+            s->numCUs = cus;
+    }
     s->sortTiles = (n + kSortTile - 1) / kSortTile;
     s->boxBlocks = (n + 255) / 256;
     if (s->boxBlocks > 1024) s->boxBlocks = 1024;
@@ -418,12 +425,14 @@ static int reset_scene_ctl(yb_sim* s) {
 static int clear_scenes(yb_sim* s) {
     Buffers& b = s->b;
     for (void* p : {(void*)b.sceneOf, (void*)b.sceneOff, (void*)b.sceneBoxes, (void*)b.sceneRoot,
-                    (void*)b.sceneEndFlags, (void*)b.sceneCtl, (void*)b.scenePart})
+                    (void*)b.sceneEndFlags, (void*)b.sceneCtl, (void*)b.scenePart,
+                    (void*)b.tileList, (void*)b.tileCount})
         if (p) HIPCHECK(hipFree(p));
     b.sceneOf = b.sceneOff = b.sceneRoot = b.sceneEndFlags = nullptr;
     b.sceneBoxes = nullptr;
     b.sceneCtl = nullptr;
     b.scenePart = nullptr;
+    b.tileList = b.tileCount = nullptr;
     s->numScenes = 1;
     s->sceneOffHost.clear();
     s->treeBuilt = false;
@@ -462,6 +471,8 @@ extern "C" int yb_set_scenes(yb_sim* s, int numScenes, const int32_t* offsets) {
     HIPCHECK(hipMalloc(&b.sceneEndFlags, sizeof(int32_t) * numScenes));
     HIPCHECK(hipMalloc(&b.sceneCtl, sizeof(SceneCtl) * numScenes));
     HIPCHECK(hipMalloc(&b.scenePart, sizeof(double2) * 6 * (size_t)sweep_waves(s->n)));
+    HIPCHECK(hipMalloc(&b.tileList, sizeof(int32_t) * (size_t)sweep_waves(s->n)));
+    HIPCHECK(hipMalloc(&b.tileCount, sizeof(int32_t) * (numScenes + 1)));
     std::vector<int32_t> of(N);
     for (int k = 0; k < numScenes; k++)
         for (int v = offsets[k]; v < offsets[k + 1]; v++) of[v] = k;
@@ -689,6 +700,12 @@ static void enqueue_prologue(yb_sim* s, int mode, bool adaptive, int warm) {
 static bool checks_residual(const yb_sim* s, int k) {
     const bool plainStart = s->warmStart == 0 && !s->strandSolve;
     return (k == 0 && plainStart) || (k + 1) % s->checkEvery == 0 || k == s->maxSweeps - 1;
+}
+
+int yb::first_check(const yb_sim* s) {
+    for (int k = 0; k < s->maxSweeps; k++)
+        if (checks_residual(s, k)) return k;
+    return s->maxSweeps;
 }
 
 // The residual of a check sweep is reduced, and convergence decided, by the

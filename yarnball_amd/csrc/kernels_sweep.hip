@@ -241,6 +241,20 @@ __device__ __forceinline__ void wave_residual_scenes(float numU, float numF, boo
     }
 }
 
+// Tile-list kernels of a residual-targeted batch (after its first check): a
+// fixed grid of waves loops over the listed tiles -- those of the scenes
+// still running -- so that sweeps late in a substep, when few scenes remain,
+// neither dispatch nor retire a wave per tile of the whole batch.  tileCount
+// < 0 lists every tile.
+__device__ __forceinline__ int listed_tiles(const int32_t* __restrict__ tileCount, int tiles) {
+    const int c = tileCount[0];
+    return c < 0 ? tiles : c;
+}
+__device__ __forceinline__ int listed_tile(const int32_t* __restrict__ tileList,
+                                           const int32_t* __restrict__ tileCount, int idx) {
+    return tileCount[0] < 0 ? idx : tileList[idx];
+}
+
 template <int kMode, bool kBatched = false>
 __device__ __forceinline__ void sweep_body(const DevParams* __restrict__ P,
                                            const float4* __restrict__ pos,
@@ -258,7 +272,8 @@ __device__ __forceinline__ void sweep_body(const DevParams* __restrict__ P,
                                            const double2* __restrict__ partialPrev,
                                            const double2* __restrict__ denomPart,
                                            float tol, int maxSweeps, LineOut lo,
-                                           SceneArgs sa = SceneArgs{nullptr, nullptr}) {
+                                           SceneArgs sa = SceneArgs{nullptr, nullptr},
+                                           int tileIn = -1) {
     // Residual-targeted mode: sweeps after convergence do nothing.
     if (ctl && ctl->doneAt < sweep) return;
     const int n = P->n;
@@ -270,8 +285,10 @@ __device__ __forceinline__ void sweep_body(const DevParams* __restrict__ P,
     if (!kBatched && partialPrev && blockIdx.x == 0 && threadIdx.x < 64)
         residual_wave(ctl, partialPrev, denomPart, (n + 61) / 62, sweep - 1, tol, maxSweeps);
     const int lane = threadIdx.x & 63;
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
-    const int wave = (int)((tile * blockDim.x + threadIdx.x) >> 6);
+    // the wave tile: given by a tile-list kernel, else from the block index
+    const int wave = tileIn >= 0 ? tileIn
+                                 : (int)((xcd_tile(blockIdx.x, gridDim.x) * blockDim.x +
+                                          threadIdx.x) >> 6);
     const int v = wave * 62 + lane - 1;
     const bool in = (v >= 0) && (v < n);
 
@@ -612,6 +629,107 @@ __global__ __launch_bounds__(256) void k_line_tile(const DevParams* __restrict__
                                 cols, nullptr, sweep, nullptr, nullptr, nullptr, 0.0f, 0, lo, sa);
 }
 
+__global__ __launch_bounds__(256) void k_sweep_list(
+    const DevParams* __restrict__ P, const float4* __restrict__ pos,
+    const float4* __restrict__ prm, const float4* __restrict__ yv,
+    const float4* __restrict__ qIn, float4* __restrict__ qOut, const float4* __restrict__ qRest,
+    const float4* __restrict__ dxIn, float4* __restrict__ dxOut,
+    const int32_t* __restrict__ numCols, const int32_t* __restrict__ cols,
+    SolverCtl* __restrict__ ctl, int sweep, double2* __restrict__ partial, SceneArgs sa,
+    const int32_t* __restrict__ tileList, const int32_t* __restrict__ tileCount) {
+    if (ctl->doneAt < sweep) return;
+    const int tiles = (P->n + 61) / 62;
+    const int count = listed_tiles(tileCount, tiles);
+    const int waves = (int)(gridDim.x * blockDim.x >> 6);
+    for (int k = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6); k < count; k += waves)
+        sweep_body<kSweep, true>(P, pos, prm, yv, qIn, qOut, qRest, dxIn, dxOut, numCols, cols,
+                                 ctl, sweep, partial, nullptr, nullptr, 0.0f, 0,
+                                 LineOut{nullptr, nullptr, nullptr}, sa,
+                                 listed_tile(tileList, tileCount, k));
+}
+
+__global__ __launch_bounds__(256) void k_line_tile_list(
+    const DevParams* __restrict__ P, const float4* __restrict__ pos,
+    const float4* __restrict__ prm, const float4* __restrict__ yv,
+    const float4* __restrict__ qIn, const float4* __restrict__ qRest,
+    const float4* __restrict__ dxIn, const int32_t* __restrict__ numCols,
+    const int32_t* __restrict__ cols, LineOut lo, const SolverCtl* __restrict__ ctl, int sweep,
+    SceneArgs sa, const int32_t* __restrict__ tileList, const int32_t* __restrict__ tileCount) {
+    if (strand_skip(ctl, sweep)) return;
+    const int tiles = (P->n + 61) / 62;
+    const int count = listed_tiles(tileCount, tiles);
+    const int waves = (int)(gridDim.x * blockDim.x >> 6);
+    for (int k = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6); k < count; k += waves)
+        sweep_body<kLine, true>(P, pos, prm, yv, qIn, nullptr, qRest, dxIn, nullptr, numCols,
+                                cols, nullptr, sweep, nullptr, nullptr, nullptr, 0.0f, 0, lo, sa,
+                                listed_tile(tileList, tileCount, k));
+}
+
+// The tile list after a check of a batch: the tiles of every scene that has
+// not converged, in scene order.  A tile shared by two scenes is listed once,
+// by the first of them that is still running.  k_tile_scan (one block)
+// counts each scene's entries and sums them; k_tile_fill (a block per scene)
+// writes them.
+constexpr int kTileScanThreads = 1024;
+__device__ __forceinline__ void scene_tiles(const SceneCtl* __restrict__ sc,
+                                            const int32_t* __restrict__ sceneOff,
+                                            const int32_t* __restrict__ sceneOf, int k,
+                                            int& first, int& count) {
+    const int t0 = sceneOff[k] / 62, t1 = (sceneOff[k + 1] - 1) / 62;
+    first = t0;
+    count = 0;
+    if (sc[k].doneAt != 0x7fffffff) return;  // converged
+    // the first tile starts in scene k - 1: that scene lists it if still running
+    if (k > 0 && sceneOf[62 * t0] != k && sc[k - 1].doneAt == 0x7fffffff) first = t0 + 1;
+    count = t1 - first + 1;
+}
+
+__global__ __launch_bounds__(kTileScanThreads) void k_tile_scan(const SceneCtl* __restrict__ sc,
+                                                            const int32_t* __restrict__ sceneOff,
+                                                            const int32_t* __restrict__ sceneOf,
+                                                            int numScenes,
+                                                            int32_t* __restrict__ tileCount) {
+    __shared__ int part[kTileScanThreads];
+    const int t = threadIdx.x;
+    const int per = (numScenes + kTileScanThreads - 1) / kTileScanThreads;
+    const int k0 = t * per, k1 = min(k0 + per, numScenes);
+    int sum = 0;
+    for (int k = k0; k < k1; k++) {
+        int first, count;
+        scene_tiles(sc, sceneOff, sceneOf, k, first, count);
+        sum += count;
+    }
+    part[t] = sum;
+    __syncthreads();
+    // inclusive scan of the per-thread sums (Hillis-Steele)
+    for (int off = 1; off < kTileScanThreads; off <<= 1) {
+        const int add = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += add;
+        __syncthreads();
+    }
+    int at = part[t] - sum;
+    for (int k = k0; k < k1; k++) {
+        int first, count;
+        scene_tiles(sc, sceneOff, sceneOf, k, first, count);
+        tileCount[1 + k] = at;
+        at += count;
+    }
+    if (t == kTileScanThreads - 1) tileCount[0] = part[t];
+}
+
+__global__ __launch_bounds__(64) void k_tile_fill(const SceneCtl* __restrict__ sc,
+                                                  const int32_t* __restrict__ sceneOff,
+                                                  const int32_t* __restrict__ sceneOf,
+                                                  const int32_t* __restrict__ tileCount,
+                                                  int32_t* __restrict__ tileList) {
+    const int k = blockIdx.x;
+    int first, count;
+    scene_tiles(sc, sceneOff, sceneOf, k, first, count);
+    const int at = tileCount[1 + k];
+    for (int j = threadIdx.x; j < count; j += 64) tileList[at + j] = first + j;
+}
+
 // Residual partials of an iterate without changing it: the denominator |f(dx0)|
 // of the relative residual when the sweeps do not start from the plain warm
 // start.  With qOut set, also the frames refreshed from that iterate (the
@@ -683,16 +801,12 @@ __global__ void k_line_pairs(const float4* __restrict__ g, const float2* __restr
 // w-2 .. w+1 (the outer values start at zero); with two iterations, lanes 1
 // and 2 solve boundaries w-1 and w again from their neighbours' results.  The
 // same arithmetic as k_line_pairs, so the result equals that path's.
-__global__ __launch_bounds__(256) void k_line_pairs_apply(int n, int tiles, int iters,
-                                                          const float4* __restrict__ g,
-                                                          const float2* __restrict__ ql,
-                                                          float4* __restrict__ dx,
-                                                          const SolverCtl* __restrict__ ctl,
-                                                          int sweep, SceneArgs sa) {
-    if (strand_skip(ctl, sweep)) return;
+__device__ __forceinline__ void pairs_apply_tile(int w, int n, int tiles, int iters,
+                                                 const float4* __restrict__ g,
+                                                 const float2* __restrict__ ql,
+                                                 float4* __restrict__ dx, int sweep,
+                                                 SceneArgs sa) {
     const int lane = threadIdx.x & 63;
-    const int w = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6);
-    if (w >= tiles) return;
     // The apply's own rows are loaded first, so that their latency overlaps
     // the pair solves (which depend only on the boundary rows).
     const int v = 62 * w + lane - 1;
@@ -735,6 +849,30 @@ __global__ __launch_bounds__(256) void k_line_pairs_apply(int n, int tiles, int 
     const float l = len(nd);
     if (l > QL.y && l > 0.0f) nd *= QL.y / l;
     dx[v] = f4(nd, d4.w);
+}
+
+__global__ __launch_bounds__(256) void k_line_pairs_apply(int n, int tiles, int iters,
+                                                          const float4* __restrict__ g,
+                                                          const float2* __restrict__ ql,
+                                                          float4* __restrict__ dx,
+                                                          const SolverCtl* __restrict__ ctl,
+                                                          int sweep, SceneArgs sa) {
+    if (strand_skip(ctl, sweep)) return;
+    const int w = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6);
+    if (w >= tiles) return;
+    pairs_apply_tile(w, n, tiles, iters, g, ql, dx, sweep, sa);
+}
+
+__global__ __launch_bounds__(256) void k_line_pairs_apply_list(
+    int n, int tiles, int iters, const float4* __restrict__ g, const float2* __restrict__ ql,
+    float4* __restrict__ dx, const SolverCtl* __restrict__ ctl, int sweep, SceneArgs sa,
+    const int32_t* __restrict__ tileList, const int32_t* __restrict__ tileCount) {
+    if (strand_skip(ctl, sweep)) return;
+    const int count = listed_tiles(tileCount, tiles);
+    const int waves = (int)(gridDim.x * blockDim.x >> 6);
+    for (int k = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6); k < count; k += waves)
+        pairs_apply_tile(listed_tile(tileList, tileCount, k), n, tiles, iters, g, ql, dx, sweep,
+                         sa);
 }
 
 // Strand solve, apply: x = g + p E_(w-1) + q F_(w+1) for every vertex of tile
@@ -866,12 +1004,13 @@ __global__ __launch_bounds__(64) void k_residual(SolverCtl* __restrict__ ctl,
 }
 
 __global__ void k_ctl_reset(SolverCtl* __restrict__ ctl, SceneCtl* __restrict__ sceneCtl,
-                            int numScenes) {
+                            int numScenes, int32_t* __restrict__ tileCount) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k == 0) {
         ctl->doneAt = 0x7fffffff;
         ctl->denomF = -1.0;
         ctl->scenesDone = 0;
+        if (tileCount) tileCount[0] = -1;  // every tile until the first check
     }
     if (sceneCtl && k < numScenes) {
         sceneCtl[k].doneAt = 0x7fffffff;
@@ -911,22 +1050,40 @@ __device__ __forceinline__ double2 reduce_range(const double2* __restrict__ p, i
 
 // Sum over scene sc of per-scene-side tile partials (wave_residual_scenes):
 // its first tile contributes the side that holds it, every later tile its
-// side A.
+// side A.  A block of kSceneThreads: each wave sums a fixed quarter of the
+// tiles (reduce_range), and the quarters are added in order, so the result
+// does not depend on scheduling.  Every thread returns the sum.
+constexpr int kSceneThreads = 256;
 __device__ __forceinline__ double2 scene_sum(const double2* __restrict__ part,
                                              const int32_t* __restrict__ sceneOff,
                                              const int32_t* __restrict__ sceneOf, int sc,
                                              int tiles) {
+    __shared__ double2 waveSum[kSceneThreads / 64];
     const int v0 = sceneOff[sc], v1 = sceneOff[sc + 1] - 1;
     const int t0 = v0 / 62, t1 = v1 / 62;
-    const double2 rest = reduce_range(part, t0 + 1, t1 + 1);
+    constexpr int kWaves = kSceneThreads / 64;
+    const int w = threadIdx.x >> 6;
+    const int len = t1 - t0;  // tiles t0 + 1 .. t1
+    const int a = t0 + 1 + (int)((long long)len * w / kWaves);
+    const int b = t0 + 1 + (int)((long long)len * (w + 1) / kWaves);
+    const double2 q = reduce_range(part, a, b);
+    __syncthreads();  // a previous call's sums have been read
+    if ((threadIdx.x & 63) == 0) waveSum[w] = q;
+    __syncthreads();
     const double2 first = sceneOf[62 * t0] == sc ? part[t0] : part[tiles + t0];
-    return make_double2(first.x + rest.x, first.y + rest.y);
+    double x = first.x, y = first.y;
+#pragma unroll
+    for (int k = 0; k < kWaves; k++) {
+        x += waveSum[k].x;
+        y += waveSum[k].y;
+    }
+    return make_double2(x, y);
 }
 
 // Residual check of sweep `sweep` for batched scenes: one wave per scene
 // decides that scene's convergence (the test of residual_wave on the scene's
 // own sums); the last scene to converge ends the substep for the batch.
-__global__ __launch_bounds__(64) void k_residual_scenes(SolverCtl* __restrict__ ctl,
+__global__ __launch_bounds__(kSceneThreads) void k_residual_scenes(SolverCtl* __restrict__ ctl,
                                                         SceneCtl* __restrict__ sc,
                                                         const int32_t* __restrict__ sceneOff,
                                                         const int32_t* __restrict__ sceneOf,
@@ -1217,6 +1374,18 @@ static double2* sweep_partials(yb_sim* s, int sweep, bool batched) {
     return batched ? s->b.scenePart + 2 * nbw * (sweep & 1) : s->b.partial + nbw * (sweep & 1);
 }
 
+// Grid of a tile-list kernel: enough 4-wave blocks to fill the device, at
+// most one wave per tile.
+static int list_blocks(const yb_sim* s) {
+    const int waves = std::min(sweep_waves(s->n), s->numCUs * 24);
+    return (waves + 3) / 4;
+}
+// In a batch, the sweeps and strand steps after the substep's first check run
+// over the tile list.
+static bool tile_list_mode(const yb_sim* s, bool adaptive, int sweep) {
+    return per_scene(s, adaptive) && sweep > first_check(s);
+}
+
 void launch_sweep(yb_sim* s, int sweep, bool adaptive, bool partials, bool checkPrev) {
     const int in = sweep & 1;
     const int out = in ^ 1;
@@ -1227,7 +1396,13 @@ void launch_sweep(yb_sim* s, int sweep, bool adaptive, bool partials, bool check
     double2* part = sweep_partials(s, sweep, batched);
     const double2* prev = sweep_partials(s, sweep ^ 1, batched);
     const dim3 grid(blocks(sweep_waves(s->n) * 64, threads));
-    if (batched)
+    if (tile_list_mode(s, adaptive, sweep))
+        hipLaunchKernelGGL(k_sweep_list, dim3(list_blocks(s)), dim3(256), 0, s->stream,
+                           s->b.params, s->b.pos, s->b.prm, s->b.yv, s->b.q[in], s->b.q[out],
+                           s->b.qRest, s->b.dx[in], s->b.dx[out], s->b.numCols, s->b.cols,
+                           s->b.ctl, sweep, partials ? part : nullptr, scene_args(s, adaptive),
+                           s->b.tileList, s->b.tileCount);
+    else if (batched)
         hipLaunchKernelGGL(k_sweep<true>, grid, dim3(threads), 0, s->stream, s->b.params,
                            s->b.pos, s->b.prm, s->b.yv, s->b.q[in], s->b.q[out], s->b.qRest,
                            s->b.dx[in], s->b.dx[out], s->b.numCols, s->b.cols, s->b.ctl, sweep,
@@ -1245,10 +1420,16 @@ void launch_sweep(yb_sim* s, int sweep, bool adaptive, bool partials, bool check
 void launch_residual(yb_sim* s, int sweep) {
     const int nbw = sweep_waves(s->n);
     if (per_scene(s, true)) {
-        hipLaunchKernelGGL(k_residual_scenes, dim3(s->numScenes), dim3(64), 0, s->stream, s->b.ctl,
+        hipLaunchKernelGGL(k_residual_scenes, dim3(s->numScenes), dim3(kSceneThreads), 0, s->stream,
+                           s->b.ctl,
                            s->b.sceneCtl, s->b.sceneOff, s->b.sceneOf,
                            sweep_partials(s, sweep, true), denom_partials(s), nbw, s->numScenes,
                            sweep, s->tol, s->maxSweeps);
+        // the tiles of the scenes still running, for the sweeps that follow
+        hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(kTileScanThreads), 0, s->stream, s->b.sceneCtl,
+                           s->b.sceneOff, s->b.sceneOf, s->numScenes, s->b.tileCount);
+        hipLaunchKernelGGL(k_tile_fill, dim3(s->numScenes), dim3(64), 0, s->stream, s->b.sceneCtl,
+                           s->b.sceneOff, s->b.sceneOf, s->b.tileCount, s->b.tileList);
         return;
     }
     hipLaunchKernelGGL(k_residual, dim3(1), dim3(64), 0, s->stream, s->b.ctl,
@@ -1268,7 +1449,13 @@ void launch_strand_solve(yb_sim* s, int buf, SolverCtl* ctl, int sweep, float* s
     const LineOut lo{s->b.lineG, s->b.lineQL, sys};
     // in a batch, converged scenes are left alone (before sweep `sweep`)
     const SceneArgs sa = scene_args(s, ctl != nullptr);
-    if (sa.ctl)
+    const bool list = tile_list_mode(s, ctl != nullptr, sweep);
+    if (list)
+        hipLaunchKernelGGL(k_line_tile_list, dim3(list_blocks(s)), dim3(256), 0, s->stream,
+                           s->b.params, s->b.pos, s->b.prm, s->b.yv, s->b.q[buf], s->b.qRest,
+                           s->b.dx[buf], s->b.numCols, s->b.cols, lo, ctl, sweep, sa,
+                           s->b.tileList, s->b.tileCount);
+    else if (sa.ctl)
         hipLaunchKernelGGL(k_line_tile<true>, dim3(blocks(tiles * 64, 256)), dim3(256), 0,
                            s->stream, s->b.params, s->b.pos, s->b.prm, s->b.yv, s->b.q[buf],
                            s->b.qRest, s->b.dx[buf], s->b.numCols, s->b.cols, lo, ctl, sweep, sa);
@@ -1277,9 +1464,14 @@ void launch_strand_solve(yb_sim* s, int buf, SolverCtl* ctl, int sweep, float* s
                            s->stream, s->b.params, s->b.pos, s->b.prm, s->b.yv, s->b.q[buf],
                            s->b.qRest, s->b.dx[buf], s->b.numCols, s->b.cols, lo, ctl, sweep, sa);
     if (s->strandIters <= 2) {
-        hipLaunchKernelGGL(k_line_pairs_apply, dim3(blocks(tiles * 64, 256)), dim3(256), 0,
-                           s->stream, s->n, tiles, s->strandIters, s->b.lineG, s->b.lineQL,
-                           s->b.dx[buf], ctl, sweep, sa);
+        if (list)
+            hipLaunchKernelGGL(k_line_pairs_apply_list, dim3(list_blocks(s)), dim3(256), 0,
+                               s->stream, s->n, tiles, s->strandIters, s->b.lineG, s->b.lineQL,
+                               s->b.dx[buf], ctl, sweep, sa, s->b.tileList, s->b.tileCount);
+        else
+            hipLaunchKernelGGL(k_line_pairs_apply, dim3(blocks(tiles * 64, 256)), dim3(256), 0,
+                               s->stream, s->n, tiles, s->strandIters, s->b.lineG, s->b.lineQL,
+                               s->b.dx[buf], ctl, sweep, sa);
         return;
     }
     int cur = 0;
@@ -1299,7 +1491,8 @@ void launch_ctl_reset(yb_sim* s) {
     const bool batched = per_scene(s, true);
     hipLaunchKernelGGL(k_ctl_reset, dim3(batched ? blocks(s->numScenes, 256) : 1),
                        dim3(batched ? 256 : 64), 0, s->stream, s->b.ctl,
-                       batched ? s->b.sceneCtl : nullptr, s->numScenes);
+                       batched ? s->b.sceneCtl : nullptr, s->numScenes,
+                       batched ? s->b.tileCount : nullptr);
 }
 
 // The Newton-residual denominator's partials: the third part of b.partial

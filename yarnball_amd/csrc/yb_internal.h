@@ -167,6 +167,9 @@ struct Buffers {
     int32_t* sceneEndFlags = nullptr;
     SceneCtl* sceneCtl = nullptr;  // per-scene solver state (residual-targeted batches)
     double2* scenePart = nullptr;  // 3 parts x 2 sides x tiles: residual partials per scene
+    int32_t* tileList = nullptr;   // wave tiles of the scenes still running (after a check)
+    int32_t* tileCount = nullptr;  // [0]: entries of tileList (< 0: every tile);
+                                   // [1..numScenes]: each scene's first entry
     float4* drvInit = nullptr;  // scenario driver: initial position xyz, initial inverse mass
     float4* blockBox = nullptr;   // 2 per reduction block
     uint32_t* morton = nullptr;   // n, indexed by segment
@@ -226,6 +229,7 @@ struct yb_sim {
     int refitBlocks = 0;
     int refitLevels = 0;
     int refitTopBlocks = 0;
+    int numCUs = 256;        // compute units of the device (persistent grids)
     int queryStackLds = 8;   // k_query: traversal-stack entries in LDS
     int queryCandidates = 6; // k_query: candidate leaves buffered per thread
     bool treeBuilt = false;
@@ -320,6 +324,8 @@ void launch_residual_denom(yb_sim* s, bool frames);
 const double2* denom_partials(const yb_sim* s);
 void launch_end(yb_sim* s, int fin, bool adaptive);         // iteration.cu:48-61
 SceneArgs scene_args(const yb_sim* s, bool adaptive);
+// The first sweep of a residual-targeted substep whose residual is checked.
+int first_check(const yb_sim* s);
 int sweep_waves(int n);
 // learned warm start (mlp.hip)
 void launch_mlp_pack(yb_sim* s);
