@@ -110,3 +110,27 @@ def test_frame_update_is_rotation_equivariant():
     assert np.allclose(a @ R.T, b, atol=1e-9)
     qa_rot = np.array([qmul(qR, qq) for qq in qa])
     assert np.allclose(qa_rot, qb, atol=1e-6)
+
+
+def test_frame_update_is_sign_flip_robust():
+    """q and -q are the same rotation: negating any frames of the state leaves
+    the positions of the following substeps unchanged and every frame equal up
+    to its sign (the update picks each neighbour term's sign from a dot
+    product, cosserat.cu:228-245)."""
+    sc = scenes.readme_scene()
+    p = sc.params.copy()
+    a = Oracle(sc)
+    b = Oracle(sc)
+    pos, vel, lv, q = a.state()
+    flip = np.zeros(sc.n, bool)
+    flip[np.random.default_rng(5).random(sc.n) < 0.5] = True
+    q2 = np.where(flip[:, None], -q, q)
+    b.load64(pos=pos, vels=vel, lastVels=lv, qs=q2)
+    for _ in range(4):
+        a.advance(p, sc.maxH, sc.maxH)
+        b.advance(p, sc.maxH, sc.maxH)
+    pa, _, _, qa = a.state()
+    pb, _, _, qb = b.state()
+    assert np.max(np.abs(pa - pb)) <= 1e-12 * np.max(np.abs(pa))
+    same = np.minimum(np.abs(qa - qb).max(axis=1), np.abs(qa + qb).max(axis=1))
+    assert same.max() <= 1e-12

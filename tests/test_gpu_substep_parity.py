@@ -88,3 +88,26 @@ def test_twisted_cable_small_parity():
     sc = scenes.twisted_cable(strands=7, per=256)
     ep, eq, ncont = run_parity(sc, 12, label="cable-small")
     print(f"cable small: worst pos rel {ep:.2e}, worst q {eq:.2e}, contacts {ncont}")
+
+
+@pytest.mark.gpu
+def test_frames_negated_give_the_same_substeps():
+    """q and -q are the same rotation: uploading the knit patch with a random
+    half of its frames negated gives bit-identical positions after several
+    substeps, and every frame equal up to its sign (the device restates the
+    sign choices of cosserat.cu:228-245; rotate_e1 is quadratic in q)."""
+    sc = scenes.knit_patch(rows=8, per=256)
+    flip = np.random.default_rng(11).random(sc.n) < 0.5
+    sc2 = sc.copy()
+    sc2.qs = np.where(flip[:, None], -sc.qs, sc.qs).astype(np.float32)
+    a, b = gpu_sim(sc), gpu_sim(sc2)
+    for _ in range(6):
+        a.advance(a.maxH)
+        b.advance(b.maxH)
+    va, _, qa, _ = gpu_state(a)
+    vb, _, qb, _ = gpu_state(b)
+    assert np.array_equal(va["pos"], vb["pos"])
+    same = np.all(qa == qb, axis=1) | np.all(qa == -qb, axis=1)
+    assert same.all(), f"{int((~same).sum())} frames differ beyond their sign"
+    a.close()
+    b.close()
