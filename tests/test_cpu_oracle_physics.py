@@ -134,3 +134,31 @@ def test_frame_update_is_sign_flip_robust():
     assert np.max(np.abs(pa - pb)) <= 1e-12 * np.max(np.abs(pa))
     same = np.minimum(np.abs(qa - qb).max(axis=1), np.abs(qa + qb).max(axis=1))
     assert same.max() <= 1e-12
+
+
+@pytest.mark.parametrize("ratio,s", [(1.0, 1.0), (0.5, 0.5), (-1.0, 0.0), (2.0, 1.0), (0.0, 0.0)])
+def test_warm_start_acceleration_clamp(ratio, s):
+    """The plain warm start (iteration.cu:7-41) in closed form:
+    dx0 = h v + h^2 s g with s = clamp(a.g / |g|^2, 0, 1), a = (v - v_last) /
+    lastH, for a free vertex; y - x = h v + h^2 g; a pinned vertex gets h v.
+    Cases: a = ratio * g along g (s = ratio clamped), and a perpendicular to g
+    (s = 0), with lastH != h."""
+    from oracle import orc32
+    from yarnball_amd._lib import Params
+
+    p = Params()
+    p.h = 1e-3
+    p.lastH = 2e-3
+    g = np.array([0.0, 0.0, -9.8])
+    p.gravity[0], p.gravity[1], p.gravity[2] = g
+    v = np.array([[0.3, -0.2, 0.1]] * 3)
+    a_par = ratio * g
+    a_perp = np.array([2.0, 1.0, 0.0])  # perpendicular to g
+    lastV = np.stack([v[0] - a_par * p.lastH, v[1] - a_perp * p.lastH, v[2] - a_par * p.lastH])
+    invMass = np.array([1.0, 1.0, 0.0])
+    dx, y = orc32.warm_start(p, invMass, v.astype(np.float32), lastV.astype(np.float32))
+    h = p.h
+    expect = [h * v[0] + h * h * s * g, h * v[1], h * v[2]]
+    for k in range(3):
+        assert np.allclose(dx[k], expect[k], rtol=1e-5, atol=1e-9), (k, dx[k], expect[k])
+    assert np.allclose(y[0], h * v[0] + h * h * g, rtol=1e-5, atol=1e-9)
