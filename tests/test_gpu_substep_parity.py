@@ -111,3 +111,27 @@ def test_frames_negated_give_the_same_substeps():
     assert same.all(), f"{int((~same).sum())} frames differ beyond their sign"
     a.close()
     b.close()
+
+
+@pytest.mark.gpu
+def test_advance_splits_h_into_equal_substeps():
+    """advance(h) runs ceil(h / maxH) substeps of h / steps each (step.cpp:36-55),
+    and each substep's warm start uses the previous substep's length (lastH):
+    the device against the oracle's restatement of advance, for h = 2.5 maxH,
+    then h = maxH / 3, then h = 4 maxH exactly."""
+    sc = scenes.readme_scene()
+    sim = gpu_sim(sc)
+    orc = Oracle(sc)
+    p = sc.params.copy()
+    steps = 0
+    for ratio, n in ((2.5, 3), (1.0 / 3.0, 1), (4.0, 4)):
+        h = float(np.float32(ratio * sc.maxH))
+        sim.advance(h)
+        orc.advance(p, h, sc.maxH)
+        steps += n
+        assert sim.stepCount() == steps
+        v, _, q, _ = gpu_state(sim)
+        pos, _, _, oq = orc.state()
+        assert rel_err(v["pos"], pos) <= 1e-5, ratio
+        assert float(np.max(np.abs(q - oq))) <= 1e-5, ratio
+    sim.close()
