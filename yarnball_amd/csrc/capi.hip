@@ -88,7 +88,8 @@ static void free_all(yb_sim* s) {
                     b.aabb,    b.sceneBox, b.blockBox, b.morton, b.keys[0], b.keys[1],
                     b.vals[0], b.vals[1], b.hist, b.scanPart, b.nodes,   b.parent,  b.rng,
                     b.kids,    b.strad,   b.stradCount, b.leafNeed, b.sufBox, b.preBox, b.blkTab,
-                    b.err,     b.params,  b.stage, b.ctl, b.partial, b.minSep,
+                    b.err,     b.params,  b.stage, b.ctl, b.partial, b.resBlocks, b.resCounter,
+                    b.minSep,
                     b.mlpPrm,  b.mlpTiles, b.mlpBias, b.corrPrev, b.dxPlain, b.predC,
                     b.lineG,   b.lineQL,  b.lineE[0], b.lineE[1], b.lineF[0], b.lineF[1],
                     b.mlpCount, b.recX, b.recT, b.recCount, b.sceneOf, b.sceneOff,
@@ -226,6 +227,8 @@ extern "C" int yb_create(int n, yb_sim** out) {
     ALLOC(b.mlpCount, 2 * ((N + 63) / 64));
     ALLOC(b.recCount, 2);
     ALLOC(b.partial, 3 * (size_t)sweep_waves(n));
+    ALLOC(b.resBlocks, 2 * 64);
+    ALLOC(b.resCounter, 1);
     ALLOC(b.lineG, N);
     ALLOC(b.lineQL, N);
     for (int k = 0; k < 2; k++) {
@@ -708,29 +711,27 @@ int yb::first_check(const yb_sim* s) {
     return s->maxSweeps;
 }
 
-// The residual of a check sweep is reduced, and convergence decided, by the
-// next sweep's first wave; only the last sweep a substep may run is checked by
-// a launch of its own.
+// The residual of a check sweep is reduced, and convergence decided, either by
+// a launch of its own right after it (k_residual; per scene in a batch) or by
+// the next sweep's first wave (separate_checks); the last sweep a substep may
+// run is always checked by a launch of its own.
 // last: the final sweep of a run that yb_profile times as one interval.
 // A strand step between sweeps k - 1 and k (residual mode, every
-// s->strandEvery sweeps) works on sweep k's input buffers.  The decision about
-// sweep k - 1 is made before it, by a launch of its own, and every kernel of
+// s->strandEvery sweeps) works on sweep k's input buffers.  Every kernel of
 // the step does nothing once the solve has converged: a converged iterate is
 // never changed.
 // Steps at iterations first, first + every, ...; at most kStrandSteps per
 // substep: a substep that has not converged by then finishes with plain sweeps.
 constexpr int kStrandSteps = 16;
 static void enqueue_sweep(yb_sim* s, int k, bool adaptive, bool last = false) {
-    // batched scenes decide each scene's convergence in a launch of their own
-    // right after every check sweep (no in-sweep check of the previous one)
-    const bool perScene = adaptive && s->numScenes > 1;
+    const bool sep = adaptive && separate_checks(s);
     const bool check = adaptive && checks_residual(s, k);
-    bool checkPrev = !perScene && adaptive && k > 0 && checks_residual(s, k - 1);
+    bool checkPrev = adaptive && !sep && k > 0 && checks_residual(s, k - 1);
     const int first = s->strandFirst > 0 ? s->strandFirst : s->strandEvery;
     if (adaptive && s->strandSolve && s->strandEvery > 0 && k >= first &&
         (k - first) % s->strandEvery == 0 && (k - first) / s->strandEvery < kStrandSteps) {
         flush_held(s);
-        if (checkPrev) {
+        if (checkPrev) {  // the step must not change a converged iterate
             launch_residual(s, k - 1);
             mark(s, YB_STAGE_RESIDUAL);
             checkPrev = false;
@@ -741,7 +742,7 @@ static void enqueue_sweep(yb_sim* s, int k, bool adaptive, bool last = false) {
         mark(s, YB_STAGE_MLP);
     }
     launch_sweep(s, k, adaptive, check, checkPrev);
-    const bool standalone = check && (perScene || k == s->maxSweeps - 1);
+    const bool standalone = check && (sep || k == s->maxSweeps - 1);
     if (last || standalone)
         mark(s, YB_STAGE_SWEEP);
     else

@@ -122,10 +122,6 @@ __global__ __launch_bounds__(256) void k_step_limit(const DevParams* __restrict_
 
 __device__ __forceinline__ void wave_residual(float numU, float numF, double2* partial, int wave,
                                               int n);
-__device__ __forceinline__ void residual_wave(SolverCtl* __restrict__ ctl,
-                                              const double2* __restrict__ partial,
-                                              const double2* __restrict__ denomPart, int nb,
-                                              int sweep, float tol, int maxSweeps);
 
 // Closest points of segment e = (0 -> e) and (B0 -> B0 + A1), specialised from
 // closest_uv (Common.h:318-334) with the segment's |e|^2 and 1/|e|^2 hoisted
@@ -241,6 +237,83 @@ __device__ __forceinline__ void wave_residual_scenes(float numU, float numF, boo
     }
 }
 
+// Sum of nb partials by one wave in a fixed order of additions.  Eight running
+// sums per lane keep eight loads in flight (the reduction is latency-bound: one
+// wave reads 17k partials on the 1M-vertex knit).
+__device__ __forceinline__ double2 reduce_partials(const double2* __restrict__ partial, int nb) {
+    const int lane = threadIdx.x & 63;
+    double2 acc[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) acc[j] = make_double2(0.0, 0.0);
+    int k = lane;
+    for (; k + 7 * 64 < nb; k += 8 * 64) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const double2 t = partial[k + 64 * j];
+            acc[j].x += t.x;
+            acc[j].y += t.y;
+        }
+    }
+    for (; k < nb; k += 64) {
+        const double2 t = partial[k];
+        acc[0].x += t.x;
+        acc[0].y += t.y;
+    }
+    double a = ((acc[0].x + acc[1].x) + (acc[2].x + acc[3].x)) +
+               ((acc[4].x + acc[5].x) + (acc[6].x + acc[7].x));
+    double b = ((acc[0].y + acc[1].y) + (acc[2].y + acc[3].y)) +
+               ((acc[4].y + acc[5].y) + (acc[6].y + acc[7].y));
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off, 64);
+        b += __shfl_xor(b, off, 64);
+    }
+    return make_double2(a, b);
+}
+
+// Stopping decision of check sweep `sweep` from its sums a = (sum |dx' - dx|^2,
+// sum |r|^2) and the Newton-residual denominator denF (one thread).
+__device__ __forceinline__ void decide(SolverCtl* __restrict__ ctl, double2 a, double denF,
+                                       int sweep, float tol, int maxSweeps) {
+    ctl->denomF = denF;
+    const double denU = ctl->denom;
+    const float rU = denU > 0.0 ? (float)sqrt(a.x / denU) : 0.0f;
+    const float rF = denF > 0.0 ? (float)sqrt(a.y / denF) : 0.0f;
+    const float r = ctl->kind ? rF : rU;
+    ctl->resid = r;
+    ctl->fres = rF;
+    if (r <= tol || sweep == maxSweeps - 1) {
+        ctl->doneAt = sweep;
+        ctl->fin = (sweep & 1) ^ 1;
+        ctl->sweeps += (unsigned long long)(sweep + 1);
+        ctl->substeps += 1ull;
+        if (sweep + 1 > ctl->maxUsed) ctl->maxUsed = sweep + 1;
+        if (r > tol) ctl->notConverged += 1;
+        if (r > ctl->maxResid) ctl->maxResid = r;
+        if (rF > ctl->maxFres) ctl->maxFres = rF;
+        ctl->sumFres += (double)rF;
+    }
+}
+
+// The in-sweep check (plain and learned sessions, which run hundreds of sweeps
+// per substep): the first wave of the sweep after a check sweep reduces that
+// sweep's partials in a fixed order and decides, while the rest of the launch
+// runs.  It costs no launch of its own; a converged substep is recognised one
+// sweep late (that sweep's output is not used).
+struct CheckPrev {
+    const double2* partial;    // the previous (check) sweep's partials, or null
+    const double2* denomPart;  // the prologue's denominator partials, or null
+    float tol;
+    int maxSweeps;
+};
+__device__ __forceinline__ void residual_wave(SolverCtl* __restrict__ ctl, CheckPrev cp, int nb,
+                                              int sweep) {
+    const double2 a = reduce_partials(cp.partial, nb);
+    double denF = ctl->denomF;  // uniform across the wave
+    if (denF < 0.0) denF = cp.denomPart ? reduce_partials(cp.denomPart, nb).y : a.y;
+    if ((threadIdx.x & 63) == 0) decide(ctl, a, denF, sweep, cp.tol, cp.maxSweeps);
+}
+
 // Tile-list kernels of a residual-targeted batch (after its first check): a
 // fixed grid of waves loops over the listed tiles -- those of the scenes
 // still running -- so that sweeps late in a substep, when few scenes remain,
@@ -268,22 +341,16 @@ __device__ __forceinline__ void sweep_body(const DevParams* __restrict__ P,
                                            const int32_t* __restrict__ numCols,
                                            const int32_t* __restrict__ cols,
                                            SolverCtl* __restrict__ ctl, int sweep,
-                                           double2* __restrict__ partial,
-                                           const double2* __restrict__ partialPrev,
-                                           const double2* __restrict__ denomPart,
-                                           float tol, int maxSweeps, LineOut lo,
+                                           double2* __restrict__ partial, LineOut lo,
                                            SceneArgs sa = SceneArgs{nullptr, nullptr},
-                                           int tileIn = -1) {
+                                           int tileIn = -1,
+                                           CheckPrev cp = CheckPrev{nullptr, nullptr, 0.0f, 0}) {
     // Residual-targeted mode: sweeps after convergence do nothing.
     if (ctl && ctl->doneAt < sweep) return;
     const int n = P->n;
-    // The previous sweep was a check sweep: its residual and the stopping
-    // decision are made here, by the first wave of the launch, while the rest
-    // of this sweep runs.  A sweep that converged is thus recognised one sweep
-    // later (this sweep's result is not used; the next one exits at once), and
-    // no separate reduction launch sits between two sweeps.
-    if (!kBatched && partialPrev && blockIdx.x == 0 && threadIdx.x < 64)
-        residual_wave(ctl, partialPrev, denomPart, (n + 61) / 62, sweep - 1, tol, maxSweeps);
+    // In-sweep check of the previous (check) sweep, by the launch's first wave.
+    if (!kBatched && cp.partial && blockIdx.x == 0 && threadIdx.x < 64)
+        residual_wave(ctl, cp, (n + 61) / 62, sweep - 1);
     const int lane = threadIdx.x & 63;
     // the wave tile: given by a tile-list kernel, else from the block index
     const int wave = tileIn >= 0 ? tileIn
@@ -602,13 +669,11 @@ __global__ __launch_bounds__(512) void k_sweep(const DevParams* __restrict__ P,
                                                const int32_t* __restrict__ numCols,
                                                const int32_t* __restrict__ cols,
                                                SolverCtl* __restrict__ ctl, int sweep,
-                                               double2* __restrict__ partial,
-                                               const double2* __restrict__ partialPrev,
-                                               const double2* __restrict__ denomPart,
-                                               float tol, int maxSweeps, SceneArgs sa) {
+                                               double2* __restrict__ partial, SceneArgs sa,
+                                               CheckPrev cp) {
     sweep_body<kSweep, kBatched>(P, pos, prm, yv, qIn, qOut, qRest, dxIn, dxOut, numCols, cols,
-                                 ctl, sweep, partial, partialPrev, denomPart, tol, maxSweeps,
-                                 LineOut{nullptr, nullptr, nullptr}, sa);
+                                 ctl, sweep, partial, LineOut{nullptr, nullptr, nullptr}, sa, -1,
+                                 cp);
 }
 
 // Strand solve, tile part (see the line mode of sweep_body).
@@ -626,7 +691,7 @@ __global__ __launch_bounds__(256) void k_line_tile(const DevParams* __restrict__
                                                    SceneArgs sa) {
     if (strand_skip(ctl, sweep)) return;
     sweep_body<kLine, kBatched>(P, pos, prm, yv, qIn, nullptr, qRest, dxIn, nullptr, numCols,
-                                cols, nullptr, sweep, nullptr, nullptr, nullptr, 0.0f, 0, lo, sa);
+                                cols, nullptr, sweep, nullptr, lo, sa);
 }
 
 __global__ __launch_bounds__(256) void k_sweep_list(
@@ -643,8 +708,7 @@ __global__ __launch_bounds__(256) void k_sweep_list(
     const int waves = (int)(gridDim.x * blockDim.x >> 6);
     for (int k = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6); k < count; k += waves)
         sweep_body<kSweep, true>(P, pos, prm, yv, qIn, qOut, qRest, dxIn, dxOut, numCols, cols,
-                                 ctl, sweep, partial, nullptr, nullptr, 0.0f, 0,
-                                 LineOut{nullptr, nullptr, nullptr}, sa,
+                                 ctl, sweep, partial, LineOut{nullptr, nullptr, nullptr}, sa,
                                  listed_tile(tileList, tileCount, k));
 }
 
@@ -661,7 +725,7 @@ __global__ __launch_bounds__(256) void k_line_tile_list(
     const int waves = (int)(gridDim.x * blockDim.x >> 6);
     for (int k = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6); k < count; k += waves)
         sweep_body<kLine, true>(P, pos, prm, yv, qIn, nullptr, qRest, dxIn, nullptr, numCols,
-                                cols, nullptr, sweep, nullptr, nullptr, nullptr, 0.0f, 0, lo, sa,
+                                cols, nullptr, sweep, nullptr, lo, sa,
                                 listed_tile(tileList, tileCount, k));
 }
 
@@ -748,8 +812,7 @@ __global__ __launch_bounds__(256) void k_force_eval(const DevParams* __restrict_
                                                     double2* __restrict__ partial,
                                                     SceneArgs sa) {
     sweep_body<kEval, kBatched>(P, pos, prm, yv, qIn, qOut, qRest, dxIn, nullptr, numCols, cols,
-                                nullptr, 0, partial, nullptr, nullptr, 0.0f, 0,
-                                LineOut{nullptr, nullptr, nullptr}, sa);
+                                nullptr, 0, partial, LineOut{nullptr, nullptr, nullptr}, sa);
 }
 
 // Strand solve, boundary part: one iteration over the pairs of vertices that
@@ -925,92 +988,97 @@ __device__ __forceinline__ void wave_residual(float numU, float numF, double2* p
         partial[wave] = make_double2((double)numU, (double)numF);
 }
 
-// Sum of nb partials by one wave in a fixed order of additions.  Eight running
-// sums per lane keep eight loads in flight (the reduction is latency-bound: one
-// wave reads 17k partials on the 1M-vertex knit).
-__device__ __forceinline__ double2 reduce_partials(const double2* __restrict__ partial, int nb) {
-    const int lane = threadIdx.x & 63;
-    double2 acc[8];
-#pragma unroll
-    for (int j = 0; j < 8; j++) acc[j] = make_double2(0.0, 0.0);
-    int k = lane;
-    for (; k + 7 * 64 < nb; k += 8 * 64) {
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-            const double2 t = partial[k + 64 * j];
-            acc[j].x += t.x;
-            acc[j].y += t.y;
-        }
+// Fixed-order sum of partials [lo, hi) by one block: thread t sums
+// lo + t, lo + t + blockDim.x, ... in order, then a tree over the threads with
+// a fixed pairing.  Every thread returns the sum.
+constexpr int kResThreads = 256;
+__device__ __forceinline__ double2 block_sum(const double2* __restrict__ p, int lo, int hi) {
+    __shared__ double2 red[kResThreads];
+    const int t = threadIdx.x;
+    double x = 0.0, y = 0.0;
+    for (int k = lo + t; k < hi; k += kResThreads) {
+        const double2 v = p[k];
+        x += v.x;
+        y += v.y;
     }
-    for (; k < nb; k += 64) {
-        const double2 t = partial[k];
-        acc[0].x += t.x;
-        acc[0].y += t.y;
+    __syncthreads();  // a previous call's values have been read
+    red[t] = make_double2(x, y);
+    __syncthreads();
+    for (int half = kResThreads / 2; half > 0; half >>= 1) {
+        if (t < half) red[t] = make_double2(red[t].x + red[t + half].x, red[t].y + red[t + half].y);
+        __syncthreads();
     }
-    double a = ((acc[0].x + acc[1].x) + (acc[2].x + acc[3].x)) +
-               ((acc[4].x + acc[5].x) + (acc[6].x + acc[7].x));
-    double b = ((acc[0].y + acc[1].y) + (acc[2].y + acc[3].y)) +
-               ((acc[4].y + acc[5].y) + (acc[6].y + acc[7].y));
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        a += __shfl_xor(a, off, 64);
-        b += __shfl_xor(b, off, 64);
-    }
-    return make_double2(a, b);
+    return red[0];
 }
 
-// Residuals of sweep `sweep` and the stopping decision, by one wave.
-// No Newton-residual denominator yet (the substep's first check): it is the
-// sum of denomPart, the partials of the prologue's pass over the plain warm
-// start, or -- in a plain session, whose first check is sweep 0 and whose
-// sweep 0 input is the plain warm start itself -- this check's own sum.
-__device__ __forceinline__ void residual_wave(SolverCtl* __restrict__ ctl,
-                                              const double2* __restrict__ partial,
-                                              const double2* __restrict__ denomPart, int nb,
-                                              int sweep, float tol, int maxSweeps) {
-    const double2 a = reduce_partials(partial, nb);
-    double denF = ctl->denomF;  // uniform across the wave
-    if (denF < 0.0) denF = denomPart ? reduce_partials(denomPart, nb).y : a.y;
-    if ((threadIdx.x & 63) == 0) {
-        ctl->denomF = denF;
-        const double denU = ctl->denom;
-        const float rU = denU > 0.0 ? (float)sqrt(a.x / denU) : 0.0f;
-        const float rF = denF > 0.0 ? (float)sqrt(a.y / denF) : 0.0f;
-        const float r = ctl->kind ? rF : rU;
-        ctl->resid = r;
-        ctl->fres = rF;
-        if (r <= tol || sweep == maxSweeps - 1) {
-            ctl->doneAt = sweep;
-            ctl->fin = (sweep & 1) ^ 1;
-            ctl->sweeps += (unsigned long long)(sweep + 1);
-            ctl->substeps += 1ull;
-            if (sweep + 1 > ctl->maxUsed) ctl->maxUsed = sweep + 1;
-            if (r > tol) ctl->notConverged += 1;
-            if (r > ctl->maxResid) ctl->maxResid = r;
-            if (rF > ctl->maxFres) ctl->maxFres = rF;
-            ctl->sumFres += (double)rF;
-        }
-    }
-}
-
-// Stand-alone check, used only for the last sweep a substep may run (every
-// other check is made by the following sweep's first wave, in k_sweep).
-__global__ __launch_bounds__(64) void k_residual(SolverCtl* __restrict__ ctl,
-                                                 const double2* __restrict__ partial,
-                                                 const double2* __restrict__ denomPart, int nb,
-                                                 int sweep, float tol, int maxSweeps) {
+// Residual check of sweep `sweep`, launched right after it: kResBlocks blocks
+// each sum a fixed range of the per-tile partials (and of the prologue's
+// denominator partials when the substep has no denominator yet); the last
+// block to finish adds the block sums in block order and makes the stopping
+// decision.  The next sweep, launched after this kernel, then exits at once
+// when the substep has converged.  The sums do not depend on scheduling.
+constexpr int kResBlocks = 32;
+__global__ __launch_bounds__(kResThreads) void k_residual(SolverCtl* __restrict__ ctl,
+                                                          const double2* __restrict__ partial,
+                                                          const double2* __restrict__ denomPart,
+                                                          int nb, int sweep, float tol,
+                                                          int maxSweeps,
+                                                          double2* __restrict__ blockSums,
+                                                          unsigned int* __restrict__ counter) {
     if (ctl->doneAt < sweep) return;
-    residual_wave(ctl, partial, denomPart, nb, sweep, tol, maxSweeps);
+    // No Newton-residual denominator yet (the substep's first check): it is the
+    // sum of denomPart, the partials of the prologue's pass over the plain warm
+    // start, or -- in a plain session, whose first check is sweep 0 and whose
+    // sweep 0 input is the plain warm start itself -- this check's own sum.
+    const bool needDen = ctl->denomF < 0.0 && denomPart;
+    const int b = blockIdx.x, R = gridDim.x;
+    const int lo = (int)((long long)nb * b / R), hi = (int)((long long)nb * (b + 1) / R);
+    const double2 sum = block_sum(partial, lo, hi);
+    const double2 den = needDen ? block_sum(denomPart, lo, hi) : make_double2(0.0, 0.0);
+    __shared__ bool last;
+    __shared__ double3 all[kResBlocks];
+    if (threadIdx.x == 0) {
+        blockSums[2 * b] = sum;
+        blockSums[2 * b + 1] = den;
+        __threadfence();
+        last = atomicAdd(counter, 1u) == (unsigned)(R - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    // the last block: every block's sums, one per thread, read at device
+    // scope (they were written by other CUs), then added in block order
+    if (threadIdx.x < R) {
+        const double* q = reinterpret_cast<const double*>(blockSums + 2 * threadIdx.x);
+        all[threadIdx.x] =
+            make_double3(__hip_atomic_load(q + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                         __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                         __hip_atomic_load(q + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double2 a = make_double2(0.0, 0.0);
+    double d = 0.0;
+    for (int k = 0; k < R; k++) {
+        a.x += all[k].x;
+        a.y += all[k].y;
+        d += all[k].z;
+    }
+    *counter = 0u;  // for the next check (a later launch)
+    const double denF = ctl->denomF >= 0.0 ? ctl->denomF : (needDen ? d : a.y);
+    decide(ctl, a, denF, sweep, tol, maxSweeps);
 }
 
 __global__ void k_ctl_reset(SolverCtl* __restrict__ ctl, SceneCtl* __restrict__ sceneCtl,
-                            int numScenes, int32_t* __restrict__ tileCount) {
+                            int numScenes, int32_t* __restrict__ tileCount,
+                            unsigned int* __restrict__ resCounter) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k == 0) {
         ctl->doneAt = 0x7fffffff;
         ctl->denomF = -1.0;
         ctl->scenesDone = 0;
         if (tileCount) tileCount[0] = -1;  // every tile until the first check
+        if (resCounter) *resCounter = 0u;
     }
     if (sceneCtl && k < numScenes) {
         sceneCtl[k].doneAt = 0x7fffffff;
@@ -1394,7 +1462,8 @@ void launch_sweep(yb_sim* s, int sweep, bool adaptive, bool partials, bool check
     // partials of sweep k go to half k & 1, so a sweep can reduce its
     // predecessor's while writing its own
     double2* part = sweep_partials(s, sweep, batched);
-    const double2* prev = sweep_partials(s, sweep ^ 1, batched);
+    const CheckPrev cp{checkPrev ? sweep_partials(s, sweep ^ 1, false) : nullptr,
+                       denom_partials(s), s->tol, s->maxSweeps};
     const dim3 grid(blocks(sweep_waves(s->n) * 64, threads));
     if (tile_list_mode(s, adaptive, sweep))
         hipLaunchKernelGGL(k_sweep_list, dim3(list_blocks(s)), dim3(256), 0, s->stream,
@@ -1406,15 +1475,14 @@ void launch_sweep(yb_sim* s, int sweep, bool adaptive, bool partials, bool check
         hipLaunchKernelGGL(k_sweep<true>, grid, dim3(threads), 0, s->stream, s->b.params,
                            s->b.pos, s->b.prm, s->b.yv, s->b.q[in], s->b.q[out], s->b.qRest,
                            s->b.dx[in], s->b.dx[out], s->b.numCols, s->b.cols, s->b.ctl, sweep,
-                           partials ? part : nullptr, nullptr, nullptr, s->tol, s->maxSweeps,
-                           scene_args(s, adaptive));
+                           partials ? part : nullptr, scene_args(s, adaptive),
+                           CheckPrev{nullptr, nullptr, 0.0f, 0});
     else
         hipLaunchKernelGGL(k_sweep<false>, grid, dim3(threads), 0, s->stream, s->b.params,
                            s->b.pos, s->b.prm, s->b.yv, s->b.q[in], s->b.q[out], s->b.qRest,
                            s->b.dx[in], s->b.dx[out], s->b.numCols, s->b.cols,
                            adaptive ? s->b.ctl : nullptr, sweep, partials ? part : nullptr,
-                           checkPrev ? prev : nullptr, denom_partials(s), s->tol, s->maxSweeps,
-                           SceneArgs{nullptr, nullptr});
+                           SceneArgs{nullptr, nullptr}, cp);
 }
 
 void launch_residual(yb_sim* s, int sweep) {
@@ -1432,9 +1500,9 @@ void launch_residual(yb_sim* s, int sweep) {
                            s->b.sceneOff, s->b.sceneOf, s->b.tileCount, s->b.tileList);
         return;
     }
-    hipLaunchKernelGGL(k_residual, dim3(1), dim3(64), 0, s->stream, s->b.ctl,
-                       sweep_partials(s, sweep, false), denom_partials(s), nbw, sweep, s->tol,
-                       s->maxSweeps);
+    hipLaunchKernelGGL(k_residual, dim3(std::min(kResBlocks, nbw)), dim3(kResThreads), 0,
+                       s->stream, s->b.ctl, sweep_partials(s, sweep, false), denom_partials(s),
+                       nbw, sweep, s->tol, s->maxSweeps, s->b.resBlocks, s->b.resCounter);
 }
 
 void launch_frame_refresh(yb_sim* s) {
@@ -1492,7 +1560,7 @@ void launch_ctl_reset(yb_sim* s) {
     hipLaunchKernelGGL(k_ctl_reset, dim3(batched ? blocks(s->numScenes, 256) : 1),
                        dim3(batched ? 256 : 64), 0, s->stream, s->b.ctl,
                        batched ? s->b.sceneCtl : nullptr, s->numScenes,
-                       batched ? s->b.tileCount : nullptr);
+                       batched ? s->b.tileCount : nullptr, s->b.resCounter);
 }
 
 // The Newton-residual denominator's partials: the third part of b.partial

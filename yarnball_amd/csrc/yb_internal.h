@@ -193,7 +193,9 @@ struct Buffers {
     float* minSep = nullptr;      // smallest contact distance / (2 radius) seen
     DevParams* params = nullptr;
     SolverCtl* ctl = nullptr;
-    double2* partial = nullptr;   // residual partial sums (update, force) per sweep wave, two halves
+    double2* partial = nullptr;  // residual partial sums (update, force) per sweep wave, three parts
+    double2* resBlocks = nullptr;        // residual check: per-block sums (2 per block)
+    unsigned int* resCounter = nullptr;  // residual check: blocks done
     int32_t* hostFlag = nullptr;  // pinned: doneAt read back after each chunk
     // learned warm start
     float* mlpPrm = nullptr;          // kMlpParams floats
@@ -304,9 +306,17 @@ void launch_separation(yb_sim* s);              // contact distances of the curr
 // Sweep k reads dx[k&1], q[k&1] and writes the other buffer.  adaptive: the
 // kernel honours SolverCtl (early exit after convergence, residual partials).
 // cosserat.cu:20-253; partials: write this sweep's residual partials;
-// checkPrev: also decide convergence of the previous sweep (a check sweep)
+// checkPrev: the launch's first wave also checks the previous sweep.
 void launch_sweep(yb_sim* s, int sweep, bool adaptive, bool partials, bool checkPrev);
+// The check of sweep `sweep` as a launch of its own (right after it):
+// residuals and the stopping decision; in a batch, per scene, and the tile
+// list of the scenes still running.
 void launch_residual(yb_sim* s, int sweep);
+// Whether every check is a launch of its own (the strand solve and batches,
+// which converge within tens of sweeps, so that the sweep after the converged
+// check exits at once), or is made by the next sweep's first wave (plain and
+// learned sessions, hundreds of sweeps: no launch, one sweep late).
+inline bool separate_checks(const yb_sim* s) { return s->strandSolve || s->numScenes > 1; }
 void launch_scene_end_flags(yb_sim* s);  // flags of each scene's last vertex -> sceneEndFlags
 void launch_ctl_reset(yb_sim* s);
 void launch_frame_refresh(yb_sim* s);  // closed-form frame update from dx[0]
