@@ -475,6 +475,51 @@ def test_residual_force_at_a_held_vertex_is_what_the_limit_leaves():
     assert np.linalg.norm(fixed["resid"]) <= 1e-9 * np.linalg.norm(free["f"])
 
 
+def test_frames_enter_the_force_only_through_the_stretch_terms():
+    """With every segment's frame changed from q to q', a vertex's force moves
+    by d_v - d_(v-1), d_s = kStretch_s / l_s (q'_s e1 q'_s* - q_s e1 q_s*)
+    (d of a missing segment 0), and its Hessian not at all: inertia, damping,
+    the barrier and lagged friction do not see the frames.  The strand step
+    that starts a substep makes its system at refreshed frames and takes the
+    Newton-residual denominator at the frames before the refresh through this
+    correction."""
+    r, b = 1e-4, 8e-4
+    sc = crossing_scene(2 * r + 0.4 * b, frictionCoeff=0.3, kFriction=1e-3)
+    o = Oracle(sc)
+    o.set_contacts(*contact_lists(6, [(0, 3), (1, 4)]))
+    p = sc.params.copy()
+    p.h = np.float32(1e-3)
+    rng = np.random.default_rng(11)
+    dx = base_dx(6, 5, scale=1e-7)
+    y = base_dx(6, 6, scale=1e-6)
+    o.set_iterate(dx=dx, y=y, step_limit=np.full(6, np.inf))
+    q = sc.qs.astype(np.float64)
+    q2 = (Rotation.from_rotvec(rng.normal(0.0, 0.1, (6, 3))) * Rotation.from_quat(q)).as_quat()
+    o.load64(qs=q)
+    before = [o.vertex_terms(p, v, dx) for v in range(6)]
+    o.load64(qs=q2)
+    after = [o.vertex_terms(p, v, dx) for v in range(6)]
+    flags = sc.verts["flags"]
+    k = sc.verts["kStretch"].astype(np.float64)
+    l = sc.verts["lRest"].astype(np.float64)
+    d = [k[s] / l[s] * (e1_of(q2[s]) - e1_of(q[s])) if flags[s] & FLAG_HAS_NEXT
+         else np.zeros(3) for s in range(6)]
+    moved = 0.0
+    for v in range(6):
+        want = d[v] - (d[v - 1] if v > 0 and flags[v - 1] & FLAG_HAS_NEXT else 0.0)
+        got = before[v]["f"] - after[v]["f"]
+        scale = np.abs(before[v]["f"]).max()
+        assert np.allclose(got, want, rtol=1e-9, atol=1e-12 * scale), f"vertex {v}"
+        assert np.array_equal(before[v]["H"], after[v]["H"]), f"vertex {v} Hessian"
+        moved = max(moved, np.abs(want).max() / scale)
+    assert moved > 1e-3  # the frames did move the forces
+    # the contacts are active: the barrier makes the Hessians differ from the
+    # contact-free ones
+    o.set_contacts(*contact_lists(6, []))
+    free = o.vertex_terms(p, 0, dx)
+    assert not np.allclose(free["H"], before[0]["H"])
+
+
 # ---------------------------------------------------------------------------
 @pytest.mark.parametrize("gap", [2.0e-4, 2.6e-4, 3.5e-4])
 def test_step_limit_of_crossing_segments(gap):

@@ -662,16 +662,23 @@ static void enqueue_prologue(yb_sim* s, int mode, bool adaptive, int warm) {
     // it here, before the start is changed.
     // With a plain start this substep, the frame refresh below is made by the
     // same pass (both read the plain warm start and the previous frames).
+    // In a strand session the first strand step joins that pass too, unless
+    // features are recorded first; the plain start a learned session keeps
+    // for its correction is then copied by the same pass.
     const bool refresh = (warm != 0 || s->strandSolve) && s->frameRefresh;
     const bool evalDenom = adaptive && (s->warmStart != 0 || s->strandSolve);
     const bool fused = evalDenom && refresh && warm == 0;
+    const bool learned = warm == 1, record = s->recCap > 0;
+    const bool lineStart = fused && s->strandSolve && !record;
     if (adaptive) {
         launch_ctl_reset(s);
-        if (evalDenom) launch_residual_denom(s, fused);
+        if (lineStart)
+            launch_line_start(s, s->warmStart == 1);
+        else if (evalDenom)
+            launch_residual_denom(s, fused);
         mark(s, YB_STAGE_RESIDUAL);
     }
-    const bool learned = warm == 1, record = s->recCap > 0;
-    if (learned || record || s->warmStart == 1) {
+    if (learned || record || (s->warmStart == 1 && !lineStart)) {
         // the inference kernel computes its own features; the recording
         // kernel runs for training data and for the plain warm start alone
         if (record || !learned) launch_features(s, record);
@@ -690,7 +697,7 @@ static void enqueue_prologue(yb_sim* s, int mode, bool adaptive, int warm) {
         launch_frame_refresh(s);
         mark(s, YB_STAGE_MLP);
     }
-    if (s->strandSolve) {
+    if (s->strandSolve && !lineStart) {
         launch_strand_solve(s);
         mark(s, YB_STAGE_MLP);
     }

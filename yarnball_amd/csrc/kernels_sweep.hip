@@ -49,6 +49,26 @@ __device__ __forceinline__ rot vb_e1(v3 v, rot b) {
     return rot{-dot(v, bq), mq.z, -mq.y, -mq.x};
 }
 
+// Closed-form frame update of segment i (cosserat.cu:209-253): q0 its frame,
+// e its edge in the configuration x + dx, kl = -2 kStretch_i / l_i, and the
+// neighbouring frames with the rest rotors between them (qr: of segments i and
+// i + 1, qrPrev: of segments i - 1 and i).
+__device__ __forceinline__ rot frame_update(rot q0, v3 e, float kl, uint32_t flags, rot qPrev,
+                                            rot qrPrev, rot qNext, rot qr) {
+    const v3 vv = e * kl;
+    rot b = rot{0.f, 0.f, 0.f, 0.f};
+    if (flags & YB_FLAG_HAS_PREV) {
+        const float sg = rdot(rmul(rinv(qPrev), q0), qrPrev) > 0.0f ? 1.0f : -1.0f;
+        b = rscale(rmul(qPrev, qrPrev), sg);
+    }
+    if (flags & YB_FLAG_HAS_NEXT_ORIENTATION) {
+        const float sg = rdot(rmul(rinv(q0), qNext), qr) > 0.0f ? 1.0f : -1.0f;
+        b = radd(b, rscale(rmul(qNext, rinv(qr)), sg));
+    }
+    const float lambda = len(vv) + rlen(b);
+    return rnormalize_fast(radd(vb_e1(vv, b), rscale(b, lambda)));
+}
+
 // H^{-1} f through the adjugate (sym_solve with the hardware reciprocal).
 __device__ __forceinline__ v3 sym_solve_fast(const sym3& m, v3 f) {
     const float a = m.a[0], b = m.a[1], c = m.a[2], d = m.a[3], e = m.a[4], g = m.a[5];
@@ -208,8 +228,15 @@ __device__ __forceinline__ void pcr64(float& A, float& Lo, float& Up, float B[5]
 // output is written.  kEval -- only the residual partials of the input iterate.
 // kSweep and kEval write, when `partial` is set, the per-wave sums of the
 // squared update |dx' - dx|^2 (kSweep) and of the squared residual force |r|^2
-// at the input iterate (r: see the Newton step below).
-constexpr int kSweep = 0, kLine = 1, kEval = 2;
+// at the input iterate (r: see the Newton step below).  kLineStart -- the
+// first strand step of a substep, from the plain warm start: each lane first
+// refreshes its segment's frame from the input dx (the sweep's frame update at
+// dx' = dx) and writes it to qOut, then assembles and solves the strand system
+// as kLine does, at the refreshed frames, and writes kEval's residual partials
+// of the input iterate at the input frames; with dxOut set, it also copies the
+// input dx there (the plain warm start a learned session measures its
+// correction from).
+constexpr int kSweep = 0, kLine = 1, kEval = 2, kLineStart = 3;
 
 // Batched scenes in the residual-targeted schedule (kBatched): a wave tile
 // holds at most two scenes (A: the scene of its first owned vertex, B: of its
@@ -367,7 +394,7 @@ __device__ __forceinline__ void sweep_body(const DevParams* __restrict__ P,
         const bool liveA = sa.ctl[scA].doneAt >= sweep;
         const bool liveB = sa.ctl[scB].doneAt >= sweep;
         if (!liveA && !liveB) {
-            if constexpr (kMode == kLine) {
+            if constexpr (kMode == kLine || kMode == kLineStart) {
                 if (in && lane >= 1 && lane <= 62) {
                     lo.g[v] = make_float4(0.f, 0.f, 0.f, 0.f);
                     lo.ql[v] = make_float2(0.0f, -1.0f);
@@ -402,6 +429,34 @@ __device__ __forceinline__ void sweep_body(const DevParams* __restrict__ P,
     const v3 x0 = xyz(p4);
     const v3 dx = xyz(d4);
 
+    // kLineStart: the segment's frame refreshed from the input dx, stored at
+    // once; lane 0 reads the frame and rest rotor before it from memory (that
+    // vertex is not in the wave).  dl = k (e1(qs) - e1(q)) is by how much the
+    // stretch force at the input frame exceeds the one at qs.
+    rot qs = asrot(q4);
+    v3 dl = v3{0.f, 0.f, 0.f};
+    if constexpr (kMode == kLineStart) {
+        if (in) qr4 = qRest[v];
+        rot qP = rot{__shfl_up(q4.x, 1, 64), __shfl_up(q4.y, 1, 64), __shfl_up(q4.z, 1, 64),
+                     __shfl_up(q4.w, 1, 64)};
+        rot qrP = rot{__shfl_up(qr4.x, 1, 64), __shfl_up(qr4.y, 1, 64), __shfl_up(qr4.z, 1, 64),
+                      __shfl_up(qr4.w, 1, 64)};
+        const rot qN = rot{__shfl_down(q4.x, 1, 64), __shfl_down(q4.y, 1, 64),
+                           __shfl_down(q4.z, 1, 64), __shfl_down(q4.w, 1, 64)};
+        if (lane == 0 && (flags & YB_FLAG_HAS_PREV)) {
+            qP = asrot(qIn[v - 1]);
+            qrP = asrot(qRest[v - 1]);
+        }
+        if (!(flags & YB_FLAG_FIX_ORIENTATION) && (flags & YB_FLAG_HAS_NEXT)) {
+            const v3 e = (xyz(pos[vc + 1]) - x0) + (xyz(dxIn[vc + 1]) - dx);
+            qs = frame_update(qs, e, -2.0f * r4.y * frcp(r4.x), flags, qP, qrP, qN, asrot(qr4));
+        }
+        if (in && lane >= 1 && lane <= 62 && live) {
+            qOut[v] = f4r(qs);
+            if (dxOut) dxOut[v] = d4;
+        }
+    }
+
     // Inertia (cosserat.cu:40-44); yv holds y - x.
     const float mh2 = frcp(p4.w * h * h);
     sym3 H = sym_diag(mh2);
@@ -424,8 +479,9 @@ __device__ __forceinline__ void sweep_body(const DevParams* __restrict__ P,
         // Cosserat stretch-shear (cosserat.cu:67-81).
         invl = frcp(r4.x);
         const v3 e = (p1 - x0) + (p1dx - dx);
-        const v3 c = e * invl - rotate_e1(asrot(q4));
+        const v3 c = e * invl - rotate_e1(qs);
         const float k = r4.y * invl;
+        if constexpr (kMode == kLineStart) dl = k * (rotate_e1(qs) - rotate_e1(asrot(q4)));
         float d = k * invl;
         f += k * c - (damping * d) * dx;
         f2 = -k * c - (damping * d) * p1dx;
@@ -510,7 +566,9 @@ __device__ __forceinline__ void sweep_body(const DevParams* __restrict__ P,
 
     // The rest rotor is needed only from here on: loading it after the contact
     // loop keeps it out of the loop's register budget.
-    if (in) qr4 = qRest[v];
+    if constexpr (kMode != kLineStart) {
+        if (in) qr4 = qRest[v];
+    }
 
     // Segment (i-1)'s endpoint-1 terms, frame and rest rotor (cosserat.cu:175-193).
     const float pf0 = __shfl_up(f2.x, 1, 64);
@@ -524,6 +582,9 @@ __device__ __forceinline__ void sweep_body(const DevParams* __restrict__ P,
                           __shfl_up(q4.w, 1, 64)};
     const rot qrPrev = rot{__shfl_up(qr4.x, 1, 64), __shfl_up(qr4.y, 1, 64),
                            __shfl_up(qr4.z, 1, 64), __shfl_up(qr4.w, 1, 64)};
+    v3 dlPrev = v3{0.f, 0.f, 0.f};
+    if constexpr (kMode == kLineStart)
+        dlPrev = v3{__shfl_up(dl.x, 1, 64), __shfl_up(dl.y, 1, 64), __shfl_up(dl.z, 1, 64)};
 
     // Newton step.
     v3 dxNew = xyz(d4);
@@ -542,10 +603,17 @@ __device__ __forceinline__ void sweep_body(const DevParams* __restrict__ P,
             // Local 3x3 solve (cosserat.cu:195-205).  The residual force is f,
             // or, where the step limit shortens the step, H (dx' - dx) /
             // accelerationRatio: the part of f the limit does not absorb, which
-            // vanishes at a clamped fixed point of the sweeps.
-            dxNew += P->accelerationRatio * sym_solve_fast(H, f);
+            // vanishes at a clamped fixed point of the sweeps.  kLineStart
+            // takes it at the input frames: only the stretch terms of the two
+            // segments at the vertex depend on the frames (H does not).
+            v3 fr = f;
+            if constexpr (kMode == kLineStart) {
+                fr += dl;
+                if (flags & YB_FLAG_HAS_PREV) fr -= dlPrev;
+            }
+            dxNew += P->accelerationRatio * sym_solve_fast(H, fr);
             const float l = len(dxNew);
-            v3 r = f;
+            v3 r = fr;
             if (l > limit && l > 0.0f) {
                 dxNew *= limit / l;
                 r = sym_apply(H, dxNew - xyz(d4)) * frcp(P->accelerationRatio);
@@ -565,7 +633,14 @@ __device__ __forceinline__ void sweep_body(const DevParams* __restrict__ P,
         dxNew = dx;
     }
 
-    if constexpr (kMode == kLine) {
+    if constexpr (kMode == kLineStart) {
+        if constexpr (kBatched)
+            wave_residual_scenes(0.0f, owner ? numF : 0.0f, sideA, partial, wave, n);
+        else
+            wave_residual(0.0f, owner ? numF : 0.0f, partial, wave, n);
+    }
+
+    if constexpr (kMode == kLine || kMode == kLineStart) {
         // Strand solve, tile part: the tile's owned vertices (lanes 1..62)
         // form a tridiagonal system along their strands -- on the diagonal a
         // scalar stiffness of each local Hessian, between neighbours the
@@ -627,22 +702,9 @@ __device__ __forceinline__ void sweep_body(const DevParams* __restrict__ P,
     if (owner && live) {
         if constexpr (kMode == kSweep) dxOut[v] = f4(dxNew, d4.w);
         rot q0 = asrot(q4);
-        if (!(flags & YB_FLAG_FIX_ORIENTATION) && (flags & YB_FLAG_HAS_NEXT)) {
-            const v3 e = (p1 - xyz(p4)) + (dxNext - dxNew);
-            const v3 vv = e * (-2.0f * r4.y * invl);
-            rot b = rot{0.f, 0.f, 0.f, 0.f};
-            if (flags & YB_FLAG_HAS_PREV) {
-                const float sg = rdot(rmul(rinv(qPrev), q0), qrPrev) > 0.0f ? 1.0f : -1.0f;
-                b = rscale(rmul(qPrev, qrPrev), sg);
-            }
-            if (flags & YB_FLAG_HAS_NEXT_ORIENTATION) {
-                const rot qr = asrot(qr4);
-                const float sg = rdot(rmul(rinv(q0), qNext), qr) > 0.0f ? 1.0f : -1.0f;
-                b = radd(b, rscale(rmul(qNext, rinv(qr)), sg));
-            }
-            const float lambda = len(vv) + rlen(b);
-            q0 = rnormalize_fast(radd(vb_e1(vv, b), rscale(b, lambda)));
-        }
+        if (!(flags & YB_FLAG_FIX_ORIENTATION) && (flags & YB_FLAG_HAS_NEXT))
+            q0 = frame_update(q0, (p1 - xyz(p4)) + (dxNext - dxNew), -2.0f * r4.y * invl, flags,
+                              qPrev, qrPrev, qNext, asrot(qr4));
         qOut[v] = f4r(q0);
     }
     if constexpr (kMode == kSweep) {
@@ -692,6 +754,27 @@ __global__ __launch_bounds__(256) void k_line_tile(const DevParams* __restrict__
     if (strand_skip(ctl, sweep)) return;
     sweep_body<kLine, kBatched>(P, pos, prm, yv, qIn, nullptr, qRest, dxIn, nullptr, numCols,
                                 cols, nullptr, sweep, nullptr, lo, sa);
+}
+
+// First strand step of a substep from the plain warm start, fused with the
+// frame refresh and the Newton-residual denominator (the line-start mode of
+// sweep_body): reads the frames qIn, writes the refreshed ones to qOut and,
+// with dxPlain set, a copy of dxIn there.
+template <bool kBatched>
+__global__ __launch_bounds__(256) void k_line_start(const DevParams* __restrict__ P,
+                                                    const float4* __restrict__ pos,
+                                                    const float4* __restrict__ prm,
+                                                    const float4* __restrict__ yv,
+                                                    const float4* __restrict__ qIn,
+                                                    float4* __restrict__ qOut,
+                                                    const float4* __restrict__ qRest,
+                                                    const float4* __restrict__ dxIn,
+                                                    float4* __restrict__ dxPlain,
+                                                    const int32_t* __restrict__ numCols,
+                                                    const int32_t* __restrict__ cols, LineOut lo,
+                                                    double2* __restrict__ partial, SceneArgs sa) {
+    sweep_body<kLineStart, kBatched>(P, pos, prm, yv, qIn, qOut, qRest, dxIn, dxPlain, numCols,
+                                     cols, nullptr, 0, partial, lo, sa);
 }
 
 __global__ __launch_bounds__(256) void k_sweep_list(
@@ -1512,6 +1595,8 @@ void launch_frame_refresh(yb_sim* s) {
                        s->b.q[0], s->n);
 }
 
+static void launch_strand_pairs(yb_sim* s, int buf, SolverCtl* ctl, int sweep, bool list);
+
 void launch_strand_solve(yb_sim* s, int buf, SolverCtl* ctl, int sweep, float* sys) {
     const int tiles = sweep_waves(s->n);
     const LineOut lo{s->b.lineG, s->b.lineQL, sys};
@@ -1531,6 +1616,13 @@ void launch_strand_solve(yb_sim* s, int buf, SolverCtl* ctl, int sweep, float* s
         hipLaunchKernelGGL(k_line_tile<false>, dim3(blocks(tiles * 64, 256)), dim3(256), 0,
                            s->stream, s->b.params, s->b.pos, s->b.prm, s->b.yv, s->b.q[buf],
                            s->b.qRest, s->b.dx[buf], s->b.numCols, s->b.cols, lo, ctl, sweep, sa);
+    launch_strand_pairs(s, buf, ctl, sweep, list);
+}
+
+// The boundary part of a strand step on dx[buf], and the apply.
+static void launch_strand_pairs(yb_sim* s, int buf, SolverCtl* ctl, int sweep, bool list) {
+    const int tiles = sweep_waves(s->n);
+    const SceneArgs sa = scene_args(s, ctl != nullptr);
     if (s->strandIters <= 2) {
         if (list)
             hipLaunchKernelGGL(k_line_pairs_apply_list, dim3(list_blocks(s)), dim3(256), 0,
@@ -1589,6 +1681,26 @@ void launch_residual_denom(yb_sim* s, bool frames) {
     if (frames)
         hipLaunchKernelGGL(k_copy4, dim3(blocks(s->n, 256)), dim3(256), 0, s->stream, s->b.q[1],
                            s->b.q[0], s->n);
+}
+
+void launch_line_start(yb_sim* s, bool keepPlain) {
+    double2* part = const_cast<double2*>(denom_partials(s));
+    float4* plain = keepPlain ? s->b.dxPlain : nullptr;
+    const LineOut lo{s->b.lineG, s->b.lineQL, nullptr};
+    const dim3 grid(blocks(sweep_waves(s->n) * 64, 256));
+    if (per_scene(s, true))
+        hipLaunchKernelGGL(k_line_start<true>, grid, dim3(256), 0, s->stream, s->b.params,
+                           s->b.pos, s->b.prm, s->b.yv, s->b.q[0], s->b.q[1], s->b.qRest,
+                           s->b.dx[0], plain, s->b.numCols, s->b.cols, lo, part,
+                           scene_args(s, true));
+    else
+        hipLaunchKernelGGL(k_line_start<false>, grid, dim3(256), 0, s->stream, s->b.params,
+                           s->b.pos, s->b.prm, s->b.yv, s->b.q[0], s->b.q[1], s->b.qRest,
+                           s->b.dx[0], plain, s->b.numCols, s->b.cols, lo, part,
+                           SceneArgs{nullptr, nullptr});
+    hipLaunchKernelGGL(k_copy4, dim3(blocks(s->n, 256)), dim3(256), 0, s->stream, s->b.q[1],
+                       s->b.q[0], s->n);
+    launch_strand_pairs(s, 0, nullptr, 0, false);
 }
 
 void launch_end(yb_sim* s, int fin, bool adaptive) {

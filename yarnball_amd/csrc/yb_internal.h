@@ -331,6 +331,12 @@ void launch_strand_solve(yb_sim* s, int buf = 0, SolverCtl* ctl = nullptr, int s
 // is changed), reduced by the substep's first residual check.  frames: also
 // refresh the frames from dx[0] (q[0] updated in place through q[1]).
 void launch_residual_denom(yb_sim* s, bool frames);
+// Both of the above in one pass over the tiles, for a substep whose sweeps
+// start from the plain warm start: refresh the frames from dx[0], take the
+// denominator's partials at the frames before the refresh, and make the
+// strand step at the refreshed ones.  keepPlain: also copy dx[0] to dxPlain
+// (in place of launch_features in a learned session's plain substep).
+void launch_line_start(yb_sim* s, bool keepPlain);
 const double2* denom_partials(const yb_sim* s);
 void launch_end(yb_sim* s, int fin, bool adaptive);         // iteration.cu:48-61
 SceneArgs scene_args(const yb_sim* s, bool adaptive);
