@@ -588,10 +588,11 @@ def main():
         "mean_contacts_per_segment": round(c, 3),
     }
     # Broadphase: the query (traversal 24 + 8p, filter 8p + 8k + 4 bytes per
-    # segment) and the refit (52 bytes per segment), p candidate leaves and k
-    # accepted contacts per segment as counted on the device.
+    # segment, and the step limit it now also writes: dx read 16, limit 4) and
+    # the refit (52 bytes per segment), p candidate leaves and k accepted
+    # contacts per segment as counted on the device.
     broad = {}
-    for st_, b_seg in (("query", 28.0 + 16.0 * p_cand + 8.0 * k_acc), ("refit", 52.0)):
+    for st_, b_seg in (("query", 48.0 + 16.0 * p_cand + 8.0 * k_acc), ("refit", 52.0)):
         i = STAGES.index(st_)
         if not nl[i]:
             continue

@@ -650,11 +650,12 @@ static void enqueue_prologue(yb_sim* s, int mode, bool adaptive, int warm) {
         }
         launch_refit(s);
         mark(s, YB_STAGE_REFIT);
-        launch_query(s);
+        launch_query(s);  // also the step limit, from the distances it measures
         mark(s, YB_STAGE_QUERY);
+    } else {
+        launch_step_limit(s);
+        mark(s, YB_STAGE_STEP_LIMIT);
     }
-    launch_step_limit(s);
-    mark(s, YB_STAGE_STEP_LIMIT);
     // Residual-targeted mode: the Newton residual's denominator is the force
     // at the plain warm start (now in dx[0], with the previous substep's
     // frames) for every warm start, so that all of them are held to the same

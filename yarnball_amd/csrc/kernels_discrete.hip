@@ -788,9 +788,12 @@ struct SegInfo {
     bool glued;
 };
 
+// d2: the squared distance of the two segments' closest points when the pair
+// reaches the distance test.
 __device__ __forceinline__ bool pair_accepted(const DevParams* __restrict__ P,
                                               const float4* __restrict__ prm, int i,
-                                              const SegInfo& si, int j, const SegInfo& sj) {
+                                              const SegInfo& si, int j, const SegInfo& sj,
+                                              float& d2) {
     const bool iFirst = i < j;
     const int x = iFirst ? i : j;
     const int y = iFirst ? j : i;
@@ -809,7 +812,7 @@ __device__ __forceinline__ bool pair_accepted(const DevParams* __restrict__ P,
     float u, w;
     closest_uv(v3{0.f, 0.f, 0.f}, a1, b0, b1, u, w);
     const v3 nrm = u * a1 - mixv(b0, b1, w);
-    const float d2 = len2(nrm);
+    d2 = len2(nrm);
     float r = 2.0f * P->scaledDetectionRadius;
     if (P->useVelocityRadius) {
         r += sx.vpad;
@@ -822,6 +825,12 @@ __device__ __forceinline__ bool pair_accepted(const DevParams* __restrict__ P,
 // scratch memory); kCand: candidate leaves buffered per thread before the
 // narrowphase.  Both size the LDS footprint, which sets how many waves a CU
 // holds: (kStackL + 2 kCand) x 4 B per thread.
+// With dxLimit set, the query also does the step limit's work (collision.cu:
+// 121-159) from the distances its pair tests have just measured: the limit of
+// each vertex into dxLimit[i].w (dxLimit holds the warm start), the
+// interpenetration warning into err[1] and the smallest separation into
+// *minSep -- the contact lists are the pairs it accepts, so these are the
+// distances the step limit kernel would measure again.
 template <int kQueryThreads, int kStackL, int kCand>
 __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __restrict__ P,
                                                          const int32_t* __restrict__ order,
@@ -835,7 +844,9 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
                                                          const int32_t* __restrict__ sceneOf,
                                                          const int32_t* __restrict__ sceneRoot,
                                                          int stackLimit,
-                                                         int32_t* __restrict__ candCount) {
+                                                         int32_t* __restrict__ candCount,
+                                                         float4* __restrict__ dxLimit,
+                                                         float* __restrict__ minSep) {
     __shared__ int32_t stk[kStackL][kQueryThreads];
     __shared__ int32_t candJ[kCand][kQueryThreads];
     __shared__ float candPad[kCand][kQueryThreads];
@@ -846,138 +857,158 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
     // range XCD by XCD (xcd_tile), so the tree leaves and partner segments
     // that one stretch of strands visits stay in one L2.
     const int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    if (!(__float_as_uint(prm[i].w) & YB_FLAG_HAS_NEXT)) {
-        numCols[i] = 0;
-        if (candCount) {
-            candCount[i] = 0;
-            candCount[n + i] = 0;
-            candCount[2 * n + i] = 0;
-        }
-        return;
-    }
-    const float4 qlo4 = aabb[2 * i];
-    const float4 qhi4 = aabb[2 * i + 1];
-    const v3 qlo = xyz(qlo4);
-    const v3 qhi = xyz(qhi4);
-    SegInfo si;
-    si.p0 = xyz(pos[i]);
-    si.p1 = xyz(pos[i + 1]);
-    si.vpad = fabsf(qhi4.w);
-    si.glued = signbit(qhi4.w);
-    const int t = threadIdx.x;
-    int sp = 0;
-    volatile int32_t deep[kStack > kStackL ? kStack - kStackL : 1];  // entries past the LDS part
-    // batched scenes: traverse only this segment's own scene subtree
-    int node = sceneOf ? sceneRoot[sceneOf[i]] : 0;
-    int cnt = 0;
-    bool overflowStack = false;
-    // The traversal only collects overlapping leaves, in visit order; the
-    // exact pair test runs over the buffered candidates when some thread of
-    // the wave has nearly filled its buffer, so the wave evaluates pair tests
-    // together instead of once per node visit at which any thread met a leaf.
-    // Candidates keep their visit order, hence so do the contact lists.
-    int nb = 0;
-    // measurement only: leaves more than two indices away whose boxes overlap
-    // the query box, node visits, deepest stack
-    int cand = 0, visits = 0, maxSp = 0;
-    // The first four partners stay in registers and are stored at the end,
-    // slot by slot, by the whole wave at once (one coalesced 256-byte store
-    // per slot); storing each partner when it is found left every slot's lines
-    // partly written at different times, and the cache wrote them back several
-    // times over.  Later partners are stored as they are found.
-    int32_t r0 = 0, r1 = 0, r2 = 0, r3 = 0;
-    auto narrowphase = [&]() {
-        cand += nb;
-#pragma unroll 1
-        for (int k = 0; k < nb; k++) {
-            const int j = candJ[k][t];
-            const float pw = candPad[k][t];
-            SegInfo sj;
-            sj.p0 = xyz(gather4(pos, j));
-            sj.p1 = xyz(gather4<true>(pos, j));
-            sj.vpad = fabsf(pw);
-            sj.glued = signbit(pw);
-            if (pair_accepted(P, prm, i, si, j, sj)) {
-                r0 = cnt == 0 ? j : r0;
-                r1 = cnt == 1 ? j : r1;
-                r2 = cnt == 2 ? j : r2;
-                r3 = cnt == 3 ? j : r3;
-                if (cnt >= 4 && cnt < kMaxCols) cols[i + (size_t)n * cnt] = j;
-                cnt++;
+    const bool seg = i < n && (__float_as_uint(prm[min(i, n - 1)].w) & YB_FLAG_HAS_NEXT);
+    float minD2 = INFINITY;  // closest listed contact, squared
+    if (!seg) {
+        if (i < n) {
+            numCols[i] = 0;
+            if (candCount) {
+                candCount[i] = 0;
+                candCount[n + i] = 0;
+                candCount[2 * n + i] = 0;
             }
         }
-        nb = 0;
-    };
-    while (true) {
+    } else {
+        const float4 qlo4 = aabb[2 * i];
+        const float4 qhi4 = aabb[2 * i + 1];
+        const v3 qlo = xyz(qlo4);
+        const v3 qhi = xyz(qhi4);
+        SegInfo si;
+        si.p0 = xyz(pos[i]);
+        si.p1 = xyz(pos[i + 1]);
+        si.vpad = fabsf(qhi4.w);
+        si.glued = signbit(qhi4.w);
+        const int t = threadIdx.x;
+        int sp = 0;
+        volatile int32_t deep[kStack > kStackL ? kStack - kStackL : 1];  // entries past the LDS part
+        // batched scenes: traverse only this segment's own scene subtree
+        int node = sceneOf ? sceneRoot[sceneOf[i]] : 0;
+        int cnt = 0;
+        bool overflowStack = false;
+        // The traversal only collects overlapping leaves, in visit order; the
+        // exact pair test runs over the buffered candidates when some thread of
+        // the wave has nearly filled its buffer, so the wave evaluates pair tests
+        // together instead of once per node visit at which any thread met a leaf.
+        // Candidates keep their visit order, hence so do the contact lists.
+        int nb = 0;
+        // measurement only: leaves more than two indices away whose boxes overlap
+        // the query box, node visits, deepest stack
+        int cand = 0, visits = 0, maxSp = 0;
+        // The first four partners stay in registers and are stored at the end,
+        // slot by slot, by the whole wave at once (one coalesced 256-byte store
+        // per slot); storing each partner when it is found left every slot's lines
+        // partly written at different times, and the cache wrote them back several
+        // times over.  Later partners are stored as they are found.
+        int32_t r0 = 0, r1 = 0, r2 = 0, r3 = 0;
+        auto narrowphase = [&]() {
+            cand += nb;
+    #pragma unroll 1
+            for (int k = 0; k < nb; k++) {
+                const int j = candJ[k][t];
+                const float pw = candPad[k][t];
+                SegInfo sj;
+                sj.p0 = xyz(gather4(pos, j));
+                sj.p1 = xyz(gather4<true>(pos, j));
+                sj.vpad = fabsf(pw);
+                sj.glued = signbit(pw);
+                float d2;
+                if (pair_accepted(P, prm, i, si, j, sj, d2)) {
+                    minD2 = fminf(minD2, d2);
+                    r0 = cnt == 0 ? j : r0;
+                    r1 = cnt == 1 ? j : r1;
+                    r2 = cnt == 2 ? j : r2;
+                    r3 = cnt == 3 ? j : r3;
+                    if (cnt >= 4 && cnt < kMaxCols) cols[i + (size_t)n * cnt] = j;
+                    cnt++;
+                }
+            }
+            nb = 0;
+        };
+        while (true) {
+            if (candCount) {
+                visits++;
+                maxSp = max(maxSp, sp);
+            }
+            const float4 l0 = nodes[4 * node + 0];
+            const float4 l1 = nodes[4 * node + 1];
+            const float4 r0 = nodes[4 * node + 2];
+            const float4 r1 = nodes[4 * node + 3];
+            const int cl = __float_as_int(l0.w);
+            const int cr = __float_as_int(r0.w);
+            const bool hl = overlap(qlo, qhi, l0, l1);
+            const bool hr = overlap(qlo, qhi, r0, r1);
+            // leaves within two indices of i (itself and its strand neighbours,
+            // which always overlap) are exempt from the pair test (collision.cu:54):
+            // they are dropped here, before their positions are gathered.  Both
+            // leaves are written without a branch: slot nb and the one after it
+            // are free (nb <= kCand - 2 at every visit), and nb advances only for
+            // an accepted leaf.
+            const bool addL = hl && cl < 0 && abs(~cl - i) > 2;
+            const bool addR = hr && cr < 0 && abs(~cr - i) > 2;
+            candJ[nb][t] = ~cl;
+            candPad[nb][t] = l1.w;
+            nb += addL ? 1 : 0;
+            candJ[nb][t] = ~cr;
+            candPad[nb][t] = r1.w;
+            nb += addR ? 1 : 0;
+            // at most two leaves per visit: flush before a buffer could overflow
+            if (__any(nb > kCand - 2)) narrowphase();
+            const bool tl = hl && cl >= 0;
+            const bool tr = hr && cr >= 0;
+            const bool descend = tl || tr;
+            if (!descend && sp == 0) break;
+            // Next node: descend left first (the right child is pushed when both
+            // overlap), else pop.  stackLimit <= kStack (host-checked); tests lower
+            // it to force the overflow path.  The LDS part of the stack is used
+            // without branches: the slot above the top is free, so the write is
+            // harmless when nothing is pushed, and the pop reads the top entry.
+            const bool both = tl && tr;
+            const bool push = both && sp < stackLimit;
+            overflowStack = overflowStack || (both && !push);
+            if (sp < kStackL) stk[sp][t] = cr;
+            const int popped = stk[min(max(sp - 1, 0), kStackL - 1)][t];
+            if (push && sp >= kStackL) deep[sp - kStackL] = cr;
+            node = descend ? (tl ? cl : cr) : popped;
+            sp = descend ? sp + (push ? 1 : 0) : sp - 1;
+            if (!descend && sp >= kStackL) {
+                // keeps the compiler from merging the two pops into one generic
+                // (flat) load, which would make every pop a slow volatile one
+                asm volatile("" ::: "memory");
+                node = deep[sp - kStackL];
+            }
+        }
+        narrowphase();
+        if (cnt > 0) cols[i] = r0;
+        if (cnt > 1) cols[i + (size_t)n] = r1;
+        if (cnt > 2) cols[i + 2 * (size_t)n] = r2;
+        if (cnt > 3) cols[i + 3 * (size_t)n] = r3;
+        // err[0] collects one bit per failure kind (kErrBit*), read by the host
+        if (cnt > kMaxCols) {
+            atomicOr(&err[0], kErrBitContacts);
+            cnt = kMaxCols;
+        }
+        if (overflowStack) atomicOr(&err[0], kErrBitStack);
+        numCols[i] = cnt;
         if (candCount) {
-            visits++;
-            maxSp = max(maxSp, sp);
-        }
-        const float4 l0 = nodes[4 * node + 0];
-        const float4 l1 = nodes[4 * node + 1];
-        const float4 r0 = nodes[4 * node + 2];
-        const float4 r1 = nodes[4 * node + 3];
-        const int cl = __float_as_int(l0.w);
-        const int cr = __float_as_int(r0.w);
-        const bool hl = overlap(qlo, qhi, l0, l1);
-        const bool hr = overlap(qlo, qhi, r0, r1);
-        // leaves within two indices of i (itself and its strand neighbours,
-        // which always overlap) are exempt from the pair test (collision.cu:54):
-        // they are dropped here, before their positions are gathered.  Both
-        // leaves are written without a branch: slot nb and the one after it
-        // are free (nb <= kCand - 2 at every visit), and nb advances only for
-        // an accepted leaf.
-        const bool addL = hl && cl < 0 && abs(~cl - i) > 2;
-        const bool addR = hr && cr < 0 && abs(~cr - i) > 2;
-        candJ[nb][t] = ~cl;
-        candPad[nb][t] = l1.w;
-        nb += addL ? 1 : 0;
-        candJ[nb][t] = ~cr;
-        candPad[nb][t] = r1.w;
-        nb += addR ? 1 : 0;
-        // at most two leaves per visit: flush before a buffer could overflow
-        if (__any(nb > kCand - 2)) narrowphase();
-        const bool tl = hl && cl >= 0;
-        const bool tr = hr && cr >= 0;
-        const bool descend = tl || tr;
-        if (!descend && sp == 0) break;
-        // Next node: descend left first (the right child is pushed when both
-        // overlap), else pop.  stackLimit <= kStack (host-checked); tests lower
-        // it to force the overflow path.  The LDS part of the stack is used
-        // without branches: the slot above the top is free, so the write is
-        // harmless when nothing is pushed, and the pop reads the top entry.
-        const bool both = tl && tr;
-        const bool push = both && sp < stackLimit;
-        overflowStack = overflowStack || (both && !push);
-        if (sp < kStackL) stk[sp][t] = cr;
-        const int popped = stk[min(max(sp - 1, 0), kStackL - 1)][t];
-        if (push && sp >= kStackL) deep[sp - kStackL] = cr;
-        node = descend ? (tl ? cl : cr) : popped;
-        sp = descend ? sp + (push ? 1 : 0) : sp - 1;
-        if (!descend && sp >= kStackL) {
-            // keeps the compiler from merging the two pops into one generic
-            // (flat) load, which would make every pop a slow volatile one
-            asm volatile("" ::: "memory");
-            node = deep[sp - kStackL];
+            candCount[i] = cand;
+            candCount[n + i] = visits;
+            candCount[2 * n + i] = maxSp;
         }
     }
-    narrowphase();
-    if (cnt > 0) cols[i] = r0;
-    if (cnt > 1) cols[i + (size_t)n] = r1;
-    if (cnt > 2) cols[i + 2 * (size_t)n] = r2;
-    if (cnt > 3) cols[i + 3 * (size_t)n] = r3;
-    // err[0] collects one bit per failure kind (kErrBit*), read by the host
-    if (cnt > kMaxCols) {
-        atomicOr(&err[0], kErrBitContacts);
-        cnt = kMaxCols;
-    }
-    if (overflowStack) atomicOr(&err[0], kErrBitStack);
-    numCols[i] = cnt;
-    if (candCount) {
-        candCount[i] = cand;
-        candCount[n + i] = visits;
-        candCount[2 * n + i] = maxSp;
+    if (dxLimit) {
+        // the step limit (collision.cu:121-159), every lane of the wave
+        const float minContact = sqrtf(minD2);
+        if (i < n) {
+            float minDist = INFINITY;
+            if (seg && P->useStepSizeLimit) {
+                minDist = P->detectionRadius * (P->detectionScaler - 1.0f);
+                if (P->useVelocityRadius) minDist += len(xyz(dxLimit[i]));
+                minDist = fminf(minDist, 0.45f * minContact);
+            }
+            dxLimit[i].w = minDist;
+            if (minContact < 2.0f * P->radius) err[1] = YB_WARNING_SEGMENT_INTERPENETRATION;
+        }
+        reduce_min_sep(minContact / (2.0f * P->radius), minSep);
     }
 }
 
@@ -1093,7 +1124,8 @@ static void launch_query_t(yb_sim* s) {
     hipLaunchKernelGGL((k_query<T, SL, C>), dim3(blocks(s->n, T)), dim3(T), 0, s->stream, s->b.params,
                        s->b.order, s->b.aabb, s->b.nodes, s->b.pos, s->b.prm, s->b.numCols,
                        s->b.cols, s->b.err, s->numScenes > 1 ? s->b.sceneOf : nullptr,
-                       s->b.sceneRoot, s->queryStack, s->countCandidates ? s->b.candCount : nullptr);
+                       s->b.sceneRoot, s->queryStack, s->countCandidates ? s->b.candCount : nullptr,
+                       s->b.dx[0], s->b.minSep);
 }
 
 // Query shapes (LDS stack entries, buffered candidates) the autotuner picks

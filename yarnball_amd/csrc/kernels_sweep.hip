@@ -90,16 +90,10 @@ __device__ __forceinline__ v3 sym_solve_fast(const sym3& m, v3 f) {
 // distance to every listed contact.  The same contact distances also feed the
 // interpenetration check the reference declares but never sets (YarnBall.h:121):
 // a listed pair closer than 2 * radius raises WARNING_SEGMENT_INTERPENETRATION,
-// and the smallest distance / (2 * radius) seen is kept in *minSep.
-__device__ __forceinline__ void reduce_min_sep(float sep, float* minSep) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) sep = fminf(sep, __shfl_xor(sep, off, 64));
-    // Positive floats order like their bit patterns.  Only a wave that beats
-    // the running minimum issues the atomic, so after the first substep almost
-    // none do.
-    if ((threadIdx.x & 63) == 0 && sep < INFINITY && sep < *(volatile float*)minSep)
-        atomicMin(reinterpret_cast<int*>(minSep), __float_as_int(sep));
-}
+// and the smallest distance / (2 * radius) seen is kept in *minSep.  A substep
+// that detects contacts gets all of this from the query, which has just
+// measured the same distances; this kernel serves the substeps that reuse
+// their lists.
 
 template <bool WRITE_LIMIT>
 __global__ __launch_bounds__(256) void k_step_limit(const DevParams* __restrict__ P,

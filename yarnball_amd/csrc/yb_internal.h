@@ -105,6 +105,18 @@ __device__ __forceinline__ float4 gather4(const float4* __restrict__ a, int i) {
                                             (next ? 16 : 0));
 }
 
+// The smallest contact distance / (2 * radius) over a wave, folded into the
+// running minimum *minSep (the interpenetration monitor of the step limit).
+// Positive floats order like their bit patterns.  Only a wave that beats the
+// running minimum issues the atomic, so after the first substep almost none
+// do.  Every lane of the wave must take part.
+__device__ __forceinline__ void reduce_min_sep(float sep, float* minSep) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) sep = fminf(sep, __shfl_xor(sep, off, 64));
+    if ((threadIdx.x & 63) == 0 && sep < INFINITY && sep < *(volatile float*)minSep)
+        atomicMin(reinterpret_cast<int*>(minSep), __float_as_int(sep));
+}
+
 // Learned warm start: parameter vector layout (floats), see yarnball_hip.h.
 constexpr int kMlpIn = YB_MLP_INPUTS;
 constexpr int kMlpHid = YB_MLP_HIDDEN;
@@ -354,7 +366,7 @@ void launch_fill_i32(yb_sim* s, int32_t* p, int32_t v, int count);
 void launch_aabb(yb_sim* s);
 void launch_rebuild(yb_sim* s);    // scene box, Morton, sort, Karras
 void launch_refit(yb_sim* s);
-void launch_query(yb_sim* s);      // traversal + narrowphase -> contact lists
+void launch_query(yb_sim* s);      // traversal + narrowphase -> contact lists, step limit
 bool query_shape_supported(int stackLds, int candidates);
 
 }  // namespace yb
