@@ -847,11 +847,9 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
                                                          int32_t* __restrict__ candCount,
                                                          float4* __restrict__ dxLimit,
                                                          float* __restrict__ minSep) {
-    static_assert(kCand >= 2 && kCand <= 15, "a buffer count must fit four bits");
-    constexpr int kCandBits = kCand < 4 ? 2 : (kCand < 8 ? 3 : 4);
     __shared__ int32_t stk[kStackL][kQueryThreads];
-    __shared__ int32_t candJ[kCand][kQueryThreads];  // each thread's buffered leaves
-    __shared__ int32_t work[kCand][kQueryThreads];   // the wave's pair list, then its results
+    __shared__ int32_t candJ[kCand][kQueryThreads];
+    __shared__ float candPad[kCand][kQueryThreads];
     const int n = P->n;
     // Segments in index order: consecutive segments of a strand are spatial
     // neighbours (coherent traversals) and the slot-major list writes
@@ -859,95 +857,63 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
     // range XCD by XCD (xcd_tile), so the tree leaves and partner segments
     // that one stretch of strands visits stay in one L2.
     const int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    const int t = threadIdx.x;
-    const int lane = t & 63;
-    const int col0 = t - lane;  // this wave's first column in the LDS arrays
-    const int ic = min(i, n - 1);
-    const bool seg = i < n && (__float_as_uint(prm[ic].w) & YB_FLAG_HAS_NEXT);
-    const float4 qlo4 = aabb[2 * ic];
-    const float4 qhi4 = aabb[2 * ic + 1];
-    const v3 qlo = xyz(qlo4);
-    const v3 qhi = xyz(qhi4);
-    int sp = 0;
-    volatile int32_t deep[kStack > kStackL ? kStack - kStackL : 1];  // entries past the LDS part
-    // batched scenes: traverse only this segment's own scene subtree
-    int node = sceneOf ? sceneRoot[sceneOf[ic]] : 0;
-    int cnt = 0;
-    bool overflowStack = false;
+    const bool seg = i < n && (__float_as_uint(prm[min(i, n - 1)].w) & YB_FLAG_HAS_NEXT);
     float minD2 = INFINITY;  // closest listed contact, squared
-    // The traversal only collects overlapping leaves, in visit order.  When
-    // some thread of the wave has nearly filled its buffer, the wave tests
-    // the buffered pairs together: the pairs of all its threads are laid out
-    // in one list (thread by thread, each in visit order), every lane tests
-    // one pair of the list at a time, and each thread then takes its own
-    // results in its visit order.  Testing each thread's pairs in its own lane
-    // left most lanes idle while the busiest ran (about 2.2 tests per pair).
-    // The contact lists keep their order.
-    int nb = 0;
-    // measurement only: leaves more than two indices away whose boxes overlap
-    // the query box, node visits, deepest stack
-    int cand = 0, visits = 0, maxSp = 0;
-    // The first four partners stay in registers and are stored at the end,
-    // slot by slot, by the whole wave at once (one coalesced 256-byte store
-    // per slot); storing each partner when it is found left every slot's lines
-    // partly written at different times, and the cache wrote them back several
-    // times over.  Later partners are stored as they are found.
-    int32_t r0 = 0, r1 = 0, r2 = 0, r3 = 0;
-    // Called by every lane of the wave.
-    auto narrowphase = [&]() {
-        cand += nb;
-        // exclusive prefix of the buffer counts over the wave, and their total
-        int pre = 0, total = 0;
-#pragma unroll
-        for (int b = 0; b < kCandBits; b++) {
-            const uint64_t m = __ballot((nb >> b) & 1);
-            pre += (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
-                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))
-                   << b;
-            total += __popcll(m) << b;
-        }
-        // entry g of the wave's list lives at work[g / 64][col0 + g % 64] and
-        // names its owner lane o and buffer slot k as o + 64 k
-#pragma unroll
-        for (int k = 0; k < kCand; k++) {
-            if (k < nb) {
-                const int g = pre + k;
-                work[g >> 6][col0 + (g & 63)] = lane + (k << 6);
+    if (!seg) {
+        if (i < n) {
+            numCols[i] = 0;
+            if (candCount) {
+                candCount[i] = 0;
+                candCount[n + i] = 0;
+                candCount[2 * n + i] = 0;
             }
         }
-        __builtin_amdgcn_wave_barrier();
-        for (int base = 0; base < total; base += 64) {
-            const int g = base + lane;  // this lane's entry
-            if (g < total) {
-                int32_t* e = &work[g >> 6][col0 + (g & 63)];
-                const int ok = *e;
-                const int o = ok & 63;
-                const int j = candJ[ok >> 6][col0 + o];
-                const int io = i - lane + o;  // the owner's segment
-                const float pwo = aabb[2 * io + 1].w;
-                const float pwj = aabb[2 * j + 1].w;
-                SegInfo so, sj;
-                so.p0 = xyz(pos[io]);
-                so.p1 = xyz(pos[io + 1]);
-                so.vpad = fabsf(pwo);
-                so.glued = signbit(pwo);
+    } else {
+        const float4 qlo4 = aabb[2 * i];
+        const float4 qhi4 = aabb[2 * i + 1];
+        const v3 qlo = xyz(qlo4);
+        const v3 qhi = xyz(qhi4);
+        SegInfo si;
+        si.p0 = xyz(pos[i]);
+        si.p1 = xyz(pos[i + 1]);
+        si.vpad = fabsf(qhi4.w);
+        si.glued = signbit(qhi4.w);
+        const int t = threadIdx.x;
+        int sp = 0;
+        volatile int32_t deep[kStack > kStackL ? kStack - kStackL : 1];  // entries past the LDS part
+        // batched scenes: traverse only this segment's own scene subtree
+        int node = sceneOf ? sceneRoot[sceneOf[i]] : 0;
+        int cnt = 0;
+        bool overflowStack = false;
+        // The traversal only collects overlapping leaves, in visit order; the
+        // exact pair test runs over the buffered candidates when some thread of
+        // the wave has nearly filled its buffer, so the wave evaluates pair tests
+        // together instead of once per node visit at which any thread met a leaf.
+        // Candidates keep their visit order, hence so do the contact lists.
+        int nb = 0;
+        // measurement only: leaves more than two indices away whose boxes overlap
+        // the query box, node visits, deepest stack
+        int cand = 0, visits = 0, maxSp = 0;
+        // The first four partners stay in registers and are stored at the end,
+        // slot by slot, by the whole wave at once (one coalesced 256-byte store
+        // per slot); storing each partner when it is found left every slot's lines
+        // partly written at different times, and the cache wrote them back several
+        // times over.  Later partners are stored as they are found.
+        int32_t r0 = 0, r1 = 0, r2 = 0, r3 = 0;
+        auto narrowphase = [&]() {
+            cand += nb;
+    #pragma unroll 1
+            for (int k = 0; k < nb; k++) {
+                const int j = candJ[k][t];
+                const float pw = candPad[k][t];
+                SegInfo sj;
                 sj.p0 = xyz(gather4(pos, j));
                 sj.p1 = xyz(gather4<true>(pos, j));
-                sj.vpad = fabsf(pwj);
-                sj.glued = signbit(pwj);
+                sj.vpad = fabsf(pw);
+                sj.glued = signbit(pw);
                 float d2;
-                *e = pair_accepted(P, prm, io, so, j, sj, d2) ? __float_as_int(d2) : -1;
-            }
-        }
-        __builtin_amdgcn_wave_barrier();
-#pragma unroll
-        for (int k = 0; k < kCand; k++) {
-            if (k < nb) {
-                const int g = pre + k;
-                const int res = work[g >> 6][col0 + (g & 63)];
-                if (res >= 0) {
-                    const int j = candJ[k][t];
-                    minD2 = fminf(minD2, __int_as_float(res));
+                if (pair_accepted(P, prm, i, si, j, sj, d2)) {
+                    minD2 = fminf(minD2, d2);
                     r0 = cnt == 0 ? j : r0;
                     r1 = cnt == 1 ? j : r1;
                     r2 = cnt == 2 ? j : r2;
@@ -956,15 +922,9 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
                     cnt++;
                 }
             }
-        }
-        __builtin_amdgcn_wave_barrier();
-        nb = 0;
-    };
-    // Every lane stays in the loop until the whole wave is done (a finished
-    // lane idles), so that each narrowphase has the full wave.
-    bool done = !seg;
-    while (true) {
-        if (!done) {
+            nb = 0;
+        };
+        while (true) {
             if (candCount) {
                 visits++;
                 maxSp = max(maxSp, sp);
@@ -986,44 +946,38 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
             const bool addL = hl && cl < 0 && abs(~cl - i) > 2;
             const bool addR = hr && cr < 0 && abs(~cr - i) > 2;
             candJ[nb][t] = ~cl;
+            candPad[nb][t] = l1.w;
             nb += addL ? 1 : 0;
             candJ[nb][t] = ~cr;
+            candPad[nb][t] = r1.w;
             nb += addR ? 1 : 0;
+            // at most two leaves per visit: flush before a buffer could overflow
+            if (__any(nb > kCand - 2)) narrowphase();
             const bool tl = hl && cl >= 0;
             const bool tr = hr && cr >= 0;
             const bool descend = tl || tr;
-            if (!descend && sp == 0) {
-                done = true;
-            } else {
-                // Next node: descend left first (the right child is pushed when
-                // both overlap), else pop.  stackLimit <= kStack (host-checked);
-                // tests lower it to force the overflow path.  The LDS part of
-                // the stack is used without branches: the slot above the top is
-                // free, so the write is harmless when nothing is pushed, and the
-                // pop reads the top entry.
-                const bool both = tl && tr;
-                const bool push = both && sp < stackLimit;
-                overflowStack = overflowStack || (both && !push);
-                if (sp < kStackL) stk[sp][t] = cr;
-                const int popped = stk[min(max(sp - 1, 0), kStackL - 1)][t];
-                if (push && sp >= kStackL) deep[sp - kStackL] = cr;
-                node = descend ? (tl ? cl : cr) : popped;
-                sp = descend ? sp + (push ? 1 : 0) : sp - 1;
-                if (!descend && sp >= kStackL) {
-                    // keeps the compiler from merging the two pops into one
-                    // generic (flat) load, which would make every pop a slow
-                    // volatile one
-                    asm volatile("" ::: "memory");
-                    node = deep[sp - kStackL];
-                }
+            if (!descend && sp == 0) break;
+            // Next node: descend left first (the right child is pushed when both
+            // overlap), else pop.  stackLimit <= kStack (host-checked); tests lower
+            // it to force the overflow path.  The LDS part of the stack is used
+            // without branches: the slot above the top is free, so the write is
+            // harmless when nothing is pushed, and the pop reads the top entry.
+            const bool both = tl && tr;
+            const bool push = both && sp < stackLimit;
+            overflowStack = overflowStack || (both && !push);
+            if (sp < kStackL) stk[sp][t] = cr;
+            const int popped = stk[min(max(sp - 1, 0), kStackL - 1)][t];
+            if (push && sp >= kStackL) deep[sp - kStackL] = cr;
+            node = descend ? (tl ? cl : cr) : popped;
+            sp = descend ? sp + (push ? 1 : 0) : sp - 1;
+            if (!descend && sp >= kStackL) {
+                // keeps the compiler from merging the two pops into one generic
+                // (flat) load, which would make every pop a slow volatile one
+                asm volatile("" ::: "memory");
+                node = deep[sp - kStackL];
             }
         }
-        // at most two leaves per visit: flush before a buffer could overflow
-        if (__any(nb > kCand - 2)) narrowphase();
-        if (__all(done)) break;
-    }
-    narrowphase();
-    if (i < n) {
+        narrowphase();
         if (cnt > 0) cols[i] = r0;
         if (cnt > 1) cols[i + (size_t)n] = r1;
         if (cnt > 2) cols[i + 2 * (size_t)n] = r2;
