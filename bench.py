@@ -59,12 +59,13 @@ def stage_bytes(stage, c):
 
 
 # HBM bytes per vertex of one launch, from rocprofv3 FETCH_SIZE and WRITE_SIZE
-# scaled by a copy kernel's counts: k_sweep and k_query on the knit patch
-# (256 x 4096, residual 1e-6, learned + strand, c = 4.47 contacts per segment;
-# profiles/substep_knit1M_hbm_traffic.md) and k_sweep on the 512-scene batch
-# (8 sweeps, c = 0.24; profiles/sweep_batched512_hbm_traffic.md).
-MEASURED_TRAFFIC = {("knit_patch", "sweep"): 160.3, ("batched_walks", "sweep"): 131.3,
-                    ("knit_patch", "query"): 329.3}
+# scaled by a copy kernel's counts: k_sweep and k_query (with the step limit it
+# computes) on the knit patch (256 x 4096, residual 1e-6, learned + strand,
+# frames 5-6, c = 4.58 contacts per segment; profiles/substep_knit1M_hbm_traffic.md)
+# and k_sweep on the 512-scene batch (8 sweeps, c = 0.24;
+# profiles/sweep_batched512_hbm_traffic.md).
+MEASURED_TRAFFIC = {("knit_patch", "sweep"): 160.5, ("batched_walks", "sweep"): 131.3,
+                    ("knit_patch", "query"): 397.3}
 
 
 def build_workload(name, numItr):
