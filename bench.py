@@ -159,6 +159,8 @@ def measure_batched(args, steps, warmup, per_scene_steps, with_cpu):
         b.setLaunchConfig(*launch_config)
     if tile_config:
         b.setTileConfig(*tile_config)
+    if args.skin:
+        b.setContactSkin(*args.skin)
     for w in range(warmup):
         b.advance(frame)
     t0 = time.perf_counter()
@@ -390,6 +392,8 @@ def run_method(sc, args, method, model, strand_model=None):
         sim.setLaunchConfig(*args.launch_config)
     if args.tile_config:
         sim.setTileConfig(*args.tile_config)
+    if args.skin:
+        sim.setContactSkin(*args.skin)
     frame = np.float32(sc.frame_dt)
     substeps_per_frame = int(np.ceil(frame / np.float32(sc.maxH)))
     for w in range(args.warmup):
@@ -401,6 +405,7 @@ def run_method(sc, args, method, model, strand_model=None):
         learned.warm_substeps(sim, reset=True)
         learned.counters(sim, reset=True)
     sim.minSeparation(reset=True)
+    sim.contactSkinStats(reset=True)
     t0 = time.perf_counter()
     for k in range(args.steps):
         sim.advance(frame)  # synchronous: returns after the stream has drained
@@ -408,7 +413,10 @@ def run_method(sc, args, method, model, strand_model=None):
     elapsed = time.perf_counter() - t0
     r = {"sim": sim, "elapsed": elapsed, "substeps_per_frame": substeps_per_frame,
          "value": sc.n * substeps_per_frame * args.steps / elapsed,
-         "min_separation": sim.minSeparation()}
+         "min_separation": sim.minSeparation(), "skin": sim.contactSkinStats(reset=True),
+         "sim_skin": sim.contactSkin()}
+    print(f"[bench] {method}: contact skin: {r['skin'][0]} walking, {r['skin'][1]} reusing "
+          f"substeps", file=sys.stderr, flush=True)
     if args.residual > 0:
         st = sim.solverStats()
         r["stats"] = st
@@ -430,7 +438,9 @@ def run_method(sc, args, method, model, strand_model=None):
               file=sys.stderr, flush=True)
     # live per-stage timing right after the timed frames, while the GPU runs at
     # its working clocks (a handle left idle first starts with slower kernels)
+    sim.contactSkinStats(reset=True)
     r["profile"] = live_profile(sim, sc, args, substeps_per_frame)
+    r["profile_skin"] = sim.contactSkinStats(reset=True)
     print(f"[bench] {method}: timed {args.steps} frames in {elapsed:.3f}s "
           f"({r['value']:.4e} vertex-substeps/s), min separation {r['min_separation']:.3f} x 2r",
           file=sys.stderr, flush=True)
@@ -488,6 +498,9 @@ def main():
     ap.add_argument("--launch-config", type=int, nargs=2, default=None,
                     metavar=("SWEEP_THREADS", "QUERY_THREADS"),
                     help="fixed block sizes (skips the autotuner)")
+    ap.add_argument("--skin", type=float, nargs=2, default=None, metavar=("MARGIN", "SWEEP"),
+                    help="contact skin: margin in detection radii and sweep in substeps (default: the "
+                         "library's; margin 0 walks the tree in every substep)")
     ap.add_argument("--tile-config", type=int, nargs=3, default=None,
                     metavar=("REFIT_LEAVES", "QUERY_STACK_LDS", "QUERY_CANDIDATES"),
                     help="fixed refit tile and query shape (default when the autotuner is "
@@ -637,7 +650,9 @@ def main():
     method_rows = {}
     for m, r in runs.items():
         row = {"value": round(r["value"], 1), "ms_per_step": round(r["elapsed"] / args.steps * 1e3, 3),
-               "min_separation_x2r": round(r["min_separation"], 3)}
+               "min_separation_x2r": round(r["min_separation"], 3),
+               "skin_walking_substeps": int(r["skin"][0]),
+               "skin_reusing_substeps": int(r["skin"][1])}
         if "stats" in r:
             row["sweeps_per_substep"] = round(r["sweeps_per_substep"], 1)
             row["not_converged_substeps"] = int(r["stats"]["notConverged"])
@@ -696,6 +711,8 @@ def main():
             "not_converged_substeps": stats["notConverged"] if stats else None,
             "method": method_names[main_m],
             "detection_period": int(sc.params.detectionPeriod),
+            "contact_skin": dict(zip(("margin_detection_radii", "sweep_substeps"),
+                                     (round(float(x), 4) for x in runs[main_m]["sim_skin"]))),
             "launch": "eager" if args.no_graphs else "hip graphs",
         },
         "methods": method_rows,

@@ -384,6 +384,36 @@ int yb_get_launch_config(yb_sim* s, int* sweepThreads, int* queryThreads);
 int yb_set_tile_config(yb_sim* s, int refitLeaves, int queryStackLds, int queryCandidates);
 int yb_get_tile_config(yb_sim* s, int* refitLeaves, int* queryStackLds, int* queryCandidates);
 
+/* Contact skin (no counterpart in the reference, whose detectCollisions
+ * refits the tree and walks it in every detecting substep, collision.cu:94-119;
+ * the contact lists are the same either way).  With margin > 0, a detecting
+ * substep that must walk the tree grows every segment's box by its warm-start
+ * displacement times `sweep` (the motion of `sweep` substeps at that speed)
+ * and by margin x meta.detectionRadius on every side, walks the tree refit on those
+ * grown boxes, and keeps each segment's candidate leaves (at most
+ * YB_SKIN_CANDIDATES, in the walk's order).  Later detecting substeps skip
+ * the refit and the walk while every segment's current box lies inside its
+ * grown box, and test only the kept candidates: two segments whose current
+ * boxes overlap have overlapping grown boxes, so the walk found the pair.
+ * The leaf order of a walk is the tree's in-order leaf order, so the contact
+ * lists equal those of a walk, slot for slot.  Every other case -- a box
+ * outside its grown box (in a batch, per scene), a full candidate list, a
+ * traversal-stack overflow, a tree rebuild, new scenes -- walks again.
+ * margin <= 0 turns the skin off (the tree is refit and walked in every
+ * detecting substep).  Default: margin 0.1, sweep 8. */
+#define YB_SKIN_CANDIDATES 64
+int yb_set_contact_skin(yb_sim* s, float margin, float sweep);
+int yb_get_contact_skin(yb_sim* s, float* margin, float* sweep);
+/* Detecting substeps that walked the tree (out[0]) and that reused the kept
+ * candidates (out[1]) since the last reset; reset != 0 clears the counts. */
+int yb_contact_skin_stats(yb_sim* s, int64_t out[2], int reset);
+/* Contact skin state (tests): each segment's kept candidate count (n ints,
+ * -1: incomplete, the segment walks at its next detection), grown box
+ * (8n floats, lo xyzw, hi xyzw) and each scene's policy (2 ints per scene:
+ * substeps left walking without the skin, 0 or 1 + substeps its kept lists
+ * have lasted); any pointer may be null. */
+int yb_debug_read_skin(yb_sim* s, int32_t* candN, float* sbox, int32_t* policy);
+
 /* Launch mode.  on = 1 (default): each substep's kernels run as captured HIP
  * graphs (step.cpp:5-34 likewise captures the sweep chain).  on = 0: the same
  * kernels are enqueued one by one on the handle's stream -- same results,

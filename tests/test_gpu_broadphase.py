@@ -13,12 +13,14 @@ from yarnball_amd._lib import check, ptr
 from helpers import contact_sets, gpu_sim
 
 
-def prologue(sc, seed=0, vel_scale=0.02):
+def prologue(sc, seed=0, vel_scale=0.02, skin=True):
     rng = np.random.default_rng(seed)
     sc = sc.copy()
     free = sc.verts["invMass"] != 0
     sc.vels[free] = (rng.normal(0, vel_scale, (free.sum(), 3))).astype(np.float32)
     sim = gpu_sim(sc)
+    if not skin:
+        sim.setContactSkin(0.0)
     n = sc.n
     lv = (rng.normal(0, vel_scale, (n, 3))).astype(np.float32)
     check(sim._lib.yb_debug_write_last_vels(sim.handle, ptr(lv)))
@@ -79,8 +81,11 @@ TREE_SCENES = [
 ]
 
 
-def check_tree(sim, n, order, aabb):
-    """Structural invariants of the Karras tree and exactness of refit boxes."""
+def check_tree(sim, n, order, aabb, leafbox=None):
+    """Structural invariants of the Karras tree and exactness of refit boxes:
+    leaf slots hold leafbox (the segment boxes, or with the contact skin the
+    grown boxes of the last walk), internal slots their unions; the bounds are
+    the union of the segment boxes."""
     import sys
 
     child = np.zeros(2 * (n - 1), np.int32)
@@ -91,6 +96,8 @@ def check_tree(sim, n, order, aabb):
     box = box.reshape(n - 1, 4, 4)
     rank = np.empty(n, np.int64)
     rank[order] = np.arange(n)
+    if leafbox is None:
+        leafbox = aabb
     assert parent[0] == -1
     seen = np.zeros(n, np.int32)
     sys.setrecursionlimit(10000)
@@ -105,7 +112,7 @@ def check_tree(sim, n, order, aabb):
                 k = rank[seg]
                 assert parent[n - 1 + k] == (node << 1 | side)
                 seen[seg] += 1
-                lo, hi = aabb[seg, 0:3], aabb[seg, 4:7]
+                lo, hi = leafbox[seg, 0:3], leafbox[seg, 4:7]
                 rlo, rhi = k, k
                 # leaf slot carries the leaf's signed velocity pad
                 assert box[node, 2 * side + 1, 3] == aabb[seg, 7]
@@ -126,26 +133,50 @@ def check_tree(sim, n, order, aabb):
     lo_r, hi_r, rlo, rhi = walk(0)
     assert (lo_r, hi_r) == (0, n - 1)
     assert np.all(seen == 1), "every leaf exactly once"
+    if leafbox is aabb:
+        root = np.concatenate(sim.bounds())
+        assert np.array_equal(root[:3], rlo) and np.array_equal(root[3:], rhi), "root box"
+    seg = np.isfinite(aabb[:, 0]) & (aabb[:, 0] <= aabb[:, 4])
     root = np.concatenate(sim.bounds())
-    assert np.array_equal(root[:3], rlo) and np.array_equal(root[3:], rhi), "root box"
+    assert np.array_equal(root[:3], aabb[seg, 0:3].min(axis=0)), "bounds lo"
+    assert np.array_equal(root[3:], aabb[seg, 4:7].max(axis=0)), "bounds hi"
+
+
+def skin_boxes(sim, n):
+    sbox = np.zeros((n, 8), np.float32)
+    check(sim._lib.yb_debug_read_skin(sim.handle, None, ptr(sbox), None))
+    return sbox
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("skin", [False, True], ids=["walk", "skin"])
 @pytest.mark.parametrize("name,make", TREE_SCENES, ids=[s[0] for s in TREE_SCENES])
-def test_karras_tree_invariants(name, make):
-    sim, sc, p, lv, g = prologue(make())
-    check_tree(sim, sc.n, g["order"], g["aabb"])
-    # Move the scene, then refit the same topology (no rebuild) and re-check.
+def test_karras_tree_invariants(name, make, skin):
+    """Without the contact skin the tree is refit on the segment boxes; with
+    it, on the grown boxes of the walk (contained in them the segment boxes)."""
+    sc0 = make()
+    sim, sc, p, lv, g = prologue(sc0, skin=skin)
+    n = sc.n
+    leaf = skin_boxes(sim, n) if skin else None
+    if skin:
+        live = g["aabb"][:, 0] <= g["aabb"][:, 4]
+        assert np.all(leaf[live, 0:3] <= g["aabb"][live, 0:3]), "grown lo"
+        assert np.all(leaf[live, 4:7] >= g["aabb"][live, 4:7]), "grown hi"
+    check_tree(sim, n, g["order"], g["aabb"], leaf)
+    # Move the scene, then refit the same topology (no rebuild) and re-check;
+    # with the skin, its kept lists are dropped first so that this substep walks.
     for _ in range(3):
         sim.advance(sim.maxH)
+    if skin:
+        sim.setContactSkin(0.0)
+        sim.setContactSkin()
     p2 = sim.meta.copy()
     check(sim._lib.yb_debug_prologue(sim.handle, ctypes.byref(p2), np.float32(sim.maxH), 1, 0))
-    n = sc.n
     aabb = np.zeros((n, 8), np.float32)
     order = np.zeros(n, np.int32)
     check(sim._lib.yb_debug_read(sim.handle, None, ptr(aabb), None, ptr(order), None, None))
     assert np.array_equal(order, g["order"]), "refit must keep the sorted order"
-    check_tree(sim, n, order, aabb)
+    check_tree(sim, n, order, aabb, skin_boxes(sim, n) if skin else None)
     sim.close()
 
 
@@ -195,10 +226,13 @@ def max_pending(child, box, aabb, flags):
 
 
 @pytest.mark.gpu
-def test_deep_traversal_contact_sets_bit_exact():
+@pytest.mark.parametrize("skin", [False, True], ids=["walk", "skin"])
+def test_deep_traversal_contact_sets_bit_exact(skin):
     """Walks deeper than the 8 stack entries the query keeps in LDS use its
-    scratch spill; the contact sets must still equal the FP32 twin's."""
-    sim, sc, p, lv, g = prologue(starburst())
+    scratch spill; the contact sets must still equal the FP32 twin's (with
+    the contact skin, the centre's candidate lists are too long to keep, so
+    those segments walk again with their own boxes)."""
+    sim, sc, p, lv, g = prologue(starburst(), skin=skin)
     n = sc.n
     child = np.zeros(2 * (n - 1), np.int32)
     parent = np.zeros(2 * n - 1, np.int32)
@@ -225,6 +259,7 @@ def test_refit_top_nodes_of_a_tree_over_2048_refit_blocks():
     n = sc.n
     assert n > 2048 * 1024
     sim = gpu_sim(sc)
+    sim.setContactSkin(0.0)  # the tree refit on the segment boxes in every substep
     for _ in range(2):
         sim.advance(sim.maxH)
     p = sim.meta.copy()

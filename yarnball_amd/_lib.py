@@ -169,6 +169,10 @@ SIGNATURES = {
     "yb_get_launch_config": (_I, [_P, _P, _P]),
     "yb_set_tile_config": (_I, [_P, _I, _I, _I]),
     "yb_get_tile_config": (_I, [_P, _P, _P, _P]),
+    "yb_set_contact_skin": (_I, [_P, _F, _F]),
+    "yb_get_contact_skin": (_I, [_P, _P, _P]),
+    "yb_contact_skin_stats": (_I, [_P, _P, _I]),
+    "yb_debug_read_skin": (_I, [_P, _P, _P, _P]),
     "yb_record": (_I, [_P, ctypes.c_int64]),
     "yb_record_enable": (_I, [_P, _I]),
     "yb_record_fetch": (_I, [_P, _P, _P, ctypes.POINTER(ctypes.c_int64)]),

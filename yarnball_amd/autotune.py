@@ -8,7 +8,8 @@ Picks, by timing each candidate with HIP events on the stepper's stream
   - the query's LDS shape: traversal-stack entries kept in LDS and candidate
     leaves buffered per thread, which together set the waves a CU can hold.
 The caller's simulation is never touched: apply the result with
-Sim.setLaunchConfig and Sim.setTileConfig.  Every kernel's output per vertex
+Sim.setLaunchConfig and Sim.setTileConfig.  The candidates are timed with
+the contact skin off, so that every substep walks the tree.  Every kernel's output per vertex
 or segment is a function of its inputs only, refit boxes are min/max unions
 and contact lists keep their order, so the choice changes the speed and never
 the results (checked by tests/test_gpu_launch_modes.py).
@@ -36,6 +37,9 @@ def tune(scene, offsets=None, residual=0.0, max_sweeps=6000, substeps=2, rounds=
     sim = Sim(scene.n)
     sim.printErrors = False
     scenes.load_into(sim, scene)
+    # the tree is refit and walked in every timed substep (the contact skin,
+    # which reuses the candidates of a walk, would skip both in most of them)
+    sim.setContactSkin(0.0)
     if offsets is not None:
         sim.setScenes(offsets)
     if residual > 0:

@@ -93,7 +93,9 @@ static void free_all(yb_sim* s) {
                     b.mlpPrm,  b.mlpTiles, b.mlpBias, b.corrPrev, b.dxPlain, b.predC,
                     b.lineG,   b.lineQL,  b.lineE[0], b.lineE[1], b.lineF[0], b.lineF[1],
                     b.mlpCount, b.recX, b.recT, b.recCount, b.sceneOf, b.sceneOff,
-                    b.sceneBoxes, b.sceneRoot, b.sceneEndFlags, b.drvInit, b.candCount};
+                    b.sceneBoxes, b.sceneRoot, b.sceneEndFlags, b.drvInit, b.candCount,
+                    b.sbox,    b.cand,    b.candN,    b.skinState, b.skinStats, b.skinRoot,
+                    b.skinPolicy};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (b.hostFlag) (void)hipHostFree(b.hostFlag);
@@ -235,6 +237,13 @@ extern "C" int yb_create(int n, yb_sim** out) {
         ALLOC(b.lineE[k], (size_t)sweep_waves(n));
         ALLOC(b.lineF[k], (size_t)sweep_waves(n));
     }
+    ALLOC(b.sbox, 2 * N);
+    ALLOC(b.cand, N * kSkinCand);
+    ALLOC(b.candN, N);
+    ALLOC(b.skinState, 1 + (size_t)kMaxScenes);
+    ALLOC(b.skinStats, 2);
+    ALLOC(b.skinRoot, 2);
+    ALLOC(b.skinPolicy, (size_t)kMaxScenes);
 #undef ALLOC
     if (e == hipSuccess) e = hipHostMalloc((void**)&b.hostFlag, 2 * sizeof(int32_t));
     if (e == hipSuccess) e = hipEventCreateWithFlags(&s->chunkEv[0], hipEventDisableTiming);
@@ -265,6 +274,11 @@ extern "C" int yb_create(int n, yb_sim** out) {
     if (e == hipSuccess)
         e = hipMemset(b.mlpCount, 0, sizeof(unsigned long long) * 2 * ((N + 63) / 64));
     if (e == hipSuccess) e = hipMemset(b.recCount, 0, 2 * sizeof(int64_t));
+    // contact skin: no kept candidates yet (every segment walks first)
+    if (e == hipSuccess) e = hipMemset(b.candN, 0xff, sizeof(int32_t) * N);
+    if (e == hipSuccess) e = hipMemset(b.skinState, 0, sizeof(int32_t) * (1 + (size_t)kMaxScenes));
+    if (e == hipSuccess) e = hipMemset(b.skinStats, 0, 2 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMemset(b.skinPolicy, 0, sizeof(int2) * (size_t)kMaxScenes);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
         set_message(std::string("yb_create: ") + hipGetErrorString(e));
@@ -449,7 +463,7 @@ extern "C" int yb_set_scenes(yb_sim* s, int numScenes, const int32_t* offsets) {
         set_message("yb_set_scenes: invalid argument");
         return YB_ERR_INVALID_ARGUMENT;
     }
-    if (numScenes > 65536) {
+    if (numScenes > kMaxScenes) {
         set_message("yb_set_scenes: at most 65536 scenes");
         return YB_ERR_INVALID_ARGUMENT;
     }
@@ -501,6 +515,9 @@ extern "C" int yb_set_scenes(yb_sim* s, int numScenes, const int32_t* offsets) {
                 return rc == YB_OK ? YB_ERR_INVALID_ARGUMENT : rc;
             }
     }
+    // the contact skin's per-scene state starts over
+    HIPCHECK(hipMemsetAsync(b.skinPolicy, 0, sizeof(int2) * (size_t)kMaxScenes, s->stream));
+    HIPCHECK(hipStreamSynchronize(s->stream));
     // the tree and the captured graphs depend on the scene layout
     s->treeBuilt = false;
     s->graphItr = -1;
@@ -563,6 +580,9 @@ static DevParams make_dev_params(const yb_params* p, int n) {
 
 static int upload_params(yb_sim* s, const yb_params* p) {
     DevParams d = make_dev_params(p, s->n);
+    d.skinSweep = s->skinSweep;
+    d.skinMargin = s->skinMargin * p->detectionRadius;
+    d.skinCap = kSkinCapRadii * p->detectionRadius;
     HIPCHECK(hipMemcpyAsync(s->b.params, &d, sizeof(DevParams), hipMemcpyHostToDevice, s->stream));
     // The source is a stack object: wait for the (tiny) copy before returning.
     HIPCHECK(hipStreamSynchronize(s->stream));
@@ -642,7 +662,7 @@ static void enqueue_prologue(yb_sim* s, int mode, bool adaptive, int warm) {
     launch_init(s);
     mark(s, YB_STAGE_INIT);
     if (mode >= 1) {
-        launch_aabb(s);
+        launch_aabb(s, mode == 2);  // a new tree: every segment walks
         mark(s, YB_STAGE_AABB);
         if (mode == 2) {
             launch_rebuild(s);
@@ -1024,6 +1044,65 @@ extern "C" int yb_get_tile_config(yb_sim* s, int* refitLeaves, int* queryStackLd
     return YB_OK;
 }
 
+extern "C" int yb_set_contact_skin(yb_sim* s, float margin, float sweep) {
+    if (!s) return YB_ERR_INVALID_ARGUMENT;
+    if (!(sweep >= 0.f && sweep <= 1e3f) || !(margin <= 1e3f)) {
+        set_message("yb_set_contact_skin: sweep must lie in [0, 1000], margin at most 1000");
+        return YB_ERR_INVALID_ARGUMENT;
+    }
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    const bool was = s->skinMargin > 0.f, on = margin > 0.f;
+    s->skinMargin = on ? margin : 0.f;
+    s->skinSweep = sweep;
+    if (on != was) {
+        // kept candidates from an earlier period may follow an older tree's
+        // leaf order: every segment walks first
+        HIPCHECK(hipMemsetAsync(s->b.candN, 0xff, sizeof(int32_t) * (size_t)s->n, s->stream));
+        HIPCHECK(hipMemsetAsync(s->b.skinPolicy, 0, sizeof(int2) * (size_t)kMaxScenes, s->stream));
+        HIPCHECK(hipStreamSynchronize(s->stream));
+        s->graphItr = -1;
+        s->adaptiveReady = false;
+    }
+    return YB_OK;
+}
+
+extern "C" int yb_get_contact_skin(yb_sim* s, float* margin, float* sweep) {
+    if (!s || !margin || !sweep) return YB_ERR_INVALID_ARGUMENT;
+    *margin = s->skinMargin;
+    *sweep = s->skinSweep;
+    return YB_OK;
+}
+
+extern "C" int yb_contact_skin_stats(yb_sim* s, int64_t out[2], int reset) {
+    if (!s || !out) return YB_ERR_INVALID_ARGUMENT;
+    unsigned long long c[2];
+    HIPCHECK(hipMemcpyAsync(c, s->b.skinStats, sizeof(c), hipMemcpyDeviceToHost, s->stream));
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    out[0] = (int64_t)c[0];
+    out[1] = (int64_t)c[1];
+    if (reset) {
+        HIPCHECK(hipMemsetAsync(s->b.skinStats, 0, sizeof(c), s->stream));
+        HIPCHECK(hipStreamSynchronize(s->stream));
+    }
+    return YB_OK;
+}
+
+extern "C" int yb_debug_read_skin(yb_sim* s, int32_t* candN, float* sbox, int32_t* policy) {
+    if (!s) return YB_ERR_INVALID_ARGUMENT;
+    const size_t N = (size_t)s->n;
+    if (candN)
+        HIPCHECK(hipMemcpyAsync(candN, s->b.candN, sizeof(int32_t) * N, hipMemcpyDeviceToHost,
+                                s->stream));
+    if (sbox)
+        HIPCHECK(hipMemcpyAsync(sbox, s->b.sbox, sizeof(float4) * 2 * N, hipMemcpyDeviceToHost,
+                                s->stream));
+    if (policy)
+        HIPCHECK(hipMemcpyAsync(policy, s->b.skinPolicy, sizeof(int2) * (size_t)s->numScenes,
+                                hipMemcpyDeviceToHost, s->stream));
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    return YB_OK;
+}
+
 extern "C" int yb_set_graphs(yb_sim* s, int on) {
     if (!s) return YB_ERR_INVALID_ARGUMENT;
     HIPCHECK(hipStreamSynchronize(s->stream));
@@ -1315,6 +1394,12 @@ extern "C" int64_t yb_step_count(yb_sim* s) { return s ? s->stepCounter : 0; }
 extern "C" int yb_bounds(yb_sim* s, float out[6]) {
     if (!s || !out) return YB_ERR_INVALID_ARGUMENT;
     float4 box[2];
+    // with the contact skin the tree holds grown boxes and is not refit in
+    // every substep: the union of the current boxes is formed here
+    if (s->skinMargin > 0.f && s->treeBuilt) {
+        launch_scene_box(s);
+        HIPCHECK(hipGetLastError());
+    }
     HIPCHECK(hipMemcpyAsync(box, s->b.sceneBox, sizeof(box), hipMemcpyDeviceToHost, s->stream));
     HIPCHECK(hipStreamSynchronize(s->stream));
     if (!s->treeBuilt) {

@@ -31,10 +31,37 @@ __global__ __launch_bounds__(256) void k_init(const DevParams* __restrict__ P,
                                               const float4* __restrict__ vel,
                                               float4* __restrict__ lastVel,
                                               float4* __restrict__ yv,
-                                              float4* __restrict__ dx0) {
+                                              float4* __restrict__ dx0,
+                                              int32_t* __restrict__ skinState,
+                                              int2* __restrict__ skinPolicy, int skinScenes) {
     const int n = P->n;
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n) return;
+    // Contact skin, one thread per scene: the scene's walk flag of the last
+    // substep feeds its policy (lists reused fewer than kSkinMinReuse times:
+    // walk without the skin for kSkinBackoff substeps), then this substep's
+    // flags start clear (k_aabb sets them).
+    if (v < skinScenes) {
+        const int f = skinScenes > 1 ? 1 + v : 0;
+        int2 pol = skinPolicy[v];
+        const bool walked = skinState[f] != 0;
+        if (pol.x > 0) {
+            pol.x--;
+            pol.y = 0;
+        } else if (walked) {
+            if (pol.y >= 1 && pol.y - 1 < kSkinMinReuse) {
+                pol.x = kSkinBackoff;
+                pol.y = 0;
+            } else {
+                pol.y = 1;
+            }
+        } else if (pol.y >= 1) {
+            pol.y = min(pol.y + 1, 1 << 20);
+        }
+        skinPolicy[v] = pol;
+        skinState[f] = 0;
+        if (v == 0 && skinScenes > 1) skinState[0] = 0;
+    }
     const float h = P->h;
     const v3 g = v3{P->gx, P->gy, P->gz};
     const v3 u = xyz(vel[v]);
@@ -59,11 +86,16 @@ __global__ __launch_bounds__(256) void k_init(const DevParams* __restrict__ P,
 // empty box (+inf, -inf), which overlaps nothing.  hi.w carries the segment's
 // velocity pad sqrt(max(|dx_i|^2, |dx_i+1|^2)) for the narrowphase radius, with
 // the sign bit set when either endpoint is glued (connectionIndex >= 0).
+// Contact skin (sk.state set): a segment whose box has left the grown box of
+// the last walk, or whose kept candidate list is incomplete, flags its scene
+// (and sk.state[0]) to walk the tree in this substep; force flags every one.
+// Every flagging lane stores the same value, so the stores need no atomics.
 __global__ __launch_bounds__(256) void k_aabb(const DevParams* __restrict__ P,
                                               const float4* __restrict__ pos,
                                               const float4* __restrict__ prm,
                                               const float4* __restrict__ dx,
-                                              float4* __restrict__ aabb) {
+                                              float4* __restrict__ aabb, SkinArgs sk,
+                                              const int32_t* __restrict__ sceneOf, int force) {
     const int n = P->n;
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n) return;
@@ -87,6 +119,41 @@ __global__ __launch_bounds__(256) void k_aabb(const DevParams* __restrict__ P,
     }
     aabb[2 * v] = f4(lo, 0.f);
     aabb[2 * v + 1] = f4(hi, glued ? -vpad : vpad);
+    if (sk.state) {
+        // a scene walking without the skin walks in every detection
+        bool walk = force != 0 || sk.policy[sceneOf ? sceneOf[v] : 0].x > 0;
+        if (!walk && (flags & YB_FLAG_HAS_NEXT)) {
+            const float4 s0 = sk.sbox[2 * v];
+            const float4 s1 = sk.sbox[2 * v + 1];
+            const bool inside = (s0.x <= lo.x) & (s0.y <= lo.y) & (s0.z <= lo.z) &
+                                (hi.x <= s1.x) & (hi.y <= s1.y) & (hi.z <= s1.z);
+            walk = !inside || sk.candN[v] < 0;
+        }
+        if (walk) {
+            sk.state[0] = 1;
+            if (sceneOf) sk.state[1 + sceneOf[v]] = 1;
+        }
+    }
+}
+
+// Grown box of a segment for the contact skin: its box swept by `sweep`
+// substeps of its endpoints' warm-start displacement, at most `cap` along an
+// axis (per axis, the box of both endpoints at t = 0 and at the end of the
+// sweep lies inside it, and so does every box between), then grown by
+// `margin` on every side.  hi.w (the velocity pad) is kept.  The empty box of
+// a vertex without a next vertex stays empty.
+__device__ __forceinline__ void skin_box(float4& lo, float4& hi, const float4* __restrict__ dx,
+                                         int seg, int n, const DevParams* __restrict__ P) {
+    if (!(lo.x <= hi.x) || seg + 1 >= n) return;
+    const float sweep = P->skinSweep, margin = P->skinMargin, cap = P->skinCap;
+    const float4 d0 = dx[seg];
+    const float4 d1 = dx[seg + 1];
+    lo.x += fmaxf(sweep * fminf(0.f, fminf(d0.x, d1.x)), -cap) - margin;
+    lo.y += fmaxf(sweep * fminf(0.f, fminf(d0.y, d1.y)), -cap) - margin;
+    lo.z += fmaxf(sweep * fminf(0.f, fminf(d0.z, d1.z)), -cap) - margin;
+    hi.x += fminf(sweep * fmaxf(0.f, fmaxf(d0.x, d1.x)), cap) + margin;
+    hi.y += fminf(sweep * fmaxf(0.f, fmaxf(d0.y, d1.y)), cap) + margin;
+    hi.z += fminf(sweep * fmaxf(0.f, fmaxf(d0.z, d1.z)), cap) + margin;
 }
 
 // ---------------------------------------------------------------------------
@@ -521,7 +588,10 @@ __global__ __launch_bounds__(kLeaves) void k_refit_local(
     float4* __restrict__ nodes, const int32_t* __restrict__ parent, const int2* __restrict__ rng,
     const int2* __restrict__ kids, const int32_t* __restrict__ leafNeed,
     float4* __restrict__ sufBox, float4* __restrict__ preBox, float4* __restrict__ blkBox,
-    float4* __restrict__ sceneBox) {
+    float4* __restrict__ sceneBox, const DevParams* __restrict__ P,
+    const float4* __restrict__ dx, SkinArgs sk, const int32_t* __restrict__ sceneOf) {
+    // contact skin: no segment walks in this substep, so nothing reads the tree
+    if (sk.state && sk.state[0] == 0) return;
     __shared__ float4 lbox[kLeaves][2];
     __shared__ float4 nbox[kLeaves][2];
     __shared__ int2 lkids[kLeaves];          // kids[] of nodes base..end-1
@@ -547,8 +617,20 @@ __global__ __launch_bounds__(kLeaves) void k_refit_local(
             lpar[t] = parent[base + t];
             pr = rng[base + t];
         }
-        lbox[t][0] = aabb[2 * seg];
-        lbox[t][1] = aabb[2 * seg + 1];
+        float4 lo = aabb[2 * seg];
+        float4 hi = aabb[2 * seg + 1];
+        const int sc = sceneOf ? sceneOf[seg] : 0;
+        if (sk.state && sk.policy[sc].x == 0) {
+            // a scene that keeps candidates has grown boxes in the tree; a
+            // walking one keeps them for its next detections
+            skin_box(lo, hi, dx, seg, n, P);
+            if (!sceneOf || sk.state[1 + sc]) {
+                sk.sbox[2 * seg] = lo;
+                sk.sbox[2 * seg + 1] = hi;
+            }
+        }
+        lbox[t][0] = lo;
+        lbox[t][1] = hi;
     }
     __syncthreads();
 
@@ -673,7 +755,9 @@ __global__ __launch_bounds__(kLeaves) void k_refit_local(
 // Higher levels (more than 2048 blocks): one launch each over all entries.
 constexpr int kTabLocal = 10;
 __global__ __launch_bounds__(1024) void k_refit_table(int nb, int levels,
-                                                      float4* __restrict__ tab) {
+                                                      float4* __restrict__ tab,
+                                                      const int32_t* __restrict__ skinState) {
+    if (skinState && skinState[0] == 0) return;
     __shared__ float4 wlo[2048], whi[2048];
     const int t = threadIdx.x;
     const int b0 = blockIdx.x * 1024;
@@ -708,7 +792,9 @@ __global__ __launch_bounds__(1024) void k_refit_table(int nb, int levels,
 }
 
 __global__ __launch_bounds__(256) void k_refit_table_level(int nb, int L,
-                                                           float4* __restrict__ tab) {
+                                                           float4* __restrict__ tab,
+                                                           const int32_t* __restrict__ skinState) {
+    if (skinState && skinState[0] == 0) return;
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nb) return;
     const int half = 1 << (L - 1);
@@ -729,7 +815,9 @@ __global__ __launch_bounds__(256) void k_refit_top(int n, int nb, int leafShift,
                                                    const float4* __restrict__ preBox,
                                                    const float4* __restrict__ tab,
                                                    float4* __restrict__ nodes,
-                                                   float4* __restrict__ sceneBox) {
+                                                   float4* __restrict__ sceneBox,
+                                                   const int32_t* __restrict__ skinState) {
+    if (skinState && skinState[0] == 0) return;
     const int count = *stradCount;
     for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < count; s += gridDim.x * blockDim.x) {
         const int i = strad[s];
@@ -831,6 +919,12 @@ __device__ __forceinline__ bool pair_accepted(const DevParams* __restrict__ P,
 // interpenetration warning into err[1] and the smallest separation into
 // *minSep -- the contact lists are the pairs it accepts, so these are the
 // distances the step limit kernel would measure again.
+// Contact skin (sk.state set; the tree then holds grown boxes): a segment of a
+// walking scene walks with its grown box, keeps every leaf it meets as a
+// candidate and tests each against its current box before the pair test; a
+// segment of any other scene walks nothing and tests its kept candidates.
+// Both visit the candidates in the tree's in-order leaf order, as the walk
+// without the skin does, so every mode gives the same lists.
 template <int kQueryThreads, int kStackL, int kCand>
 __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __restrict__ P,
                                                          const int32_t* __restrict__ order,
@@ -846,7 +940,8 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
                                                          int stackLimit,
                                                          int32_t* __restrict__ candCount,
                                                          float4* __restrict__ dxLimit,
-                                                         float* __restrict__ minSep) {
+                                                         float* __restrict__ minSep,
+                                                         SkinArgs sk) {
     __shared__ int32_t stk[kStackL][kQueryThreads];
     __shared__ int32_t candJ[kCand][kQueryThreads];
     __shared__ float candPad[kCand][kQueryThreads];
@@ -858,10 +953,21 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
     // that one stretch of strands visits stay in one L2.
     const int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const bool seg = i < n && (__float_as_uint(prm[min(i, n - 1)].w) & YB_FLAG_HAS_NEXT);
+    // 0: walk (no skin), 1: walk with the grown box and keep the candidates,
+    // 2: test the kept candidates, 3: walk without keeping (the scene's lists
+    // did not last; the tree holds its current boxes)
+    int mode = 0;
+    if (sk.state) {
+        const int sc = sceneOf ? sceneOf[min(i, n - 1)] : 0;
+        mode = sk.policy[sc].x > 0 ? 3 : (sk.state[sceneOf ? 1 + sc : 0] ? 1 : 2);
+        if (sk.stats && blockIdx.x == 0 && threadIdx.x == 0)
+            atomicAdd(&sk.stats[sk.state[0] ? 0 : 1], 1ull);
+    }
     float minD2 = INFINITY;  // closest listed contact, squared
     if (!seg) {
         if (i < n) {
             numCols[i] = 0;
+            if (mode == 1) sk.candN[i] = 0;
             if (candCount) {
                 candCount[i] = 0;
                 candCount[n + i] = 0;
@@ -879,10 +985,6 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
         si.vpad = fabsf(qhi4.w);
         si.glued = signbit(qhi4.w);
         const int t = threadIdx.x;
-        int sp = 0;
-        volatile int32_t deep[kStack > kStackL ? kStack - kStackL : 1];  // entries past the LDS part
-        // batched scenes: traverse only this segment's own scene subtree
-        int node = sceneOf ? sceneRoot[sceneOf[i]] : 0;
         int cnt = 0;
         bool overflowStack = false;
         // The traversal only collects overlapping leaves, in visit order; the
@@ -891,8 +993,8 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
         // together instead of once per node visit at which any thread met a leaf.
         // Candidates keep their visit order, hence so do the contact lists.
         int nb = 0;
-        // measurement only: leaves more than two indices away whose boxes overlap
-        // the query box, node visits, deepest stack
+        // measurement only: leaves tested by the narrowphase, node visits (kept
+        // candidates scanned, when reusing), deepest stack
         int cand = 0, visits = 0, maxSp = 0;
         // The first four partners stay in registers and are stored at the end,
         // slot by slot, by the whole wave at once (one coalesced 256-byte store
@@ -900,19 +1002,28 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
         // partly written at different times, and the cache wrote them back several
         // times over.  Later partners are stored as they are found.
         int32_t r0 = 0, r1 = 0, r2 = 0, r3 = 0;
-        auto narrowphase = [&]() {
+        // exact: the buffered leaves met the grown box only; their current
+        // boxes are tested here first
+        auto narrowphase = [&](bool exact) {
             cand += nb;
     #pragma unroll 1
             for (int k = 0; k < nb; k++) {
                 const int j = candJ[k][t];
-                const float pw = candPad[k][t];
+                float pw = candPad[k][t];
+                bool hit = true;
+                if (exact) {
+                    const float4 blo = aabb[2 * j];
+                    const float4 bhi = aabb[2 * j + 1];
+                    hit = overlap(qlo, qhi, blo, bhi);
+                    pw = bhi.w;
+                }
                 SegInfo sj;
                 sj.p0 = xyz(gather4(pos, j));
                 sj.p1 = xyz(gather4<true>(pos, j));
                 sj.vpad = fabsf(pw);
                 sj.glued = signbit(pw);
                 float d2;
-                if (pair_accepted(P, prm, i, si, j, sj, d2)) {
+                if (pair_accepted(P, prm, i, si, j, sj, d2) && hit) {
                     minD2 = fminf(minD2, d2);
                     r0 = cnt == 0 ? j : r0;
                     r1 = cnt == 1 ? j : r1;
@@ -924,60 +1035,133 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
             }
             nb = 0;
         };
-        while (true) {
-            if (candCount) {
-                visits++;
-                maxSp = max(maxSp, sp);
+        // Tree walk from `node` with the box (tlo, thi).  rec: keep every leaf
+        // met (its grown box overlaps) in this segment's candidate list; nar:
+        // test the leaves met (exact: against their current boxes first, when
+        // the tree holds grown boxes).  Returns the leaves kept.
+        auto walk = [&](v3 tlo, v3 thi, bool rec, bool nar, bool exact) {
+            int nc = 0;
+            int sp = 0;
+            volatile int32_t deep[kStack > kStackL ? kStack - kStackL : 1];  // entries past the LDS part
+            // batched scenes: traverse only this segment's own scene subtree
+            int node = sceneOf ? sceneRoot[sceneOf[i]] : 0;
+            while (true) {
+                if (candCount) {
+                    visits++;
+                    maxSp = max(maxSp, sp);
+                }
+                const float4 l0 = nodes[4 * node + 0];
+                const float4 l1 = nodes[4 * node + 1];
+                const float4 r0 = nodes[4 * node + 2];
+                const float4 r1 = nodes[4 * node + 3];
+                const int cl = __float_as_int(l0.w);
+                const int cr = __float_as_int(r0.w);
+                const bool hl = overlap(tlo, thi, l0, l1);
+                const bool hr = overlap(tlo, thi, r0, r1);
+                // leaves within two indices of i (itself and its strand neighbours,
+                // which always overlap) are exempt from the pair test (collision.cu:54):
+                // they are dropped here, before their positions are gathered.  Both
+                // leaves are written without a branch: slot nb and the one after it
+                // are free (nb <= kCand - 2 at every visit), and nb advances only for
+                // an accepted leaf.
+                const bool addL = hl && cl < 0 && abs(~cl - i) > 2;
+                const bool addR = hr && cr < 0 && abs(~cr - i) > 2;
+                if (rec) {
+                    // each segment's list is contiguous: the lines one lane
+                    // writes over its walk are completed in the L2
+                    int32_t* mine = sk.cand + (size_t)i * kSkinCand;
+                    if (addL && nc < kSkinCand) mine[nc] = ~cl;
+                    nc += addL ? 1 : 0;
+                    if (addR && nc < kSkinCand) mine[nc] = ~cr;
+                    nc += addR ? 1 : 0;
+                }
+                if (nar) {
+                    candJ[nb][t] = ~cl;
+                    candPad[nb][t] = l1.w;
+                    nb += addL ? 1 : 0;
+                    candJ[nb][t] = ~cr;
+                    candPad[nb][t] = r1.w;
+                    nb += addR ? 1 : 0;
+                }
+                // at most two leaves per visit: flush before a buffer could overflow
+                if (__any(nb > kCand - 2)) narrowphase(exact);
+                const bool tl = hl && cl >= 0;
+                const bool tr = hr && cr >= 0;
+                const bool descend = tl || tr;
+                if (!descend && sp == 0) break;
+                // Next node: descend left first (the right child is pushed when both
+                // overlap), else pop.  stackLimit <= kStack (host-checked); tests lower
+                // it to force the overflow path.  The LDS part of the stack is used
+                // without branches: the slot above the top is free, so the write is
+                // harmless when nothing is pushed, and the pop reads the top entry.
+                const bool both = tl && tr;
+                const bool push = both && sp < stackLimit;
+                overflowStack = overflowStack || (both && !push);
+                if (sp < kStackL) stk[sp][t] = cr;
+                const int popped = stk[min(max(sp - 1, 0), kStackL - 1)][t];
+                if (push && sp >= kStackL) deep[sp - kStackL] = cr;
+                node = descend ? (tl ? cl : cr) : popped;
+                sp = descend ? sp + (push ? 1 : 0) : sp - 1;
+                if (!descend && sp >= kStackL) {
+                    // keeps the compiler from merging the two pops into one generic
+                    // (flat) load, which would make every pop a slow volatile one
+                    asm volatile("" ::: "memory");
+                    node = deep[sp - kStackL];
+                }
             }
-            const float4 l0 = nodes[4 * node + 0];
-            const float4 l1 = nodes[4 * node + 1];
-            const float4 r0 = nodes[4 * node + 2];
-            const float4 r1 = nodes[4 * node + 3];
-            const int cl = __float_as_int(l0.w);
-            const int cr = __float_as_int(r0.w);
-            const bool hl = overlap(qlo, qhi, l0, l1);
-            const bool hr = overlap(qlo, qhi, r0, r1);
-            // leaves within two indices of i (itself and its strand neighbours,
-            // which always overlap) are exempt from the pair test (collision.cu:54):
-            // they are dropped here, before their positions are gathered.  Both
-            // leaves are written without a branch: slot nb and the one after it
-            // are free (nb <= kCand - 2 at every visit), and nb advances only for
-            // an accepted leaf.
-            const bool addL = hl && cl < 0 && abs(~cl - i) > 2;
-            const bool addR = hr && cr < 0 && abs(~cr - i) > 2;
-            candJ[nb][t] = ~cl;
-            candPad[nb][t] = l1.w;
-            nb += addL ? 1 : 0;
-            candJ[nb][t] = ~cr;
-            candPad[nb][t] = r1.w;
-            nb += addR ? 1 : 0;
-            // at most two leaves per visit: flush before a buffer could overflow
-            if (__any(nb > kCand - 2)) narrowphase();
-            const bool tl = hl && cl >= 0;
-            const bool tr = hr && cr >= 0;
-            const bool descend = tl || tr;
-            if (!descend && sp == 0) break;
-            // Next node: descend left first (the right child is pushed when both
-            // overlap), else pop.  stackLimit <= kStack (host-checked); tests lower
-            // it to force the overflow path.  The LDS part of the stack is used
-            // without branches: the slot above the top is free, so the write is
-            // harmless when nothing is pushed, and the pop reads the top entry.
-            const bool both = tl && tr;
-            const bool push = both && sp < stackLimit;
-            overflowStack = overflowStack || (both && !push);
-            if (sp < kStackL) stk[sp][t] = cr;
-            const int popped = stk[min(max(sp - 1, 0), kStackL - 1)][t];
-            if (push && sp >= kStackL) deep[sp - kStackL] = cr;
-            node = descend ? (tl ? cl : cr) : popped;
-            sp = descend ? sp + (push ? 1 : 0) : sp - 1;
-            if (!descend && sp >= kStackL) {
-                // keeps the compiler from merging the two pops into one generic
-                // (flat) load, which would make every pop a slow volatile one
-                asm volatile("" ::: "memory");
-                node = deep[sp - kStackL];
+            narrowphase(exact);
+            return nc;
+        };
+        // Kept candidates, in the order the walk met them: each one whose
+        // current box overlaps this segment's goes to the narrowphase buffer.
+        // Four at a time, with one 16-byte load of the list.
+        auto reuse = [&](int cn) {
+            const int4* list = reinterpret_cast<const int4*>(sk.cand + (size_t)i * kSkinCand);
+            for (int c = 0; __any(c < cn); c += 4) {
+                const int4 q = c < cn ? list[c >> 2] : make_int4(i, i, i, i);
+                const int js[4] = {c < cn ? q.x : i, c + 1 < cn ? q.y : i, c + 2 < cn ? q.z : i,
+                                   c + 3 < cn ? q.w : i};
+                float4 blo[4], bhi[4];
+    #pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    blo[k] = aabb[2 * js[k]];
+                    bhi[k] = aabb[2 * js[k] + 1];
+                }
+    #pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const bool add = c + k < cn && overlap(qlo, qhi, blo[k], bhi[k]);
+                    candJ[nb][t] = js[k];
+                    candPad[nb][t] = bhi[k].w;
+                    nb += add ? 1 : 0;
+                    if (__any(nb > kCand - 1)) narrowphase(false);
+                }
+                if (candCount) visits += min(4, max(cn - c, 0));
             }
+            narrowphase(false);
+        };
+        if (mode == 2) {
+            reuse(min(sk.candN[i], kSkinCand));
+        } else if (mode == 1) {
+            // Walk with the grown box (the refit kept it in sbox) only to keep
+            // the candidates, then test them as a reusing substep does.  A list
+            // that is full or missed a subtree is not kept (the next detection
+            // walks again) and is not used: its segment walks once more with
+            // its current box, testing every leaf it meets.
+            const int nc = walk(xyz(sk.sbox[2 * i]), xyz(sk.sbox[2 * i + 1]), true, false, false);
+            const bool kept = nc <= kSkinCand && !overflowStack;
+            sk.candN[i] = kept ? nc : -1;
+            if (kept) {
+                reuse(nc);
+            } else {
+                // the stack overflow reported is the one of the walk that
+                // makes the list, as without the skin
+                overflowStack = false;
+                walk(qlo, qhi, false, true, true);
+            }
+        } else {
+            walk(qlo, qhi, false, true, false);
+            if (mode == 3) sk.candN[i] = -1;  // an old list may follow an older tree
         }
-        narrowphase();
         if (cnt > 0) cols[i] = r0;
         if (cnt > 1) cols[i + (size_t)n] = r1;
         if (cnt > 2) cols[i + 2 * (size_t)n] = r2;
@@ -1022,14 +1206,25 @@ __global__ void k_scene_end_flags(int numScenes, const int32_t* __restrict__ off
 static inline int blocks(int n, int b) { return (n + b - 1) / b; }
 
 // ---- launchers -------------------------------------------------------------
-void launch_init(yb_sim* s) {
-    hipLaunchKernelGGL(k_init, dim3(blocks(s->n, 256)), dim3(256), 0, s->stream, s->b.params,
-                       s->b.pos, s->b.vel, s->b.lastVel, s->b.yv, s->b.dx[0]);
+SkinArgs skin_args(const yb_sim* s) {
+    SkinArgs a{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    if (s->skinMargin > 0.f)
+        a = SkinArgs{s->b.skinState, s->b.sbox,      s->b.cand,
+                     s->b.candN,     s->b.skinStats, s->b.skinPolicy};
+    return a;
 }
 
-void launch_aabb(yb_sim* s) {
+void launch_init(yb_sim* s) {
+    const int scenes = s->skinMargin > 0.f ? s->numScenes : 0;
+    hipLaunchKernelGGL(k_init, dim3(blocks(s->n, 256)), dim3(256), 0, s->stream, s->b.params,
+                       s->b.pos, s->b.vel, s->b.lastVel, s->b.yv, s->b.dx[0], s->b.skinState,
+                       s->b.skinPolicy, scenes);
+}
+
+void launch_aabb(yb_sim* s, bool force) {
     hipLaunchKernelGGL(k_aabb, dim3(blocks(s->n, 256)), dim3(256), 0, s->stream, s->b.params,
-                       s->b.pos, s->b.prm, s->b.dx[0], s->b.aabb);
+                       s->b.pos, s->b.prm, s->b.dx[0], s->b.aabb, skin_args(s),
+                       s->numScenes > 1 ? s->b.sceneOf : nullptr, force ? 1 : 0);
 }
 
 void launch_rebuild(yb_sim* s) {
@@ -1079,6 +1274,13 @@ void launch_rebuild(yb_sim* s) {
                        s->b.strad, s->b.stradCount, s->b.leafNeed, sk, s->refitShift);
 }
 
+void launch_scene_box(yb_sim* s) {
+    hipLaunchKernelGGL(k_box_partial, dim3(s->boxBlocks), dim3(kBoxThreads), 0, s->stream, s->n,
+                       s->b.aabb, s->b.blockBox);
+    hipLaunchKernelGGL(k_box_final, dim3(1), dim3(kBoxThreads), 0, s->stream, s->boxBlocks,
+                       s->b.blockBox, s->b.sceneBox);
+}
+
 void launch_scene_end_flags(yb_sim* s) {
     hipLaunchKernelGGL(k_scene_end_flags, dim3(blocks(s->numScenes, 256)), dim3(256), 0, s->stream,
                        s->numScenes, s->b.sceneOff, s->b.prm, s->b.sceneEndFlags);
@@ -1086,9 +1288,13 @@ void launch_scene_end_flags(yb_sim* s) {
 
 template <int kLeaves>
 static void launch_refit_local(yb_sim* s) {
+    const SkinArgs sk = skin_args(s);
+    // with the skin the tree holds grown boxes: its root box goes to a spare slot
     hipLaunchKernelGGL(k_refit_local<kLeaves>, dim3(s->refitBlocks), dim3(kLeaves), 0, s->stream,
                        s->n, s->b.order, s->b.aabb, s->b.nodes, s->b.parent, s->b.rng, s->b.kids,
-                       s->b.leafNeed, s->b.sufBox, s->b.preBox, s->b.blkTab, s->b.sceneBox);
+                       s->b.leafNeed, s->b.sufBox, s->b.preBox, s->b.blkTab,
+                       sk.state ? s->b.skinRoot : s->b.sceneBox, s->b.params, s->b.dx[0], sk,
+                       s->numScenes > 1 ? s->b.sceneOf : nullptr);
 }
 
 void launch_refit(yb_sim* s) {
@@ -1101,14 +1307,16 @@ void launch_refit(yb_sim* s) {
         launch_refit_local<1024>(s);
     if (s->refitBlocks > 1) {
         const int nb = s->refitBlocks;
+        const int32_t* st = skin_args(s).state;
         hipLaunchKernelGGL(k_refit_table, dim3(blocks(nb, 1024)), dim3(1024), 0, s->stream, nb,
-                           s->refitLevels, s->b.blkTab);
+                           s->refitLevels, s->b.blkTab, st);
         for (int L = kTabLocal + 1; L < s->refitLevels; L++)
             hipLaunchKernelGGL(k_refit_table_level, dim3(blocks(nb, 256)), dim3(256), 0, s->stream,
-                               nb, L, s->b.blkTab);
+                               nb, L, s->b.blkTab, st);
         hipLaunchKernelGGL(k_refit_top, dim3(s->refitTopBlocks), dim3(256), 0, s->stream, n,
                            s->refitBlocks, s->refitShift, s->b.strad, s->b.stradCount, s->b.rng, s->b.parent,
-                           s->b.sufBox, s->b.preBox, s->b.blkTab, s->b.nodes, s->b.sceneBox);
+                           s->b.sufBox, s->b.preBox, s->b.blkTab, s->b.nodes,
+                           st ? s->b.skinRoot : s->b.sceneBox, st);
     }
 }
 
@@ -1125,7 +1333,7 @@ static void launch_query_t(yb_sim* s) {
                        s->b.order, s->b.aabb, s->b.nodes, s->b.pos, s->b.prm, s->b.numCols,
                        s->b.cols, s->b.err, s->numScenes > 1 ? s->b.sceneOf : nullptr,
                        s->b.sceneRoot, s->queryStack, s->countCandidates ? s->b.candCount : nullptr,
-                       s->b.dx[0], s->b.minSep);
+                       s->b.dx[0], s->b.minSep, skin_args(s));
 }
 
 // Query shapes (LDS stack entries, buffered candidates) the autotuner picks

@@ -35,6 +35,9 @@ struct DevParams {
     int32_t useVelocityRadius;
     int32_t n;
     int32_t pad_;
+    float skinSweep;   // contact skin: substeps of warm-start motion a grown box covers
+    float skinMargin;  // contact skin: absolute margin on every side of a grown box
+    float skinCap;     // contact skin: largest growth by motion along an axis
 };
 
 // Residual-targeted solver state (device side, one per handle).
@@ -66,6 +69,27 @@ struct SceneCtl {
     double sumFres;  // final Newton residuals summed over substeps
     unsigned long long sweeps;  // the scene's own sweeps summed over substeps
 };
+// Contact-skin arguments of the broadphase kernels (state null: skin off).
+struct SkinArgs {
+    int32_t* state;               // [0] some segment walks, [1 + scene] the scene walks
+    float4* sbox;                 // grown box of each segment's last walk
+    int32_t* cand;                // kept candidates, slot-major cand[seg + n * slot]
+    int32_t* candN;               // kept candidates per segment (< 0: incomplete)
+    unsigned long long* stats;    // [0] walking, [1] reusing detection substeps
+    int2* policy;                 // per scene: (substeps left walking without the skin,
+                                  //  0 or 1 + substeps its kept lists have lasted)
+};
+constexpr int kSkinCand = YB_SKIN_CANDIDATES;
+// A scene whose kept lists are reused fewer than kSkinMinReuse times before
+// its next walk (a walk with grown boxes costs about two plain walks, a reuse
+// about half of one) walks without keeping candidates for kSkinBackoff
+// substeps, then tries the skin again.
+constexpr int kSkinMinReuse = 2;
+constexpr int kSkinBackoff = 16;
+// Growth by motion of a grown box, per axis, in detection radii at most.
+constexpr float kSkinCapRadii = 4.0f;
+constexpr int kMaxScenes = 65536;  // scenes of one handle (yb_set_scenes)
+
 // Per-scene arguments of the kernels that honour per-scene convergence (both
 // null outside a residual-targeted batch).
 struct SceneArgs {
@@ -201,6 +225,18 @@ struct Buffers {
     float4* preBox = nullptr;     // 2n
     float4* blkTab = nullptr;     // sparse table of refit-block unions, 2*nb per level
     int32_t* candCount = nullptr; // n: broadphase candidates per segment (measurement, lazy)
+    // contact skin: grown boxes of the last walk (2n), kept candidate leaves
+    // (slot-major, YB_SKIN_CANDIDATES x n), their counts (< 0: incomplete),
+    // walk flags ([0] any, [1 + scene] per scene; set by k_aabb, cleared by
+    // k_init), counts of walking / reusing substeps, and the refit's root box
+    // (unused: the tree holds grown boxes)
+    float4* sbox = nullptr;
+    int32_t* cand = nullptr;
+    int32_t* candN = nullptr;
+    int32_t* skinState = nullptr;
+    unsigned long long* skinStats = nullptr;
+    int2* skinPolicy = nullptr;
+    float4* skinRoot = nullptr;
     int32_t* err = nullptr;       // [0] error bits, [1] warning code
     float* minSep = nullptr;      // smallest contact distance / (2 radius) seen
     DevParams* params = nullptr;
@@ -260,6 +296,10 @@ struct yb_sim {
     int sweepThreads = 256;  // block size of k_sweep (64, 128, 256 or 512)
     int queryThreads = 128;  // block size of k_query (64, 128 or 256)
     int queryStack = YB_QUERY_STACK_DEPTH;  // traversal stack entries a query may use
+    // contact skin (yb_set_contact_skin): margin in detection radii (<= 0: off) and
+    // the substeps of warm-start motion a grown box covers
+    float skinMargin = 0.1f;
+    float skinSweep = 8.0f;
     bool countCandidates = false;           // k_query writes b.candCount (measurement)
     // residual-targeted mode (tol > 0): prologue graphs per detection mode,
     // one graph per chunk of sweeps, one end graph
@@ -363,8 +403,10 @@ void launch_temporal(yb_sim* s);
 void launch_correction(yb_sim* s, int fin, bool adaptive, bool record);
 void launch_fill_i32(yb_sim* s, int32_t* p, int32_t v, int count);
 // broadphase (collision.cu:10-119 + the LBVH the reference links)
-void launch_aabb(yb_sim* s);
+void launch_aabb(yb_sim* s, bool force = false);  // force: every segment walks
+SkinArgs skin_args(const yb_sim* s);
 void launch_rebuild(yb_sim* s);    // scene box, Morton, sort, Karras
+void launch_scene_box(yb_sim* s);  // union of the current segment boxes -> sceneBox
 void launch_refit(yb_sim* s);
 void launch_query(yb_sim* s);      // traversal + narrowphase -> contact lists, step limit
 bool query_shape_supported(int stackLds, int candidates);

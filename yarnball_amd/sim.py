@@ -215,6 +215,29 @@ class Sim:
                                            ctypes.byref(c)))
         return a.value, b.value, c.value
 
+    def setContactSkin(self, margin=0.1, sweep=8.0):
+        """Contact skin: reuse each segment's broadphase candidates while its
+        box stays inside its box of the last walk grown by `sweep` substeps of
+        warm-start motion and by margin x detectionRadius (margin <= 0: walk the tree in
+        every detecting substep).  The contact lists are the same either way."""
+        self._need_handle()
+        check(self._lib.yb_set_contact_skin(self._h, float(margin), float(sweep)),
+              "setContactSkin")
+
+    def contactSkin(self):
+        self._need_handle()
+        a, b = ctypes.c_float(0), ctypes.c_float(0)
+        check(self._lib.yb_get_contact_skin(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def contactSkinStats(self, reset=False):
+        """(detecting substeps that walked the tree, that reused the kept
+        candidates) since the last reset."""
+        self._need_handle()
+        out = np.zeros(2, np.int64)
+        check(self._lib.yb_contact_skin_stats(self._h, ptr(out), int(bool(reset))))
+        return int(out[0]), int(out[1])
+
     def profile(self, substeps=2):
         """Run `substeps` substeps of maxH with HIP events around every stage
         (yb_profile); returns {stage: (milliseconds, launches)}."""
