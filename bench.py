@@ -348,6 +348,119 @@ def live_profile(sim, sc, args, substeps_per_frame):
     return ms, nl, c
 
 
+def broadphase_modes(sim, sc, reps=4):
+    """The contact query and the refit in each way a detecting substep runs
+    on the handle's current state, timed live (yb_profile, one substep each,
+    the fastest of `reps`) and counted on the device (yb_debug_query_counts):
+      reuse  the kept candidates of the last walk are tested (the skin's common
+             case; no refit);
+      walk   every segment walks with its grown box and keeps its candidates
+             (the kept lists are dropped first, so this substep walks);
+      plain  the skin off: the tree is refit on the segment boxes and walked
+             with them (the reference's way).
+    Each mode's counts come from one more substep run the same way."""
+    from yarnball_amd._lib import check, ptr
+
+    margin, sweep = sim.contactSkin()
+    h = np.float32(sc.maxH)
+
+    def counts(reuse):
+        # a prologue moves nothing: when one walks, the next reuses its lists
+        for _ in range(3):
+            qc = np.zeros(6, np.int64)
+            qp = sim.meta.copy()
+            sim.contactSkinStats(reset=True)
+            check(sim._lib.yb_debug_query_counts(sim.handle, ctypes.byref(qp), h, ptr(qc)))
+            if not reuse or sim.contactSkinStats(reset=True)[1] == 1:
+                return qc
+        return None
+
+    def drop_lists():
+        sim.setContactSkin(0.0)
+        sim.setContactSkin(margin, sweep)
+
+    out = {}
+    modes = (("reuse", None), ("walk", drop_lists), ("plain", None)) if margin > 0 else \
+        (("plain", None),)
+    for mode, before in modes:
+        if mode == "plain" and margin > 0:
+            sim.setContactSkin(0.0)
+        q, r, got = [], [], 0
+        for _ in range(reps * 3):
+            if before:
+                before()
+            sim.contactSkinStats(reset=True)
+            pr = sim.profile(1)
+            w_, u_ = sim.contactSkinStats(reset=True)
+            if mode == "reuse" and u_ != 1:
+                continue  # this substep walked: not a reuse sample
+            q.append(pr["query"][0] * 1e3)
+            r.append(pr["refit"][0] * 1e3)
+            got += 1
+            if got == reps:
+                break
+        if before:
+            before()
+        qc = counts(mode == "reuse") if q else None
+        out[mode] = {"query_us": min(q) if q else None, "refit_us": min(r) if r else None,
+                     "counts": qc}
+    if margin > 0:
+        sim.setContactSkin(margin, sweep)
+    return out
+
+
+def broadphase_records(bp, workload, n, skin_stats):
+    """Roofline records of the contact query per mode and of the refit.
+    Algorithmic bytes per segment (p leaves tested by the narrowphase and k
+    contacts per segment, pk kept candidates scanned, as counted on the
+    device): a walk reads its own box, endpoints and flags and writes its
+    list length and step limit (48), each tested leaf's endpoints (16) and
+    each contact's list entry and step-limit distance (8): 48 + 16p + 8k; a
+    reuse also reads its list length (4) and, per kept candidate, the entry
+    and that segment's box (4 + 32): 52 + 36pk + 16p + 8k.  The refit reads
+    a box and writes two child boxes per segment (52); a walk with the skin
+    also reads the warm start at both ends (+ 32)."""
+    recs = {}
+    for mode, r in bp.items():
+        qc = r["counts"]
+        if r["query_us"] is None or qc is None:
+            continue
+        segs = max(int(qc[0]), 1)
+        p, k = qc[1] / segs, qc[2] / segs
+        scanned = qc[3] / segs
+        b_seg = 48.0 + 16.0 * p + 8.0 * k
+        if mode == "reuse":
+            b_seg += 4.0 + 36.0 * scanned
+        us = r["query_us"]
+        ach = b_seg * segs / (us * 1e-6) / 1e9
+        rec = {"kernel": "query", "mode": mode, "bound": "hbm", "achieved": round(ach, 1),
+               "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+               "traffic": (round(MEASURED_TRAFFIC[(workload, "query_" + mode)] * n)
+                           if (workload, "query_" + mode) in MEASURED_TRAFFIC else None),
+               "alg_bytes_per_launch": int(b_seg * segs), "avg_launch_us": round(us, 2),
+               "tested_per_segment": round(float(p), 3), "accepted_per_segment": round(float(k), 3)}
+        if mode == "reuse":
+            rec["kept_scanned_per_segment"] = round(float(scanned), 2)
+        else:
+            rec["node_visits_per_segment"] = round(float(scanned), 2)
+            rec["stack_past_lds_fraction"] = round(float(qc[4] / segs), 4)
+            rec["max_stack_depth"] = int(qc[5])
+        recs["query_" + mode] = rec
+        if mode != "reuse" and r["refit_us"]:
+            rb = 52.0 + (32.0 if mode == "walk" else 0.0)
+            ach_r = rb * segs / (r["refit_us"] * 1e-6) / 1e9
+            recs["refit_" + mode] = {"kernel": "refit", "mode": mode, "bound": "hbm",
+                                     "achieved": round(ach_r, 1), "peak": HBM_PEAK_GBS,
+                                     "unit": "GB/s", "frac": round(ach_r / HBM_PEAK_GBS, 4),
+                                     "traffic": None, "alg_bytes_per_launch": int(rb * segs),
+                                     "avg_launch_us": round(r["refit_us"], 2)}
+    walks, reuses = skin_stats
+    recs["skin"] = {"walking_substeps": int(walks), "reusing_substeps": int(reuses),
+                    "refit_skipped_us": round(bp["reuse"]["refit_us"], 2)
+                    if "reuse" in bp and bp["reuse"]["refit_us"] is not None else None}
+    return recs
+
+
 def substep_timeline(sim, frame, substeps_per_frame):
     """One more frame (untimed) with the residual-mode substep timeline on:
     mean microseconds per substep of GPU idle before it, prologue, sweeps up
@@ -549,15 +662,10 @@ def main():
     main_m = max(learned_ms or [m for m in methods if m != "plain"] or methods,
                  key=lambda m: runs[m]["value"])
     ref = runs["plain" if "plain" in runs else main_m]
-    # broadphase counts on the headline method's final state: candidate leaves
-    # p and accepted contacts k per segment, for the query's byte model
+    # broadphase on the headline method's final state, in each way a
+    # detecting substep can run (counts for the byte models, and times)
     qsim = runs[main_m]["sim"]
-    qc = np.zeros(6, np.int64)
-    qp = qsim.meta.copy()
-    check(qsim._lib.yb_debug_query_counts(qsim.handle, ctypes.byref(qp), np.float32(sc.maxH),
-                                          ptr(qc)))
-    segs_q = max(int(qc[0]), 1)
-    p_cand, k_acc = qc[1] / segs_q, qc[2] / segs_q
+    bp = broadphase_modes(qsim, sc)
     for m in methods:
         if runs[m] is not ref:
             runs[m]["sim"].close()
@@ -601,31 +709,10 @@ def main():
         "avg_launch_us": round(per_launch[dominant] * 1e3, 2),
         "mean_contacts_per_segment": round(c, 3),
     }
-    # Broadphase: the query (traversal 24 + 8p, filter 8p + 8k + 4 bytes per
-    # segment, and the step limit it now also writes: dx read 16, limit 4) and
-    # the refit (52 bytes per segment), p candidate leaves and k accepted
-    # contacts per segment as counted on the device.
-    broad = {}
-    for st_, b_seg in (("query", 48.0 + 16.0 * p_cand + 8.0 * k_acc), ("refit", 52.0)):
-        i = STAGES.index(st_)
-        if not nl[i]:
-            continue
-        us = per_launch[st_] * 1e3
-        ach = b_seg * segs_q / (us * 1e-6) / 1e9
-        broad[st_] = {"kernel": st_, "bound": "hbm", "achieved": round(ach, 1),
-                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                      "traffic": (round(MEASURED_TRAFFIC[(args.workload, st_)] * sc.n)
-                                  if (args.workload, st_) in MEASURED_TRAFFIC else None),
-                      "alg_bytes_per_launch": int(b_seg * segs_q), "avg_launch_us": round(us, 2)}
-    if "query" in broad:
-        broad["query"]["candidates_per_segment"] = round(float(p_cand), 3)
-        broad["query"]["accepted_per_segment"] = round(float(k_acc), 3)
-        broad["query"]["node_visits_per_segment"] = round(float(qc[3] / segs_q), 2)
-        broad["query"]["stack_past_lds_fraction"] = round(float(qc[4] / segs_q), 4)
-        broad["query"]["max_stack_depth"] = int(qc[5])
-    print(f"[bench] broadphase: p {p_cand:.2f} candidates, k {k_acc:.2f} contacts per segment; "
-          + ", ".join(f"{k_} {v['avg_launch_us']:.1f} us {v['frac']:.3f} of peak"
-                      for k_, v in broad.items()), file=sys.stderr, flush=True)
+    broad = broadphase_records(bp, args.workload, sc.n, runs[main_m]["skin"])
+    print("[bench] broadphase: " + "; ".join(
+        f"{k_} {v['avg_launch_us']:.1f} us {v['frac']:.3f} of peak" for k_, v in broad.items()
+        if isinstance(v, dict) and "avg_launch_us" in v), file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu:
@@ -725,7 +812,7 @@ def main():
                           "query_shape": list((args.tile_config or (1024, 8, 6))[1:]),
                           "autotune": tuned},
         "roofline": roofline,
-        "roofline_broadphase": broad.get("query"),
+        "roofline_broadphase": broad.get("query_reuse") or broad.get("query_plain"),
         "broadphase": broad,
         "cpu_baseline": cpu,
         "batched": batched,

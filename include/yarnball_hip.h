@@ -244,8 +244,10 @@ int yb_debug_prologue(yb_sim* s, yb_params* p, float h, int detect, int rebuild)
  * that call does) and returns, summed over segments: out[0] segments, out[1]
  * candidate leaves whose boxes overlap a query box (leaves within two indices,
  * always exempt, not counted), out[2] accepted contacts
- * (the list lengths), out[3] tree nodes visited; out[4] the segments whose
- * traversal stack grew past 8 entries, out[5] the deepest stack. */
+ * (the list lengths), out[3] tree nodes visited (with the contact skin, in a
+ * substep that reuses the kept candidates: the candidates scanned); out[4]
+ * the segments whose traversal stack grew past 8 entries, out[5] the deepest
+ * stack. */
 int yb_debug_query_counts(yb_sim* s, yb_params* p, float h, int64_t out[6]);
 /* Substep timeline of the residual-targeted schedule (tol > 0), from HIP
  * events recorded on the handle's stream between its launches: one before

@@ -136,6 +136,17 @@ __global__ __launch_bounds__(256) void k_aabb(const DevParams* __restrict__ P,
     }
 }
 
+// Slot c of segment seg's kept candidate list: the first kSkinHead slots of
+// every segment are packed together (a segment's usual list is one 64-byte
+// piece of a line shared with its neighbours), the rest lie in an overflow
+// area.  Four consecutive slots from a multiple of 4 are one aligned int4.
+constexpr int kSkinHead = 16;
+__device__ __forceinline__ size_t skin_slot(int n, int seg, int c) {
+    return c < kSkinHead ? (size_t)seg * kSkinHead + c
+                         : (size_t)n * kSkinHead + (size_t)seg * (kSkinCand - kSkinHead) +
+                               (c - kSkinHead);
+}
+
 // Grown box of a segment for the contact skin: its box swept by `sweep`
 // substeps of its endpoints' warm-start displacement, at most `cap` along an
 // axis (per axis, the box of both endpoints at t = 0 and at the end of the
@@ -993,8 +1004,8 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
         // together instead of once per node visit at which any thread met a leaf.
         // Candidates keep their visit order, hence so do the contact lists.
         int nb = 0;
-        // measurement only: leaves tested by the narrowphase, node visits (kept
-        // candidates scanned, when reusing), deepest stack
+        // measurement only: leaves tested by the narrowphase, node visits (when
+        // reusing: kept candidates scanned), deepest stack
         int cand = 0, visits = 0, maxSp = 0;
         // The first four partners stay in registers and are stored at the end,
         // slot by slot, by the whole wave at once (one coalesced 256-byte store
@@ -1039,8 +1050,24 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
         // met (its grown box overlaps) in this segment's candidate list; nar:
         // test the leaves met (exact: against their current boxes first, when
         // the tree holds grown boxes).  Returns the leaves kept.
+        // A recording walk keeps each lane's leaves in its LDS columns (the
+        // narrowphase buffer, unused then: 2 kCand entries) and stores them
+        // to its contiguous list in bursts, which the L2 merges into whole
+        // lines; one store per leaf as it is met left most lines partly
+        // written at different times (5x the list's bytes written).
+        auto recSlot = [&](int k) -> int32_t& {
+            return k < kCand ? candJ[k][t]
+                             : reinterpret_cast<int32_t*>(&candPad[0][0])[(k - kCand) * kQueryThreads + t];
+        };
         auto walk = [&](v3 tlo, v3 thi, bool rec, bool nar, bool exact) {
-            int nc = 0;
+            int nc = 0, nr = 0;
+            auto flushRec = [&]() {
+    #pragma unroll 1
+                for (int k = 0; k < nr; k++)
+                    if (nc + k < kSkinCand) sk.cand[skin_slot(n, i, nc + k)] = recSlot(k);
+                nc += nr;
+                nr = 0;
+            };
             int sp = 0;
             volatile int32_t deep[kStack > kStackL ? kStack - kStackL : 1];  // entries past the LDS part
             // batched scenes: traverse only this segment's own scene subtree
@@ -1067,13 +1094,12 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
                 const bool addL = hl && cl < 0 && abs(~cl - i) > 2;
                 const bool addR = hr && cr < 0 && abs(~cr - i) > 2;
                 if (rec) {
-                    // each segment's list is contiguous: the lines one lane
-                    // writes over its walk are completed in the L2
-                    int32_t* mine = sk.cand + (size_t)i * kSkinCand;
-                    if (addL && nc < kSkinCand) mine[nc] = ~cl;
-                    nc += addL ? 1 : 0;
-                    if (addR && nc < kSkinCand) mine[nc] = ~cr;
-                    nc += addR ? 1 : 0;
+                    // slots nr and nr + 1 are free (nr <= 2 kCand - 2 here)
+                    recSlot(nr) = ~cl;
+                    nr += addL ? 1 : 0;
+                    recSlot(nr) = ~cr;
+                    nr += addR ? 1 : 0;
+                    if (__any(nr > 2 * kCand - 2)) flushRec();
                 }
                 if (nar) {
                     candJ[nb][t] = ~cl;
@@ -1110,15 +1136,16 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
                 }
             }
             narrowphase(exact);
+            flushRec();
             return nc;
         };
         // Kept candidates, in the order the walk met them: each one whose
         // current box overlaps this segment's goes to the narrowphase buffer.
         // Four at a time, with one 16-byte load of the list.
-        auto reuse = [&](int cn) {
-            const int4* list = reinterpret_cast<const int4*>(sk.cand + (size_t)i * kSkinCand);
+        auto reuse = [&](int cn, bool countScan) {
             for (int c = 0; __any(c < cn); c += 4) {
-                const int4 q = c < cn ? list[c >> 2] : make_int4(i, i, i, i);
+                const int4 q = c < cn ? *reinterpret_cast<const int4*>(sk.cand + skin_slot(n, i, c))
+                                      : make_int4(i, i, i, i);
                 const int js[4] = {c < cn ? q.x : i, c + 1 < cn ? q.y : i, c + 2 < cn ? q.z : i,
                                    c + 3 < cn ? q.w : i};
                 float4 blo[4], bhi[4];
@@ -1135,12 +1162,12 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
                     nb += add ? 1 : 0;
                     if (__any(nb > kCand - 1)) narrowphase(false);
                 }
-                if (candCount) visits += min(4, max(cn - c, 0));
+                if (candCount && countScan) visits += min(4, max(cn - c, 0));
             }
             narrowphase(false);
         };
         if (mode == 2) {
-            reuse(min(sk.candN[i], kSkinCand));
+            reuse(min(sk.candN[i], kSkinCand), true);
         } else if (mode == 1) {
             // Walk with the grown box (the refit kept it in sbox) only to keep
             // the candidates, then test them as a reusing substep does.  A list
@@ -1151,7 +1178,7 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
             const bool kept = nc <= kSkinCand && !overflowStack;
             sk.candN[i] = kept ? nc : -1;
             if (kept) {
-                reuse(nc);
+                reuse(nc, false);
             } else {
                 // the stack overflow reported is the one of the walk that
                 // makes the list, as without the skin

@@ -73,7 +73,7 @@ struct SceneCtl {
 struct SkinArgs {
     int32_t* state;               // [0] some segment walks, [1 + scene] the scene walks
     float4* sbox;                 // grown box of each segment's last walk
-    int32_t* cand;                // kept candidates, slot-major cand[seg + n * slot]
+    int32_t* cand;                // kept candidates (kSkinCand per segment, skin_slot)
     int32_t* candN;               // kept candidates per segment (< 0: incomplete)
     unsigned long long* stats;    // [0] walking, [1] reusing detection substeps
     int2* policy;                 // per scene: (substeps left walking without the skin,
@@ -226,7 +226,7 @@ struct Buffers {
     float4* blkTab = nullptr;     // sparse table of refit-block unions, 2*nb per level
     int32_t* candCount = nullptr; // n: broadphase candidates per segment (measurement, lazy)
     // contact skin: grown boxes of the last walk (2n), kept candidate leaves
-    // (slot-major, YB_SKIN_CANDIDATES x n), their counts (< 0: incomplete),
+    // (YB_SKIN_CANDIDATES x n, laid out by skin_slot), their counts (< 0: incomplete),
     // walk flags ([0] any, [1 + scene] per scene; set by k_aabb, cleared by
     // k_init), counts of walking / reusing substeps, and the refit's root box
     // (unused: the tree holds grown boxes)
