@@ -60,12 +60,12 @@ def stage_bytes(stage, c):
 
 # HBM bytes per vertex of one launch, from rocprofv3 FETCH_SIZE and WRITE_SIZE
 # scaled by a copy kernel's counts: k_sweep and k_query (with the step limit it
-# computes) on the knit patch (256 x 4096, residual 1e-6, learned + strand,
-# frames 5-6, c = 4.58 contacts per segment; profiles/substep_knit1M_hbm_traffic.md)
-# and k_sweep on the 512-scene batch (8 sweeps, c = 0.24;
-# profiles/sweep_batched512_hbm_traffic.md).
+# computes; reusing its kept candidates, and walking) on the knit patch
+# (256 x 4096, residual 1e-6, learned + strand, frames 0-6;
+# profiles/substep_knit1M_hbm_traffic.md) and k_sweep on the 512-scene batch
+# (8 sweeps, c = 0.24; profiles/sweep_batched512_hbm_traffic.md).
 MEASURED_TRAFFIC = {("knit_patch", "sweep"): 160.5, ("batched_walks", "sweep"): 131.3,
-                    ("knit_patch", "query"): 397.3}
+                    ("knit_patch", "query_reuse"): 285.6, ("knit_patch", "query_walk"): 526.9}
 
 
 def build_workload(name, numItr):
@@ -411,13 +411,12 @@ def broadphase_modes(sim, sc, reps=4):
 
 def broadphase_records(bp, workload, n, skin_stats):
     """Roofline records of the contact query per mode and of the refit.
-    Algorithmic bytes per segment (p leaves tested by the narrowphase and k
-    contacts per segment, pk kept candidates scanned, as counted on the
-    device): a walk reads its own box, endpoints and flags and writes its
-    list length and step limit (48), each tested leaf's endpoints (16) and
-    each contact's list entry and step-limit distance (8): 48 + 16p + 8k; a
-    reuse also reads its list length (4) and, per kept candidate, the entry
-    and that segment's box (4 + 32): 52 + 36pk + 16p + 8k.  The refit reads
+    Algorithmic bytes per segment, in the survey's convention (p leaves
+    tested by the narrowphase, k contacts and pk kept candidates scanned per
+    segment, as counted on the device): a walk is traversal 24 + 8p, pair
+    filter 8p + 8k + 4 and step limit 20 (the warm start read, the limit
+    written): 48 + 16p + 8k; a reuse reads its list length and entries
+    (4 + 4pk) in place of the traversal: 28 + 4pk + 8p + 8k.  The refit reads
     a box and writes two child boxes per segment (52); a walk with the skin
     also reads the warm start at both ends (+ 32)."""
     recs = {}
@@ -428,9 +427,8 @@ def broadphase_records(bp, workload, n, skin_stats):
         segs = max(int(qc[0]), 1)
         p, k = qc[1] / segs, qc[2] / segs
         scanned = qc[3] / segs
-        b_seg = 48.0 + 16.0 * p + 8.0 * k
-        if mode == "reuse":
-            b_seg += 4.0 + 36.0 * scanned
+        b_seg = (28.0 + 4.0 * scanned + 8.0 * p + 8.0 * k) if mode == "reuse" else \
+            (48.0 + 16.0 * p + 8.0 * k)
         us = r["query_us"]
         ach = b_seg * segs / (us * 1e-6) / 1e9
         rec = {"kernel": "query", "mode": mode, "bound": "hbm", "achieved": round(ach, 1),

@@ -390,9 +390,10 @@ int yb_get_tile_config(yb_sim* s, int* refitLeaves, int* queryStackLds, int* que
  * refits the tree and walks it in every detecting substep, collision.cu:94-119;
  * the contact lists are the same either way).  With margin > 0, a detecting
  * substep that must walk the tree grows every segment's box by its warm-start
- * displacement times `sweep` (the motion of `sweep` substeps at that speed)
- * and by margin x meta.detectionRadius on every side, walks the tree refit on those
- * grown boxes, and keeps each segment's candidate leaves (at most
+ * displacement times `sweep` (the motion of `sweep` substeps at that speed,
+ * at most 8 detection radii along an axis) and by margin x
+ * meta.detectionRadius on every side, walks the tree refit on those grown
+ * boxes, and keeps each segment's candidate leaves (at most
  * YB_SKIN_CANDIDATES, in the walk's order).  Later detecting substeps skip
  * the refit and the walk while every segment's current box lies inside its
  * grown box, and test only the kept candidates: two segments whose current
@@ -400,9 +401,11 @@ int yb_get_tile_config(yb_sim* s, int* refitLeaves, int* queryStackLds, int* que
  * The leaf order of a walk is the tree's in-order leaf order, so the contact
  * lists equal those of a walk, slot for slot.  Every other case -- a box
  * outside its grown box (in a batch, per scene), a full candidate list, a
- * traversal-stack overflow, a tree rebuild, new scenes -- walks again.
- * margin <= 0 turns the skin off (the tree is refit and walked in every
- * detecting substep).  Default: margin 0.1, sweep 8. */
+ * traversal-stack overflow, a tree rebuild, new scenes -- walks again.  A
+ * scene whose lists are reused fewer than twice before it must walk again
+ * walks the plain way (no grown boxes, nothing kept) for 16 substeps, then
+ * tries the skin again.  margin <= 0 turns the skin off (the tree is refit
+ * and walked in every detecting substep).  Default: margin 0.05, sweep 8. */
 #define YB_SKIN_CANDIDATES 64
 int yb_set_contact_skin(yb_sim* s, float margin, float sweep);
 int yb_get_contact_skin(yb_sim* s, float* margin, float* sweep);

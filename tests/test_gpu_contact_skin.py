@@ -33,7 +33,7 @@ def _sim(sc, skin, residual=0.0, strand=False, offs=None):
     return s
 
 
-def _run_pair(sc, substeps, skin=(0.1, 8.0), residual=0.0, strand=False, offs=None,
+def _run_pair(sc, substeps, skin=(0.05, 8.0), residual=0.0, strand=False, offs=None,
               lists_every=1):
     """Advance a handle with the skin and one without, one substep at a time,
     comparing the contact lists after every `lists_every` substeps and the
@@ -69,7 +69,7 @@ def test_skin_is_on_by_default_and_settable():
     sc = scenes.readme_scene()
     s = Sim(sc.n)
     scenes.load_into(s, sc)
-    assert s.contactSkin() == pytest.approx((0.1, 8.0))
+    assert s.contactSkin() == pytest.approx((0.05, 8.0))
     s.setContactSkin(0.5, 3.0)
     assert s.contactSkin() == pytest.approx((0.5, 3.0))
     s.setContactSkin(0.0)
@@ -124,7 +124,7 @@ def test_cable_twist_with_the_skin():
     still equal the walk's."""
     sc = scenes.twisted_cable(strands=7, per=256)
     frame = np.float32(sc.frame_dt)
-    a = _sim(sc, (0.1, 8.0))
+    a = _sim(sc, (0.05, 8.0))
     b = _sim(sc, None)
     d = Sim.defaultDriver(DRIVE_TWIST)
     d.speed, d.start, d.duration, d.release, d.side = 2 * np.pi, 0.0, 1e9, -1.0, 1
@@ -178,7 +178,7 @@ def test_full_candidate_lists_walk_again():
     and the same contact-overflow error as without the skin."""
     from test_gpu_glue_and_errors import overflow_scene
     sc = overflow_scene()
-    a = _sim(sc, (0.1, 8.0))
+    a = _sim(sc, (0.05, 8.0))
     b = _sim(sc, None)
     for k in range(4):
         codes = []

@@ -44,10 +44,12 @@ __global__ __launch_bounds__(256) void k_init(const DevParams* __restrict__ P,
     if (v < skinScenes) {
         const int f = skinScenes > 1 ? 1 + v : 0;
         int2 pol = skinPolicy[v];
-        const bool walked = skinState[f] != 0;
+        const int32_t walked = skinState[f];
         if (pol.x > 0) {
             pol.x--;
             pol.y = 0;
+        } else if (walked == 2) {
+            pol.y = 1;  // a new tree: the lists it made are fresh, not short-lived
         } else if (walked) {
             if (pol.y >= 1 && pol.y - 1 < kSkinMinReuse) {
                 pol.x = kSkinBackoff;
@@ -129,9 +131,11 @@ __global__ __launch_bounds__(256) void k_aabb(const DevParams* __restrict__ P,
                                 (hi.x <= s1.x) & (hi.y <= s1.y) & (hi.z <= s1.z);
             walk = !inside || sk.candN[v] < 0;
         }
+        // 2: every segment walks (a new tree), 1: some segment's list ended
         if (walk) {
-            sk.state[0] = 1;
-            if (sceneOf) sk.state[1 + sceneOf[v]] = 1;
+            const int32_t why = force ? 2 : 1;
+            sk.state[0] = why;
+            if (sceneOf) sk.state[1 + sceneOf[v]] = why;
         }
     }
 }

@@ -72,6 +72,7 @@ struct SceneCtl {
 // Contact-skin arguments of the broadphase kernels (state null: skin off).
 struct SkinArgs {
     int32_t* state;               // [0] some segment walks, [1 + scene] the scene walks
+                                  // (1: a list ended, 2: a new tree)
     float4* sbox;                 // grown box of each segment's last walk
     int32_t* cand;                // kept candidates (kSkinCand per segment, skin_slot)
     int32_t* candN;               // kept candidates per segment (< 0: incomplete)
@@ -87,7 +88,7 @@ constexpr int kSkinCand = YB_SKIN_CANDIDATES;
 constexpr int kSkinMinReuse = 2;
 constexpr int kSkinBackoff = 16;
 // Growth by motion of a grown box, per axis, in detection radii at most.
-constexpr float kSkinCapRadii = 4.0f;
+constexpr float kSkinCapRadii = 8.0f;
 constexpr int kMaxScenes = 65536;  // scenes of one handle (yb_set_scenes)
 
 // Per-scene arguments of the kernels that honour per-scene convergence (both
@@ -298,7 +299,7 @@ struct yb_sim {
     int queryStack = YB_QUERY_STACK_DEPTH;  // traversal stack entries a query may use
     // contact skin (yb_set_contact_skin): margin in detection radii (<= 0: off) and
     // the substeps of warm-start motion a grown box covers
-    float skinMargin = 0.1f;
+    float skinMargin = 0.05f;
     float skinSweep = 8.0f;
     bool countCandidates = false;           // k_query writes b.candCount (measurement)
     // residual-targeted mode (tol > 0): prologue graphs per detection mode,

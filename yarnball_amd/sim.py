@@ -215,7 +215,7 @@ class Sim:
                                            ctypes.byref(c)))
         return a.value, b.value, c.value
 
-    def setContactSkin(self, margin=0.1, sweep=8.0):
+    def setContactSkin(self, margin=0.05, sweep=8.0):
         """Contact skin: reuse each segment's broadphase candidates while its
         box stays inside its box of the last walk grown by `sweep` substeps of
         warm-start motion and by margin x detectionRadius (margin <= 0: walk the tree in
