@@ -935,12 +935,16 @@ __device__ __forceinline__ bool pair_accepted(const DevParams* __restrict__ P,
 // *minSep -- the contact lists are the pairs it accepts, so these are the
 // distances the step limit kernel would measure again.
 // Contact skin (sk.state set; the tree then holds grown boxes): a segment of a
-// walking scene walks with its grown box, keeps every leaf it meets as a
-// candidate and tests each against its current box before the pair test; a
-// segment of any other scene walks nothing and tests its kept candidates.
-// Both visit the candidates in the tree's in-order leaf order, as the walk
-// without the skin does, so every mode gives the same lists.
-template <int kQueryThreads, int kStackL, int kCand>
+// walking scene walks with its grown box and keeps every leaf it meets as a
+// candidate; the kept candidates (fresh or from an earlier walk) are tested
+// against the current box before the pair test.  Both visit the candidates
+// in the tree's in-order leaf order, as the walk without the skin does, so
+// every mode gives the same lists.  Two launches share the work, each with
+// its own register budget: phase 0 makes every walk, phase 1 tests kept
+// lists; a segment's lists, list length and step limit are written by the
+// phase that finishes it, and a wave with nothing to do in a phase returns
+// at once.
+template <int kQueryThreads, int kStackL, int kCand, int kPhase>
 __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __restrict__ P,
                                                          const int32_t* __restrict__ order,
                                                          const float4* __restrict__ aabb,
@@ -957,9 +961,14 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
                                                          float4* __restrict__ dxLimit,
                                                          float* __restrict__ minSep,
                                                          SkinArgs sk) {
-    __shared__ int32_t stk[kStackL][kQueryThreads];
-    __shared__ int32_t candJ[kCand][kQueryThreads];
-    __shared__ float candPad[kCand][kQueryThreads];
+    // One LDS array, kRows words per thread: the traversal stack (kStackL
+    // rows), the narrowphase buffer (kCand rows of leaves, kCand of pads as
+    // float bits); a reusing query lays its wave's work list over all of it.
+    constexpr int kRows = kStackL + 2 * kCand;
+    __shared__ int32_t lds[kRows][kQueryThreads];
+    int32_t(*stk)[kQueryThreads] = lds;
+    int32_t(*candJ)[kQueryThreads] = lds + kStackL;
+    int32_t(*candPad)[kQueryThreads] = lds + kStackL + kCand;
     const int n = P->n;
     // Segments in index order: consecutive segments of a strand are spatial
     // neighbours (coherent traversals) and the slot-major list writes
@@ -975,12 +984,21 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
     if (sk.state) {
         const int sc = sceneOf ? sceneOf[min(i, n - 1)] : 0;
         mode = sk.policy[sc].x > 0 ? 3 : (sk.state[sceneOf ? 1 + sc : 0] ? 1 : 2);
-        if (sk.stats && blockIdx.x == 0 && threadIdx.x == 0)
+        if (kPhase == 0 && sk.stats && blockIdx.x == 0 && threadIdx.x == 0)
             atomicAdd(&sk.stats[sk.state[0] ? 0 : 1], 1ull);
     }
+    // fin: this phase writes the segment's lists and step limit; phase 0
+    // finishes every mode but the reuse of a kept list, phase 1 that reuse
+    // (of a list kept now, or earlier)
+    bool fin;
+    if constexpr (kPhase == 0)
+        fin = i < n && mode != 2;
+    else
+        fin = i < n && (mode == 2 || (mode == 1 && seg && sk.candN[i] >= 0));
+    if (!__any(fin)) return;  // the whole wave is another phase's
     float minD2 = INFINITY;  // closest listed contact, squared
     if (!seg) {
-        if (i < n) {
+        if (fin) {
             numCols[i] = 0;
             if (mode == 1) sk.candN[i] = 0;
             if (candCount) {
@@ -1024,7 +1042,7 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
     #pragma unroll 1
             for (int k = 0; k < nb; k++) {
                 const int j = candJ[k][t];
-                float pw = candPad[k][t];
+                float pw = __int_as_float(candPad[k][t]);
                 bool hit = true;
                 if (exact) {
                     const float4 blo = aabb[2 * j];
@@ -1060,8 +1078,7 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
         // lines; one store per leaf as it is met left most lines partly
         // written at different times (5x the list's bytes written).
         auto recSlot = [&](int k) -> int32_t& {
-            return k < kCand ? candJ[k][t]
-                             : reinterpret_cast<int32_t*>(&candPad[0][0])[(k - kCand) * kQueryThreads + t];
+            return k < kCand ? candJ[k][t] : candPad[k - kCand][t];
         };
         auto walk = [&](v3 tlo, v3 thi, bool rec, bool nar, bool exact) {
             int nc = 0, nr = 0;
@@ -1107,10 +1124,10 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
                 }
                 if (nar) {
                     candJ[nb][t] = ~cl;
-                    candPad[nb][t] = l1.w;
+                    candPad[nb][t] = __float_as_int(l1.w);
                     nb += addL ? 1 : 0;
                     candJ[nb][t] = ~cr;
-                    candPad[nb][t] = r1.w;
+                    candPad[nb][t] = __float_as_int(r1.w);
                     nb += addR ? 1 : 0;
                 }
                 // at most two leaves per visit: flush before a buffer could overflow
@@ -1143,56 +1160,151 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
             flushRec();
             return nc;
         };
-        // Kept candidates, in the order the walk met them: each one whose
-        // current box overlaps this segment's goes to the narrowphase buffer.
-        // Four at a time, with one 16-byte load of the list.
+        // Kept candidates, in the order the walk met them, kG per lane and
+        // step.  Their current boxes are tested against this segment's; the
+        // pair tests of the ones that overlap are then shared out over the
+        // wave: each step lays the wave's overlapping candidates out in one
+        // LDS list (by candidate slot, then lane), every lane tests one entry
+        // per round (the owner segment's endpoints and pad read from LDS),
+        // and each owner takes its results in slot order -- so the lists are
+        // the same as testing them lane by lane, in about half the rounds
+        // (a wave otherwise runs as many tests as its busiest lane needs).
+        constexpr int kOwnRows = 7;                          // owner p0, p1, signed pad
+        constexpr int kEntRows = (kRows - kOwnRows) / 3;     // rows per entry field
+        constexpr int kG = kEntRows >= 4 ? 4 : 1;            // candidates per lane per step
+        static_assert(kEntRows >= 1, "LDS shape too small for the reuse list");
+        const int wb = threadIdx.x & ~63;  // the wave's first column
+        const int ln = threadIdx.x & 63;
+        const int iBase = i - ln;          // segment of the wave's lane 0
+        auto wave_sync = []() {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        };
+        auto ent = [&](int field, int e) -> int32_t& {
+            return lds[kOwnRows + field * kEntRows + (e >> 6)][wb + (e & 63)];
+        };
         auto reuse = [&](int cn, bool countScan) {
-            for (int c = 0; __any(c < cn); c += 4) {
-                const int4 q = c < cn ? *reinterpret_cast<const int4*>(sk.cand + skin_slot(n, i, c))
-                                      : make_int4(i, i, i, i);
-                const int js[4] = {c < cn ? q.x : i, c + 1 < cn ? q.y : i, c + 2 < cn ? q.z : i,
-                                   c + 3 < cn ? q.w : i};
-                float4 blo[4], bhi[4];
+            lds[0][wb + ln] = __float_as_int(si.p0.x);
+            lds[1][wb + ln] = __float_as_int(si.p0.y);
+            lds[2][wb + ln] = __float_as_int(si.p0.z);
+            lds[3][wb + ln] = __float_as_int(si.p1.x);
+            lds[4][wb + ln] = __float_as_int(si.p1.y);
+            lds[5][wb + ln] = __float_as_int(si.p1.z);
+            lds[6][wb + ln] = __float_as_int(si.glued ? -si.vpad : si.vpad);
+            for (int c = 0; __any(c < cn); c += kG) {
+                int js[kG];
+                if constexpr (kG == 4) {
+                    const int4 q = c < cn ? *reinterpret_cast<const int4*>(sk.cand + skin_slot(n, i, c))
+                                          : make_int4(i, i, i, i);
+                    js[0] = c < cn ? q.x : i;
+                    js[1] = c + 1 < cn ? q.y : i;
+                    js[2] = c + 2 < cn ? q.z : i;
+                    js[3] = c + 3 < cn ? q.w : i;
+                } else {
+                    js[0] = c < cn ? sk.cand[skin_slot(n, i, c)] : i;
+                }
+                float4 blo[kG], bhi[kG];
     #pragma unroll
-                for (int k = 0; k < 4; k++) {
+                for (int k = 0; k < kG; k++) {
                     blo[k] = aabb[2 * js[k]];
                     bhi[k] = aabb[2 * js[k] + 1];
                 }
+                // list positions: slot k of every lane, then slot k + 1
+                bool hitk[kG];
+                int at[kG];
+                int m = 0;
     #pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    const bool add = c + k < cn && overlap(qlo, qhi, blo[k], bhi[k]);
-                    candJ[nb][t] = js[k];
-                    candPad[nb][t] = bhi[k].w;
-                    nb += add ? 1 : 0;
-                    if (__any(nb > kCand - 1)) narrowphase(false);
+                for (int k = 0; k < kG; k++) {
+                    hitk[k] = c + k < cn && overlap(qlo, qhi, blo[k], bhi[k]);
+                    const uint64_t b = __ballot(hitk[k]);
+                    at[k] = m + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32),
+                                                                __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0));
+                    m += __popcll(b);
+                    if (hitk[k]) {
+                        ent(0, at[k]) = js[k];
+                        ent(1, at[k]) = ln;
+                        ent(2, at[k]) = __float_as_int(bhi[k].w);
+                    }
                 }
-                if (candCount && countScan) visits += min(4, max(cn - c, 0));
+                wave_sync();
+                // every lane tests entries rank, rank + A, ... of the m listed
+                const uint64_t act = __ballot(1);
+                const int A = __popcll(act);
+                const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(act >> 32),
+                                                                __builtin_amdgcn_mbcnt_lo((uint32_t)act, 0));
+    #pragma unroll 1
+                for (int e = rank; e < m; e += A) {
+                    const int j = ent(0, e);
+                    const int ow = ent(1, e);
+                    const float pw = __int_as_float(ent(2, e));
+                    SegInfo so;
+                    so.p0 = v3{__int_as_float(lds[0][wb + ow]), __int_as_float(lds[1][wb + ow]),
+                               __int_as_float(lds[2][wb + ow])};
+                    so.p1 = v3{__int_as_float(lds[3][wb + ow]), __int_as_float(lds[4][wb + ow]),
+                               __int_as_float(lds[5][wb + ow])};
+                    const float vo = __int_as_float(lds[6][wb + ow]);
+                    so.vpad = fabsf(vo);
+                    so.glued = signbit(vo);
+                    SegInfo sj;
+                    sj.p0 = xyz(gather4(pos, j));
+                    sj.p1 = xyz(gather4<true>(pos, j));
+                    sj.vpad = fabsf(pw);
+                    sj.glued = signbit(pw);
+                    float d2;
+                    const bool acc = pair_accepted(P, prm, iBase + ow, so, j, sj, d2);
+                    ent(2, e) = __float_as_int(acc ? d2 : -1.0f);
+                }
+                wave_sync();
+                // each owner takes its results in slot order
+    #pragma unroll
+                for (int k = 0; k < kG; k++) {
+                    if (!hitk[k]) continue;
+                    const float d2 = __int_as_float(ent(2, at[k]));
+                    cand++;
+                    if (d2 >= 0.0f) {
+                        const int j = js[k];
+                        minD2 = fminf(minD2, d2);
+                        r0 = cnt == 0 ? j : r0;
+                        r1 = cnt == 1 ? j : r1;
+                        r2 = cnt == 2 ? j : r2;
+                        r3 = cnt == 3 ? j : r3;
+                        if (cnt >= 4 && cnt < kMaxCols) cols[i + (size_t)n * cnt] = j;
+                        cnt++;
+                    }
+                }
+                wave_sync();
+                if (candCount && countScan) visits += min(kG, max(cn - c, 0));
             }
-            narrowphase(false);
         };
-        if (mode == 2) {
-            reuse(min(sk.candN[i], kSkinCand), true);
+        if constexpr (kPhase == 1) {
+            if (fin) reuse(min(sk.candN[i], kSkinCand), mode == 2);
         } else if (mode == 1) {
             // Walk with the grown box (the refit kept it in sbox) only to keep
-            // the candidates, then test them as a reusing substep does.  A list
-            // that is full or missed a subtree is not kept (the next detection
-            // walks again) and is not used: its segment walks once more with
-            // its current box, testing every leaf it meets.
+            // the candidates; phase 1 tests them as for a reusing substep.  A
+            // list that is full or missed a subtree is not kept (the next
+            // detection walks again) and is not used: its segment walks once
+            // more with its current box, testing every leaf it meets.
             const int nc = walk(xyz(sk.sbox[2 * i]), xyz(sk.sbox[2 * i + 1]), true, false, false);
             const bool kept = nc <= kSkinCand && !overflowStack;
             sk.candN[i] = kept ? nc : -1;
             if (kept) {
-                reuse(nc, false);
+                fin = false;
+                if (candCount) {
+                    candCount[n + i] = visits;
+                    candCount[2 * n + i] = maxSp;
+                }
             } else {
                 // the stack overflow reported is the one of the walk that
                 // makes the list, as without the skin
                 overflowStack = false;
                 walk(qlo, qhi, false, true, true);
             }
-        } else {
+        } else if (mode != 2) {
             walk(qlo, qhi, false, true, false);
             if (mode == 3) sk.candN[i] = -1;  // an old list may follow an older tree
         }
+      if (fin) {
         if (cnt > 0) cols[i] = r0;
         if (cnt > 1) cols[i + (size_t)n] = r1;
         if (cnt > 2) cols[i + 2 * (size_t)n] = r2;
@@ -1206,14 +1318,17 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(const DevParams* __rest
         numCols[i] = cnt;
         if (candCount) {
             candCount[i] = cand;
-            candCount[n + i] = visits;
-            candCount[2 * n + i] = maxSp;
+            if (kPhase == 0 || mode == 2) {
+                candCount[n + i] = visits;
+                candCount[2 * n + i] = maxSp;
+            }
         }
+      }
     }
     if (dxLimit) {
         // the step limit (collision.cu:121-159), every lane of the wave
         const float minContact = sqrtf(minD2);
-        if (i < n) {
+        if (fin) {
             float minDist = INFINITY;
             if (seg && P->useStepSizeLimit) {
                 minDist = P->detectionRadius * (P->detectionScaler - 1.0f);
@@ -1358,13 +1473,23 @@ void launch_refit(yb_sim* s) {
 // LDS stack with 16 candidates (384 B per thread, 1.5 waves per SIMD): the
 // traversal waits little on memory, but a SIMD needs several waves to
 // interleave its VALU, scalar, branch and LDS instructions.
+// Phase 0 (every walk) with the tuned LDS shape; with the contact skin,
+// phase 1 (the tests of kept lists) follows, with the shape whose LDS holds
+// its 256-entry work list.
+template <int T, int SL, int C, int Ph>
+static void launch_query_phase(yb_sim* s) {
+    hipLaunchKernelGGL((k_query<T, SL, C, Ph>), dim3(blocks(s->n, T)), dim3(T), 0, s->stream,
+                       s->b.params, s->b.order, s->b.aabb, s->b.nodes, s->b.pos, s->b.prm,
+                       s->b.numCols, s->b.cols, s->b.err,
+                       s->numScenes > 1 ? s->b.sceneOf : nullptr, s->b.sceneRoot, s->queryStack,
+                       s->countCandidates ? s->b.candCount : nullptr, s->b.dx[0], s->b.minSep,
+                       skin_args(s));
+}
+
 template <int T, int SL, int C>
 static void launch_query_t(yb_sim* s) {
-    hipLaunchKernelGGL((k_query<T, SL, C>), dim3(blocks(s->n, T)), dim3(T), 0, s->stream, s->b.params,
-                       s->b.order, s->b.aabb, s->b.nodes, s->b.pos, s->b.prm, s->b.numCols,
-                       s->b.cols, s->b.err, s->numScenes > 1 ? s->b.sceneOf : nullptr,
-                       s->b.sceneRoot, s->queryStack, s->countCandidates ? s->b.candCount : nullptr,
-                       s->b.dx[0], s->b.minSep, skin_args(s));
+    launch_query_phase<T, SL, C, 0>(s);
+    if (skin_args(s).state) launch_query_phase<T, 8, 6, 1>(s);
 }
 
 // Query shapes (LDS stack entries, buffered candidates) the autotuner picks
