@@ -161,6 +161,7 @@ def measure_batched(args, steps, warmup, per_scene_steps, with_cpu):
         b.setTileConfig(*tile_config)
     if args.skin:
         b.setContactSkin(*args.skin)
+    skin_m = b.contactSkin()[0]  # a batch runs without the skin unless asked
     for w in range(warmup):
         b.advance(frame)
     t0 = time.perf_counter()
@@ -310,6 +311,7 @@ def measure_batched(args, steps, warmup, per_scene_steps, with_cpu):
                    "scene_vertices": [int(sizes.min()), int(sizes.max())],
                    "substeps_per_step": spf, "numItr": int(args.numItr),
                    "launch": "eager" if args.no_graphs else "hip graphs",
+                   "contact_skin_margin": round(float(skin_m), 4),
                    "min_separation_x2r": round(min_sep, 3)},
         "per_scene_launches": per,
         "launch_config": {"sweep_threads": (launch_config or (256, 128))[0],

@@ -405,7 +405,9 @@ int yb_get_tile_config(yb_sim* s, int* refitLeaves, int* queryStackLds, int* que
  * scene whose lists are reused fewer than twice before it must walk again
  * walks the plain way (no grown boxes, nothing kept) for 16 substeps, then
  * tries the skin again.  margin <= 0 turns the skin off (the tree is refit
- * and walked in every detecting substep).  Default: margin 0.05, sweep 8. */
+ * and walked in every detecting substep).  Default: margin 0.05, sweep 8 for
+ * a handle of one scene; a handle given several scenes (yb_set_scenes)
+ * starts with the skin off unless this was called. */
 #define YB_SKIN_CANDIDATES 64
 int yb_set_contact_skin(yb_sim* s, float margin, float sweep);
 int yb_get_contact_skin(yb_sim* s, float* margin, float* sweep);

@@ -80,6 +80,25 @@ def test_skin_is_on_by_default_and_settable():
 
 
 @pytest.mark.gpu
+def test_batches_start_without_the_skin():
+    """A handle given several scenes starts with the skin off (unless the
+    caller chose a setting); back to one scene, the default returns."""
+    scs = scenes.random_walks([300, 400], seed=2)
+    big, offs = scenes.batch(scs)
+    s = Sim(big.n)
+    scenes.load_into(s, big)
+    assert s.contactSkin()[0] == pytest.approx(0.05)
+    s.setScenes(offs)
+    assert s.contactSkin()[0] == 0.0
+    s.setScenes([0, big.n])
+    assert s.contactSkin()[0] == pytest.approx(0.05)
+    s.setContactSkin(0.1, 4.0)
+    s.setScenes(offs)
+    assert s.contactSkin() == pytest.approx((0.1, 4.0))
+    s.close()
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("residual", [0.0, 1e-5])
 def test_knit_lists_and_states_equal_a_walk_every_substep(residual):
     """The pulled knit (one edge moving at 5 cm/s): the lists are reused in

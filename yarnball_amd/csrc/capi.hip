@@ -79,6 +79,7 @@ static int timeline_mark(yb_sim* s) {
 }
 
 static int reset_scene_ctl(yb_sim* s);
+static int set_skin(yb_sim* s, float margin, float sweep);
 
 static void free_all(yb_sim* s) {
     timeline_free(s);
@@ -478,6 +479,8 @@ extern "C" int yb_set_scenes(yb_sim* s, int numScenes, const int32_t* offsets) {
         }
     HIPCHECK(hipStreamSynchronize(s->stream));
     int rc = clear_scenes(s);
+    if (rc == YB_OK && numScenes == 1 && !s->skinSet)
+        rc = set_skin(s, kSkinDefaultMargin, kSkinDefaultSweep);
     if (rc != YB_OK || numScenes == 1) return rc;
     Buffers& b = s->b;
     const size_t N = (size_t)s->n;
@@ -515,9 +518,16 @@ extern "C" int yb_set_scenes(yb_sim* s, int numScenes, const int32_t* offsets) {
                 return rc == YB_OK ? YB_ERR_INVALID_ARGUMENT : rc;
             }
     }
-    // the contact skin's per-scene state starts over
+    // the contact skin's per-scene state starts over; a batch starts without
+    // the skin unless the caller chose a setting (its scenes' motion varies,
+    // and where most move too fast for kept lists the launches over the whole
+    // batch cost more than the reuses save)
     HIPCHECK(hipMemsetAsync(b.skinPolicy, 0, sizeof(int2) * (size_t)kMaxScenes, s->stream));
     HIPCHECK(hipStreamSynchronize(s->stream));
+    if (!s->skinSet) {
+        rc = set_skin(s, 0.f, s->skinSweep);
+        if (rc != YB_OK) return rc;
+    }
     // the tree and the captured graphs depend on the scene layout
     s->treeBuilt = false;
     s->graphItr = -1;
@@ -1050,6 +1060,13 @@ extern "C" int yb_set_contact_skin(yb_sim* s, float margin, float sweep) {
         set_message("yb_set_contact_skin: sweep must lie in [0, 1000], margin at most 1000");
         return YB_ERR_INVALID_ARGUMENT;
     }
+    s->skinSet = true;
+    return set_skin(s, margin, sweep);
+}
+
+// Applies a contact-skin setting; a new on/off state drops the kept lists
+// and the per-scene policy (and recaptures the graphs).
+static int set_skin(yb_sim* s, float margin, float sweep) {
     HIPCHECK(hipStreamSynchronize(s->stream));
     const bool was = s->skinMargin > 0.f, on = margin > 0.f;
     s->skinMargin = on ? margin : 0.f;

@@ -86,6 +86,8 @@ constexpr int kSkinCand = YB_SKIN_CANDIDATES;
 // about half of one) walks without keeping candidates for kSkinBackoff
 // substeps, then tries the skin again.
 constexpr int kSkinMinReuse = 2;
+constexpr float kSkinDefaultMargin = 0.05f;  // detection radii
+constexpr float kSkinDefaultSweep = 8.0f;    // substeps
 constexpr int kSkinBackoff = 16;
 // Growth by motion of a grown box, per axis, in detection radii at most.
 constexpr float kSkinCapRadii = 8.0f;
@@ -299,8 +301,9 @@ struct yb_sim {
     int queryStack = YB_QUERY_STACK_DEPTH;  // traversal stack entries a query may use
     // contact skin (yb_set_contact_skin): margin in detection radii (<= 0: off) and
     // the substeps of warm-start motion a grown box covers
-    float skinMargin = 0.05f;
-    float skinSweep = 8.0f;
+    float skinMargin = yb::kSkinDefaultMargin;
+    float skinSweep = yb::kSkinDefaultSweep;
+    bool skinSet = false;  // yb_set_contact_skin was called (else batches start without it)
     bool countCandidates = false;           // k_query writes b.candCount (measurement)
     // residual-targeted mode (tol > 0): prologue graphs per detection mode,
     // one graph per chunk of sweeps, one end graph
