@@ -404,7 +404,8 @@ int yb_get_tile_config(yb_sim* s, int* refitLeaves, int* queryStackLds, int* que
  * traversal-stack overflow, a tree rebuild, new scenes -- walks again.  A
  * scene whose lists are reused fewer than twice before it must walk again
  * walks the plain way (no grown boxes, nothing kept) for 16 substeps, then
- * tries the skin again.  margin <= 0 turns the skin off (the tree is refit
+ * tries the skin again (after each retry that fails too, for twice as long,
+ * up to 1024 substeps; lists that last reset it).  margin <= 0 turns the skin off (the tree is refit
  * and walked in every detecting substep).  Default: margin 0.05, sweep 8 for
  * a handle of one scene; a handle given several scenes (yb_set_scenes)
  * starts with the skin off unless this was called. */
@@ -416,9 +417,9 @@ int yb_get_contact_skin(yb_sim* s, float* margin, float* sweep);
 int yb_contact_skin_stats(yb_sim* s, int64_t out[2], int reset);
 /* Contact skin state (tests): each segment's kept candidate count (n ints,
  * -1: incomplete, the segment walks at its next detection), grown box
- * (8n floats, lo xyzw, hi xyzw) and each scene's policy (2 ints per scene:
+ * (8n floats, lo xyzw, hi xyzw) and each scene's policy (4 ints per scene:
  * substeps left walking without the skin, 0 or 1 + substeps its kept lists
- * have lasted); any pointer may be null. */
+ * have lasted, length of the last back-off, 0); any pointer may be null. */
 int yb_debug_read_skin(yb_sim* s, int32_t* candN, float* sbox, int32_t* policy);
 
 /* Launch mode.  on = 1 (default): each substep's kernels run as captured HIP

@@ -279,7 +279,7 @@ extern "C" int yb_create(int n, yb_sim** out) {
     if (e == hipSuccess) e = hipMemset(b.candN, 0xff, sizeof(int32_t) * N);
     if (e == hipSuccess) e = hipMemset(b.skinState, 0, sizeof(int32_t) * (1 + (size_t)kMaxScenes));
     if (e == hipSuccess) e = hipMemset(b.skinStats, 0, 2 * sizeof(unsigned long long));
-    if (e == hipSuccess) e = hipMemset(b.skinPolicy, 0, sizeof(int2) * (size_t)kMaxScenes);
+    if (e == hipSuccess) e = hipMemset(b.skinPolicy, 0, sizeof(int4) * (size_t)kMaxScenes);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
         set_message(std::string("yb_create: ") + hipGetErrorString(e));
@@ -522,7 +522,7 @@ extern "C" int yb_set_scenes(yb_sim* s, int numScenes, const int32_t* offsets) {
     // the skin unless the caller chose a setting (its scenes' motion varies,
     // and where most move too fast for kept lists the launches over the whole
     // batch cost more than the reuses save)
-    HIPCHECK(hipMemsetAsync(b.skinPolicy, 0, sizeof(int2) * (size_t)kMaxScenes, s->stream));
+    HIPCHECK(hipMemsetAsync(b.skinPolicy, 0, sizeof(int4) * (size_t)kMaxScenes, s->stream));
     HIPCHECK(hipStreamSynchronize(s->stream));
     if (!s->skinSet) {
         rc = set_skin(s, 0.f, s->skinSweep);
@@ -1075,7 +1075,7 @@ static int set_skin(yb_sim* s, float margin, float sweep) {
         // kept candidates from an earlier period may follow an older tree's
         // leaf order: every segment walks first
         HIPCHECK(hipMemsetAsync(s->b.candN, 0xff, sizeof(int32_t) * (size_t)s->n, s->stream));
-        HIPCHECK(hipMemsetAsync(s->b.skinPolicy, 0, sizeof(int2) * (size_t)kMaxScenes, s->stream));
+        HIPCHECK(hipMemsetAsync(s->b.skinPolicy, 0, sizeof(int4) * (size_t)kMaxScenes, s->stream));
         HIPCHECK(hipStreamSynchronize(s->stream));
         s->graphItr = -1;
         s->adaptiveReady = false;
@@ -1114,7 +1114,7 @@ extern "C" int yb_debug_read_skin(yb_sim* s, int32_t* candN, float* sbox, int32_
         HIPCHECK(hipMemcpyAsync(sbox, s->b.sbox, sizeof(float4) * 2 * N, hipMemcpyDeviceToHost,
                                 s->stream));
     if (policy)
-        HIPCHECK(hipMemcpyAsync(policy, s->b.skinPolicy, sizeof(int2) * (size_t)s->numScenes,
+        HIPCHECK(hipMemcpyAsync(policy, s->b.skinPolicy, sizeof(int4) * (size_t)s->numScenes,
                                 hipMemcpyDeviceToHost, s->stream));
     HIPCHECK(hipStreamSynchronize(s->stream));
     return YB_OK;

@@ -33,17 +33,18 @@ __global__ __launch_bounds__(256) void k_init(const DevParams* __restrict__ P,
                                               float4* __restrict__ yv,
                                               float4* __restrict__ dx0,
                                               int32_t* __restrict__ skinState,
-                                              int2* __restrict__ skinPolicy, int skinScenes) {
+                                              int4* __restrict__ skinPolicy, int skinScenes) {
     const int n = P->n;
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n) return;
     // Contact skin, one thread per scene: the scene's walk flag of the last
     // substep feeds its policy (lists reused fewer than kSkinMinReuse times:
-    // walk without the skin for kSkinBackoff substeps), then this substep's
-    // flags start clear (k_aabb sets them).
+    // walk without the skin for a back-off of kSkinBackoff substeps, doubled
+    // after each retry that fails too), then this substep's flags start
+    // clear (k_aabb sets them).
     if (v < skinScenes) {
         const int f = skinScenes > 1 ? 1 + v : 0;
-        int2 pol = skinPolicy[v];
+        int4 pol = skinPolicy[v];
         const int32_t walked = skinState[f];
         if (pol.x > 0) {
             pol.x--;
@@ -52,9 +53,11 @@ __global__ __launch_bounds__(256) void k_init(const DevParams* __restrict__ P,
             pol.y = 1;  // a new tree: the lists it made are fresh, not short-lived
         } else if (walked) {
             if (pol.y >= 1 && pol.y - 1 < kSkinMinReuse) {
-                pol.x = kSkinBackoff;
+                pol.z = pol.z > 0 ? min(2 * pol.z, kSkinBackoffMax) : kSkinBackoff;
+                pol.x = pol.z;
                 pol.y = 0;
             } else {
+                if (pol.y - 1 >= kSkinMinReuse) pol.z = 0;  // those lists lasted
                 pol.y = 1;
             }
         } else if (pol.y >= 1) {

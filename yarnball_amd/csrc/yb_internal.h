@@ -77,15 +77,19 @@ struct SkinArgs {
     int32_t* cand;                // kept candidates (kSkinCand per segment, skin_slot)
     int32_t* candN;               // kept candidates per segment (< 0: incomplete)
     unsigned long long* stats;    // [0] walking, [1] reusing detection substeps
-    int2* policy;                 // per scene: (substeps left walking without the skin,
-                                  //  0 or 1 + substeps its kept lists have lasted)
+    int4* policy;                 // per scene: (substeps left walking without the skin,
+                                  //  0 or 1 + substeps its kept lists have lasted,
+                                  //  length of the last back-off, unused)
 };
 constexpr int kSkinCand = YB_SKIN_CANDIDATES;
 // A scene whose kept lists are reused fewer than kSkinMinReuse times before
 // its next walk (a walk with grown boxes costs about two plain walks, a reuse
-// about half of one) walks without keeping candidates for kSkinBackoff
-// substeps, then tries the skin again.
+// about half of one) walks without keeping candidates for a while, then
+// tries the skin again: kSkinBackoff substeps, doubled after every retry
+// whose lists did not last either, up to kSkinBackoffMax; lists that last
+// reset it.
 constexpr int kSkinMinReuse = 2;
+constexpr int kSkinBackoffMax = 1024;
 constexpr float kSkinDefaultMargin = 0.05f;  // detection radii
 constexpr float kSkinDefaultSweep = 8.0f;    // substeps
 constexpr int kSkinBackoff = 16;
@@ -238,7 +242,7 @@ struct Buffers {
     int32_t* candN = nullptr;
     int32_t* skinState = nullptr;
     unsigned long long* skinStats = nullptr;
-    int2* skinPolicy = nullptr;
+    int4* skinPolicy = nullptr;
     float4* skinRoot = nullptr;
     int32_t* err = nullptr;       // [0] error bits, [1] warning code
     float* minSep = nullptr;      // smallest contact distance / (2 radius) seen
