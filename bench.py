@@ -13,7 +13,7 @@ each method ends at is reported beside it.  Four methods run the same frames
 from the same initial state: the plain warm start, the learned one (the
 committed model, with the substep-level plain fallback), and each of them
 followed by the strand solve (a tridiagonal Newton step along the strands
-after the warm start and before sweeps 1, 5, 9, ...).  value = vertices x
+after the warm start and before sweeps 2, 6, 10, ...).  value = vertices x
 substeps / second over the timed frames of the learned warm start (the
 faster of its two forms); every method's rate, sweeps per substep, residuals
 and ratio to the plain one are beside it.
@@ -593,7 +593,7 @@ def main():
     ap.add_argument("--strand-every", type=int, default=4,
                     help="strand methods: also run the strand step before every k-th sweep "
                          "(0: only after the warm start)")
-    ap.add_argument("--strand-first", type=int, default=1,
+    ap.add_argument("--strand-first", type=int, default=2,
                     help="strand methods: the sweep before which the first such step runs "
                          "(0: the strand-every-th)")
     ap.add_argument("--scenes", type=int, default=512, help="batched_walks: number of scenes")

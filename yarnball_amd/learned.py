@@ -137,7 +137,7 @@ def strand(sim, frame_refresh=True, every=0, pair_iters=2, first=0):
     diagonal, stretch coupling between neighbours): it removes the smooth
     error modes that block-Jacobi sweeps damp slowly on long stiff strands.
     No network.  every > 0 (residual mode): repeat the step before sweeps
-    first, first + every, ... (the 1M knit: every=4, first=1)."""
+    first, first + every, ... (the 1M knit: every=4, first=2)."""
     strand_schedule(sim, every, pair_iters, first)
     check(sim._lib.yb_set_warm_start(
         sim.handle, STRAND_SOLVE | (0 if frame_refresh else NO_FRAME_REFRESH)))
