@@ -9,7 +9,10 @@ opposite edge pulled at 5 cm/s, gravity -z), 1/30 s frames, maxH = 1e-3
 A step is one frame: Sim.advance(1/30) over the whole patch, state resident
 in HBM, every substep solved until the update residual (RMS change of the
 iterate per sweep in rest lengths) is <= 1e-6; the Newton residual |f|/|f0|
-each method ends at is reported beside it.  Four methods run the same frames
+each method ends at is reported beside it (by default recorded in the timed
+frames for the substeps that start from the plain warm start, whose prologue
+evaluates |f0| anyway, and for every substep of one more, untimed frame of
+the learned methods).  Four methods run the same frames
 from the same initial state: the plain warm start, the learned one (the
 committed model, with the substep-level plain fallback), and each of them
 followed by the strand solve (a tridiagonal Newton step along the strands
@@ -507,6 +510,8 @@ def run_method(sc, args, method, model, strand_model=None):
         sim.setTileConfig(*args.tile_config)
     if args.skin:
         sim.setContactSkin(*args.skin)
+    if args.residual > 0:
+        sim.setNewtonResidual(args.newton_residual)
     frame = np.float32(sc.frame_dt)
     substeps_per_frame = int(np.ceil(frame / np.float32(sc.maxH)))
     for w in range(args.warmup):
@@ -554,6 +559,21 @@ def run_method(sc, args, method, model, strand_model=None):
     sim.contactSkinStats(reset=True)
     r["profile"] = live_profile(sim, sc, args, substeps_per_frame)
     r["profile_skin"] = sim.contactSkinStats(reset=True)
+    if args.residual > 0 and method.startswith("learned") and args.newton_residual == "plain":
+        # the timed frames recorded the Newton residual of the plain-start
+        # substeps only; one more frame (untimed, after every timing) records
+        # it for every start
+        sim.setNewtonResidual("all")
+        sim.solverStats(reset=True)
+        sim.advance(frame)
+        sa = sim.solverStats()
+        r["newton_all_starts"] = {"substeps": int(sa["newtonSubsteps"]),
+                                  "mean": float(sa["meanNewtonResidual"]),
+                                  "max": float(sa["maxNewtonResidual"])}
+        sim.setNewtonResidual(args.newton_residual)
+        print(f"[bench] {method}: one more frame with the Newton residual of every start: mean "
+              f"{sa['meanNewtonResidual']:.2e} max {sa['maxNewtonResidual']:.2e} over "
+              f"{sa['newtonSubsteps']} substeps", file=sys.stderr, flush=True)
     print(f"[bench] {method}: timed {args.steps} frames in {elapsed:.3f}s "
           f"({r['value']:.4e} vertex-substeps/s), min separation {r['min_separation']:.3f} x 2r",
           file=sys.stderr, flush=True)
@@ -576,6 +596,11 @@ def main():
     ap.add_argument("--max-sweeps", type=int, default=6000)
     ap.add_argument("--chunk", type=int, default=16,
                     help="sweeps per chunk graph in residual mode")
+    ap.add_argument("--newton-residual", choices=["all", "plain"], default="plain",
+                    help="residual mode: which substeps record the Newton residual (a "
+                         "diagnostic; the stopping test is the update residual): every one, or "
+                         "those starting from the plain warm start, whose prologue evaluates "
+                         "its denominator anyway")
     ap.add_argument("--check-every", type=int, default=4,
                     help="residual mode: sweeps between residual evaluations")
     ap.add_argument("--method", choices=["plain", "learned", "both", "strand", "learned+strand",
@@ -744,8 +769,12 @@ def main():
             row["sweeps_per_substep"] = round(r["sweeps_per_substep"], 1)
             row["not_converged_substeps"] = int(r["stats"]["notConverged"])
             row["max_final_residual"] = float(r["stats"]["maxResidual"])
-            row["newton_residual_mean"] = float(r["stats"]["meanNewtonResidual"])
+            nr = r["stats"]["meanNewtonResidual"]
+            row["newton_residual_mean"] = float(nr) if np.isfinite(nr) else None
             row["newton_residual_max"] = float(r["stats"]["maxNewtonResidual"])
+            row["newton_residual_substeps"] = int(r["stats"]["newtonSubsteps"])
+            if "newton_all_starts" in r:
+                row["newton_residual_all_starts_frame"] = r["newton_all_starts"]
             row["substep_timeline"] = r.get("timeline")
         if m.startswith("learned"):
             ws = r["warm_substeps"]

@@ -337,11 +337,27 @@ typedef struct yb_solver_stats_t {
     int64_t sceneSubsteps;     /* batched scenes: substeps x scenes (else = substeps) */
     int64_t sceneSweeps;       /* batched scenes: each scene's own sweeps, summed
                                   (else = sweeps) */
+    int64_t newtonSubsteps;    /* substeps (scene-substeps) that recorded r_F: the
+                                  count behind sumNewtonResidual and maxNewtonResidual */
 } yb_solver_stats_t;
 int yb_set_solver(yb_sim* s, float residualTol, int maxSweeps, int chunk, int checkEvery);
 /* Stopping test of the residual-targeted mode: YB_RESIDUAL_UPDATE (default) or
  * YB_RESIDUAL_NEWTON. */
 int yb_set_residual_kind(yb_sim* s, int kind);
+/* Which substeps record the Newton residual r_F while the stopping test is the
+ * update residual (no counterpart in the reference, which measures no
+ * residual).  r_F's denominator is the force at the plain warm start, which a
+ * substep starting from the plain warm start evaluates in a pass it makes
+ * anyway (its first check, or the pass that also refreshes the frames and
+ * makes the first strand step).  A substep starting from the learned or
+ * temporal warm start needs one more force pass over the plain start for it
+ * (about one sweep).  YB_NEWTON_ALL_STARTS (default): every substep records
+ * r_F.  YB_NEWTON_PLAIN_STARTS: only substeps that start from the plain warm
+ * start; the others skip the extra pass.  The iterates are the same either
+ * way.  Batches and the YB_RESIDUAL_NEWTON stopping test always record r_F. */
+#define YB_NEWTON_ALL_STARTS 1
+#define YB_NEWTON_PLAIN_STARTS 0
+int yb_set_newton_residual(yb_sim* s, int mode);
 int yb_solver_stats(yb_sim* s, yb_solver_stats_t* out, int reset);
 
 /* ---- batched scenes ---------------------------------------------------------

@@ -39,6 +39,7 @@ SIGS = {
     "orc_set_contacts": (None, [_P, _P, _P]),
     "orc_set_iterate": (None, [_P, _P, _P, _P]),
     "orc_seg_terms": (None, [_P, _PP, _I, _P, _P]),
+    "orc_frame_update": (None, [_P, _P, _P]),
     "orc_vertex_terms": (None, [_P, _PP, _I, _P, _P]),
     "orc_force_norm2": (ctypes.c_double, [_P, _PP, _P]),
     "orc32_warm_start": (None, [_I, _PP, _P, _P, _P, _P, _P]),
@@ -183,6 +184,13 @@ class Oracle:
             return None if a is None else ptr(_f64(a, shape))
         self.lib.orc_set_iterate(self.o, f(dx, (self.n, 3)), f(y, (self.n, 3)),
                                  f(step_limit, (self.n,)))
+
+    def frame_update(self, dx):
+        """One sweep's frame update (cosserat.cu:209-253) of the loaded frames
+        at iterate dx (n x 3), without changing the state: (n x 4)."""
+        q = np.zeros((self.n, 4))
+        self.lib.orc_frame_update(self.o, ptr(_f64(dx, (self.n, 3))), ptr(q))
+        return q
 
     def seg_terms(self, params, s, dx):
         out = np.zeros(18)

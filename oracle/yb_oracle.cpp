@@ -506,6 +506,41 @@ inline void s4(std::vector<real>& a, int i, DR v) {
     a[4 * i + 3] = v.w;
 }
 
+// quaternionLambdaItr (cosserat.cu:209-253): every frame's closed-form update
+// from the frames o->q and the iterate dx (3n), into qOut (4n).
+static void frame_update(const orc_sim* o, const std::vector<real>& dx, std::vector<real>& qOut) {
+    const int n = o->n;
+#pragma omp parallel for schedule(static)
+    for (int v = 0; v < n; v++) {
+        DR q0 = g4(o->q, v);
+        const uint32_t fl = o->flags[v];
+        if (!has(fl, YB_FLAG_FIX_ORIENTATION) && has(fl, YB_FLAG_HAS_NEXT)) {
+            const D3 e = add(sub(g3(o->pos, v + 1), g3(o->pos, v)),
+                             sub(g3(dx, v + 1), g3(dx, v)));
+            D3 vv = {e.x / o->lRest[v], e.y / o->lRest[v], e.z / o->lRest[v]};
+            vv = mulr(vv, real(-2.0) * o->kStretch[v]);
+            DR b = {0, 0, 0, 0};
+            if (has(fl, YB_FLAG_HAS_PREV)) {
+                const DR qr = g4(o->qRest, v - 1);
+                const DR qq = g4(o->q, v - 1);
+                const real sg = rdot(rmul(rinv(qq), q0), qr) > 0 ? 1.0 : -1.0;
+                b = radd(b, rscale(rmul(qq, qr), sg));
+            }
+            if (has(fl, YB_FLAG_HAS_NEXT_ORIENTATION)) {
+                const DR qr = g4(o->qRest, v);
+                const DR qq = g4(o->q, v + 1);
+                const real sg = rdot(rmul(rinv(q0), qq), qr) > 0 ? 1.0 : -1.0;
+                b = radd(b, rscale(rmul(qq, rinv(qr)), sg));
+            }
+            const real lambda = len(vv) + rlen(b);
+            const DR t = rmul(rmul(mkr(vv, real(0.0)), b), DR{1, 0, 0, 0});
+            const DR r = radd(t, rscale(b, lambda));
+            q0 = rscale(r, real(1.0) / rlen(r));
+        }
+        s4(qOut, v, q0);
+    }
+}
+
 // Contact detection on the FP32 image of the current state.
 int detect(orc_sim* o, const yb_params* p, const std::vector<float>& dxw32) {
     const int n = o->n;
@@ -785,35 +820,7 @@ extern "C" int orc_substep(orc_sim* o, const yb_params* p, int doDetect) {
         std::swap(o->dx, o->dxNext);
 
         // quaternionLambdaItr with the new dx and the previous frames.
-#pragma omp parallel for schedule(static)
-        for (int v = 0; v < n; v++) {
-            DR q0 = g4(o->q, v);
-            const uint32_t fl = o->flags[v];
-            if (!has(fl, YB_FLAG_FIX_ORIENTATION) && has(fl, YB_FLAG_HAS_NEXT)) {
-                const D3 e = add(sub(g3(o->pos, v + 1), g3(o->pos, v)),
-                                 sub(g3(o->dx, v + 1), g3(o->dx, v)));
-                D3 vv = {e.x / o->lRest[v], e.y / o->lRest[v], e.z / o->lRest[v]};
-                vv = mulr(vv, real(-2.0) * o->kStretch[v]);
-                DR b = {0, 0, 0, 0};
-                if (has(fl, YB_FLAG_HAS_PREV)) {
-                    const DR qr = g4(o->qRest, v - 1);
-                    const DR qq = g4(o->q, v - 1);
-                    const real sg = rdot(rmul(rinv(qq), q0), qr) > 0 ? 1.0 : -1.0;
-                    b = radd(b, rscale(rmul(qq, qr), sg));
-                }
-                if (has(fl, YB_FLAG_HAS_NEXT_ORIENTATION)) {
-                    const DR qr = g4(o->qRest, v);
-                    const DR qq = g4(o->q, v + 1);
-                    const real sg = rdot(rmul(rinv(q0), qq), qr) > 0 ? 1.0 : -1.0;
-                    b = radd(b, rscale(rmul(qq, rinv(qr)), sg));
-                }
-                const real lambda = len(vv) + rlen(b);
-                const DR t = rmul(rmul(mkr(vv, real(0.0)), b), DR{1, 0, 0, 0});
-                const DR r = radd(t, rscale(b, lambda));
-                q0 = rscale(r, real(1.0) / rlen(r));
-            }
-            s4(o->qNext, v, q0);
-        }
+        frame_update(o, o->dx, o->qNext);
         std::swap(o->q, o->qNext);
         o->lastForceResidual = o->forceNorm0 > 0 ? std::sqrt(numF / o->forceNorm0) : 0.0;
         o->lastUpdateResidual = denU > 0 ? std::sqrt(numU / denU) : 0.0;
@@ -890,6 +897,13 @@ extern "C" void orc_set_iterate(orc_sim* o, const double* dx, const double* y,
     if (dx) o->dx = vec_of(dx, 3 * N);
     if (y) o->y = vec_of(y, 3 * N);
     if (stepLimit) o->stepLimit = vec_of(stepLimit, N);
+}
+
+extern "C" void orc_frame_update(const orc_sim* o, const double* dx, double* qOut) {
+    const std::vector<real> d = vec_of(dx, 3 * (size_t)o->n);
+    std::vector<real> q(4 * (size_t)o->n);
+    frame_update(o, d, q);
+    for (size_t i = 0; i < q.size(); i++) qOut[i] = (double)q[i];
 }
 
 // out: f0[3], f1[3], H0[6], H1[6] of segment s at iterate dx (3n).

@@ -85,6 +85,10 @@ double orc_last_force_norm0(const orc_sim* o);
 void orc_set_contacts(orc_sim* o, const int32_t* numCols, const int32_t* cols);
 void orc_set_iterate(orc_sim* o, const double* dx, const double* y, const double* stepLimit);
 void orc_seg_terms(const orc_sim* o, const yb_params* p, int s, const double* dx, double* out);
+/* orc_frame_update: the frame update of one sweep (quaternionLambdaItr,
+ * cosserat.cu:209-253) from the loaded frames and positions and the iterate
+ * dx (3n), into qOut (4n); the state is not changed. */
+void orc_frame_update(const orc_sim* o, const double* dx, double* qOut);
 void orc_vertex_terms(const orc_sim* o, const yb_params* p, int v, const double* dx,
                       double* out);
 double orc_force_norm2(const orc_sim* o, const yb_params* p, const double* dx);

@@ -136,6 +136,49 @@ def test_frame_update_is_sign_flip_robust():
     assert same.max() <= 1e-12
 
 
+@pytest.mark.parametrize("sweeps", [1, 3])
+def test_frame_update_export_is_the_sweeps_frame_update(sweeps):
+    """orc_frame_update, on the state before a substep and at the iterate that
+    substep ends with (its position change), gives the frames the substep
+    leaves: the frame update of its last sweep, whose dx is that iterate.  With
+    one sweep the frames before it are the substep's own; with three, the
+    export is applied to the frames after the second sweep, read from a
+    two-sweep run of the same state."""
+    sc = scenes.readme_scene()
+    sc.vels[:, 2] = np.float32(0.05)  # some motion besides gravity
+    p = sc.params.copy()
+    p.h = np.float32(sc.maxH)
+    p.detectionPeriod = -1
+    start = Oracle(sc)
+    pos0, vel0, lv0, q0 = start.state()
+    qs_before = q0
+    if sweeps > 1:
+        pre = Oracle(sc)
+        pp = p.copy()
+        pp.numItr = sweeps - 1
+        pre.substep(pp, False)
+        # the frames after sweep (sweeps - 1) of this substep
+        qs_before = pre.state()[3]
+    p.numItr = sweeps
+    run = Oracle(sc)
+    run.substep(p, False)
+    pos1, _, _, q1 = run.state()
+    dx = pos1 - pos0
+    probe = Oracle(sc)
+    probe.load64(pos=pos0, vels=vel0, lastVels=lv0, qs=qs_before)
+    got = probe.frame_update(dx)
+    assert np.abs(got - q1).max() <= 1e-10
+    assert np.abs(q1 - q0).max() > 1e-6  # the frames did turn
+    # the rest configuration: a straight rod with rest frames stays put
+    rod = np.zeros((10, 3), np.float32)
+    rod[:, 0] = np.arange(10) * 0.002
+    rs = scenes._configure("rest", rod, breaks_after=[], pinned=[0], kCollision=1e-7,
+                           kBend=1e-8, kStretch=2e-2, maxH=1e-3, gravity=(0.0, 0.0, 0.0))
+    ro = Oracle(rs)
+    qr = ro.state()[3]
+    assert np.abs(ro.frame_update(np.zeros((10, 3))) - qr).max() <= 1e-12
+
+
 @pytest.mark.parametrize("ratio,s", [(1.0, 1.0), (0.5, 0.5), (-1.0, 0.0), (2.0, 1.0), (0.0, 0.0)])
 def test_warm_start_acceleration_clamp(ratio, s):
     """The plain warm start (iteration.cu:7-41) in closed form:

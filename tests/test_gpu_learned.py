@@ -4,13 +4,17 @@ applies equals the standalone forward kernel on the same features (after the
 another at the same residual, and the substep-level fallback bounds the cost
 of a useless model."""
 import ctypes
+import os
 import time
 
 import numpy as np
 import pytest
 
+from oracle import Oracle
 from yarnball_amd import Sim, learned, scenes, train_warmstart
 from yarnball_amd._lib import check, ptr
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 FLAG_HAS_PREV, FLAG_HAS_NEXT = 1, 2
 
@@ -186,3 +190,92 @@ def test_fallback_bounds_cost_of_useless_model():
     assert net["sweeps"] > 1.2 * plain["sweeps"], "the untrained model should hurt"
     assert auto["sweeps"] < 1.25 * plain["sweeps"]
     assert auto["warm"]["plain"] > auto["warm"]["learned"]
+
+
+@pytest.mark.gpu
+def test_learned_apply_refreshes_the_frames():
+    """The learned start's apply also refreshes the frames, in the same pass:
+    after a prologue the frames equal one frame update (cosserat.cu:209-253,
+    the FP64 oracle's) of the previous frames at the learned warm start; with
+    the refresh off they are the previous frames, bit for bit."""
+    sc = scenes.knit_patch(rows=8, per=256)
+    a = Sim(sc.n)
+    scenes.load_into(a, sc)
+    a.advance(np.float32(5 * sc.maxH))
+    a.download()
+    lv = np.zeros((sc.n, 3), np.float32)
+    check(a._lib.yb_debug_read_last_vels(a.handle, ptr(lv)))
+    state = sc.copy()
+    state.verts[:] = a.verts
+    state.vels[:] = a.vels
+    state.qs[:] = a.qs
+    state.params = a.meta.copy()
+    a.close()
+    model = np.load(os.path.join(ROOT, "yarnball_amd", "models", "warmstart.npy"))
+    out = {}
+    for refresh in (True, False):
+        s = Sim(sc.n)
+        scenes.load_into(s, state)
+        check(s._lib.yb_debug_write_last_vels(s.handle, ptr(lv)))
+        learned.record(s, sc.n)  # (keeps the plain start readable)
+        learned.use(s, model, frame_refresh=refresh)
+        dx, plain, _ = prologue_dx(s)
+        s.download()
+        out[refresh] = (dx, plain, s.qs.astype(np.float64).copy())
+        s.close()
+    dx, plain, q = out[True]
+    assert np.array_equal(dx, out[False][0]), "the refresh must not change the warm start"
+    assert np.array_equal(out[False][2], state.qs.astype(np.float64)), "frames left alone"
+    assert not np.array_equal(dx, plain), "the learned start moved the iterate"
+    o = Oracle(n=sc.n)
+    o.load_state(state.verts, state.vels, state.qs, lv, state.qRests)
+    want = o.frame_update(dx[:, :3])
+    err = np.abs(q - want).max()
+    moved = np.abs(want - state.qs).max()
+    print(f"refreshed frames vs the oracle's frame update: max err {err:.2e} "
+          f"(frames moved by up to {moved:.2e})")
+    assert err <= 1e-5 and moved > 10 * err
+    o.close()
+
+
+def _headline_run(sc, frames, newton):
+    sim = Sim(sc.n)
+    scenes.load_into(sim, sc)
+    sim.setSolver(1e-6, maxSweeps=6000, chunk=16, checkEvery=4)
+    model = np.load(os.path.join(ROOT, "yarnball_amd", "models", "warmstart_strand.npy"))
+    learned.use(sim, model, strand=True)
+    learned.strand_schedule(sim, every=4, first=2)
+    sim.setNewtonResidual(newton)
+    sim.solverStats(reset=True)
+    learned.warm_substeps(sim, reset=True)
+    sim.advance(np.float32(frames * sc.frame_dt))
+    sim.download()
+    out = {"stats": sim.solverStats(), "warm": learned.warm_substeps(sim),
+           "pos": sim.verts["pos"].copy(), "qs": sim.qs.copy(), "vels": sim.vels.copy()}
+    sim.close()
+    return out
+
+
+@pytest.mark.gpu
+def test_newton_residual_at_plain_starts_only():
+    """Recording the Newton residual only in substeps that start from the plain
+    warm start skips the learned substeps' extra force pass and changes no
+    iterate: the learned start + strand solve ends in the same state, bit for
+    bit, with the same sweeps, and records the residual for exactly the
+    plain-start substeps."""
+    sc = scenes.knit_patch(16, 512, seed=3)
+    a = _headline_run(sc, 2, "all")
+    b = _headline_run(sc, 2, "plain")
+    sa, sb = a["stats"], b["stats"]
+    print(f"substeps {sa['substeps']}, learned starts {b['warm']['learned']}, plain starts "
+          f"{b['warm']['plain']}; Newton residual recorded in {sa['newtonSubsteps']} / "
+          f"{sb['newtonSubsteps']} substeps, mean {sa['meanNewtonResidual']:.3e} / "
+          f"{sb['meanNewtonResidual']:.3e}")
+    for k in ("pos", "qs", "vels"):
+        assert np.array_equal(a[k], b[k]), k
+    assert sa["sweeps"] == sb["sweeps"] and a["warm"] == b["warm"]
+    assert b["warm"]["learned"] > 0 and b["warm"]["plain"] > 0
+    assert sa["newtonSubsteps"] == sa["substeps"]
+    assert sb["newtonSubsteps"] == b["warm"]["plain"]
+    assert sb["maxNewtonResidual"] <= sa["maxNewtonResidual"]
+    assert 0.0 < sb["sumNewtonResidual"] <= sa["sumNewtonResidual"]

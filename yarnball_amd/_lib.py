@@ -58,6 +58,7 @@ class SolverStats(ctypes.Structure):
         ("sumNewtonResidual", ctypes.c_double),
         ("sceneSubsteps", ctypes.c_int64),
         ("sceneSweeps", ctypes.c_int64),
+        ("newtonSubsteps", ctypes.c_int64),
     ]
 
 
@@ -141,6 +142,7 @@ SIGNATURES = {
     "yb_profile": (_I, [_P, ctypes.POINTER(Params), _F, _I, _P, _P]),
     "yb_set_solver": (_I, [_P, _F, _I, _I, _I]),
     "yb_set_residual_kind": (_I, [_P, _I]),
+    "yb_set_newton_residual": (_I, [_P, _I]),
     "yb_debug_set_query_stack": (_I, [_P, _I]),
     "yb_debug_query_counts": (_I, [_P, ctypes.POINTER(Params), _F, _P]),
     "yb_debug_timeline": (_I, [_P, _I]),

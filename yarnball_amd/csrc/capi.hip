@@ -696,13 +696,20 @@ static void enqueue_prologue(yb_sim* s, int mode, bool adaptive, int warm) {
     // In a strand session the first strand step joins that pass too, unless
     // features are recorded first; the plain start a learned session keeps
     // for its correction is then copied by the same pass.
+    // With the Newton residual asked for at plain starts only (and the update
+    // residual as the stopping test), a substep of a single handle that starts
+    // elsewhere skips that pass and records no Newton residual.
     const bool refresh = (warm != 0 || s->strandSolve) && s->frameRefresh;
-    const bool evalDenom = adaptive && (s->warmStart != 0 || s->strandSolve);
+    const bool noNewton = warm != 0 && s->newtonMode == YB_NEWTON_PLAIN_STARTS &&
+                          s->residualKind == YB_RESIDUAL_UPDATE && s->numScenes <= 1;
+    const bool evalDenom = adaptive && (s->warmStart != 0 || s->strandSolve) && !noNewton;
     const bool fused = evalDenom && refresh && warm == 0;
     const bool learned = warm == 1, record = s->recCap > 0;
     const bool lineStart = fused && s->strandSolve && !record;
+    // a learned start's apply refreshes the frames in the same pass
+    const bool applyFrames = learned && refresh && !fused;
     if (adaptive) {
-        launch_ctl_reset(s);
+        launch_ctl_reset(s, noNewton);
         if (lineStart)
             launch_line_start(s, s->warmStart == 1);
         else if (evalDenom)
@@ -713,7 +720,7 @@ static void enqueue_prologue(yb_sim* s, int mode, bool adaptive, int warm) {
         // the inference kernel computes its own features; the recording
         // kernel runs for training data and for the plain warm start alone
         if (record || !learned) launch_features(s, record);
-        if (learned) launch_mlp_apply(s);
+        if (learned) launch_mlp_apply(s, applyFrames);
         mark(s, YB_STAGE_MLP);
     } else if (warm == 2) {
         launch_temporal(s);
@@ -724,7 +731,7 @@ static void enqueue_prologue(yb_sim* s, int mode, bool adaptive, int warm) {
     // the previous substep would hold every segment to its old direction and
     // undo the rotation the warm start predicts (the sweeps restore it only
     // slowly where bending is soft).
-    if (refresh && !fused) {
+    if (refresh && !fused && !applyFrames) {
         launch_frame_refresh(s);
         mark(s, YB_STAGE_MLP);
     }
@@ -980,6 +987,18 @@ extern "C" int yb_set_residual_kind(yb_sim* s, int kind) {
     const int32_t k = kind;
     HIPCHECK(hipMemcpyAsync(&s->b.ctl->kind, &k, sizeof(k), hipMemcpyHostToDevice, s->stream));
     HIPCHECK(hipStreamSynchronize(s->stream));
+    s->adaptiveReady = false;  // the prologue graphs depend on it (yb_set_newton_residual)
+    return YB_OK;
+}
+
+extern "C" int yb_set_newton_residual(yb_sim* s, int mode) {
+    if (!s || (mode != YB_NEWTON_ALL_STARTS && mode != YB_NEWTON_PLAIN_STARTS)) {
+        set_message("yb_set_newton_residual: YB_NEWTON_ALL_STARTS or YB_NEWTON_PLAIN_STARTS");
+        return YB_ERR_INVALID_ARGUMENT;
+    }
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    s->newtonMode = mode;
+    s->adaptiveReady = false;
     return YB_OK;
 }
 
@@ -1143,6 +1162,7 @@ extern "C" int yb_solver_stats(yb_sim* s, yb_solver_stats_t* out, int reset) {
     out->lastNewtonResidual = c.fres;
     out->maxNewtonResidual = c.maxFres;
     out->sumNewtonResidual = c.sumFres;
+    out->newtonSubsteps = (int64_t)c.newtonSubsteps;
     out->sceneSubsteps = out->substeps;
     out->sceneSweeps = out->sweeps;
     if (s->numScenes > 1 && s->b.sceneCtl) {
@@ -1160,6 +1180,7 @@ extern "C" int yb_solver_stats(yb_sim* s, yb_solver_stats_t* out, int reset) {
         out->sceneSubsteps = out->substeps * s->numScenes;
         out->sceneSweeps = sw;
         out->sumNewtonResidual = fr;
+        out->newtonSubsteps = out->sceneSubsteps;
     }
     if (reset) {
         SolverCtl z;

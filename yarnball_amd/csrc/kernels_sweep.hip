@@ -293,10 +293,13 @@ __device__ __forceinline__ double2 reduce_partials(const double2* __restrict__ p
 }
 
 // Stopping decision of check sweep `sweep` from its sums a = (sum |dx' - dx|^2,
-// sum |r|^2) and the Newton-residual denominator denF (one thread).
+// sum |r|^2) and the Newton-residual denominator denF (one thread).  denF ==
+// kNoNewton: the substep records no Newton residual (its stopping test is then
+// the update residual).
 __device__ __forceinline__ void decide(SolverCtl* __restrict__ ctl, double2 a, double denF,
                                        int sweep, float tol, int maxSweeps) {
     ctl->denomF = denF;
+    const bool hasF = denF != kNoNewton;
     const double denU = ctl->denom;
     const float rU = denU > 0.0 ? (float)sqrt(a.x / denU) : 0.0f;
     const float rF = denF > 0.0 ? (float)sqrt(a.y / denF) : 0.0f;
@@ -311,8 +314,11 @@ __device__ __forceinline__ void decide(SolverCtl* __restrict__ ctl, double2 a, d
         if (sweep + 1 > ctl->maxUsed) ctl->maxUsed = sweep + 1;
         if (r > tol) ctl->notConverged += 1;
         if (r > ctl->maxResid) ctl->maxResid = r;
-        if (rF > ctl->maxFres) ctl->maxFres = rF;
-        ctl->sumFres += (double)rF;
+        if (hasF) {
+            if (rF > ctl->maxFres) ctl->maxFres = rF;
+            ctl->sumFres += (double)rF;
+            ctl->newtonSubsteps += 1ull;
+        }
     }
 }
 
@@ -331,7 +337,7 @@ __device__ __forceinline__ void residual_wave(SolverCtl* __restrict__ ctl, Check
                                               int sweep) {
     const double2 a = reduce_partials(cp.partial, nb);
     double denF = ctl->denomF;  // uniform across the wave
-    if (denF < 0.0) denF = cp.denomPart ? reduce_partials(cp.denomPart, nb).y : a.y;
+    if (denF == -1.0) denF = cp.denomPart ? reduce_partials(cp.denomPart, nb).y : a.y;
     if ((threadIdx.x & 63) == 0) decide(ctl, a, denF, sweep, cp.tol, cp.maxSweeps);
 }
 
@@ -1107,7 +1113,7 @@ __global__ __launch_bounds__(kResThreads) void k_residual(SolverCtl* __restrict_
     // sum of denomPart, the partials of the prologue's pass over the plain warm
     // start, or -- in a plain session, whose first check is sweep 0 and whose
     // sweep 0 input is the plain warm start itself -- this check's own sum.
-    const bool needDen = ctl->denomF < 0.0 && denomPart;
+    const bool needDen = ctl->denomF == -1.0 && denomPart;
     const int b = blockIdx.x, R = gridDim.x;
     const int lo = (int)((long long)nb * b / R), hi = (int)((long long)nb * (b + 1) / R);
     const double2 sum = block_sum(partial, lo, hi);
@@ -1142,17 +1148,18 @@ __global__ __launch_bounds__(kResThreads) void k_residual(SolverCtl* __restrict_
         d += all[k].z;
     }
     *counter = 0u;  // for the next check (a later launch)
-    const double denF = ctl->denomF >= 0.0 ? ctl->denomF : (needDen ? d : a.y);
+    const double denF = ctl->denomF != -1.0 ? ctl->denomF : (needDen ? d : a.y);
     decide(ctl, a, denF, sweep, tol, maxSweeps);
 }
 
+// noNewton: this substep records no Newton residual (single handles only).
 __global__ void k_ctl_reset(SolverCtl* __restrict__ ctl, SceneCtl* __restrict__ sceneCtl,
                             int numScenes, int32_t* __restrict__ tileCount,
-                            unsigned int* __restrict__ resCounter) {
+                            unsigned int* __restrict__ resCounter, int noNewton) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k == 0) {
         ctl->doneAt = 0x7fffffff;
-        ctl->denomF = -1.0;
+        ctl->denomF = noNewton ? kNoNewton : -1.0;
         ctl->scenesDone = 0;
         if (tileCount) tileCount[0] = -1;  // every tile until the first check
         if (resCounter) *resCounter = 0u;
@@ -1273,6 +1280,34 @@ __global__ __launch_bounds__(kSceneThreads) void k_residual_scenes(SolverCtl* __
 // Frame refresh after a non-plain warm start: one closed-form frame update
 // (cosserat.cu:209-253) from the warm-start dx, q[0] -> q[1], then copied
 // back so that the first sweep sees frames consistent with its dx.
+// The refreshed frame of segment v (flags and rest data r4 = prm[v]) whose
+// edge in the configuration x + dx is e, from its frame q0, the neighbouring
+// frames qPrev, qNext and the rest rotors qrPrev = qRest[v - 1], qr = qRest[v]
+// (those of absent neighbours are not read).  The callers load every input
+// first, from clamped indices, so that all loads are in flight at once.
+__device__ __forceinline__ float4 refreshed_frame(float4 r4, uint32_t flags, v3 e, float4 q0f,
+                                                  float4 qPrev, float4 qNext, float4 qrPrev,
+                                                  float4 qrf) {
+    const rot q0 = asrot(q0f);
+    v3 vv = v3{e.x / r4.x, e.y / r4.x, e.z / r4.x};
+    vv *= -2.0f * r4.y;
+    rot b = rot{0.f, 0.f, 0.f, 0.f};
+    if (flags & YB_FLAG_HAS_PREV) {
+        const rot qr = asrot(qrPrev);
+        const rot qq = asrot(qPrev);
+        const float sg = rdot(rmul(rinv(qq), q0), qr) > 0.0f ? 1.0f : -1.0f;
+        b = rscale(rmul(qq, qr), sg);
+    }
+    if (flags & YB_FLAG_HAS_NEXT_ORIENTATION) {
+        const rot qr = asrot(qrf);
+        const rot qq = asrot(qNext);
+        const float sg = rdot(rmul(rinv(q0), qq), qr) > 0.0f ? 1.0f : -1.0f;
+        b = radd(b, rscale(rmul(qq, rinv(qr)), sg));
+    }
+    const float lambda = len(vv) + rlen(b);
+    return f4r(rnormalize(radd(vb_e1(vv, b), rscale(b, lambda))));
+}
+
 __global__ __launch_bounds__(256) void k_frames(const DevParams* __restrict__ P,
                                                 const float4* __restrict__ pos,
                                                 const float4* __restrict__ prm,
@@ -1283,30 +1318,120 @@ __global__ __launch_bounds__(256) void k_frames(const DevParams* __restrict__ P,
     const int n = P->n;
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n) return;
-    const float4 r4 = prm[v];
+    const int vm = v > 0 ? v - 1 : v, vp = v + 1 < n ? v + 1 : v;
+    const float4 r4 = prm[v], q0 = qIn[v], qm = qIn[vm], qp = qIn[vp];
+    const float4 qrm = qRest[vm], qr = qRest[v];
+    const float4 x0 = pos[v], x1 = pos[vp], d0 = dx[v], d1 = dx[vp];
     const uint32_t flags = __float_as_uint(r4.w);
-    rot q0 = asrot(qIn[v]);
     if (!(flags & YB_FLAG_FIX_ORIENTATION) && (flags & YB_FLAG_HAS_NEXT)) {
-        const v3 e = (xyz(pos[v + 1]) - xyz(pos[v])) + (xyz(dx[v + 1]) - xyz(dx[v]));
-        v3 vv = v3{e.x / r4.x, e.y / r4.x, e.z / r4.x};
-        vv *= -2.0f * r4.y;
-        rot b = rot{0.f, 0.f, 0.f, 0.f};
-        if (flags & YB_FLAG_HAS_PREV) {
-            const rot qr = asrot(qRest[v - 1]);
-            const rot qq = asrot(qIn[v - 1]);
-            const float sg = rdot(rmul(rinv(qq), q0), qr) > 0.0f ? 1.0f : -1.0f;
-            b = rscale(rmul(qq, qr), sg);
-        }
-        if (flags & YB_FLAG_HAS_NEXT_ORIENTATION) {
-            const rot qr = asrot(qRest[v]);
-            const rot qq = asrot(qIn[v + 1]);
-            const float sg = rdot(rmul(rinv(q0), qq), qr) > 0.0f ? 1.0f : -1.0f;
-            b = radd(b, rscale(rmul(qq, rinv(qr)), sg));
-        }
-        const float lambda = len(vv) + rlen(b);
-        q0 = rnormalize(radd(vb_e1(vv, b), rscale(b, lambda)));
+        const v3 e = (xyz(x1) - xyz(x0)) + (xyz(d1) - xyz(d0));
+        qOut[v] = refreshed_frame(r4, flags, e, q0, qm, qp, qrm, qr);
+    } else {
+        qOut[v] = q0;
     }
-    qOut[v] = f4r(q0);
+}
+
+// ---------------------------------------------------------------------------
+// The learned warm start's apply (no counterpart in the reference, whose warm
+// start is iteration.cu:18-34): the predicted correction c of vertex v,
+// low-passed along the strand with the binomial [1 2 1]/4 filter when passes
+// > 0 (a missing neighbour counts as the vertex itself), in units of the rest
+// length L, on top of the plain warm start: dx0 + L c, clamped to the step
+// limit like a sweep step.  Block-Jacobi damps the alternating (odd-even) mode
+// along a strand as slowly as the smoothest one, and a per-vertex regression
+// error is close to white noise; the filter removes that mode from the
+// prediction before the sweeps see it.  A pinned vertex keeps dx0 (kind 0),
+// and so does one whose correction is not finite (kind 2, a fallback); kind 1:
+// the correction was applied.
+struct LearnedDx {
+    float4 dx;
+    int kind;
+};
+// Of a vertex with plain start d4, mass weight posW (0: pinned), rest data
+// r4, the previous vertex's rest length prevLen, and the corrections cPrev,
+// c0, cNext of it and its neighbours (those of absent neighbours not used).
+__device__ __forceinline__ LearnedDx learned_dx(float4 d4, float posW, float4 r4, float prevLen,
+                                                float4 cPrev, float4 c0, float4 cNext,
+                                                int passes) {
+    if (posW == 0.0f) return LearnedDx{d4, 0};
+    const uint32_t fl = __float_as_uint(r4.w);
+    v3 c = v3{c0.x, c0.y, c0.z};
+    if (passes > 0) {
+        const float4 cm = (fl & YB_FLAG_HAS_PREV) ? cPrev : c0;
+        const float4 cq = (fl & YB_FLAG_HAS_NEXT) ? cNext : c0;
+        c = v3{0.25f * (cm.x + 2.0f * c0.x + cq.x), 0.25f * (cm.y + 2.0f * c0.y + cq.y),
+               0.25f * (cm.z + 2.0f * c0.z + cq.z)};
+    }
+    const float L = (!(fl & YB_FLAG_HAS_NEXT) && (fl & YB_FLAG_HAS_PREV)) ? prevLen : r4.x;
+    const v3 cL = c * L;
+    if (!(isfinite(cL.x) && isfinite(cL.y) && isfinite(cL.z))) return LearnedDx{d4, 2};
+    v3 nd = xyz(d4) + cL;
+    const float l = len(nd);
+    if (l > d4.w && l > 0.0f) nd *= d4.w / l;
+    return LearnedDx{make_float4(nd.x, nd.y, nd.z, d4.w), 1};
+}
+
+// dx = the learned warm start from the plain one in dx0 (the inference
+// kernel's copy, so that no lane reads a dx another lane has replaced), and
+// with kFrames the frames refreshed from it into qOut (as k_frames; vertex
+// v + 1's warm start is recomputed for the edge).  Every input is loaded
+// first, from clamped indices (values of absent neighbours are not used), so
+// that a lane has all its loads in flight at once.  Per-wave counters of
+// applied and fallback vertices: one pair per wave, so no two waves contend
+// for an address; only this wave touches its pair and launches on the stream
+// are ordered, so a plain read-modify-write is exact.
+template <bool kFrames>
+__global__ __launch_bounds__(256) void k_learned_apply(
+    const DevParams* __restrict__ P, const float4* __restrict__ predC,
+    const float4* __restrict__ pos, const float4* __restrict__ prm,
+    const float4* __restrict__ dx0, float4* __restrict__ dx,
+    unsigned long long* __restrict__ counters, int passes, const float4* __restrict__ qIn,
+    float4* __restrict__ qOut, const float4* __restrict__ qRest) {
+    const int n = P->n;
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    int kind = 0;
+    if (v < n) {
+        const int vm = v > 0 ? v - 1 : v, vp = v + 1 < n ? v + 1 : v;
+        const int vpp = vp + 1 < n ? vp + 1 : vp;
+        const float4 rm = prm[vm], r0 = prm[v], cm = predC[vm], c0 = predC[v], cp = predC[vp];
+        const float4 x0 = pos[v], d0 = dx0[v];
+        float4 r1, cpp, x1, d1, q0, qm, qp, qrm, qr;
+        if (kFrames) {
+            r1 = prm[vp];
+            cpp = predC[vpp];
+            x1 = pos[vp];
+            d1 = dx0[vp];
+            q0 = qIn[v];
+            qm = qIn[vm];
+            qp = qIn[vp];
+            qrm = qRest[vm];
+            qr = qRest[v];
+        }
+        const LearnedDx d = learned_dx(d0, x0.w, r0, rm.x, cm, c0, cp, passes);
+        kind = d.kind;
+        if (kind == 1) dx[v] = d.dx;
+        if (kFrames) {
+            const uint32_t flags = __float_as_uint(r0.w);
+            if (!(flags & YB_FLAG_FIX_ORIENTATION) && (flags & YB_FLAG_HAS_NEXT)) {
+                const float4 e1 = learned_dx(d1, x1.w, r1, r0.x, c0, cp, cpp, passes).dx;
+                const v3 e = (xyz(x1) - xyz(x0)) + (xyz(e1) - xyz(d.dx));
+                qOut[v] = refreshed_frame(r0, flags, e, q0, qm, qp, qrm, qr);
+            } else {
+                qOut[v] = q0;
+            }
+        }
+    }
+    unsigned applied = kind == 1 ? 1u : 0u, fallback = kind == 2 ? 1u : 0u;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        applied += __shfl_xor(applied, off, 64);
+        fallback += __shfl_xor(fallback, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0 && (applied | fallback)) {
+        const int wave = v >> 6;
+        counters[2 * wave] += applied;
+        counters[2 * wave + 1] += fallback;
+    }
 }
 
 __global__ void k_copy4(const float4* __restrict__ src, float4* __restrict__ dst, int n) {
@@ -1582,6 +1707,21 @@ void launch_residual(yb_sim* s, int sweep) {
                        nbw, sweep, s->tol, s->maxSweeps, s->b.resBlocks, s->b.resCounter);
 }
 
+void launch_learned_apply(yb_sim* s, bool frames) {
+    if (frames) {
+        hipLaunchKernelGGL(k_learned_apply<true>, dim3(blocks(s->n, 256)), dim3(256), 0,
+                           s->stream, s->b.params, s->b.predC, s->b.pos, s->b.prm, s->b.dxPlain,
+                           s->b.dx[0], s->b.mlpCount, s->mlpSmooth, s->b.q[0], s->b.q[1],
+                           s->b.qRest);
+        hipLaunchKernelGGL(k_copy4, dim3(blocks(s->n, 256)), dim3(256), 0, s->stream, s->b.q[1],
+                           s->b.q[0], s->n);
+    } else {
+        hipLaunchKernelGGL(k_learned_apply<false>, dim3(blocks(s->n, 256)), dim3(256), 0,
+                           s->stream, s->b.params, s->b.predC, s->b.pos, s->b.prm, s->b.dxPlain,
+                           s->b.dx[0], s->b.mlpCount, s->mlpSmooth, nullptr, nullptr, nullptr);
+    }
+}
+
 void launch_frame_refresh(yb_sim* s) {
     hipLaunchKernelGGL(k_frames, dim3(blocks(s->n, 256)), dim3(256), 0, s->stream, s->b.params,
                        s->b.pos, s->b.prm, s->b.dx[0], s->b.q[0], s->b.q[1], s->b.qRest);
@@ -1641,12 +1781,13 @@ static void launch_strand_pairs(yb_sim* s, int buf, SolverCtl* ctl, int sweep, b
                        sweep, sa);
 }
 
-void launch_ctl_reset(yb_sim* s) {
+void launch_ctl_reset(yb_sim* s, bool noNewton) {
     const bool batched = per_scene(s, true);
     hipLaunchKernelGGL(k_ctl_reset, dim3(batched ? blocks(s->numScenes, 256) : 1),
                        dim3(batched ? 256 : 64), 0, s->stream, s->b.ctl,
                        batched ? s->b.sceneCtl : nullptr, s->numScenes,
-                       batched ? s->b.tileCount : nullptr, s->b.resCounter);
+                       batched ? s->b.tileCount : nullptr, s->b.resCounter,
+                       (noNewton && !batched) ? 1 : 0);
 }
 
 // The Newton-residual denominator's partials: the third part of b.partial

@@ -205,11 +205,6 @@ __device__ __forceinline__ void mlp_forward(const uint16_t* tiles, const float* 
 //  36-44  previous substep's converged correction at i, i-1, i+1
 //  45     step limit / L (clamped to 10)   46  |g| h^2 / L
 //  47     log10(1 + k_stretch h^2 / (m L^2)) / 4
-__device__ __forceinline__ float rest_len(const float4* prm, int v, uint32_t flags) {
-    if (!(flags & YB_FLAG_HAS_NEXT) && (flags & YB_FLAG_HAS_PREV)) return prm[v - 1].x;
-    return prm[v].x;
-}
-
 struct FeatSrc {
     const DevParams* P;
     const float4 *pos, *prm, *vel, *yv, *dx0, *q;
@@ -530,62 +525,6 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(FeatSrc S,
     }
 }
 
-// Low-pass the predicted correction along each strand with the binomial
-// [1 2 1]/4 filter (a missing neighbour counts as the vertex itself), then
-// apply it: dx0 + L c, clamped to the step limit.  Block-Jacobi damps the
-// alternating (odd-even) mode along a strand as slowly as the smoothest one,
-// and a per-vertex regression error is close to white noise; the filter
-// removes that mode from the prediction before the sweeps see it.
-__global__ __launch_bounds__(256) void k_mlp_smooth_apply(const DevParams* __restrict__ P,
-                                                          const float4* __restrict__ predC,
-                                                          const float4* __restrict__ pos,
-                                                          const float4* __restrict__ prm,
-                                                          float4* __restrict__ dx,
-                                                          unsigned long long* __restrict__ counters,
-                                                          int passes) {
-    const int n = P->n;
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    unsigned applied = 0, fallback = 0;
-    if (v < n && pos[v].w != 0.0f) {
-        const uint32_t fl = __float_as_uint(prm[v].w);
-        const float4 c0 = predC[v];
-        v3 c = v3{c0.x, c0.y, c0.z};
-        if (passes > 0) {
-            const float4 cm = (fl & YB_FLAG_HAS_PREV) ? predC[v - 1] : c0;
-            const float4 cq = (fl & YB_FLAG_HAS_NEXT) ? predC[v + 1] : c0;
-            c = v3{0.25f * (cm.x + 2.0f * c0.x + cq.x), 0.25f * (cm.y + 2.0f * c0.y + cq.y),
-                   0.25f * (cm.z + 2.0f * c0.z + cq.z)};
-        }
-        const float L = rest_len(prm, v, fl);
-        float4 d4 = dx[v];
-        const v3 cL = c * L;
-        if (isfinite(cL.x) && isfinite(cL.y) && isfinite(cL.z)) {
-            v3 nd = xyz4(d4) + cL;
-            const float l = len(nd);
-            if (l > d4.w && l > 0.0f) nd *= d4.w / l;
-            dx[v] = make_float4(nd.x, nd.y, nd.z, d4.w);
-            applied = 1;
-        } else {
-            fallback = 1;
-        }
-    }
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        applied += __shfl_xor(applied, off, 64);
-        fallback += __shfl_xor(fallback, off, 64);
-    }
-    // One counter pair per wave, so no two waves contend for an address (two
-    // atomics per wave on one shared pair took 0.4 ms per launch at 1M
-    // vertices).  Only this wave touches its pair, and launches on the
-    // stream are ordered, so a plain read-modify-write is exact.
-    if (lane == 0 && (applied | fallback)) {
-        const int wave = v >> 6;
-        counters[2 * wave] += applied;
-        counters[2 * wave + 1] += fallback;
-    }
-}
-
 // ---- temporal warm start (ablation baseline, no network) --------------------------
 // dx0 + L * (previous substep's converged correction), clamped to the step limit.
 __global__ __launch_bounds__(256) void k_temporal(const DevParams* __restrict__ P,
@@ -882,14 +821,12 @@ void launch_temporal(yb_sim* s) {
                        s->b.pos, s->b.prm, s->b.corrPrev, s->b.dx[0], s->b.dxPlain);
 }
 
-void launch_mlp_apply(yb_sim* s) {
+void launch_mlp_apply(yb_sim* s, bool frames) {
     const int blocks = std::min(nblk((s->n + 63) / 64, kInferWaves), std::max(s->mlpGrid, 1));
     hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(64 * kInferWaves), 0, s->stream,
                        feat_src(s), s->b.mlpTiles, s->b.mlpBias, s->b.mlpPrm, s->b.dxPlain,
                        s->b.predC);
-    hipLaunchKernelGGL(k_mlp_smooth_apply, dim3(nblk(s->n, 256)), dim3(256), 0, s->stream,
-                       s->b.params, s->b.predC, s->b.pos, s->b.prm, s->b.dx[0], s->b.mlpCount,
-                       s->mlpSmooth);
+    launch_learned_apply(s, frames);
 }
 
 void launch_correction(yb_sim* s, int fin, bool adaptive, bool record) {

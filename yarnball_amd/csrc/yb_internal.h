@@ -57,7 +57,12 @@ struct SolverCtl {
     float maxFres;                // largest final Newton residual
     int32_t kind;                 // stopping test: 0 update residual, 1 Newton residual
     int32_t scenesDone;           // batched scenes: scenes converged in this substep
+    unsigned long long newtonSubsteps;  // substeps with a Newton residual (sumFres, maxFres)
 };
+// SolverCtl::denomF of a substep that evaluates no Newton residual (a start
+// other than the plain one, with the residual asked for at plain starts only);
+// -1 marks a denominator still to be taken at the substep's first check.
+constexpr double kNoNewton = -2.0;
 
 // Per-scene solver state of a batch in the residual-targeted schedule: each
 // scene stops at its own sweep and keeps its final iterate in its own buffer.
@@ -134,6 +139,13 @@ __device__ __forceinline__ float4 gather4(const float4* __restrict__ a, int i) {
     const uint32_t off = (uint32_t)i << 4;
     return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(a) + off +
                                             (next ? 16 : 0));
+}
+
+// Rest length of vertex v's segment, or of the previous one at a strand end
+// (the length unit of the learned warm start's features and corrections).
+__device__ __forceinline__ float rest_len(const float4* prm, int v, uint32_t flags) {
+    if (!(flags & YB_FLAG_HAS_NEXT) && (flags & YB_FLAG_HAS_PREV)) return prm[v - 1].x;
+    return prm[v].x;
 }
 
 // The smallest contact distance / (2 * radius) over a wave, folded into the
@@ -327,6 +339,7 @@ struct yb_sim {
     void* timeline = nullptr;  // residual-mode substep timeline (capi.hip)
     double denomHost = 0.0;  // update-residual normaliser (sum of l_i^2 over free vertices)
     int residualKind = 0;    // stopping test: YB_RESIDUAL_UPDATE or YB_RESIDUAL_NEWTON
+    int newtonMode = 1;      // YB_NEWTON_ALL_STARTS or YB_NEWTON_PLAIN_STARTS
     int warmStart = 0;     // 0 plain (iteration.cu:18-34), 1 learned, 2 temporal
     int frameRefresh = 1;  // refresh frames after a non-plain warm start
     int strandSolve = 0;   // one Newton step along the strands after the warm start
@@ -378,7 +391,8 @@ void launch_residual(yb_sim* s, int sweep);
 // learned sessions, hundreds of sweeps: no launch, one sweep late).
 inline bool separate_checks(const yb_sim* s) { return s->strandSolve || s->numScenes > 1; }
 void launch_scene_end_flags(yb_sim* s);  // flags of each scene's last vertex -> sceneEndFlags
-void launch_ctl_reset(yb_sim* s);
+// noNewton: the substep records no Newton residual (ignored in a batch)
+void launch_ctl_reset(yb_sim* s, bool noNewton = false);
 void launch_frame_refresh(yb_sim* s);  // closed-form frame update from dx[0]
 // Tridiagonal Newton step along the strands on dx[buf] and q[buf]; with ctl
 // (between sweeps), nothing once the substep converged before `sweep`.
@@ -406,7 +420,10 @@ int sweep_waves(int n);
 // learned warm start (mlp.hip)
 void launch_mlp_pack(yb_sim* s);
 void launch_features(yb_sim* s, bool record);
-void launch_mlp_apply(yb_sim* s);
+// inference, then the apply into dx[0]; frames: the apply also refreshes the
+// frames from the new dx[0] (q[0] updated in place through q[1])
+void launch_mlp_apply(yb_sim* s, bool frames);
+void launch_learned_apply(yb_sim* s, bool frames);  // kernels_sweep.hip
 void launch_temporal(yb_sim* s);
 void launch_correction(yb_sim* s, int fin, bool adaptive, bool record);
 void launch_fill_i32(yb_sim* s, int32_t* p, int32_t v, int count);

@@ -171,16 +171,28 @@ class Sim:
             k = {"update": 0, "newton": 1}[kind]
             check(self._lib.yb_set_residual_kind(self._h, k))
 
+    def setNewtonResidual(self, starts="all"):
+        """Which substeps record the Newton residual |f| / |f0| while the
+        stopping test is the update residual: "all" (the default) or "plain"
+        -- only substeps starting from the plain warm start, whose prologue
+        evaluates f0 anyway; a learned or temporal start then skips that extra
+        force pass.  The iterates do not change."""
+        self._need_handle()
+        check(self._lib.yb_set_newton_residual(self._h, {"all": 1, "plain": 0}[starts]))
+
     def solverStats(self, reset=False):
         """Counters of the residual-targeted mode since the last reset;
-        meanNewtonResidual is the mean over substeps (scene-substeps in a
-        batch) of the final |f| / |f0|; meanSceneSweeps the mean sweeps a
-        scene ran per substep (= sweeps per substep without batching)."""
+        meanNewtonResidual is the mean over the substeps that recorded one
+        (newtonSubsteps: every substep, or scene-substep in a batch, unless
+        setNewtonResidual("plain")) of the final |f| / |f0|; meanSceneSweeps
+        the mean sweeps a scene ran per substep (= sweeps per substep without
+        batching)."""
         st = _lib.SolverStats()
         self._need_handle()
         check(self._lib.yb_solver_stats(self._h, ctypes.byref(st), int(bool(reset))))
         out = {f: getattr(st, f) for f, _ in st._fields_}
-        out["meanNewtonResidual"] = out["sumNewtonResidual"] / max(out["sceneSubsteps"], 1)
+        out["meanNewtonResidual"] = (out["sumNewtonResidual"] / out["newtonSubsteps"]
+                                     if out["newtonSubsteps"] > 0 else float("nan"))
         out["meanSceneSweeps"] = out["sceneSweeps"] / max(out["sceneSubsteps"], 1)
         return out
 
