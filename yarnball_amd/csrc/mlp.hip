@@ -214,63 +214,70 @@ struct FeatSrc {
 
 // Raw features of vertex v into f; returns the plain warm start dx0[v].  The
 // recording kernel and the inference kernel both call this, so the network
-// sees exactly the inputs that are recorded for training.
+// sees exactly the inputs that are recorded for training.  Every input is
+// loaded first, from indices clamped to [0, n) (the values of absent
+// neighbours are not used), so that a lane has all its gathers in flight at
+// once instead of waiting on each neighbour's flags before the next load.
 __device__ __forceinline__ float4 vertex_features(const FeatSrc& S, int v, float f[kMlpIn]) {
-    const float4 r4 = S.prm[v];
+    const int n = S.P->n;
+    const int vm = v > 0 ? v - 1 : v, vmm = v > 1 ? v - 2 : vm;
+    const int vp = v + 1 < n ? v + 1 : v, vpp = v + 2 < n ? v + 2 : vp;
+    const float4 r4 = S.prm[v], rm = S.prm[vm], rp = S.prm[vp];
+    const float4 x4 = S.pos[v], xm = S.pos[vm], xmm = S.pos[vmm], xp = S.pos[vp],
+                 xpp = S.pos[vpp];
+    const float4 vel = S.vel[v], yv = S.yv[v], d0 = S.dx0[v];
+    const int32_t ncv = S.numCols[v], ncm = S.numCols[vm];
+    const float4 qv = S.q[v], qm = S.q[vm];
+    const float4 c0 = S.corrPrev[v], cm = S.corrPrev[vm], cp = S.corrPrev[vp];
+    const float h = S.P->h;
     const uint32_t fl = __float_as_uint(r4.w);
     const bool hp = fl & YB_FLAG_HAS_PREV, hn = fl & YB_FLAG_HAS_NEXT;
-    const float L = rest_len(S.prm, v, fl);
+    const float L = (!hn && hp) ? rm.x : r4.x;  // rest_len
     const float iL = 1.0f / L;
-    const float h = S.P->h;
-    const float4 x4 = S.pos[v];
     const v3 x = xyz4(x4);
 #pragma unroll
     for (int k = 0; k < kMlpIn; k++) f[k] = 0.f;
     bool hpp = false, hnn = false;
     if (hp) {
-        const v3 d = (xyz4(S.pos[v - 1]) - x) * iL;
+        const v3 d = (xyz4(xm) - x) * iL;
         f[0] = d.x; f[1] = d.y; f[2] = d.z;
-        hpp = __float_as_uint(S.prm[v - 1].w) & YB_FLAG_HAS_PREV;
+        hpp = __float_as_uint(rm.w) & YB_FLAG_HAS_PREV;
         if (hpp) {
-            const v3 e = (xyz4(S.pos[v - 2]) - x) * iL;
+            const v3 e = (xyz4(xmm) - x) * iL;
             f[6] = e.x; f[7] = e.y; f[8] = e.z;
         }
     }
     if (hn) {
-        const v3 d = (xyz4(S.pos[v + 1]) - x) * iL;
+        const v3 d = (xyz4(xp) - x) * iL;
         f[3] = d.x; f[4] = d.y; f[5] = d.z;
-        hnn = __float_as_uint(S.prm[v + 1].w) & YB_FLAG_HAS_NEXT;
+        hnn = __float_as_uint(rp.w) & YB_FLAG_HAS_NEXT;
         if (hnn) {
-            const v3 e = (xyz4(S.pos[v + 2]) - x) * iL;
+            const v3 e = (xyz4(xpp) - x) * iL;
             f[9] = e.x; f[10] = e.y; f[11] = e.z;
         }
     }
-    const v3 u = xyz4(S.vel[v]) * (h * iL);
+    const v3 u = xyz4(vel) * (h * iL);
     f[12] = u.x; f[13] = u.y; f[14] = u.z;
     const bool pinned = x4.w == 0.0f;
     if (!pinned) {
-        const v3 y = xyz4(S.yv[v]) * iL;
+        const v3 y = xyz4(yv) * iL;
         f[15] = y.x; f[16] = y.y; f[17] = y.z;
     }
-    const float4 d0 = S.dx0[v];
     const v3 d = xyz4(d0) * iL;
     f[18] = d.x; f[19] = d.y; f[20] = d.z;
-    f[21] = hn ? S.numCols[v] * 0.125f : 0.f;
-    f[22] = hp ? S.numCols[v - 1] * 0.125f : 0.f;
-    const float4 qi = S.q[hn ? v : (hp ? v - 1 : v)];
-    const float4 qp = S.q[hp ? v - 1 : v];
+    f[21] = hn ? ncv * 0.125f : 0.f;
+    f[22] = hp ? ncm * 0.125f : 0.f;
+    const float4 qi = hn ? qv : (hp ? qm : qv);
+    const float4 qp = hp ? qm : qv;
     f[23] = qi.x; f[24] = qi.y; f[25] = qi.z; f[26] = qi.w;
     f[27] = qp.x; f[28] = qp.y; f[29] = qp.z; f[30] = qp.w;
     f[31] = hp; f[32] = hn; f[33] = hpp; f[34] = hnn; f[35] = pinned;
-    const float4 c0 = S.corrPrev[v];
     f[36] = c0.x; f[37] = c0.y; f[38] = c0.z;
     if (hp) {
-        const float4 c = S.corrPrev[v - 1];
-        f[39] = c.x; f[40] = c.y; f[41] = c.z;
+        f[39] = cm.x; f[40] = cm.y; f[41] = cm.z;
     }
     if (hn) {
-        const float4 c = S.corrPrev[v + 1];
-        f[42] = c.x; f[43] = c.y; f[44] = c.z;
+        f[42] = cp.x; f[43] = cp.y; f[44] = cp.z;
     }
     f[45] = fminf(d0.w * iL, 10.0f);
     const float g2 = sqrtf(S.P->gx * S.P->gx + S.P->gy * S.P->gy + S.P->gz * S.P->gz);
