@@ -502,6 +502,8 @@ def run_method(sc, args, method, model, strand_model=None):
     elif method == "learned+strand":
         learned.use(sim, strand_model, strand=True)
         learned.strand_schedule(sim, every=args.strand_every, first=args.strand_first)
+    if method.startswith("learned") and args.learned_cost > 0:
+        learned.start_cost(sim, args.learned_cost)
     if args.no_graphs:
         sim.setGraphs(False)
     if args.launch_config:
@@ -596,6 +598,9 @@ def main():
     ap.add_argument("--max-sweeps", type=int, default=6000)
     ap.add_argument("--chunk", type=int, default=16,
                     help="sweeps per chunk graph in residual mode")
+    ap.add_argument("--learned-cost", type=float, default=0.0,
+                    help="the substep chooser's offset: a learned start's extra prologue time "
+                         "in sweeps")
     ap.add_argument("--newton-residual", choices=["all", "plain"], default="plain",
                     help="residual mode: which substeps record the Newton residual (a "
                          "diagnostic; the stopping test is the update residual): every one, or "

@@ -493,6 +493,12 @@ int yb_mlp_load(yb_sim* s, const float* params, int count);
  * the reference. */
 #define YB_WARM_STRAND_SOLVE 0x800
 int yb_set_warm_start(yb_sim* s, int mode);
+/* The substep-level chooser of a learned session with the fallback (residual-
+ * targeted mode) runs the learned start while its running sweep average plus
+ * `sweeps` is at most the plain start's: the extra time of a learned prologue
+ * (inference, apply) expressed in sweeps.  Default 0 (fewest sweeps wins).
+ * No counterpart in the reference. */
+int yb_set_learned_start_cost(yb_sim* s, float sweeps);
 /* Strand-solve schedule (with YB_WARM_STRAND_SOLVE): in residual-targeted
  * mode, also run the strand step before sweeps first, first + every,
  * first + 2 every, ... (every 0: only after the warm start; first 0: first =

@@ -151,6 +151,7 @@ SIGNATURES = {
     "yb_min_separation": (_I, [_P, _P, _I]),
     "yb_mlp_load": (_I, [_P, _P, _I]),
     "yb_set_warm_start": (_I, [_P, _I]),
+    "yb_set_learned_start_cost": (_I, [_P, _F]),
     "yb_set_strand_schedule": (_I, [_P, _I, _I, _I]),
     "yb_mlp_counters": (_I, [_P, _P, _I]),
     "yb_warm_start_substeps": (_I, [_P, _P, _I]),

@@ -1204,7 +1204,9 @@ extern "C" int yb_solver_stats(yb_sim* s, yb_solver_stats_t* out, int reset) {
 // 0 = learned, 1 = plain.  The start with the lower running sweep average is
 // used; the other one is probed after s->probeGap substeps, a gap that starts
 // at kProbeMin and doubles (up to kProbeMax) each time the probe loses.
-static int best_start(const yb_sim* s) { return s->avgSweeps[0] <= s->avgSweeps[1] ? 0 : 1; }
+static int best_start(const yb_sim* s) {
+    return s->avgSweeps[0] + s->learnedCost <= s->avgSweeps[1] ? 0 : 1;
+}
 static int choose_start(const yb_sim* s) {
     if (!(s->warmStart == 1 && s->fallback)) return 0;
     if (s->avgSweeps[0] == 0.0) return 0;  // first a learned sample,

@@ -1143,6 +1143,15 @@ extern "C" int yb_set_warm_start(yb_sim* s, int mode) {
     return YB_OK;
 }
 
+extern "C" int yb_set_learned_start_cost(yb_sim* s, float sweeps) {
+    if (!s || !(sweeps >= 0.0f) || sweeps > 1e6f) {
+        set_message("yb_set_learned_start_cost: 0 <= sweeps <= 1e6");
+        return YB_ERR_INVALID_ARGUMENT;
+    }
+    s->learnedCost = sweeps;
+    return YB_OK;
+}
+
 extern "C" int yb_warm_start_substeps(yb_sim* s, int64_t out[2], int reset) {
     if (!s || !out) return YB_ERR_INVALID_ARGUMENT;
     out[0] = s->warmSubsteps[0];

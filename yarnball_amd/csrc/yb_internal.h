@@ -354,6 +354,7 @@ struct yb_sim {
     // probes the other one from time to time.
     int fallback = 1;
     double avgSweeps[2] = {0.0, 0.0};  // [0] learned, [1] plain; 0 = no sample yet
+    float learnedCost = 0.0f;  // extra cost of a learned start, in sweeps (the chooser's offset)
     int64_t chooserStep = 0, nextProbe = 8;
     int probeGap = 8;
     int64_t warmSubsteps[2] = {0, 0};  // substeps run with [0] learned, [1] plain start

@@ -143,6 +143,13 @@ def strand(sim, frame_refresh=True, every=0, pair_iters=2, first=0):
         sim.handle, STRAND_SOLVE | (0 if frame_refresh else NO_FRAME_REFRESH)))
 
 
+def start_cost(sim, sweeps):
+    """The chooser's offset: run the learned start only while its sweep
+    average plus `sweeps` (its prologue's extra time, in sweeps) is at most
+    the plain start's."""
+    check(sim._lib.yb_set_learned_start_cost(sim.handle, float(sweeps)))
+
+
 def warm_substeps(sim, reset=False):
     """Substeps started from the session's warm start and from the plain fallback."""
     out = np.zeros(2, np.int64)
