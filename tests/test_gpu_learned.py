@@ -245,6 +245,10 @@ def _headline_run(sc, frames, newton):
     model = np.load(os.path.join(ROOT, "yarnball_amd", "models", "warmstart_strand.npy"))
     learned.use(sim, model, strand=True)
     learned.strand_schedule(sim, every=4, first=2)
+    # out-of-range settings are refused with the handle left as it was
+    assert sim._lib.yb_set_newton_residual(sim.handle, 2) != 0
+    assert sim._lib.yb_set_learned_start_cost(sim.handle, -1.0) != 0
+    assert sim._lib.yb_set_learned_start_cost(sim.handle, float("nan")) != 0
     sim.setNewtonResidual(newton)
     sim.solverStats(reset=True)
     learned.warm_substeps(sim, reset=True)
