@@ -799,8 +799,11 @@ static void enqueue_sweep(yb_sim* s, int k, bool adaptive, bool last = false) {
 }
 
 static void enqueue_end(yb_sim* s, int fin, bool adaptive) {
-    launch_end(s, fin, adaptive);
-    if (s->warmStart != 0 || s->recCap > 0) launch_correction(s, fin, adaptive, s->recCap > 0);
+    // the converged correction: in the end pass, or (recording training data)
+    // by the kernel that also writes the targets
+    const bool record = s->recCap > 0;
+    launch_end(s, fin, adaptive, s->warmStart != 0 && !record);
+    if (record) launch_correction(s, fin, adaptive, true);
     mark(s, YB_STAGE_END);
 }
 

@@ -1451,10 +1451,19 @@ __global__ __launch_bounds__(256) void k_end(const DevParams* __restrict__ P,
                                              float4* __restrict__ q0,
                                              const SolverCtl* __restrict__ ctl,
                                              const float4* __restrict__ dxAlt,
-                                             const float4* __restrict__ qAlt, SceneArgs sa) {
+                                             const float4* __restrict__ qAlt, SceneArgs sa,
+                                             const float4* __restrict__ prm,
+                                             const float4* __restrict__ dxPlain,
+                                             float4* __restrict__ corrPrev) {
     const int n = P->n;
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n) return;
+    float4 plain = make_float4(0.f, 0.f, 0.f, 0.f), r4 = plain, rm = plain;
+    if (corrPrev) {  // issued before the final iterate's buffer is known
+        plain = dxPlain[v];
+        r4 = prm[v];
+        rm = prm[v > 0 ? v - 1 : v];
+    }
     // residual-targeted mode: final iterate in buffer 1 (per scene in a batch)
     const int fin = sa.ctl ? sa.ctl[sa.sceneOf[v]].fin : (ctl ? ctl->fin : 0);
     if (fin == 1) {
@@ -1474,6 +1483,17 @@ __global__ __launch_bounds__(256) void k_end(const DevParams* __restrict__ P,
     p4.z += dx.z;
     pos[v] = p4;
     if (qFin != q0) q0[v] = qFin[v];
+    // A learned (or temporal) session: the converged correction (dx* - dx0) / L
+    // that the next substep's warm start builds on, L the rest length of the
+    // vertex's segment (of the previous one at a strand end).
+    if (corrPrev) {
+        const uint32_t fl = __float_as_uint(r4.w);
+        const float L =
+            (!(fl & YB_FLAG_HAS_NEXT) && (fl & YB_FLAG_HAS_PREV)) ? rm.x : r4.x;
+        const float iL = 1.0f / L;
+        corrPrev[v] = make_float4((dx.x - plain.x) * iL, (dx.y - plain.y) * iL,
+                                  (dx.z - plain.z) * iL, 0.f);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1838,11 +1858,12 @@ void launch_line_start(yb_sim* s, bool keepPlain) {
     launch_strand_pairs(s, 0, nullptr, 0, false);
 }
 
-void launch_end(yb_sim* s, int fin, bool adaptive) {
+void launch_end(yb_sim* s, int fin, bool adaptive, bool correction) {
     hipLaunchKernelGGL(k_end, dim3(blocks(s->n, 256)), dim3(256), 0, s->stream, s->b.params,
                        s->b.pos, s->b.vel, s->b.dx[fin], s->b.q[fin], s->b.q[0],
                        adaptive ? s->b.ctl : nullptr, s->b.dx[1], s->b.q[1],
-                       scene_args(s, adaptive));
+                       scene_args(s, adaptive), s->b.prm, s->b.dxPlain,
+                       correction ? s->b.corrPrev : nullptr);
 }
 
 }  // namespace yb

@@ -413,7 +413,9 @@ void launch_residual_denom(yb_sim* s, bool frames);
 // (in place of launch_features in a learned session's plain substep).
 void launch_line_start(yb_sim* s, bool keepPlain);
 const double2* denom_partials(const yb_sim* s);
-void launch_end(yb_sim* s, int fin, bool adaptive);         // iteration.cu:48-61
+// iteration.cu:48-61; correction: also the learned start's converged
+// correction (k_correction's, without its training record)
+void launch_end(yb_sim* s, int fin, bool adaptive, bool correction = false);
 SceneArgs scene_args(const yb_sim* s, bool adaptive);
 // The first sweep of a residual-targeted substep whose residual is checked.
 int first_check(const yb_sim* s);
