@@ -706,8 +706,10 @@ static void enqueue_prologue(yb_sim* s, int mode, bool adaptive, int warm) {
     const bool fused = evalDenom && refresh && warm == 0;
     const bool learned = warm == 1, record = s->recCap > 0;
     const bool lineStart = fused && s->strandSolve && !record;
-    // a learned start's apply refreshes the frames in the same pass
+    // a learned start's apply refreshes the frames in the same pass (into
+    // q[1]: the strand step that follows reads them there and copies them)
     const bool applyFrames = learned && refresh && !fused;
+    const bool deferCopy = applyFrames && s->strandSolve;
     if (adaptive) {
         launch_ctl_reset(s, noNewton);
         if (lineStart)
@@ -720,7 +722,7 @@ static void enqueue_prologue(yb_sim* s, int mode, bool adaptive, int warm) {
         // the inference kernel computes its own features; the recording
         // kernel runs for training data and for the plain warm start alone
         if (record || !learned) launch_features(s, record);
-        if (learned) launch_mlp_apply(s, applyFrames);
+        if (learned) launch_mlp_apply(s, applyFrames, !deferCopy);
         mark(s, YB_STAGE_MLP);
     } else if (warm == 2) {
         launch_temporal(s);
@@ -736,7 +738,7 @@ static void enqueue_prologue(yb_sim* s, int mode, bool adaptive, int warm) {
         mark(s, YB_STAGE_MLP);
     }
     if (s->strandSolve && !lineStart) {
-        launch_strand_solve(s);
+        launch_strand_solve(s, 0, nullptr, 0, nullptr, deferCopy ? 1 : -1, deferCopy);
         mark(s, YB_STAGE_MLP);
     }
 }

@@ -997,15 +997,24 @@ __device__ __forceinline__ void pairs_apply_tile(int w, int n, int tiles, int it
     dx[v] = f4(nd, d4.w);
 }
 
+// qDst set (a prologue step whose tile pass read frames refreshed into qSrc):
+// each lane also copies its vertex's frame qSrc -> qDst for the sweeps.
 __global__ __launch_bounds__(256) void k_line_pairs_apply(int n, int tiles, int iters,
                                                           const float4* __restrict__ g,
                                                           const float2* __restrict__ ql,
                                                           float4* __restrict__ dx,
                                                           const SolverCtl* __restrict__ ctl,
-                                                          int sweep, SceneArgs sa) {
+                                                          int sweep, SceneArgs sa,
+                                                          const float4* __restrict__ qSrc,
+                                                          float4* __restrict__ qDst) {
     if (strand_skip(ctl, sweep)) return;
     const int w = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6);
     if (w >= tiles) return;
+    if (qDst) {
+        const int lane = threadIdx.x & 63;
+        const int v = 62 * w + lane - 1;
+        if (lane >= 1 && lane <= 62 && v < n) qDst[v] = qSrc[v];
+    }
     pairs_apply_tile(w, n, tiles, iters, g, ql, dx, sweep, sa);
 }
 
@@ -1727,14 +1736,15 @@ void launch_residual(yb_sim* s, int sweep) {
                        nbw, sweep, s->tol, s->maxSweeps, s->b.resBlocks, s->b.resCounter);
 }
 
-void launch_learned_apply(yb_sim* s, bool frames) {
+void launch_learned_apply(yb_sim* s, bool frames, bool copyFrames) {
     if (frames) {
         hipLaunchKernelGGL(k_learned_apply<true>, dim3(blocks(s->n, 256)), dim3(256), 0,
                            s->stream, s->b.params, s->b.predC, s->b.pos, s->b.prm, s->b.dxPlain,
                            s->b.dx[0], s->b.mlpCount, s->mlpSmooth, s->b.q[0], s->b.q[1],
                            s->b.qRest);
-        hipLaunchKernelGGL(k_copy4, dim3(blocks(s->n, 256)), dim3(256), 0, s->stream, s->b.q[1],
-                           s->b.q[0], s->n);
+        if (copyFrames)
+            hipLaunchKernelGGL(k_copy4, dim3(blocks(s->n, 256)), dim3(256), 0, s->stream,
+                               s->b.q[1], s->b.q[0], s->n);
     } else {
         hipLaunchKernelGGL(k_learned_apply<false>, dim3(blocks(s->n, 256)), dim3(256), 0,
                            s->stream, s->b.params, s->b.predC, s->b.pos, s->b.prm, s->b.dxPlain,
@@ -1749,9 +1759,12 @@ void launch_frame_refresh(yb_sim* s) {
                        s->b.q[0], s->n);
 }
 
-static void launch_strand_pairs(yb_sim* s, int buf, SolverCtl* ctl, int sweep, bool list);
+static void launch_strand_pairs(yb_sim* s, int buf, SolverCtl* ctl, int sweep, bool list,
+                                bool copyFrames = false);
 
-void launch_strand_solve(yb_sim* s, int buf, SolverCtl* ctl, int sweep, float* sys) {
+void launch_strand_solve(yb_sim* s, int buf, SolverCtl* ctl, int sweep, float* sys, int qbuf,
+                         bool copyFrames) {
+    const float4* qf = s->b.q[qbuf < 0 ? buf : qbuf];
     const int tiles = sweep_waves(s->n);
     const LineOut lo{s->b.lineG, s->b.lineQL, sys};
     // in a batch, converged scenes are left alone (before sweep `sweep`)
@@ -1759,33 +1772,41 @@ void launch_strand_solve(yb_sim* s, int buf, SolverCtl* ctl, int sweep, float* s
     const bool list = tile_list_mode(s, ctl != nullptr, sweep);
     if (list)
         hipLaunchKernelGGL(k_line_tile_list, dim3(list_blocks(s)), dim3(256), 0, s->stream,
-                           s->b.params, s->b.pos, s->b.prm, s->b.yv, s->b.q[buf], s->b.qRest,
+                           s->b.params, s->b.pos, s->b.prm, s->b.yv, qf, s->b.qRest,
                            s->b.dx[buf], s->b.numCols, s->b.cols, lo, ctl, sweep, sa,
                            s->b.tileList, s->b.tileCount);
     else if (sa.ctl)
         hipLaunchKernelGGL(k_line_tile<true>, dim3(blocks(tiles * 64, 256)), dim3(256), 0,
-                           s->stream, s->b.params, s->b.pos, s->b.prm, s->b.yv, s->b.q[buf],
+                           s->stream, s->b.params, s->b.pos, s->b.prm, s->b.yv, qf,
                            s->b.qRest, s->b.dx[buf], s->b.numCols, s->b.cols, lo, ctl, sweep, sa);
     else
         hipLaunchKernelGGL(k_line_tile<false>, dim3(blocks(tiles * 64, 256)), dim3(256), 0,
-                           s->stream, s->b.params, s->b.pos, s->b.prm, s->b.yv, s->b.q[buf],
+                           s->stream, s->b.params, s->b.pos, s->b.prm, s->b.yv, qf,
                            s->b.qRest, s->b.dx[buf], s->b.numCols, s->b.cols, lo, ctl, sweep, sa);
-    launch_strand_pairs(s, buf, ctl, sweep, list);
+    launch_strand_pairs(s, buf, ctl, sweep, list, copyFrames);
 }
 
-// The boundary part of a strand step on dx[buf], and the apply.
-static void launch_strand_pairs(yb_sim* s, int buf, SolverCtl* ctl, int sweep, bool list) {
+// The boundary part of a strand step on dx[buf], and the apply.  copyFrames:
+// then copy the frames q[1] -> q[0] (refreshed by the prologue into q[1]), in
+// the apply's pass where it is the one-launch kind.
+static void launch_strand_pairs(yb_sim* s, int buf, SolverCtl* ctl, int sweep, bool list,
+                                bool copyFrames) {
     const int tiles = sweep_waves(s->n);
     const SceneArgs sa = scene_args(s, ctl != nullptr);
+    if (s->strandIters <= 2 && !list) {
+        hipLaunchKernelGGL(k_line_pairs_apply, dim3(blocks(tiles * 64, 256)), dim3(256), 0,
+                           s->stream, s->n, tiles, s->strandIters, s->b.lineG, s->b.lineQL,
+                           s->b.dx[buf], ctl, sweep, sa, copyFrames ? s->b.q[1] : nullptr,
+                           copyFrames ? s->b.q[0] : nullptr);
+        return;
+    }
+    if (copyFrames)
+        hipLaunchKernelGGL(k_copy4, dim3(blocks(s->n, 256)), dim3(256), 0, s->stream, s->b.q[1],
+                           s->b.q[0], s->n);
     if (s->strandIters <= 2) {
-        if (list)
-            hipLaunchKernelGGL(k_line_pairs_apply_list, dim3(list_blocks(s)), dim3(256), 0,
-                               s->stream, s->n, tiles, s->strandIters, s->b.lineG, s->b.lineQL,
-                               s->b.dx[buf], ctl, sweep, sa, s->b.tileList, s->b.tileCount);
-        else
-            hipLaunchKernelGGL(k_line_pairs_apply, dim3(blocks(tiles * 64, 256)), dim3(256), 0,
-                               s->stream, s->n, tiles, s->strandIters, s->b.lineG, s->b.lineQL,
-                               s->b.dx[buf], ctl, sweep, sa);
+        hipLaunchKernelGGL(k_line_pairs_apply_list, dim3(list_blocks(s)), dim3(256), 0,
+                           s->stream, s->n, tiles, s->strandIters, s->b.lineG, s->b.lineQL,
+                           s->b.dx[buf], ctl, sweep, sa, s->b.tileList, s->b.tileCount);
         return;
     }
     int cur = 0;
@@ -1853,9 +1874,7 @@ void launch_line_start(yb_sim* s, bool keepPlain) {
                            s->b.pos, s->b.prm, s->b.yv, s->b.q[0], s->b.q[1], s->b.qRest,
                            s->b.dx[0], plain, s->b.numCols, s->b.cols, lo, part,
                            SceneArgs{nullptr, nullptr});
-    hipLaunchKernelGGL(k_copy4, dim3(blocks(s->n, 256)), dim3(256), 0, s->stream, s->b.q[1],
-                       s->b.q[0], s->n);
-    launch_strand_pairs(s, 0, nullptr, 0, false);
+    launch_strand_pairs(s, 0, nullptr, 0, false, true);  // also the frames q[1] -> q[0]
 }
 
 void launch_end(yb_sim* s, int fin, bool adaptive, bool correction) {

@@ -828,12 +828,12 @@ void launch_temporal(yb_sim* s) {
                        s->b.pos, s->b.prm, s->b.corrPrev, s->b.dx[0], s->b.dxPlain);
 }
 
-void launch_mlp_apply(yb_sim* s, bool frames) {
+void launch_mlp_apply(yb_sim* s, bool frames, bool copyFrames) {
     const int blocks = std::min(nblk((s->n + 63) / 64, kInferWaves), std::max(s->mlpGrid, 1));
     hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(64 * kInferWaves), 0, s->stream,
                        feat_src(s), s->b.mlpTiles, s->b.mlpBias, s->b.mlpPrm, s->b.dxPlain,
                        s->b.predC);
-    launch_learned_apply(s, frames);
+    launch_learned_apply(s, frames, copyFrames);
 }
 
 void launch_correction(yb_sim* s, int fin, bool adaptive, bool record) {

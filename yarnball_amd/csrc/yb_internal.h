@@ -399,8 +399,10 @@ void launch_frame_refresh(yb_sim* s);  // closed-form frame update from dx[0]
 // (between sweeps), nothing once the substep converged before `sweep`.
 // sys (tests): also write each vertex's system row (diagonal, coupling to the
 // next vertex, right-hand side xyz).
+// qbuf >= 0: the frames are read from q[qbuf]; copyFrames: the step's last
+// pass also copies q[1] -> q[0] (frames refreshed into q[1] by the prologue).
 void launch_strand_solve(yb_sim* s, int buf = 0, SolverCtl* ctl = nullptr, int sweep = 0,
-                         float* sys = nullptr);
+                         float* sys = nullptr, int qbuf = -1, bool copyFrames = false);
 // Newton-residual denominator of the substep: the partials of |f(dx_0)|^2 at
 // the plain warm start in dx[0] (after launch_ctl_reset; before the warm start
 // is changed), reduced by the substep's first residual check.  frames: also
@@ -425,8 +427,10 @@ void launch_mlp_pack(yb_sim* s);
 void launch_features(yb_sim* s, bool record);
 // inference, then the apply into dx[0]; frames: the apply also refreshes the
 // frames from the new dx[0] (q[0] updated in place through q[1])
-void launch_mlp_apply(yb_sim* s, bool frames);
-void launch_learned_apply(yb_sim* s, bool frames);  // kernels_sweep.hip
+// copyFrames (with frames): the refreshed frames q[1] -> q[0] in a pass of its
+// own; false when the strand step that follows reads q[1] and copies them
+void launch_mlp_apply(yb_sim* s, bool frames, bool copyFrames = true);
+void launch_learned_apply(yb_sim* s, bool frames, bool copyFrames = true);  // kernels_sweep.hip
 void launch_temporal(yb_sim* s);
 void launch_correction(yb_sim* s, int fin, bool adaptive, bool record);
 void launch_fill_i32(yb_sim* s, int32_t* p, int32_t v, int count);
