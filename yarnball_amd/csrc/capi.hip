@@ -669,23 +669,6 @@ static void mark(yb_sim* s, int stage) {
 // plain prologue of a learned session still writes the plain start for the
 // end-of-substep correction that the next learned substep reads.
 static void enqueue_prologue(yb_sim* s, int mode, bool adaptive, int warm) {
-    launch_init(s);
-    mark(s, YB_STAGE_INIT);
-    if (mode >= 1) {
-        launch_aabb(s, mode == 2);  // a new tree: every segment walks
-        mark(s, YB_STAGE_AABB);
-        if (mode == 2) {
-            launch_rebuild(s);
-            mark(s, YB_STAGE_REBUILD);
-        }
-        launch_refit(s);
-        mark(s, YB_STAGE_REFIT);
-        launch_query(s);  // also the step limit, from the distances it measures
-        mark(s, YB_STAGE_QUERY);
-    } else {
-        launch_step_limit(s);
-        mark(s, YB_STAGE_STEP_LIMIT);
-    }
     // Residual-targeted mode: the Newton residual's denominator is the force
     // at the plain warm start (now in dx[0], with the previous substep's
     // frames) for every warm start, so that all of them are held to the same
@@ -702,6 +685,24 @@ static void enqueue_prologue(yb_sim* s, int mode, bool adaptive, int warm) {
     const bool refresh = (warm != 0 || s->strandSolve) && s->frameRefresh;
     const bool noNewton = warm != 0 && s->newtonMode == YB_NEWTON_PLAIN_STARTS &&
                           s->residualKind == YB_RESIDUAL_UPDATE && s->numScenes <= 1;
+    // the residual schedule's solver state is reset by the first kernel
+    launch_init(s, adaptive ? ctl_reset_args(s, noNewton) : CtlReset{});
+    mark(s, YB_STAGE_INIT);
+    if (mode >= 1) {
+        launch_aabb(s, mode == 2);  // a new tree: every segment walks
+        mark(s, YB_STAGE_AABB);
+        if (mode == 2) {
+            launch_rebuild(s);
+            mark(s, YB_STAGE_REBUILD);
+        }
+        launch_refit(s);
+        mark(s, YB_STAGE_REFIT);
+        launch_query(s);  // also the step limit, from the distances it measures
+        mark(s, YB_STAGE_QUERY);
+    } else {
+        launch_step_limit(s);
+        mark(s, YB_STAGE_STEP_LIMIT);
+    }
     const bool evalDenom = adaptive && (s->warmStart != 0 || s->strandSolve) && !noNewton;
     const bool fused = evalDenom && refresh && warm == 0;
     const bool learned = warm == 1, record = s->recCap > 0;
@@ -711,7 +712,6 @@ static void enqueue_prologue(yb_sim* s, int mode, bool adaptive, int warm) {
     const bool applyFrames = learned && refresh && !fused;
     const bool deferCopy = applyFrames && s->strandSolve;
     if (adaptive) {
-        launch_ctl_reset(s, noNewton);
         if (lineStart)
             launch_line_start(s, s->warmStart == 1);
         else if (evalDenom)

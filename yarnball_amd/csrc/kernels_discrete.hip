@@ -33,10 +33,26 @@ __global__ __launch_bounds__(256) void k_init(const DevParams* __restrict__ P,
                                               float4* __restrict__ yv,
                                               float4* __restrict__ dx0,
                                               int32_t* __restrict__ skinState,
-                                              int4* __restrict__ skinPolicy, int skinScenes) {
+                                              int4* __restrict__ skinPolicy, int skinScenes,
+                                              CtlReset cr) {
     const int n = P->n;
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n) return;
+    // Residual-targeted schedule: the substep's solver state starts running
+    // (nothing reads it before the substep's first residual check).
+    if (cr.ctl) {
+        if (v == 0) {
+            cr.ctl->doneAt = 0x7fffffff;
+            cr.ctl->denomF = cr.noNewton ? kNoNewton : -1.0;
+            cr.ctl->scenesDone = 0;
+            if (cr.tileCount) cr.tileCount[0] = -1;  // every tile until the first check
+            if (cr.resCounter) *cr.resCounter = 0u;
+        }
+        if (cr.sceneCtl && v < cr.numScenes) {
+            cr.sceneCtl[v].doneAt = 0x7fffffff;
+            cr.sceneCtl[v].denomF = -1.0;
+        }
+    }
     // Contact skin, one thread per scene: the scene's walk flag of the last
     // substep feeds its policy (lists reused fewer than kSkinMinReuse times:
     // walk without the skin for a back-off of kSkinBackoff substeps, doubled
@@ -1363,11 +1379,11 @@ SkinArgs skin_args(const yb_sim* s) {
     return a;
 }
 
-void launch_init(yb_sim* s) {
+void launch_init(yb_sim* s, const CtlReset& cr) {
     const int scenes = s->skinMargin > 0.f ? s->numScenes : 0;
     hipLaunchKernelGGL(k_init, dim3(blocks(s->n, 256)), dim3(256), 0, s->stream, s->b.params,
                        s->b.pos, s->b.vel, s->b.lastVel, s->b.yv, s->b.dx[0], s->b.skinState,
-                       s->b.skinPolicy, scenes);
+                       s->b.skinPolicy, scenes, cr);
 }
 
 void launch_aabb(yb_sim* s, bool force) {

@@ -1822,6 +1822,18 @@ static void launch_strand_pairs(yb_sim* s, int buf, SolverCtl* ctl, int sweep, b
                        sweep, sa);
 }
 
+CtlReset ctl_reset_args(const yb_sim* s, bool noNewton) {
+    const bool batched = per_scene(s, true);
+    CtlReset r;
+    r.ctl = s->b.ctl;
+    r.sceneCtl = batched ? s->b.sceneCtl : nullptr;
+    r.numScenes = s->numScenes;
+    r.tileCount = batched ? s->b.tileCount : nullptr;
+    r.resCounter = s->b.resCounter;
+    r.noNewton = (noNewton && !batched) ? 1 : 0;
+    return r;
+}
+
 void launch_ctl_reset(yb_sim* s, bool noNewton) {
     const bool batched = per_scene(s, true);
     hipLaunchKernelGGL(k_ctl_reset, dim3(batched ? blocks(s->numScenes, 256) : 1),

@@ -374,7 +374,18 @@ void launch_upload(yb_sim* s);     // staging AoS -> SoA
 void launch_download(yb_sim* s);   // SoA -> staging AoS
 void launch_zero_vel(yb_sim* s);
 void launch_drive(yb_sim* s, const yb_driver& d, float t, float dt);
-void launch_init(yb_sim* s);                    // iteration.cu:7-41
+// The residual-targeted schedule's per-substep reset of its solver state
+// (ctl null: none), made by the first prologue kernel.
+struct CtlReset {
+    SolverCtl* ctl = nullptr;
+    SceneCtl* sceneCtl = nullptr;
+    int numScenes = 0;
+    int32_t* tileCount = nullptr;
+    unsigned int* resCounter = nullptr;
+    int noNewton = 0;
+};
+CtlReset ctl_reset_args(const yb_sim* s, bool noNewton);  // kernels_sweep.hip
+void launch_init(yb_sim* s, const CtlReset& cr = CtlReset{});  // iteration.cu:7-41
 void launch_step_limit(yb_sim* s);              // collision.cu:121-172
 void launch_separation(yb_sim* s);              // contact distances of the current state
 // Sweep k reads dx[k&1], q[k&1] and writes the other buffer.  adaptive: the
